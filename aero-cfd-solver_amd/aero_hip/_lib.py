@@ -1,0 +1,159 @@
+"""ctypes bindings of the C ABI in include/aero_hip.h (libaero_hip.so, built in this directory).
+
+The library is required: importing this module raises if it is missing, so nothing silently falls back
+to CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libaero_hip.so"
+
+AERO_OK, AERO_EINVAL, AERO_EHIP, AERO_ENODEV = 0, 1, 2, 3
+FLUX = {"rusanov": 0, "hllc": 1}
+LIMITER = {"minmod": 0, "venkat": 1}
+BC_INTERIOR, BC_FARFIELD, BC_SLIP_WALL = 0, 1, 2
+CELLS = {"tri": 0, "quad": 1, "mixed": 2}
+HIST_FIELDS = 12
+H_ITER, H_L1, H_L2, H_LINF, H_RATIO, H_CL, H_CD, H_CM, H_RHO_L2, H_CFL, H_FO, H_DIFF = range(12)
+
+i32, f64, vp = C.c_int32, C.c_double, C.c_void_p
+PI = C.POINTER(C.c_int32)
+PD = C.POINTER(C.c_double)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("num_cells", i32), ("num_faces", i32), ("face_owner", PI), ("face_neighbor", PI),
+                ("face_bc", PI), ("face_normal", PD), ("face_center", PD), ("face_area", PD),
+                ("cell_volume", PD), ("cell_center", PD), ("wall_start", i32), ("wall_count", i32)]
+
+
+class Options(C.Structure):
+    _fields_ = [("renumber", i32), ("block_cells", i32), ("gradient", i32)]
+
+
+class ResidualConfig(C.Structure):
+    _fields_ = [("farfield", f64 * 5), ("gamma", f64), ("flux_scheme", i32), ("limiter", i32),
+                ("second_order", i32), ("low_mach_fix", i32), ("low_mach_cutoff", f64),
+                ("stab_mach_floor", f64), ("precond_farfield_bc", i32), ("precond_enabled", i32),
+                ("precond_mach_ref", f64), ("precond_mach_min", f64), ("precond_beta_min", f64),
+                ("precond_beta_max", f64), ("all_speed_fix", i32), ("all_speed_cutoff", f64),
+                ("all_speed_f_min", f64), ("all_speed_ramp", f64)]
+
+
+class ResidualDiag(C.Structure):
+    _fields_ = [("first_order_fallback", i32), ("diffusive_fallback", i32), ("first_failure_face", i32)]
+
+
+CASE_FIELDS = [
+    ("iterations", i32), ("min_iterations", i32), ("cfl_ramp_iters", i32),
+    ("gamma", f64), ("gas_constant", f64), ("mach", f64), ("aoa_deg", f64), ("p_inf", f64), ("t_inf", f64),
+    ("rho_inf", f64), ("cfl_start", f64), ("cfl_max", f64),
+    ("residual_reduction_target", f64), ("force_stability_tol", f64),
+    ("force_stability_window", i32), ("force_mean_drift_tol", f64),
+    ("startup_first_order_iters", i32), ("rk_stages", i32), ("local_time_stepping", i32),
+    ("precond_on", i32), ("precond_mach_ref", f64), ("precond_mach_min", f64), ("precond_beta_min", f64),
+    ("precond_beta_max", f64), ("precond_farfield_bc", i32),
+    ("stab_floor_start", f64), ("stab_floor_target", f64), ("stab_floor_k_start", f64),
+    ("stab_floor_k_target", f64), ("stab_ramp_iters", i32),
+    ("x_ref", f64), ("y_ref", f64), ("chord", f64),
+    ("flux_scheme", i32), ("limiter", i32), ("low_mach_fix", i32), ("mach_cutoff", f64),
+    ("all_speed_flux_fix", i32), ("all_speed_mach_cutoff", f64), ("all_speed_f_min", f64),
+    ("all_speed_ramp_start_iter", i32), ("all_speed_ramp_iters", i32),
+    ("all_speed_staged_controller", i32), ("all_speed_stage_a_min_iters", i32),
+    ("all_speed_stage_a_max_iters", i32), ("all_speed_stage_f_min_high", f64),
+    ("all_speed_stage_f_min_mid", f64), ("all_speed_stage_f_min_low", f64),
+    ("all_speed_pjump_wall_target", f64), ("all_speed_pjump_spike_factor", f64),
+    ("all_speed_pjump_hold_iters", i32), ("all_speed_freeze_iters", i32),
+    ("all_speed_cfl_drop_factor", f64), ("all_speed_cfl_restore_factor", f64),
+    ("all_speed_cfl_min_scale", f64), ("aoa0_symmetry_enforce", i32),
+    ("aoa0_symmetry_enforce_interval", i32), ("precond_experimental_enabled", i32),
+]
+
+
+class CaseConfigC(C.Structure):
+    _fields_ = CASE_FIELDS
+
+
+class RunOutput(C.Structure):
+    _fields_ = [("hist_capacity", i32), ("n_iter", i32), ("nonfinite", i32), ("history", PD),
+                ("conserved", PD), ("residual", PD), ("spectral_radius", PD), ("forces", PD),
+                ("wall_cp", PD), ("device_seconds", f64)]
+
+
+class MeshGenConfig(C.Structure):
+    _fields_ = [("naca_code", C.c_char * 8), ("chord", f64), ("num_circumferential", i32),
+                ("num_radial", i32), ("farfield_radius", f64), ("radial_stretch", f64),
+                ("cell_type", i32), ("quad_layers", i32), ("jitter", f64), ("seed", C.c_uint64),
+                ("renumber_rcm", i32)]
+
+
+class MeshArrays(C.Structure):
+    _fields_ = [("precision", i32), ("num_cells", i32), ("num_faces", i32), ("num_points", i32),
+                ("face_owner", PI), ("face_neighbor", PI), ("face_bc", PI), ("face_vertices", PI),
+                ("face_normal", vp), ("face_center", vp), ("face_area", vp), ("cell_volume", vp),
+                ("cell_center", vp), ("points", vp), ("cell_offsets", PI), ("cell_connectivity", PI),
+                ("wall_start", i32), ("wall_count", i32), ("farfield_start", i32), ("farfield_count", i32)]
+
+
+# Every symbol include/aero_hip.h declares, with its ctypes prototype.
+PROTOTYPES = {
+    "aero_hip_available": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "aero_hip_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Options), C.POINTER(vp), C.c_char_p,
+                                       C.c_size_t]),
+    "aero_hip_mesh_destroy": (None, [vp]),
+    "aero_hip_residual": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, PD, C.POINTER(ResidualDiag),
+                                    C.c_char_p, C.c_size_t]),
+    "aero_hip_set_state": (C.c_int, [vp, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_residual_async": (C.c_int, [vp, C.POINTER(ResidualConfig), C.c_char_p, C.c_size_t]),
+    "aero_hip_get_residual": (C.c_int, [vp, PD, PD, C.POINTER(ResidualDiag), C.c_char_p, C.c_size_t]),
+    "aero_hip_get_gradients": (C.c_int, [vp, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_stream": (vp, [vp]),
+    "aero_hip_timing_begin": (C.c_int, [vp, i32, C.c_char_p, C.c_size_t]),
+    "aero_hip_timing_end": (C.c_int, [vp, PD, PD, PD, C.POINTER(i32), C.c_char_p, C.c_size_t]),
+    "aero_hip_layout": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+    "aero_hip_stage": (C.c_int, [vp, C.POINTER(ResidualConfig), i32, i32, f64, i32, i32, C.c_char_p, C.c_size_t]),
+    "aero_hip_run_case": (C.c_int, [vp, C.POINTER(CaseConfigC), C.POINTER(RunOutput), C.c_char_p, C.c_size_t]),
+    "aero_mesh_generate": (C.c_int, [C.POINTER(MeshGenConfig), i32, C.POINTER(MeshArrays), C.c_char_p,
+                                     C.c_size_t]),
+    "aero_mesh_free": (None, [C.POINTER(MeshArrays)]),
+}
+
+
+def load(path: Path | str | None = None) -> C.CDLL:
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP backend library {p} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C aero-cfd-solver_amd`")
+    lib = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load(os.environ.get("AERO_HIP_LIB") or None)
+    return _LIB
+
+
+class HipError(RuntimeError):
+    """Raised for a non-zero status of a C-ABI call (the reference throws std::runtime_error)."""
+
+
+def check(rc: int, err: C.Array, ctx: str) -> None:
+    if rc != AERO_OK:
+        raise HipError(f"{ctx}: {err.value.decode(errors='replace')} (status {rc})")
+
+
+def errbuf() -> C.Array:
+    return C.create_string_buffer(1024)

@@ -1,0 +1,232 @@
+"""Host-side mirror of the reference's Euler backend interface over the HIP C ABI.
+
+`HipMesh` plays the role of cfd::cuda_backend::EulerDeviceBuffers + its RAII guard
+(src/cfd_core/solvers/euler_solver.cpp:452-460): it owns the device mesh, state and stream.
+`CaseConfig` mirrors EulerAirfoilCaseConfig (src/cfd_core/include/cfd_core/solvers/euler_solver.hpp:14-74)
+with the reference's defaults.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import asdict, dataclass, fields
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh
+
+
+@dataclass
+class CaseConfig:
+    iterations: int = 400
+    min_iterations: int = 40
+    cfl_ramp_iters: int = 120
+    gamma: float = 1.4
+    gas_constant: float = 287.05
+    mach: float = 0.15
+    aoa_deg: float = 2.0
+    p_inf: float = 101325.0
+    t_inf: float = 288.15
+    rho_inf: float = 0.0
+    cfl_start: float = 0.2
+    cfl_max: float = 1.2
+    residual_reduction_target: float = 1.0e-3
+    force_stability_tol: float = 2.0e-5
+    force_stability_window: int = 6
+    force_mean_drift_tol: float = 2.0e-5
+    startup_first_order_iters: int = 25
+    rk_stages: int = 3
+    local_time_stepping: int = 1
+    precond_on: int = 0
+    precond_mach_ref: float = 0.15
+    precond_mach_min: float = 1.0e-3
+    precond_beta_min: float = 0.2
+    precond_beta_max: float = 1.0
+    precond_farfield_bc: int = 0
+    stab_floor_start: float = 0.1
+    stab_floor_target: float = 0.02
+    stab_floor_k_start: float = -1.0
+    stab_floor_k_target: float = -1.0
+    stab_ramp_iters: int = 400
+    x_ref: float = 0.25
+    y_ref: float = 0.0
+    chord: float = 1.0
+    flux_scheme: int = 0
+    limiter: int = 0
+    low_mach_fix: int = 0
+    mach_cutoff: float = 0.3
+    all_speed_flux_fix: int = 0
+    all_speed_mach_cutoff: float = 0.25
+    all_speed_f_min: float = 0.05
+    all_speed_ramp_start_iter: int = 0
+    all_speed_ramp_iters: int = 0
+    all_speed_staged_controller: int = 0
+    all_speed_stage_a_min_iters: int = 600
+    all_speed_stage_a_max_iters: int = 0
+    all_speed_stage_f_min_high: float = 0.2
+    all_speed_stage_f_min_mid: float = 0.1
+    all_speed_stage_f_min_low: float = 0.05
+    all_speed_pjump_wall_target: float = 0.0
+    all_speed_pjump_spike_factor: float = 1.5
+    all_speed_pjump_hold_iters: int = 120
+    all_speed_freeze_iters: int = 120
+    all_speed_cfl_drop_factor: float = 0.8
+    all_speed_cfl_restore_factor: float = 1.02
+    all_speed_cfl_min_scale: float = 0.1
+    aoa0_symmetry_enforce: int = 0
+    aoa0_symmetry_enforce_interval: int = 0
+    precond_experimental_enabled: int = -1  # -1: read AERO_ENABLE_EXPERIMENTAL_PRECOND like the reference
+
+    def resolved(self) -> "CaseConfig":
+        c = CaseConfig(**asdict(self))
+        if c.precond_experimental_enabled < 0:  # euler_solver.cpp:38-45, :482-489
+            v = os.environ.get("AERO_ENABLE_EXPERIMENTAL_PRECOND", "")
+            c.precond_experimental_enabled = 1 if v in ("1", "true", "TRUE", "on", "ON") else 0
+        return c
+
+    def to_c(self) -> _lib.CaseConfigC:
+        r = self.resolved()
+        return _lib.CaseConfigC(*[getattr(r, name) for name, _ in _lib.CASE_FIELDS])
+
+
+def freestream(mach: float, aoa_deg: float, p_inf: float = 101325.0, t_inf: float = 288.15, gamma: float = 1.4,
+               gas_constant: float = 287.05, rho_inf: float = 0.0):
+    """Freestream primitive and conservative states (euler_solver.cpp:543-561, euler_flux.cpp:101-113)."""
+    rho = rho_inf if rho_inf > 0.0 else p_inf / (gas_constant * t_inf)
+    alpha = aoa_deg * (math.pi / 180.0)
+    a = math.sqrt(gamma * p_inf / rho)
+    v = max(mach, 0.0) * a
+    u, w = v * math.cos(alpha), v * math.sin(alpha)
+    r = max(rho, 1e-8)
+    ke = 0.5 * r * (u * u + w * w + 0.0 * 0.0)
+    cons = np.array([r, r * u, r * w, r * 0.0, max(p_inf, 1e-8) / (gamma - 1.0) + ke])
+    return np.array([rho, u, w, 0.0, p_inf]), cons
+
+
+def residual_config(farfield, gamma=1.4, flux="hllc", limiter="minmod", second_order=True, low_mach_fix=False,
+                    low_mach_cutoff=0.3, stab_mach_floor=0.0, precond_farfield_bc=False, precond=False,
+                    mach_ref=0.15, mach_min=1e-3, beta_min=0.2, beta_max=1.0, all_speed_fix=False,
+                    all_speed_cutoff=0.25, all_speed_f_min=0.05, all_speed_ramp=1.0) -> _lib.ResidualConfig:
+    """EulerResidualAssemblyConfig scalars (solvers/euler/residual_assembly.hpp:12-31)."""
+    f = lambda v: _lib.FLUX[v] if isinstance(v, str) else int(v)
+    l = lambda v: _lib.LIMITER[v] if isinstance(v, str) else int(v)
+    return _lib.ResidualConfig((C.c_double * 5)(*[float(x) for x in farfield]), gamma, f(flux), l(limiter),
+                               int(bool(second_order)), int(bool(low_mach_fix)), low_mach_cutoff, stab_mach_floor,
+                               int(bool(precond_farfield_bc)), int(bool(precond)), mach_ref, mach_min, beta_min,
+                               beta_max, int(bool(all_speed_fix)), all_speed_cutoff, all_speed_f_min, all_speed_ramp)
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def hip_available() -> bool:
+    err = _lib.errbuf()
+    return bool(_lib.lib().aero_hip_available(err, len(err)))
+
+
+class HipMesh:
+    """A mesh resident on the GPU with its state buffers and stream."""
+
+    def __init__(self, mesh: Mesh, renumber: bool = True, block_cells: int = 256):
+        self.lib = _lib.lib()
+        self.mesh = mesh
+        self.N, self.F = mesh.num_cells, mesh.num_faces
+        self._desc = mesh.desc()
+        opts = _lib.Options(1 if renumber else 0, block_cells, 0)
+        h = C.c_void_p()
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_mesh_create(C.byref(self._desc), C.byref(opts), C.byref(h), err, len(err)),
+                   err, "aero_hip_mesh_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aero_hip_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _call(self, name, *args):
+        err = _lib.errbuf()
+        _lib.check(getattr(self.lib, name)(self.h, *args, err, len(err)), err, name)
+
+    def residual(self, U: np.ndarray, cfg: _lib.ResidualConfig):
+        """One residual evaluation: U[N,5] -> (R[N,5], lambda[N], diag dict)."""
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.N, 5)
+        R = np.zeros((self.N, 5))
+        lam = np.zeros(self.N)
+        d = _lib.ResidualDiag()
+        self._call("aero_hip_residual", C.byref(cfg), _pd(U), _pd(R), _pd(lam), C.byref(d))
+        return R, lam, dict(first_order_fallback=d.first_order_fallback, diffusive_fallback=d.diffusive_fallback,
+                            first_failure_face=d.first_failure_face)
+
+    def set_state(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.N, 5)
+        self._call("aero_hip_set_state", _pd(U))
+
+    def residual_async(self, cfg: _lib.ResidualConfig):
+        self._call("aero_hip_residual_async", C.byref(cfg))
+
+    def get_residual(self):
+        R = np.zeros((self.N, 5))
+        lam = np.zeros(self.N)
+        d = _lib.ResidualDiag()
+        self._call("aero_hip_get_residual", _pd(R), _pd(lam), C.byref(d))
+        return R, lam, dict(first_order_fallback=d.first_order_fallback, diffusive_fallback=d.diffusive_fallback,
+                            first_failure_face=d.first_failure_face)
+
+    def gradients(self) -> np.ndarray:
+        g = np.zeros((self.N, 5, 2))
+        self._call("aero_hip_get_gradients", _pd(g))
+        return g
+
+    def timing_begin(self, max_evals: int):
+        self._call("aero_hip_timing_begin", max_evals)
+
+    def timing_end(self) -> dict:
+        g, f, sp = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int32()
+        self._call("aero_hip_timing_end", C.byref(g), C.byref(f), C.byref(sp), C.byref(n))
+        return dict(gradient_ms=g.value, flux_ms=f.value, span_ms=sp.value, n=n.value)
+
+    def layout(self) -> dict:
+        a = (C.c_int64 * 8)()
+        self.lib.aero_hip_layout(self.h, a)
+        return dict(cells=a[0], faces=a[1], blocks=a[2], block_cells=a[3], slots=a[4], max_slots=a[5], grid=a[7])
+
+    def stream(self) -> int:
+        return self.lib.aero_hip_stream(self.h) or 0
+
+    def stage(self, cfg: _lib.ResidualConfig, stage: int, last_stage: bool, cfl: float, lts: bool = True,
+              write_residual: bool = False):
+        self._call("aero_hip_stage", C.byref(cfg), stage, int(last_stage), cfl, int(lts), int(write_residual))
+
+    def run_case(self, case: CaseConfig, hist_capacity: int | None = None) -> dict:
+        """run_euler_airfoil_case on the device (euler_solver.cpp:463-1208)."""
+        cap = hist_capacity or case.iterations
+        hist = np.zeros((cap, _lib.HIST_FIELDS))
+        U = np.zeros((self.N, 5))
+        R = np.zeros((self.N, 5))
+        lam = np.zeros(self.N)
+        forces = np.zeros(8)
+        cp = np.zeros(max(self.mesh.wall_count, 1))
+        out = _lib.RunOutput(cap, 0, 0, _pd(hist), _pd(U), _pd(R), _pd(lam), _pd(forces), _pd(cp), 0.0)
+        cc = case.to_c()
+        self._call("aero_hip_run_case", C.byref(cc), C.byref(out))
+        n = out.n_iter
+        return dict(history=hist[:n].copy(), n_iter=n, nonfinite=bool(out.nonfinite), conserved=U, residual=R,
+                    spectral_radius=lam, forces=forces, cl=forces[5], cd=forces[6], cm=forces[7],
+                    wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)

@@ -1,0 +1,1076 @@
+// aero_capi.cpp -- host side of the C ABI declared in include/aero_hip.h: device mesh setup,
+// residual / stage entry points and the device-resident pseudo-time driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "aero_hip.h"
+#include "aero_internal.hpp"
+
+using aero_dev::DevMesh;
+using aero_dev::FaceOpts;
+using aero_dev::StageArgs;
+using aero_dev::WallArgs;
+
+namespace {
+
+void set_err(char* err, size_t errlen, const std::string& msg) {
+  if (err && errlen > 0) {
+    std::strncpy(err, msg.c_str(), errlen - 1);
+    err[errlen - 1] = '\0';
+  }
+}
+
+#define AERO_HIP_TRY(expr, ctx)                                                         \
+  do {                                                                                  \
+    const hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                             \
+      set_err(err, errlen, std::string(ctx) + ": " + hipGetErrorString(e_));            \
+      return AERO_EHIP;                                                                 \
+    }                                                                                   \
+  } while (0)
+
+__global__ void k_init_diag(int* d) {
+  if (threadIdx.x == 0) {
+    d[0] = 0;
+    d[1] = 0;
+    d[2] = INT_MAX;
+    d[3] = 0;
+  }
+}
+
+// aoa = 0 symmetry enforcement on the state (euler_solver.cpp:380-410).
+__global__ void k_aoa0_symmetry(int N, int ld, double* U, double g) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  aero_dev::C4 s{U[c], U[ld + c], U[2 * ld + c], U[3 * ld + c]};
+  aero_dev::P4 p = aero_dev::c2p(s, g);
+  p.v = 0.0;
+  const aero_dev::C4 y = aero_dev::p2c(p, g);
+  U[c] = y.r;
+  U[ld + c] = y.ru;
+  U[2 * ld + c] = y.rv;
+  U[3 * ld + c] = y.E;
+}
+
+// Recursive coordinate bisection of cell centres into blocks of BC cells (every block full except the
+// last); cells inside a block keep increasing reference order.
+void rcb(std::vector<int>& idx, int lo, int hi, const double* cc, int BC) {
+  const int n = hi - lo;
+  if (n <= BC) {
+    std::sort(idx.begin() + lo, idx.begin() + hi);
+    return;
+  }
+  const int nb = (n + BC - 1) / BC;
+  const int left = (nb / 2) * BC;
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (int i = lo; i < hi; ++i) {
+    const double x = cc[3 * idx[i]], y = cc[3 * idx[i] + 1];
+    x0 = std::min(x0, x);
+    x1 = std::max(x1, x);
+    y0 = std::min(y0, y);
+    y1 = std::max(y1, y);
+  }
+  const int ax = (x1 - x0 >= y1 - y0) ? 0 : 1;
+  std::nth_element(idx.begin() + lo, idx.begin() + lo + left, idx.begin() + hi, [&](int a, int b) {
+    const double va = cc[3 * a + ax], vb = cc[3 * b + ax];
+    return va < vb || (va == vb && a < b);
+  });
+  rcb(idx, lo, lo + left, cc, BC);
+  rcb(idx, lo + left, hi, cc, BC);
+}
+
+FaceOpts face_opts(const aero_residual_config* c) {
+  FaceOpts o;
+  o.ff = aero_dev::C4{c->farfield[0], c->farfield[1], c->farfield[2], c->farfield[4]};
+  o.g = c->gamma;
+  o.low_mach_fix = c->low_mach_fix;
+  o.low_mach_cutoff = c->low_mach_cutoff;
+  o.stab_floor = c->stab_mach_floor;
+  o.char_bc = c->precond_farfield_bc;
+  o.precond = c->precond_enabled;
+  o.mach_ref = c->precond_mach_ref;
+  o.mach_min = c->precond_mach_min;
+  o.beta_min = c->precond_beta_min;
+  o.beta_max = c->precond_beta_max;
+  o.as_fix = c->all_speed_fix;
+  o.as_cut = c->all_speed_cutoff;
+  o.as_fmin = c->all_speed_f_min;
+  o.as_w = c->all_speed_ramp;
+  return o;
+}
+
+}  // namespace
+
+struct aero_hip_mesh {
+  int N = 0, F = 0, BC = 256, nblk = 0, S = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> perm;   // internal -> reference cell
+  std::vector<int> iperm;  // reference -> internal cell
+  std::vector<void*> allocs;
+  DevMesh dm{};
+  double* U[3] = {nullptr, nullptr, nullptr};
+  int cur = 0, base = 0;
+  double* G = nullptr;
+  double* R = nullptr;
+  double* lam = nullptr;
+  int* diag = nullptr;
+  double* partials = nullptr;
+  double* dt_scratch = nullptr;
+  double* iter_out = nullptr;
+  double* cp = nullptr;
+  WallArgs wall{};
+  double* h_iter = nullptr;  // pinned: 9 doubles + 4 ints
+  std::vector<hipEvent_t> tev;  // per-kernel timing events (3 per residual evaluation)
+  int tcap = 0, tcount = 0;
+  std::vector<double> h_nx, h_ny, h_area, h_fc;  // for host-side diagnostics
+  std::vector<int> h_owner, h_neighbor;
+  int wall_start = 0, wall_count = 0;
+};
+
+namespace {
+
+template <class T> hipError_t dev_alloc(aero_hip_mesh* m, T** p, size_t n) {
+  void* q = nullptr;
+  const hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    m->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+  }
+  return e;
+}
+
+template <class T> hipError_t upload(aero_hip_mesh* m, T** p, const std::vector<T>& v) {
+  hipError_t e = dev_alloc(m, p, v.size());
+  if (e != hipSuccess) return e;
+  if (!v.empty()) e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+void free_events(aero_hip_mesh* m) {
+  for (hipEvent_t e : m->tev) (void)hipEventDestroy(e);
+  m->tev.clear();
+  m->tcap = m->tcount = 0;
+}
+
+void free_all(aero_hip_mesh* m) {
+  if (!m) return;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  free_events(m);
+  for (void* p : m->allocs) (void)hipFree(p);
+  m->allocs.clear();
+  if (m->h_iter) (void)hipHostFree(m->h_iter);
+  m->h_iter = nullptr;
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  m->stream = nullptr;
+}
+
+int to_internal_state(aero_hip_mesh* m, const double* conserved, std::vector<double>* soa) {
+  soa->assign(static_cast<size_t>(m->N) * 4, 0.0);
+  for (int c = 0; c < m->N; ++c) {
+    const double* u = conserved + 5 * static_cast<size_t>(m->perm[c]);
+    (*soa)[c] = u[0];
+    (*soa)[m->N + c] = u[1];
+    (*soa)[2 * static_cast<size_t>(m->N) + c] = u[2];
+    (*soa)[3 * static_cast<size_t>(m->N) + c] = u[4];
+  }
+  return 0;
+}
+
+void from_internal4(const aero_hip_mesh* m, const std::vector<double>& soa, double* aos5) {
+  for (int c = 0; c < m->N; ++c) {
+    double* o = aos5 + 5 * static_cast<size_t>(m->perm[c]);
+    o[0] = soa[c];
+    o[1] = soa[m->N + c];
+    o[2] = soa[2 * static_cast<size_t>(m->N) + c];
+    o[3] = 0.0;
+    o[4] = soa[3 * static_cast<size_t>(m->N) + c];
+  }
+}
+
+bool env_on(const char* name) {  // euler_solver.cpp:38-45
+  const char* v = std::getenv(name);
+  if (!v) return false;
+  const std::string t(v);
+  return t == "1" || t == "true" || t == "TRUE" || t == "on" || t == "ON";
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_hip_available(char* err, size_t errlen) {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    set_err(err, errlen, std::string("hipGetDeviceCount: ") + (e != hipSuccess ? hipGetErrorString(e) : "no device"));
+    return 0;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    set_err(err, errlen, "hipGetDeviceProperties failed");
+    return 0;
+  }
+  if (std::strstr(prop.gcnArchName, "gfx950") == nullptr) {
+    set_err(err, errlen, std::string("device 0 is ") + prop.gcnArchName + ", this library is built for gfx950");
+    return 0;
+  }
+  return 1;
+}
+
+int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, aero_hip_mesh** out, char* err,
+                         size_t errlen) {
+  if (!d || !out) {
+    set_err(err, errlen, "aero_hip_mesh_create: null argument");
+    return AERO_EINVAL;
+  }
+  *out = nullptr;
+  const int N = d->num_cells, F = d->num_faces;
+  if (N <= 0 || F <= 0 || !d->face_owner || !d->face_neighbor || !d->face_normal || !d->face_area ||
+      !d->cell_volume || !d->cell_center || !d->face_center) {
+    set_err(err, errlen, "aero_hip_mesh_create: empty mesh or missing arrays");
+    return AERO_EINVAL;
+  }
+  for (int f = 0; f < F; ++f) {
+    const int o = d->face_owner[f], n = d->face_neighbor[f];
+    if (o < 0 || o >= N || n >= N || n == o || n < -1) {
+      set_err(err, errlen, "aero_hip_mesh_create: face " + std::to_string(f) + " has an invalid owner/neighbor");
+      return AERO_EINVAL;
+    }
+  }
+  if (d->wall_count < 0 || d->wall_start < 0 || d->wall_start + d->wall_count > F) {
+    set_err(err, errlen, "aero_hip_mesh_create: wall patch range out of bounds");
+    return AERO_EINVAL;
+  }
+  int BC = opts && opts->block_cells > 0 ? opts->block_cells : 256;
+  if (BC != 128 && BC != 256 && BC != 512) {
+    set_err(err, errlen, "aero_hip_mesh_create: block_cells must be 128, 256 or 512");
+    return AERO_EINVAL;
+  }
+  const bool renumber = opts ? opts->renumber != 0 : true;
+  char why[256] = {0};
+  if (!aero_hip_available(why, sizeof(why))) {
+    set_err(err, errlen, std::string("aero_hip_mesh_create: no gfx950 device: ") + why);
+    return AERO_ENODEV;
+  }
+
+  aero_hip_mesh* m = new aero_hip_mesh();
+  m->N = N;
+  m->F = F;
+  m->BC = BC;
+  m->wall_start = d->wall_start;
+  m->wall_count = d->wall_count;
+  (void)hipGetDevice(&m->device);
+  // Face normals re-normalised like the driver does before any use (mesh_geometry.cpp:49-80,
+  // euler_solver.cpp:475-478): only normals deviating by more than 1e-5 from unit length change.
+  m->h_nx.resize(F);
+  m->h_ny.resize(F);
+  for (int f = 0; f < F; ++f) {
+    const double nx = d->face_normal[3 * f], ny = d->face_normal[3 * f + 1], nz = d->face_normal[3 * f + 2];
+    const double nrm = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (!std::isfinite(nrm) || nrm < 1.0e-12) {
+      delete m;
+      set_err(err, errlen, "Mesh contains invalid face normals.");
+      return AERO_EINVAL;
+    }
+    if (std::abs(nrm - 1.0) > 1.0e-5) {
+      const double inv = 1.0 / nrm;
+      m->h_nx[f] = nx * inv;
+      m->h_ny[f] = ny * inv;
+    } else {
+      m->h_nx[f] = nx;
+      m->h_ny[f] = ny;
+    }
+  }
+  m->h_owner.assign(d->face_owner, d->face_owner + F);
+  m->h_neighbor.assign(d->face_neighbor, d->face_neighbor + F);
+
+  // Internal cell order.
+  m->perm.resize(N);
+  for (int i = 0; i < N; ++i) m->perm[i] = i;
+  if (renumber) rcb(m->perm, 0, N, d->cell_center, BC);
+  m->iperm.resize(N);
+  for (int i = 0; i < N; ++i) m->iperm[m->perm[i]] = i;
+  m->nblk = (N + BC - 1) / BC;
+
+  // Per-cell face lists in increasing reference face index.
+  std::vector<int> cf_off(N + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    ++cf_off[d->face_owner[f] + 1];
+    if (d->face_neighbor[f] >= 0) ++cf_off[d->face_neighbor[f] + 1];
+  }
+  for (int c = 0; c < N; ++c) cf_off[c + 1] += cf_off[c];
+  std::vector<int> cf(cf_off[N]);
+  {
+    std::vector<int> fill(cf_off.begin(), cf_off.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      cf[fill[d->face_owner[f]]++] = f;
+      if (d->face_neighbor[f] >= 0) cf[fill[d->face_neighbor[f]]++] = f;
+    }
+  }
+  // Per-block slot tables and per-cell gather entries.
+  std::vector<int> slot_off(m->nblk + 1, 0), s_o, s_n, s_meta, s_face;
+  std::vector<double> s_nx, s_ny, s_A, s_fx, s_fy;
+  std::vector<int> c_off(N + 1, 0), c_ent;
+  c_ent.reserve(cf.size());
+  std::vector<int> stamp(F, -1), local(F, -1), bf;
+  int smax = 0;
+  for (int b = 0; b < m->nblk; ++b) {
+    bf.clear();
+    const int c0 = b * BC, c1 = std::min(N, c0 + BC);
+    for (int c = c0; c < c1; ++c) {
+      const int oc = m->perm[c];
+      for (int e = cf_off[oc]; e < cf_off[oc + 1]; ++e)
+        if (stamp[cf[e]] != b) {
+          stamp[cf[e]] = b;
+          bf.push_back(cf[e]);
+        }
+    }
+    std::sort(bf.begin(), bf.end());
+    slot_off[b + 1] = slot_off[b] + static_cast<int>(bf.size());
+    smax = std::max(smax, static_cast<int>(bf.size()));
+    for (int i = 0; i < static_cast<int>(bf.size()); ++i) {
+      const int f = bf[i];
+      local[f] = i;
+      const int o = d->face_owner[f], n = d->face_neighbor[f];
+      const int io = m->iperm[o];
+      s_o.push_back(io);
+      s_n.push_back(n >= 0 ? m->iperm[n] : -1);
+      int bc = 0;
+      if (n < 0) bc = (d->face_bc && d->face_bc[f] == AERO_BC_SLIP_WALL) ? 2 : 1;
+      s_meta.push_back(bc | ((io / BC) == b ? aero_dev::kMetaPrimary : 0));
+      s_face.push_back(f);
+      s_nx.push_back(m->h_nx[f]);
+      s_ny.push_back(m->h_ny[f]);
+      s_A.push_back(d->face_area[f]);
+      s_fx.push_back(d->face_center[3 * f]);
+      s_fy.push_back(d->face_center[3 * f + 1]);
+    }
+    for (int c = c0; c < c1; ++c) {
+      const int oc = m->perm[c];
+      for (int e = cf_off[oc]; e < cf_off[oc + 1]; ++e) {
+        const int f = cf[e];
+        c_ent.push_back((local[f] << 1) | (d->face_neighbor[f] == oc ? 1 : 0));
+      }
+      c_off[c + 1] = static_cast<int>(c_ent.size());
+    }
+  }
+  m->S = slot_off[m->nblk];
+  if (static_cast<size_t>(smax) * 6 * sizeof(double) > 64 * 1024) {
+    delete m;
+    set_err(err, errlen, "aero_hip_mesh_create: a block touches " + std::to_string(smax) +
+                           " faces; use renumber=1 or a smaller block_cells");
+    return AERO_EINVAL;
+  }
+  std::vector<double> vol(N), ccx(N), ccy(N);
+  for (int c = 0; c < N; ++c) {
+    vol[c] = d->cell_volume[m->perm[c]];
+    ccx[c] = d->cell_center[3 * m->perm[c]];
+    ccy[c] = d->cell_center[3 * m->perm[c] + 1];
+  }
+  std::vector<int> w_o(d->wall_count);
+  std::vector<double> w_nx(d->wall_count), w_ny(d->wall_count), w_A(d->wall_count), w_fx(d->wall_count),
+    w_fy(d->wall_count);
+  for (int i = 0; i < d->wall_count; ++i) {
+    const int f = d->wall_start + i;
+    w_o[i] = m->iperm[d->face_owner[f]];
+    w_nx[i] = m->h_nx[f];
+    w_ny[i] = m->h_ny[f];
+    w_A[i] = d->face_area[f];
+    w_fx[i] = d->face_center[3 * f];
+    w_fy[i] = d->face_center[3 * f + 1];
+  }
+  m->h_area.assign(d->face_area, d->face_area + F);
+
+#define AERO_SETUP(expr)                                                             \
+  do {                                                                               \
+    const hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(err, errlen, std::string("aero_hip_mesh_create: ") + hipGetErrorString(e_)); \
+      free_all(m);                                                                   \
+      delete m;                                                                      \
+      return AERO_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+  AERO_SETUP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  int *p_slot_off, *p_so, *p_sn, *p_meta, *p_face, *p_coff, *p_cent, *p_wo;
+  double *p_nx, *p_ny, *p_A, *p_fx, *p_fy, *p_vol, *p_ccx, *p_ccy, *p_wnx, *p_wny, *p_wA, *p_wfx, *p_wfy;
+  AERO_SETUP(upload(m, &p_slot_off, slot_off));
+  AERO_SETUP(upload(m, &p_so, s_o));
+  AERO_SETUP(upload(m, &p_sn, s_n));
+  AERO_SETUP(upload(m, &p_meta, s_meta));
+  AERO_SETUP(upload(m, &p_face, s_face));
+  AERO_SETUP(upload(m, &p_nx, s_nx));
+  AERO_SETUP(upload(m, &p_ny, s_ny));
+  AERO_SETUP(upload(m, &p_A, s_A));
+  AERO_SETUP(upload(m, &p_fx, s_fx));
+  AERO_SETUP(upload(m, &p_fy, s_fy));
+  AERO_SETUP(upload(m, &p_coff, c_off));
+  AERO_SETUP(upload(m, &p_cent, c_ent));
+  AERO_SETUP(upload(m, &p_vol, vol));
+  AERO_SETUP(upload(m, &p_ccx, ccx));
+  AERO_SETUP(upload(m, &p_ccy, ccy));
+  AERO_SETUP(upload(m, &p_wo, w_o));
+  AERO_SETUP(upload(m, &p_wnx, w_nx));
+  AERO_SETUP(upload(m, &p_wny, w_ny));
+  AERO_SETUP(upload(m, &p_wA, w_A));
+  AERO_SETUP(upload(m, &p_wfx, w_fx));
+  AERO_SETUP(upload(m, &p_wfy, w_fy));
+  for (int k = 0; k < 3; ++k) AERO_SETUP(dev_alloc(m, &m->U[k], static_cast<size_t>(4) * N));
+  AERO_SETUP(dev_alloc(m, &m->G, static_cast<size_t>(8) * N));
+  AERO_SETUP(dev_alloc(m, &m->R, static_cast<size_t>(4) * N));
+  AERO_SETUP(dev_alloc(m, &m->lam, static_cast<size_t>(N)));
+  AERO_SETUP(dev_alloc(m, &m->diag, 4));
+  AERO_SETUP(dev_alloc(m, &m->partials, static_cast<size_t>(4) * m->nblk));
+  AERO_SETUP(dev_alloc(m, &m->dt_scratch, 1));
+  AERO_SETUP(dev_alloc(m, &m->iter_out, 16));
+  AERO_SETUP(dev_alloc(m, &m->cp, std::max(1, d->wall_count)));
+  AERO_SETUP(hipHostMalloc(reinterpret_cast<void**>(&m->h_iter), 32 * sizeof(double), hipHostMallocDefault));
+  AERO_SETUP(hipMemsetAsync(m->U[0], 0, static_cast<size_t>(4) * N * sizeof(double), m->stream));
+  AERO_SETUP(hipMemsetAsync(m->G, 0, static_cast<size_t>(8) * N * sizeof(double), m->stream));
+  AERO_SETUP(hipStreamSynchronize(m->stream));
+#undef AERO_SETUP
+
+  DevMesh& dm = m->dm;
+  dm.N = N;
+  dm.ld = N;
+  dm.nblk = m->nblk;
+  dm.BC = BC;
+  dm.smax = std::max(smax, 1);
+  dm.xcd_remap = renumber ? 1 : 0;
+  dm.grid = dm.xcd_remap ? ((m->nblk + 7) / 8) * 8 : m->nblk;
+  dm.slot_off = p_slot_off;
+  dm.s_o = p_so;
+  dm.s_n = p_sn;
+  dm.s_meta = p_meta;
+  dm.s_face = p_face;
+  dm.s_nx = p_nx;
+  dm.s_ny = p_ny;
+  dm.s_A = p_A;
+  dm.s_fx = p_fx;
+  dm.s_fy = p_fy;
+  dm.c_off = p_coff;
+  dm.c_ent = p_cent;
+  dm.vol = p_vol;
+  dm.ccx = p_ccx;
+  dm.ccy = p_ccy;
+  m->wall.n = d->wall_count;
+  m->wall.owner = p_wo;
+  m->wall.nx = p_wnx;
+  m->wall.ny = p_wny;
+  m->wall.A = p_wA;
+  m->wall.fx = p_wfx;
+  m->wall.fy = p_wfy;
+  m->cur = m->base = 0;
+  *out = m;
+  return AERO_OK;
+}
+
+void aero_hip_mesh_destroy(aero_hip_mesh* m) {
+  if (!m) return;
+  free_all(m);
+  delete m;
+}
+
+void* aero_hip_stream(aero_hip_mesh* m) { return m ? static_cast<void*>(m->stream) : nullptr; }
+
+int aero_hip_set_state(aero_hip_mesh* m, const double* conserved, char* err, size_t errlen) {
+  if (!m || !conserved) {
+    set_err(err, errlen, "aero_hip_set_state: null argument");
+    return AERO_EINVAL;
+  }
+  std::vector<double> soa;
+  to_internal_state(m, conserved, &soa);
+  m->cur = m->base = 0;
+  AERO_HIP_TRY(hipMemcpyAsync(m->U[0], soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, m->stream),
+               "aero_hip_set_state: hipMemcpyAsync");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_set_state: hipStreamSynchronize");
+  return AERO_OK;
+}
+
+int aero_hip_residual_async(aero_hip_mesh* m, const aero_residual_config* cfg, char* err, size_t errlen) {
+  if (!m || !cfg) {
+    set_err(err, errlen, "aero_hip_residual_async: null argument");
+    return AERO_EINVAL;
+  }
+  if (cfg->flux_scheme < 0 || cfg->flux_scheme > 1 || cfg->limiter < 0 || cfg->limiter > 1) {
+    set_err(err, errlen, "aero_hip_residual_async: unknown flux scheme or limiter");
+    return AERO_EINVAL;
+  }
+  const FaceOpts o = face_opts(cfg);
+  hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
+  AERO_HIP_TRY(hipGetLastError(), "aero_hip_residual_async: k_init_diag");
+  const double* U = m->U[m->cur];
+  const bool timed = m->tcount < m->tcap;
+  if (timed) AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 0], m->stream), "hipEventRecord");
+  if (cfg->second_order) AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream), "k_gradient");
+  if (timed) AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 1], m->stream), "hipEventRecord");
+  StageArgs st{};
+  AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 0, U, m->G,
+                                      m->R, m->lam, m->diag, st, m->stream),
+               "k_flux");
+  if (timed) {
+    AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 2], m->stream), "hipEventRecord");
+    ++m->tcount;
+  }
+  return AERO_OK;
+}
+
+int aero_hip_timing_begin(aero_hip_mesh* m, int32_t max_evals, char* err, size_t errlen) {
+  if (!m || max_evals < 0) {
+    set_err(err, errlen, "aero_hip_timing_begin: bad argument");
+    return AERO_EINVAL;
+  }
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_timing_begin: hipStreamSynchronize");
+  free_events(m);
+  m->tev.resize(static_cast<size_t>(3) * max_evals);
+  for (auto& e : m->tev) AERO_HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  m->tcap = max_evals;
+  m->tcount = 0;
+  return AERO_OK;
+}
+
+int aero_hip_timing_end(aero_hip_mesh* m, double* gradient_ms, double* flux_ms, double* span_ms, int32_t* n_evals,
+                        char* err, size_t errlen) {
+  if (!m) {
+    set_err(err, errlen, "aero_hip_timing_end: null mesh");
+    return AERO_EINVAL;
+  }
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_timing_end: hipStreamSynchronize");
+  double g = 0.0, f = 0.0, span = 0.0;
+  for (int i = 0; i < m->tcount; ++i) {
+    float a = 0.f, b = 0.f;
+    AERO_HIP_TRY(hipEventElapsedTime(&a, m->tev[3 * i], m->tev[3 * i + 1]), "hipEventElapsedTime");
+    AERO_HIP_TRY(hipEventElapsedTime(&b, m->tev[3 * i + 1], m->tev[3 * i + 2]), "hipEventElapsedTime");
+    g += a;
+    f += b;
+  }
+  if (m->tcount > 0) {
+    float a = 0.f;
+    AERO_HIP_TRY(hipEventElapsedTime(&a, m->tev[0], m->tev[3 * (m->tcount - 1) + 2]), "hipEventElapsedTime");
+    span = a;
+  }
+  if (gradient_ms) *gradient_ms = g;
+  if (flux_ms) *flux_ms = f;
+  if (span_ms) *span_ms = span;
+  if (n_evals) *n_evals = m->tcount;
+  free_events(m);
+  return AERO_OK;
+}
+
+int aero_hip_layout(aero_hip_mesh* m, int64_t* out8) {
+  if (!m || !out8) return AERO_EINVAL;
+  out8[0] = m->N;
+  out8[1] = m->F;
+  out8[2] = m->nblk;
+  out8[3] = m->BC;
+  out8[4] = m->S;
+  out8[5] = m->dm.smax;
+  out8[6] = m->dm.c_off ? 0 : 0;
+  out8[7] = m->dm.grid;
+  return AERO_OK;
+}
+
+int aero_hip_get_residual(aero_hip_mesh* m, double* residual, double* spectral_radius, aero_residual_diag* diag,
+                          char* err, size_t errlen) {
+  if (!m) {
+    set_err(err, errlen, "aero_hip_get_residual: null mesh");
+    return AERO_EINVAL;
+  }
+  std::vector<double> r(static_cast<size_t>(4) * m->N), l(m->N);
+  int dg[4];
+  AERO_HIP_TRY(hipMemcpyAsync(r.data(), m->R, r.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream), "R copy");
+  AERO_HIP_TRY(hipMemcpyAsync(l.data(), m->lam, l.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+               "lambda copy");
+  AERO_HIP_TRY(hipMemcpyAsync(dg, m->diag, sizeof(dg), hipMemcpyDeviceToHost, m->stream), "diag copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_get_residual: hipStreamSynchronize");
+  if (residual) from_internal4(m, r, residual);
+  if (spectral_radius)
+    for (int c = 0; c < m->N; ++c) spectral_radius[m->perm[c]] = l[c];
+  if (diag) {
+    diag->first_order_fallback = dg[0];
+    diag->diffusive_fallback = dg[1];
+    diag->first_failure_face = dg[2] == INT_MAX ? -1 : dg[2];
+  }
+  return AERO_OK;
+}
+
+int aero_hip_get_gradients(aero_hip_mesh* m, double* gradients, char* err, size_t errlen) {
+  if (!m || !gradients) {
+    set_err(err, errlen, "aero_hip_get_gradients: null argument");
+    return AERO_EINVAL;
+  }
+  std::vector<double> g(static_cast<size_t>(8) * m->N);
+  AERO_HIP_TRY(hipMemcpyAsync(g.data(), m->G, g.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream), "G copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_get_gradients: hipStreamSynchronize");
+  const size_t N = m->N;
+  for (int c = 0; c < m->N; ++c) {
+    double* o = gradients + 10 * static_cast<size_t>(m->perm[c]);
+    o[0] = g[0 * N + c];
+    o[1] = g[1 * N + c];
+    o[2] = g[2 * N + c];
+    o[3] = g[3 * N + c];
+    o[4] = g[4 * N + c];
+    o[5] = g[5 * N + c];
+    o[6] = 0.0;
+    o[7] = 0.0;
+    o[8] = g[6 * N + c];
+    o[9] = g[7 * N + c];
+  }
+  return AERO_OK;
+}
+
+int aero_hip_residual(aero_hip_mesh* m, const aero_residual_config* cfg, const double* conserved, double* residual,
+                      double* spectral_radius, aero_residual_diag* diag, char* err, size_t errlen) {
+  int rc = aero_hip_set_state(m, conserved, err, errlen);
+  if (rc) return rc;
+  rc = aero_hip_residual_async(m, cfg, err, errlen);
+  if (rc) return rc;
+  return aero_hip_get_residual(m, residual, spectral_radius, diag, err, errlen);
+}
+
+int aero_hip_stage(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t stage, int32_t last_stage, double cfl,
+                   int32_t lts, int32_t write_residual, char* err, size_t errlen) {
+  if (!m || !cfg) {
+    set_err(err, errlen, "aero_hip_stage: null argument");
+    return AERO_EINVAL;
+  }
+  if (stage < 0 || stage > 2) {
+    set_err(err, errlen, "aero_hip_stage: stage must be 0, 1 or 2");
+    return AERO_EINVAL;
+  }
+  if (stage == 0) m->base = m->cur;
+  int out = 0;
+  while (out == m->cur || out == m->base) ++out;
+  const FaceOpts o = face_opts(cfg);
+  const double* U = m->U[m->cur];
+  StageArgs st{};
+  st.stage = stage;
+  st.last_stage = last_stage;
+  st.write_res = write_residual || !lts;
+  st.cfl = cfl;
+  st.Un = m->U[m->base];
+  st.Unext = m->U[out];
+  st.partials = m->partials;
+  if (cfg->second_order) AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream), "k_gradient");
+  if (lts) {
+    AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 1, U, m->G,
+                                        m->R, m->lam, m->diag, st, m->stream),
+                 "k_flux(stage)");
+  } else {
+    AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 0, U, m->G,
+                                        m->R, m->lam, m->diag, st, m->stream),
+                 "k_flux");
+    AERO_HIP_TRY(aero_host::launch_global_update(m->dm, o, U, m->R, m->lam, cfl, m->dt_scratch, m->diag, st, m->stream),
+                 "global update");
+    if (last_stage) AERO_HIP_TRY(aero_host::launch_norm_partials(m->dm, m->R, m->partials, m->stream), "norms");
+  }
+  m->cur = out;
+  return AERO_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// The driver: run_euler_airfoil_case (euler_solver.cpp:463-1208) on the device-resident state.
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct Coef { double fx, fy, lift, drag, moment, cl, cd, cm; };
+
+class Monitor {  // convergence_monitor.cpp:7-127
+ public:
+  Monitor(double target, double tol, int window, int min_it, double drift)
+      : target_(target), tol_(tol), window_(window), min_it_(min_it), drift_(drift > 0.0 ? drift : tol) {}
+  double update_residual(double l2) {
+    const double b = std::max(l2, 1.0e-20);
+    if (init_ < 0.0) init_ = b;
+    return b / std::max(init_, 1.0e-20);
+  }
+  void update_forces(const Coef& f) {
+    const int req = std::max(window_, 1);
+    win_.push_back(f);
+    while (static_cast<int>(win_.size()) > 2 * req) win_.pop_front();
+  }
+  bool stable() const {
+    const int req = std::max(window_, 1);
+    if (static_cast<int>(win_.size()) < req) return false;
+    const int b = static_cast<int>(win_.size()) - req;
+    double l0 = win_[b].cl, l1 = l0, d0 = win_[b].cd, d1 = d0, m0 = win_[b].cm, m1 = m0;
+    for (int i = b; i < static_cast<int>(win_.size()); ++i) {
+      l0 = std::min(l0, win_[i].cl);
+      l1 = std::max(l1, win_[i].cl);
+      d0 = std::min(d0, win_[i].cd);
+      d1 = std::max(d1, win_[i].cd);
+      m0 = std::min(m0, win_[i].cm);
+      m1 = std::max(m1, win_[i].cm);
+    }
+    return (l1 - l0) <= tol_ && (d1 - d0) <= tol_ && (m1 - m0) <= tol_;
+  }
+  bool drift_stable() const {
+    const int req = std::max(window_, 1);
+    if (static_cast<int>(win_.size()) < 2 * req) return false;
+    const int pb = static_cast<int>(win_.size()) - 2 * req, cb = static_cast<int>(win_.size()) - req;
+    double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    for (int i = pb; i < cb; ++i) {
+      a0 += win_[i].cl;
+      a1 += win_[i].cd;
+      a2 += win_[i].cm;
+    }
+    for (int i = cb; i < static_cast<int>(win_.size()); ++i) {
+      b0 += win_[i].cl;
+      b1 += win_[i].cd;
+      b2 += win_[i].cm;
+    }
+    const double inv = 1.0 / static_cast<double>(req);
+    return std::abs(b0 * inv - a0 * inv) <= drift_ && std::abs(b1 * inv - a1 * inv) <= drift_ &&
+           std::abs(b2 * inv - a2 * inv) <= drift_;
+  }
+  bool converged(int iter, double ratio) const {
+    if (iter + 1 < min_it_) return false;
+    if (ratio > target_) return false;
+    return stable() && drift_stable();
+  }
+
+ private:
+  double target_, tol_;
+  int window_, min_it_;
+  double drift_;
+  double init_ = -1.0;
+  std::deque<Coef> win_;
+};
+
+double clamp(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+
+}  // namespace
+
+extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, aero_run_output* out, char* err,
+                                 size_t errlen) {
+  if (!m || !c || !out) {
+    set_err(err, errlen, "aero_hip_run_case: null argument");
+    return AERO_EINVAL;
+  }
+  if (c->iterations <= 0) {
+    set_err(err, errlen, "Euler iterations must be positive.");
+    return AERO_EINVAL;
+  }
+  if (c->rk_stages != 1 && c->rk_stages != 3) {
+    set_err(err, errlen, "Euler pseudo-time rk_stages must be 1 or 3.");
+    return AERO_EINVAL;
+  }
+  if (m->wall_count <= 0) {
+    set_err(err, errlen, "Wall patch face set is empty.");
+    return AERO_EINVAL;
+  }
+  const double g = c->gamma;
+  const bool precond = c->precond_on && c->precond_experimental_enabled;
+  // build_precondition_config (euler_solver.cpp:341-355)
+  const double minf = std::max(c->mach, 1.0e-4);
+  const double auto_bmin = std::max(1.0e-4, std::min(1.0e-3, 0.1 * minf));
+  const double mach_ref = std::max(c->precond_mach_ref > 0.0 ? c->precond_mach_ref : minf, 1.0e-4);
+  const double mach_min = std::max(c->precond_mach_min > 0.0 ? c->precond_mach_min : 1.0e-6, 1.0e-6);
+  const double beta_min = std::max(c->precond_beta_min > 0.0 ? c->precond_beta_min : auto_bmin, 1.0e-4);
+  const double beta_max = std::max(c->precond_beta_max, beta_min);
+  // resolve_stabilization_floor_schedule (:141-157)
+  const double k0 = c->stab_floor_k_start >= 0.0 ? c->stab_floor_k_start : (c->stab_floor_start / minf);
+  const double k1 = c->stab_floor_k_target >= 0.0 ? c->stab_floor_k_target : (c->stab_floor_target / minf);
+  const double floor_start = clamp(std::max(k0, 0.0) * minf, 0.0, 1.0);
+  const double floor_target = clamp(std::max(k1, 0.0) * minf, 0.0, 1.0);
+  const int stab_ramp = std::max(c->stab_ramp_iters, 0);
+  // all-speed schedule and staged controller (:492-528)
+  const double as_fmin = clamp(c->all_speed_f_min, 1.0e-6, 1.0);
+  const int as_on = std::max(c->all_speed_ramp_start_iter, 0);
+  const int as_len = std::max(c->all_speed_ramp_iters, 0);
+  const bool staged = c->all_speed_staged_controller != 0;
+  const int a_min = std::max(c->all_speed_stage_a_min_iters, 0);
+  const int a_max = std::max(c->all_speed_stage_a_max_iters, 0);
+  const double f_high = clamp(c->all_speed_stage_f_min_high, 1.0e-6, 1.0);
+  const double f_mid = clamp(std::min(c->all_speed_stage_f_min_mid, f_high), 1.0e-6, 1.0);
+  const double f_low = clamp(std::min(c->all_speed_stage_f_min_low, f_mid), 1.0e-6, 1.0);
+  const double auto_pjump = clamp(1.6e-2 * minf * minf * minf * minf, 1.0e-9, 2.0e-6);
+  const double pjump_target = c->all_speed_pjump_wall_target <= 0.0 ? auto_pjump : c->all_speed_pjump_wall_target;
+  const double spike_factor = std::max(c->all_speed_pjump_spike_factor, 1.0);
+  const int hold_iters = std::max(c->all_speed_pjump_hold_iters, 1);
+  const int freeze_iters = std::max(c->all_speed_freeze_iters, 0);
+  const double drop = clamp(c->all_speed_cfl_drop_factor, 0.05, 1.0);
+  const double restore = clamp(c->all_speed_cfl_restore_factor, 1.0, 1.2);
+  const double cfl_min_scale = clamp(c->all_speed_cfl_min_scale, 0.01, 1.0);
+  // freestream (:543-561)
+  double rho_inf = c->rho_inf;
+  if (rho_inf <= 0.0) rho_inf = c->p_inf / (c->gas_constant * c->t_inf);
+  if (rho_inf <= 0.0) {
+    set_err(err, errlen, "Invalid freestream density. Set rho_inf or p_inf/T_inf.");
+    return AERO_EINVAL;
+  }
+  const double pi = 3.14159265358979323846;
+  const double alpha = c->aoa_deg * (pi / 180.0);
+  const double a_inf = std::sqrt(g * c->p_inf / rho_inf);
+  const double v_inf = std::max(c->mach, 0.0) * a_inf;
+  const double u_inf = v_inf * std::cos(alpha), vv_inf = v_inf * std::sin(alpha);
+  double ff[5];
+  {  // primitive_to_conservative (euler_flux.cpp:101-113)
+    const double r = std::max(rho_inf, 1.0e-8);
+    ff[0] = r;
+    ff[1] = r * u_inf;
+    ff[2] = r * vv_inf;
+    ff[3] = r * 0.0;
+    const double ke = 0.5 * r * (u_inf * u_inf + vv_inf * vv_inf + 0.0 * 0.0);
+    ff[4] = std::max(c->p_inf, 1.0e-8) / (g - 1.0) + ke;
+  }
+  {
+    std::vector<double> u0(static_cast<size_t>(m->N) * 5);
+    for (int i = 0; i < m->N; ++i) std::memcpy(&u0[5 * static_cast<size_t>(i)], ff, sizeof(ff));
+    const int rc = aero_hip_set_state(m, u0.data(), err, errlen);
+    if (rc) return rc;
+  }
+  const double q_inf = 0.5 * rho_inf * v_inf * v_inf + 1.0e-12;
+  const double pressure_ref = std::max(rho_inf * a_inf * a_inf, 1.0e-12);
+  WallArgs wa = m->wall;
+  wa.p_inf = c->p_inf;
+  wa.q_inf = q_inf;
+  wa.x_ref = c->x_ref;
+  wa.y_ref = c->y_ref;
+  const double ca = std::cos(alpha), sa = std::sin(alpha);
+  const double chord = std::max(c->chord, 1.0e-8);
+  const double sref = chord * 1.0;
+  auto coefs = [&](const double* sums) {  // forces.cpp:263-293 (gauge pressure)
+    Coef k{};
+    k.fx = -sums[6];
+    k.fy = -sums[7];
+    k.moment = sums[8];
+    k.drag = k.fx * ca + k.fy * sa;
+    k.lift = -k.fx * sa + k.fy * ca;
+    k.cd = k.drag / (q_inf * sref);
+    k.cl = k.lift / (q_inf * sref);
+    k.cm = k.moment / (q_inf * sref * chord);
+    return k;
+  };
+  std::vector<unsigned char> wall_adj;
+  if (staged) {
+    wall_adj.assign(m->N, 0u);
+    for (int i = 0; i < m->wall_count; ++i) wall_adj[m->h_owner[m->wall_start + i]] = 1u;
+  }
+
+  Monitor mon(c->residual_reduction_target, c->force_stability_tol, c->force_stability_window, c->min_iterations,
+              c->force_mean_drift_tol);
+  aero_residual_config rc{};
+  std::memcpy(rc.farfield, ff, sizeof(ff));
+  rc.gamma = g;
+  rc.flux_scheme = c->flux_scheme;
+  rc.limiter = c->limiter;
+  rc.low_mach_fix = c->low_mach_fix;
+  rc.low_mach_cutoff = c->mach_cutoff;
+  rc.precond_farfield_bc = c->precond_farfield_bc;
+  rc.precond_enabled = precond ? 1 : 0;
+  rc.precond_mach_ref = mach_ref;
+  rc.precond_mach_min = mach_min;
+  rc.precond_beta_min = beta_min;
+  rc.precond_beta_max = beta_max;
+  rc.all_speed_cutoff = c->all_speed_mach_cutoff;
+
+  double cfl_scale = 1.0, prev_l2 = -1.0;
+  int stage_ab = staged ? 1 : 0;
+  double b_fmin = f_high;
+  int b_progress = 0, below = 0, freeze = 0;
+  out->n_iter = 0;
+  out->nonfinite = 0;
+  Coef last{};
+  const double* h = m->h_iter;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int iter = 0; iter < c->iterations; ++iter) {
+    const bool second = iter >= std::max(c->startup_first_order_iters, 0);
+    double base;
+    if (c->cfl_ramp_iters <= 0) {
+      base = c->cfl_max;
+    } else {  // compute_cfl (:69-79)
+      const double t = std::min(1.0, static_cast<double>(iter + 1) / static_cast<double>(std::max(c->cfl_ramp_iters, 1)));
+      base = c->cfl_start + t * (c->cfl_max - c->cfl_start);
+    }
+    const double cfl = base * cfl_scale;
+    double floor_cur;  // compute_stabilization_mach_floor (:95-106)
+    {
+      const double s0 = clamp(floor_start, 0.0, 1.0), s1 = clamp(floor_target, 0.0, 1.0);
+      if (stab_ramp <= 0) {
+        floor_cur = s1;
+      } else {
+        const double x = clamp(static_cast<double>(std::max(iter, 0)) / static_cast<double>(std::max(stab_ramp, 1)), 0.0, 1.0);
+        floor_cur = s0 + x * (s1 - s0);
+      }
+    }
+    if (!staged) {  // (:776-797)
+      rc.all_speed_fix = c->all_speed_flux_fix;
+      rc.all_speed_f_min = as_fmin;
+      if (as_len <= 0)
+        rc.all_speed_ramp = iter >= as_on ? 1.0 : 0.0;
+      else
+        rc.all_speed_ramp = clamp(static_cast<double>(iter - as_on) / static_cast<double>(std::max(as_len, 1)), 0.0, 1.0);
+    } else if (stage_ab == 1) {
+      rc.all_speed_fix = 0;
+      rc.all_speed_f_min = f_high;
+      rc.all_speed_ramp = 0.0;
+    } else {
+      rc.all_speed_fix = c->all_speed_flux_fix;
+      rc.all_speed_f_min = clamp(b_fmin, f_low, f_high);
+      rc.all_speed_ramp =
+        as_len <= 0 ? 1.0 : clamp(static_cast<double>(b_progress) / static_cast<double>(std::max(as_len, 1)), 0.0, 1.0);
+    }
+    rc.second_order = second ? 1 : 0;
+    rc.stab_mach_floor = floor_cur;
+    hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
+    AERO_HIP_TRY(hipGetLastError(), "k_init_diag");
+    for (int st = 0; st < c->rk_stages; ++st) {
+      const int rcode = aero_hip_stage(m, &rc, st, st == c->rk_stages - 1, cfl, c->local_time_stepping, 1, err, errlen);
+      if (rcode) return rcode;
+    }
+    if (c->aoa0_symmetry_enforce && c->aoa0_symmetry_enforce_interval > 0 && !(std::abs(c->aoa_deg) > 1.0e-12) &&
+        ((iter + 1) % c->aoa0_symmetry_enforce_interval) == 0) {
+      hipLaunchKernelGGL(k_aoa0_symmetry, dim3((m->N + 255) / 256), dim3(256), 0, m->stream, m->N, m->dm.ld,
+                         m->U[m->cur], g);
+      AERO_HIP_TRY(hipGetLastError(), "k_aoa0_symmetry");
+    }
+    AERO_HIP_TRY(aero_host::launch_iter_reduce(m->dm, m->partials, wa, m->U[m->cur], g, m->iter_out, nullptr, m->stream),
+                 "k_iter_reduce");
+    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter, m->iter_out, 9 * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "iteration scalars");
+    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter + 10, m->diag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream),
+                 "iteration counters");
+    AERO_HIP_TRY(hipStreamSynchronize(m->stream), "iteration sync");
+    const int* hd = reinterpret_cast<const int*>(m->h_iter + 10);
+    if (hd[3]) {  // non-finite trap (:908-946)
+      std::fprintf(stderr, "non_finite_trap iter=%d location=conserved\n", iter);
+      out->nonfinite = 1;
+      break;
+    }
+    const double N5 = static_cast<double>(std::max(1, m->N * 5));
+    const double l1 = h[0], l2 = std::sqrt(h[1] / N5), linf = h[2];
+    const double rho_l2 = std::sqrt(h[3] / static_cast<double>(std::max(1, m->N)));
+    const double ratio = mon.update_residual(l2);
+    if (prev_l2 > 0.0) {  // (:951-958)
+      if (l2 > prev_l2 * 1.05)
+        cfl_scale = std::max(cfl_scale * 0.65, 0.05);
+      else if (l2 < prev_l2 * 0.90)
+        cfl_scale = std::min(cfl_scale * 1.03, 1.0);
+    }
+    prev_l2 = l2;
+    const Coef fg = coefs(h);
+    last = fg;
+    mon.update_forces(fg);
+    const bool window_stable = mon.stable();
+    if (staged) {  // p-jump diagnostic + staged all-speed controller (:1026-1139)
+      std::vector<double> u(static_cast<size_t>(4) * m->N);
+      AERO_HIP_TRY(hipMemcpy(u.data(), m->U[m->cur], u.size() * sizeof(double), hipMemcpyDeviceToHost), "state copy");
+      std::vector<double> p(m->N);
+      for (int ic = 0; ic < m->N; ++ic) {
+        const aero_dev::C4 s{u[ic], u[m->N + ic], u[2 * static_cast<size_t>(m->N) + ic],
+                             u[3 * static_cast<size_t>(m->N) + ic]};
+        const double r = std::max(s.r, 1.0e-8), inv = 1.0 / r, uu = s.ru * inv, vv = s.rv * inv;
+        const double ke = 0.5 * r * (uu * uu + vv * vv);
+        p[m->perm[ic]] = std::max((g - 1.0) * (s.E - ke), 1.0e-8);
+      }
+      double pj = 0.0;
+      int pn = 0;
+      for (int f = 0; f < m->F; ++f) {
+        const int o = m->h_owner[f], nb = m->h_neighbor[f];
+        if (nb < 0) continue;
+        if (wall_adj[o] || wall_adj[nb]) {
+          const double dp = p[o] - p[nb];
+          pj += dp * dp;
+          ++pn;
+        }
+      }
+      const double pref2 = pressure_ref * pressure_ref + 1.0e-24;
+      const double pjw = pn > 0 ? (pj / static_cast<double>(pn)) / pref2 : 0.0;
+      if (stage_ab == 1) {
+        const bool ready = (iter + 1 >= a_min) && window_stable;
+        const bool timeout = (a_max > 0) && (iter + 1 >= a_max);
+        if ((ready || timeout) && c->all_speed_flux_fix) {
+          stage_ab = 2;
+          b_fmin = f_high;
+          b_progress = below = freeze = 0;
+        }
+      } else if (stage_ab == 2) {
+        const double thr = pjump_target > 0.0 ? pjump_target * spike_factor : std::numeric_limits<double>::max();
+        const bool spike = pjw > thr;
+        const bool good = (pjump_target <= 0.0) || (pjw <= pjump_target);
+        if (spike) {
+          freeze = freeze_iters;
+          below = 0;
+          b_fmin = std::min(f_high, b_fmin + 0.05);
+          cfl_scale = std::max(cfl_scale * drop, cfl_min_scale);
+        } else {
+          if (freeze > 0) --freeze;
+          below = good ? below + 1 : 0;
+          if (freeze == 0) {
+            if (b_fmin > f_mid + 1.0e-6 && below >= hold_iters) {
+              b_fmin = f_mid;
+              below = 0;
+            } else if (b_fmin > f_low + 1.0e-6 && below >= hold_iters) {
+              b_fmin = f_low;
+              below = 0;
+            } else if (b_fmin <= f_low + 1.0e-6 && good) {
+              ++b_progress;
+            }
+            if (good)
+              cfl_scale = std::min(cfl_scale * restore, 1.0);
+            else if (pjump_target > 0.0 && pjw > pjump_target)
+              cfl_scale = std::max(cfl_scale * drop, cfl_min_scale);
+          }
+        }
+      }
+    }
+    if (out->history && out->n_iter < out->hist_capacity) {
+      double* row = out->history + static_cast<size_t>(out->n_iter) * AERO_HIST_FIELDS;
+      row[AERO_H_ITER] = iter;
+      row[AERO_H_L1] = l1;
+      row[AERO_H_L2] = l2;
+      row[AERO_H_LINF] = linf;
+      row[AERO_H_RATIO] = ratio;
+      row[AERO_H_CL] = fg.cl;
+      row[AERO_H_CD] = fg.cd;
+      row[AERO_H_CM] = fg.cm;
+      row[AERO_H_RHO_L2] = rho_l2;
+      row[AERO_H_CFL] = cfl;
+      row[AERO_H_FO_FALLBACK] = hd[0];
+      row[AERO_H_DIFF_FALLBACK] = hd[1];
+      ++out->n_iter;
+    }
+    if (mon.converged(iter, ratio)) break;
+  }
+  out->device_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // Final outputs in the reference numbering.
+  if (out->wall_cp) {
+    AERO_HIP_TRY(aero_host::launch_iter_reduce(m->dm, m->partials, wa, m->U[m->cur], g, m->iter_out, m->cp, m->stream),
+                 "final reduce");
+    AERO_HIP_TRY(hipMemcpyAsync(out->wall_cp, m->cp, m->wall_count * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "cp copy");
+  }
+  if (out->conserved) {
+    std::vector<double> u(static_cast<size_t>(4) * m->N);
+    AERO_HIP_TRY(hipMemcpyAsync(u.data(), m->U[m->cur], u.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "state copy");
+    AERO_HIP_TRY(hipStreamSynchronize(m->stream), "final sync");
+    from_internal4(m, u, out->conserved);
+  }
+  if (out->residual || out->spectral_radius) {
+    const int rcode = aero_hip_get_residual(m, out->residual, out->spectral_radius, nullptr, err, errlen);
+    if (rcode) return rcode;
+  }
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "final sync");
+  if (out->forces) {
+    const double f8[8] = {last.fx, last.fy, last.lift, last.drag, last.moment, last.cl, last.cd, last.cm};
+    std::memcpy(out->forces, f8, sizeof(f8));
+  }
+  return AERO_OK;
+}

@@ -1,0 +1,658 @@
+// aero_kernels.hip -- gfx950 FP64 kernels of the Euler residual path and their launchers.
+//
+// Layout (device, internal cell order):
+//   state U[4][N] SoA (rho, rhou, rhov, rhoE), gradients G[8][N] (rho_x, rho_y, u_x, u_y, v_x, v_y,
+//   p_x, p_y), residual R[4][N], spectral radius lam[N].
+//   Cells are grouped in blocks of BC consecutive cells; one workgroup owns one block.  Every face
+//   touching a block's cells is one "slot" of that block (faces on a block boundary are slots of both
+//   blocks), stored in slot-major SoA streams so that phase 1 reads them coalesced.
+// Each kernel runs in two phases:
+//   1. one thread per slot computes the face quantity (Green-Gauss face value, or the full
+//      reconstruction + Riemann flux) once per block and stages it in LDS;
+//   2. one thread per cell gathers its faces' staged values in increasing reference face index, which
+//      is the summation order of the reference's serial face loop (reconstruction.cpp:85-107,
+//      residual_assembly.cpp:97-181).  No floating-point atomics; results are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "aero_internal.hpp"
+#include "euler_device.hpp"
+
+namespace aero_dev {
+
+__device__ __forceinline__ C4 load_u(const double* __restrict__ U, int ld, int c) {
+  C4 s;
+  s.r = U[c];
+  s.ru = U[ld + c];
+  s.rv = U[2 * ld + c];
+  s.E = U[3 * ld + c];
+  return s;
+}
+
+// Workgroup -> block.  Workgroups are dispatched round-robin over the 8 XCDs; with `xcd_remap` the
+// blocks handled by one XCD form a contiguous run, so neighbouring blocks share that XCD's L2.
+__device__ __forceinline__ int block_of(const DevMesh& m) {
+  const int i = blockIdx.x;
+  if (!m.xcd_remap) return i;
+  const int per = (m.nblk + 7) / 8;
+  return (i & 7) * per + (i >> 3);
+}
+
+// Green-Gauss gradient (reconstruction.cpp:73-119), per block.
+template <int BC>
+__global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
+                                                  double g) {
+  extern __shared__ double lds[];
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  const int s0 = m.slot_off[b];
+  const int ns = m.slot_off[b + 1] - s0;
+  const int smax = m.smax;
+  double* q0 = lds;
+  double* q1 = lds + smax;
+  double* q2 = lds + 2 * smax;
+  double* q3 = lds + 3 * smax;
+  double* nax = lds + 4 * smax;
+  double* nay = lds + 5 * smax;
+  const int ld = m.ld;
+  for (int i = threadIdx.x; i < ns; i += BC) {
+    const int s = s0 + i;
+    const int o = m.s_o[s];
+    const int n = m.s_n[s];
+    const P4 wo = c2p(load_u(U, ld, o), g);
+    double a0 = wo.r, a1 = wo.u, a2 = wo.v, a3 = wo.p;
+    if (n >= 0) {
+      const P4 wn = c2p(load_u(U, ld, n), g);
+      a0 = 0.5 * (wo.r + wn.r);
+      a1 = 0.5 * (wo.u + wn.u);
+      a2 = 0.5 * (wo.v + wn.v);
+      a3 = 0.5 * (wo.p + wn.p);
+    }
+    const double A = m.s_A[s];
+    q0[i] = a0;
+    q1[i] = a1;
+    q2[i] = a2;
+    q3[i] = a3;
+    nax[i] = m.s_nx[s] * A;
+    nay[i] = m.s_ny[s] * A;
+  }
+  __syncthreads();
+  const int c = b * BC + threadIdx.x;
+  if (c >= m.N) return;
+  double gx0 = 0.0, gy0 = 0.0, gx1 = 0.0, gy1 = 0.0, gx2 = 0.0, gy2 = 0.0, gx3 = 0.0, gy3 = 0.0;
+  const int e1 = m.c_off[c + 1];
+  for (int e = m.c_off[c]; e < e1; ++e) {
+    const int ent = m.c_ent[e];
+    const int l = ent >> 1;
+    const double x = nax[l], y = nay[l];
+    const double v0 = q0[l], v1 = q1[l], v2 = q2[l], v3 = q3[l];
+    if (ent & 1) {
+      gx0 -= v0 * x; gy0 -= v0 * y;
+      gx1 -= v1 * x; gy1 -= v1 * y;
+      gx2 -= v2 * x; gy2 -= v2 * y;
+      gx3 -= v3 * x; gy3 -= v3 * y;
+    } else {
+      gx0 += v0 * x; gy0 += v0 * y;
+      gx1 += v1 * x; gy1 += v1 * y;
+      gx2 += v2 * x; gy2 += v2 * y;
+      gx3 += v3 * x; gy3 += v3 * y;
+    }
+  }
+  const double iv = 1.0 / dmax(m.vol[c], 1.0e-12);
+  G[0 * ld + c] = gx0 * iv;
+  G[1 * ld + c] = gy0 * iv;
+  G[2 * ld + c] = gx1 * iv;
+  G[3 * ld + c] = gy1 * iv;
+  G[4 * ld + c] = gx2 * iv;
+  G[5 * ld + c] = gy2 * iv;
+  G[6 * ld + c] = gx3 * iv;
+  G[7 * ld + c] = gy3 * iv;
+}
+
+// MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
+template <int LIMITER>
+__device__ __forceinline__ void recon_var(double qo, double qn, double gox, double goy, double gnx, double gny,
+                                          double dox, double doy, double dnx, double dny, double* ql, double* qr) {
+  const double dc = qn - qo;
+  double da = 0.0;
+  da += gox * dox;
+  da += goy * doy;
+  double db = 0.0;
+  db += gnx * dnx;
+  db += gny * dny;
+  *ql = qo + limiter<LIMITER>(da, dc);
+  *qr = qn + limiter<LIMITER>(db, -dc);
+}
+
+// Wavefront sum of an int (64 lanes).
+__device__ __forceinline__ int wave_sum_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
+// MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
+// preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+__global__ __launch_bounds__(BC) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
+                                              const double* __restrict__ G, double* __restrict__ R,
+                                              double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
+  extern __shared__ double lds[];
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  const int s0 = m.slot_off[b];
+  const int ns = m.slot_off[b + 1] - s0;
+  const int smax = m.smax;
+  double* f0 = lds;
+  double* f1 = lds + smax;
+  double* f2 = lds + 2 * smax;
+  double* f3 = lds + 3 * smax;
+  double* fw = lds + 4 * smax;
+  const int ld = m.ld;
+  const double g = o.g;
+  int n_fo = 0, n_diff = 0, first_fail = 0x7fffffff;
+  for (int i = threadIdx.x; i < ns; i += BC) {
+    const int s = s0 + i;
+    const int co = m.s_o[s];
+    const int cn = m.s_n[s];
+    const int meta = m.s_meta[s];
+    const double nx = m.s_nx[s];
+    const double ny = m.s_ny[s];
+    const double A = m.s_A[s];
+    Flux F;
+    if (cn >= 0) {
+      P4 L = c2p(load_u(U, ld, co), g);
+      P4 Rt = c2p(load_u(U, ld, cn), g);
+      if (SECOND) {
+        const double fx = m.s_fx[s], fy = m.s_fy[s];
+        const double dox = fx - m.ccx[co], doy = fy - m.ccy[co];
+        const double dnx = fx - m.ccx[cn], dny = fy - m.ccy[cn];
+        P4 l2, r2;
+        recon_var<LIMITER>(L.r, Rt.r, G[0 * ld + co], G[1 * ld + co], G[0 * ld + cn], G[1 * ld + cn], dox, doy, dnx,
+                           dny, &l2.r, &r2.r);
+        recon_var<LIMITER>(L.u, Rt.u, G[2 * ld + co], G[3 * ld + co], G[2 * ld + cn], G[3 * ld + cn], dox, doy, dnx,
+                           dny, &l2.u, &r2.u);
+        recon_var<LIMITER>(L.v, Rt.v, G[4 * ld + co], G[5 * ld + co], G[4 * ld + cn], G[5 * ld + cn], dox, doy, dnx,
+                           dny, &l2.v, &r2.v);
+        recon_var<LIMITER>(L.p, Rt.p, G[6 * ld + co], G[7 * ld + co], G[6 * ld + cn], G[7 * ld + cn], dox, doy, dnx,
+                           dny, &l2.p, &r2.p);
+        l2.r = dmax(l2.r, kFloor);
+        r2.r = dmax(r2.r, kFloor);
+        l2.p = dmax(l2.p, kFloor);
+        r2.p = dmax(r2.p, kFloor);
+        if (valid_prim(l2) && valid_prim(r2)) {
+          L = l2;
+          Rt = r2;
+        } else if (meta & kMetaPrimary) {
+          ++n_fo;
+          first_fail = min(first_fail, m.s_face[s]);
+        }
+      }
+      const C4 ul = p2c(L, g);
+      const C4 ur = p2c(Rt, g);
+      const double fb = face_beta(ul, ur, o);
+      const double sc = acoustic_scale(ul, ur, fb, o);
+      F = face_flux<SCHEME>(ul, ur, nx, ny, sc, o);
+    } else {
+      const C4 in = p2c(c2p(load_u(U, ld, co), g), g);
+      const bool wall = (meta & kMetaBcMask) == 2;
+      const double fb = wall ? face_beta(in, in, o) : face_beta(in, o.ff, o);
+      const double sc = acoustic_scale(in, wall ? in : o.ff, fb, o);
+      if (wall) {  // boundary_conditions.cpp:97-111
+        const P4 p = c2p(in, g);
+        const double un = p.u * nx + p.v * ny;
+        const double a = sound(p, g);
+        F.lam = pspeeds(un, a, sc).sr;
+        F.f0 = 0.0;
+        F.f1 = p.p * nx;
+        F.f2 = p.p * ny;
+        F.f3 = 0.0;
+        F.fallback = 0;
+      } else {  // boundary_conditions.cpp:113-122
+        const C4 bs = o.char_bc ? char_farfield_state(in, nx, ny, sc, o) : o.ff;
+        F = face_flux<SCHEME>(in, bs, nx, ny, sc, o);
+      }
+    }
+    if (F.fallback && (meta & kMetaPrimary)) ++n_diff;
+    f0[i] = F.f0 * A;
+    f1[i] = F.f1 * A;
+    f2[i] = F.f2 * A;
+    f3[i] = F.f3 * A;
+    fw[i] = F.lam * A;
+  }
+  // Fallback counters: one atomic per wavefront that saw any.
+  {
+    const int a = wave_sum_i(n_fo);
+    const int d = wave_sum_i(n_diff);
+    const int ff = wave_min_i(first_fail);
+    if ((threadIdx.x & 63) == 0) {
+      if (a) atomicAdd(&diag[0], a);
+      if (d) atomicAdd(&diag[1], d);
+      if (ff != 0x7fffffff) atomicMin(&diag[2], ff);
+    }
+  }
+  __syncthreads();
+  const int c = b * BC + threadIdx.x;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0, w = 0.0;
+  const bool active = c < m.N;
+  if (active) {
+    const int e1 = m.c_off[c + 1];
+    for (int e = m.c_off[c]; e < e1; ++e) {
+      const int ent = m.c_ent[e];
+      const int l = ent >> 1;
+      if (ent & 1) {
+        r0 -= f0[l];
+        r1 -= f1[l];
+        r2 -= f2[l];
+        r3 -= f3[l];
+      } else {
+        r0 += f0[l];
+        r1 += f1[l];
+        r2 += f2[l];
+        r3 += f3[l];
+      }
+      w += fw[l];
+    }
+  }
+  if (MODE == 0) {
+    if (active) {
+      R[c] = r0;
+      R[ld + c] = r1;
+      R[2 * ld + c] = r2;
+      R[3 * ld + c] = r3;
+      lam[c] = w;
+    }
+    return;
+  } else {
+    double nl1 = 0.0, nl2 = 0.0, nlinf = 0.0, nr2 = 0.0;
+    int bad = 0;
+    if (active) {
+      if (st.write_res) {
+        R[c] = r0;
+        R[ld + c] = r1;
+        R[2 * ld + c] = r2;
+        R[3 * ld + c] = r3;
+        lam[c] = w;
+      }
+      const C4 us = load_u(U, ld, c);
+      double p3 = r3;
+      if (o.precond) {  // preconditioning.cpp:186-199, on the stage state's primitive
+        const P4 wp = c2p(us, g);
+        const double bt = cell_beta(wp, o);
+        const double b2 = bt * bt;
+        const double rho = dmax(wp.r, 1.0e-8);
+        const double inv = 1.0 / rho;
+        const double udr = wp.u * r1 + wp.v * r2;
+        p3 = b2 * r3 + (b2 - 1.0) * inv * udr;
+      }
+      const double dt = st.cfl / dmax(w, 1.0e-8);  // time_integrator.cpp:268-273
+      C4 s;
+      s.r = us.r - dt * r0;
+      s.ru = us.ru - dt * r1;
+      s.rv = us.rv - dt * r2;
+      s.E = us.E - dt * p3;
+      if (st.stage >= 1) {
+        const C4 un = load_u(st.Un, ld, c);
+        if (st.stage == 1) {
+          s.r = 0.75 * un.r + 0.25 * s.r;
+          s.ru = 0.75 * un.ru + 0.25 * s.ru;
+          s.rv = 0.75 * un.rv + 0.25 * s.rv;
+          s.E = 0.75 * un.E + 0.25 * s.E;
+        } else {
+          const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+          s.r = t1 * un.r + t2 * s.r;
+          s.ru = t1 * un.ru + t2 * s.ru;
+          s.rv = t1 * un.rv + t2 * s.rv;
+          s.E = t1 * un.E + t2 * s.E;
+        }
+      }
+      enforce_physical(&s, g);
+      st.Unext[c] = s.r;
+      st.Unext[ld + c] = s.ru;
+      st.Unext[2 * ld + c] = s.rv;
+      st.Unext[3 * ld + c] = s.E;
+      if (st.last_stage) {
+        if (!(fin(s.r) && fin(s.ru) && fin(s.rv) && fin(s.E))) {
+          bad = 1;
+        } else {
+          const P4 q = c2p(s, g);
+          if (!(fin(q.r) && fin(q.u) && fin(q.v) && fin(q.p))) bad = 1;
+        }
+        // residual norms of the un-preconditioned residual (euler_solver.cpp:315-333)
+        const double iv = 1.0 / dmax(m.vol[c], 1.0e-12);
+        const double v0 = fabs(r0 * iv), v1 = fabs(r1 * iv), v2 = fabs(r2 * iv), v3 = fabs(r3 * iv);
+        nl1 = v0 + v1 + v2 + v3;
+        nl2 = v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3;
+        nlinf = dmax(dmax(v0, v1), dmax(v2, v3));
+        nr2 = v0 * v0;
+      }
+    }
+    if (st.last_stage) {
+      // Block reduction in a fixed order: wavefront butterflies, then wavefronts in index order.
+      for (int off = 32; off > 0; off >>= 1) {
+        nl1 += __shfl_xor(nl1, off, 64);
+        nl2 += __shfl_xor(nl2, off, 64);
+        nlinf = dmax(nlinf, __shfl_xor(nlinf, off, 64));
+        nr2 += __shfl_xor(nr2, off, 64);
+        bad |= __shfl_xor(bad, off, 64);
+      }
+      __syncthreads();
+      constexpr int NW = BC / 64;
+      double* red = lds;  // reuse: slot data no longer needed
+      const int wv = threadIdx.x >> 6;
+      if ((threadIdx.x & 63) == 0) {
+        red[4 * wv + 0] = nl1;
+        red[4 * wv + 1] = nl2;
+        red[4 * wv + 2] = nlinf;
+        red[4 * wv + 3] = nr2;
+        if (bad) atomicOr(&diag[3], 1);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int k = 0; k < NW; ++k) {
+          a0 += red[4 * k + 0];
+          a1 += red[4 * k + 1];
+          a2 = dmax(a2, red[4 * k + 2]);
+          a3 += red[4 * k + 3];
+        }
+        st.partials[4 * b + 0] = a0;
+        st.partials[4 * b + 1] = a1;
+        st.partials[4 * b + 2] = a2;
+        st.partials[4 * b + 3] = a3;
+      }
+    }
+  }
+}
+
+// Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
+__global__ void k_min_dt(int N, const double* __restrict__ lam, double cfl, double* __restrict__ out) {
+  __shared__ double red[256];
+  double v = 1.79769313486231570815e+308;
+  for (int c = threadIdx.x; c < N; c += blockDim.x) v = dmin(v, cfl / dmax(lam[c], 1.0e-8));
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = dmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = fin(red[0]) ? red[0] : 0.0;
+}
+
+__global__ void k_update_global(DevMesh m, FaceOpts o, const double* __restrict__ U, const double* __restrict__ R,
+                                const double* __restrict__ dtp, int* __restrict__ diag, StageArgs st) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.N) return;
+  const int ld = m.ld;
+  const double g = o.g;
+  const C4 us = load_u(U, ld, c);
+  const double r0 = R[c], r1 = R[ld + c], r2 = R[2 * ld + c], r3 = R[3 * ld + c];
+  double p3 = r3;
+  if (o.precond) {
+    const P4 wp = c2p(us, g);
+    const double bt = cell_beta(wp, o);
+    const double b2 = bt * bt;
+    const double rho = dmax(wp.r, 1.0e-8);
+    const double inv = 1.0 / rho;
+    const double udr = wp.u * r1 + wp.v * r2;
+    p3 = b2 * r3 + (b2 - 1.0) * inv * udr;
+  }
+  const double dt = dtp[0];
+  C4 s;
+  s.r = us.r - dt * r0;
+  s.ru = us.ru - dt * r1;
+  s.rv = us.rv - dt * r2;
+  s.E = us.E - dt * p3;
+  if (st.stage >= 1) {
+    const C4 un = load_u(st.Un, ld, c);
+    if (st.stage == 1) {
+      s.r = 0.75 * un.r + 0.25 * s.r;
+      s.ru = 0.75 * un.ru + 0.25 * s.ru;
+      s.rv = 0.75 * un.rv + 0.25 * s.rv;
+      s.E = 0.75 * un.E + 0.25 * s.E;
+    } else {
+      const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+      s.r = t1 * un.r + t2 * s.r;
+      s.ru = t1 * un.ru + t2 * s.ru;
+      s.rv = t1 * un.rv + t2 * s.rv;
+      s.E = t1 * un.E + t2 * s.E;
+    }
+  }
+  enforce_physical(&s, g);
+  st.Unext[c] = s.r;
+  st.Unext[ld + c] = s.ru;
+  st.Unext[2 * ld + c] = s.rv;
+  st.Unext[3 * ld + c] = s.E;
+  if (st.last_stage) {
+    bool bad = !(fin(s.r) && fin(s.ru) && fin(s.rv) && fin(s.E));
+    if (!bad) {
+      const P4 q = c2p(s, g);
+      bad = !(fin(q.r) && fin(q.u) && fin(q.v) && fin(q.p));
+    }
+    if (bad) atomicOr(&diag[3], 1);
+  }
+}
+
+// Norm partials from a stored residual (used with the non-LTS path), one workgroup per block.
+template <int BC>
+__global__ __launch_bounds__(BC) void k_norm_partials(DevMesh m, const double* __restrict__ R, double* __restrict__ partials) {
+  __shared__ double red[4 * (BC / 64)];
+  const int b = blockIdx.x;
+  const int c = b * BC + threadIdx.x;
+  const int ld = m.ld;
+  double nl1 = 0.0, nl2 = 0.0, nlinf = 0.0, nr2 = 0.0;
+  if (c < m.N) {
+    const double iv = 1.0 / dmax(m.vol[c], 1.0e-12);
+    const double v0 = fabs(R[c] * iv), v1 = fabs(R[ld + c] * iv), v2 = fabs(R[2 * ld + c] * iv),
+                 v3 = fabs(R[3 * ld + c] * iv);
+    nl1 = v0 + v1 + v2 + v3;
+    nl2 = v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3;
+    nlinf = dmax(dmax(v0, v1), dmax(v2, v3));
+    nr2 = v0 * v0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nl1 += __shfl_xor(nl1, off, 64);
+    nl2 += __shfl_xor(nl2, off, 64);
+    nlinf = dmax(nlinf, __shfl_xor(nlinf, off, 64));
+    nr2 += __shfl_xor(nr2, off, 64);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[4 * wv + 0] = nl1;
+    red[4 * wv + 1] = nl2;
+    red[4 * wv + 2] = nlinf;
+    red[4 * wv + 3] = nr2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int k = 0; k < BC / 64; ++k) {
+      a0 += red[4 * k + 0];
+      a1 += red[4 * k + 1];
+      a2 = dmax(a2, red[4 * k + 2]);
+      a3 += red[4 * k + 3];
+    }
+    partials[4 * b + 0] = a0;
+    partials[4 * b + 1] = a1;
+    partials[4 * b + 2] = a2;
+    partials[4 * b + 3] = a3;
+  }
+}
+
+// Final fixed-order reduction of per-block norm partials into out[0..3] = (sum|v|, sum v^2, max|v|,
+// sum v_rho^2), then wall pressure-force sums into out[4..8]:
+//   sum p nA_x, sum p nA_y (absolute), sum (p - p_inf) nA_x, sum (p - p_inf) nA_y, sum moment term,
+// with nA = -n A (forces.cpp:134-149, :220-246, :274-285).  One workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __restrict__ partials, WallArgs wa,
+                                                      const double* __restrict__ U, int ld, double g,
+                                                      double* __restrict__ out, double* __restrict__ cp) {
+  __shared__ double red[9][256];
+  const int t = threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int b = t; b < nblk; b += 256) {
+    a0 += partials[4 * b + 0];
+    a1 += partials[4 * b + 1];
+    a2 = dmax(a2, partials[4 * b + 2]);
+    a3 += partials[4 * b + 3];
+  }
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0;
+  for (int i = t; i < wa.n; i += 256) {
+    const int c = wa.owner[i];
+    const C4 s = load_u(U, ld, c);
+    const double p = c2p(s, g).p;
+    const double A = wa.A[i];
+    const double nA0 = -wa.nx[i] * A;
+    const double nA1 = -wa.ny[i] * A;
+    const double gp = p - wa.p_inf;
+    f0 += p * nA0;
+    f1 += p * nA1;
+    f2 += gp * nA0;
+    f3 += gp * nA1;
+    const double dfx = -gp * nA0;
+    const double dfy = -gp * nA1;
+    const double x = wa.fx[i] - wa.x_ref;
+    const double y = wa.fy[i] - wa.y_ref;
+    f4 += x * dfy - y * dfx;
+    if (cp) cp[i] = gp / wa.q_inf;
+  }
+  red[0][t] = a0;
+  red[1][t] = a1;
+  red[2][t] = a2;
+  red[3][t] = a3;
+  red[4][t] = f0;
+  red[5][t] = f1;
+  red[6][t] = f2;
+  red[7][t] = f3;
+  red[8][t] = f4;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      for (int k = 0; k < 9; ++k) {
+        if (k == 2)
+          red[k][t] = dmax(red[k][t], red[k][t + w]);
+        else
+          red[k][t] += red[k][t + w];
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 9) out[t] = red[t][0];
+}
+
+}  // namespace aero_dev
+
+// ------------------------------------------------------------------------------------------------
+// Launchers (host side of this translation unit)
+// ------------------------------------------------------------------------------------------------
+namespace aero_host {
+
+using namespace aero_dev;
+
+template <int BC>
+static hipError_t launch_gradient_bc(const DevMesh& m, const double* U, double* G, double g, hipStream_t s) {
+  const size_t lds = static_cast<size_t>(m.smax) * 6 * sizeof(double);
+  hipLaunchKernelGGL(k_gradient<BC>, dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient(const DevMesh& m, const double* U, double* G, double g, hipStream_t s) {
+  switch (m.BC) {
+    case 128: return launch_gradient_bc<128>(m, U, G, g, s);
+    case 256: return launch_gradient_bc<256>(m, U, G, g, s);
+    case 512: return launch_gradient_bc<512>(m, U, G, g, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+static hipError_t launch_flux_t(const DevMesh& m, const FaceOpts& o, const double* U, const double* G, double* R,
+                                double* lam, int* diag, const StageArgs& st, hipStream_t s) {
+  // The fused stage tail reuses the slot staging area for 4 doubles per wavefront.
+  const size_t lds = static_cast<size_t>(std::max(m.smax * 5, 4 * (BC / 64))) * sizeof(double);
+  hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE>), dim3(m.grid), dim3(BC), lds, s, m, o, U, G, R, lam,
+                     diag, st);
+  return hipGetLastError();
+}
+
+template <int BC, int MODE>
+static hipError_t launch_flux_bc(const DevMesh& m, const FaceOpts& o, int scheme, int limiter, bool second,
+                                 const double* U, const double* G, double* R, double* lam, int* diag,
+                                 const StageArgs& st, hipStream_t s) {
+#define AERO_FLUX_CASE(SC, LI, SE) \
+  if (scheme == SC && limiter == LI && second == SE) return launch_flux_t<BC, SC, LI, SE, MODE>(m, o, U, G, R, lam, diag, st, s);
+  AERO_FLUX_CASE(0, 0, false)
+  AERO_FLUX_CASE(1, 0, false)
+  AERO_FLUX_CASE(0, 0, true)
+  AERO_FLUX_CASE(0, 1, true)
+  AERO_FLUX_CASE(1, 0, true)
+  AERO_FLUX_CASE(1, 1, true)
+#undef AERO_FLUX_CASE
+  if (!second) return launch_flux_t<BC, 0, 0, false, MODE>(m, o, U, G, R, lam, diag, st, s);  // limiter unused
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_flux(const DevMesh& m, const FaceOpts& o, int scheme, int limiter, bool second, int mode,
+                       const double* U, const double* G, double* R, double* lam, int* diag, const StageArgs& st,
+                       hipStream_t s) {
+  if (!second && scheme == 1) {
+    if (mode == 0) {
+      switch (m.BC) {
+        case 128: return launch_flux_t<128, 1, 0, false, 0>(m, o, U, G, R, lam, diag, st, s);
+        case 256: return launch_flux_t<256, 1, 0, false, 0>(m, o, U, G, R, lam, diag, st, s);
+        case 512: return launch_flux_t<512, 1, 0, false, 0>(m, o, U, G, R, lam, diag, st, s);
+      }
+    } else {
+      switch (m.BC) {
+        case 128: return launch_flux_t<128, 1, 0, false, 1>(m, o, U, G, R, lam, diag, st, s);
+        case 256: return launch_flux_t<256, 1, 0, false, 1>(m, o, U, G, R, lam, diag, st, s);
+        case 512: return launch_flux_t<512, 1, 0, false, 1>(m, o, U, G, R, lam, diag, st, s);
+      }
+    }
+    return hipErrorInvalidValue;
+  }
+  switch (m.BC) {
+    case 128: return mode == 0 ? launch_flux_bc<128, 0>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s)
+                               : launch_flux_bc<128, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
+    case 256: return mode == 0 ? launch_flux_bc<256, 0>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s)
+                               : launch_flux_bc<256, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
+    case 512: return mode == 0 ? launch_flux_bc<512, 0>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s)
+                               : launch_flux_bc<512, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_global_update(const DevMesh& m, const FaceOpts& o, const double* U, const double* R,
+                                const double* lam, double cfl, double* dt_scratch, int* diag, const StageArgs& st,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_min_dt, dim3(1), dim3(256), 0, s, m.N, lam, cfl, dt_scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_update_global, dim3((m.N + 255) / 256), dim3(256), 0, s, m, o, U, R, dt_scratch, diag, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm_partials(const DevMesh& m, const double* R, double* partials, hipStream_t s) {
+  switch (m.BC) {
+    case 128: hipLaunchKernelGGL(k_norm_partials<128>, dim3(m.nblk), dim3(128), 0, s, m, R, partials); break;
+    case 256: hipLaunchKernelGGL(k_norm_partials<256>, dim3(m.nblk), dim3(256), 0, s, m, R, partials); break;
+    case 512: hipLaunchKernelGGL(k_norm_partials<512>, dim3(m.nblk), dim3(512), 0, s, m, R, partials); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_iter_reduce(const DevMesh& m, const double* partials, const WallArgs& wa, const double* U, double g,
+                              double* out, double* cp, hipStream_t s) {
+  hipLaunchKernelGGL(k_iter_reduce, dim3(1), dim3(256), 0, s, m.nblk, partials, wa, U, m.ld, g, out, cp);
+  return hipGetLastError();
+}
+
+}  // namespace aero_host

@@ -1,0 +1,235 @@
+/* aero_hip.h -- C ABI of the MI355X (gfx950) FP64 Euler residual backend.
+ *
+ * This is the drop-in boundary for AeroCFD's `--backend hip`.  It replaces the role of the CUDA
+ * backend interface cfd::cuda_backend (src/cfd_core/include/cfd_core/cuda_backend.hpp:10-53 in
+ * AstaWasTaken/Aero-CFD-Solver v0.1.1); each entry point below names the reference function it
+ * stands in for.  Conventions:
+ *   - plain C types only; every call returns 0 on success or a non-zero AERO_E* code, and fills the
+ *     caller buffer `err` (length `errlen`) with "context: hipGetErrorString(...)"-style text, the
+ *     way the reference fills `std::string* error_message` (cuda/euler_backend.cu:11-19);
+ *   - host arrays use the reference layout (AoS [cell][5] conserved / residual, xyz-interleaved face
+ *     normals and centres) and the reference's cell / face numbering, in FP64;
+ *   - a handle owns all its device memory and one HIP stream; a handle is not thread-safe, separate
+ *     handles are;  calls taking host pointers are synchronous on return.
+ */
+#ifndef AERO_HIP_H
+#define AERO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { AERO_OK = 0, AERO_EINVAL = 1, AERO_EHIP = 2, AERO_ENODEV = 3, AERO_ENOMEM = 4, AERO_ENONFINITE = 5 };
+enum { AERO_FLUX_RUSANOV = 0, AERO_FLUX_HLLC = 1 };           /* EulerFluxScheme, euler_flux.hpp:24-27 */
+enum { AERO_LIMITER_MINMOD = 0, AERO_LIMITER_VENKAT = 1 };    /* LimiterType, reconstruction.hpp:10-13 */
+enum { AERO_BC_INTERIOR = 0, AERO_BC_FARFIELD = 1, AERO_BC_SLIP_WALL = 2 }; /* mesh_geometry.hpp:9-13 */
+enum { AERO_GRAD_GREEN_GAUSS = 0, AERO_GRAD_LEAST_SQUARES = 1 };
+
+/* ------------------------------------------------------------------------------------------------
+ * Mesh description in the reference layout (UnstructuredMesh, cfd_core/mesh.hpp:14-36), FP64.
+ * Face normals are unit and point out of the owner; boundary faces have neighbor == -1.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct aero_mesh_desc {
+  int32_t num_cells;
+  int32_t num_faces;
+  const int32_t* face_owner;    /* [F] */
+  const int32_t* face_neighbor; /* [F], -1 on boundary faces */
+  const int32_t* face_bc;       /* [F], AERO_BC_* (build_euler_boundary_types, mesh_geometry.cpp:82-104) */
+  const double* face_normal;    /* [3F] xyz */
+  const double* face_center;    /* [3F] xyz */
+  const double* face_area;      /* [F] */
+  const double* cell_volume;    /* [N] */
+  const double* cell_center;    /* [3N] xyz */
+  int32_t wall_start;           /* contiguous "wall" patch range (forces.cpp:152-175) */
+  int32_t wall_count;
+} aero_mesh_desc;
+
+/* Device layout options for aero_hip_mesh_create. */
+typedef struct aero_hip_options {
+  int32_t renumber;      /* 1: renumber cells into compact blocks (recursive coordinate bisection); 0: keep order */
+  int32_t block_cells;   /* cells per workgroup block (64..1024, multiple of 64); 0 = autotuned default */
+  int32_t gradient;      /* AERO_GRAD_* */
+} aero_hip_options;
+
+/* Per-evaluation scalars of EulerResidualAssemblyConfig (solvers/euler/residual_assembly.hpp:12-31). */
+typedef struct aero_residual_config {
+  double farfield[5];        /* conservative freestream state (rho, rhou, rhov, rhow, rhoE) */
+  double gamma;
+  int32_t flux_scheme;       /* AERO_FLUX_* */
+  int32_t limiter;           /* AERO_LIMITER_* */
+  int32_t second_order;      /* 0: first order (gradients zero) */
+  int32_t low_mach_fix;
+  double low_mach_cutoff;
+  double stab_mach_floor;
+  int32_t precond_farfield_bc;
+  int32_t precond_enabled;
+  double precond_mach_ref, precond_mach_min, precond_beta_min, precond_beta_max;
+  int32_t all_speed_fix;
+  double all_speed_cutoff, all_speed_f_min, all_speed_ramp;
+} aero_residual_config;
+
+/* Counters of EulerResidualAssemblyDiagnostics (residual_assembly.hpp:33-41). */
+typedef struct aero_residual_diag {
+  int32_t first_order_fallback;
+  int32_t diffusive_fallback;
+  int32_t first_failure_face; /* lowest failing face index in the reference numbering, -1 if none */
+} aero_residual_diag;
+
+/* The driver configuration: EulerAirfoilCaseConfig (solvers/euler_solver.hpp:14-74) minus the mesh
+ * generator settings, plus the resolved AERO_ENABLE_EXPERIMENTAL_PRECOND gate. */
+typedef struct aero_case_config {
+  int32_t iterations, min_iterations, cfl_ramp_iters;
+  double gamma, gas_constant, mach, aoa_deg, p_inf, t_inf, rho_inf, cfl_start, cfl_max;
+  double residual_reduction_target, force_stability_tol;
+  int32_t force_stability_window;
+  double force_mean_drift_tol;
+  int32_t startup_first_order_iters, rk_stages, local_time_stepping;
+  int32_t precond_on;
+  double precond_mach_ref, precond_mach_min, precond_beta_min, precond_beta_max;
+  int32_t precond_farfield_bc;
+  double stab_floor_start, stab_floor_target, stab_floor_k_start, stab_floor_k_target;
+  int32_t stab_ramp_iters;
+  double x_ref, y_ref, chord;
+  int32_t flux_scheme, limiter;
+  int32_t low_mach_fix;
+  double mach_cutoff;
+  int32_t all_speed_flux_fix;
+  double all_speed_mach_cutoff, all_speed_f_min;
+  int32_t all_speed_ramp_start_iter, all_speed_ramp_iters;
+  int32_t all_speed_staged_controller, all_speed_stage_a_min_iters, all_speed_stage_a_max_iters;
+  double all_speed_stage_f_min_high, all_speed_stage_f_min_mid, all_speed_stage_f_min_low;
+  double all_speed_pjump_wall_target, all_speed_pjump_spike_factor;
+  int32_t all_speed_pjump_hold_iters, all_speed_freeze_iters;
+  double all_speed_cfl_drop_factor, all_speed_cfl_restore_factor, all_speed_cfl_min_scale;
+  int32_t aoa0_symmetry_enforce, aoa0_symmetry_enforce_interval;
+  int32_t precond_experimental_enabled;
+} aero_case_config;
+
+/* Per-iteration history row layout (residuals.csv / forces.csv columns, euler_solver.cpp:1161-1203). */
+enum {
+  AERO_H_ITER = 0, AERO_H_L1, AERO_H_L2, AERO_H_LINF, AERO_H_RATIO, AERO_H_CL, AERO_H_CD, AERO_H_CM,
+  AERO_H_RHO_L2, AERO_H_CFL, AERO_H_FO_FALLBACK, AERO_H_DIFF_FALLBACK, AERO_HIST_FIELDS
+};
+
+/* Outputs of a full run (EulerRunResult, euler_solver.hpp:86-148, the fields the parity contract uses).
+ * All arrays are caller-owned; any pointer may be NULL to skip it. */
+typedef struct aero_run_output {
+  int32_t hist_capacity;   /* rows available in `history` */
+  int32_t n_iter;          /* rows written */
+  int32_t nonfinite;       /* non-finite trap fired (euler_solver.cpp:908-946) */
+  double* history;         /* [hist_capacity][AERO_HIST_FIELDS] */
+  double* conserved;       /* [N][5] final state */
+  double* residual;        /* [N][5] last stage residual */
+  double* spectral_radius; /* [N] */
+  double* forces;          /* [8]: fx, fy, lift, drag, moment, cl, cd, cm (gauge) */
+  double* wall_cp;         /* [wall_count] Cp in wall-patch order */
+  double device_seconds;   /* wall time of the iteration loop */
+} aero_run_output;
+
+typedef struct aero_hip_mesh aero_hip_mesh; /* opaque device mesh + state + stream */
+
+/* Replaces cfd::cuda_backend::cuda_runtime_available / cuda_euler_available (cuda_backend.hpp:36-37):
+ * returns 1 when a gfx950 device is visible, else 0 (with the reason in err). */
+int aero_hip_available(char* err, size_t errlen);
+
+/* Replaces cfd::cuda_backend::init_euler_device_buffers (cuda_backend.hpp:44-46): uploads the mesh
+ * (AoS FP64 -> SoA FP64, optional renumbering, per-block face tables) and allocates state buffers. */
+int aero_hip_mesh_create(const aero_mesh_desc* desc, const aero_hip_options* opts, aero_hip_mesh** out,
+                         char* err, size_t errlen);
+
+/* Replaces cfd::cuda_backend::free_euler_device_buffers (cuda_backend.hpp:47). NULL is a no-op. */
+void aero_hip_mesh_destroy(aero_hip_mesh* mesh);
+
+/* Replaces cfd::cuda_backend::euler_residual_cuda (cuda_backend.hpp:49-52): one residual evaluation
+ * -- primitive conversion, gradient (computed on the device), limiter, MUSCL, flux, face->cell
+ * accumulation and spectral radius -- from host conserved[N][5] to host residual[N][5] and
+ * spectral_radius[N], in the reference's numbering.  diag may be NULL. */
+int aero_hip_residual(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                      double* residual, double* spectral_radius, aero_residual_diag* diag, char* err,
+                      size_t errlen);
+
+/* Device-resident variants for timed loops: load the state once, then evaluate the residual on the
+ * handle's stream without host traffic (asynchronous), then read results back. */
+int aero_hip_set_state(aero_hip_mesh* mesh, const double* conserved, char* err, size_t errlen);
+int aero_hip_residual_async(aero_hip_mesh* mesh, const aero_residual_config* cfg, char* err, size_t errlen);
+int aero_hip_get_residual(aero_hip_mesh* mesh, double* residual, double* spectral_radius, aero_residual_diag* diag,
+                          char* err, size_t errlen);
+int aero_hip_get_gradients(aero_hip_mesh* mesh, double* gradients /* [N][5][2] reference layout */, char* err,
+                           size_t errlen);
+/* The handle's HIP stream (hipStream_t), for event timing by the caller. */
+void* aero_hip_stream(aero_hip_mesh* mesh);
+
+/* Per-kernel timing of the next `max_evals` aero_hip_residual_async calls with HIP events recorded on
+ * the handle's stream around the gradient and the flux kernel; _end synchronises and returns the summed
+ * gradient / flux kernel milliseconds, the first-to-last event span and the number of timed calls.
+ * (Measurement support; the CUDA backend has no counterpart.) */
+int aero_hip_timing_begin(aero_hip_mesh* mesh, int32_t max_evals, char* err, size_t errlen);
+int aero_hip_timing_end(aero_hip_mesh* mesh, double* gradient_ms, double* flux_ms, double* span_ms,
+                        int32_t* n_evals, char* err, size_t errlen);
+
+/* Device layout summary: cells, faces, blocks, cells per block, face slots, max slots per block, -,
+ * workgroups launched.  Returns AERO_EINVAL on NULL. */
+int aero_hip_layout(aero_hip_mesh* mesh, int64_t* out8);
+
+/* One RK stage on the device-resident state: residual, low-Mach preconditioned residual,
+ * dt/V and the SSP-RK3 stage update with enforce_physical_state (euler_solver.cpp:802-900).
+ * The stage-0 call snapshots U^n.  last_stage additionally produces the residual-norm partials and the
+ * non-finite flag; write_residual keeps R and the spectral radius for aero_hip_get_residual. */
+int aero_hip_stage(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t stage, int32_t last_stage,
+                   double cfl, int32_t local_time_stepping, int32_t write_residual, char* err, size_t errlen);
+
+/* The full pseudo-time run of run_euler_airfoil_case (euler_solver.cpp:463-1208) with the stage
+ * loop, norms and forces on the device and the ramp / CFL / convergence control on the host. */
+int aero_hip_run_case(aero_hip_mesh* mesh, const aero_case_config* cfg, aero_run_output* out, char* err,
+                      size_t errlen);
+
+/* ------------------------------------------------------------------------------------------------
+ * Mesh generation (host side).  The NACA-4 O-grid of make_airfoil_ogrid_mesh
+ * (mesh/airfoil_mesh.cpp:395-544) with the reference's triangles, plus quad and mixed
+ * (quad wall block + triangles) cells, seeded interior-node jitter and RCM cell renumbering.
+ * precision 4 reproduces the reference's FP32 generator; precision 8 computes in FP64.
+ * ---------------------------------------------------------------------------------------------- */
+enum { AERO_CELLS_TRI = 0, AERO_CELLS_QUAD = 1, AERO_CELLS_MIXED = 2 };
+
+typedef struct aero_mesh_gen_config {
+  char naca_code[8];
+  double chord;
+  int32_t num_circumferential, num_radial;
+  double farfield_radius, radial_stretch;
+  int32_t cell_type;     /* AERO_CELLS_* */
+  int32_t quad_layers;   /* AERO_CELLS_MIXED: wall-normal layers kept as quads */
+  double jitter;         /* interior-node jitter as a fraction of local spacing (0 = none) */
+  uint64_t seed;
+  int32_t renumber_rcm;  /* 1: reverse Cuthill-McKee cell order before faces are built */
+} aero_mesh_gen_config;
+
+typedef struct aero_mesh_arrays {
+  int32_t precision; /* 4 or 8: element size of the floating arrays */
+  int32_t num_cells, num_faces, num_points;
+  int32_t* face_owner;
+  int32_t* face_neighbor;
+  int32_t* face_bc;
+  int32_t* face_vertices; /* [2F] */
+  void* face_normal;      /* [3F] */
+  void* face_center;      /* [3F] */
+  void* face_area;        /* [F] */
+  void* cell_volume;      /* [N] */
+  void* cell_center;      /* [3N] */
+  void* points;           /* [3P] */
+  int32_t* cell_offsets;  /* [N] running end offsets into cell_connectivity (VTU style) */
+  int32_t* cell_connectivity;
+  int32_t wall_start, wall_count, farfield_start, farfield_count;
+} aero_mesh_arrays;
+
+int aero_mesh_generate(const aero_mesh_gen_config* cfg, int32_t precision, aero_mesh_arrays* out, char* err,
+                       size_t errlen);
+void aero_mesh_free(aero_mesh_arrays* arrays);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AERO_HIP_H */

@@ -1,0 +1,400 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// extern "C" entry points over the templated restatement in aero_oracle.hpp / aero_oracle_driver.hpp,
+// one per scalar type (_f32 reproduces the reference's FP32 arithmetic, _f64 is the FP64 parity oracle).
+// Loaded by tests/ (ctypes) and by bench.py's cpu_baseline leg only.
+#include <chrono>
+#include <cstring>
+
+#include "aero_oracle.hpp"
+#include "aero_oracle_driver.hpp"
+
+using namespace aero_oracle;
+
+extern "C" {
+
+// Mesh arrays in the reference layout; the pointer type follows the function suffix.
+typedef struct ao_mesh {
+  int num_cells, num_faces;
+  const int* owner;
+  const int* neighbor;
+  const int* bc;
+  const void* normal;       // [3F]
+  const void* face_center;  // [3F]
+  const void* area;         // [F]
+  const void* volume;       // [N]
+  const void* cell_center;  // [3N]
+  int wall_start, wall_count;
+} ao_mesh;
+
+// EulerResidualAssemblyConfig scalars (residual_assembly.hpp:12-31), carried as double.
+typedef struct ao_residual_cfg {
+  double farfield[5];
+  double gamma;
+  int scheme, limiter, second_order;
+  int low_mach_fix;
+  double low_mach_cutoff, stab_mach_floor;
+  int precond_farfield_bc;
+  int precond_enabled;
+  double mach_ref, mach_min, beta_min, beta_max;
+  int as_fix;
+  double as_cutoff, as_fmin, as_ramp;
+} ao_residual_cfg;
+
+// EulerAirfoilCaseConfig scalars (euler_solver.hpp:14-74), carried as double / int.
+typedef struct ao_case_cfg {
+  int iterations, min_iterations, cfl_ramp_iters;
+  double gamma, gas_constant, mach, aoa_deg, p_inf, t_inf, rho_inf, cfl_start, cfl_max;
+  double residual_reduction_target, force_stability_tol;
+  int force_stability_window;
+  double force_mean_drift_tol;
+  int startup_first_order_iters, rk_stages, local_time_stepping;
+  int precond_on;
+  double precond_mach_ref, precond_mach_min, precond_beta_min, precond_beta_max;
+  int precond_farfield_bc;
+  double stab_floor_start, stab_floor_target, stab_floor_k_start, stab_floor_k_target;
+  int stab_ramp_iters;
+  double x_ref, y_ref, chord;
+  int flux_scheme, limiter;
+  int low_mach_fix;
+  double mach_cutoff;
+  int all_speed_flux_fix;
+  double all_speed_mach_cutoff, all_speed_f_min;
+  int all_speed_ramp_start_iter, all_speed_ramp_iters;
+  int all_speed_staged_controller, all_speed_stage_a_min_iters, all_speed_stage_a_max_iters;
+  double all_speed_stage_f_min_high, all_speed_stage_f_min_mid, all_speed_stage_f_min_low;
+  double all_speed_pjump_wall_target, all_speed_pjump_spike_factor;
+  int all_speed_pjump_hold_iters, all_speed_freeze_iters;
+  double all_speed_cfl_drop_factor, all_speed_cfl_restore_factor, all_speed_cfl_min_scale;
+  int aoa0_symmetry_enforce, aoa0_symmetry_enforce_interval;
+  int precond_experimental_enabled;
+} ao_case_cfg;
+
+int ao_hist_fields(void) { return kHistFields; }
+int ao_sizeof_case_cfg(void) { return static_cast<int>(sizeof(ao_case_cfg)); }
+int ao_sizeof_residual_cfg(void) { return static_cast<int>(sizeof(ao_residual_cfg)); }
+}
+
+namespace {
+
+template <class R> MeshView<R> view(const ao_mesh* m) {
+  MeshView<R> v;
+  v.num_cells = m->num_cells;
+  v.num_faces = m->num_faces;
+  v.owner = m->owner;
+  v.neighbor = m->neighbor;
+  v.bc = m->bc;
+  v.normal = static_cast<const R*>(m->normal);
+  v.face_center = static_cast<const R*>(m->face_center);
+  v.area = static_cast<const R*>(m->area);
+  v.volume = static_cast<const R*>(m->volume);
+  v.cell_center = static_cast<const R*>(m->cell_center);
+  v.wall_start = m->wall_start;
+  v.wall_count = m->wall_count;
+  return v;
+}
+
+template <class R> AssemblyCfg<R> acfg(const ao_residual_cfg* c) {
+  AssemblyCfg<R> a;
+  a.farfield = Cons<R>{R(c->farfield[0]), R(c->farfield[1]), R(c->farfield[2]), R(c->farfield[3]), R(c->farfield[4])};
+  a.gamma = R(c->gamma);
+  a.scheme = c->scheme;
+  a.limiter = c->limiter;
+  a.second_order = c->second_order != 0;
+  a.low_mach_fix = c->low_mach_fix != 0;
+  a.low_mach_cutoff = R(c->low_mach_cutoff);
+  a.stab_mach_floor = R(c->stab_mach_floor);
+  a.precond_farfield_bc = c->precond_farfield_bc != 0;
+  a.precond.enabled = c->precond_enabled != 0;
+  a.precond.mach_ref = R(c->mach_ref);
+  a.precond.mach_min = R(c->mach_min);
+  a.precond.beta_min = R(c->beta_min);
+  a.precond.beta_max = R(c->beta_max);
+  a.as_fix = c->as_fix != 0;
+  a.as_cutoff = R(c->as_cutoff);
+  a.as_fmin = R(c->as_fmin);
+  a.as_ramp = R(c->as_ramp);
+  return a;
+}
+
+template <class R> CaseCfg<R> ccfg(const ao_case_cfg* c) {
+  CaseCfg<R> o;
+  o.iterations = c->iterations;
+  o.min_iterations = c->min_iterations;
+  o.cfl_ramp_iters = c->cfl_ramp_iters;
+  o.gamma = R(c->gamma);
+  o.gas_constant = R(c->gas_constant);
+  o.mach = R(c->mach);
+  o.aoa_deg = R(c->aoa_deg);
+  o.p_inf = R(c->p_inf);
+  o.t_inf = R(c->t_inf);
+  o.rho_inf = R(c->rho_inf);
+  o.cfl_start = R(c->cfl_start);
+  o.cfl_max = R(c->cfl_max);
+  o.residual_reduction_target = R(c->residual_reduction_target);
+  o.force_stability_tol = R(c->force_stability_tol);
+  o.force_stability_window = c->force_stability_window;
+  o.force_mean_drift_tol = R(c->force_mean_drift_tol);
+  o.startup_first_order_iters = c->startup_first_order_iters;
+  o.rk_stages = c->rk_stages;
+  o.local_time_stepping = c->local_time_stepping != 0;
+  o.precond_on = c->precond_on != 0;
+  o.precond_mach_ref = R(c->precond_mach_ref);
+  o.precond_mach_min = R(c->precond_mach_min);
+  o.precond_beta_min = R(c->precond_beta_min);
+  o.precond_beta_max = R(c->precond_beta_max);
+  o.precond_farfield_bc = c->precond_farfield_bc != 0;
+  o.stab_floor_start = R(c->stab_floor_start);
+  o.stab_floor_target = R(c->stab_floor_target);
+  o.stab_floor_k_start = R(c->stab_floor_k_start);
+  o.stab_floor_k_target = R(c->stab_floor_k_target);
+  o.stab_ramp_iters = c->stab_ramp_iters;
+  o.x_ref = R(c->x_ref);
+  o.y_ref = R(c->y_ref);
+  o.chord = R(c->chord);
+  o.flux_scheme = c->flux_scheme;
+  o.limiter = c->limiter;
+  o.low_mach_fix = c->low_mach_fix != 0;
+  o.mach_cutoff = R(c->mach_cutoff);
+  o.all_speed_flux_fix = c->all_speed_flux_fix != 0;
+  o.all_speed_mach_cutoff = R(c->all_speed_mach_cutoff);
+  o.all_speed_f_min = R(c->all_speed_f_min);
+  o.all_speed_ramp_start_iter = c->all_speed_ramp_start_iter;
+  o.all_speed_ramp_iters = c->all_speed_ramp_iters;
+  o.all_speed_staged_controller = c->all_speed_staged_controller != 0;
+  o.all_speed_stage_a_min_iters = c->all_speed_stage_a_min_iters;
+  o.all_speed_stage_a_max_iters = c->all_speed_stage_a_max_iters;
+  o.all_speed_stage_f_min_high = R(c->all_speed_stage_f_min_high);
+  o.all_speed_stage_f_min_mid = R(c->all_speed_stage_f_min_mid);
+  o.all_speed_stage_f_min_low = R(c->all_speed_stage_f_min_low);
+  o.all_speed_pjump_wall_target = R(c->all_speed_pjump_wall_target);
+  o.all_speed_pjump_spike_factor = R(c->all_speed_pjump_spike_factor);
+  o.all_speed_pjump_hold_iters = c->all_speed_pjump_hold_iters;
+  o.all_speed_freeze_iters = c->all_speed_freeze_iters;
+  o.all_speed_cfl_drop_factor = R(c->all_speed_cfl_drop_factor);
+  o.all_speed_cfl_restore_factor = R(c->all_speed_cfl_restore_factor);
+  o.all_speed_cfl_min_scale = R(c->all_speed_cfl_min_scale);
+  o.aoa0_symmetry_enforce = c->aoa0_symmetry_enforce != 0;
+  o.aoa0_symmetry_enforce_interval = c->aoa0_symmetry_enforce_interval;
+  o.precond_experimental_enabled = c->precond_experimental_enabled != 0;
+  return o;
+}
+
+template <class R> Cons<R> cons_at(const R* u, int i) {
+  return Cons<R>{u[5 * i + 0], u[5 * i + 1], u[5 * i + 2], u[5 * i + 3], u[5 * i + 4]};
+}
+template <class R> A3<R> n_at(const R* n, int i) { return {n[3 * i + 0], n[3 * i + 1], n[3 * i + 2]}; }
+
+template <class R> void limiter_t(int lim, const R* a, const R* b, R* out, int n) {
+  for (int i = 0; i < n; ++i) out[i] = apply_limiter<R>(lim, a[i], b[i]);
+}
+
+// Pointwise interior-face flux through the reference's dispatcher (euler_flux.cpp:366-390).
+template <class R>
+void flux_t(int n, int scheme, const R* ul, const R* ur, const R* nrm, const R* scale, R g, int as_fix, R as_cut,
+            R as_fmin, R as_w, R* F, R* lam, int* fb) {
+  for (int i = 0; i < n; ++i) {
+    bool f = false;
+    R w = R(0);
+    const A5<R> x = face_flux(cons_at(ul, i), cons_at(ur, i), n_at(nrm, i), g, scheme, &w, scale[i], as_fix != 0,
+                              as_cut, as_fmin, as_w, &f);
+    for (int k = 0; k < 5; ++k) F[5 * i + k] = x[k];
+    lam[i] = w;
+    fb[i] = f ? 1 : 0;
+  }
+}
+
+template <class R>
+void bflux_t(int n, int bc, int scheme, const R* ui, const R* ff, const R* nrm, const R* scale, R g, int char_bc,
+             int as_fix, R as_cut, R as_fmin, R as_w, R* F, R* lam, int* fb) {
+  const Cons<R> far = cons_at(ff, 0);
+  for (int i = 0; i < n; ++i) {
+    bool f = false;
+    R w = R(0);
+    const A5<R> x = boundary_flux(bc, cons_at(ui, i), far, n_at(nrm, i), scheme, g, scale[i], char_bc != 0,
+                                  as_fix != 0, as_cut, as_fmin, as_w, &w, &f);
+    for (int k = 0; k < 5; ++k) F[5 * i + k] = x[k];
+    lam[i] = w;
+    fb[i] = f ? 1 : 0;
+  }
+}
+
+template <class R> std::vector<Prim<R>> prims(int n, const R* U, R g) {
+  std::vector<Prim<R>> p(static_cast<std::size_t>(n));
+  for (int i = 0; i < n; ++i) p[i] = cons_to_prim(cons_at(U, i), g);
+  return p;
+}
+
+template <class R> void gradients_t(const ao_mesh* m, const R* U, R g, R* grad) {
+  const MeshView<R> v = view<R>(m);
+  std::vector<R> gr;
+  green_gauss(v, prims(v.num_cells, U, g), &gr);
+  std::memcpy(grad, gr.data(), gr.size() * sizeof(R));
+}
+
+// One residual evaluation as the driver's stage body does it (euler_solver.cpp:803-869):
+// primitive conversion, Green-Gauss (or zeros), residual assembly.
+template <class R>
+void residual_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, R* res, R* lam, int* diag) {
+  const MeshView<R> v = view<R>(m);
+  const AssemblyCfg<R> a = acfg<R>(c);
+  const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
+  std::vector<R> gr;
+  if (a.second_order)
+    green_gauss(v, p, &gr);
+  else
+    gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
+  std::vector<R> r, l;
+  AssemblyDiag d;
+  assemble_residual(v, a, p, gr, &r, &l, &d);
+  std::memcpy(res, r.data(), r.size() * sizeof(R));
+  std::memcpy(lam, l.data(), l.size() * sizeof(R));
+  if (diag) {
+    diag[0] = d.first_order_fallback;
+    diag[1] = d.diffusive_fallback;
+    diag[2] = d.first_failure_face;
+  }
+}
+
+// Per cell and component, sum over its faces of |F_k A| (the scale of the scaled residual tolerance).
+template <class R> void flux_abs_sum_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, R* out) {
+  const MeshView<R> v = view<R>(m);
+  const AssemblyCfg<R> a = acfg<R>(c);
+  const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
+  std::vector<R> gr, r, l, s;
+  if (a.second_order)
+    green_gauss(v, p, &gr);
+  else
+    gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
+  assemble_residual(v, a, p, gr, &r, &l, nullptr, &s);
+  std::memcpy(out, s.data(), s.size() * sizeof(R));
+}
+
+// Stage tail (euler_solver.cpp:893-899): preconditioned residual copy, dt/V, RK stage update.
+template <class R>
+void stage_tail_t(int n, int stage, const R* un, const R* us, const R* res, const R* lam, R cfl, int lts,
+                  const ao_residual_cfg* c, R* next) {
+  const AssemblyCfg<R> a = acfg<R>(c);
+  const std::vector<Prim<R>> p = prims(n, us, a.gamma);
+  std::vector<R> rp(res, res + 5 * static_cast<std::size_t>(n));
+  precond_residual(p, a.gamma, a.precond, &rp);
+  const std::vector<R> l(lam, lam + n);
+  const std::vector<R> dt = dt_over_volume(l, cfl, lts != 0);
+  std::vector<R> vn(un, un + 5 * static_cast<std::size_t>(n)), vs(us, us + 5 * static_cast<std::size_t>(n)), nx;
+  rk3_stage_update(stage, vn, vs, rp, dt, a.gamma, &nx);
+  std::memcpy(next, nx.data(), nx.size() * sizeof(R));
+}
+
+template <class R> void norms_t(const ao_mesh* m, const R* res, R* out4) {
+  const MeshView<R> v = view<R>(m);
+  const std::vector<R> r(res, res + 5 * static_cast<std::size_t>(v.num_cells));
+  const Norms<R> nr = residual_norms(v, r);
+  out4[0] = nr.l1;
+  out4[1] = nr.l2;
+  out4[2] = nr.linf;
+  out4[3] = nr.rho_l2;
+}
+
+template <class R>
+void forces_t(const ao_mesh* m, const R* p, const R* ref7, int subtract, R* out8) {
+  const MeshView<R> v = view<R>(m);
+  FreeRef<R> f;
+  f.rho_inf = ref7[0];
+  f.p_inf = ref7[1];
+  f.aoa_deg = ref7[2];
+  f.speed_inf = ref7[3];
+  f.chord = ref7[4];
+  f.x_ref = ref7[5];
+  f.y_ref = ref7[6];
+  const std::vector<R> pv(p, p + v.num_cells);
+  const Coeffs<R> c = integrate_forces(v, pv, f, subtract != 0);
+  const R o[8] = {c.fx, c.fy, c.lift, c.drag, c.moment, c.cl, c.cd, c.cm};
+  std::memcpy(out8, o, sizeof(o));
+}
+
+// Full run; returns the number of history rows written (<= hist_cap rows), -1 on error.
+template <class R>
+int run_case_t(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, R* hist, R* U, R* res, R* lam, R* forces8,
+               R* cp, int* nonfinite, char* err, int errlen) {
+  try {
+    const RunOut<R> o = run_case(view<R>(m), ccfg<R>(c));
+    const int rows = std::min(o.n_iter, hist_cap);
+    std::memcpy(hist, o.hist.data(), static_cast<std::size_t>(rows) * kHistFields * sizeof(R));
+    if (U) std::memcpy(U, o.U.data(), o.U.size() * sizeof(R));
+    if (res && !o.res.empty()) std::memcpy(res, o.res.data(), o.res.size() * sizeof(R));
+    if (lam && !o.lam.empty()) std::memcpy(lam, o.lam.data(), o.lam.size() * sizeof(R));
+    if (forces8) {
+      const R f[8] = {o.forces.fx, o.forces.fy, o.forces.lift, o.forces.drag,
+                      o.forces.moment, o.forces.cl, o.forces.cd, o.forces.cm};
+      std::memcpy(forces8, f, sizeof(f));
+    }
+    if (cp) std::memcpy(cp, o.cp.data(), o.cp.size() * sizeof(R));
+    if (nonfinite) *nonfinite = o.nonfinite ? 1 : 0;
+    return rows;
+  } catch (const std::exception& e) {
+    if (err && errlen > 0) {
+      std::strncpy(err, e.what(), static_cast<std::size_t>(errlen) - 1);
+      err[errlen - 1] = '\0';
+    }
+    return -1;
+  }
+}
+
+// CPU baseline timing: `n_evals` back-to-back residual evaluations (primitive + gradient + assembly),
+// timed with steady_clock around the loop only.  Returns seconds.
+template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, int n_evals) {
+  const MeshView<R> v = view<R>(m);
+  const AssemblyCfg<R> a = acfg<R>(c);
+  std::vector<R> gr, r, l;
+  AssemblyDiag d;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0; it < n_evals; ++it) {
+    const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
+    if (a.second_order)
+      green_gauss(v, p, &gr);
+    else
+      gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
+    assemble_residual(v, a, p, gr, &r, &l, &d);
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // namespace
+
+#define AO_EXPORT_ALL(SUF, T)                                                                                    \
+  extern "C" void ao_limiter_##SUF(int lim, const T* a, const T* b, T* out, int n) { limiter_t<T>(lim, a, b, out, n); } \
+  extern "C" void ao_flux_##SUF(int n, int scheme, const T* ul, const T* ur, const T* nrm, const T* scale, double g,    \
+                                int as_fix, double as_cut, double as_fmin, double as_w, T* F, T* lam, int* fb) {         \
+    flux_t<T>(n, scheme, ul, ur, nrm, scale, T(g), as_fix, T(as_cut), T(as_fmin), T(as_w), F, lam, fb);                  \
+  }                                                                                                                      \
+  extern "C" void ao_boundary_flux_##SUF(int n, int bc, int scheme, const T* ui, const T* ff, const T* nrm,             \
+                                         const T* scale, double g, int char_bc, int as_fix, double as_cut,                \
+                                         double as_fmin, double as_w, T* F, T* lam, int* fb) {                           \
+    bflux_t<T>(n, bc, scheme, ui, ff, nrm, scale, T(g), char_bc, as_fix, T(as_cut), T(as_fmin), T(as_w), F, lam, fb);   \
+  }                                                                                                                      \
+  extern "C" void ao_gradients_##SUF(const ao_mesh* m, const T* U, double g, T* grad) { gradients_t<T>(m, U, T(g), grad); } \
+  extern "C" void ao_residual_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, T* res, T* lam, int* diag) { \
+    residual_t<T>(m, c, U, res, lam, diag);                                                                              \
+  }                                                                                                                      \
+  extern "C" void ao_flux_abs_sum_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, T* out) {           \
+    flux_abs_sum_t<T>(m, c, U, out);                                                                                     \
+  }                                                                                                                      \
+  extern "C" void ao_stage_tail_##SUF(int n, int stage, const T* un, const T* us, const T* res, const T* lam,            \
+                                      double cfl, int lts, const ao_residual_cfg* c, T* next) {                          \
+    stage_tail_t<T>(n, stage, un, us, res, lam, T(cfl), lts, c, next);                                                   \
+  }                                                                                                                      \
+  extern "C" void ao_norms_##SUF(const ao_mesh* m, const T* res, T* out4) { norms_t<T>(m, res, out4); }                 \
+  extern "C" void ao_forces_##SUF(const ao_mesh* m, const T* p, const T* ref7, int subtract, T* out8) {                 \
+    forces_t<T>(m, p, ref7, subtract, out8);                                                                             \
+  }                                                                                                                      \
+  extern "C" int ao_run_case_##SUF(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, T* hist, T* U, T* res,        \
+                                   T* lam, T* forces8, T* cp, int* nonfinite, char* err, int errlen) {                   \
+    return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen);                        \
+  }                                                                                                                      \
+  extern "C" double ao_time_residual_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, int n_evals) {      \
+    return time_residual_t<T>(m, c, U, n_evals);                                                                         \
+  }
+
+AO_EXPORT_ALL(f32, float)
+AO_EXPORT_ALL(f64, double)

@@ -1,0 +1,172 @@
+"""HIP (gfx950, FP64) residual path vs the FP64 oracle, through the C ABI.
+
+Gate: per cell and component |R_hip - R_oracle| <= 1e-12 * sum_f |F_f,k A_f| (the scale stays well posed
+where R cancels), spectral radius within 1e-12 relative, identical fallback counters; bitwise equality is
+reported (the kernels keep the reference's per-cell summation order).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import helpers
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import aero_hip
+    assert aero_hip.hip_available(), "no gfx950 device visible to the HIP backend"
+    return aero_hip
+
+
+def ff64(mach=0.3, aoa=2.0):
+    from aero_hip import freestream
+    return freestream(mach, aoa)[1]
+
+
+def compare(hm, om, U, combo, ff, label):
+    vals = cases.residual_cfg_values(combo, ff)
+    cfg = helpers.res_cfg(vals)
+    say(f"[parity] {label} combo={combo} N={om.N}")
+    Rh, lh, dh = hm.residual(U, cfg)
+    Ro, lo, do = ol.residual(om, cfg, U)
+    scale = helpers.flux_scale(om, cfg, U)
+    comp = [0, 1, 2, 4]
+    err = helpers.scaled_err(Rh[:, comp], Ro[:, comp], scale[:, comp])
+    lam_err = np.abs(lh - lo) / np.maximum(np.abs(lo), 1e-300)
+    assert np.all(np.isfinite(Rh)), label
+    assert err.max() <= TOL, f"{label} {combo}: scaled residual error {err.max():.3e}"
+    assert lam_err.max() <= TOL, f"{label} {combo}: spectral radius error {lam_err.max():.3e}"
+    assert dh["first_order_fallback"] == do[0] and dh["diffusive_fallback"] == do[1], (label, combo, dh, do)
+    assert dh["first_failure_face"] == do[2], (label, combo, dh, do)
+    return bool(np.array_equal(Rh[:, comp], Ro[:, comp]) and np.array_equal(lh, lo))
+
+
+@pytest.mark.parametrize("mname", ["40x8", "64x14"])
+def test_residual_parity_reference_meshes(hip, mname):
+    m = helpers.mesh(mname)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64()
+    bitwise = 0
+    with hip.HipMesh(m) as hm:
+        for si, amp in enumerate((0.05, 0.9)):
+            U = helpers.perturbed_state(m.cell_center, ff, amp, 7 + si)
+            for combo in cases.RESIDUAL_COMBOS:
+                bitwise += compare(hm, om, U, combo, ff, f"{mname} U{si}")
+    say(f"[parity] {mname}: {bitwise}/{2 * len(cases.RESIDUAL_COMBOS)} evaluations bitwise identical")
+
+
+@pytest.mark.parametrize("cells,kw", [("tri", {}), ("quad", {}),
+                                      ("mixed", dict(quad_layers=12, jitter=0.1, seed=12345, rcm=True))])
+def test_residual_parity_256x64(hip, cells, kw):
+    m = helpers.mesh("256x64", cells=cells, **kw)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64(0.5, 1.25)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.02, 20260101)
+    with hip.HipMesh(m) as hm:
+        for combo in [(1, 1, 1, 1, 0, 0, 0), (0, 0, 1, 0, 0, 0, 0), (1, 1, 1, 1, 1, 1, 1), (1, 0, 0, 0, 0, 0, 0)]:
+            compare(hm, om, U, combo, ff, f"256x64 {cells}")
+
+
+def test_gradients_parity(hip):
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=5, rcm=True)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64()
+    U = helpers.perturbed_state(m.cell_center, ff, 0.1, 3)
+    say("[parity] gradients 64x14 mixed")
+    with hip.HipMesh(m) as hm:
+        hm.residual(U, helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff)))
+        Gh = hm.gradients()
+    Go = ol.gradients(om, U)
+    scale = np.abs(Go).max(axis=0, keepdims=True) + 1e-300
+    assert (np.abs(Gh - Go) / scale).max() <= TOL
+
+
+def test_layout_invariance_and_determinism(hip):
+    """Renumbering, block size and repetition leave the residual bitwise unchanged (the per-cell summation
+    order is the reference face order in every layout; no floating-point atomics)."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=9, rcm=True)
+    ff = ff64()
+    U = helpers.perturbed_state(m.cell_center, ff, 0.1, 4)
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 1, 1, 1, 1), ff))
+    outs = []
+    for renumber, bc in ((True, 256), (False, 256), (True, 128), (True, 512)):
+        say(f"[determinism] renumber={renumber} block={bc}")
+        with hip.HipMesh(m, renumber=renumber, block_cells=bc) as hm:
+            a = hm.residual(U, cfg)
+            b = hm.residual(U, cfg)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        outs.append(a)
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+
+
+def test_freestream_preservation_large(hip):
+    """Uniform freestream on the ~1M-cell jittered mixed mesh: the residual vanishes to round-off of the
+    face-flux scale (tests/cpp/test_euler_uniform_invariance.cpp restated at full size)."""
+    say("[large] generating the 1M-cell mixed mesh")
+    m = helpers.mesh("256x64", num_circumferential=1024, num_radial=512, radial_stretch=163.9, cells="mixed",
+                     quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    assert m.num_cells > 1_000_000
+    ff = ff64(0.3, 2.0)
+    U = np.tile(ff, (m.num_cells, 1))
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff))
+    say(f"[large] residual N={m.num_cells}")
+    with hip.HipMesh(m) as hm:
+        R, lam, d = hm.residual(U, cfg)
+    om = ol.OracleMesh(m, np.float64)
+    scale = helpers.flux_scale(om, cfg, U)
+    comp = [0, 1, 2, 4]
+    assert (np.abs(R[:, comp]) / scale[:, comp]).max() < 1e-13
+    assert np.all(lam > 0) and d["first_order_fallback"] == 0
+
+
+def test_residual_parity_large(hip):
+    """Config-2-size mixed mesh (~1M cells): one HLLC + venkat evaluation against the FP64 oracle."""
+    m = helpers.mesh("256x64", num_circumferential=1024, num_radial=512, radial_stretch=163.9, cells="mixed",
+                     quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64(0.3, 2.0)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.02, 11)
+    with hip.HipMesh(m) as hm:
+        compare(hm, om, U, (1, 1, 1, 0, 0, 0, 0), ff, "1M mixed")
+
+
+@pytest.mark.parametrize("name", list(cases.RUNS))
+def test_run_case_parity(hip, name):
+    """Full pseudo-time runs: residual history, CL / CD / CM, final state and wall Cp match the FP64
+    oracle's driver to 1e-9 relative."""
+    run = cases.RUNS[name]
+    m = helpers.mesh(run["mesh"])
+    om = ol.OracleMesh(m, np.float64)
+    cfg = helpers.case_cfg(run["cfg"], run["env_precond"])
+    say(f"[run] {name} mesh={run['mesh']} iterations={run['cfg']['iterations']}")
+    ref = ol.run_case(om, cfg, run["cfg"]["iterations"])
+    from aero_hip import CaseConfig
+    case = CaseConfig(**run["cfg"])
+    case.precond_experimental_enabled = 1 if run["env_precond"] else 0
+    with hip.HipMesh(m) as hm:
+        out = hm.run_case(case)
+    hr, ho = ref["history"], out["history"]
+    assert ho.shape == hr.shape, (ho.shape, hr.shape)
+    for col in (1, 2, 3, 5, 6, 7, 8):
+        a, b = ho[:, col], hr[:, col]
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+        atol = 1e-9 * np.abs(hr[:, col]).max()
+        assert np.all((rel <= 1e-9) | (np.abs(a - b) <= atol)), (name, col, rel.max())
+    assert np.array_equal(ho[:, 10:12], hr[:, 10:12]), "fallback counters differ"
+    sU = np.abs(ref["conserved"]).max(axis=0) + 1e-300
+    assert (np.abs(out["conserved"] - ref["conserved"]) / sU).max() <= 1e-9
+    for k in (5, 6, 7):
+        assert abs(out["forces"][k] - ref["forces"][k]) <= 1e-9 * max(abs(ref["forces"][k]), 1e-12) + 1e-14
+    assert np.allclose(out["wall_cp"], ref["cp"], rtol=1e-9, atol=1e-12)
