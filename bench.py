@@ -66,6 +66,7 @@ def cpu_baseline(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
     """FP64 oracle (serial restatement of the reference CPU path) on the same mesh and state."""
     try:
         sys.path.insert(0, str(ROOT / "tests"))
+        sys.path.insert(0, str(ROOT / "tests" / "golden"))
         import numpy as np
         import oracle_lib as ol
         import helpers
@@ -102,7 +103,20 @@ def main() -> int:
     ap.add_argument("--block-cells", type=int, default=256)
     ap.add_argument("--no-renumber", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample length (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the path has no "
               "multi-GPU execution in this build", file=sys.stderr)
