@@ -141,8 +141,11 @@ __device__ __forceinline__ int wave_min_i(int v) {
 // Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
 // MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
 // preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
+#ifndef AERO_FLUX_MIN_WAVES
+#define AERO_FLUX_MIN_WAVES 1
+#endif
 template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
-__global__ __launch_bounds__(BC) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
+__global__ __launch_bounds__(BC, AERO_FLUX_MIN_WAVES) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
                                               const double* __restrict__ G, double* __restrict__ R,
                                               double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
   extern __shared__ double lds[];

@@ -407,6 +407,20 @@ template <class R> Mesh<R> generate(const GenConfig& g) {  // make_airfoil_ogrid
       pts.push_back({in[0] + sigma * (ox - in[0]), in[1] + sigma * (oy - in[1])});
     }
   }
+  std::vector<std::vector<int>> cells;
+  cells.reserve(static_cast<std::size_t>(2 * nt * nr));
+  for (int j = 0; j < nr; ++j)
+    for (int i = 0; i < nt; ++i) {
+      const int i1 = (i + 1) % nt;
+      const int a = pid(i, j), b = pid(i1, j), c = pid(i1, j + 1), d = pid(i, j + 1);
+      const bool quad = g.cells == kQuad || (g.cells == kMixed && j < g.quad_layers);
+      if (quad) {
+        cells.push_back({a, b, c, d});
+      } else {
+        cells.push_back({a, b, c});
+        cells.push_back({a, c, d});
+      }
+    }
   if (g.jitter > 0.0) {  // interior nodes only; displacements from the unjittered grid
     Rng rng{g.seed};
     std::vector<P2<R>> moved = pts;
@@ -427,22 +441,33 @@ template <class R> Mesh<R> generate(const GenConfig& g) {  // make_airfoil_ogrid
         moved[pid(i, j)][0] += a * tx / tl + b * rx / rl;
         moved[pid(i, j)][1] += a * ty / tl + b * ry / rl;
       }
+    // Thin trailing-edge wake cells can be narrower than 10% of their edge lengths: a node keeps its
+    // displacement only if every cell around it keeps at least half its unjittered signed area.
+    auto sarea = [](const std::vector<P2<R>>& p, const std::vector<int>& cv) {
+      R s = R(0);
+      for (std::size_t q = 0; q < cv.size(); ++q) {
+        const auto& a = p[cv[q]];
+        const auto& b = p[cv[(q + 1) % cv.size()]];
+        s += a[0] * b[1] - b[0] * a[1];
+      }
+      return s;
+    };
+    for (int pass = 0; pass < 16; ++pass) {
+      bool changed = false;
+      for (const auto& cv : cells) {
+        const R a0 = sarea(pts, cv), a1 = sarea(moved, cv);
+        if (!(a1 / a0 >= AM_K(0.5))) {
+          for (int v : cv)
+            if (moved[v][0] != pts[v][0] || moved[v][1] != pts[v][1]) {
+              moved[v] = pts[v];
+              changed = true;
+            }
+        }
+      }
+      if (!changed) break;
+    }
     pts.swap(moved);
   }
-  std::vector<std::vector<int>> cells;
-  cells.reserve(static_cast<std::size_t>(2 * nt * nr));
-  for (int j = 0; j < nr; ++j)
-    for (int i = 0; i < nt; ++i) {
-      const int i1 = (i + 1) % nt;
-      const int a = pid(i, j), b = pid(i1, j), c = pid(i1, j + 1), d = pid(i, j + 1);
-      const bool quad = g.cells == kQuad || (g.cells == kMixed && j < g.quad_layers);
-      if (quad) {
-        cells.push_back({a, b, c, d});
-      } else {
-        cells.push_back({a, b, c});
-        cells.push_back({a, c, d});
-      }
-    }
   if (g.rcm) {
     std::map<std::pair<int, int>, std::vector<int>> ec;
     for (int c = 0; c < static_cast<int>(cells.size()); ++c) {
