@@ -578,7 +578,7 @@ int aero_hip_layout(aero_hip_mesh* m, int64_t* out8) {
   out8[3] = m->BC;
   out8[4] = m->S;
   out8[5] = m->dm.smax;
-  out8[6] = m->dm.c_off ? 0 : 0;
+  out8[6] = 0;
   out8[7] = m->dm.grid;
   return AERO_OK;
 }
