@@ -98,6 +98,15 @@ class MeshArrays(C.Structure):
                 ("wall_start", i32), ("wall_count", i32), ("farfield_start", i32), ("farfield_count", i32)]
 
 
+class BatchCaseC(C.Structure):
+    _fields_ = [("mesh_index", i32), ("mach", f64), ("aoa_deg", f64)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("iterations", i32), ("converged", i32), ("failed", i32), ("l2", f64), ("rho_l2", f64),
+                ("rho_ratio", f64), ("cl", f64), ("cd", f64), ("cm", f64)]
+
+
 # Every symbol include/aero_hip.h declares, with its ctypes prototype.
 PROTOTYPES = {
     "aero_hip_available": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -116,6 +125,13 @@ PROTOTYPES = {
     "aero_hip_layout": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     "aero_hip_stage": (C.c_int, [vp, C.POINTER(ResidualConfig), i32, i32, f64, i32, i32, C.c_char_p, C.c_size_t]),
     "aero_hip_run_case": (C.c_int, [vp, C.POINTER(CaseConfigC), C.POINTER(RunOutput), C.c_char_p, C.c_size_t]),
+    "aero_hip_batch_create": (C.c_int, [C.POINTER(vp), i32, C.POINTER(BatchCaseC), i32, C.POINTER(CaseConfigC), i32,
+                                        C.POINTER(vp), C.c_char_p, C.c_size_t]),
+    "aero_hip_batch_destroy": (None, [vp]),
+    "aero_hip_batch_run": (C.c_int, [vp, i32, f64, i32, i32, i32, C.POINTER(BatchResult), PD, C.c_char_p,
+                                     C.c_size_t]),
+    "aero_hip_batch_get_state": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_batch_history": (C.c_int, [vp, i32, PD, i32, C.c_char_p, C.c_size_t]),
     "aero_mesh_generate": (C.c_int, [C.POINTER(MeshGenConfig), i32, C.POINTER(MeshArrays), C.c_char_p,
                                      C.c_size_t]),
     "aero_mesh_free": (None, [C.POINTER(MeshArrays)]),

@@ -230,3 +230,67 @@ class HipMesh:
         return dict(history=hist[:n].copy(), n_iter=n, nonfinite=bool(out.nonfinite), conserved=U, residual=R,
                     spectral_radius=lam, forces=forces, cl=forces[5], cd=forces[6], cm=forces[7],
                     wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)
+
+
+class HipBatch:
+    """Batched solves over device geometries (HipMesh handles built with the same block_cells): one
+    launch per RK stage over every (case, block), device-side iteration control, hipGraph replay."""
+
+    def __init__(self, meshes: list, cases: list, base: CaseConfig, hist_capacity: int = 0):
+        self.lib = _lib.lib()
+        self.meshes = list(meshes)
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.meshes))(*[m.h.value for m in self.meshes])
+        cs = (_lib.BatchCaseC * len(self.cases))(*[_lib.BatchCaseC(int(i), float(mach), float(aoa))
+                                                   for (i, mach, aoa) in self.cases])
+        self._cfg = base.to_c()
+        self.hist_capacity = hist_capacity
+        h = C.c_void_p()
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_create(arr, len(self.meshes), cs, len(self.cases), C.byref(self._cfg),
+                                                  hist_capacity, C.byref(h), err, len(err)),
+                   err, "aero_hip_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aero_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, max_iters: int, rho_drop: float = 1e-8, fixed_iters: bool = False, iters_per_sync: int = 50,
+            use_graph: bool = True) -> dict:
+        res = (_lib.BatchResult * len(self.cases))()
+        sec = C.c_double()
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_run(self.h, max_iters, rho_drop, int(fixed_iters), iters_per_sync,
+                                               int(use_graph), res, C.byref(sec), err, len(err)),
+                   err, "aero_hip_batch_run")
+        rows = [dict(iterations=r.iterations, converged=bool(r.converged), failed=bool(r.failed), l2=r.l2,
+                     rho_l2=r.rho_l2, rho_ratio=r.rho_ratio, cl=r.cl, cd=r.cd, cm=r.cm) for r in res]
+        return dict(cases=rows, seconds=sec.value)
+
+    def state(self, i: int) -> np.ndarray:
+        g = self.meshes[self.cases[i][0]]
+        U = np.zeros((g.N, 5))
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_get_state(self.h, i, _pd(U), err, len(err)), err, "batch state")
+        return U
+
+    def history(self, i: int) -> np.ndarray:
+        out = np.zeros((max(self.hist_capacity, 1), 4))
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_history(self.h, i, _pd(out), self.hist_capacity, err, len(err)), err,
+                   "batch history")
+        return out[: self.hist_capacity]

@@ -1,0 +1,434 @@
+// aero_batch.cpp -- batched pseudo-time solves: many cases (an AoA / Mach sweep, or meshes of uneven
+// size) advanced together, one launch per RK stage over all (case, block) work items, with the driver's
+// per-iteration control logic on the device so that a whole iteration is one replayable hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "aero_hip.h"
+#include "aero_host.hpp"
+#include "aero_internal.hpp"
+
+using aero_dev::BatchCase;
+using aero_dev::BatchParams;
+using aero_dev::BatchStage;
+using aero_dev::CaseCtl;
+using aero_dev::DevMesh;
+using aero_dev::FaceOpts;
+
+struct aero_hip_batch {
+  int ncases = 0, ngeom = 0, BC = 256, smax = 1, nwork = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  std::vector<aero_hip_mesh*> geoms;
+  std::vector<BatchCase> h_cases;
+  std::vector<aero_batch_case> spec;
+  DevMesh* d_geoms = nullptr;
+  BatchCase* d_cases = nullptr;
+  CaseCtl* d_ctl = nullptr;
+  int2* d_work = nullptr;
+  double* d_hist = nullptr;
+  int hist_cap = 0;
+  aero_case_config cfg{};
+  FaceOpts base{};
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [second order][start buffer]
+  std::vector<double> alpha, q_inf;
+  int state_buf = 0;  // ring buffer that holds every case's current state
+};
+
+namespace {
+
+void set_err(char* err, size_t errlen, const std::string& msg) {
+  if (err && errlen > 0) {
+    std::strncpy(err, msg.c_str(), errlen - 1);
+    err[errlen - 1] = '\0';
+  }
+}
+
+#define AERO_TRY(expr, ctx)                                                              \
+  do {                                                                                   \
+    const hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                              \
+      set_err(err, errlen, std::string(ctx) + ": " + hipGetErrorString(e_));             \
+      return AERO_EHIP;                                                                  \
+    }                                                                                    \
+  } while (0)
+
+template <class T> hipError_t balloc(aero_hip_batch* b, T** p, size_t n) {
+  void* q = nullptr;
+  const hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    b->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+  }
+  return e;
+}
+
+void free_batch(aero_hip_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& row : b->graph)
+    for (auto& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
+  for (void* p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+double clampd(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+
+// One pseudo-time iteration of every active case, enqueued on the batch stream.
+int enqueue_iteration(aero_hip_batch* b, bool second, int start, char* err, size_t errlen) {
+  const aero_case_config& c = b->cfg;
+  AERO_TRY(aero_host::launch_batch_init_diag(b->d_cases, b->ncases, b->stream), "k_batch_init_diag");
+  // Ring: RK3 ends in its start buffer (stage 2 updates in place: each thread reads U^n of its own cell
+  // before writing it); RK1 ping-pongs between two buffers.
+  int cur = start;
+  int final_buf = start;
+  for (int st = 0; st < c.rk_stages; ++st) {
+    BatchStage bs{};
+    bs.stage = st;
+    bs.last = st == c.rk_stages - 1;
+    bs.cur = cur;
+    bs.base = start;
+    if (c.rk_stages == 3)
+      bs.out = st == 0 ? (start + 1) % 3 : (st == 1 ? (start + 2) % 3 : start);
+    else
+      bs.out = 1 - start;
+    if (second)
+      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
+                                                b->smax, c.gamma, cur, b->stream),
+               "k_gradient_batch");
+    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, b->d_geoms, b->d_cases, b->d_ctl,
+                                          b->d_work, b->nwork, b->smax, b->base, bs, b->stream),
+             "k_flux_batch");
+    cur = bs.out;
+    final_buf = bs.out;
+  }
+  AERO_TRY(aero_host::launch_batch_reduce(b->d_cases, b->d_ctl, b->ncases, c.gamma, final_buf, b->stream),
+           "k_batch_reduce");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const aero_batch_case* cases,
+                          int32_t n_cases, const aero_case_config* c, int32_t hist_capacity, aero_hip_batch** out,
+                          char* err, size_t errlen) {
+  if (!meshes || n_meshes <= 0 || !cases || n_cases <= 0 || !c || !out) {
+    set_err(err, errlen, "aero_hip_batch_create: bad argument");
+    return AERO_EINVAL;
+  }
+  *out = nullptr;
+  if (c->rk_stages != 1 && c->rk_stages != 3) {
+    set_err(err, errlen, "Euler pseudo-time rk_stages must be 1 or 3.");
+    return AERO_EINVAL;
+  }
+  if (!c->local_time_stepping) {
+    set_err(err, errlen, "aero_hip_batch_create: batched solves use local time stepping");
+    return AERO_EINVAL;
+  }
+  if (c->all_speed_staged_controller) {
+    set_err(err, errlen, "aero_hip_batch_create: the staged all-speed controller is single-case only");
+    return AERO_EINVAL;
+  }
+  const int BC = meshes[0]->BC;
+  for (int g = 0; g < n_meshes; ++g)
+    if (!meshes[g] || meshes[g]->BC != BC) {
+      set_err(err, errlen, "aero_hip_batch_create: all meshes need the same block_cells");
+      return AERO_EINVAL;
+    }
+  for (int i = 0; i < n_cases; ++i)
+    if (cases[i].mesh_index < 0 || cases[i].mesh_index >= n_meshes) {
+      set_err(err, errlen, "aero_hip_batch_create: case " + std::to_string(i) + " has a bad mesh_index");
+      return AERO_EINVAL;
+    }
+  aero_hip_batch* b = new aero_hip_batch();
+  b->ncases = n_cases;
+  b->ngeom = n_meshes;
+  b->BC = BC;
+  b->cfg = *c;
+  b->spec.assign(cases, cases + n_cases);
+  b->geoms.assign(meshes, meshes + n_meshes);
+  b->hist_cap = std::max(hist_capacity, 0);
+  for (auto* m : b->geoms) b->smax = std::max(b->smax, m->dm.smax);
+
+#define AERO_BSETUP(expr)                                                                      \
+  do {                                                                                         \
+    const hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                                    \
+      set_err(err, errlen, std::string("aero_hip_batch_create: ") + hipGetErrorString(e_));   \
+      free_batch(b);                                                                           \
+      delete b;                                                                                \
+      return AERO_EHIP;                                                                        \
+    }                                                                                          \
+  } while (0)
+
+  AERO_BSETUP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  const double g = c->gamma;
+  const bool precond = c->precond_on && c->precond_experimental_enabled;
+  b->base.g = g;
+  b->base.low_mach_fix = c->low_mach_fix;
+  b->base.low_mach_cutoff = c->mach_cutoff;
+  b->base.char_bc = c->precond_farfield_bc;
+  b->base.precond = precond ? 1 : 0;
+  b->base.as_fix = c->all_speed_flux_fix;
+  b->base.as_cut = c->all_speed_mach_cutoff;
+  b->base.as_fmin = clampd(c->all_speed_f_min, 1.0e-6, 1.0);
+  std::vector<CaseCtl> ctl(n_cases);
+  b->h_cases.resize(n_cases);
+  b->alpha.resize(n_cases);
+  b->q_inf.resize(n_cases);
+  for (int i = 0; i < n_cases; ++i) {
+    aero_hip_mesh* m = meshes[cases[i].mesh_index];
+    BatchCase& cs = b->h_cases[i];
+    cs.geom = cases[i].mesh_index;
+    cs.nblk = m->nblk;
+    cs.N = m->N;
+    const size_t N = m->N;
+    for (int k = 0; k < 3; ++k) AERO_BSETUP(balloc(b, &cs.U[k], 4 * N));
+    AERO_BSETUP(balloc(b, &cs.G, 8 * N));
+    cs.R = nullptr;
+    cs.lam = nullptr;
+    AERO_BSETUP(balloc(b, &cs.partials, 4 * static_cast<size_t>(m->nblk)));
+    AERO_BSETUP(balloc(b, &cs.diag, 4));
+    // freestream of this case (euler_solver.cpp:543-561) and its schedules
+    const double mach = cases[i].mach;
+    double rho = c->rho_inf;
+    if (rho <= 0.0) rho = c->p_inf / (c->gas_constant * c->t_inf);
+    const double pi = 3.14159265358979323846;
+    const double alpha = cases[i].aoa_deg * (pi / 180.0);
+    const double a = std::sqrt(g * c->p_inf / rho);
+    const double v = std::max(mach, 0.0) * a;
+    const double u = v * std::cos(alpha), w = v * std::sin(alpha);
+    const double r = std::max(rho, 1.0e-8);
+    const double ke = 0.5 * r * (u * u + w * w + 0.0 * 0.0);
+    cs.ff[0] = r;
+    cs.ff[1] = r * u;
+    cs.ff[2] = r * w;
+    cs.ff[3] = std::max(c->p_inf, 1.0e-8) / (g - 1.0) + ke;
+    const double minf = std::max(mach, 1.0e-4);
+    const double k0 = c->stab_floor_k_start >= 0.0 ? c->stab_floor_k_start : (c->stab_floor_start / minf);
+    const double k1 = c->stab_floor_k_target >= 0.0 ? c->stab_floor_k_target : (c->stab_floor_target / minf);
+    cs.floor_start = clampd(std::max(k0, 0.0) * minf, 0.0, 1.0);
+    cs.floor_target = clampd(std::max(k1, 0.0) * minf, 0.0, 1.0);
+    const double auto_bmin = std::max(1.0e-4, std::min(1.0e-3, 0.1 * minf));
+    cs.mach_ref = std::max(c->precond_mach_ref > 0.0 ? c->precond_mach_ref : minf, 1.0e-4);
+    cs.mach_min = std::max(c->precond_mach_min > 0.0 ? c->precond_mach_min : 1.0e-6, 1.0e-6);
+    cs.beta_min = std::max(c->precond_beta_min > 0.0 ? c->precond_beta_min : auto_bmin, 1.0e-4);
+    cs.beta_max = std::max(c->precond_beta_max, cs.beta_min);
+    cs.wall = m->wall;
+    cs.wall.p_inf = c->p_inf;
+    cs.wall.q_inf = 0.5 * rho * v * v + 1.0e-12;
+    cs.wall.x_ref = c->x_ref;
+    cs.wall.y_ref = c->y_ref;
+    b->alpha[i] = alpha;
+    b->q_inf[i] = cs.wall.q_inf;
+    std::vector<double> u0(4 * N);
+    for (size_t j = 0; j < N; ++j) {
+      u0[j] = cs.ff[0];
+      u0[N + j] = cs.ff[1];
+      u0[2 * N + j] = cs.ff[2];
+      u0[3 * N + j] = cs.ff[3];
+    }
+    AERO_BSETUP(hipMemcpy(cs.U[0], u0.data(), u0.size() * sizeof(double), hipMemcpyHostToDevice));
+    AERO_BSETUP(hipMemset(cs.G, 0, 8 * N * sizeof(double)));
+    // iteration-0 parameters (euler_solver.cpp:772-797)
+    CaseCtl& k = ctl[i];
+    std::memset(&k, 0, sizeof(k));
+    k.active = 1;
+    k.cfl_scale = 1.0;
+    k.prev_l2 = -1.0;
+    k.l2_0 = -1.0;
+    k.rho_l2_0 = -1.0;
+    double base;
+    if (c->cfl_ramp_iters <= 0) {
+      base = c->cfl_max;
+    } else {
+      const double t = std::min(1.0, 1.0 / static_cast<double>(std::max(c->cfl_ramp_iters, 1)));
+      base = c->cfl_start + t * (c->cfl_max - c->cfl_start);
+    }
+    k.cfl = base;
+    k.floor_cur = std::max(c->stab_ramp_iters, 0) <= 0 ? cs.floor_target : cs.floor_start;
+    const int as_on = std::max(c->all_speed_ramp_start_iter, 0), as_len = std::max(c->all_speed_ramp_iters, 0);
+    k.as_w = as_len <= 0 ? (0 >= as_on ? 1.0 : 0.0) : clampd(static_cast<double>(0 - as_on) / std::max(as_len, 1), 0.0, 1.0);
+  }
+  // Work list: per geometry, block-major / case-minor (the cases of a block share its face data in L2),
+  // then spread so that each XCD (workgroup index mod 8) receives a contiguous run.
+  std::vector<int2> list;
+  for (int gi = 0; gi < n_meshes; ++gi)
+    for (int blk = 0; blk < meshes[gi]->nblk; ++blk)
+      for (int i = 0; i < n_cases; ++i)
+        if (cases[i].mesh_index == gi) list.push_back(make_int2(i, blk));
+  const int W = static_cast<int>(list.size());
+  const int per = (W + 7) / 8;
+  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
+  for (int j = 0; j < static_cast<int>(work.size()); ++j) {
+    const int src = (j & 7) * per + (j >> 3);
+    if (src < W) work[j] = list[src];
+  }
+  b->nwork = static_cast<int>(work.size());
+  std::vector<DevMesh> gtab(n_meshes);
+  for (int gi = 0; gi < n_meshes; ++gi) gtab[gi] = meshes[gi]->dm;
+  AERO_BSETUP(balloc(b, &b->d_geoms, gtab.size()));
+  AERO_BSETUP(hipMemcpy(b->d_geoms, gtab.data(), gtab.size() * sizeof(DevMesh), hipMemcpyHostToDevice));
+  AERO_BSETUP(balloc(b, &b->d_cases, b->h_cases.size()));
+  AERO_BSETUP(hipMemcpy(b->d_cases, b->h_cases.data(), b->h_cases.size() * sizeof(BatchCase), hipMemcpyHostToDevice));
+  AERO_BSETUP(balloc(b, &b->d_ctl, ctl.size()));
+  AERO_BSETUP(hipMemcpy(b->d_ctl, ctl.data(), ctl.size() * sizeof(CaseCtl), hipMemcpyHostToDevice));
+  AERO_BSETUP(balloc(b, &b->d_work, work.size()));
+  AERO_BSETUP(hipMemcpy(b->d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice));
+  if (b->hist_cap > 0) AERO_BSETUP(balloc(b, &b->d_hist, static_cast<size_t>(n_cases) * b->hist_cap * 4));
+#undef AERO_BSETUP
+  *out = b;
+  return AERO_OK;
+}
+
+void aero_hip_batch_destroy(aero_hip_batch* b) {
+  if (!b) return;
+  free_batch(b);
+  delete b;
+}
+
+int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, int32_t fixed_iters,
+                       int32_t iters_per_sync, int32_t use_graph, aero_batch_result* results, double* seconds,
+                       char* err, size_t errlen) {
+  if (!b || max_iters <= 0) {
+    set_err(err, errlen, "aero_hip_batch_run: bad argument");
+    return AERO_EINVAL;
+  }
+  const aero_case_config& c = b->cfg;
+  BatchParams p{};
+  p.ncases = b->ncases;
+  p.max_iters = max_iters;
+  p.fixed_iters = fixed_iters ? 1 : 0;
+  p.rho_drop = rho_drop;
+  p.cfl_start = c.cfl_start;
+  p.cfl_max = c.cfl_max;
+  p.cfl_ramp = c.cfl_ramp_iters;
+  p.floor_ramp = std::max(c.stab_ramp_iters, 0);
+  p.as_on = std::max(c.all_speed_ramp_start_iter, 0);
+  p.as_len = std::max(c.all_speed_ramp_iters, 0);
+  p.hist = b->d_hist;
+  p.hist_cap = b->hist_cap;
+  const int startup = std::max(c.startup_first_order_iters, 0);
+  const int sync_every = std::max(iters_per_sync, 1);
+  auto enqueue = [&](bool second, int start) -> int {
+    int rc = enqueue_iteration(b, second, start, err, errlen);
+    if (rc) return rc;
+    AERO_TRY(aero_host::launch_batch_control(b->d_cases, b->d_ctl, p, b->stream), "k_batch_control");
+    return 0;
+  };
+  // Graphs are specific to (order, start buffer) and to the run parameters; rebuild them per run.
+  for (auto& row : b->graph)
+    for (auto& gx : row)
+      if (gx) {
+        (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+      }
+  std::vector<CaseCtl> h(b->ncases);
+  const auto t0 = std::chrono::steady_clock::now();
+  int it = 0, start = b->state_buf;
+  while (it < max_iters) {
+    const bool second = it >= startup;
+    const int phase_end = second ? max_iters : std::min(startup, max_iters);
+    const int n = std::min(sync_every, phase_end - it);
+    for (int k = 0; k < n; ++k) {
+      if (use_graph) {
+        hipGraphExec_t& gx = b->graph[second ? 1 : 0][start];
+        if (!gx) {
+          hipGraph_t graph = nullptr;
+          AERO_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+          const int rc = enqueue(second, start);
+          hipGraph_t captured = nullptr;
+          const hipError_t ec = hipStreamEndCapture(b->stream, &captured);
+          if (rc) return rc;
+          AERO_TRY(ec, "hipStreamEndCapture");
+          graph = captured;
+          AERO_TRY(hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+          (void)hipGraphDestroy(graph);
+        }
+        AERO_TRY(hipGraphLaunch(gx, b->stream), "hipGraphLaunch");
+      } else {
+        const int rc = enqueue(second, start);
+        if (rc) return rc;
+      }
+      if (c.rk_stages == 1) start = 1 - start;
+    }
+    it += n;
+    AERO_TRY(hipMemcpyAsync(h.data(), b->d_ctl, h.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost, b->stream),
+             "control readback");
+    AERO_TRY(hipStreamSynchronize(b->stream), "aero_hip_batch_run: hipStreamSynchronize");
+    bool any = false;
+    for (const auto& k : h) any = any || k.active;
+    if (!any) break;
+  }
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < b->ncases; ++i) {
+    const CaseCtl& k = h[i];
+    aero_batch_result& r = results[i];
+    r.iterations = k.iter;
+    r.converged = k.converged;
+    r.failed = k.failed;
+    r.l2 = k.l2;
+    r.rho_l2 = k.rho_l2;
+    r.rho_ratio = k.rho_ratio;
+    // forces.cpp:263-293 from the gauge sums of the last iteration
+    const double fx = -k.sums[6], fy = -k.sums[7], mo = k.sums[8];
+    const double ca = std::cos(b->alpha[i]), sa = std::sin(b->alpha[i]);
+    const double chord = std::max(c.chord, 1.0e-8), q = b->q_inf[i];
+    const double drag = fx * ca + fy * sa, lift = -fx * sa + fy * ca;
+    r.cd = drag / (q * chord);
+    r.cl = lift / (q * chord);
+    r.cm = mo / (q * chord * chord);
+  }
+  b->state_buf = start;  // where every case's state is after this run
+  return AERO_OK;
+}
+
+int aero_hip_batch_get_state(aero_hip_batch* b, int32_t case_index, double* conserved, char* err, size_t errlen) {
+  if (!b || case_index < 0 || case_index >= b->ncases || !conserved) {
+    set_err(err, errlen, "aero_hip_batch_get_state: bad argument");
+    return AERO_EINVAL;
+  }
+  const BatchCase& cs = b->h_cases[case_index];
+  aero_hip_mesh* m = b->geoms[cs.geom];
+  const size_t N = cs.N;
+  std::vector<double> u(4 * N);
+  const int buf = b->state_buf;
+  AERO_TRY(hipMemcpy(u.data(), cs.U[buf], u.size() * sizeof(double), hipMemcpyDeviceToHost), "state copy");
+  for (size_t ic = 0; ic < N; ++ic) {
+    double* o = conserved + 5 * static_cast<size_t>(m->perm[ic]);
+    o[0] = u[ic];
+    o[1] = u[N + ic];
+    o[2] = u[2 * N + ic];
+    o[3] = 0.0;
+    o[4] = u[3 * N + ic];
+  }
+  return AERO_OK;
+}
+
+int aero_hip_batch_history(aero_hip_batch* b, int32_t case_index, double* hist, int32_t cap, char* err,
+                           size_t errlen) {
+  if (!b || case_index < 0 || case_index >= b->ncases || !hist || cap < 0) {
+    set_err(err, errlen, "aero_hip_batch_history: bad argument");
+    return AERO_EINVAL;
+  }
+  const int n = std::min(cap, b->hist_cap);
+  if (n <= 0) return AERO_OK;
+  AERO_TRY(hipMemcpy(hist, b->d_hist + static_cast<size_t>(case_index) * b->hist_cap * 4, n * 4 * sizeof(double),
+                     hipMemcpyDeviceToHost),
+           "history copy");
+  return AERO_OK;
+}
+
+}  // extern "C"

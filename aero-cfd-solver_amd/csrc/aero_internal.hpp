@@ -60,6 +60,54 @@ struct WallArgs {
   double p_inf, q_inf, x_ref, y_ref;
 };
 
+// ---- batched execution ---------------------------------------------------------------------------
+// One case of a batch: its own state ring and work buffers over a shared geometry.
+struct BatchCase {
+  int geom;          // index into the geometry table
+  int nblk;          // blocks of that geometry
+  int N;             // cells of that geometry
+  double* U[3];      // state ring
+  double* G;
+  double* R;
+  double* lam;
+  double* partials;  // [nblk][4]
+  int* diag;         // [4]
+  double ff[4];      // freestream (rho, rhou, rhov, rhoE)
+  double floor_start, floor_target;         // stabilisation-floor schedule (euler_solver.cpp:141-157)
+  double mach_ref, mach_min, beta_min, beta_max;  // build_precondition_config (euler_solver.cpp:341-355)
+  WallArgs wall;     // wall faces of the geometry + this case's reference values
+};
+
+// Device-resident control state of one case, advanced by k_batch_control after every iteration with the
+// driver's per-iteration logic (euler_solver.cpp:771-797, :948-958).
+struct CaseCtl {
+  int active;     // still iterating
+  int converged;  // reached the density-residual drop
+  int failed;     // non-finite trap
+  int iter;       // iterations completed
+  double cfl, floor_cur, as_w;  // parameters of the next iteration
+  double cfl_scale, prev_l2, l2_0, rho_l2_0;
+  double l1, l2, linf, rho_l2, ratio, rho_ratio;
+  double sums[9];  // k_iter_reduce output of the last iteration
+};
+
+struct BatchParams {
+  int ncases;
+  int max_iters;
+  int fixed_iters;   // 1: ignore the convergence test (run exactly max_iters)
+  double rho_drop;   // converged when rho_l2 <= rho_drop * rho_l2(iteration 0)
+  double cfl_start, cfl_max;
+  int cfl_ramp;
+  int floor_ramp;
+  int as_on, as_len;
+  double* hist;      // [ncases][hist_cap][4]: l2, rho_l2, cfl, cl-sum proxy (may be null)
+  int hist_cap;
+};
+
+struct BatchStage {
+  int stage, last, cur, base, out;
+};
+
 }  // namespace aero_dev
 
 namespace aero_host {
@@ -73,4 +121,16 @@ hipError_t launch_global_update(const aero_dev::DevMesh& m, const aero_dev::Face
 hipError_t launch_norm_partials(const aero_dev::DevMesh& m, const double* R, double* partials, hipStream_t s);
 hipError_t launch_iter_reduce(const aero_dev::DevMesh& m, const double* partials, const aero_dev::WallArgs& wa,
                               const double* U, double g, double* out, double* cp, hipStream_t s);
+hipError_t launch_batch_gradient(int BC, const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
+                                 const aero_dev::CaseCtl* ctl, const int2* work, int nwork, int smax, double g, int cur,
+                                 hipStream_t s);
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, const aero_dev::DevMesh* geoms,
+                             const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl, const int2* work,
+                             int nwork, int smax, const aero_dev::FaceOpts& base, const aero_dev::BatchStage& bs,
+                             hipStream_t s);
+hipError_t launch_batch_reduce(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl, int ncases, double g, int cur,
+                               hipStream_t s);
+hipError_t launch_batch_control(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl,
+                                const aero_dev::BatchParams& p, hipStream_t s);
+hipError_t launch_batch_init_diag(const aero_dev::BatchCase* cases, int ncases, hipStream_t s);
 }  // namespace aero_host

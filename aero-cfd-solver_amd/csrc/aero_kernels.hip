@@ -41,11 +41,8 @@ __device__ __forceinline__ int block_of(const DevMesh& m) {
 
 // Green-Gauss gradient (reconstruction.cpp:73-119), per block.
 template <int BC>
-__global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
-                                                  double g) {
-  extern __shared__ double lds[];
-  const int b = block_of(m);
-  if (b >= m.nblk) return;
+__device__ __forceinline__ void gradient_body(const DevMesh& m, int b, const double* __restrict__ U,
+                                              double* __restrict__ G, double g, double* lds) {
   const int s0 = m.slot_off[b];
   const int ns = m.slot_off[b + 1] - s0;
   const int smax = m.smax;
@@ -110,6 +107,15 @@ __global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __rest
   G[7 * ld + c] = gy3 * iv;
 }
 
+template <int BC>
+__global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
+                                                  double g) {
+  extern __shared__ double lds[];
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  gradient_body<BC>(m, b, U, G, g, lds);
+}
+
 // MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
 template <int LIMITER>
 __device__ __forceinline__ void recon_var(double qo, double qn, double gox, double goy, double gnx, double gny,
@@ -141,16 +147,11 @@ __device__ __forceinline__ int wave_min_i(int v) {
 // Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
 // MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
 // preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
-#ifndef AERO_FLUX_MIN_WAVES
-#define AERO_FLUX_MIN_WAVES 1
-#endif
 template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
-__global__ __launch_bounds__(BC, AERO_FLUX_MIN_WAVES) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
-                                              const double* __restrict__ G, double* __restrict__ R,
-                                              double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
-  extern __shared__ double lds[];
-  const int b = block_of(m);
-  if (b >= m.nblk) return;
+__device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpts& o, const double* __restrict__ U,
+                                          const double* __restrict__ G, double* __restrict__ R,
+                                          double* __restrict__ lam, int* __restrict__ diag, const StageArgs& st,
+                                          double* lds) {
   const int s0 = m.slot_off[b];
   const int ns = m.slot_off[b + 1] - s0;
   const int smax = m.smax;
@@ -376,6 +377,16 @@ __global__ __launch_bounds__(BC, AERO_FLUX_MIN_WAVES) void k_flux(DevMesh m, Fac
   }
 }
 
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+__global__ __launch_bounds__(BC) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
+                                              const double* __restrict__ G, double* __restrict__ R,
+                                              double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
+  extern __shared__ double lds[];
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  flux_body<BC, SCHEME, LIMITER, SECOND, MODE>(m, b, o, U, G, R, lam, diag, st, lds);
+}
+
 // Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
 __global__ void k_min_dt(int N, const double* __restrict__ lam, double cfl, double* __restrict__ out) {
   __shared__ double red[256];
@@ -494,10 +505,10 @@ __global__ __launch_bounds__(BC) void k_norm_partials(DevMesh m, const double* _
 // sum v_rho^2), then wall pressure-force sums into out[4..8]:
 //   sum p nA_x, sum p nA_y (absolute), sum (p - p_inf) nA_x, sum (p - p_inf) nA_y, sum moment term,
 // with nA = -n A (forces.cpp:134-149, :220-246, :274-285).  One workgroup of 256 threads.
-__global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __restrict__ partials, WallArgs wa,
-                                                      const double* __restrict__ U, int ld, double g,
-                                                      double* __restrict__ out, double* __restrict__ cp) {
-  __shared__ double red[9][256];
+__device__ __forceinline__ void iter_reduce_body(int nblk, const double* __restrict__ partials, const WallArgs& wa,
+                                                 const double* __restrict__ U, int ld, double g,
+                                                 double* __restrict__ out, double* __restrict__ cp,
+                                                 double (*red)[256]) {
   const int t = threadIdx.x;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   for (int b = t; b < nblk; b += 256) {
@@ -548,6 +559,144 @@ __global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __r
     __syncthreads();
   }
   if (t < 9) out[t] = red[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __restrict__ partials, WallArgs wa,
+                                                      const double* __restrict__ U, int ld, double g,
+                                                      double* __restrict__ out, double* __restrict__ cp) {
+  __shared__ double red[9][256];
+  iter_reduce_body(nblk, partials, wa, U, ld, g, out, cp, red);
+}
+
+// ---- batched kernels: one workgroup per (case, block) work item --------------------------------
+template <int BC>
+__global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict__ geoms,
+                                                        const BatchCase* __restrict__ cases,
+                                                        const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
+                                                        double g, int cur) {
+  extern __shared__ double lds[];
+  const int2 w = work[blockIdx.x];
+  if (w.x < 0 || !ctl[w.x].active) return;
+  const BatchCase& cs = cases[w.x];
+  gradient_body<BC>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, lds);
+}
+
+template <int BC, int SCHEME, int LIMITER, bool SECOND>
+__global__ __launch_bounds__(BC) void k_flux_batch(const DevMesh* __restrict__ geoms,
+                                                    const BatchCase* __restrict__ cases,
+                                                    const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
+                                                    FaceOpts o, BatchStage bs) {
+  extern __shared__ double lds[];
+  const int2 w = work[blockIdx.x];
+  if (w.x < 0) return;
+  const CaseCtl& k = ctl[w.x];
+  if (!k.active) return;
+  const BatchCase& cs = cases[w.x];
+  o.ff = C4{cs.ff[0], cs.ff[1], cs.ff[2], cs.ff[3]};
+  o.stab_floor = k.floor_cur;
+  o.as_w = k.as_w;
+  o.mach_ref = cs.mach_ref;
+  o.mach_min = cs.mach_min;
+  o.beta_min = cs.beta_min;
+  o.beta_max = cs.beta_max;
+  StageArgs st;
+  st.stage = bs.stage;
+  st.last_stage = bs.last;
+  st.write_res = 0;
+  st.cfl = k.cfl;
+  st.Un = cs.U[bs.base];
+  st.Unext = cs.U[bs.out];
+  st.partials = cs.partials;
+  flux_body<BC, SCHEME, LIMITER, SECOND, 1>(geoms[cs.geom], w.y, o, cs.U[bs.cur], cs.G, nullptr, nullptr, cs.diag, st,
+                                            lds);
+}
+
+// Per case: norm partials + wall force sums of the updated state into ctl[case].sums (same fixed-order
+// reduction as the single-case path, so a case's history is identical to a solo run).
+__global__ __launch_bounds__(256) void k_batch_reduce(const BatchCase* __restrict__ cases, CaseCtl* __restrict__ ctl,
+                                                       double g, int cur) {
+  __shared__ double red[9][256];
+  const int c = blockIdx.x;
+  if (!ctl[c].active) return;
+  const BatchCase& cs = cases[c];
+  iter_reduce_body(cs.nblk, cs.partials, cs.wall, cs.U[cur], cs.N, g, ctl[c].sums, nullptr, red);
+}
+
+__global__ void k_batch_init_diag(const BatchCase* __restrict__ cases, int ncases) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncases) return;
+  int* d = cases[c].diag;
+  d[0] = 0;
+  d[1] = 0;
+  d[2] = 0x7fffffff;
+  d[3] = 0;
+}
+
+// The driver's per-iteration scalar logic on the device, one thread per case: norms (euler_solver.cpp:309-339),
+// residual ratio (convergence_monitor.cpp:13-19), adaptive CFL (euler_solver.cpp:951-958), then the CFL,
+// stabilisation-floor and all-speed ramps of the next iteration (:69-114).  Convergence is the density
+// residual drop rho_l2 <= rho_drop * rho_l2(iteration 0).
+__global__ void k_batch_control(const BatchCase* __restrict__ cases, CaseCtl* __restrict__ ctl, BatchParams p) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= p.ncases) return;
+  CaseCtl& k = ctl[c];
+  if (!k.active) return;
+  const BatchCase& cs = cases[c];
+  if (cs.diag[3]) {
+    k.failed = 1;
+    k.active = 0;
+    return;
+  }
+  const double N5 = static_cast<double>(cs.N * 5 > 1 ? cs.N * 5 : 1);
+  k.l1 = k.sums[0];
+  k.l2 = sqrt(k.sums[1] / N5);
+  k.linf = k.sums[2];
+  k.rho_l2 = sqrt(k.sums[3] / static_cast<double>(cs.N > 1 ? cs.N : 1));
+  const double b2 = dmax(k.l2, 1.0e-20);
+  if (k.l2_0 < 0.0) k.l2_0 = b2;
+  k.ratio = b2 / dmax(k.l2_0, 1.0e-20);
+  const double r2 = dmax(k.rho_l2, 1.0e-300);
+  if (k.rho_l2_0 < 0.0) k.rho_l2_0 = r2;
+  k.rho_ratio = r2 / k.rho_l2_0;
+  if (k.prev_l2 > 0.0) {
+    if (k.l2 > k.prev_l2 * 1.05)
+      k.cfl_scale = dmax(k.cfl_scale * 0.65, 0.05);
+    else if (k.l2 < k.prev_l2 * 0.90)
+      k.cfl_scale = dmin(k.cfl_scale * 1.03, 1.0);
+  }
+  k.prev_l2 = k.l2;
+  if (p.hist && k.iter < p.hist_cap) {
+    double* h = p.hist + (static_cast<long long>(c) * p.hist_cap + k.iter) * 4;
+    h[0] = k.l2;
+    h[1] = k.rho_l2;
+    h[2] = k.cfl;
+    h[3] = k.linf;
+  }
+  k.iter += 1;
+  if (!p.fixed_iters && k.rho_ratio <= p.rho_drop) k.converged = 1;
+  if (k.converged || k.iter >= p.max_iters) {
+    k.active = 0;
+    return;
+  }
+  const int it = k.iter;
+  double base;
+  if (p.cfl_ramp <= 0) {
+    base = p.cfl_max;
+  } else {
+    const double t = dmin(1.0, static_cast<double>(it + 1) / static_cast<double>(p.cfl_ramp > 1 ? p.cfl_ramp : 1));
+    base = p.cfl_start + t * (p.cfl_max - p.cfl_start);
+  }
+  k.cfl = base * k.cfl_scale;
+  if (p.floor_ramp <= 0) {
+    k.floor_cur = cs.floor_target;
+  } else {
+    const double x = dclamp(static_cast<double>(it) / static_cast<double>(p.floor_ramp > 1 ? p.floor_ramp : 1), 0.0, 1.0);
+    k.floor_cur = cs.floor_start + x * (cs.floor_target - cs.floor_start);
+  }
+  if (p.as_len <= 0)
+    k.as_w = it >= p.as_on ? 1.0 : 0.0;
+  else
+    k.as_w = dclamp(static_cast<double>(it - p.as_on) / static_cast<double>(p.as_len > 1 ? p.as_len : 1), 0.0, 1.0);
 }
 
 }  // namespace aero_dev
@@ -649,6 +798,64 @@ hipError_t launch_norm_partials(const DevMesh& m, const double* R, double* parti
     case 512: hipLaunchKernelGGL(k_norm_partials<512>, dim3(m.nblk), dim3(512), 0, s, m, R, partials); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_gradient(int BC, const DevMesh* geoms, const BatchCase* cases, const CaseCtl* ctl,
+                                 const int2* work, int nwork, int smax, double g, int cur, hipStream_t s) {
+  const size_t lds = static_cast<size_t>(smax) * 6 * sizeof(double);
+  switch (BC) {
+    case 128: hipLaunchKernelGGL(k_gradient_batch<128>, dim3(nwork), dim3(128), lds, s, geoms, cases, ctl, work, g, cur); break;
+    case 256: hipLaunchKernelGGL(k_gradient_batch<256>, dim3(nwork), dim3(256), lds, s, geoms, cases, ctl, work, g, cur); break;
+    case 512: hipLaunchKernelGGL(k_gradient_batch<512>, dim3(nwork), dim3(512), lds, s, geoms, cases, ctl, work, g, cur); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int BC>
+static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, const DevMesh* geoms,
+                                       const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork,
+                                       size_t lds, const FaceOpts& o, const BatchStage& bs, hipStream_t s) {
+#define AERO_BFLUX(SC, LI, SE)                                                                              \
+  if (scheme == SC && (limiter == LI || !SE) && second == SE) {                                             \
+    hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o, bs); \
+    return hipGetLastError();                                                                               \
+  }
+  AERO_BFLUX(0, 0, false)
+  AERO_BFLUX(1, 0, false)
+  AERO_BFLUX(0, 0, true)
+  AERO_BFLUX(0, 1, true)
+  AERO_BFLUX(1, 0, true)
+  AERO_BFLUX(1, 1, true)
+#undef AERO_BFLUX
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, const DevMesh* geoms,
+                             const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork, int smax,
+                             const FaceOpts& base, const BatchStage& bs, hipStream_t s) {
+  const size_t lds = std::max<size_t>(static_cast<size_t>(smax) * 5, 4 * (BC / 64)) * sizeof(double);
+  switch (BC) {
+    case 128: return launch_batch_flux_bc<128>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    case 256: return launch_batch_flux_bc<256>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    case 512: return launch_batch_flux_bc<512>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_batch_reduce(const BatchCase* cases, CaseCtl* ctl, int ncases, double g, int cur, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reduce, dim3(ncases), dim3(256), 0, s, cases, ctl, g, cur);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_control(const BatchCase* cases, CaseCtl* ctl, const BatchParams& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_control, dim3((p.ncases + 63) / 64), dim3(64), 0, s, cases, ctl, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_init_diag(const BatchCase* cases, int ncases, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_init_diag, dim3((ncases + 63) / 64), dim3(64), 0, s, cases, ncases);
   return hipGetLastError();
 }
 

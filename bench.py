@@ -94,6 +94,69 @@ def traffic_for(workload: str):
     return None
 
 
+def run_sweep(args) -> int:
+    """64-case AoA x Mach sweep on one ~66k-cell jittered mixed mesh: batched (one launch per RK stage over
+    all cases, device-side control, hipGraph replay, host sync every 100 iterations) against the same cases
+    solved one after another (batch of 1, no graph, host sync every iteration).  A case stops at an 8-decade
+    density-residual drop or at the iteration cap."""
+    import numpy as np
+    import torch
+    import aero_hip
+    from aero_hip import CaseConfig, HipBatch, HipMesh, generate_mesh
+
+    if not aero_hip.hip_available():
+        raise SystemExit("bench.py: no gfx950 device visible")
+    mkw = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
+               quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    log("[sweep] generating mesh")
+    mesh = generate_mesh(precision=8, **mkw)
+    g = HipMesh(mesh, block_cells=args.block_cells)
+    aoas = np.linspace(-4.0, 8.0, 16)
+    machs = [0.3, 0.5, 0.6, 0.7]
+    cases = [(0, float(m), float(a)) for m in machs for a in aoas]
+    base = CaseConfig(flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=1.1, cfl_ramp_iters=120,
+                      startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
+                      stab_ramp_iters=400, precond_experimental_enabled=0)
+    cap = args.sweep_iters
+    log(f"[sweep] batched: {len(cases)} cases x {mesh.num_cells} cells, cap {cap} iterations")
+    with HipBatch([g], cases, base) as b:
+        torch.cuda.synchronize()
+        rb = b.run(cap, 1e-8, iters_per_sync=100, use_graph=True)
+    t_batch = rb["seconds"]
+    nseq = min(args.sweep_seq_cases, len(cases))
+    log(f"[sweep] sequential: {nseq} cases one after another")
+    t_seq = 0.0
+    seq_rows = []
+    for i in range(nseq):
+        with HipBatch([g], [cases[i]], base) as b1:
+            r1 = b1.run(cap, 1e-8, iters_per_sync=1, use_graph=False)
+        t_seq += r1["seconds"]
+        seq_rows.append(r1["cases"][0])
+    # each sequential case ran the same iterations as in the batch (identical per-case arithmetic)
+    same = all(seq_rows[i]["iterations"] == rb["cases"][i]["iterations"] and
+               seq_rows[i]["rho_l2"] == rb["cases"][i]["rho_l2"] for i in range(nseq))
+    t_seq_all = t_seq * len(cases) / nseq
+    g.close()
+    rows = rb["cases"]
+    value = len(cases) / t_batch * 3600.0
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "solves/hr", "n_gpus": 1, "steps": cap,
+           "warmup": 0, "ms_per_step": round(t_batch / max(max(r["iterations"] for r in rows), 1) * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "sweep64_mixed66k_hllc_venkat", "cells": mesh.num_cells, "cases": len(cases),
+                      "mesh": mkw, "aoa_deg": [float(a) for a in aoas], "mach": machs, "iteration_cap": cap,
+                      "rho_drop_target": 1e-8},
+           "sweep": {"batched_seconds": round(t_batch, 3), "sequential_seconds": round(t_seq_all, 3),
+                     "sequential_cases_timed": nseq, "speedup_vs_sequential": round(t_seq_all / t_batch, 2),
+                     "sequential_solves_per_hr": round(len(cases) / t_seq_all * 3600.0, 1),
+                     "sequential_matches_batched": same,
+                     "converged_8_decades": sum(r["converged"] for r in rows),
+                     "failed_nonfinite": sum(r["failed"] for r in rows),
+                     "median_rho_drop": float(np.median([r["rho_ratio"] for r in rows])),
+                     "cl": [round(r["cl"], 6) for r in rows], "cd": [round(r["cd"], 6) for r in rows]}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +166,9 @@ def main() -> int:
     ap.add_argument("--block-cells", type=int, default=256)
     ap.add_argument("--no-renumber", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample length (0 = skip)")
+    ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
+    ap.add_argument("--sweep-iters", type=int, default=3000, help="iteration cap per case in --sweep")
+    ap.add_argument("--sweep-seq-cases", type=int, default=64, help="cases timed sequentially in --sweep")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -117,6 +183,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
         return 0
+    if args.sweep:
+        return run_sweep(args)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the path has no "
               "multi-GPU execution in this build", file=sys.stderr)

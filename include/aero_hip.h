@@ -187,6 +187,44 @@ int aero_hip_run_case(aero_hip_mesh* mesh, const aero_case_config* cfg, aero_run
                       size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
+ * Batched solves (no reference counterpart: the reference sweep runs cases one after another,
+ * python/cfd/sweep.py:14-59).  Cases share device geometries (aero_hip_mesh handles, all built with the
+ * same block_cells) and advance together, one launch per RK stage over every (case, block); the driver's
+ * per-iteration control (CFL ramp and adaptation, floor / all-speed ramps, norms, forces) runs on the
+ * device, so an iteration is a single replayable hipGraph.  A case stops at the density-residual drop
+ * rho_l2 <= rho_drop * rho_l2(iteration 0), at a non-finite state, or at max_iters.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct aero_hip_batch aero_hip_batch;
+
+typedef struct aero_batch_case {
+  int32_t mesh_index; /* index into the meshes passed to aero_hip_batch_create */
+  double mach;
+  double aoa_deg;
+} aero_batch_case;
+
+typedef struct aero_batch_result {
+  int32_t iterations, converged, failed;
+  double l2, rho_l2, rho_ratio, cl, cd, cm;
+} aero_batch_result;
+
+/* `base` supplies everything but Mach and AoA (flux, limiter, CFL schedule, startup, precond, ...);
+ * rk_stages 1 or 3, local time stepping only.  hist_capacity rows of (l2, rho_l2, cfl, linf) are kept
+ * per case. */
+int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const aero_batch_case* cases,
+                          int32_t n_cases, const aero_case_config* base, int32_t hist_capacity,
+                          aero_hip_batch** out, char* err, size_t errlen);
+void aero_hip_batch_destroy(aero_hip_batch* batch);
+/* Runs up to max_iters iterations (fixed_iters = 1: ignore the convergence test); the host synchronises
+ * every iters_per_sync iterations; use_graph = 1 replays captured hipGraphs.  results: [n_cases]. */
+int aero_hip_batch_run(aero_hip_batch* batch, int32_t max_iters, double rho_drop, int32_t fixed_iters,
+                       int32_t iters_per_sync, int32_t use_graph, aero_batch_result* results, double* seconds,
+                       char* err, size_t errlen);
+int aero_hip_batch_get_state(aero_hip_batch* batch, int32_t case_index, double* conserved /* [N][5] */, char* err,
+                             size_t errlen);
+int aero_hip_batch_history(aero_hip_batch* batch, int32_t case_index, double* hist /* [cap][4] */, int32_t cap,
+                           char* err, size_t errlen);
+
+/* ------------------------------------------------------------------------------------------------
  * Mesh generation (host side).  The NACA-4 O-grid of make_airfoil_ogrid_mesh
  * (mesh/airfoil_mesh.cpp:395-544) with the reference's triangles, plus quad and mixed
  * (quad wall block + triangles) cells, seeded interior-node jitter and RCM cell renumbering.

@@ -1,0 +1,98 @@
+"""Batched solves (many cases in one launch per RK stage, device-side control, hipGraph replay) against
+solo runs of the same cases: at a fixed iteration count every case's state and residual history is
+bitwise identical to its solo run, and graph replay equals eager launches."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import aero_hip
+    assert aero_hip.hip_available()
+    return aero_hip
+
+
+BASE = dict(flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=1.1, cfl_ramp_iters=40, startup_first_order_iters=5,
+            residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, low_mach_fix=1,
+            precond_on=1, precond_experimental_enabled=0)
+
+
+def solo(hip, mesh, mach, aoa, iters):
+    case = hip.CaseConfig(**dict(BASE, iterations=iters, mach=mach, aoa_deg=aoa))
+    with hip.HipMesh(mesh) as hm:
+        out = hm.run_case(case)
+    return out
+
+
+def test_batch_matches_solo_runs(hip):
+    m1 = helpers.mesh("40x8")
+    m2 = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    cases = [(0, 0.3, 1.0), (1, 0.5, 2.5), (0, 0.6, -2.0), (1, 0.2, 4.0)]
+    iters = 30
+    g1, g2 = hip.HipMesh(m1), hip.HipMesh(m2)
+    say("[batch] fixed-iteration batch of 4 cases on 2 geometries")
+    with hip.HipBatch([g1, g2], cases, hip.CaseConfig(**BASE), hist_capacity=iters) as b:
+        r = b.run(iters, fixed_iters=True, iters_per_sync=7, use_graph=True)
+        states = [b.state(i) for i in range(len(cases))]
+        hists = [b.history(i) for i in range(len(cases))]
+    for i, (gi, mach, aoa) in enumerate(cases):
+        say(f"[batch] solo case {i}")
+        ref = solo(hip, [m1, m2][gi], mach, aoa, iters)
+        assert r["cases"][i]["iterations"] == iters
+        assert np.array_equal(states[i], ref["conserved"]), f"case {i} state differs from its solo run"
+        assert np.array_equal(hists[i][:, 0], ref["history"][:, 2]), f"case {i} L2 history differs"
+        assert np.array_equal(hists[i][:, 1], ref["history"][:, 8]), f"case {i} density L2 history differs"
+        assert r["cases"][i]["cl"] == pytest.approx(ref["cl"], rel=1e-12, abs=1e-14)
+    g1.close()
+    g2.close()
+
+
+def test_graph_replay_equals_eager(hip):
+    m = helpers.mesh("64x14")
+    cases = [(0, 0.4, 1.0), (0, 0.7, 3.0)]
+    g = hip.HipMesh(m)
+    out = []
+    for use_graph, sync in ((True, 10), (False, 1), (True, 3)):
+        say(f"[batch] graph={use_graph} sync={sync}")
+        with hip.HipBatch([g], cases, hip.CaseConfig(**BASE)) as b:
+            b.run(25, fixed_iters=True, iters_per_sync=sync, use_graph=use_graph)
+            out.append([b.state(i) for i in range(len(cases))])
+    for o in out[1:]:
+        for a, bb in zip(o, out[0]):
+            assert np.array_equal(a, bb)
+    g.close()
+
+
+def test_converged_cases_are_masked(hip):
+    """A case that reaches its drop stops iterating while the others continue; its state is frozen."""
+    m = helpers.mesh("64x14")
+    g = hip.HipMesh(m)
+    first_order = dict(BASE, startup_first_order_iters=100000)
+    # the first case converges (first order, M 0.3) long before the second (M 0.6) is allowed to stop
+    cases = [(0, 0.3, 0.0), (0, 0.6, 3.0)]
+    cap = 1500
+    with hip.HipBatch([g], cases, hip.CaseConfig(**first_order), hist_capacity=cap) as b:
+        r = b.run(cap, rho_drop=1e-2, iters_per_sync=20)
+        c0, c1 = r["cases"]
+        assert c0["converged"] and c0["rho_ratio"] <= 1e-2 and c0["iterations"] < cap
+        s0 = b.state(0)
+        h = b.history(0)
+        n = c0["iterations"]
+        assert h[n - 1, 1] / h[0, 1] <= 1e-2 < h[n - 2, 1] / h[0, 1]
+        assert c1["iterations"] > n or c1["converged"]
+    case = hip.CaseConfig(**dict(first_order, iterations=n, mach=0.3, aoa_deg=0.0))
+    with hip.HipMesh(m) as hm:
+        ref = hm.run_case(case)
+    assert np.array_equal(s0, ref["conserved"]), "a masked case must keep its state from the iteration it converged"
+    g.close()
