@@ -130,12 +130,17 @@ def hip_available() -> bool:
 class HipMesh:
     """A mesh resident on the GPU with its state buffers and stream."""
 
-    def __init__(self, mesh: Mesh, renumber: bool = True, block_cells: int = 256):
+    GRADIENTS = {"green_gauss": 0, "least_squares": 1}
+
+    def __init__(self, mesh: Mesh, renumber: bool = True, block_cells: int = 256, gradient: str = "green_gauss"):
         self.lib = _lib.lib()
         self.mesh = mesh
         self.N, self.F = mesh.num_cells, mesh.num_faces
         self._desc = mesh.desc()
-        opts = _lib.Options(1 if renumber else 0, block_cells, 0)
+        if gradient not in self.GRADIENTS:
+            raise ValueError(f"gradient must be one of {sorted(self.GRADIENTS)}")
+        self.gradient = gradient
+        opts = _lib.Options(1 if renumber else 0, block_cells, self.GRADIENTS[gradient])
         h = C.c_void_p()
         err = _lib.errbuf()
         _lib.check(self.lib.aero_hip_mesh_create(C.byref(self._desc), C.byref(opts), C.byref(h), err, len(err)),

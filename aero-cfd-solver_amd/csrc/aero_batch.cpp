@@ -23,7 +23,7 @@ using aero_dev::DevMesh;
 using aero_dev::FaceOpts;
 
 struct aero_hip_batch {
-  int ncases = 0, ngeom = 0, BC = 256, smax = 1, nwork = 0;
+  int ncases = 0, ngeom = 0, BC = 256, grad = 0, smax = 1, nwork = 0;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   std::vector<aero_hip_mesh*> geoms;
@@ -103,7 +103,7 @@ int enqueue_iteration(aero_hip_batch* b, bool second, int start, char* err, size
     else
       bs.out = 1 - start;
     if (second)
-      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
+      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
     AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, b->d_geoms, b->d_cases, b->d_ctl,
@@ -143,8 +143,8 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
   }
   const int BC = meshes[0]->BC;
   for (int g = 0; g < n_meshes; ++g)
-    if (!meshes[g] || meshes[g]->BC != BC) {
-      set_err(err, errlen, "aero_hip_batch_create: all meshes need the same block_cells");
+    if (!meshes[g] || meshes[g]->BC != BC || meshes[g]->dm.grad != meshes[0]->dm.grad) {
+      set_err(err, errlen, "aero_hip_batch_create: all meshes need the same block_cells and gradient");
       return AERO_EINVAL;
     }
   for (int i = 0; i < n_cases; ++i)
@@ -156,6 +156,7 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
   b->ncases = n_cases;
   b->ngeom = n_meshes;
   b->BC = BC;
+  b->grad = meshes[0]->dm.grad;
   b->cfg = *c;
   b->spec.assign(cases, cases + n_cases);
   b->geoms.assign(meshes, meshes + n_meshes);

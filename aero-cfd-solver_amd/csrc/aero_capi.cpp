@@ -234,6 +234,10 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
     return AERO_EINVAL;
   }
   const bool renumber = opts ? opts->renumber != 0 : true;
+  if (opts && opts->gradient != AERO_GRAD_GREEN_GAUSS && opts->gradient != AERO_GRAD_LEAST_SQUARES) {
+    set_err(err, errlen, "aero_hip_mesh_create: gradient must be AERO_GRAD_GREEN_GAUSS or AERO_GRAD_LEAST_SQUARES");
+    return AERO_EINVAL;
+  }
   char why[256] = {0};
   if (!aero_hip_available(why, sizeof(why))) {
     set_err(err, errlen, std::string("aero_hip_mesh_create: no gfx950 device: ") + why);
@@ -342,7 +346,9 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
     }
   }
   m->S = slot_off[m->nblk];
-  if (static_cast<size_t>(smax) * 6 * sizeof(double) > 64 * 1024) {
+  const int grad = opts ? opts->gradient : AERO_GRAD_GREEN_GAUSS;
+  const size_t lds_per_slot = grad == AERO_GRAD_LEAST_SQUARES ? 8 : 6;  // the gradient kernels' staging
+  if (static_cast<size_t>(smax) * lds_per_slot * sizeof(double) > 64 * 1024) {
     delete m;
     set_err(err, errlen, "aero_hip_mesh_create: a block touches " + std::to_string(smax) +
                            " faces; use renumber=1 or a smaller block_cells");
@@ -425,6 +431,7 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   dm.BC = BC;
   dm.smax = std::max(smax, 1);
   dm.xcd_remap = renumber ? 1 : 0;
+  dm.grad = grad;
   dm.grid = dm.xcd_remap ? ((m->nblk + 7) / 8) * 8 : m->nblk;
   dm.slot_off = p_slot_off;
   dm.s_o = p_so;

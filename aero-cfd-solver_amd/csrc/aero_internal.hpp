@@ -22,6 +22,7 @@ struct DevMesh {
   int smax;      // max slots of any block (LDS sizing)
   int grid;      // workgroups launched (>= nblk when XCD-remapped)
   int xcd_remap;
+  int grad;      // AERO_GRAD_*: 0 Green-Gauss, 1 weighted least squares
   const int* slot_off;  // [nblk + 1]
   const int* s_o;       // [S] owner (internal id)
   const int* s_n;       // [S] neighbour (internal id) or -1
@@ -121,7 +122,7 @@ hipError_t launch_global_update(const aero_dev::DevMesh& m, const aero_dev::Face
 hipError_t launch_norm_partials(const aero_dev::DevMesh& m, const double* R, double* partials, hipStream_t s);
 hipError_t launch_iter_reduce(const aero_dev::DevMesh& m, const double* partials, const aero_dev::WallArgs& wa,
                               const double* U, double g, double* out, double* cp, hipStream_t s);
-hipError_t launch_batch_gradient(int BC, const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
+hipError_t launch_batch_gradient(int BC, int grad, const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
                                  const aero_dev::CaseCtl* ctl, const int2* work, int nwork, int smax, double g, int cur,
                                  hipStream_t s);
 hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, const aero_dev::DevMesh* geoms,

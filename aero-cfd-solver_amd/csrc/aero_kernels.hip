@@ -39,10 +39,96 @@ __device__ __forceinline__ int block_of(const DevMesh& m) {
   return (i & 7) * per + (i >> 3);
 }
 
+// Weighted least-squares gradient, per block (no reference function; the reference roadmap lists it,
+// docs/project_outline.md:49).  Per interior face, with d = x_neighbour - x_owner, w = 1 / (d.d) and
+// dq = q(neighbour) - q(owner), both cells add (w dx) dx, (w dx) dy, (w dy) dy, (w dx) dq, (w dy) dq in
+// increasing reference face index; the 2x2 normal equations are solved by Cramer's rule, and a cell
+// whose det = a c - b b is not above 1e-12 (a + c)^2 gets a zero gradient.  Boundary slots stage zeros,
+// which leave the sums unchanged.
+template <int BC>
+__device__ __forceinline__ void lsq_gradient_body(const DevMesh& m, int b, const double* __restrict__ U,
+                                                  double* __restrict__ G, double g, double* lds) {
+  const int s0 = m.slot_off[b];
+  const int ns = m.slot_off[b + 1] - s0;
+  const int smax = m.smax;
+  double* swx = lds;
+  double* swy = lds + smax;
+  double* sdx = lds + 2 * smax;
+  double* sdy = lds + 3 * smax;
+  double* q0 = lds + 4 * smax;
+  double* q1 = lds + 5 * smax;
+  double* q2 = lds + 6 * smax;
+  double* q3 = lds + 7 * smax;
+  const int ld = m.ld;
+  for (int i = threadIdx.x; i < ns; i += BC) {
+    const int s = s0 + i;
+    const int o = m.s_o[s];
+    const int n = m.s_n[s];
+    double wdx = 0.0, wdy = 0.0, dx = 0.0, dy = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (n >= 0) {
+      const P4 wo = c2p(load_u(U, ld, o), g);
+      const P4 wn = c2p(load_u(U, ld, n), g);
+      dx = m.ccx[n] - m.ccx[o];
+      dy = m.ccy[n] - m.ccy[o];
+      const double w = 1.0 / (dx * dx + dy * dy);
+      wdx = w * dx;
+      wdy = w * dy;
+      d0 = wn.r - wo.r;
+      d1 = wn.u - wo.u;
+      d2 = wn.v - wo.v;
+      d3 = wn.p - wo.p;
+    }
+    swx[i] = wdx;
+    swy[i] = wdy;
+    sdx[i] = dx;
+    sdy[i] = dy;
+    q0[i] = d0;
+    q1[i] = d1;
+    q2[i] = d2;
+    q3[i] = d3;
+  }
+  __syncthreads();
+  const int c = b * BC + threadIdx.x;
+  if (c >= m.N) return;
+  double a = 0.0, bb = 0.0, cc = 0.0;
+  double rx0 = 0.0, ry0 = 0.0, rx1 = 0.0, ry1 = 0.0, rx2 = 0.0, ry2 = 0.0, rx3 = 0.0, ry3 = 0.0;
+  const int e1 = m.c_off[c + 1];
+  for (int e = m.c_off[c]; e < e1; ++e) {
+    const int l = m.c_ent[e] >> 1;
+    const double wdx = swx[l], wdy = swy[l], dx = sdx[l], dy = sdy[l];
+    a += wdx * dx;
+    bb += wdx * dy;
+    cc += wdy * dy;
+    const double v0 = q0[l], v1 = q1[l], v2 = q2[l], v3 = q3[l];
+    rx0 += wdx * v0; ry0 += wdy * v0;
+    rx1 += wdx * v1; ry1 += wdy * v1;
+    rx2 += wdx * v2; ry2 += wdy * v2;
+    rx3 += wdx * v3; ry3 += wdy * v3;
+  }
+  const double det = a * cc - bb * bb;
+  const double tr = a + cc;
+  double gx0 = 0.0, gy0 = 0.0, gx1 = 0.0, gy1 = 0.0, gx2 = 0.0, gy2 = 0.0, gx3 = 0.0, gy3 = 0.0;
+  if (det > 1.0e-12 * (tr * tr)) {
+    const double inv = 1.0 / det;
+    gx0 = (cc * rx0 - bb * ry0) * inv; gy0 = (a * ry0 - bb * rx0) * inv;
+    gx1 = (cc * rx1 - bb * ry1) * inv; gy1 = (a * ry1 - bb * rx1) * inv;
+    gx2 = (cc * rx2 - bb * ry2) * inv; gy2 = (a * ry2 - bb * rx2) * inv;
+    gx3 = (cc * rx3 - bb * ry3) * inv; gy3 = (a * ry3 - bb * rx3) * inv;
+  }
+  G[0 * ld + c] = gx0;
+  G[1 * ld + c] = gy0;
+  G[2 * ld + c] = gx1;
+  G[3 * ld + c] = gy1;
+  G[4 * ld + c] = gx2;
+  G[5 * ld + c] = gy2;
+  G[6 * ld + c] = gx3;
+  G[7 * ld + c] = gy3;
+}
+
 // Green-Gauss gradient (reconstruction.cpp:73-119), per block.
 template <int BC>
-__device__ __forceinline__ void gradient_body(const DevMesh& m, int b, const double* __restrict__ U,
-                                              double* __restrict__ G, double g, double* lds) {
+__device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const double* __restrict__ U,
+                                                 double* __restrict__ G, double g, double* lds) {
   const int s0 = m.slot_off[b];
   const int ns = m.slot_off[b + 1] - s0;
   const int smax = m.smax;
@@ -107,13 +193,25 @@ __device__ __forceinline__ void gradient_body(const DevMesh& m, int b, const dou
   G[7 * ld + c] = gy3 * iv;
 }
 
-template <int BC>
+template <int BC, int GK>
+__device__ __forceinline__ void gradient_body(const DevMesh& m, int b, const double* __restrict__ U,
+                                              double* __restrict__ G, double g, double* lds) {
+  if (GK == 1)
+    lsq_gradient_body<BC>(m, b, U, G, g, lds);
+  else
+    gg_gradient_body<BC>(m, b, U, G, g, lds);
+}
+
+// LDS doubles per face slot of the gradient kernels.
+constexpr int grad_lds_per_slot(int gk) { return gk == 1 ? 8 : 6; }
+
+template <int BC, int GK>
 __global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
                                                   double g) {
   extern __shared__ double lds[];
   const int b = block_of(m);
   if (b >= m.nblk) return;
-  gradient_body<BC>(m, b, U, G, g, lds);
+  gradient_body<BC, GK>(m, b, U, G, g, lds);
 }
 
 // MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
@@ -569,7 +667,7 @@ __global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __r
 }
 
 // ---- batched kernels: one workgroup per (case, block) work item --------------------------------
-template <int BC>
+template <int BC, int GK>
 __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict__ geoms,
                                                         const BatchCase* __restrict__ cases,
                                                         const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
@@ -578,7 +676,7 @@ __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict
   const int2 w = work[blockIdx.x];
   if (w.x < 0 || !ctl[w.x].active) return;
   const BatchCase& cs = cases[w.x];
-  gradient_body<BC>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, lds);
+  gradient_body<BC, GK>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, lds);
 }
 
 template <int BC, int SCHEME, int LIMITER, bool SECOND>
@@ -710,8 +808,11 @@ using namespace aero_dev;
 
 template <int BC>
 static hipError_t launch_gradient_bc(const DevMesh& m, const double* U, double* G, double g, hipStream_t s) {
-  const size_t lds = static_cast<size_t>(m.smax) * 6 * sizeof(double);
-  hipLaunchKernelGGL(k_gradient<BC>, dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
+  const size_t lds = static_cast<size_t>(m.smax) * grad_lds_per_slot(m.grad) * sizeof(double);
+  if (m.grad == 1)
+    hipLaunchKernelGGL((k_gradient<BC, 1>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
+  else
+    hipLaunchKernelGGL((k_gradient<BC, 0>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
   return hipGetLastError();
 }
 
@@ -801,16 +902,23 @@ hipError_t launch_norm_partials(const DevMesh& m, const double* R, double* parti
   return hipGetLastError();
 }
 
-hipError_t launch_batch_gradient(int BC, const DevMesh* geoms, const BatchCase* cases, const CaseCtl* ctl,
-                                 const int2* work, int nwork, int smax, double g, int cur, hipStream_t s) {
-  const size_t lds = static_cast<size_t>(smax) * 6 * sizeof(double);
+template <int GK>
+static hipError_t launch_batch_gradient_gk(int BC, const DevMesh* geoms, const BatchCase* cases, const CaseCtl* ctl,
+                                           const int2* work, int nwork, int smax, double g, int cur, hipStream_t s) {
+  const size_t lds = static_cast<size_t>(smax) * grad_lds_per_slot(GK) * sizeof(double);
   switch (BC) {
-    case 128: hipLaunchKernelGGL(k_gradient_batch<128>, dim3(nwork), dim3(128), lds, s, geoms, cases, ctl, work, g, cur); break;
-    case 256: hipLaunchKernelGGL(k_gradient_batch<256>, dim3(nwork), dim3(256), lds, s, geoms, cases, ctl, work, g, cur); break;
-    case 512: hipLaunchKernelGGL(k_gradient_batch<512>, dim3(nwork), dim3(512), lds, s, geoms, cases, ctl, work, g, cur); break;
+    case 128: hipLaunchKernelGGL((k_gradient_batch<128, GK>), dim3(nwork), dim3(128), lds, s, geoms, cases, ctl, work, g, cur); break;
+    case 256: hipLaunchKernelGGL((k_gradient_batch<256, GK>), dim3(nwork), dim3(256), lds, s, geoms, cases, ctl, work, g, cur); break;
+    case 512: hipLaunchKernelGGL((k_gradient_batch<512, GK>), dim3(nwork), dim3(512), lds, s, geoms, cases, ctl, work, g, cur); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_batch_gradient(int BC, int grad, const DevMesh* geoms, const BatchCase* cases, const CaseCtl* ctl,
+                                 const int2* work, int nwork, int smax, double g, int cur, hipStream_t s) {
+  if (grad == 1) return launch_batch_gradient_gk<1>(BC, geoms, cases, ctl, work, nwork, smax, g, cur, s);
+  return launch_batch_gradient_gk<0>(BC, geoms, cases, ctl, work, nwork, smax, g, cur, s);
 }
 
 template <int BC>

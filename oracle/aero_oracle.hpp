@@ -332,6 +332,7 @@ template <class R> struct MeshView {
   const R* volume = nullptr;           // N
   const R* cell_center = nullptr;      // 3N
   int wall_start = 0, wall_count = 0;  // the "wall" patch range
+  int gradient = 0;                    // 0: Green-Gauss, 1: weighted least squares
 };
 
 template <class R> A3<R> normal_of(const MeshView<R>& m, int f) {  // mesh_geometry.cpp:33-39
@@ -387,6 +388,67 @@ void green_gauss(const MeshView<R>& m, const std::vector<Prim<R>>& prim, std::ve
     for (int k = 0; k < 5; ++k)
       for (int d = 0; d < nc; ++d) G(c, k, d) *= iv;
   }
+}
+
+// Weighted least-squares primitive gradients, layout [cell][var 5][comp 2].  The reference has no such
+// function (its roadmap lists one, docs/project_outline.md:49); this is the definition both backends
+// follow.  For every interior face in increasing face index, with d = x_neighbour - x_owner,
+// w = 1 / (d.d) and dq_k = q_k(neighbour) - q_k(owner), both adjacent cells add
+//   a += (w dx) dx,  b += (w dx) dy,  c += (w dy) dy,  r_kx += (w dx) dq_k,  r_ky += (w dy) dq_k
+// (the terms are the same seen from either side: d and dq both change sign).  Boundary faces add
+// nothing.  The 2x2 normal equations are solved by Cramer's rule; when det = a c - b b is not above
+// 1e-12 (a + c)^2 (collinear neighbours) the cell's gradient is zero.  Exact for linear fields.
+template <class R>
+void least_squares(const MeshView<R>& m, const std::vector<Prim<R>>& prim, std::vector<R>* grad) {
+  const int nc = 2;
+  const std::size_t N = static_cast<std::size_t>(m.num_cells);
+  std::vector<R> acc(N * 13, R(0));  // a, b, c, then r_kx, r_ky for k = 0..4
+  for (int f = 0; f < m.num_faces; ++f) {
+    const int o = m.owner[f];
+    const int nb = m.neighbor[f];
+    if (nb < 0) continue;
+    const R dx = m.cell_center[3 * nb + 0] - m.cell_center[3 * o + 0];
+    const R dy = m.cell_center[3 * nb + 1] - m.cell_center[3 * o + 1];
+    const R w = AO_K(1.0) / (dx * dx + dy * dy);
+    const R wdx = w * dx;
+    const R wdy = w * dy;
+    R t[13];
+    t[0] = wdx * dx;
+    t[1] = wdx * dy;
+    t[2] = wdy * dy;
+    for (int k = 0; k < 5; ++k) {
+      const R dq = prim_var(prim[nb], k) - prim_var(prim[o], k);
+      t[3 + 2 * k] = wdx * dq;
+      t[4 + 2 * k] = wdy * dq;
+    }
+    for (int j = 0; j < 13; ++j) {
+      acc[static_cast<std::size_t>(o) * 13 + j] += t[j];
+      acc[static_cast<std::size_t>(nb) * 13 + j] += t[j];
+    }
+  }
+  grad->assign(N * 5 * nc, R(0));
+  for (std::size_t c = 0; c < N; ++c) {
+    const R* q = &acc[c * 13];
+    const R a = q[0], b = q[1], cc = q[2];
+    const R det = a * cc - b * b;
+    const R tr = a + cc;
+    if (!(det > AO_K(1.0e-12) * (tr * tr))) continue;
+    const R inv = AO_K(1.0) / det;
+    for (int k = 0; k < 5; ++k) {
+      const R rx = q[3 + 2 * k], ry = q[4 + 2 * k];
+      (*grad)[(c * 5 + k) * nc + 0] = (cc * rx - b * ry) * inv;
+      (*grad)[(c * 5 + k) * nc + 1] = (a * ry - b * rx) * inv;
+    }
+  }
+}
+
+// The cell gradients the mesh asks for.
+template <class R>
+void cell_gradients(const MeshView<R>& m, const std::vector<Prim<R>>& prim, std::vector<R>* grad) {
+  if (m.gradient == 1)
+    least_squares(m, prim, grad);
+  else
+    green_gauss(m, prim, grad);
 }
 
 // MUSCL face states (reconstruction.cpp:148-219).

@@ -231,7 +231,7 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
         prim[i] = cons_to_prim(s, g);
       }
       if (second)
-        green_gauss(m, prim, &grad);
+        cell_gradients(m, prim, &grad);
       else
         grad.assign(static_cast<std::size_t>(N) * 10, R(0));
       AssemblyCfg<R> ac;

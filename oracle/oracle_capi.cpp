@@ -25,6 +25,7 @@ typedef struct ao_mesh {
   const void* volume;       // [N]
   const void* cell_center;  // [3N]
   int wall_start, wall_count;
+  int gradient;  // 0: Green-Gauss, 1: weighted least squares
 } ao_mesh;
 
 // EulerResidualAssemblyConfig scalars (residual_assembly.hpp:12-31), carried as double.
@@ -91,6 +92,7 @@ template <class R> MeshView<R> view(const ao_mesh* m) {
   v.cell_center = static_cast<const R*>(m->cell_center);
   v.wall_start = m->wall_start;
   v.wall_count = m->wall_count;
+  v.gradient = m->gradient;
   return v;
 }
 
@@ -228,7 +230,7 @@ template <class R> std::vector<Prim<R>> prims(int n, const R* U, R g) {
 template <class R> void gradients_t(const ao_mesh* m, const R* U, R g, R* grad) {
   const MeshView<R> v = view<R>(m);
   std::vector<R> gr;
-  green_gauss(v, prims(v.num_cells, U, g), &gr);
+  cell_gradients(v, prims(v.num_cells, U, g), &gr);
   std::memcpy(grad, gr.data(), gr.size() * sizeof(R));
 }
 
@@ -241,7 +243,7 @@ void residual_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, R* res, 
   const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
   std::vector<R> gr;
   if (a.second_order)
-    green_gauss(v, p, &gr);
+    cell_gradients(v, p, &gr);
   else
     gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
   std::vector<R> r, l;
@@ -263,7 +265,7 @@ template <class R> void flux_abs_sum_t(const ao_mesh* m, const ao_residual_cfg* 
   const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
   std::vector<R> gr, r, l, s;
   if (a.second_order)
-    green_gauss(v, p, &gr);
+    cell_gradients(v, p, &gr);
   else
     gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
   assemble_residual(v, a, p, gr, &r, &l, nullptr, &s);
@@ -351,7 +353,7 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
   for (int it = 0; it < n_evals; ++it) {
     const std::vector<Prim<R>> p = prims(v.num_cells, U, a.gamma);
     if (a.second_order)
-      green_gauss(v, p, &gr);
+      cell_gradients(v, p, &gr);
     else
       gr.assign(static_cast<std::size_t>(v.num_cells) * 10, R(0));
     assemble_residual(v, a, p, gr, &r, &l, &d);

@@ -27,7 +27,8 @@ HIST_FIELDS = 12
 class AoMesh(C.Structure):
     _fields_ = [("num_cells", i32), ("num_faces", i32), ("owner", C.POINTER(i32)), ("neighbor", C.POINTER(i32)),
                 ("bc", C.POINTER(i32)), ("normal", C.c_void_p), ("face_center", C.c_void_p), ("area", C.c_void_p),
-                ("volume", C.c_void_p), ("cell_center", C.c_void_p), ("wall_start", i32), ("wall_count", i32)]
+                ("volume", C.c_void_p), ("cell_center", C.c_void_p), ("wall_start", i32), ("wall_count", i32),
+                ("gradient", i32)]
 
 
 # The oracle's ao_residual_cfg / ao_case_cfg share the C ABI's aero_residual_config / aero_case_config
@@ -76,7 +77,8 @@ def suffix(dtype) -> str:
 class OracleMesh:
     """Keeps numpy arrays alive behind an ao_mesh view."""
 
-    def __init__(self, mesh, dtype):
+    def __init__(self, mesh, dtype, gradient: int = 0):
+        """gradient: 0 Green-Gauss, 1 weighted least squares."""
         self.dtype = np.dtype(dtype)
         self.owner = np.ascontiguousarray(mesh.owner, np.int32)
         self.neighbor = np.ascontiguousarray(mesh.neighbor, np.int32)
@@ -91,7 +93,7 @@ class OracleMesh:
         P = lambda a: a.ctypes.data_as(C.POINTER(i32))
         self.c = AoMesh(self.N, self.F, P(self.owner), P(self.neighbor), P(self.bc), ptr(self.normal),
                         ptr(self.face_center), ptr(self.area), ptr(self.volume), ptr(self.cell_center),
-                        self.wall_start, self.wall_count)
+                        self.wall_start, self.wall_count, gradient)
 
 
 def residual(om: OracleMesh, cfg, U):

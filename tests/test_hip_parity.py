@@ -173,3 +173,45 @@ def test_run_case_parity(hip, name):
     for k in (5, 6, 7):
         assert abs(out["forces"][k] - ref["forces"][k]) <= 1e-9 * abs(ref["forces"][k]) + 1e-12
     assert np.allclose(out["wall_cp"], ref["cp"], rtol=1e-9, atol=1e-12)
+
+
+def test_lsq_gradients_and_residual_parity(hip):
+    """Weighted least-squares gradients: device gradients and second-order residuals against the oracle's
+    least-squares path on a jittered mixed mesh, for both block sizes."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.15, seed=21, rcm=True)
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    ff = ff64(0.5, 1.25)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.1, 5)
+    Go = ol.gradients(om, U)
+    for bc in (256, 128):
+        say(f"[lsq] gradients + residual block={bc}")
+        with hip.HipMesh(m, block_cells=bc, gradient="least_squares") as hm:
+            hm.residual(U, helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff)))
+            Gh = hm.gradients()
+            scale = np.abs(Go).max(axis=0, keepdims=True) + 1e-300
+            assert (np.abs(Gh - Go) / scale).max() <= TOL
+            for combo in [(1, 1, 1, 0, 0, 0, 0), (0, 0, 1, 0, 0, 0, 0), (1, 1, 1, 1, 1, 1, 1)]:
+                compare(hm, om, U, combo, ff, f"lsq block={bc}")
+
+
+def test_lsq_run_case_parity(hip):
+    """A full driver run with least-squares gradients against the oracle's driver."""
+    run = cases.RUNS["regression"]
+    m = helpers.mesh(run["mesh"])
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    over = dict(run["cfg"], iterations=300)
+    cfg = helpers.case_cfg(over, run["env_precond"])
+    say("[lsq] run_case regression 300 iterations")
+    ref = ol.run_case(om, cfg, 300)
+    from aero_hip import CaseConfig
+    case = CaseConfig(**over)
+    case.precond_experimental_enabled = 1 if run["env_precond"] else 0
+    with hip.HipMesh(m, gradient="least_squares") as hm:
+        out = hm.run_case(case)
+    hr, ho = ref["history"], out["history"]
+    assert ho.shape == hr.shape
+    for col in (1, 2, 3, 5, 6, 7, 8):
+        rel = np.abs(ho[:, col] - hr[:, col]) / np.maximum(np.abs(hr[:, col]), 1e-300)
+        assert np.all((rel <= 1e-9) | (np.abs(ho[:, col] - hr[:, col]) <= 1e-12)), (col, rel.max())
+    sU = np.abs(ref["conserved"]).max(axis=0) + 1e-300
+    assert (np.abs(out["conserved"] - ref["conserved"]) / sU).max() <= 1e-9

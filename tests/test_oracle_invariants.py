@@ -207,3 +207,87 @@ def test_forces_vanish_for_freestream_pressure(oracle_lib):
     ol.oracle().ao_forces_f64(C.byref(om.c), ol.ptr(p), ol.ptr(ref7), 0, ol.ptr(o))
     # absolute pressure on a closed body: sum n A = 0 up to round-off
     assert abs(o[0]) < 1e-9 * 101325.0 and abs(o[1]) < 1e-9 * 101325.0
+
+
+# ---- weighted least-squares gradient (no reference function) -----------------------------------------
+def _prim_field_to_U(rho, u, v, p, g=1.4):
+    U = np.zeros((len(rho), 5))
+    U[:, 0] = rho
+    U[:, 1] = rho * u
+    U[:, 2] = rho * v
+    U[:, 4] = p / (g - 1.0) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def _lsq_mesh():
+    return helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.15, seed=21, rcm=True)
+
+
+def test_lsq_gradient_exact_for_linear_fields(oracle_lib):
+    """A linear primitive field has the exact gradient in every cell, boundary cells included (only
+    interior neighbours enter, and they determine a linear field)."""
+    m = _lsq_mesh()
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    x, y = m.cell_center[:, 0], m.cell_center[:, 1]
+    coef = np.array([[1.2, 0.03, -0.02], [0.5, 0.01, 0.04], [-0.1, -0.03, 0.02], [2.0, 0.05, -0.07]])
+    q = [c[0] + c[1] * x + c[2] * y for c in coef]
+    G = ol.gradients(om, _prim_field_to_U(*q))
+    for k, kk in enumerate((0, 1, 2, 4)):
+        assert np.abs(G[:, kk, 0] - coef[k, 1]).max() <= 1e-10 * (1 + abs(coef[k, 1]))
+        assert np.abs(G[:, kk, 1] - coef[k, 2]).max() <= 1e-10 * (1 + abs(coef[k, 2]))
+    assert np.all(G[:, 3, :] == 0.0)
+
+
+def test_lsq_gradient_matches_weighted_lstsq(oracle_lib):
+    """Per cell, the oracle equals numpy's least-squares solution of sqrt(w) d . g = sqrt(w) dq over the
+    cell's interior neighbours with w = 1 / |d|^2, for a non-linear field."""
+    m = _lsq_mesh()
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    from aero_hip import freestream
+    ff = freestream(0.5, 3.0)[1]
+    U = helpers.perturbed_state(m.cell_center, ff, 0.3, 77)
+    G = ol.gradients(om, U)
+    g = 1.4
+    rho = U[:, 0]
+    prim = np.stack([rho, U[:, 1] / rho, U[:, 2] / rho,
+                     (g - 1) * (U[:, 4] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / rho)], axis=1)
+    nbrs = [[] for _ in range(m.num_cells)]
+    for o, n in zip(m.owner, m.neighbor):
+        if n >= 0:
+            nbrs[o].append(n)
+            nbrs[n].append(o)
+    cc = m.cell_center[:, :2]
+    worst = 0.0
+    for c in range(m.num_cells):
+        d = cc[nbrs[c]] - cc[c]
+        sw = 1.0 / np.sqrt((d * d).sum(axis=1))
+        A = d * sw[:, None]
+        for k, kk in enumerate((0, 1, 2, 4)):
+            b = (prim[nbrs[c], k] - prim[c, k]) * sw
+            sol = np.linalg.lstsq(A, b, rcond=None)[0]
+            scale = np.abs(sol).max() + 1e-30
+            worst = max(worst, np.abs(G[c, kk] - sol).max() / scale)
+    assert worst <= 1e-9, worst
+
+
+def test_lsq_gradient_zero_for_constant_field(oracle_lib):
+    m = _lsq_mesh()
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    from aero_hip import freestream
+    U = np.tile(freestream(0.3, 2.0)[1], (m.num_cells, 1))
+    assert np.all(ol.gradients(om, U) == 0.0)
+
+
+def test_lsq_second_order_residual_preserves_freestream(oracle_lib):
+    m = _lsq_mesh()
+    om = ol.OracleMesh(m, np.float64, gradient=1)
+    from aero_hip import freestream
+    ff = freestream(0.5, 1.25)[1]
+    U = np.tile(ff, (m.num_cells, 1))
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff))
+    R, lam, d = ol.residual(om, cfg, U)
+    scale = helpers.flux_scale(om, cfg, U)
+    off_wall = np.ones(m.num_cells, bool)
+    off_wall[m.owner[m.wall_start: m.wall_start + m.wall_count]] = False
+    comp = [0, 1, 2, 4]
+    assert (np.abs(R[off_wall][:, comp]) / scale[off_wall][:, comp]).max() < 1e-13
