@@ -4,9 +4,10 @@ Importing the package loads libaero_hip.so (the HIP kernels + C ABI); a missing 
 """
 from . import _lib
 from .mesh import Mesh, generate_mesh
-from .solver import CaseConfig, HipBatch, HipMesh, freestream, hip_available, residual_config
+from .solver import (CaseConfig, HipBatch, HipMesh, Seed, case_linearization, freestream, hip_available,
+                     residual_config)
 
-__all__ = ["Mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "freestream", "hip_available", "residual_config",
-           "_lib"]
+__all__ = ["Mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "Seed", "case_linearization", "freestream",
+           "hip_available", "residual_config", "_lib"]
 
 _lib.lib()

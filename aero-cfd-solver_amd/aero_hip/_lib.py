@@ -108,6 +108,24 @@ class BatchResult(C.Structure):
 
 
 # Every symbol include/aero_hip.h declares, with its ctypes prototype.
+class TangentSeed(C.Structure):
+    _fields_ = [("d_farfield", f64 * 5), ("d_stab_floor", f64), ("d_mach_ref", f64), ("d_mach_min", f64),
+                ("d_beta_min", f64), ("d_beta_max", f64), ("d_face_normal", PD), ("d_face_center", PD),
+                ("d_face_area", PD), ("d_cell_center", PD), ("d_cell_volume", PD)]
+
+
+class Objective(C.Structure):
+    _fields_ = [("w_cl", f64), ("w_cd", f64), ("w_cm", f64), ("aoa_deg", f64), ("q_inf", f64), ("p_inf", f64),
+                ("chord", f64), ("x_ref", f64), ("y_ref", f64), ("gamma", f64)]
+
+
+class LinearSolve(C.Structure):
+    _fields_ = [("rk_stages", i32), ("cfl", f64), ("max_iters", i32), ("tol", f64), ("check_every", i32),
+                ("iterations", i32), ("res0", f64), ("res", f64), ("seconds", f64)]
+
+
+PARAMS = {"none": 0, "aoa": 1, "mach": 2}
+
 PROTOTYPES = {
     "aero_hip_available": (C.c_int, [C.c_char_p, C.c_size_t]),
     "aero_hip_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Options), C.POINTER(vp), C.c_char_p,
@@ -135,6 +153,17 @@ PROTOTYPES = {
     "aero_mesh_generate": (C.c_int, [C.POINTER(MeshGenConfig), i32, C.POINTER(MeshArrays), C.c_char_p,
                                      C.c_size_t]),
     "aero_mesh_free": (None, [C.POINTER(MeshArrays)]),
+    "aero_case_linearization": (C.c_int, [C.POINTER(CaseConfigC), i32, i32, C.POINTER(ResidualConfig),
+                                          C.POINTER(TangentSeed), C.POINTER(Objective), PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_residual_jvp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(TangentSeed), PD, PD,
+                                        C.c_char_p, C.c_size_t]),
+    "aero_hip_residual_vjp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_objective": (C.c_int, [vp, C.POINTER(Objective), PD, PD, PD, PD, PD, C.POINTER(TangentSeed), PD,
+                                     C.c_char_p, C.c_size_t]),
+    "aero_hip_adjoint_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(LinearSolve), PD, PD,
+                                         C.c_char_p, C.c_size_t]),
+    "aero_hip_tangent_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, C.POINTER(TangentSeed),
+                                         C.POINTER(LinearSolve), PD, PD, C.c_char_p, C.c_size_t]),
 }
 
 

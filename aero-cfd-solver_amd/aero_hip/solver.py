@@ -237,6 +237,109 @@ class HipMesh:
                     wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)
 
 
+    # ---- tangent / adjoint ------------------------------------------------------------------------
+    def _state_ptr(self, U):
+        if U is None:
+            return None, None
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.N, 5)
+        return U, _pd(U)
+
+    def residual_jvp(self, cfg: _lib.ResidualConfig, U=None, dU=None, seed: "Seed | None" = None):
+        """(R, dR) with dR = R_U dU + R_p . seed at U (None: the handle's current state)."""
+        _, pu = self._state_ptr(U)
+        dUa, pdu = self._state_ptr(dU)
+        R = np.zeros((self.N, 5))
+        dR = np.zeros((self.N, 5))
+        self._call("aero_hip_residual_jvp", C.byref(cfg), pu, pdu, seed.ptr() if seed else None, _pd(R), _pd(dR))
+        return R, dR
+
+    def residual_vjp(self, cfg: _lib.ResidualConfig, psi, U=None):
+        """R_U^T psi at U (None: the handle's current state)."""
+        _, pu = self._state_ptr(U)
+        psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(self.N, 5)
+        out = np.zeros((self.N, 5))
+        self._call("aero_hip_residual_vjp", C.byref(cfg), pu, _pd(psi), _pd(out))
+        return out
+
+    def objective(self, obj: _lib.Objective, U=None, seed: "Seed | None" = None, gradient: bool = True) -> dict:
+        """J = w_cl CL + w_cd CD + w_cm CM of the gauge wall pressure forces, dJ/dU and explicit partials."""
+        _, pu = self._state_ptr(U)
+        J, da, dq, ds = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        g = np.zeros((self.N, 5)) if gradient else None
+        self._call("aero_hip_objective", C.byref(obj), pu, C.byref(J), _pd(g) if gradient else None, C.byref(da),
+                   C.byref(dq), seed.ptr() if seed else None, C.byref(ds))
+        return dict(J=J.value, dJdU=g, dJ_daoa=da.value, dJ_dq=dq.value, dJ_seed=ds.value)
+
+    def adjoint_solve(self, cfg: _lib.ResidualConfig, rhs, U=None, cfl: float = 1.0, rk_stages: int = 3,
+                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100):
+        """psi with R_U^T psi = rhs (transposed pseudo-time iteration)."""
+        _, pu = self._state_ptr(U)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(self.N, 5)
+        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, 0, 0.0, 0.0, 0.0)
+        psi = np.zeros((self.N, 5))
+        hist = np.zeros(max_iters)
+        self._call("aero_hip_adjoint_solve", C.byref(cfg), pu, _pd(rhs), C.byref(ctl), _pd(psi), _pd(hist))
+        return psi, dict(iterations=ctl.iterations, res0=ctl.res0, res=ctl.res, seconds=ctl.seconds,
+                         history=hist[: ctl.iterations].copy())
+
+    def tangent_solve(self, cfg: _lib.ResidualConfig, seed: "Seed", U=None, cfl: float = 1.0, rk_stages: int = 3,
+                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100):
+        """v with R_U v = -R_p . seed (linearised pseudo-time iteration)."""
+        _, pu = self._state_ptr(U)
+        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, 0, 0.0, 0.0, 0.0)
+        v = np.zeros((self.N, 5))
+        hist = np.zeros(max_iters)
+        self._call("aero_hip_tangent_solve", C.byref(cfg), pu, seed.ptr(), C.byref(ctl), _pd(v), _pd(hist))
+        return v, dict(iterations=ctl.iterations, res0=ctl.res0, res=ctl.res, seconds=ctl.seconds,
+                       history=hist[: ctl.iterations].copy())
+
+
+class Seed:
+    """A parameter direction (aero_tangent_seed) that keeps its numpy arrays alive."""
+
+    def __init__(self, d_farfield=None, d_stab_floor=0.0, d_mach_ref=0.0, d_mach_min=0.0, d_beta_min=0.0,
+                 d_beta_max=0.0, d_face_normal=None, d_face_center=None, d_face_area=None, d_cell_center=None,
+                 d_cell_volume=None):
+        self._keep = {}
+
+        def arr(name, a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            self._keep[name] = a
+            return _pd(a)
+
+        ff = (C.c_double * 5)(*(list(d_farfield) if d_farfield is not None else [0.0] * 5))
+        self.c = _lib.TangentSeed(ff, d_stab_floor, d_mach_ref, d_mach_min, d_beta_min, d_beta_max,
+                                  arr("n", d_face_normal), arr("fc", d_face_center), arr("A", d_face_area),
+                                  arr("cc", d_cell_center), arr("V", d_cell_volume))
+
+    @classmethod
+    def from_c(cls, c: _lib.TangentSeed) -> "Seed":
+        s = cls(d_farfield=list(c.d_farfield), d_stab_floor=c.d_stab_floor, d_mach_ref=c.d_mach_ref,
+                d_mach_min=c.d_mach_min, d_beta_min=c.d_beta_min, d_beta_max=c.d_beta_max)
+        return s
+
+    def ptr(self):
+        return C.byref(self.c)
+
+
+def case_linearization(case: CaseConfig, iteration: int | None = None, param: str = "none"):
+    """(residual config at `iteration` (default: the last), parameter Seed, Objective reference values,
+    d q_inf / d param) for a case (aero_case_linearization)."""
+    lib = _lib.lib()
+    cc = case.to_c()
+    it = case.iterations - 1 if iteration is None else iteration
+    rc = _lib.ResidualConfig()
+    seed = _lib.TangentSeed()
+    obj = _lib.Objective()
+    dq = C.c_double()
+    err = _lib.errbuf()
+    _lib.check(lib.aero_case_linearization(C.byref(cc), it, _lib.PARAMS[param], C.byref(rc), C.byref(seed),
+                                           C.byref(obj), C.byref(dq), err, len(err)), err, "aero_case_linearization")
+    return rc, Seed.from_c(seed), obj, dq.value
+
+
 class HipBatch:
     """Batched solves over device geometries (HipMesh handles built with the same block_cells): one
     launch per RK stage over every (case, block), device-side iteration control, hipGraph replay."""

@@ -1,0 +1,654 @@
+// aero_adjoint.hip -- gfx950 kernels of the discrete tangent and adjoint of the residual.
+//
+// The reference has no derivative code; these kernels differentiate the residual the other kernels
+// compute (face_eval.hpp, instantiated on the dual-number type), at a fixed state.
+//
+//   tangent:  dR = R_U dU + R_p dp           k_tan_gradient -> k_tan_face -> k_tan_gather
+//   adjoint:  out = R_U^T psi                 k_adj_face -> k_adj_cell -> k_adj_grad, over the face
+//             Jacobians J_f = d(F_f A_f)/d(U_o, U_n, G_o, G_n) that k_face_jacobian stores once per state
+//
+// All sums run per cell over its faces in increasing reference face index (no atomics), so results are
+// deterministic.  Face-indexed arrays are SoA over the F faces in reference order; cell arrays are SoA
+// over the N cells in the device (internal) order.
+#include <hip/hip_runtime.h>
+
+#include "aero_internal.hpp"
+#include "face_eval.hpp"
+
+namespace aero_dev {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ Dual dual_at(const double* v, const double* d, int i) {
+  return Dual(v[i], d ? d[i] : 0.0);
+}
+
+__device__ __forceinline__ C4T<Dual> load_dual_u(const double* U, const double* dU, int ld, int c) {
+  C4T<Dual> s;
+  s.r = dual_at(U, dU, c);
+  s.ru = dual_at(U + ld, dU ? dU + ld : nullptr, c);
+  s.rv = dual_at(U + 2 * ld, dU ? dU + 2 * ld : nullptr, c);
+  s.E = dual_at(U + 3 * ld, dU ? dU + 3 * ld : nullptr, c);
+  return s;
+}
+
+__device__ __forceinline__ C4 load_c4(const double* U, int ld, int c) {
+  C4 s;
+  s.r = U[c];
+  s.ru = U[ld + c];
+  s.rv = U[2 * ld + c];
+  s.E = U[3 * ld + c];
+  return s;
+}
+
+// ---- tangent of the gradient ---------------------------------------------------------------------
+// Same arithmetic as the residual's gradient kernels (Green-Gauss: reconstruction.cpp:73-119; least
+// squares: the weighted normal equations), per cell over its faces in face order, on dual numbers.
+template <int GK>
+__global__ __launch_bounds__(kThreads) void k_tan_gradient(AdjMesh am, const double* __restrict__ U,
+                                                           const double* __restrict__ dU, GeoTan gt, double g,
+                                                           double* __restrict__ G, double* __restrict__ dG) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int ld = am.ld;
+  Dual acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = Dual(0.0);
+  const int e1 = am.cf_off[c + 1];
+  if (GK == 1) {
+    Dual a(0.0), bb(0.0), cc(0.0);
+    for (int e = am.cf_off[c]; e < e1; ++e) {
+      const int f = am.cf_ent[e] >> 1;
+      const int o = am.fo[f], n = am.fn[f];
+      if (n < 0) continue;
+      const P4T<Dual> wo = c2p(load_dual_u(U, dU, ld, o), g);
+      const P4T<Dual> wn = c2p(load_dual_u(U, dU, ld, n), g);
+      const Dual dx = dual_at(am.ccx, gt.dccx, n) - dual_at(am.ccx, gt.dccx, o);
+      const Dual dy = dual_at(am.ccy, gt.dccy, n) - dual_at(am.ccy, gt.dccy, o);
+      const Dual w = 1.0 / (dx * dx + dy * dy);
+      const Dual wdx = w * dx, wdy = w * dy;
+      a = a + wdx * dx;
+      bb = bb + wdx * dy;
+      cc = cc + wdy * dy;
+      const Dual d0 = wn.r - wo.r, d1 = wn.u - wo.u, d2 = wn.v - wo.v, d3 = wn.p - wo.p;
+      acc[0] = acc[0] + wdx * d0; acc[1] = acc[1] + wdy * d0;
+      acc[2] = acc[2] + wdx * d1; acc[3] = acc[3] + wdy * d1;
+      acc[4] = acc[4] + wdx * d2; acc[5] = acc[5] + wdy * d2;
+      acc[6] = acc[6] + wdx * d3; acc[7] = acc[7] + wdy * d3;
+    }
+    const Dual det = a * cc - bb * bb;
+    const Dual tr = a + cc;
+    Dual out[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] = Dual(0.0);
+    if (det > 1.0e-12 * (tr * tr)) {
+      const Dual inv = 1.0 / det;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        out[2 * k] = (cc * acc[2 * k] - bb * acc[2 * k + 1]) * inv;
+        out[2 * k + 1] = (a * acc[2 * k + 1] - bb * acc[2 * k]) * inv;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      G[j * ld + c] = out[j].v;
+      dG[j * ld + c] = out[j].d;
+    }
+    return;
+  }
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    const int o = am.fo[f], n = am.fn[f];
+    const P4T<Dual> wo = c2p(load_dual_u(U, dU, ld, o), g);
+    Dual q0 = wo.r, q1 = wo.u, q2 = wo.v, q3 = wo.p;
+    if (n >= 0) {
+      const P4T<Dual> wn = c2p(load_dual_u(U, dU, ld, n), g);
+      q0 = 0.5 * (wo.r + wn.r);
+      q1 = 0.5 * (wo.u + wn.u);
+      q2 = 0.5 * (wo.v + wn.v);
+      q3 = 0.5 * (wo.p + wn.p);
+    }
+    const Dual A = dual_at(am.fA, gt.dA, f);
+    const Dual x = dual_at(am.fnx, gt.dnx, f) * A;
+    const Dual y = dual_at(am.fny, gt.dny, f) * A;
+    if (ent & 1) {
+      acc[0] = acc[0] - q0 * x; acc[1] = acc[1] - q0 * y;
+      acc[2] = acc[2] - q1 * x; acc[3] = acc[3] - q1 * y;
+      acc[4] = acc[4] - q2 * x; acc[5] = acc[5] - q2 * y;
+      acc[6] = acc[6] - q3 * x; acc[7] = acc[7] - q3 * y;
+    } else {
+      acc[0] = acc[0] + q0 * x; acc[1] = acc[1] + q0 * y;
+      acc[2] = acc[2] + q1 * x; acc[3] = acc[3] + q1 * y;
+      acc[4] = acc[4] + q2 * x; acc[5] = acc[5] + q2 * y;
+      acc[6] = acc[6] + q3 * x; acc[7] = acc[7] + q3 * y;
+    }
+  }
+  const Dual iv = 1.0 / dmax(dual_at(am.vol, gt.dvol, c), Dual(1.0e-12));
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const Dual v = acc[j] * iv;
+    G[j * ld + c] = v.v;
+    dG[j * ld + c] = v.d;
+  }
+}
+
+// ---- tangent of the face fluxes ------------------------------------------------------------------
+// fa[k][F] (k = 0..3 values F_k A, 4..7 their derivatives).
+template <int SCHEME, int LIMITER, bool SECOND>
+__global__ __launch_bounds__(kThreads) void k_tan_face(AdjMesh am, const double* __restrict__ U,
+                                                       const double* __restrict__ dU, const double* __restrict__ G,
+                                                       const double* __restrict__ dG, GeoTan gt, FaceOptsT<Dual> o,
+                                                       double* __restrict__ fa) {
+  const int f = blockIdx.x * kThreads + threadIdx.x;
+  if (f >= am.F) return;
+  const int ld = am.ld;
+  const int co = am.fo[f], cn = am.fn[f];
+  const Dual nx = dual_at(am.fnx, gt.dnx, f);
+  const Dual ny = dual_at(am.fny, gt.dny, f);
+  const Dual A = dual_at(am.fA, gt.dA, f);
+  FluxT<Dual> Fl;
+  if (cn >= 0) {
+    Dual go[8], gn[8];
+    Dual dox(0.0), doy(0.0), dnx(0.0), dny(0.0);
+    if (SECOND) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        go[j] = dual_at(G + j * ld, dG ? dG + j * ld : nullptr, co);
+        gn[j] = dual_at(G + j * ld, dG ? dG + j * ld : nullptr, cn);
+      }
+      const Dual fx = dual_at(am.ffx, gt.dfx, f), fy = dual_at(am.ffy, gt.dfy, f);
+      dox = fx - dual_at(am.ccx, gt.dccx, co);
+      doy = fy - dual_at(am.ccy, gt.dccy, co);
+      dnx = fx - dual_at(am.ccx, gt.dccx, cn);
+      dny = fy - dual_at(am.ccy, gt.dccy, cn);
+    }
+    bool reverted;
+    Fl = interior_face<SCHEME, LIMITER, SECOND>(load_dual_u(U, dU, ld, co), load_dual_u(U, dU, ld, cn), go, gn, nx,
+                                                ny, dox, doy, dnx, dny, o, &reverted);
+  } else {
+    Fl = boundary_face<SCHEME>(load_dual_u(U, dU, ld, co), am.fbc[f] == 2, nx, ny, o);
+  }
+  const Dual f0 = Fl.f0 * A, f1 = Fl.f1 * A, f2 = Fl.f2 * A, f3 = Fl.f3 * A;
+  const int F = am.F;
+  fa[0 * F + f] = f0.v;
+  fa[1 * F + f] = f1.v;
+  fa[2 * F + f] = f2.v;
+  fa[3 * F + f] = f3.v;
+  fa[4 * F + f] = f0.d;
+  fa[5 * F + f] = f1.d;
+  fa[6 * F + f] = f2.d;
+  fa[7 * F + f] = f3.d;
+}
+
+// Per cell: R = sum_f +-F A in face order (residual_assembly.cpp:168-180), values and derivatives.
+__global__ __launch_bounds__(kThreads) void k_tan_gather(AdjMesh am, const double* __restrict__ fa,
+                                                         double* __restrict__ R, double* __restrict__ dR) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int F = am.F, ld = am.ld;
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, d[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = am.cf_off[c + 1];
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    if (ent & 1) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        r[k] -= fa[k * F + f];
+        d[k] -= fa[(4 + k) * F + f];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        r[k] += fa[k * F + f];
+        d[k] += fa[(4 + k) * F + f];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (R) R[k * ld + c] = r[k];
+    dR[k * ld + c] = d[k];
+  }
+}
+
+// ---- face Jacobians ----------------------------------------------------------------------------------
+// Thread (dir, f): column `dir` of d(F_f A_f)/d(U_o[0..3], U_n[0..3], G_o[0..7], G_n[0..7]), stored at
+// J[(dir * 4 + k) * F + f].  Boundary faces depend on U_o only; their other columns are zero.
+template <int SCHEME, int LIMITER, bool SECOND>
+__global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const double* __restrict__ U,
+                                                            const double* __restrict__ G, FaceOpts o0,
+                                                            double* __restrict__ J) {
+  const long long tid = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  const int F = am.F;
+  if (tid >= 24LL * F) return;
+  const int dir = static_cast<int>(tid / F);
+  const int f = static_cast<int>(tid - static_cast<long long>(dir) * F);
+  const int ld = am.ld;
+  const int co = am.fo[f], cn = am.fn[f];
+  FaceOptsT<Dual> o;
+  o.ff.r = o0.ff.r; o.ff.ru = o0.ff.ru; o.ff.rv = o0.ff.rv; o.ff.E = o0.ff.E;
+  o.g = o0.g;
+  o.low_mach_fix = o0.low_mach_fix;
+  o.low_mach_cutoff = o0.low_mach_cutoff;
+  o.stab_floor = o0.stab_floor;
+  o.char_bc = o0.char_bc;
+  o.precond = o0.precond;
+  o.mach_ref = o0.mach_ref;
+  o.mach_min = o0.mach_min;
+  o.beta_min = o0.beta_min;
+  o.beta_max = o0.beta_max;
+  o.as_fix = o0.as_fix;
+  o.as_cut = o0.as_cut;
+  o.as_fmin = o0.as_fmin;
+  o.as_w = o0.as_w;
+  const double A = am.fA[f];
+  const Dual nx(am.fnx[f]), ny(am.fny[f]);
+  double col[4] = {0.0, 0.0, 0.0, 0.0};
+  if (cn >= 0) {
+    C4T<Dual> uo = load_dual_u(U, nullptr, ld, co), un = load_dual_u(U, nullptr, ld, cn);
+    Dual go[8], gn[8];
+    Dual dox(0.0), doy(0.0), dnx(0.0), dny(0.0);
+    if (SECOND) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        go[j] = Dual(G[j * ld + co]);
+        gn[j] = Dual(G[j * ld + cn]);
+      }
+      const double fx = am.ffx[f], fy = am.ffy[f];
+      dox = Dual(fx - am.ccx[co]);
+      doy = Dual(fy - am.ccy[co]);
+      dnx = Dual(fx - am.ccx[cn]);
+      dny = Dual(fy - am.ccy[cn]);
+    }
+    bool active = true;
+    if (dir < 4) {
+      (dir == 0 ? uo.r : dir == 1 ? uo.ru : dir == 2 ? uo.rv : uo.E).d = 1.0;
+    } else if (dir < 8) {
+      (dir == 4 ? un.r : dir == 5 ? un.ru : dir == 6 ? un.rv : un.E).d = 1.0;
+    } else if (dir < 16) {
+      if (SECOND) go[dir - 8].d = 1.0; else active = false;
+    } else {
+      if (SECOND) gn[dir - 16].d = 1.0; else active = false;
+    }
+    if (active) {
+      bool reverted;
+      const FluxT<Dual> Fl =
+          interior_face<SCHEME, LIMITER, SECOND>(uo, un, go, gn, nx, ny, dox, doy, dnx, dny, o, &reverted);
+      col[0] = Fl.f0.d * A;
+      col[1] = Fl.f1.d * A;
+      col[2] = Fl.f2.d * A;
+      col[3] = Fl.f3.d * A;
+    }
+  } else if (dir < 4) {
+    C4T<Dual> uo = load_dual_u(U, nullptr, ld, co);
+    (dir == 0 ? uo.r : dir == 1 ? uo.ru : dir == 2 ? uo.rv : uo.E).d = 1.0;
+    const FluxT<Dual> Fl = boundary_face<SCHEME>(uo, am.fbc[f] == 2, nx, ny, o);
+    col[0] = Fl.f0.d * A;
+    col[1] = Fl.f1.d * A;
+    col[2] = Fl.f2.d * A;
+    col[3] = Fl.f3.d * A;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) J[static_cast<size_t>(dir * 4 + k) * F + f] = col[k];
+}
+
+// ---- adjoint product out = R_U^T psi ---------------------------------------------------------------
+// Face f adds F_f A_f to R_o and subtracts it from R_n, so its weight is w = psi_o - psi_n (psi_o on
+// boundary faces) and T[dir][f] = sum_k J[dir][k][f] w_k.
+__global__ __launch_bounds__(kThreads) void k_adj_face(AdjMesh am, const double* __restrict__ J,
+                                                       const double* __restrict__ psi, double* __restrict__ T) {
+  const int f = blockIdx.x * kThreads + threadIdx.x;
+  if (f >= am.F) return;
+  const int F = am.F, ld = am.ld;
+  const int o = am.fo[f], n = am.fn[f];
+  double w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = psi[k * ld + o] - (n >= 0 ? psi[k * ld + n] : 0.0);
+  const int ndir = n >= 0 ? 24 : 4;
+  for (int dir = 0; dir < 24; ++dir) {
+    double t = 0.0;
+    if (dir < ndir) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t += J[static_cast<size_t>(dir * 4 + k) * F + f] * w[k];
+    }
+    T[static_cast<size_t>(dir) * F + f] = t;
+  }
+}
+
+// Per cell: the direct part of R_U^T psi and the gradient adjoint Gbar, over its faces in face order.
+__global__ __launch_bounds__(kThreads) void k_adj_cell(AdjMesh am, const double* __restrict__ T,
+                                                       double* __restrict__ out, double* __restrict__ Gbar) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int F = am.F, ld = am.ld;
+  double u[4] = {0.0, 0.0, 0.0, 0.0};
+  double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int e1 = am.cf_off[c + 1];
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    const int side = ent & 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] += T[static_cast<size_t>(side * 4 + k) * F + f];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) gb[j] += T[static_cast<size_t>(8 + side * 8 + j) * F + f];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[k * ld + c] = u[k];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) Gbar[j * ld + c] = gb[j];
+}
+
+// Per cell: Wbar from the gradient adjoint of the cell and its face neighbours, then
+// out += (dW/dU)^T Wbar.
+//   Green-Gauss: G_c[k] = iv_c sum_f +-q_f nA_f, q_f = (W_o + W_n) / 2 (W_o on boundary faces), so both
+//     cells of an interior face get (1/2) nA . (iv_o Gbar_o[k] - iv_n Gbar_n[k]); a boundary face gives
+//     its owner nA . iv_o Gbar_o[k].
+//   Least squares: G_c[k] = H_c sum_f (w d)_f (W_n - W_o)_k, so s = (w d)_f . (H_o Gbar_o[k] +
+//     H_n Gbar_n[k]) goes to the neighbour with + and to the owner with -.
+template <int GK>
+__global__ __launch_bounds__(kThreads) void k_adj_grad(AdjMesh am, const double* __restrict__ U, double g,
+                                                       const double* __restrict__ Gbar, double* __restrict__ out) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int ld = am.ld;
+  double wb[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = am.cf_off[c + 1];
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    const int o = am.fo[f], n = am.fn[f];
+    if (GK == 1) {
+      if (n < 0) continue;
+      const double wx = am.fwdx[f], wy = am.fwdy[f];
+      const double hxo = am.H[o], hyo = am.H[ld + o], hzo = am.H[2 * ld + o];
+      const double hxn = am.H[n], hyn = am.H[ld + n], hzn = am.H[2 * ld + n];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double gxo = Gbar[(2 * k) * ld + o], gyo = Gbar[(2 * k + 1) * ld + o];
+        const double gxn = Gbar[(2 * k) * ld + n], gyn = Gbar[(2 * k + 1) * ld + n];
+        const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
+        const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
+        const double s = wx * ax + wy * ay;
+        wb[k] += (ent & 1) ? s : -s;
+      }
+    } else {
+      const double A = am.fA[f];
+      const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
+      const double ivo = 1.0 / dmax(am.vol[o], 1.0e-12);
+      if (n >= 0) {
+        const double ivn = 1.0 / dmax(am.vol[n], 1.0e-12);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double tx = ivo * Gbar[(2 * k) * ld + o] - ivn * Gbar[(2 * k) * ld + n];
+          const double ty = ivo * Gbar[(2 * k + 1) * ld + o] - ivn * Gbar[(2 * k + 1) * ld + n];
+          wb[k] += 0.5 * (nax * tx + nay * ty);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + o]) + nay * (ivo * Gbar[(2 * k + 1) * ld + o]);
+      }
+    }
+  }
+  // (dW/dU)^T Wbar by four dual evaluations of the primitive map (euler_flux.cpp:115-127).
+  const C4 u0 = load_c4(U, ld, c);
+  double add[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    C4T<Dual> s;
+    s.r = Dual(u0.r, j == 0 ? 1.0 : 0.0);
+    s.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
+    s.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
+    s.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
+    const P4T<Dual> w = c2p(s, g);
+    add[j] = w.r.d * wb[0] + w.u.d * wb[1] + w.v.d * wb[2] + w.p.d * wb[3];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) out[j * ld + c] += add[j];
+}
+
+// ---- objective gradient ----------------------------------------------------------------------------
+// dJ/dU of J = sum over wall faces of coef_f * p(owner) (the gauge pressure forces of forces.cpp:208-294
+// are linear in the owner pressures).  One thread walks the wall patch in order, so cells with several
+// wall faces sum them deterministically.  `out` is zeroed by the caller.
+__global__ void k_objective_gradient(AdjMesh am, const double* __restrict__ U, double g,
+                                     const double* __restrict__ coef, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int ld = am.ld;
+  for (int i = 0; i < am.wall_count; ++i) {
+    const int f = am.wall_start + i;
+    const int o = am.fo[f];
+    const C4 u0 = load_c4(U, ld, o);
+    for (int j = 0; j < 4; ++j) {
+      C4T<Dual> s;
+      s.r = Dual(u0.r, j == 0 ? 1.0 : 0.0);
+      s.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
+      s.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
+      s.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
+      const P4T<Dual> w = c2p(s, g);
+      out[j * ld + o] += coef[i] * w.p.d;
+    }
+  }
+}
+
+// ---- pseudo-time iteration of the linearised problem --------------------------------------------
+// The primal stage is U* = U_s - D P R(U_s), D = cfl / max(lambda, 1e-8) per cell and P the low-Mach
+// energy-row preconditioner (preconditioning.cpp:186-199).  Linearised at a fixed state it is
+// dU* = dU_s - D P (R_U dU_s + R_p); the adjoint iterates the transposed map,
+// phi* = phi_s - (R_U^T P^T D phi_s - rhs), whose fixed point gives psi = P^T D phi with
+// R_U^T psi = rhs.  Both use the RK stage combinations of time_integrator.cpp:61-104, so they converge
+// at the rate of the linearised primal.
+__device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, double* b2m1_inv_u, double* b2m1_inv_v,
+                                            double* b2) {
+  const P4 wp = c2p(us, o.g);
+  const double bt = cell_beta(wp, o);
+  *b2 = bt * bt;
+  const double rho = dmax(wp.r, 1.0e-8);
+  const double inv = 1.0 / rho;
+  *b2m1_inv_u = (*b2 - 1.0) * inv * wp.u;
+  *b2m1_inv_v = (*b2 - 1.0) * inv * wp.v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_adj_scale(AdjMesh am, const double* __restrict__ lam, double cfl,
+                                                        const double* __restrict__ U, FaceOpts o,
+                                                        const double* __restrict__ phi, double* __restrict__ psi) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int ld = am.ld;
+  const double dt = cfl / dmax(lam[c], 1.0e-8);
+  double y[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y[k] = dt * phi[k * ld + c];
+  if (o.precond) {
+    double au, av, b2;
+    precond_row(load_c4(U, ld, c), o, &au, &av, &b2);
+    psi[c] = y[0];
+    psi[ld + c] = y[1] + au * y[3];
+    psi[2 * ld + c] = y[2] + av * y[3];
+    psi[3 * ld + c] = b2 * y[3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) psi[k * ld + c] = y[k];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_lin_update(AdjMesh am, int stage, int rk3, int forward,
+                                                         const double* __restrict__ x0, const double* __restrict__ xs,
+                                                         const double* __restrict__ r, const double* __restrict__ rhs,
+                                                         const double* __restrict__ lam, double cfl,
+                                                         const double* __restrict__ U, FaceOpts o,
+                                                         double* __restrict__ xout, double* __restrict__ partials) {
+  __shared__ double red[kThreads];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  double sq = 0.0;
+  if (c < am.N) {
+    const int ld = am.ld;
+    double res[4], inc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) res[k] = r[k * ld + c] - (rhs ? rhs[k * ld + c] : 0.0);
+    if (forward) {
+      const double dt = cfl / dmax(lam[c], 1.0e-8);
+      double p3 = res[3];
+      if (o.precond) {
+        double au, av, b2;
+        precond_row(load_c4(U, ld, c), o, &au, &av, &b2);
+        p3 = b2 * res[3] + (au * res[1] + av * res[2]);
+      }
+      inc[0] = dt * res[0];
+      inc[1] = dt * res[1];
+      inc[2] = dt * res[2];
+      inc[3] = dt * p3;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) inc[k] = res[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double x = xs[k * ld + c] - inc[k];
+      if (rk3 && stage == 1) x = 0.75 * x0[k * ld + c] + 0.25 * x;
+      if (rk3 && stage == 2) x = (1.0 / 3.0) * x0[k * ld + c] + (2.0 / 3.0) * x;
+      xout[k * ld + c] = x;
+      sq += res[k] * res[k];
+    }
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && partials) partials[blockIdx.x] = red[0];
+}
+
+// Fixed-order sum of block partials (one workgroup).
+__global__ __launch_bounds__(kThreads) void k_partials_sum(int n, const double* __restrict__ partials,
+                                                           double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) s += partials[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+}  // namespace
+}  // namespace aero_dev
+
+// ------------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------------
+namespace aero_host {
+
+using namespace aero_dev;
+
+namespace {
+inline unsigned blocks_for(long long n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+}  // namespace
+
+hipError_t launch_tan_gradient(const AdjMesh& am, const double* U, const double* dU, const GeoTan& gt, double g,
+                               double* G, double* dG, hipStream_t s) {
+  if (am.grad == 1)
+    hipLaunchKernelGGL(k_tan_gradient<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, dU, gt, g, G, dG);
+  else
+    hipLaunchKernelGGL(k_tan_gradient<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, dU, gt, g, G, dG);
+  return hipGetLastError();
+}
+
+template <int SCHEME, int LIMITER, bool SECOND>
+static void tan_face_t(const AdjMesh& am, const double* U, const double* dU, const double* G, const double* dG,
+                       const GeoTan& gt, const FaceOptsT<Dual>& o, double* fa, hipStream_t s) {
+  hipLaunchKernelGGL((k_tan_face<SCHEME, LIMITER, SECOND>), dim3(blocks_for(am.F)), dim3(kThreads), 0, s, am, U, dU,
+                     G, dG, gt, o, fa);
+}
+
+hipError_t launch_tan_face(const AdjMesh& am, int scheme, int limiter, bool second, const double* U, const double* dU,
+                           const double* G, const double* dG, const GeoTan& gt, const FaceOptsT<Dual>& o, double* fa,
+                           hipStream_t s) {
+  const int key = (scheme ? 4 : 0) | (limiter ? 2 : 0) | (second ? 1 : 0);
+  switch (key) {
+    case 0: tan_face_t<0, 0, false>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 1: tan_face_t<0, 0, true>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 2: tan_face_t<0, 1, false>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 3: tan_face_t<0, 1, true>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 4: tan_face_t<1, 0, false>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 5: tan_face_t<1, 0, true>(am, U, dU, G, dG, gt, o, fa, s); break;
+    case 6: tan_face_t<1, 1, false>(am, U, dU, G, dG, gt, o, fa, s); break;
+    default: tan_face_t<1, 1, true>(am, U, dU, G, dG, gt, o, fa, s); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tan_gather(const AdjMesh& am, const double* fa, double* R, double* dR, hipStream_t s) {
+  hipLaunchKernelGGL(k_tan_gather, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, fa, R, dR);
+  return hipGetLastError();
+}
+
+template <int SCHEME, int LIMITER, bool SECOND>
+static void face_jac_t(const AdjMesh& am, const double* U, const double* G, const FaceOpts& o, double* J,
+                       hipStream_t s) {
+  hipLaunchKernelGGL((k_face_jacobian<SCHEME, LIMITER, SECOND>), dim3(blocks_for(24LL * am.F)), dim3(kThreads), 0, s,
+                     am, U, G, o, J);
+}
+
+hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool second, const double* U,
+                                const double* G, const FaceOpts& o, double* J, hipStream_t s) {
+  const int key = (scheme ? 4 : 0) | (limiter ? 2 : 0) | (second ? 1 : 0);
+  switch (key) {
+    case 0: face_jac_t<0, 0, false>(am, U, G, o, J, s); break;
+    case 1: face_jac_t<0, 0, true>(am, U, G, o, J, s); break;
+    case 2: face_jac_t<0, 1, false>(am, U, G, o, J, s); break;
+    case 3: face_jac_t<0, 1, true>(am, U, G, o, J, s); break;
+    case 4: face_jac_t<1, 0, false>(am, U, G, o, J, s); break;
+    case 5: face_jac_t<1, 0, true>(am, U, G, o, J, s); break;
+    case 6: face_jac_t<1, 1, false>(am, U, G, o, J, s); break;
+    default: face_jac_t<1, 1, true>(am, U, G, o, J, s); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
+                              double* T, double* Gbar, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_face, dim3(blocks_for(am.F)), dim3(kThreads), 0, s, am, J, psi, T);
+  hipLaunchKernelGGL(k_adj_cell, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, T, out, Gbar);
+  if (am.grad == 1)
+    hipLaunchKernelGGL(k_adj_grad<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
+  else
+    hipLaunchKernelGGL(k_adj_grad<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_objective_gradient(const AdjMesh& am, const double* U, double g, const double* coef, double* out,
+                                     hipStream_t s) {
+  hipLaunchKernelGGL(k_objective_gradient, dim3(1), dim3(64), 0, s, am, U, g, coef, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_scale(const AdjMesh& am, const double* lam, double cfl, const double* U, const FaceOpts& o,
+                            const double* phi, double* psi, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_scale, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, lam, cfl, U, o, phi, psi);
+  return hipGetLastError();
+}
+
+hipError_t launch_lin_update(const AdjMesh& am, int stage, int rk3, int forward, const double* x0, const double* xs,
+                             const double* r, const double* rhs, const double* lam, double cfl, const double* U,
+                             const FaceOpts& o, double* xout, double* partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_lin_update, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, stage, rk3, forward, x0, xs, r,
+                     rhs, lam, cfl, U, o, xout, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_partials_sum(int nparts, const double* partials, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_partials_sum, dim3(1), dim3(kThreads), 0, s, nparts, partials, out);
+  return hipGetLastError();
+}
+
+}  // namespace aero_host

@@ -22,6 +22,7 @@ using aero_dev::DevMesh;
 using aero_dev::FaceOpts;
 using aero_dev::StageArgs;
 using aero_dev::WallArgs;
+using namespace aero_hostutil;
 
 namespace {
 
@@ -91,25 +92,6 @@ void rcb(std::vector<int>& idx, int lo, int hi, const double* cc, int BC) {
   rcb(idx, lo + left, hi, cc, BC);
 }
 
-FaceOpts face_opts(const aero_residual_config* c) {
-  FaceOpts o;
-  o.ff = aero_dev::C4{c->farfield[0], c->farfield[1], c->farfield[2], c->farfield[4]};
-  o.g = c->gamma;
-  o.low_mach_fix = c->low_mach_fix;
-  o.low_mach_cutoff = c->low_mach_cutoff;
-  o.stab_floor = c->stab_mach_floor;
-  o.char_bc = c->precond_farfield_bc;
-  o.precond = c->precond_enabled;
-  o.mach_ref = c->precond_mach_ref;
-  o.mach_min = c->precond_mach_min;
-  o.beta_min = c->precond_beta_min;
-  o.beta_max = c->precond_beta_max;
-  o.as_fix = c->all_speed_fix;
-  o.as_cut = c->all_speed_cutoff;
-  o.as_fmin = c->all_speed_f_min;
-  o.as_w = c->all_speed_ramp;
-  return o;
-}
 
 }  // namespace
 
@@ -142,6 +124,7 @@ void free_events(aero_hip_mesh* m) {
 void free_all(aero_hip_mesh* m) {
   if (!m) return;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  adjoint_free(m);
   free_events(m);
   for (void* p : m->allocs) (void)hipFree(p);
   m->allocs.clear();
@@ -149,6 +132,37 @@ void free_all(aero_hip_mesh* m) {
   m->h_iter = nullptr;
   if (m->stream) (void)hipStreamDestroy(m->stream);
   m->stream = nullptr;
+}
+
+bool env_on(const char* name) {  // euler_solver.cpp:38-45
+  const char* v = std::getenv(name);
+  if (!v) return false;
+  const std::string t(v);
+  return t == "1" || t == "true" || t == "TRUE" || t == "on" || t == "ON";
+}
+
+}  // namespace
+
+namespace aero_hostutil {
+
+FaceOpts face_opts(const aero_residual_config* c) {
+  FaceOpts o;
+  o.ff = aero_dev::C4{c->farfield[0], c->farfield[1], c->farfield[2], c->farfield[4]};
+  o.g = c->gamma;
+  o.low_mach_fix = c->low_mach_fix;
+  o.low_mach_cutoff = c->low_mach_cutoff;
+  o.stab_floor = c->stab_mach_floor;
+  o.char_bc = c->precond_farfield_bc;
+  o.precond = c->precond_enabled;
+  o.mach_ref = c->precond_mach_ref;
+  o.mach_min = c->precond_mach_min;
+  o.beta_min = c->precond_beta_min;
+  o.beta_max = c->precond_beta_max;
+  o.as_fix = c->all_speed_fix;
+  o.as_cut = c->all_speed_cutoff;
+  o.as_fmin = c->all_speed_f_min;
+  o.as_w = c->all_speed_ramp;
+  return o;
 }
 
 int to_internal_state(aero_hip_mesh* m, const double* conserved, std::vector<double>* soa) {
@@ -174,14 +188,7 @@ void from_internal4(const aero_hip_mesh* m, const std::vector<double>& soa, doub
   }
 }
 
-bool env_on(const char* name) {  // euler_solver.cpp:38-45
-  const char* v = std::getenv(name);
-  if (!v) return false;
-  const std::string t(v);
-  return t == "1" || t == "true" || t == "TRUE" || t == "on" || t == "ON";
-}
-
-}  // namespace
+}  // namespace aero_hostutil
 
 extern "C" {
 
@@ -274,6 +281,23 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   }
   m->h_owner.assign(d->face_owner, d->face_owner + F);
   m->h_neighbor.assign(d->face_neighbor, d->face_neighbor + F);
+  m->h_bc.resize(F);
+  m->h_fx.resize(F);
+  m->h_fy.resize(F);
+  for (int f = 0; f < F; ++f) {
+    m->h_bc[f] = d->face_neighbor[f] >= 0 ? AERO_BC_INTERIOR
+                                           : ((d->face_bc && d->face_bc[f] == AERO_BC_SLIP_WALL) ? AERO_BC_SLIP_WALL
+                                                                                                  : AERO_BC_FARFIELD);
+    m->h_fx[f] = d->face_center[3 * f];
+    m->h_fy[f] = d->face_center[3 * f + 1];
+  }
+  m->h_vol.assign(d->cell_volume, d->cell_volume + N);
+  m->h_ccx.resize(N);
+  m->h_ccy.resize(N);
+  for (int c = 0; c < N; ++c) {
+    m->h_ccx[c] = d->cell_center[3 * c];
+    m->h_ccy[c] = d->cell_center[3 * c + 1];
+  }
 
   // Internal cell order.
   m->perm.resize(N);

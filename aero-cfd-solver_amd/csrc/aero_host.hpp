@@ -5,7 +5,10 @@
 
 #include <vector>
 
+#include "aero_hip.h"
 #include "aero_internal.hpp"
+
+struct AdjointData;  // tangent / adjoint workspace, built on first use (aero_adjoint.cpp)
 
 struct aero_hip_mesh {
   int N = 0, F = 0, BC = 256, nblk = 0, S = 0;
@@ -29,7 +32,20 @@ struct aero_hip_mesh {
   double* h_iter = nullptr;  // pinned: 9 doubles + 4 ints
   std::vector<hipEvent_t> tev;  // per-kernel timing events (3 per residual evaluation)
   int tcap = 0, tcount = 0;
-  std::vector<double> h_nx, h_ny, h_area, h_fc;  // for host-side diagnostics
-  std::vector<int> h_owner, h_neighbor;
+  // Host copies in the reference numbering (unit normals as renormalised at setup), for host-side
+  // diagnostics and the tangent / adjoint setup.
+  std::vector<double> h_nx, h_ny, h_area, h_fx, h_fy, h_vol, h_ccx, h_ccy;
+  std::vector<int> h_owner, h_neighbor, h_bc;  // h_bc: AERO_BC_* per face
   int wall_start = 0, wall_count = 0;
+  AdjointData* adj = nullptr;
 };
+
+namespace aero_hostutil {
+// The per-evaluation options of an aero_residual_config.
+aero_dev::FaceOpts face_opts(const aero_residual_config* c);
+// Host [N][5] conserved (reference numbering) <-> device SoA [4][N] (internal numbering).
+int to_internal_state(aero_hip_mesh* m, const double* conserved, std::vector<double>* soa);
+void from_internal4(const aero_hip_mesh* m, const std::vector<double>& soa, double* aos5);
+// Releases the tangent / adjoint workspace (aero_adjoint.cpp).
+void adjoint_free(aero_hip_mesh* m);
+}  // namespace aero_hostutil

@@ -109,6 +109,43 @@ struct BatchStage {
   int stage, last, cur, base, out;
 };
 
+// ---- tangent / adjoint -----------------------------------------------------------------------------
+// Faces in reference order with internal cell ids, and per-cell face lists (face << 1 | is_neighbour)
+// in increasing face index: the summation order of the residual.
+struct AdjMesh {
+  int N, F, ld;
+  int grad;              // AERO_GRAD_*
+  const int* fo;         // [F] owner (internal id)
+  const int* fn;         // [F] neighbour (internal id) or -1
+  const int* fbc;        // [F] AERO_BC_*
+  const double* fnx;     // [F]
+  const double* fny;
+  const double* fA;
+  const double* ffx;     // face centre
+  const double* ffy;
+  const double* fwdx;    // [F] least-squares face weights w dx, w dy (interior faces)
+  const double* fwdy;
+  const int* cf_off;     // [N + 1]
+  const int* cf_ent;
+  const double* vol;     // [N]
+  const double* ccx;
+  const double* ccy;
+  const double* H;       // [3][N] least-squares inverse normal matrix (hxx, hxy, hyy); zero when singular
+  int wall_start, wall_count;
+};
+
+// Geometry tangents (any pointer may be null = zero): faces in reference order, cells internal.
+struct GeoTan {
+  const double* dnx;
+  const double* dny;
+  const double* dA;
+  const double* dfx;
+  const double* dfy;
+  const double* dccx;
+  const double* dccy;
+  const double* dvol;
+};
+
 }  // namespace aero_dev
 
 namespace aero_host {
@@ -134,4 +171,25 @@ hipError_t launch_batch_reduce(const aero_dev::BatchCase* cases, aero_dev::CaseC
 hipError_t launch_batch_control(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl,
                                 const aero_dev::BatchParams& p, hipStream_t s);
 hipError_t launch_batch_init_diag(const aero_dev::BatchCase* cases, int ncases, hipStream_t s);
+
+// tangent / adjoint (aero_adjoint.hip)
+hipError_t launch_tan_gradient(const aero_dev::AdjMesh& am, const double* U, const double* dU,
+                               const aero_dev::GeoTan& gt, double g, double* G, double* dG, hipStream_t s);
+hipError_t launch_tan_face(const aero_dev::AdjMesh& am, int scheme, int limiter, bool second, const double* U,
+                           const double* dU, const double* G, const double* dG, const aero_dev::GeoTan& gt,
+                           const aero_dev::FaceOptsT<aero_dev::Dual>& o, double* fa, hipStream_t s);
+hipError_t launch_tan_gather(const aero_dev::AdjMesh& am, const double* fa, double* R, double* dR, hipStream_t s);
+hipError_t launch_face_jacobian(const aero_dev::AdjMesh& am, int scheme, int limiter, bool second, const double* U,
+                                const double* G, const aero_dev::FaceOpts& o, double* J, hipStream_t s);
+hipError_t launch_adj_product(const aero_dev::AdjMesh& am, const double* J, const double* U, const double* psi,
+                              double g, double* T, double* Gbar, double* out, hipStream_t s);
+hipError_t launch_objective_gradient(const aero_dev::AdjMesh& am, const double* U, double g, const double* coef,
+                                     double* out, hipStream_t s);
+hipError_t launch_adj_scale(const aero_dev::AdjMesh& am, const double* lam, double cfl, const double* U,
+                            const aero_dev::FaceOpts& o, const double* phi, double* psi, hipStream_t s);
+hipError_t launch_lin_update(const aero_dev::AdjMesh& am, int stage, int rk3, int forward, const double* x0,
+                             const double* xs, const double* r, const double* rhs, const double* lam, double cfl,
+                             const double* U, const aero_dev::FaceOpts& o, double* xout, double* partials,
+                             hipStream_t s);
+hipError_t launch_partials_sum(int nparts, const double* partials, double* out, hipStream_t s);
 }  // namespace aero_host

@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "aero_internal.hpp"
+#include "face_eval.hpp"
 #include "euler_device.hpp"
 
 namespace aero_dev {
@@ -214,21 +215,6 @@ __global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __rest
   gradient_body<BC, GK>(m, b, U, G, g, lds);
 }
 
-// MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
-template <int LIMITER>
-__device__ __forceinline__ void recon_var(double qo, double qn, double gox, double goy, double gnx, double gny,
-                                          double dox, double doy, double dnx, double dny, double* ql, double* qr) {
-  const double dc = qn - qo;
-  double da = 0.0;
-  da += gox * dox;
-  da += goy * doy;
-  double db = 0.0;
-  db += gnx * dnx;
-  db += gny * dny;
-  *ql = qo + limiter<LIMITER>(da, dc);
-  *qr = qn + limiter<LIMITER>(db, -dc);
-}
-
 // Wavefront sum of an int (64 lanes).
 __device__ __forceinline__ int wave_sum_i(int v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -271,57 +257,29 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
     const double A = m.s_A[s];
     Flux F;
     if (cn >= 0) {
-      P4 L = c2p(load_u(U, ld, co), g);
-      P4 Rt = c2p(load_u(U, ld, cn), g);
+      double go[8], gn[8];
+      double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
       if (SECOND) {
-        const double fx = m.s_fx[s], fy = m.s_fy[s];
-        const double dox = fx - m.ccx[co], doy = fy - m.ccy[co];
-        const double dnx = fx - m.ccx[cn], dny = fy - m.ccy[cn];
-        P4 l2, r2;
-        recon_var<LIMITER>(L.r, Rt.r, G[0 * ld + co], G[1 * ld + co], G[0 * ld + cn], G[1 * ld + cn], dox, doy, dnx,
-                           dny, &l2.r, &r2.r);
-        recon_var<LIMITER>(L.u, Rt.u, G[2 * ld + co], G[3 * ld + co], G[2 * ld + cn], G[3 * ld + cn], dox, doy, dnx,
-                           dny, &l2.u, &r2.u);
-        recon_var<LIMITER>(L.v, Rt.v, G[4 * ld + co], G[5 * ld + co], G[4 * ld + cn], G[5 * ld + cn], dox, doy, dnx,
-                           dny, &l2.v, &r2.v);
-        recon_var<LIMITER>(L.p, Rt.p, G[6 * ld + co], G[7 * ld + co], G[6 * ld + cn], G[7 * ld + cn], dox, doy, dnx,
-                           dny, &l2.p, &r2.p);
-        l2.r = dmax(l2.r, kFloor);
-        r2.r = dmax(r2.r, kFloor);
-        l2.p = dmax(l2.p, kFloor);
-        r2.p = dmax(r2.p, kFloor);
-        if (valid_prim(l2) && valid_prim(r2)) {
-          L = l2;
-          Rt = r2;
-        } else if (meta & kMetaPrimary) {
-          ++n_fo;
-          first_fail = min(first_fail, m.s_face[s]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          go[j] = G[j * ld + co];
+          gn[j] = G[j * ld + cn];
         }
+        const double fx = m.s_fx[s], fy = m.s_fy[s];
+        dox = fx - m.ccx[co];
+        doy = fy - m.ccy[co];
+        dnx = fx - m.ccx[cn];
+        dny = fy - m.ccy[cn];
       }
-      const C4 ul = p2c(L, g);
-      const C4 ur = p2c(Rt, g);
-      const double fb = face_beta(ul, ur, o);
-      const double sc = acoustic_scale(ul, ur, fb, o);
-      F = face_flux<SCHEME>(ul, ur, nx, ny, sc, o);
+      bool reverted;
+      F = interior_face<SCHEME, LIMITER, SECOND>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy, dnx,
+                                                 dny, o, &reverted);
+      if (reverted && (meta & kMetaPrimary)) {
+        ++n_fo;
+        first_fail = min(first_fail, m.s_face[s]);
+      }
     } else {
-      const C4 in = p2c(c2p(load_u(U, ld, co), g), g);
-      const bool wall = (meta & kMetaBcMask) == 2;
-      const double fb = wall ? face_beta(in, in, o) : face_beta(in, o.ff, o);
-      const double sc = acoustic_scale(in, wall ? in : o.ff, fb, o);
-      if (wall) {  // boundary_conditions.cpp:97-111
-        const P4 p = c2p(in, g);
-        const double un = p.u * nx + p.v * ny;
-        const double a = sound(p, g);
-        F.lam = pspeeds(un, a, sc).sr;
-        F.f0 = 0.0;
-        F.f1 = p.p * nx;
-        F.f2 = p.p * ny;
-        F.f3 = 0.0;
-        F.fallback = 0;
-      } else {  // boundary_conditions.cpp:113-122
-        const C4 bs = o.char_bc ? char_farfield_state(in, nx, ny, sc, o) : o.ff;
-        F = face_flux<SCHEME>(in, bs, nx, ny, sc, o);
-      }
+      F = boundary_face<SCHEME>(load_u(U, ld, co), (meta & kMetaBcMask) == 2, nx, ny, o);
     }
     if (F.fallback && (meta & kMetaPrimary)) ++n_diff;
     f0[i] = F.f0 * A;

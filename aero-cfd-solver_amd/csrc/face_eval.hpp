@@ -1,0 +1,87 @@
+// face_eval.hpp -- one face of the residual (residual_assembly.cpp:97-166), templated on the scalar.
+//
+// The residual kernels instantiate it on double; the tangent and face-Jacobian kernels on Dual, so the
+// derivative code is the residual code.
+#pragma once
+
+#include "euler_device.hpp"
+
+namespace aero_dev {
+
+// MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
+template <int LIMITER, class T>
+__device__ __forceinline__ void recon_var(T qo, T qn, T gox, T goy, T gnx, T gny, T dox, T doy, T dnx, T dny, T* ql,
+                                          T* qr) {
+  const T dc = qn - qo;
+  T da = 0.0;
+  da += gox * dox;
+  da += goy * doy;
+  T db = 0.0;
+  db += gnx * dnx;
+  db += gny * dny;
+  *ql = qo + limiter<LIMITER>(da, dc);
+  *qr = qn + limiter<LIMITER>(db, -dc);
+}
+
+// Interior face: reconstruction with the first-order fallback (residual_assembly.cpp:107-137).
+// go / gn are the cells' primitive gradients (rho_x, rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the
+// vectors from the cell centres to the face centre.  *reverted reports the fallback to cell values.
+template <int SCHEME, int LIMITER, bool SECOND, class T>
+__device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
+                                                  T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
+                                                  bool* reverted) {
+  const double g = o.g;
+  P4T<T> L = c2p(uo, g);
+  P4T<T> Rt = c2p(un, g);
+  *reverted = false;
+  if (SECOND) {
+    P4T<T> l2, r2;
+    recon_var<LIMITER>(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, &l2.r, &r2.r);
+    recon_var<LIMITER>(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, &l2.u, &r2.u);
+    recon_var<LIMITER>(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, &l2.v, &r2.v);
+    recon_var<LIMITER>(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, &l2.p, &r2.p);
+    l2.r = dmax(l2.r, kFloor);
+    r2.r = dmax(r2.r, kFloor);
+    l2.p = dmax(l2.p, kFloor);
+    r2.p = dmax(r2.p, kFloor);
+    if (valid_prim(l2) && valid_prim(r2)) {
+      L = l2;
+      Rt = r2;
+    } else {
+      *reverted = true;
+    }
+  }
+  const C4T<T> ul = p2c(L, g);
+  const C4T<T> ur = p2c(Rt, g);
+  const T fb = face_beta(ul, ur, o);
+  const T sc = acoustic_scale(ul, ur, fb, o);
+  return face_flux<SCHEME>(ul, ur, nx, ny, sc, o);
+}
+
+// Boundary face: slip wall or farfield on the owner's first-order state (residual_assembly.cpp:138-166,
+// boundary_conditions.cpp:81-123).
+template <int SCHEME, class T>
+__device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
+  const double g = o.g;
+  const C4T<T> in = p2c(c2p(uo, g), g);
+  const T fb = wall ? face_beta(in, in, o) : face_beta(in, o.ff, o);
+  const T sc = acoustic_scale(in, wall ? in : o.ff, fb, o);
+  FluxT<T> F;
+  if (wall) {  // boundary_conditions.cpp:97-111
+    const P4T<T> p = c2p(in, g);
+    const T un = p.u * nx + p.v * ny;
+    const T a = sound(p, g);
+    F.lam = pspeeds(un, a, sc).sr;
+    F.f0 = 0.0;
+    F.f1 = p.p * nx;
+    F.f2 = p.p * ny;
+    F.f3 = 0.0;
+    F.fallback = 0;
+  } else {  // boundary_conditions.cpp:113-122
+    const C4T<T> bs = o.char_bc ? char_farfield_state(in, nx, ny, sc, o) : o.ff;
+    F = face_flux<SCHEME>(in, bs, nx, ny, sc, o);
+  }
+  return F;
+}
+
+}  // namespace aero_dev

@@ -225,6 +225,80 @@ int aero_hip_batch_history(aero_hip_batch* batch, int32_t case_index, double* hi
                            char* err, size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
+ * Discrete tangent and adjoint of the residual (no reference counterpart: the reference has no
+ * derivative code).  Derivatives are exact derivatives of the residual the other entry points
+ * compute -- the same device code instantiated on dual numbers -- taken on the branch each value
+ * follows (limiter cases, HLLC fallbacks, floors).  A NULL `conserved` means the handle's current
+ * device state (for example the state aero_hip_run_case ended on); otherwise it becomes that state.
+ * Host arrays use the reference layout and numbering; the rhow column is ignored on input and zero on
+ * output.
+ * ---------------------------------------------------------------------------------------------- */
+enum { AERO_PARAM_NONE = 0, AERO_PARAM_AOA = 1, AERO_PARAM_MACH = 2 };
+
+/* A direction in parameter space: derivatives of the freestream-dependent options and, optionally,
+ * of the geometry (reference layout; any pointer may be NULL for zero). */
+typedef struct aero_tangent_seed {
+  double d_farfield[5];
+  double d_stab_floor, d_mach_ref, d_mach_min, d_beta_min, d_beta_max;
+  const double* d_face_normal; /* [3F] */
+  const double* d_face_center; /* [3F] */
+  const double* d_face_area;   /* [F] */
+  const double* d_cell_center; /* [3N] */
+  const double* d_cell_volume; /* [N] */
+} aero_tangent_seed;
+
+/* J = w_cl CL + w_cd CD + w_cm CM of the gauge wall pressure forces (forces.cpp:208-294), with the
+ * reference values that define the coefficients. */
+typedef struct aero_objective {
+  double w_cl, w_cd, w_cm;
+  double aoa_deg, q_inf, p_inf, chord, x_ref, y_ref, gamma;
+} aero_objective;
+
+/* Controls and results of a pseudo-time linear solve: rk_stages 1 or 3, cfl, at most max_iters
+ * iterations, stop when the linear residual's L2 norm is <= tol times its first value (checked every
+ * check_every iterations). */
+typedef struct aero_linear_solve {
+  int32_t rk_stages;
+  double cfl;
+  int32_t max_iters;
+  double tol;
+  int32_t check_every;
+  int32_t iterations;     /* out */
+  double res0, res;       /* out: first and last linear residual norms */
+  double seconds;         /* out: device time of the iterations */
+} aero_linear_solve;
+
+/* The residual configuration of iteration `iteration` of the case (the driver's ramps evaluated
+ * there, euler_solver.cpp:463-797), the derivative of its freestream-dependent fields with respect to
+ * `param` (AERO_PARAM_*; angle of attack per degree), the objective reference values (weights zero) and
+ * d q_inf / d param.  The staged all-speed controller is rejected (its schedule depends on the run). */
+int aero_case_linearization(const aero_case_config* cfg, int32_t iteration, int32_t param,
+                            aero_residual_config* residual_cfg, aero_tangent_seed* seed, aero_objective* objective,
+                            double* d_q_inf, char* err, size_t errlen);
+
+/* Residual R and its directional derivative dR = R_U dU + R_p . seed (dU or seed may be NULL). */
+int aero_hip_residual_jvp(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                          const double* dU, const aero_tangent_seed* seed, double* residual /* or NULL */, double* dR,
+                          char* err, size_t errlen);
+/* out = R_U^T psi, through the stored face Jacobians. */
+int aero_hip_residual_vjp(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                          const double* psi, double* out, char* err, size_t errlen);
+/* Objective value, dJ/dU [N][5] (may be NULL) and the explicit partials: dJ/d(aoa, per degree) and
+ * dJ/dq_inf at fixed state, and the derivative along the geometry of `seed` (may be NULL). */
+int aero_hip_objective(aero_hip_mesh* mesh, const aero_objective* obj, const double* conserved, double* J,
+                       double* dJdU, double* dJ_daoa, double* dJ_dq, const aero_tangent_seed* seed, double* dJ_seed,
+                       char* err, size_t errlen);
+/* Adjoint: psi with R_U^T psi = rhs, by the transposed pseudo-time iteration (same stages, CFL and
+ * local time steps as the primal, frozen at the state).  history: [max_iters] residual norms or NULL. */
+int aero_hip_adjoint_solve(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                           const double* rhs, aero_linear_solve* ctl, double* psi, double* history, char* err,
+                           size_t errlen);
+/* Tangent: v with R_U v = -R_p . seed, by the linearised pseudo-time iteration. */
+int aero_hip_tangent_solve(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                           const aero_tangent_seed* seed, aero_linear_solve* ctl, double* v, double* history, char* err,
+                           size_t errlen);
+
+/* ------------------------------------------------------------------------------------------------
  * Mesh generation (host side).  The NACA-4 O-grid of make_airfoil_ogrid_mesh
  * (mesh/airfoil_mesh.cpp:395-544) with the reference's triangles, plus quad and mixed
  * (quad wall block + triangles) cells, seeded interior-node jitter and RCM cell renumbering.

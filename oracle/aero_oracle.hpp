@@ -35,6 +35,11 @@ template <> constexpr float kc<float>(float f, double) { return f; }
 template <> constexpr double kc<double>(float, double d) { return d; }
 #define AO_K(x) ::aero_oracle::kc<R>(x##f, x)
 
+// sqrt / abs / isfinite, overloadable for other scalar types (oracle_dual.hpp).
+template <class T> inline T ao_sqrt(T x) { return std::sqrt(x); }
+template <class T> inline T ao_abs(T x) { return std::abs(x); }
+template <class T> inline bool ao_isfinite(T x) { return std::isfinite(x); }
+
 // std::max / std::min / std::clamp semantics (libstdc++), spelled out so the operand order is explicit.
 template <class T> inline T smax(T a, T b) { return (a < b) ? b : a; }
 template <class T> inline T smin(T a, T b) { return (b < a) ? b : a; }
@@ -97,14 +102,14 @@ template <class R> void enforce_physical_state(Cons<R>* s, R g) {  // euler_flux
 }
 
 template <class R> R speed_of_sound(const Prim<R>& w, R g) {  // euler_flux.cpp:150-152
-  return std::sqrt(smax(g * w.p / smax(w.rho, AO_K(1.0e-8)), R(0)));
+  return ao_sqrt(smax(g * w.p / smax(w.rho, AO_K(1.0e-8)), R(0)));
 }
 
 template <class R> A5<R> cons_array(const Cons<R>& c) { return {c.rho, c.rhou, c.rhov, c.rhow, c.rhoE}; }
 
 template <class R> bool finite5(const A5<R>& a) {  // euler_flux.cpp:19-26
   for (R x : a)
-    if (!std::isfinite(x)) return false;
+    if (!ao_isfinite(x)) return false;
   return true;
 }
 
@@ -115,7 +120,7 @@ template <class R> A5<R> physical_flux(const Cons<R>& c, const Prim<R>& w, const
 }
 
 template <class R> R with_sign_floor(R v) {  // euler_flux.cpp:45-50
-  if (std::abs(v) >= AO_K(1.0e-8)) return v;
+  if (ao_abs(v) >= AO_K(1.0e-8)) return v;
   return (v >= R(0)) ? AO_K(1.0e-8) : -AO_K(1.0e-8);
 }
 
@@ -127,11 +132,11 @@ template <class R> Speeds<R> precond_speeds(R un, R a, R s) {  // euler_flux.cpp
   const R omb2 = AO_K(1.0) - b2;
   const R ut = AO_K(0.5) * (AO_K(1.0) + b2) * un;
   const R disc = omb2 * omb2 * un * un + AO_K(4.0) * b2 * a * a;
-  const R ct = AO_K(0.5) * std::sqrt(smax(disc, R(0)));
+  const R ct = AO_K(0.5) * ao_sqrt(smax(disc, R(0)));
   Speeds<R> sp;
   sp.lm = ut - ct;
   sp.lp = ut + ct;
-  sp.sr = smax4(std::abs(un), std::abs(sp.lm), std::abs(sp.lp), AO_K(1.0e-8));
+  sp.sr = smax4(ao_abs(un), ao_abs(sp.lm), ao_abs(sp.lp), AO_K(1.0e-8));
   return sp;
 }
 
@@ -143,11 +148,11 @@ MachSensors<R> face_mach_sensors(const Prim<R>& pl, const Prim<R>& pr, const A3<
   const R unr = pr.u * n[0] + pr.v * n[1] + pr.w * n[2];
   const R al = smax(speed_of_sound(pl, g), AO_K(1.0e-8));
   const R ar = smax(speed_of_sound(pr, g), AO_K(1.0e-8));
-  const R vl = std::sqrt(pl.u * pl.u + pl.v * pl.v + pl.w * pl.w);
-  const R vr = std::sqrt(pr.u * pr.u + pr.v * pr.v + pr.w * pr.w);
+  const R vl = ao_sqrt(pl.u * pl.u + pl.v * pl.v + pl.w * pl.w);
+  const R vr = ao_sqrt(pr.u * pr.u + pr.v * pr.v + pr.w * pr.w);
   MachSensors<R> s;
   s.mach = smax(AO_K(0.5) * (vl / al + vr / ar), AO_K(1.0e-6));
-  s.mn = smax(AO_K(0.5) * (std::abs(unl) / al + std::abs(unr) / ar), AO_K(1.0e-6));
+  s.mn = smax(AO_K(0.5) * (ao_abs(unl) / al + ao_abs(unr) / ar), AO_K(1.0e-6));
   s.mn = smin(s.mn, s.mach);
   s.mn = smax(s.mn, AO_K(1.0e-6));
   return s;
@@ -161,8 +166,8 @@ R low_mach_dissipation_scale(const Cons<R>& l, const Cons<R>& r, R g, bool enabl
   const Prim<R> pr = cons_to_prim(r, g);
   const R al = smax(speed_of_sound(pl, g), AO_K(1.0e-8));
   const R ar = smax(speed_of_sound(pr, g), AO_K(1.0e-8));
-  const R vl = std::sqrt(pl.u * pl.u + pl.v * pl.v + pl.w * pl.w);
-  const R vr = std::sqrt(pr.u * pr.u + pr.v * pr.v + pr.w * pr.w);
+  const R vl = ao_sqrt(pl.u * pl.u + pl.v * pl.v + pl.w * pl.w);
+  const R vr = ao_sqrt(pr.u * pr.u + pr.v * pr.v + pr.w * pr.w);
   const R ml = vl / al;
   const R mr = vr / ar;
   const R fm = AO_K(0.5) * (ml + mr);
@@ -198,7 +203,7 @@ A5<R> rusanov_flux(const Cons<R>& l, const Cons<R>& r, const A3<R>& n, R g, R* l
   const R unr = pr.u * n[0] + pr.v * n[1] + pr.w * n[2];
   const R al = speed_of_sound(pl, g);
   const R ar = speed_of_sound(pr, g);
-  const R smax_phys = smax(std::abs(unl) + al, std::abs(unr) + ar);
+  const R smax_phys = smax(ao_abs(unl) + al, ao_abs(unr) + ar);
   const Speeds<R> sl = precond_speeds(unl, al, s);
   const Speeds<R> sr = precond_speeds(unr, ar, s);
   const R smax_d = smax(sl.sr, sr.sr);
@@ -231,9 +236,9 @@ A5<R> hllc_flux(const Cons<R>& l, const Cons<R>& r, const A3<R>& n, R g, R* lam,
   const R sR = smax(spl.lp, spr.lp);
   const R sLp = smin(unl - al, unr - ar);
   const R sRp = smax(unl + al, unr + ar);
-  if (lam) *lam = smax(smax(std::abs(sLp), std::abs(sRp)), AO_K(1.0e-8));
+  if (lam) *lam = smax(smax(ao_abs(sLp), ao_abs(sRp)), AO_K(1.0e-8));
 
-  if (std::abs(sR - sL) < AO_K(1.0e-8)) {
+  if (ao_abs(sR - sL) < AO_K(1.0e-8)) {
     if (fallback) *fallback = true;
     return rusanov_flux(l, r, n, g, lam, s);
   }
@@ -247,7 +252,7 @@ A5<R> hllc_flux(const Cons<R>& l, const Cons<R>& r, const A3<R>& n, R g, R* lam,
   const R rl = smax(pl.rho, AO_K(1.0e-8));
   const R rr = smax(pr.rho, AO_K(1.0e-8));
   const R den = rl * (sL - unl) - rr * (sR - unr);
-  if (std::abs(den) < AO_K(1.0e-8)) {
+  if (ao_abs(den) < AO_K(1.0e-8)) {
     if (fallback) *fallback = true;
     return rusanov_flux(l, r, n, g, lam, s);
   }
@@ -258,7 +263,7 @@ A5<R> hllc_flux(const Cons<R>& l, const Cons<R>& r, const A3<R>& n, R g, R* lam,
     const R rk = smax(pm.rho, AO_K(1.0e-8));
     const R skmu = sk - unk;
     const R skms = sk - sM;
-    if (std::abs(skmu) < AO_K(1.0e-8) || std::abs(skms) < AO_K(1.0e-8)) return x;
+    if (ao_abs(skmu) < AO_K(1.0e-8) || ao_abs(skms) < AO_K(1.0e-8)) return x;
     const R rs = rk * skmu / skms;
     const R sh = sM - unk;
     x[0] = rs;
@@ -306,14 +311,14 @@ template <class R> R apply_limiter(int lim, R a, R b) {
     const R e2 = AO_K(1.0e-12) * (AO_K(1.0) + a * a + b * b);
     const R num = (b * b + AO_K(2.0) * a * b + e2) * a;
     const R den = a * a + AO_K(2.0) * b * b + a * b + e2;
-    if (std::abs(den) <= AO_K(1.0e-20)) return R(0);
+    if (ao_abs(den) <= AO_K(1.0e-20)) return R(0);
     const R x = num / den;
-    if (!std::isfinite(x)) return R(0);
+    if (!ao_isfinite(x)) return R(0);
     if (a > R(0)) return sclamp(x, R(0), a);
     return sclamp(x, a, R(0));
   }
   if (a * b <= R(0)) return R(0);
-  return (std::abs(a) < std::abs(b)) ? a : b;
+  return (ao_abs(a) < ao_abs(b)) ? a : b;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -498,7 +503,7 @@ template <class R> struct PrecondCfg {  // preconditioning.hpp LowMachPreconditi
 
 template <class R> R local_mach(const Prim<R>& p, R g) {  // preconditioning.cpp:141-146
   const R a = smax(speed_of_sound(p, g), AO_K(1.0e-8));
-  const R vm = std::sqrt(p.u * p.u + p.v * p.v + p.w * p.w);
+  const R vm = ao_sqrt(p.u * p.u + p.v * p.v + p.w * p.w);
   return vm / a;
 }
 
@@ -564,7 +569,7 @@ template <class R> struct AssemblyCfg {  // residual_assembly.hpp:12-31 (pointer
 
 template <class R> R smooth_max(R a, R b) {  // residual_assembly.cpp:12-20
   const R d = a - b;
-  return AO_K(0.5) * (a + b + std::sqrt(d * d + AO_K(1.0e-6) * AO_K(1.0e-6)));
+  return AO_K(0.5) * (a + b + ao_sqrt(d * d + AO_K(1.0e-6) * AO_K(1.0e-6)));
 }
 
 template <class R> R face_mach(const Cons<R>& l, const Cons<R>& r, R g) {  // residual_assembly.cpp:22-29
@@ -590,8 +595,8 @@ R acoustic_scale(const AssemblyCfg<R>& c, const Cons<R>& l, const Cons<R>& r, R 
 }
 
 template <class R> bool valid_prim(const Prim<R>& p) {  // residual_assembly.cpp:53-57
-  return std::isfinite(p.rho) && std::isfinite(p.u) && std::isfinite(p.v) && std::isfinite(p.w) &&
-         std::isfinite(p.p) && p.rho > AO_K(1.0e-8) && p.p > AO_K(1.0e-8);
+  return ao_isfinite(p.rho) && ao_isfinite(p.u) && ao_isfinite(p.v) && ao_isfinite(p.w) &&
+         ao_isfinite(p.p) && p.rho > AO_K(1.0e-8) && p.p > AO_K(1.0e-8);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -613,7 +618,7 @@ Cons<R> precond_farfield_state(const Cons<R>& in, const Cons<R>& ff, const A3<R>
   const R rref = smax(AO_K(0.5) * (pi.rho + pf.rho), AO_K(1.0e-8));
   const R aref = smax(AO_K(0.5) * (ai + af), AO_K(1.0e-8));
   const R at = smax(sclamp(s, AO_K(1.0e-4), AO_K(1.0)) * aref, AO_K(1.0e-6));
-  if (std::abs(uni) >= at) return (uni >= R(0)) ? in : ff;
+  if (ao_abs(uni) >= at) return (uni >= R(0)) ? in : ff;
   const R rp = uni + pi.p / (rref * at);
   const R rm = unf - pf.p / (rref * at);
   const R unb = AO_K(0.5) * (rp + rm);
@@ -719,8 +724,8 @@ void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std:
       (*res)[static_cast<std::size_t>(5 * o + k)] += fa;
       if (nb >= 0) (*res)[static_cast<std::size_t>(5 * nb + k)] -= fa;
       if (abs_sum) {
-        (*abs_sum)[static_cast<std::size_t>(5 * o + k)] += std::abs(fa);
-        if (nb >= 0) (*abs_sum)[static_cast<std::size_t>(5 * nb + k)] += std::abs(fa);
+        (*abs_sum)[static_cast<std::size_t>(5 * o + k)] += ao_abs(fa);
+        if (nb >= 0) (*abs_sum)[static_cast<std::size_t>(5 * nb + k)] += ao_abs(fa);
       }
     }
     if (nb >= 0) (*lam)[nb] += wave;
@@ -739,7 +744,7 @@ template <class R> std::vector<R> dt_over_volume(const std::vector<R>& lam, R cf
   }
   R gdt = std::numeric_limits<R>::max();
   for (R x : lam) gdt = smin(gdt, cfl / smax(x, AO_K(1.0e-8)));
-  if (!std::isfinite(gdt)) gdt = R(0);
+  if (!ao_isfinite(gdt)) gdt = R(0);
   std::fill(dt.begin(), dt.end(), gdt);
   return dt;
 }
@@ -796,7 +801,7 @@ template <class R> Norms<R> residual_norms(const MeshView<R>& m, const std::vect
     const R iv = AO_K(1.0) / smax(m.volume[c], AO_K(1.0e-12));
     for (int k = 0; k < 5; ++k) {
       const R v = res[static_cast<std::size_t>(5 * c + k)] * iv;
-      const double mag = std::abs(static_cast<double>(v));
+      const double mag = ao_abs(static_cast<double>(v));
       l1 += mag;
       l2s += mag * mag;
       linf = std::max(linf, mag);
@@ -805,9 +810,9 @@ template <class R> Norms<R> residual_norms(const MeshView<R>& m, const std::vect
   }
   Norms<R> n;
   n.l1 = static_cast<R>(l1);
-  n.l2 = static_cast<R>(std::sqrt(l2s / std::max(1, m.num_cells * 5)));
+  n.l2 = static_cast<R>(ao_sqrt(l2s / std::max(1, m.num_cells * 5)));
   n.linf = static_cast<R>(linf);
-  n.rho_l2 = static_cast<R>(std::sqrt(r2 / std::max(1, m.num_cells)));
+  n.rho_l2 = static_cast<R>(ao_sqrt(r2 / std::max(1, m.num_cells)));
   return n;
 }
 
@@ -927,9 +932,9 @@ template <class R> class Monitor {
       b2 += win_[i].cm;
     }
     const R inv = AO_K(1.0) / static_cast<R>(req);
-    const R d0 = std::abs(b0 * inv - a0 * inv);
-    const R d1 = std::abs(b1 * inv - a1 * inv);
-    const R d2 = std::abs(b2 * inv - a2 * inv);
+    const R d0 = ao_abs(b0 * inv - a0 * inv);
+    const R d1 = ao_abs(b1 * inv - a1 * inv);
+    const R d2 = ao_abs(b2 * inv - a2 * inv);
     return d0 <= c_.drift_tol && d1 <= c_.drift_tol && d2 <= c_.drift_tol;
   }
   bool converged(int iter, R ratio) const {  // :119-127
@@ -951,15 +956,15 @@ template <class R> int normalize_face_normals(int nf, R* normal, R tol) {
   int invalid = 0;
   for (int f = 0; f < nf; ++f) {
     const R nx = normal[3 * f + 0], ny = normal[3 * f + 1], nz = normal[3 * f + 2];
-    const R nrm = std::sqrt(nx * nx + ny * ny + nz * nz);
-    if (!std::isfinite(nrm) || nrm < AO_K(1.0e-12)) {
+    const R nrm = ao_sqrt(nx * nx + ny * ny + nz * nz);
+    if (!ao_isfinite(nrm) || nrm < AO_K(1.0e-12)) {
       ++invalid;
       normal[3 * f + 0] = AO_K(1.0);
       normal[3 * f + 1] = R(0);
       normal[3 * f + 2] = R(0);
       continue;
     }
-    const R dev = std::abs(nrm - AO_K(1.0));
+    const R dev = ao_abs(nrm - AO_K(1.0));
     if (dev > tol) {
       const R inv = AO_K(1.0) / nrm;
       normal[3 * f + 0] = nx * inv;

@@ -400,3 +400,90 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
 
 AO_EXPORT_ALL(f32, float)
 AO_EXPORT_ALL(f64, double)
+
+// ---- tangent (dual-number) evaluation of the FP64 residual ---------------------------------------
+#include "oracle_dual.hpp"
+
+// A parameter direction: freestream-dependent options and geometry (reference layout; NULL = zero).
+// The layout equals the C ABI's aero_tangent_seed.
+typedef struct ao_tangent_seed {
+  double d_farfield[5];
+  double d_stab_floor, d_mach_ref, d_mach_min, d_beta_min, d_beta_max;
+  const double* d_normal;
+  const double* d_face_center;
+  const double* d_area;
+  const double* d_cell_center;
+  const double* d_volume;
+} ao_tangent_seed;
+
+extern "C" int ao_sizeof_tangent_seed(void) { return static_cast<int>(sizeof(ao_tangent_seed)); }
+
+// R and dR = R_U dU + R_p seed (dU and seed may be NULL) of the FP64 oracle residual.
+extern "C" void ao_residual_jvp_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, const double* dU,
+                                    const ao_tangent_seed* seed, double* res, double* dres) {
+  using D = ODual;
+  const int N = m->num_cells, F = m->num_faces;
+  auto dual_vec = [](const void* val, const double* der, std::size_t n) {
+    const double* v = static_cast<const double*>(val);
+    std::vector<D> out(n);
+    for (std::size_t i = 0; i < n; ++i) out[i] = D(v[i], der ? der[i] : 0.0);
+    return out;
+  };
+  const std::vector<D> normal = dual_vec(m->normal, seed ? seed->d_normal : nullptr, 3 * static_cast<std::size_t>(F));
+  const std::vector<D> fcen =
+    dual_vec(m->face_center, seed ? seed->d_face_center : nullptr, 3 * static_cast<std::size_t>(F));
+  const std::vector<D> area = dual_vec(m->area, seed ? seed->d_area : nullptr, static_cast<std::size_t>(F));
+  const std::vector<D> vol = dual_vec(m->volume, seed ? seed->d_volume : nullptr, static_cast<std::size_t>(N));
+  const std::vector<D> ccen =
+    dual_vec(m->cell_center, seed ? seed->d_cell_center : nullptr, 3 * static_cast<std::size_t>(N));
+  const std::vector<D> u = dual_vec(U, dU, 5 * static_cast<std::size_t>(N));
+  MeshView<D> v;
+  v.num_cells = N;
+  v.num_faces = F;
+  v.owner = m->owner;
+  v.neighbor = m->neighbor;
+  v.bc = m->bc;
+  v.normal = normal.data();
+  v.face_center = fcen.data();
+  v.area = area.data();
+  v.volume = vol.data();
+  v.cell_center = ccen.data();
+  v.wall_start = m->wall_start;
+  v.wall_count = m->wall_count;
+  v.gradient = m->gradient;
+  const double z[5] = {0, 0, 0, 0, 0};
+  const double* df = seed ? seed->d_farfield : z;
+  AssemblyCfg<D> a;
+  a.farfield = Cons<D>{D(c->farfield[0], df[0]), D(c->farfield[1], df[1]), D(c->farfield[2], df[2]),
+                       D(c->farfield[3], df[3]), D(c->farfield[4], df[4])};
+  a.gamma = D(c->gamma);
+  a.scheme = c->scheme;
+  a.limiter = c->limiter;
+  a.second_order = c->second_order != 0;
+  a.low_mach_fix = c->low_mach_fix != 0;
+  a.low_mach_cutoff = D(c->low_mach_cutoff);
+  a.stab_mach_floor = D(c->stab_mach_floor, seed ? seed->d_stab_floor : 0.0);
+  a.precond_farfield_bc = c->precond_farfield_bc != 0;
+  a.precond.enabled = c->precond_enabled != 0;
+  a.precond.mach_ref = D(c->mach_ref, seed ? seed->d_mach_ref : 0.0);
+  a.precond.mach_min = D(c->mach_min, seed ? seed->d_mach_min : 0.0);
+  a.precond.beta_min = D(c->beta_min, seed ? seed->d_beta_min : 0.0);
+  a.precond.beta_max = D(c->beta_max, seed ? seed->d_beta_max : 0.0);
+  a.as_fix = c->as_fix != 0;
+  a.as_cutoff = D(c->as_cutoff);
+  a.as_fmin = D(c->as_fmin);
+  a.as_ramp = D(c->as_ramp);
+  const std::vector<Prim<D>> p = prims(N, u.data(), a.gamma);
+  std::vector<D> gr;
+  if (a.second_order)
+    cell_gradients(v, p, &gr);
+  else
+    gr.assign(static_cast<std::size_t>(N) * 10, D(0.0));
+  std::vector<D> r, l;
+  AssemblyDiag d;
+  assemble_residual(v, a, p, gr, &r, &l, &d);
+  for (std::size_t i = 0; i < r.size(); ++i) {
+    if (res) res[i] = r[i].v;
+    dres[i] = r[i].d;
+  }
+}

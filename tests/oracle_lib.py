@@ -138,3 +138,16 @@ def time_residual(om: OracleMesh, cfg, U, n_evals: int) -> float:
     dt = om.dtype
     U = np.ascontiguousarray(U, dt).reshape(om.N, 5)
     return getattr(oracle(), f"ao_time_residual_{suffix(dt)}")(C.byref(om.c), C.byref(cfg), ptr(U), n_evals)
+
+
+def residual_jvp(om: OracleMesh, cfg, U, dU=None, seed=None):
+    """FP64 oracle residual and its tangent dR = R_U dU + R_p . seed (seed: aero_hip.Seed or None)."""
+    assert om.dtype == np.float64
+    U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
+    dUa = None if dU is None else np.ascontiguousarray(dU, np.float64).reshape(om.N, 5)
+    R = np.zeros((om.N, 5))
+    dR = np.zeros((om.N, 5))
+    lib = oracle()
+    lib.ao_residual_jvp_f64(C.byref(om.c), C.byref(cfg), ptr(U), None if dUa is None else ptr(dUa),
+                            None if seed is None else seed.ptr(), ptr(R), ptr(dR))
+    return R, dR

@@ -1,0 +1,235 @@
+"""Device tangent and adjoint of the residual (no reference counterpart), through the C ABI.
+
+* tangent: device dR = R_U dU + R_p . seed against the oracle's dual-number tangent (which
+  tests/test_oracle_tangent.py pins to central differences), in the state, the freestream options
+  and the geometry;
+* adjoint: R_U^T psi against the transpose of the oracle's dense Jacobian on a small mesh, and the
+  dot-product identity <psi, R_U v> = <R_U^T psi, v> on a larger one;
+* objective gradient against central differences of the objective;
+* sensitivities: adjoint and tangent solves agree (dual consistency), and both agree with central
+  differences of converged solutions.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import helpers
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+COMBOS = [(0, 0, 0, 0, 0, 0, 0), (1, 1, 0, 0, 0, 0, 0), (0, 0, 1, 0, 0, 0, 0), (1, 1, 1, 0, 0, 0, 0),
+          (1, 1, 1, 1, 1, 1, 1), (1, 0, 1, 1, 0, 1, 0)]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import aero_hip
+    assert aero_hip.hip_available(), "no gfx950 device visible to the HIP backend"
+    return aero_hip
+
+
+def _mesh64():
+    return helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+
+
+def _state(m, mach=0.5, aoa=2.0, amp=0.05, seed=17):
+    from aero_hip import freestream
+    ff = freestream(mach, aoa)[1]
+    return ff, helpers.perturbed_state(m.cell_center, ff, amp, seed)
+
+
+def _maxrel(a, b):
+    scale = np.abs(b).max(axis=0, keepdims=True) + 1e-300
+    return (np.abs(a - b) / scale).max()
+
+
+@pytest.mark.parametrize("gradient", ["green_gauss", "least_squares"])
+def test_tangent_parity(hip, gradient):
+    m = _mesh64()
+    gk = 1 if gradient == "least_squares" else 0
+    om = ol.OracleMesh(m, np.float64, gradient=gk)
+    ff, U = _state(m)
+    rng = np.random.default_rng(4)
+    dU = rng.standard_normal(U.shape) * np.abs(U).max(axis=0) * 1e-3
+    dU[:, 3] = 0.0
+    dn = rng.standard_normal(m.normal.shape) * 1e-2
+    dn[:, 2] = 0.0
+    dfc = rng.standard_normal(m.face_center.shape) * 1e-3
+    dfc[:, 2] = 0.0
+    dcc = rng.standard_normal(m.cell_center.shape) * 1e-3
+    dcc[:, 2] = 0.0
+    seed = hip.Seed(d_farfield=[1e-3, 2.0, -3.0, 0.0, 400.0], d_stab_floor=0.3, d_mach_ref=0.2, d_beta_min=0.1,
+                    d_face_normal=dn, d_face_center=dfc, d_face_area=rng.standard_normal(m.num_faces) * 1e-3,
+                    d_cell_center=dcc, d_cell_volume=rng.standard_normal(m.num_cells) * 1e-4)
+    with hip.HipMesh(m, gradient=gradient) as hm:
+        for combo in COMBOS:
+            cfg = helpers.res_cfg(cases.residual_cfg_values(combo, ff))
+            say(f"[tangent] {gradient} combo={combo}")
+            for s in (None, seed):
+                Rh, dRh = hm.residual_jvp(cfg, U, dU, s)
+                Ro, dRo = ol.residual_jvp(om, cfg, U, dU, s)
+                Rp, _, _ = hm.residual(U, cfg)
+                comp = [0, 1, 2, 4]
+                assert np.array_equal(Rh[:, comp], Rp[:, comp]), "tangent value part differs from the residual"
+                err = _maxrel(dRh[:, comp], dRo[:, comp])
+                assert err <= 1e-11, (combo, s is not None, err)
+
+
+def _dense_jacobian(om, cfg, U):
+    """d R[:, (0,1,2,4)] / d U[:, (0,1,2,4)] of the oracle, column by column from its tangent."""
+    N = om.N
+    comp = [0, 1, 2, 4]
+    J = np.zeros((4 * N, 4 * N))
+    for c in range(N):
+        for j, cj in enumerate(comp):
+            dU = np.zeros((N, 5))
+            dU[c, cj] = 1.0
+            _, dR = ol.residual_jvp(om, cfg, U, dU)
+            J[:, 4 * c + j] = dR[:, comp].ravel()
+    return J
+
+
+@pytest.mark.parametrize("gradient", ["green_gauss", "least_squares"])
+def test_adjoint_matches_dense_oracle_transpose(hip, gradient):
+    m = helpers.mesh("40x8")
+    gk = 1 if gradient == "least_squares" else 0
+    om = ol.OracleMesh(m, np.float64, gradient=gk)
+    ff, U = _state(m, 0.4, 1.0, 0.05, 3)
+    rng = np.random.default_rng(8)
+    psi = rng.standard_normal((m.num_cells, 5))
+    psi[:, 3] = 0.0
+    comp = [0, 1, 2, 4]
+    with hip.HipMesh(m, gradient=gradient) as hm:
+        for combo in [(0, 0, 1, 0, 0, 0, 0), (1, 1, 1, 0, 0, 0, 0), (1, 1, 1, 1, 1, 1, 1)]:
+            cfg = helpers.res_cfg(cases.residual_cfg_values(combo, ff))
+            say(f"[adjoint dense] {gradient} combo={combo}")
+            J = _dense_jacobian(om, cfg, U)
+            ref = (J.T @ psi[:, comp].ravel()).reshape(m.num_cells, 4)
+            got = hm.residual_vjp(cfg, psi, U)
+            assert np.all(got[:, 3] == 0.0)
+            err = _maxrel(got[:, comp], ref)
+            assert err <= 1e-11, (combo, err)
+
+
+@pytest.mark.parametrize("gradient", ["green_gauss", "least_squares"])
+def test_adjoint_dot_product_identity(hip, gradient):
+    m = helpers.mesh("256x64", cells="mixed", quad_layers=12, jitter=0.1, seed=12345, rcm=True)
+    ff, U = _state(m, 0.5, 1.25, 0.02, 20260101)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(U.shape) * np.abs(U).max(axis=0) * 1e-3
+    v[:, 3] = 0.0
+    psi = rng.standard_normal(U.shape)
+    psi[:, 3] = 0.0
+    with hip.HipMesh(m, gradient=gradient) as hm:
+        for combo in COMBOS:
+            cfg = helpers.res_cfg(cases.residual_cfg_values(combo, ff))
+            say(f"[adjoint dot] {gradient} combo={combo}")
+            _, Jv = hm.residual_jvp(cfg, U, v)
+            JTpsi = hm.residual_vjp(cfg, psi, U)
+            a = float(np.sum(psi * Jv))
+            b = float(np.sum(JTpsi * v))
+            scale = float(np.sum(np.abs(psi * Jv)))
+            assert abs(a - b) <= 1e-12 * scale, (combo, a, b)
+
+
+def test_objective_and_gradient(hip):
+    """J against the oracle's forces on the same state; dJ/dU against central differences of J."""
+    m = helpers.mesh("64x14")
+    ff, U = _state(m, 0.5, 3.0, 0.05, 2)
+    case = hip.CaseConfig(mach=0.5, aoa_deg=3.0)
+    _, _, obj, _ = hip.case_linearization(case, param="none")
+    obj.w_cl, obj.w_cd, obj.w_cm = 1.0, 0.7, -0.3
+    with hip.HipMesh(m) as hm:
+        out = hm.objective(obj, U)
+        # forces of the oracle (forces.cpp:208-294) from the same pressures
+        om = ol.OracleMesh(m, np.float64)
+        g = 1.4
+        inv = 1.0 / U[:, 0]
+        u, v = U[:, 1] * inv, U[:, 2] * inv
+        p = (g - 1) * (U[:, 4] - 0.5 * U[:, 0] * (u * u + v * v))
+        import ctypes as C
+        rho_inf, speed = ff[0], np.hypot(ff[1], ff[2]) / ff[0]
+        ref7 = np.array([rho_inf, obj.p_inf, obj.aoa_deg, speed, obj.chord, obj.x_ref, obj.y_ref])
+        f8 = np.zeros(8)
+        ol.oracle().ao_forces_f64(C.byref(om.c), ol.ptr(np.ascontiguousarray(p)), ol.ptr(ref7), 1, ol.ptr(f8))
+        Jo = 1.0 * f8[5] + 0.7 * f8[6] - 0.3 * f8[7]
+        assert abs(out["J"] - Jo) <= 1e-9 * abs(Jo), (out["J"], Jo)
+        g_dev = out["dJdU"]
+        wall_cells = np.unique(m.owner[m.wall_start:m.wall_start + m.wall_count])
+        rng = np.random.default_rng(3)
+        for c in rng.choice(wall_cells, 6, replace=False):
+            for k in (0, 1, 2, 4):
+                h = 1e-6 * abs(U[c, k]) + 1e-9
+                Up, Um = U.copy(), U.copy()
+                Up[c, k] += h
+                Um[c, k] -= h
+                fd = (hm.objective(obj, Up, gradient=False)["J"] - hm.objective(obj, Um, gradient=False)["J"]) / (2 * h)
+                assert abs(g_dev[c, k] - fd) <= 1e-6 * np.abs(g_dev[c]).max() + 1e-12, (c, k, g_dev[c, k], fd)
+        interior = np.setdiff1d(np.arange(m.num_cells), wall_cells)
+        assert np.all(g_dev[interior] == 0.0)
+        # explicit partials: dJ/daoa at fixed state, dJ/dq
+        h = 1e-4
+        o2 = hip.case_linearization(case, param="none")[2]
+        o2.w_cl, o2.w_cd, o2.w_cm = 1.0, 0.7, -0.3
+        o2.aoa_deg = obj.aoa_deg + h
+        Jp = hm.objective(o2, U, gradient=False)["J"]
+        o2.aoa_deg = obj.aoa_deg - h
+        Jm = hm.objective(o2, U, gradient=False)["J"]
+        assert abs(out["dJ_daoa"] - (Jp - Jm) / (2 * h)) <= 1e-7 * abs(out["dJ_daoa"]) + 1e-12
+
+
+def _converged_case(hip, m, aoa, mach, iters):
+    case = hip.CaseConfig(mach=mach, aoa_deg=aoa, flux_scheme=0, limiter=0, cfl_start=0.5, cfl_max=1.5,
+                          cfl_ramp_iters=50, startup_first_order_iters=10 ** 6, residual_reduction_target=0.0,
+                          force_stability_tol=0.0, min_iterations=1, iterations=iters, low_mach_fix=0)
+    return case
+
+
+def test_sensitivities_adjoint_tangent_and_finite_differences(hip):
+    """First-order Rusanov on the 40x8 reference mesh converges to round-off; there dCL/dalpha and
+    dCL/dMach from the adjoint equal the tangent solve's (dual consistency) and central differences of
+    converged CL."""
+    m = helpers.mesh("40x8")
+    iters = 9000
+    with hip.HipMesh(m) as hm:
+        case = _converged_case(hip, m, 2.0, 0.5, iters)
+        say("[sens] primal")
+        out = hm.run_case(case)
+        assert out["history"][-1, 8] / out["history"][0, 8] < 1e-11
+        U = out["conserved"]
+        results = {}
+        for param in ("aoa", "mach"):
+            rc, seed, obj, dq = hip.case_linearization(case, param=param)
+            obj.w_cl = 1.0
+            ob = hm.objective(obj, U, seed=None)
+            J_p = ob["dJ_daoa"] if param == "aoa" else ob["dJ_dq"] * dq
+            _, R_p = hm.residual_jvp(rc, U, None, seed)
+            say(f"[sens] adjoint {param}")
+            psi, ai = hm.adjoint_solve(rc, ob["dJdU"], U, cfl=1.5, max_iters=20000, tol=1e-12)
+            assert ai["res"] <= 1e-11 * ai["res0"], ai
+            d_adj = J_p - float(np.sum(psi * R_p))
+            say(f"[sens] tangent {param}")
+            v, ti = hm.tangent_solve(rc, seed, U, cfl=1.5, max_iters=20000, tol=1e-12)
+            assert ti["res"] <= 1e-11 * ti["res0"], ti
+            d_tan = J_p + float(np.sum(ob["dJdU"] * v))
+            assert abs(d_adj - d_tan) <= 1e-8 * abs(d_adj), (param, d_adj, d_tan)
+            results[param] = d_adj
+        h = {"aoa": 1e-3, "mach": 1e-4}
+        for param in ("aoa", "mach"):
+            cls = []
+            for s in (1, -1):
+                kw = dict(aoa=2.0 + (s * h[param] if param == "aoa" else 0.0),
+                          mach=0.5 + (s * h[param] if param == "mach" else 0.0))
+                say(f"[sens] finite difference {param} {s:+d}")
+                o = hm.run_case(_converged_case(hip, m, kw["aoa"], kw["mach"], iters))
+                cls.append(o["cl"])
+            fd = (cls[0] - cls[1]) / (2 * h[param])
+            assert abs(results[param] - fd) <= 1e-5 * abs(fd), (param, results[param], fd)

@@ -195,11 +195,13 @@ def test_lsq_gradients_and_residual_parity(hip):
 
 
 def test_lsq_run_case_parity(hip):
-    """A full driver run with least-squares gradients against the oracle's driver."""
+    """A full driver run with least-squares gradients against the oracle's driver (HLLC + venkat: with
+    Rusanov + minmod the least-squares run of this case diverges after ~220 iterations, identically in
+    the oracle and on the device)."""
     run = cases.RUNS["regression"]
     m = helpers.mesh(run["mesh"])
     om = ol.OracleMesh(m, np.float64, gradient=1)
-    over = dict(run["cfg"], iterations=300)
+    over = dict(run["cfg"], iterations=300, flux_scheme=1, limiter=1)
     cfg = helpers.case_cfg(over, run["env_precond"])
     say("[lsq] run_case regression 300 iterations")
     ref = ol.run_case(om, cfg, 300)
