@@ -3,11 +3,12 @@
 Importing the package loads libaero_hip.so (the HIP kernels + C ABI); a missing library raises.
 """
 from . import _lib
-from .mesh import Mesh, generate_mesh
+from . import adjoint
+from .mesh import Mesh, deform_mesh, generate_mesh
 from .solver import (CaseConfig, HipBatch, HipMesh, Seed, case_linearization, freestream, hip_available,
                      residual_config)
 
-__all__ = ["Mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "Seed", "case_linearization", "freestream",
-           "hip_available", "residual_config", "_lib"]
+__all__ = ["Mesh", "deform_mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "Seed", "case_linearization", "freestream",
+           "hip_available", "residual_config", "adjoint", "_lib"]
 
 _lib.lib()

@@ -40,7 +40,7 @@ class ResidualConfig(C.Structure):
                 ("stab_mach_floor", f64), ("precond_farfield_bc", i32), ("precond_enabled", i32),
                 ("precond_mach_ref", f64), ("precond_mach_min", f64), ("precond_beta_min", f64),
                 ("precond_beta_max", f64), ("all_speed_fix", i32), ("all_speed_cutoff", f64),
-                ("all_speed_f_min", f64), ("all_speed_ramp", f64)]
+                ("all_speed_f_min", f64), ("all_speed_ramp", f64), ("limiter_frozen", i32)]
 
 
 class ResidualDiag(C.Structure):
@@ -70,6 +70,7 @@ CASE_FIELDS = [
     ("all_speed_cfl_drop_factor", f64), ("all_speed_cfl_restore_factor", f64),
     ("all_speed_cfl_min_scale", f64), ("aoa0_symmetry_enforce", i32),
     ("aoa0_symmetry_enforce_interval", i32), ("precond_experimental_enabled", i32),
+    ("limiter_freeze_iter", i32),
 ]
 
 
@@ -121,7 +122,17 @@ class Objective(C.Structure):
 
 class LinearSolve(C.Structure):
     _fields_ = [("rk_stages", i32), ("cfl", f64), ("max_iters", i32), ("tol", f64), ("check_every", i32),
-                ("iterations", i32), ("res0", f64), ("res", f64), ("seconds", f64)]
+                ("gmres_restart", i32), ("gmres_sweeps", i32), ("iterations", i32), ("res0", f64), ("res", f64),
+                ("seconds", f64)]
+
+
+class ShapeConfig(C.Structure):
+    _fields_ = [("bumps_per_surface", i32), ("h_min", f64), ("h_max", f64), ("idw_exponent", f64)]
+
+
+class GeometryTangents(C.Structure):
+    _fields_ = [("n_params", i32), ("d_points", PD), ("d_face_normal", PD), ("d_face_center", PD),
+                ("d_face_area", PD), ("d_cell_center", PD), ("d_cell_volume", PD)]
 
 
 PARAMS = {"none": 0, "aoa": 1, "mach": 2}
@@ -153,6 +164,9 @@ PROTOTYPES = {
     "aero_mesh_generate": (C.c_int, [C.POINTER(MeshGenConfig), i32, C.POINTER(MeshArrays), C.c_char_p,
                                      C.c_size_t]),
     "aero_mesh_free": (None, [C.POINTER(MeshArrays)]),
+    "aero_mesh_deform": (C.c_int, [C.POINTER(MeshArrays), C.POINTER(ShapeConfig), PD, C.POINTER(MeshArrays),
+                                   C.POINTER(GeometryTangents), C.c_char_p, C.c_size_t]),
+    "aero_hip_freeze_limiter": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.c_char_p, C.c_size_t]),
     "aero_case_linearization": (C.c_int, [C.POINTER(CaseConfigC), i32, i32, C.POINTER(ResidualConfig),
                                           C.POINTER(TangentSeed), C.POINTER(Objective), PD, C.c_char_p, C.c_size_t]),
     "aero_hip_residual_jvp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(TangentSeed), PD, PD,

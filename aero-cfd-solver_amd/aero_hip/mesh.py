@@ -67,26 +67,80 @@ def generate_mesh(naca: str = "0012", num_circumferential: int = 160, num_radial
     err = _lib.errbuf()
     _lib.check(lib.aero_mesh_generate(C.byref(cfg), precision, C.byref(out), err, len(err)), err, "aero_mesh_generate")
     try:
-        ft = np.float32 if precision == 4 else np.float64
-        ct = C.c_float if precision == 4 else C.c_double
-        N, F, P = out.num_cells, out.num_faces, out.num_points
-
-        def ia(p, n):
-            return np.ctypeslib.as_array(p, shape=(n,)).copy()
-
-        def fa(p, n):
-            return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).astype(ft, copy=True)
-
-        offs = ia(out.cell_offsets, N)
-        m = Mesh(N, F, ia(out.face_owner, F), ia(out.face_neighbor, F), ia(out.face_bc, F),
-                 ia(out.face_vertices, 2 * F).reshape(F, 2), fa(out.face_normal, 3 * F).reshape(F, 3),
-                 fa(out.face_center, 3 * F).reshape(F, 3), fa(out.face_area, F), fa(out.cell_volume, N),
-                 fa(out.cell_center, 3 * N).reshape(N, 3), fa(out.points, 3 * P).reshape(P, 3), offs,
-                 ia(out.cell_connectivity, int(offs[-1]) if N else 0), out.wall_start, out.wall_count,
-                 out.farfield_start, out.farfield_count,
-                 dict(naca=naca, num_circumferential=num_circumferential, num_radial=num_radial,
-                      farfield_radius=farfield_radius, radial_stretch=radial_stretch, chord=chord, cells=cells,
-                      quad_layers=quad_layers, jitter=jitter, seed=seed, rcm=rcm))
+        m = _from_arrays(out, dict(naca=naca, num_circumferential=num_circumferential, num_radial=num_radial,
+                                   farfield_radius=farfield_radius, radial_stretch=radial_stretch, chord=chord,
+                                   cells=cells, quad_layers=quad_layers, jitter=jitter, seed=seed, rcm=rcm))
     finally:
         lib.aero_mesh_free(C.byref(out))
     return m
+
+
+def _from_arrays(out: "_lib.MeshArrays", meta: dict) -> Mesh:
+    ft = np.float32 if out.precision == 4 else np.float64
+    ct = C.c_float if out.precision == 4 else C.c_double
+    N, F, P = out.num_cells, out.num_faces, out.num_points
+
+    def ia(p, n):
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def fa(p, n):
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).astype(ft, copy=True)
+
+    offs = ia(out.cell_offsets, N)
+    return Mesh(N, F, ia(out.face_owner, F), ia(out.face_neighbor, F), ia(out.face_bc, F),
+                ia(out.face_vertices, 2 * F).reshape(F, 2), fa(out.face_normal, 3 * F).reshape(F, 3),
+                fa(out.face_center, 3 * F).reshape(F, 3), fa(out.face_area, F), fa(out.cell_volume, N),
+                fa(out.cell_center, 3 * N).reshape(N, 3), fa(out.points, 3 * P).reshape(P, 3), offs,
+                ia(out.cell_connectivity, int(offs[-1]) if N else 0), out.wall_start, out.wall_count,
+                out.farfield_start, out.farfield_count, meta)
+
+
+def _to_arrays(m: Mesh):
+    """aero_mesh_arrays over an FP64 mesh's arrays (returns the struct and the arrays to keep alive)."""
+    keep = [np.ascontiguousarray(a) for a in (m.owner, m.neighbor, m.bc, m.face_vertices)]
+    keep += [np.ascontiguousarray(a, dtype=np.float64) for a in (m.normal, m.face_center, m.area, m.volume,
+                                                                  m.cell_center, m.points)]
+    keep += [np.ascontiguousarray(a, dtype=np.int32) for a in (m.cell_offsets, m.cell_connectivity)]
+    PI = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    arr = _lib.MeshArrays(8, m.num_cells, m.num_faces, len(m.points), PI(keep[0]), PI(keep[1]), PI(keep[2]),
+                          PI(keep[3]), vp(keep[4]), vp(keep[5]), vp(keep[6]), vp(keep[7]), vp(keep[8]), vp(keep[9]),
+                          PI(keep[10]), PI(keep[11]), m.wall_start, m.wall_count, m.farfield_start, m.farfield_count)
+    return arr, keep
+
+
+def deform_mesh(mesh: Mesh, amplitudes, bumps_per_surface: int = 10, h_min: float = 0.05, h_max: float = 0.95,
+                idw_exponent: float = 3.0, tangents: bool = True):
+    """Hicks-Henne bumps + IDW deformation (aero_mesh_deform): returns (deformed Mesh, tangents) where
+    tangents holds d(points, normal, face_center, area, cell_center, volume)/d(amplitude_i), shaped
+    [2 * bumps_per_surface, ...] like the mesh arrays, or None."""
+    if mesh.area.dtype != np.float64:
+        mesh = mesh.astype(np.float64)
+    P = 2 * bumps_per_surface
+    amp = np.ascontiguousarray(amplitudes, dtype=np.float64)
+    if amp.shape != (P,):
+        raise ValueError(f"need {P} amplitudes")
+    base, keep = _to_arrays(mesh)
+    shape = _lib.ShapeConfig(bumps_per_surface, h_min, h_max, idw_exponent)
+    tan = None
+    tc = None
+    if tangents:
+        N, F, NP = mesh.num_cells, mesh.num_faces, len(mesh.points)
+        tan = dict(points=np.zeros((P, NP, 3)), normal=np.zeros((P, F, 3)), face_center=np.zeros((P, F, 3)),
+                   area=np.zeros((P, F)), cell_center=np.zeros((P, N, 3)), volume=np.zeros((P, N)))
+        pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        tc = _lib.GeometryTangents(P, pd(tan["points"]), pd(tan["normal"]), pd(tan["face_center"]), pd(tan["area"]),
+                                   pd(tan["cell_center"]), pd(tan["volume"]))
+    out = _lib.MeshArrays()
+    lib = _lib.lib()
+    err = _lib.errbuf()
+    _lib.check(lib.aero_mesh_deform(C.byref(base), C.byref(shape), amp.ctypes.data_as(C.POINTER(C.c_double)),
+                                    C.byref(out), C.byref(tc) if tc is not None else None, err, len(err)),
+               err, "aero_mesh_deform")
+    try:
+        m = _from_arrays(out, dict(mesh.meta, shape=dict(amplitudes=amp.tolist(), bumps_per_surface=bumps_per_surface,
+                                                         h_min=h_min, h_max=h_max, idw_exponent=idw_exponent)))
+    finally:
+        lib.aero_mesh_free(C.byref(out))
+    del keep
+    return m, tan

@@ -78,6 +78,7 @@ class CaseConfig:
     aoa0_symmetry_enforce: int = 0
     aoa0_symmetry_enforce_interval: int = 0
     precond_experimental_enabled: int = -1  # -1: read AERO_ENABLE_EXPERIMENTAL_PRECOND like the reference
+    limiter_freeze_iter: int = -1  # beyond the reference: freeze the limiter at this iteration (-1: never)
 
     def resolved(self) -> "CaseConfig":
         c = CaseConfig(**asdict(self))
@@ -108,14 +109,16 @@ def freestream(mach: float, aoa_deg: float, p_inf: float = 101325.0, t_inf: floa
 def residual_config(farfield, gamma=1.4, flux="hllc", limiter="minmod", second_order=True, low_mach_fix=False,
                     low_mach_cutoff=0.3, stab_mach_floor=0.0, precond_farfield_bc=False, precond=False,
                     mach_ref=0.15, mach_min=1e-3, beta_min=0.2, beta_max=1.0, all_speed_fix=False,
-                    all_speed_cutoff=0.25, all_speed_f_min=0.05, all_speed_ramp=1.0) -> _lib.ResidualConfig:
+                    all_speed_cutoff=0.25, all_speed_f_min=0.05, all_speed_ramp=1.0,
+                    limiter_frozen=False) -> _lib.ResidualConfig:
     """EulerResidualAssemblyConfig scalars (solvers/euler/residual_assembly.hpp:12-31)."""
     f = lambda v: _lib.FLUX[v] if isinstance(v, str) else int(v)
     l = lambda v: _lib.LIMITER[v] if isinstance(v, str) else int(v)
     return _lib.ResidualConfig((C.c_double * 5)(*[float(x) for x in farfield]), gamma, f(flux), l(limiter),
                                int(bool(second_order)), int(bool(low_mach_fix)), low_mach_cutoff, stab_mach_floor,
                                int(bool(precond_farfield_bc)), int(bool(precond)), mach_ref, mach_min, beta_min,
-                               beta_max, int(bool(all_speed_fix)), all_speed_cutoff, all_speed_f_min, all_speed_ramp)
+                               beta_max, int(bool(all_speed_fix)), all_speed_cutoff, all_speed_f_min, all_speed_ramp,
+                               int(bool(limiter_frozen)))
 
 
 def _pd(a: np.ndarray):
@@ -237,6 +240,14 @@ class HipMesh:
                     wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)
 
 
+    def freeze_limiter(self, cfg: _lib.ResidualConfig, U=None) -> np.ndarray:
+        """Capture the limiter ratios at U (None: the handle's state); residual configs with
+        limiter_frozen = 1 use them.  Returns the [8, F] ratios."""
+        _, pu = self._state_ptr(U)
+        out = np.zeros((8, self.F))
+        self._call("aero_hip_freeze_limiter", C.byref(cfg), pu, _pd(out))
+        return out
+
     # ---- tangent / adjoint ------------------------------------------------------------------------
     def _state_ptr(self, U):
         if U is None:
@@ -271,11 +282,14 @@ class HipMesh:
         return dict(J=J.value, dJdU=g, dJ_daoa=da.value, dJ_dq=dq.value, dJ_seed=ds.value)
 
     def adjoint_solve(self, cfg: _lib.ResidualConfig, rhs, U=None, cfl: float = 1.0, rk_stages: int = 3,
-                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100):
-        """psi with R_U^T psi = rhs (transposed pseudo-time iteration)."""
+                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100, gmres_restart: int = 0,
+                      gmres_sweeps: int = 1):
+        """psi with R_U^T psi = rhs (transposed pseudo-time iteration, GMRES-accelerated when
+        gmres_restart > 0)."""
         _, pu = self._state_ptr(U)
         rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(self.N, 5)
-        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, 0, 0.0, 0.0, 0.0)
+        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, gmres_restart, gmres_sweeps, 0, 0.0,
+                               0.0, 0.0)
         psi = np.zeros((self.N, 5))
         hist = np.zeros(max_iters)
         self._call("aero_hip_adjoint_solve", C.byref(cfg), pu, _pd(rhs), C.byref(ctl), _pd(psi), _pd(hist))
@@ -283,10 +297,13 @@ class HipMesh:
                          history=hist[: ctl.iterations].copy())
 
     def tangent_solve(self, cfg: _lib.ResidualConfig, seed: "Seed", U=None, cfl: float = 1.0, rk_stages: int = 3,
-                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100):
-        """v with R_U v = -R_p . seed (linearised pseudo-time iteration)."""
+                      max_iters: int = 5000, tol: float = 1e-10, check_every: int = 100, gmres_restart: int = 0,
+                      gmres_sweeps: int = 1):
+        """v with R_U v = -R_p . seed (linearised pseudo-time iteration, GMRES-accelerated when
+        gmres_restart > 0)."""
         _, pu = self._state_ptr(U)
-        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, 0, 0.0, 0.0, 0.0)
+        ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, check_every, gmres_restart, gmres_sweeps, 0, 0.0,
+                               0.0, 0.0)
         v = np.zeros((self.N, 5))
         hist = np.zeros(max_iters)
         self._call("aero_hip_tangent_solve", C.byref(cfg), pu, seed.ptr(), C.byref(ctl), _pd(v), _pd(hist))

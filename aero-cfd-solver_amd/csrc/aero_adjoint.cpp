@@ -45,6 +45,14 @@ struct AdjointData {
   int hist_cap = 0;
   double* coef = nullptr;  // [wall_count]
   double* geo = nullptr;   // [5F + 3N] geometry tangents
+  // GMRES(m): Krylov basis [m + 1][4N], work vectors, dot-product scratch
+  double* V = nullptr;
+  int v_cap = 0;
+  double* gx = nullptr;
+  double* gc = nullptr;
+  double* gw = nullptr;
+  double* dots = nullptr;      // [m + 1]
+  double* dot_parts = nullptr;  // [512][m + 1]
 };
 
 namespace {
@@ -255,8 +263,7 @@ int download_internal(aero_hip_mesh* m, const double* src, double* aos5, char* e
   return AERO_OK;
 }
 
-FaceOptsT<Dual> dual_opts(const aero_residual_config* cfg, const aero_tangent_seed* seed) {
-  const FaceOpts o = face_opts(cfg);
+FaceOptsT<Dual> dual_opts(const FaceOpts& o, const aero_tangent_seed* seed) {
   FaceOptsT<Dual> d;
   const double z[5] = {0, 0, 0, 0, 0};
   const double* df = seed ? seed->d_farfield : z;
@@ -278,6 +285,7 @@ FaceOptsT<Dual> dual_opts(const aero_residual_config* cfg, const aero_tangent_se
   d.as_cut = o.as_cut;
   d.as_fmin = o.as_fmin;
   d.as_w = o.as_w;
+  d.frozen = o.frozen;
   return d;
 }
 
@@ -357,7 +365,9 @@ int enqueue_jacobian(aero_hip_mesh* m, const aero_residual_config* cfg, char* er
   int rc = ensure_jacobian_buffers(m, err, errlen);
   if (rc) return rc;
   const double* U = m->U[m->cur];
-  const FaceOpts o = face_opts(cfg);
+  FaceOpts o;
+  rc = mesh_face_opts(m, cfg, &o, err, errlen);
+  if (rc) return rc;
   if (cfg->second_order)
     AERO_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream), "k_gradient");
   AERO_TRY(aero_host::launch_face_jacobian(a->am, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, U, m->G, o,
@@ -453,6 +463,7 @@ int aero_case_linearization(const aero_case_config* c, int32_t iter, int32_t par
   rc->flux_scheme = c->flux_scheme;
   rc->limiter = c->limiter;
   rc->second_order = iter >= std::max(c->startup_first_order_iters, 0) ? 1 : 0;
+  rc->limiter_frozen = (rc->second_order && c->limiter_freeze_iter >= 0 && iter >= c->limiter_freeze_iter) ? 1 : 0;
   rc->low_mach_fix = c->low_mach_fix;
   rc->low_mach_cutoff = c->mach_cutoff;
   rc->stab_mach_floor = floor_cur.v;
@@ -517,8 +528,10 @@ int aero_hip_residual_jvp(aero_hip_mesh* m, const aero_residual_config* cfg, con
     if (rc) return rc;
   }
   GeoTan gt;
-  rc = geometry_tangent(m, seed, &gt, err, errlen);
-  if (!rc) rc = enqueue_jvp(m, cfg, dU ? a->dU : nullptr, gt, dual_opts(cfg, seed), err, errlen);
+  FaceOpts fo;
+  rc = mesh_face_opts(m, cfg, &fo, err, errlen);
+  if (!rc) rc = geometry_tangent(m, seed, &gt, err, errlen);
+  if (!rc) rc = enqueue_jvp(m, cfg, dU ? a->dU : nullptr, gt, dual_opts(fo, seed), err, errlen);
   if (!rc && residual) rc = download_internal(m, a->R2, residual, err, errlen);
   if (!rc) rc = download_internal(m, a->dR, dR, err, errlen);
   return rc;
@@ -634,20 +647,62 @@ int aero_hip_objective(aero_hip_mesh* m, const aero_objective* obj, const double
 
 namespace {
 
-// Shared pseudo-time loop of the adjoint (forward = 0) and tangent (forward = 1) solves.  The iterate
-// lives in a->x[]; on return a->x[start] holds the final iterate.
-int linear_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, const GeoTan& gt,
-                   const FaceOptsT<Dual>& od, aero_linear_solve* ctl, double* history, int* start_out, char* err,
-                   size_t errlen) {
+// One pseudo-time iteration of the linearised problem on the iterate ring a->x[]: x[start] holds the
+// input; on return *start_io indexes the output.  forcing = false drops the right-hand side (adjoint) or
+// the parameter derivative (tangent), leaving the linear map G.  partials_out (or null) receives the
+// squared norm of the stage-0 linear residual.
+int one_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, bool forcing, const GeoTan& gt,
+                  const FaceOptsT<Dual>& od, const aero_linear_solve* ctl, int* start_io, double* norm_out, char* err,
+                  size_t errlen) {
   AdjointData* a = m->adj;
   const FaceOpts o = face_opts(cfg);
   const int rk3 = ctl->rk_stages == 3 ? 1 : 0;
   const int nst = rk3 ? 3 : 1;
+  const double* U = m->U[m->cur];
+  int start = *start_io;
+  const double* x0 = a->x[start];
+  const double* xs = x0;
+  static const GeoTan no_geo{};
+  FaceOptsT<Dual> od_plain = od;  // without the parameter derivatives when forcing is off
+  if (!forcing) {
+    for (Dual* d : {&od_plain.ff.r, &od_plain.ff.ru, &od_plain.ff.rv, &od_plain.ff.E, &od_plain.stab_floor,
+                    &od_plain.mach_ref, &od_plain.mach_min, &od_plain.beta_min, &od_plain.beta_max})
+      d->d = 0.0;
+  }
+  for (int st = 0; st < nst; ++st) {
+    double* xo = rk3 ? a->x[st == 0 ? (start + 1) % 3 : (st == 1 ? (start + 2) % 3 : start)] : a->x[1 - start];
+    const double* r;
+    if (forward) {
+      const int rc = enqueue_jvp(m, cfg, xs, forcing ? gt : no_geo, od_plain, err, errlen);
+      if (rc) return rc;
+      r = a->dR;
+    } else {
+      AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, U, o, xs, a->psi, m->stream), "k_adj_scale");
+      AERO_TRY(aero_host::launch_adj_product(a->am, a->J, U, a->psi, cfg->gamma, a->T, a->Gbar, a->out, m->stream),
+               "adjoint product");
+      r = a->out;
+    }
+    const bool want = st == 0 && norm_out;
+    AERO_TRY(aero_host::launch_lin_update(a->am, st, rk3, forward, x0, xs, r, (forward || !forcing) ? nullptr : a->rhs,
+                                          m->lam, ctl->cfl, U, o, xo, want ? a->partials : nullptr, m->stream),
+             "k_lin_update");
+    if (want) AERO_TRY(aero_host::launch_partials_sum(a->nparts, a->partials, norm_out, m->stream), "k_partials_sum");
+    xs = xo;
+  }
+  if (!rk3) start = 1 - start;
+  *start_io = start;
+  return AERO_OK;
+}
+
+// Plain pseudo-time iteration.  On return a->x[*start_out] holds the final iterate.
+int plain_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, const GeoTan& gt,
+                  const FaceOptsT<Dual>& od, aero_linear_solve* ctl, double* history, int* start_out, char* err,
+                  size_t errlen) {
+  AdjointData* a = m->adj;
   const int maxit = std::max(ctl->max_iters, 1);
   const int every = std::max(ctl->check_every, 1);
   int rc = ensure_history(m, maxit, err, errlen);
   if (rc) return rc;
-  const double* U = m->U[m->cur];
   const size_t n4 = 4 * static_cast<size_t>(m->N);
   AERO_TRY(hipMemsetAsync(a->x[0], 0, n4 * sizeof(double), m->stream), "linear solve init");
   int start = 0;
@@ -658,30 +713,8 @@ int linear_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forwar
   while (done < maxit) {
     const int batch = std::min(every, maxit - done);
     for (int it = 0; it < batch; ++it) {
-      const double* x0 = a->x[start];
-      const double* xs = x0;
-      for (int st = 0; st < nst; ++st) {
-        double* xo = rk3 ? a->x[st == 0 ? (start + 1) % 3 : (st == 1 ? (start + 2) % 3 : start)] : a->x[1 - start];
-        const double* r;
-        if (forward) {
-          rc = enqueue_jvp(m, cfg, xs, gt, od, err, errlen);
-          if (rc) return rc;
-          r = a->dR;
-        } else {
-          AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, U, o, xs, a->psi, m->stream), "k_adj_scale");
-          AERO_TRY(aero_host::launch_adj_product(a->am, a->J, U, a->psi, cfg->gamma, a->T, a->Gbar, a->out, m->stream),
-                   "adjoint product");
-          r = a->out;
-        }
-        AERO_TRY(aero_host::launch_lin_update(a->am, st, rk3, forward, x0, xs, r, forward ? nullptr : a->rhs, m->lam,
-                                              ctl->cfl, U, o, xo, st == 0 ? a->partials : nullptr, m->stream),
-                 "k_lin_update");
-        if (st == 0)
-          AERO_TRY(aero_host::launch_partials_sum(a->nparts, a->partials, a->hist + done + it, m->stream),
-                   "k_partials_sum");
-        xs = xo;
-      }
-      if (!rk3) start = 1 - start;
+      rc = one_iteration(m, cfg, forward, true, gt, od, ctl, &start, a->hist + done + it, err, errlen);
+      if (rc) return rc;
     }
     AERO_TRY(hipMemcpyAsync(h.data() + done, a->hist + done, batch * sizeof(double), hipMemcpyDeviceToHost, m->stream),
              "linear solve history");
@@ -701,15 +734,191 @@ int linear_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forwar
   if (history) std::memcpy(history, h.data(), done * sizeof(double));
   *start_out = start;
   if (!std::isfinite(res)) {
-    set_err(err, errlen, "linear solve: non-finite residual (the linearised pseudo-time iteration diverged)");
+    set_err(err, errlen, "linear solve: non-finite residual (the linearised pseudo-time iteration diverged; "
+                         "use gmres_restart > 0)");
     return AERO_ENONFINITE;
   }
   return AERO_OK;
 }
 
+int ensure_krylov(aero_hip_mesh* m, int mdim, char* err, size_t errlen) {
+  AdjointData* a = m->adj;
+  const size_t n4 = 4 * static_cast<size_t>(m->N);
+  if (a->v_cap < mdim + 1) {
+    for (double** p : {&a->V, &a->dots, &a->dot_parts}) {
+      if (*p) {
+        (void)hipFree(*p);
+        a->allocs.erase(std::find(a->allocs.begin(), a->allocs.end(), static_cast<void*>(*p)));
+        *p = nullptr;
+      }
+    }
+    AERO_TRY(aalloc(a, &a->V, n4 * (mdim + 1)), "Krylov basis");
+    AERO_TRY(aalloc(a, &a->dots, static_cast<size_t>(mdim + 1)), "Krylov scratch");
+    AERO_TRY(aalloc(a, &a->dot_parts, static_cast<size_t>(512) * (mdim + 1)), "Krylov scratch");
+    a->v_cap = mdim + 1;
+  }
+  if (!a->gx) AERO_TRY(aalloc(a, &a->gx, n4), "Krylov vectors");
+  if (!a->gc) AERO_TRY(aalloc(a, &a->gc, n4), "Krylov vectors");
+  if (!a->gw) AERO_TRY(aalloc(a, &a->gw, n4), "Krylov vectors");
+  return AERO_OK;
+}
+
+// y = G^k x (forcing = false) or the k-fold iteration from x with the right-hand side (forcing = true),
+// k = gmres_sweeps, through the iterate ring.
+int apply_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, bool forcing, const GeoTan& gt,
+                    const FaceOptsT<Dual>& od, const aero_linear_solve* ctl, const double* x, double* y, char* err,
+                    size_t errlen) {
+  AdjointData* a = m->adj;
+  const size_t bytes = 4 * static_cast<size_t>(m->N) * sizeof(double);
+  if (x)
+    AERO_TRY(hipMemcpyAsync(a->x[0], x, bytes, hipMemcpyDeviceToDevice, m->stream), "iterate in");
+  else
+    AERO_TRY(hipMemsetAsync(a->x[0], 0, bytes, m->stream), "iterate in");
+  int start = 0;
+  for (int k = 0; k < std::max(ctl->gmres_sweeps, 1); ++k) {
+    const int rc = one_iteration(m, cfg, forward, forcing, gt, od, ctl, &start, nullptr, err, errlen);
+    if (rc) return rc;
+  }
+  AERO_TRY(hipMemcpyAsync(y, a->x[start], bytes, hipMemcpyDeviceToDevice, m->stream), "iterate out");
+  return AERO_OK;
+}
+
+int dots_to_host(aero_hip_mesh* m, long long n, const double* V, int nv, const double* w, double* host, char* err,
+                 size_t errlen) {
+  AdjointData* a = m->adj;
+  AERO_TRY(aero_host::launch_multidot(n, V, nv, w, a->dot_parts, a->dots, m->stream), "k_multidot");
+  AERO_TRY(hipMemcpyAsync(host, a->dots, nv * sizeof(double), hipMemcpyDeviceToHost, m->stream), "dots");
+  AERO_TRY(hipStreamSynchronize(m->stream), "dots");
+  return AERO_OK;
+}
+
+// Restarted GMRES(mdim) on (I - G) x = c, the fixed point of gmres_sweeps pseudo-time iterations
+// x -> G x + c;
+// classical Gram-Schmidt with one re-orthogonalisation, Givens rotations for the residual estimate.
+// The solution is left in a->gx.
+int gmres_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, const GeoTan& gt,
+                  const FaceOptsT<Dual>& od, aero_linear_solve* ctl, double* history, char* err, size_t errlen) {
+  AdjointData* a = m->adj;
+  const int mdim = ctl->gmres_restart;
+  int rc = ensure_krylov(m, mdim, err, errlen);
+  if (rc) return rc;
+  const long long n = 4LL * m->N;
+  const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+  const int maxit = std::max(ctl->max_iters, 1);
+  std::vector<double> hist;
+  const auto t0 = std::chrono::steady_clock::now();
+  AERO_TRY(hipMemsetAsync(a->gx, 0, bytes, m->stream), "gmres init");
+  rc = apply_iteration(m, cfg, forward, true, gt, od, ctl, nullptr, a->gc, err, errlen);  // c = G 0 + c
+  if (rc) return rc;
+  double res0 = -1.0, res = 0.0;
+  int total = 0;
+  std::vector<double> H(static_cast<size_t>(mdim + 1) * mdim), cs(mdim), sn(mdim), gv(mdim + 1), h(mdim + 1),
+    h2(mdim + 1), y(mdim);
+  bool done = false;
+  while (!done) {
+    // r = c - (x - G x) = G x + c - x
+    rc = apply_iteration(m, cfg, forward, false, gt, od, ctl, a->gx, a->gw, err, errlen);
+    if (rc) return rc;
+    AERO_TRY(aero_host::launch_axpby(n, -1.0, a->gx, 1.0, a->gw, m->stream), "gmres residual");
+    AERO_TRY(aero_host::launch_axpby(n, 1.0, a->gc, 1.0, a->gw, m->stream), "gmres residual");
+    double beta2 = 0.0;
+    rc = dots_to_host(m, n, a->gw, 1, a->gw, &beta2, err, errlen);
+    if (rc) return rc;
+    const double beta = std::sqrt(beta2);
+    if (res0 < 0.0) res0 = beta;
+    res = beta;
+    if (!std::isfinite(beta)) break;
+    if (beta <= ctl->tol * res0 || total >= maxit) break;
+    AERO_TRY(aero_host::launch_axpby(n, 1.0 / beta, a->gw, 0.0, a->V, m->stream), "gmres basis");
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int jend = -1;
+    for (int j = 0; j < mdim; ++j) {
+      double* vj = a->V + static_cast<size_t>(j) * n;
+      rc = apply_iteration(m, cfg, forward, false, gt, od, ctl, vj, a->gw, err, errlen);
+      if (rc) return rc;
+      AERO_TRY(aero_host::launch_axpby(n, 1.0, vj, -1.0, a->gw, m->stream), "gmres operator");  // w = v - G v
+      rc = dots_to_host(m, n, a->V, j + 1, a->gw, h.data(), err, errlen);
+      if (rc) return rc;
+      for (int i = 0; i <= j; ++i) h2[i] = -h[i];
+      AERO_TRY(hipMemcpyAsync(a->dots, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, m->stream), "h");
+      AERO_TRY(aero_host::launch_combine(n, 1.0, a->V, j + 1, a->dots, a->gw, m->stream), "gram-schmidt");
+      rc = dots_to_host(m, n, a->V, j + 1, a->gw, h2.data(), err, errlen);  // re-orthogonalise
+      if (rc) return rc;
+      for (int i = 0; i <= j; ++i) {
+        h[i] += h2[i];
+        h2[i] = -h2[i];
+      }
+      AERO_TRY(hipMemcpyAsync(a->dots, h2.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, m->stream), "h");
+      AERO_TRY(aero_host::launch_combine(n, 1.0, a->V, j + 1, a->dots, a->gw, m->stream), "gram-schmidt");
+      double hn2 = 0.0;
+      rc = dots_to_host(m, n, a->gw, 1, a->gw, &hn2, err, errlen);
+      if (rc) return rc;
+      const double hn = std::sqrt(hn2);
+      // Givens rotations on column j
+      for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
+      }
+      const double rr = std::hypot(h[j], hn);
+      cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+      sn[j] = rr > 0.0 ? hn / rr : 0.0;
+      h[j] = rr;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * mdim + j] = h[i];
+      ++total;
+      res = std::abs(gv[j + 1]);
+      hist.push_back(res);
+      jend = j;
+      if (!std::isfinite(res) || res <= ctl->tol * res0 || total >= maxit || hn == 0.0) break;
+      AERO_TRY(aero_host::launch_axpby(n, 1.0 / hn, a->gw, 0.0, a->V + static_cast<size_t>(j + 1) * n, m->stream),
+               "gmres basis");
+    }
+    if (jend < 0) break;
+    for (int i = jend; i >= 0; --i) {  // back substitution
+      double t = gv[i];
+      for (int k = i + 1; k <= jend; ++k) t -= H[static_cast<size_t>(i) * mdim + k] * y[k];
+      y[i] = t / H[static_cast<size_t>(i) * mdim + i];
+    }
+    AERO_TRY(hipMemcpyAsync(a->dots, y.data(), (jend + 1) * sizeof(double), hipMemcpyHostToDevice, m->stream), "y");
+    AERO_TRY(aero_host::launch_combine(n, 1.0, a->V, jend + 1, a->dots, a->gx, m->stream), "gmres update");
+    if (!std::isfinite(res) || total >= maxit) done = true;
+  }
+  AERO_TRY(hipStreamSynchronize(m->stream), "gmres sync");
+  const auto t1 = std::chrono::steady_clock::now();
+  ctl->iterations = total;
+  ctl->res0 = res0;
+  ctl->res = res;
+  ctl->seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (history) std::memcpy(history, hist.data(), std::min<size_t>(hist.size(), maxit) * sizeof(double));
+  if (!std::isfinite(res)) {
+    set_err(err, errlen, "linear solve: non-finite residual in GMRES");
+    return AERO_ENONFINITE;
+  }
+  return AERO_OK;
+}
+
+// Dispatch: plain iteration or GMRES; returns the device pointer of the solution.
+int linear_solve(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, const GeoTan& gt,
+                 const FaceOptsT<Dual>& od, aero_linear_solve* ctl, double* history, const double** x_out, char* err,
+                 size_t errlen) {
+  if (ctl->gmres_restart > 0) {
+    const int rc = gmres_iterate(m, cfg, forward, gt, od, ctl, history, err, errlen);
+    *x_out = m->adj->gx;
+    return rc;
+  }
+  int start = 0;
+  const int rc = plain_iterate(m, cfg, forward, gt, od, ctl, history, &start, err, errlen);
+  *x_out = m->adj->x[start];
+  return rc;
+}
+
 int check_ctl(const aero_linear_solve* ctl, const char* who, char* err, size_t errlen) {
-  if (!ctl || (ctl->rk_stages != 1 && ctl->rk_stages != 3) || !(ctl->cfl > 0.0) || ctl->max_iters <= 0) {
-    set_err(err, errlen, std::string(who) + ": rk_stages must be 1 or 3, cfl > 0, max_iters > 0");
+  if (!ctl || (ctl->rk_stages != 1 && ctl->rk_stages != 3) || !(ctl->cfl > 0.0) || ctl->max_iters <= 0 ||
+      ctl->gmres_restart < 0 || ctl->gmres_restart > 200) {
+    set_err(err, errlen, std::string(who) + ": rk_stages must be 1 or 3, cfl > 0, max_iters > 0, 0 <= gmres_restart <= 200");
     return AERO_EINVAL;
   }
   return AERO_OK;
@@ -736,11 +945,13 @@ int aero_hip_adjoint_solve(aero_hip_mesh* m, const aero_residual_config* cfg, co
   AdjointData* a = m->adj;
   rc = upload_internal(m, rhs, a->rhs, err, errlen);
   if (rc) return rc;
-  int start = 0;
-  rc = linear_iterate(m, cfg, 0, GeoTan{}, FaceOptsT<Dual>{}, ctl, history, &start, err, errlen);
+  const double* x = nullptr;
+  FaceOpts fo;
+  rc = mesh_face_opts(m, cfg, &fo, err, errlen);
   if (rc) return rc;
-  AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, m->U[m->cur], face_opts(cfg), a->x[start], a->psi,
-                                       m->stream),
+  rc = linear_solve(m, cfg, 0, GeoTan{}, dual_opts(fo, nullptr), ctl, history, &x, err, errlen);
+  if (rc) return rc;
+  AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, m->U[m->cur], face_opts(cfg), x, a->psi, m->stream),
            "k_adj_scale");
   return download_internal(m, a->psi, psi, err, errlen);
 }
@@ -761,10 +972,13 @@ int aero_hip_tangent_solve(aero_hip_mesh* m, const aero_residual_config* cfg, co
   GeoTan gt;
   rc = geometry_tangent(m, seed, &gt, err, errlen);
   if (rc) return rc;
-  int start = 0;
-  rc = linear_iterate(m, cfg, 1, gt, dual_opts(cfg, seed), ctl, history, &start, err, errlen);
+  const double* x = nullptr;
+  FaceOpts fo;
+  rc = mesh_face_opts(m, cfg, &fo, err, errlen);
   if (rc) return rc;
-  return download_internal(m, m->adj->x[start], v, err, errlen);
+  rc = linear_solve(m, cfg, 1, gt, dual_opts(fo, seed), ctl, history, &x, err, errlen);
+  if (rc) return rc;
+  return download_internal(m, x, v, err, errlen);
 }
 
 }  // extern "C"

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_tan_gradient(AdjMesh am, const dou
 
 // ---- tangent of the face fluxes ------------------------------------------------------------------
 // fa[k][F] (k = 0..3 values F_k A, 4..7 their derivatives).
-template <int SCHEME, int LIMITER, bool SECOND>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
 __global__ __launch_bounds__(kThreads) void k_tan_face(AdjMesh am, const double* __restrict__ U,
                                                        const double* __restrict__ dU, const double* __restrict__ G,
                                                        const double* __restrict__ dG, GeoTan gt, FaceOptsT<Dual> o,
@@ -165,9 +165,14 @@ __global__ __launch_bounds__(kThreads) void k_tan_face(AdjMesh am, const double*
       dnx = fx - dual_at(am.ccx, gt.dccx, cn);
       dny = fy - dual_at(am.ccy, gt.dccy, cn);
     }
+    double fr[8];
+    if (FROZEN) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) fr[j] = o.frozen[j * am.F + f];
+    }
     bool reverted;
-    Fl = interior_face<SCHEME, LIMITER, SECOND>(load_dual_u(U, dU, ld, co), load_dual_u(U, dU, ld, cn), go, gn, nx,
-                                                ny, dox, doy, dnx, dny, o, &reverted);
+    Fl = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_dual_u(U, dU, ld, co), load_dual_u(U, dU, ld, cn), go,
+                                                        gn, nx, ny, dox, doy, dnx, dny, o, &reverted, fr);
   } else {
     Fl = boundary_face<SCHEME>(load_dual_u(U, dU, ld, co), am.fbc[f] == 2, nx, ny, o);
   }
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_tan_gather(AdjMesh am, const doubl
 // ---- face Jacobians ----------------------------------------------------------------------------------
 // Thread (dir, f): column `dir` of d(F_f A_f)/d(U_o[0..3], U_n[0..3], G_o[0..7], G_n[0..7]), stored at
 // J[(dir * 4 + k) * F + f].  Boundary faces depend on U_o only; their other columns are zero.
-template <int SCHEME, int LIMITER, bool SECOND>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
 __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const double* __restrict__ U,
                                                             const double* __restrict__ G, FaceOpts o0,
                                                             double* __restrict__ J) {
@@ -245,6 +250,7 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
   o.as_cut = o0.as_cut;
   o.as_fmin = o0.as_fmin;
   o.as_w = o0.as_w;
+  o.frozen = o0.frozen;
   const double A = am.fA[f];
   const Dual nx(am.fnx[f]), ny(am.fny[f]);
   double col[4] = {0.0, 0.0, 0.0, 0.0};
@@ -275,9 +281,14 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
       if (SECOND) gn[dir - 16].d = 1.0; else active = false;
     }
     if (active) {
+      double fr[8];
+      if (FROZEN) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fr[j] = o0.frozen[j * F + f];
+      }
       bool reverted;
       const FluxT<Dual> Fl =
-          interior_face<SCHEME, LIMITER, SECOND>(uo, un, go, gn, nx, ny, dox, doy, dnx, dny, o, &reverted);
+          interior_face<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, go, gn, nx, ny, dox, doy, dnx, dny, o, &reverted, fr);
       col[0] = Fl.f0.d * A;
       col[1] = Fl.f1.d * A;
       col[2] = Fl.f2.d * A;
@@ -540,6 +551,64 @@ __global__ __launch_bounds__(kThreads) void k_partials_sum(int n, const double* 
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// ---- Krylov vector kernels (flat vectors of n doubles) -------------------------------------------
+// Block partials of <w, V_i> for i < nv, V_i = V + i * n; then a fixed-order sum per i.
+__global__ __launch_bounds__(kThreads) void k_multidot(long long n, const double* __restrict__ V, int nv,
+                                                       const double* __restrict__ w, double* __restrict__ partials) {
+  __shared__ double red[kThreads];
+  const long long per = (n + gridDim.x - 1) / gridDim.x;
+  const long long lo = blockIdx.x * per;
+  const long long hi = lo + per < n ? lo + per : n;
+  for (int i = 0; i < nv; ++i) {
+    const double* v = V + static_cast<long long>(i) * n;
+    double s = 0.0;
+    for (long long k = lo + threadIdx.x; k < hi; k += kThreads) s += v[k] * w[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int t = kThreads / 2; t > 0; t >>= 1) {
+      if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[static_cast<long long>(i) * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_multidot_sum(int nparts, int nv, const double* __restrict__ partials,
+                                                           double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  for (int i = 0; i < nv; ++i) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += kThreads) s += partials[static_cast<long long>(i) * nparts + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int t = kThreads / 2; t > 0; t >>= 1) {
+      if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
+    __syncthreads();
+  }
+}
+
+// y = a * y + sum_i c_i V_i  (c in device memory).
+__global__ __launch_bounds__(kThreads) void k_combine(long long n, double a, const double* __restrict__ V, int nv,
+                                                      const double* __restrict__ c, double* __restrict__ y) {
+  const long long k = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= n) return;
+  double s = a == 0.0 ? 0.0 : a * y[k];
+  for (int i = 0; i < nv; ++i) s += c[i] * V[static_cast<long long>(i) * n + k];
+  y[k] = s;
+}
+
+// y = a * x + b * y.
+__global__ __launch_bounds__(kThreads) void k_axpby(long long n, double a, const double* __restrict__ x, double b,
+                                                    double* __restrict__ y) {
+  const long long k = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= n) return;
+  y[k] = a * x[k] + (b == 0.0 ? 0.0 : b * y[k]);
+}
+
 }  // namespace
 }  // namespace aero_dev
 
@@ -563,16 +632,23 @@ hipError_t launch_tan_gradient(const AdjMesh& am, const double* U, const double*
   return hipGetLastError();
 }
 
-template <int SCHEME, int LIMITER, bool SECOND>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
 static void tan_face_t(const AdjMesh& am, const double* U, const double* dU, const double* G, const double* dG,
                        const GeoTan& gt, const FaceOptsT<Dual>& o, double* fa, hipStream_t s) {
-  hipLaunchKernelGGL((k_tan_face<SCHEME, LIMITER, SECOND>), dim3(blocks_for(am.F)), dim3(kThreads), 0, s, am, U, dU,
-                     G, dG, gt, o, fa);
+  hipLaunchKernelGGL((k_tan_face<SCHEME, LIMITER, SECOND, FROZEN>), dim3(blocks_for(am.F)), dim3(kThreads), 0, s, am,
+                     U, dU, G, dG, gt, o, fa);
 }
 
 hipError_t launch_tan_face(const AdjMesh& am, int scheme, int limiter, bool second, const double* U, const double* dU,
                            const double* G, const double* dG, const GeoTan& gt, const FaceOptsT<Dual>& o, double* fa,
                            hipStream_t s) {
+  if (second && o.frozen) {  // the frozen ratios replace the limiter
+    if (scheme)
+      tan_face_t<1, 0, true, true>(am, U, dU, G, dG, gt, o, fa, s);
+    else
+      tan_face_t<0, 0, true, true>(am, U, dU, G, dG, gt, o, fa, s);
+    return hipGetLastError();
+  }
   const int key = (scheme ? 4 : 0) | (limiter ? 2 : 0) | (second ? 1 : 0);
   switch (key) {
     case 0: tan_face_t<0, 0, false>(am, U, dU, G, dG, gt, o, fa, s); break;
@@ -592,15 +668,22 @@ hipError_t launch_tan_gather(const AdjMesh& am, const double* fa, double* R, dou
   return hipGetLastError();
 }
 
-template <int SCHEME, int LIMITER, bool SECOND>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
 static void face_jac_t(const AdjMesh& am, const double* U, const double* G, const FaceOpts& o, double* J,
                        hipStream_t s) {
-  hipLaunchKernelGGL((k_face_jacobian<SCHEME, LIMITER, SECOND>), dim3(blocks_for(24LL * am.F)), dim3(kThreads), 0, s,
-                     am, U, G, o, J);
+  hipLaunchKernelGGL((k_face_jacobian<SCHEME, LIMITER, SECOND, FROZEN>), dim3(blocks_for(24LL * am.F)), dim3(kThreads),
+                     0, s, am, U, G, o, J);
 }
 
 hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool second, const double* U,
                                 const double* G, const FaceOpts& o, double* J, hipStream_t s) {
+  if (second && o.frozen) {  // the frozen ratios replace the limiter
+    if (scheme)
+      face_jac_t<1, 0, true, true>(am, U, G, o, J, s);
+    else
+      face_jac_t<0, 0, true, true>(am, U, G, o, J, s);
+    return hipGetLastError();
+  }
   const int key = (scheme ? 4 : 0) | (limiter ? 2 : 0) | (second ? 1 : 0);
   switch (key) {
     case 0: face_jac_t<0, 0, false>(am, U, G, o, J, s); break;
@@ -643,6 +726,25 @@ hipError_t launch_lin_update(const AdjMesh& am, int stage, int rk3, int forward,
                              const FaceOpts& o, double* xout, double* partials, hipStream_t s) {
   hipLaunchKernelGGL(k_lin_update, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, stage, rk3, forward, x0, xs, r,
                      rhs, lam, cfl, U, o, xout, partials);
+  return hipGetLastError();
+}
+
+constexpr int kDotBlocks = 512;
+
+hipError_t launch_multidot(long long n, const double* V, int nv, const double* w, double* partials, double* out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_multidot, dim3(kDotBlocks), dim3(kThreads), 0, s, n, V, nv, w, partials);
+  hipLaunchKernelGGL(k_multidot_sum, dim3(1), dim3(kThreads), 0, s, kDotBlocks, nv, partials, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(long long n, double a, const double* V, int nv, const double* c, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_combine, dim3(blocks_for(n)), dim3(kThreads), 0, s, n, a, V, nv, c, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3(blocks_for(n)), dim3(kThreads), 0, s, n, a, x, b, y);
   return hipGetLastError();
 }
 

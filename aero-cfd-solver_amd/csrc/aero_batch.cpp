@@ -141,6 +141,10 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     set_err(err, errlen, "aero_hip_batch_create: the staged all-speed controller is single-case only");
     return AERO_EINVAL;
   }
+  if (c->limiter_freeze_iter >= 0) {
+    set_err(err, errlen, "aero_hip_batch_create: the frozen limiter is single-case only");
+    return AERO_EINVAL;
+  }
   const int BC = meshes[0]->BC;
   for (int g = 0; g < n_meshes; ++g)
     if (!meshes[g] || meshes[g]->BC != BC || meshes[g]->dm.grad != meshes[0]->dm.grad) {

@@ -162,7 +162,20 @@ FaceOpts face_opts(const aero_residual_config* c) {
   o.as_cut = c->all_speed_cutoff;
   o.as_fmin = c->all_speed_f_min;
   o.as_w = c->all_speed_ramp;
+  o.frozen = nullptr;
   return o;
+}
+
+int mesh_face_opts(const aero_hip_mesh* m, const aero_residual_config* c, FaceOpts* o, char* err, size_t errlen) {
+  *o = face_opts(c);
+  if (c->limiter_frozen) {
+    if (!m->lim_valid) {
+      set_err(err, errlen, "limiter_frozen = 1 but no limiter ratios were captured (aero_hip_freeze_limiter)");
+      return AERO_EINVAL;
+    }
+    o->frozen = m->lim_ratio;
+  }
+  return AERO_OK;
 }
 
 int to_internal_state(aero_hip_mesh* m, const double* conserved, std::vector<double>* soa) {
@@ -450,6 +463,7 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
 
   DevMesh& dm = m->dm;
   dm.N = N;
+  dm.nfaces = F;
   dm.ld = N;
   dm.nblk = m->nblk;
   dm.BC = BC;
@@ -515,7 +529,8 @@ int aero_hip_residual_async(aero_hip_mesh* m, const aero_residual_config* cfg, c
     set_err(err, errlen, "aero_hip_residual_async: unknown flux scheme or limiter");
     return AERO_EINVAL;
   }
-  const FaceOpts o = face_opts(cfg);
+  FaceOpts o;
+  if (const int rc = mesh_face_opts(m, cfg, &o, err, errlen)) return rc;
   hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
   AERO_HIP_TRY(hipGetLastError(), "aero_hip_residual_async: k_init_diag");
   const double* U = m->U[m->cur];
@@ -647,6 +662,36 @@ int aero_hip_residual(aero_hip_mesh* m, const aero_residual_config* cfg, const d
   return aero_hip_get_residual(m, residual, spectral_radius, diag, err, errlen);
 }
 
+int aero_hip_freeze_limiter(aero_hip_mesh* m, const aero_residual_config* cfg, const double* conserved,
+                            double* ratios, char* err, size_t errlen) {
+  if (!m || !cfg) {
+    set_err(err, errlen, "aero_hip_freeze_limiter: null argument");
+    return AERO_EINVAL;
+  }
+  if (conserved) {
+    const int rc = aero_hip_set_state(m, conserved, err, errlen);
+    if (rc) return rc;
+  }
+  const size_t n = 8 * static_cast<size_t>(m->F);
+  if (!m->lim_ratio) {
+    void* q = nullptr;
+    AERO_HIP_TRY(hipMalloc(&q, n * sizeof(double)), "aero_hip_freeze_limiter: hipMalloc");
+    m->allocs.push_back(q);
+    m->lim_ratio = static_cast<double*>(q);
+  }
+  const double* U = m->U[m->cur];
+  AERO_HIP_TRY(hipMemsetAsync(m->lim_ratio, 0, n * sizeof(double), m->stream), "aero_hip_freeze_limiter");
+  AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, cfg->gamma, m->stream), "k_gradient");
+  AERO_HIP_TRY(aero_host::launch_limiter_capture(m->dm, cfg->limiter, U, m->G, cfg->gamma, m->lim_ratio, m->stream),
+               "k_limiter_capture");
+  if (ratios)
+    AERO_HIP_TRY(hipMemcpyAsync(ratios, m->lim_ratio, n * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "aero_hip_freeze_limiter: copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_freeze_limiter: sync");
+  m->lim_valid = true;
+  return AERO_OK;
+}
+
 int aero_hip_stage(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t stage, int32_t last_stage, double cfl,
                    int32_t lts, int32_t write_residual, char* err, size_t errlen) {
   if (!m || !cfg) {
@@ -660,7 +705,8 @@ int aero_hip_stage(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t st
   if (stage == 0) m->base = m->cur;
   int out = 0;
   while (out == m->cur || out == m->base) ++out;
-  const FaceOpts o = face_opts(cfg);
+  FaceOpts o;
+  if (const int rc = mesh_face_opts(m, cfg, &o, err, errlen)) return rc;
   const double* U = m->U[m->cur];
   StageArgs st{};
   st.stage = stage;
@@ -934,6 +980,11 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     }
     rc.second_order = second ? 1 : 0;
     rc.stab_mach_floor = floor_cur;
+    if (second && c->limiter_freeze_iter >= 0 && iter == c->limiter_freeze_iter) {  // beyond the reference
+      const int fr = aero_hip_freeze_limiter(m, &rc, nullptr, nullptr, err, errlen);
+      if (fr) return fr;
+      rc.limiter_frozen = 1;
+    }
     hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
     AERO_HIP_TRY(hipGetLastError(), "k_init_diag");
     for (int st = 0; st < c->rk_stages; ++st) {

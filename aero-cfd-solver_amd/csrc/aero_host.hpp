@@ -38,11 +38,17 @@ struct aero_hip_mesh {
   std::vector<int> h_owner, h_neighbor, h_bc;  // h_bc: AERO_BC_* per face
   int wall_start = 0, wall_count = 0;
   AdjointData* adj = nullptr;
+  double* lim_ratio = nullptr;  // frozen limiter ratios [8][F] (aero_hip_freeze_limiter)
+  bool lim_valid = false;
 };
 
 namespace aero_hostutil {
 // The per-evaluation options of an aero_residual_config.
 aero_dev::FaceOpts face_opts(const aero_residual_config* c);
+// face_opts plus the handle's frozen limiter ratios when the config asks for them; fails (AERO_EINVAL,
+// message in err) when none were captured.
+int mesh_face_opts(const aero_hip_mesh* m, const aero_residual_config* c, aero_dev::FaceOpts* o, char* err,
+                   size_t errlen);
 // Host [N][5] conserved (reference numbering) <-> device SoA [4][N] (internal numbering).
 int to_internal_state(aero_hip_mesh* m, const double* conserved, std::vector<double>* soa);
 void from_internal4(const aero_hip_mesh* m, const std::vector<double>& soa, double* aos5);

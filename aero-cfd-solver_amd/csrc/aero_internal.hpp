@@ -16,6 +16,7 @@ constexpr int kMetaPrimary = 4;
 // Device view of a mesh in internal (block) order.  Passed by value to kernels.
 struct DevMesh {
   int N;         // cells
+  int nfaces;    // faces (reference numbering)
   int ld;        // leading dimension of SoA cell arrays (== N)
   int nblk;      // blocks of BC cells
   int BC;        // cells per block == threads per workgroup
@@ -153,6 +154,9 @@ hipError_t launch_gradient(const aero_dev::DevMesh& m, const double* U, double* 
 hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, int scheme, int limiter, bool second,
                        int mode, const double* U, const double* G, double* R, double* lam, int* diag,
                        const aero_dev::StageArgs& st, hipStream_t s);
+// Frozen limiter ratios [8][F] at U with gradients G (k_limiter_capture).
+hipError_t launch_limiter_capture(const aero_dev::DevMesh& m, int limiter, const double* U, const double* G, double g,
+                                  double* ratios, hipStream_t s);
 hipError_t launch_global_update(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, const double* U,
                                 const double* R, const double* lam, double cfl, double* dt_scratch, int* diag,
                                 const aero_dev::StageArgs& st, hipStream_t s);
@@ -192,4 +196,10 @@ hipError_t launch_lin_update(const aero_dev::AdjMesh& am, int stage, int rk3, in
                              const double* U, const aero_dev::FaceOpts& o, double* xout, double* partials,
                              hipStream_t s);
 hipError_t launch_partials_sum(int nparts, const double* partials, double* out, hipStream_t s);
+// Krylov vector operations on flat vectors of n doubles; the dot products use kDotBlocks (512) block
+// partials per vector, summed in a fixed order.
+hipError_t launch_multidot(long long n, const double* V, int nv, const double* w, double* partials, double* out,
+                           hipStream_t s);
+hipError_t launch_combine(long long n, double a, const double* V, int nv, const double* c, double* y, hipStream_t s);
+hipError_t launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
 }  // namespace aero_host

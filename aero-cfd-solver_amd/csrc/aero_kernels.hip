@@ -231,7 +231,7 @@ __device__ __forceinline__ int wave_min_i(int v) {
 // Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
 // MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
 // preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
-template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false>
 __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpts& o, const double* __restrict__ U,
                                           const double* __restrict__ G, double* __restrict__ R,
                                           double* __restrict__ lam, int* __restrict__ diag, const StageArgs& st,
@@ -271,9 +271,15 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
         dnx = fx - m.ccx[cn];
         dny = fy - m.ccy[cn];
       }
+      double fr[8];
+      if (FROZEN && SECOND) {
+        const int f = m.s_face[s];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fr[j] = o.frozen[j * m.nfaces + f];
+      }
       bool reverted;
-      F = interior_face<SCHEME, LIMITER, SECOND>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy, dnx,
-                                                 dny, o, &reverted);
+      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
+                                                         dnx, dny, o, &reverted, fr);
       if (reverted && (meta & kMetaPrimary)) {
         ++n_fo;
         first_fail = min(first_fail, m.s_face[s]);
@@ -433,14 +439,49 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
   }
 }
 
-template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false>
 __global__ __launch_bounds__(BC) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
                                               const double* __restrict__ G, double* __restrict__ R,
                                               double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
   extern __shared__ double lds[];
   const int b = block_of(m);
   if (b >= m.nblk) return;
-  flux_body<BC, SCHEME, LIMITER, SECOND, MODE>(m, b, o, U, G, R, lam, diag, st, lds);
+  flux_body<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN>(m, b, o, U, G, R, lam, diag, st, lds);
+}
+
+// Frozen limiter capture (beyond the reference): per interior face (taken once, from the block that owns
+// its owner cell), the ratios phi(a, b) / a of both sides and the four primitive variables at the
+// state U and its gradients G, stored at ratios[j * F + face], j = owner rho, u, v, p, neighbour
+// rho, u, v, p.
+template <int BC, int LIMITER>
+__global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double* __restrict__ U,
+                                                        const double* __restrict__ G, double g,
+                                                        double* __restrict__ ratios) {
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  const int s0 = m.slot_off[b];
+  const int ns = m.slot_off[b + 1] - s0;
+  const int ld = m.ld, F = m.nfaces;
+  for (int i = threadIdx.x; i < ns; i += BC) {
+    const int s = s0 + i;
+    const int co = m.s_o[s], cn = m.s_n[s];
+    if (cn < 0 || !(m.s_meta[s] & kMetaPrimary)) continue;
+    const P4 L = c2p(load_u(U, ld, co), g);
+    const P4 Rt = c2p(load_u(U, ld, cn), g);
+    const double fx = m.s_fx[s], fy = m.s_fy[s];
+    const double dox = fx - m.ccx[co], doy = fy - m.ccy[co];
+    const double dnx = fx - m.ccx[cn], dny = fy - m.ccy[cn];
+    const double q[4][2] = {{L.r, Rt.r}, {L.u, Rt.u}, {L.v, Rt.v}, {L.p, Rt.p}};
+    const int f = m.s_face[s];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double ro, rn;
+      limiter_ratio<LIMITER>(q[k][0], q[k][1], G[(2 * k) * ld + co], G[(2 * k + 1) * ld + co], G[(2 * k) * ld + cn],
+                             G[(2 * k + 1) * ld + cn], dox, doy, dnx, dny, &ro, &rn);
+      ratios[k * F + f] = ro;
+      ratios[(4 + k) * F + f] = rn;
+    }
+  }
 }
 
 // Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
@@ -783,13 +824,13 @@ hipError_t launch_gradient(const DevMesh& m, const double* U, double* G, double 
   }
 }
 
-template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false>
 static hipError_t launch_flux_t(const DevMesh& m, const FaceOpts& o, const double* U, const double* G, double* R,
                                 double* lam, int* diag, const StageArgs& st, hipStream_t s) {
   // The fused stage tail reuses the slot staging area for 4 doubles per wavefront.
   const size_t lds = static_cast<size_t>(std::max(m.smax * 5, 4 * (BC / 64))) * sizeof(double);
-  hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE>), dim3(m.grid), dim3(BC), lds, s, m, o, U, G, R, lam,
-                     diag, st);
+  hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN>), dim3(m.grid), dim3(BC), lds, s, m, o, U, G,
+                     R, lam, diag, st);
   return hipGetLastError();
 }
 
@@ -797,6 +838,10 @@ template <int BC, int MODE>
 static hipError_t launch_flux_bc(const DevMesh& m, const FaceOpts& o, int scheme, int limiter, bool second,
                                  const double* U, const double* G, double* R, double* lam, int* diag,
                                  const StageArgs& st, hipStream_t s) {
+  if (second && o.frozen) {  // the frozen ratios replace the limiter
+    if (scheme == 0) return launch_flux_t<BC, 0, 0, true, MODE, true>(m, o, U, G, R, lam, diag, st, s);
+    return launch_flux_t<BC, 1, 0, true, MODE, true>(m, o, U, G, R, lam, diag, st, s);
+  }
 #define AERO_FLUX_CASE(SC, LI, SE) \
   if (scheme == SC && limiter == LI && second == SE) return launch_flux_t<BC, SC, LI, SE, MODE>(m, o, U, G, R, lam, diag, st, s);
   AERO_FLUX_CASE(0, 0, false)
@@ -807,6 +852,23 @@ static hipError_t launch_flux_bc(const DevMesh& m, const FaceOpts& o, int scheme
   AERO_FLUX_CASE(1, 1, true)
 #undef AERO_FLUX_CASE
   if (!second) return launch_flux_t<BC, 0, 0, false, MODE>(m, o, U, G, R, lam, diag, st, s);  // limiter unused
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_limiter_capture(const DevMesh& m, int limiter, const double* U, const double* G, double g,
+                                  double* ratios, hipStream_t s) {
+#define AERO_CAP(BCV)                                                                                          \
+  if (m.BC == BCV) {                                                                                           \
+    if (limiter == 1)                                                                                          \
+      hipLaunchKernelGGL((k_limiter_capture<BCV, 1>), dim3(m.grid), dim3(BCV), 0, s, m, U, G, g, ratios);      \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_limiter_capture<BCV, 0>), dim3(m.grid), dim3(BCV), 0, s, m, U, G, g, ratios);      \
+    return hipGetLastError();                                                                                  \
+  }
+  AERO_CAP(128)
+  AERO_CAP(256)
+  AERO_CAP(512)
+#undef AERO_CAP
   return hipErrorInvalidValue;
 }
 

@@ -71,6 +71,7 @@ template <class T> struct FaceOptsT {
   T mach_ref, mach_min, beta_min, beta_max;
   int as_fix;
   double as_cut, as_fmin, as_w;
+  const double* frozen;  // frozen limiter ratios [8][F] (beyond the reference) or null
 };
 using FaceOpts = FaceOptsT<double>;
 

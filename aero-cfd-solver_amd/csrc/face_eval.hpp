@@ -23,23 +23,61 @@ __device__ __forceinline__ void recon_var(T qo, T qn, T gox, T goy, T gnx, T gny
   *qr = qn + limiter<LIMITER>(db, -dc);
 }
 
+// Frozen limiter (beyond the reference): the limited increment is ratio * a with the ratio
+// phi(a, b) / a captured once (0 where a == 0), a smooth function of the state.
+template <class T>
+__device__ __forceinline__ void recon_var_frozen(T qo, T qn, T gox, T goy, T gnx, T gny, T dox, T doy, T dnx, T dny,
+                                                 double ro, double rn, T* ql, T* qr) {
+  T da = 0.0;
+  da += gox * dox;
+  da += goy * doy;
+  T db = 0.0;
+  db += gnx * dnx;
+  db += gny * dny;
+  *ql = qo + ro * da;
+  *qr = qn + rn * db;
+}
+
+// The capture: ratio phi(a, b) / a of both sides of one variable.
+template <int LIMITER>
+__device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, double goy, double gnx, double gny,
+                                              double dox, double doy, double dnx, double dny, double* ro, double* rn) {
+  const double dc = qn - qo;
+  double da = 0.0;
+  da += gox * dox;
+  da += goy * doy;
+  double db = 0.0;
+  db += gnx * dnx;
+  db += gny * dny;
+  *ro = da != 0.0 ? limiter<LIMITER>(da, dc) / da : 0.0;
+  *rn = db != 0.0 ? limiter<LIMITER>(db, -dc) / db : 0.0;
+}
+
 // Interior face: reconstruction with the first-order fallback (residual_assembly.cpp:107-137).
 // go / gn are the cells' primitive gradients (rho_x, rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the
 // vectors from the cell centres to the face centre.  *reverted reports the fallback to cell values.
-template <int SCHEME, int LIMITER, bool SECOND, class T>
+// FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, class T>
 __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
                                                   T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
-                                                  bool* reverted) {
+                                                  bool* reverted, const double* fr = nullptr) {
   const double g = o.g;
   P4T<T> L = c2p(uo, g);
   P4T<T> Rt = c2p(un, g);
   *reverted = false;
   if (SECOND) {
     P4T<T> l2, r2;
-    recon_var<LIMITER>(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, &l2.r, &r2.r);
-    recon_var<LIMITER>(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, &l2.u, &r2.u);
-    recon_var<LIMITER>(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, &l2.v, &r2.v);
-    recon_var<LIMITER>(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, &l2.p, &r2.p);
+    if (FROZEN) {
+      recon_var_frozen(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, fr[0], fr[4], &l2.r, &r2.r);
+      recon_var_frozen(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, fr[1], fr[5], &l2.u, &r2.u);
+      recon_var_frozen(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, fr[2], fr[6], &l2.v, &r2.v);
+      recon_var_frozen(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, fr[3], fr[7], &l2.p, &r2.p);
+    } else {
+      recon_var<LIMITER>(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, &l2.r, &r2.r);
+      recon_var<LIMITER>(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, &l2.u, &r2.u);
+      recon_var<LIMITER>(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, &l2.v, &r2.v);
+      recon_var<LIMITER>(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, &l2.p, &r2.p);
+    }
     l2.r = dmax(l2.r, kFloor);
     r2.r = dmax(r2.r, kFloor);
     l2.p = dmax(l2.p, kFloor);

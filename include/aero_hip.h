@@ -69,6 +69,10 @@ typedef struct aero_residual_config {
   double precond_mach_ref, precond_mach_min, precond_beta_min, precond_beta_max;
   int32_t all_speed_fix;
   double all_speed_cutoff, all_speed_f_min, all_speed_ramp;
+  /* Beyond the reference: 1 = MUSCL with the handle's frozen limiter ratios (aero_hip_freeze_limiter),
+   * q + ratio * a instead of q + phi(a, b); makes the residual smooth so stalled limiter cycles
+   * converge and the adjoint is well posed.  0 = the reference's limiter. */
+  int32_t limiter_frozen;
 } aero_residual_config;
 
 /* Counters of EulerResidualAssemblyDiagnostics (residual_assembly.hpp:33-41). */
@@ -106,6 +110,9 @@ typedef struct aero_case_config {
   double all_speed_cfl_drop_factor, all_speed_cfl_restore_factor, all_speed_cfl_min_scale;
   int32_t aoa0_symmetry_enforce, aoa0_symmetry_enforce_interval;
   int32_t precond_experimental_enabled;
+  /* Beyond the reference: at the start of this (second-order) iteration the limiter ratios are
+   * captured from the state and frozen for the rest of the run; < 0 = never (the reference). */
+  int32_t limiter_freeze_iter;
 } aero_case_config;
 
 /* Per-iteration history row layout (residuals.csv / forces.csv columns, euler_solver.cpp:1161-1203). */
@@ -181,6 +188,14 @@ int aero_hip_layout(aero_hip_mesh* mesh, int64_t* out8);
 int aero_hip_stage(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t stage, int32_t last_stage,
                    double cfl, int32_t local_time_stepping, int32_t write_residual, char* err, size_t errlen);
 
+/* Frozen limiter (no reference counterpart): captures, at `conserved` (NULL = the handle's state) and
+ * the gradients of that state, the ratio phi(a, b) / a of every interior face side and primitive
+ * variable (0 where a == 0) for the residual configuration's limiter.  Residual calls with
+ * limiter_frozen = 1 use them until the next capture.  ratios (may be NULL): [8][F] copy, rows
+ * owner-side rho, u, v, p then neighbour-side rho, u, v, p, faces in the reference order. */
+int aero_hip_freeze_limiter(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                            double* ratios, char* err, size_t errlen);
+
 /* The full pseudo-time run of run_euler_airfoil_case (euler_solver.cpp:463-1208) with the stage
  * loop, norms and forces on the device and the ramp / CFL / convergence control on the host. */
 int aero_hip_run_case(aero_hip_mesh* mesh, const aero_case_config* cfg, aero_run_output* out, char* err,
@@ -254,15 +269,21 @@ typedef struct aero_objective {
   double aoa_deg, q_inf, p_inf, chord, x_ref, y_ref, gamma;
 } aero_objective;
 
-/* Controls and results of a pseudo-time linear solve: rk_stages 1 or 3, cfl, at most max_iters
- * iterations, stop when the linear residual's L2 norm is <= tol times its first value (checked every
- * check_every iterations). */
+/* Controls and results of a linear solve.  The pseudo-time iteration (rk_stages 1 or 3, cfl) runs at
+ * most max_iters iterations; with gmres_restart = m > 0 it is instead accelerated by restarted GMRES(m)
+ * on its fixed-point form x = G x + c, one pseudo-time iteration per matrix-vector product (this
+ * converges where the plain iteration diverges on a few unstable modes, as at limit-cycling states).
+ * Stop when the residual's L2 norm is <= tol times its first value (checked every check_every
+ * iterations of the plain iteration; every Arnoldi step under GMRES, where the norm is that of the
+ * fixed-point residual). */
 typedef struct aero_linear_solve {
   int32_t rk_stages;
   double cfl;
   int32_t max_iters;
   double tol;
   int32_t check_every;
+  int32_t gmres_restart;
+  int32_t gmres_sweeps;   /* pseudo-time iterations per GMRES matrix-vector product (>= 1) */
   int32_t iterations;     /* out */
   double res0, res;       /* out: first and last linear residual norms */
   double seconds;         /* out: device time of the iterations */
@@ -339,6 +360,34 @@ typedef struct aero_mesh_arrays {
 int aero_mesh_generate(const aero_mesh_gen_config* cfg, int32_t precision, aero_mesh_arrays* out, char* err,
                        size_t errlen);
 void aero_mesh_free(aero_mesh_arrays* arrays);
+
+/* Shape parameters (no reference counterpart): Hicks-Henne bumps b_i(x) = sin^3(pi x^(ln 0.5 / ln h_i)),
+ * h_i = linspace(h_min, h_max, bumps_per_surface) in x/c, on the upper surface (parameters
+ * 0..n-1) then the lower surface (n..2n-1); amplitude a_i moves that surface's wall nodes by a_i b_i(x)
+ * in +y, and every other node by the inverse-distance-weighted average (weights |r|^-idw_exponent) of
+ * the wall and farfield node displacements (farfield nodes fixed). */
+typedef struct aero_shape_config {
+  int32_t bumps_per_surface;
+  double h_min, h_max;
+  double idw_exponent; /* 3 when <= 0 */
+} aero_shape_config;
+
+/* Derivatives of the deformed geometry with respect to each amplitude (caller-allocated, any NULL). */
+typedef struct aero_geometry_tangents {
+  int32_t n_params;       /* 2 * bumps_per_surface */
+  double* d_points;       /* [n_params][3P] */
+  double* d_face_normal;  /* [n_params][3F] */
+  double* d_face_center;  /* [n_params][3F] */
+  double* d_face_area;    /* [n_params][F] */
+  double* d_cell_center;  /* [n_params][3N] */
+  double* d_cell_volume;  /* [n_params][N] */
+} aero_geometry_tangents;
+
+/* Deforms `base` (precision 8, from aero_mesh_generate) by the 2 * bumps_per_surface amplitudes into
+ * `out` (same topology; free with aero_mesh_free) and fills `tangents` when not NULL.  Fails if a cell
+ * would invert. */
+int aero_mesh_deform(const aero_mesh_arrays* base, const aero_shape_config* shape, const double* amplitudes,
+                     aero_mesh_arrays* out, aero_geometry_tangents* tangents, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,7 @@ template <> constexpr double kc<double>(float, double d) { return d; }
 template <class T> inline T ao_sqrt(T x) { return std::sqrt(x); }
 template <class T> inline T ao_abs(T x) { return std::abs(x); }
 template <class T> inline bool ao_isfinite(T x) { return std::isfinite(x); }
+template <class T> inline double ao_value(T x) { return static_cast<double>(x); }
 
 // std::max / std::min / std::clamp semantics (libstdc++), spelled out so the operand order is explicit.
 template <class T> inline T smax(T a, T b) { return (a < b) ? b : a; }
@@ -457,9 +458,15 @@ void cell_gradients(const MeshView<R>& m, const std::vector<Prim<R>>& prim, std:
 }
 
 // MUSCL face states (reconstruction.cpp:148-219).
+// Frozen limiter (an option beyond the reference, off unless ratios are given): `frozen` holds, per
+// face f, the ratios phi / a of the limited to the unlimited increment captured once (layout
+// [j][F], j = 0..3 owner side rho, u, v, p and 4..7 the neighbour side); the reconstruction then uses
+// q + ratio * a, a smooth function of the state.  `capture` (may be null) receives those ratios.
+inline int frozen_slot(int k) { return k == 4 ? 3 : k; }  // rho, u, v, -, p -> 0, 1, 2, -, 3
+
 template <class R>
 void reconstruct(const MeshView<R>& m, const std::vector<Prim<R>>& prim, const std::vector<R>& grad, int f,
-                 int lim, Prim<R>* L, Prim<R>* Rt) {
+                 int lim, Prim<R>* L, Prim<R>* Rt, const double* frozen = nullptr, double* capture = nullptr) {
   const int nc = 2;
   const int o = m.owner[f];
   const int nb = m.neighbor[f];
@@ -479,8 +486,22 @@ void reconstruct(const MeshView<R>& m, const std::vector<Prim<R>>& prim, const s
       da += grad[static_cast<std::size_t>((o * 5 + k) * nc + d)] * dO[d];
       db += grad[static_cast<std::size_t>((nb * 5 + k) * nc + d)] * dN[d];
     }
-    const R ql = qo + apply_limiter(lim, da, dc);
-    const R qr = qn + apply_limiter(lim, db, -dc);
+    R la, lb;
+    if (frozen && k != 3) {
+      const std::size_t F = static_cast<std::size_t>(m.num_faces);
+      la = R(frozen[frozen_slot(k) * F + f]) * da;
+      lb = R(frozen[(4 + frozen_slot(k)) * F + f]) * db;
+    } else {
+      la = apply_limiter(lim, da, dc);
+      lb = apply_limiter(lim, db, -dc);
+    }
+    if (capture && k != 3) {
+      const std::size_t F = static_cast<std::size_t>(m.num_faces);
+      capture[frozen_slot(k) * F + f] = da != R(0) ? ao_value(la / da) : 0.0;
+      capture[(4 + frozen_slot(k)) * F + f] = db != R(0) ? ao_value(lb / db) : 0.0;
+    }
+    const R ql = qo + la;
+    const R qr = qn + lb;
     set_prim_var(L, k, ql);
     set_prim_var(Rt, k, qr);
   }
@@ -565,6 +586,7 @@ template <class R> struct AssemblyCfg {  // residual_assembly.hpp:12-31 (pointer
   R as_cutoff = AO_K(0.25);
   R as_fmin = AO_K(0.05);
   R as_ramp = AO_K(1.0);
+  const double* frozen = nullptr;  // frozen limiter ratios (see reconstruct), or null
 };
 
 template <class R> R smooth_max(R a, R b) {  // residual_assembly.cpp:12-20
@@ -665,6 +687,18 @@ struct AssemblyDiag {
   int first_failure_face = -1;
 };
 
+// Limiter ratios of every interior face at a state (see reconstruct); boundary faces keep 0.
+template <class R>
+void limiter_ratios(const MeshView<R>& m, const std::vector<Prim<R>>& prim, const std::vector<R>& grad, int lim,
+                    std::vector<double>* out) {
+  out->assign(8 * static_cast<std::size_t>(m.num_faces), 0.0);
+  for (int f = 0; f < m.num_faces; ++f) {
+    if (m.neighbor[f] < 0) continue;
+    Prim<R> L, Rt;
+    reconstruct(m, prim, grad, f, lim, &L, &Rt, nullptr, out->data());
+  }
+}
+
 template <class R>
 void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std::vector<Prim<R>>& prim,
                        const std::vector<R>& grad, std::vector<R>* res, std::vector<R>* lam,
@@ -685,7 +719,7 @@ void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std:
     if (nb >= 0) {
       Prim<R> L = prim[o], Rt = prim[nb];
       if (c.second_order) {
-        reconstruct(m, prim, grad, f, c.limiter, &L, &Rt);
+        reconstruct(m, prim, grad, f, c.limiter, &L, &Rt, c.frozen);
         if (!valid_prim(L) || !valid_prim(Rt)) {
           if (diag && diag->first_failure_face < 0) diag->first_failure_face = f;
           L = prim[o];

@@ -46,6 +46,9 @@ template <class R> struct CaseCfg {  // euler_solver.hpp:14-74 (mesh config excl
   int aoa0_symmetry_enforce_interval = 0;
   // AERO_ENABLE_EXPERIMENTAL_PRECOND, read by the reference at euler_solver.cpp:482-489.
   bool precond_experimental_enabled = false;
+  // Frozen limiter (beyond the reference): at the start of this iteration (second order) the limiter
+  // ratios are captured from the state and used from then on; < 0: never.
+  int limiter_freeze_iter = -1;
 };
 
 // One history row per iteration (the residuals.csv / forces.csv / diagnostics columns that the
@@ -203,6 +206,7 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
   std::vector<Prim<R>> prim(static_cast<std::size_t>(N));
   std::vector<R> grad, res, resp, lam, dt, un, us, nxt;
   std::vector<R> cell_p(static_cast<std::size_t>(N), c.p_inf);
+  std::vector<double> frozen;  // frozen limiter ratios once captured
 
   for (int iter = 0; iter < c.iterations; ++iter) {  // :771
     const bool second = iter >= std::max(c.startup_first_order_iters, 0);
@@ -224,6 +228,14 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
     }
     un = U;
     us = un;
+    if (second && c.limiter_freeze_iter >= 0 && iter == c.limiter_freeze_iter) {
+      for (int i = 0; i < N; ++i) {
+        Cons<R> s{us[5 * i + 0], us[5 * i + 1], us[5 * i + 2], us[5 * i + 3], us[5 * i + 4]};
+        prim[i] = cons_to_prim(s, g);
+      }
+      cell_gradients(m, prim, &grad);
+      limiter_ratios(m, prim, grad, c.limiter, &frozen);
+    }
     AssemblyDiag iter_diag;
     for (int st = 0; st < c.rk_stages; ++st) {  // :802-900
       for (int i = 0; i < N; ++i) {
@@ -243,6 +255,7 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
       ac.low_mach_fix = c.low_mach_fix;
       ac.low_mach_cutoff = c.mach_cutoff;
       ac.stab_mach_floor = floor_cur;
+      ac.frozen = frozen.empty() ? nullptr : frozen.data();
       ac.precond_farfield_bc = c.precond_farfield_bc;
       ac.precond = pc;
       ac.as_fix = as_fix_eff;

@@ -40,6 +40,7 @@ typedef struct ao_residual_cfg {
   double mach_ref, mach_min, beta_min, beta_max;
   int as_fix;
   double as_cutoff, as_fmin, as_ramp;
+  int limiter_frozen;  // the frozen-ratio entry points below take the ratios explicitly
 } ao_residual_cfg;
 
 // EulerAirfoilCaseConfig scalars (euler_solver.hpp:14-74), carried as double / int.
@@ -69,6 +70,7 @@ typedef struct ao_case_cfg {
   double all_speed_cfl_drop_factor, all_speed_cfl_restore_factor, all_speed_cfl_min_scale;
   int aoa0_symmetry_enforce, aoa0_symmetry_enforce_interval;
   int precond_experimental_enabled;
+  int limiter_freeze_iter;
 } ao_case_cfg;
 
 int ao_hist_fields(void) { return kHistFields; }
@@ -179,6 +181,7 @@ template <class R> CaseCfg<R> ccfg(const ao_case_cfg* c) {
   o.aoa0_symmetry_enforce = c->aoa0_symmetry_enforce != 0;
   o.aoa0_symmetry_enforce_interval = c->aoa0_symmetry_enforce_interval;
   o.precond_experimental_enabled = c->precond_experimental_enabled != 0;
+  o.limiter_freeze_iter = c->limiter_freeze_iter;
   return o;
 }
 
@@ -401,6 +404,41 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
 AO_EXPORT_ALL(f32, float)
 AO_EXPORT_ALL(f64, double)
 
+// ---- frozen limiter (FP64) ------------------------------------------------------------------------
+// Limiter ratios [8][F] at the state U (gradients of U, second order).
+extern "C" void ao_limiter_capture_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, double* ratios) {
+  const MeshView<double> v = view<double>(m);
+  const AssemblyCfg<double> a = acfg<double>(c);
+  const std::vector<Prim<double>> p = prims(v.num_cells, U, a.gamma);
+  std::vector<double> gr, out;
+  cell_gradients(v, p, &gr);
+  limiter_ratios(v, p, gr, a.limiter, &out);
+  std::memcpy(ratios, out.data(), out.size() * sizeof(double));
+}
+
+// Residual with frozen limiter ratios (second order).
+extern "C" void ao_residual_frozen_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U,
+                                       const double* ratios, double* res, double* lam, int* diag) {
+  const MeshView<double> v = view<double>(m);
+  AssemblyCfg<double> a = acfg<double>(c);
+  a.frozen = ratios;
+  const std::vector<Prim<double>> p = prims(v.num_cells, U, a.gamma);
+  std::vector<double> gr, r, l;
+  if (a.second_order)
+    cell_gradients(v, p, &gr);
+  else
+    gr.assign(static_cast<std::size_t>(v.num_cells) * 10, 0.0);
+  AssemblyDiag d;
+  assemble_residual(v, a, p, gr, &r, &l, &d);
+  std::memcpy(res, r.data(), r.size() * sizeof(double));
+  std::memcpy(lam, l.data(), l.size() * sizeof(double));
+  if (diag) {
+    diag[0] = d.first_order_fallback;
+    diag[1] = d.diffusive_fallback;
+    diag[2] = d.first_failure_face;
+  }
+}
+
 // ---- tangent (dual-number) evaluation of the FP64 residual ---------------------------------------
 #include "oracle_dual.hpp"
 
@@ -420,7 +458,7 @@ extern "C" int ao_sizeof_tangent_seed(void) { return static_cast<int>(sizeof(ao_
 
 // R and dR = R_U dU + R_p seed (dU and seed may be NULL) of the FP64 oracle residual.
 extern "C" void ao_residual_jvp_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, const double* dU,
-                                    const ao_tangent_seed* seed, double* res, double* dres) {
+                                    const ao_tangent_seed* seed, const double* frozen, double* res, double* dres) {
   using D = ODual;
   const int N = m->num_cells, F = m->num_faces;
   auto dual_vec = [](const void* val, const double* der, std::size_t n) {
@@ -473,6 +511,7 @@ extern "C" void ao_residual_jvp_f64(const ao_mesh* m, const ao_residual_cfg* c, 
   a.as_cutoff = D(c->as_cutoff);
   a.as_fmin = D(c->as_fmin);
   a.as_ramp = D(c->as_ramp);
+  a.frozen = frozen;
   const std::vector<Prim<D>> p = prims(N, u.data(), a.gamma);
   std::vector<D> gr;
   if (a.second_order)

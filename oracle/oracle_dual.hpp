@@ -56,6 +56,7 @@ inline ODual ao_sqrt(const ODual& a) {
 }
 inline ODual ao_abs(const ODual& a) { return a.v < 0.0 ? -a : a; }
 inline bool ao_isfinite(const ODual& a) { return std::isfinite(a.v); }
+inline double ao_value(const ODual& a) { return a.v; }
 
 template <> constexpr ODual kc<ODual>(float, double d) { return ODual(d); }
 

@@ -25,7 +25,7 @@ def res_cfg(vals: dict, dtype=np.float64) -> hl.ResidualConfig:
     names = [n for n, _ in hl.ResidualConfig._fields_]
     args = []
     for n, t in hl.ResidualConfig._fields_:
-        v = vals[n]
+        v = vals.get(n, 0)
         if n == "farfield":
             args.append((C.c_double * 5)(*[cast(x) for x in v]))
         elif t is C.c_double:

@@ -140,14 +140,33 @@ def time_residual(om: OracleMesh, cfg, U, n_evals: int) -> float:
     return getattr(oracle(), f"ao_time_residual_{suffix(dt)}")(C.byref(om.c), C.byref(cfg), ptr(U), n_evals)
 
 
-def residual_jvp(om: OracleMesh, cfg, U, dU=None, seed=None):
-    """FP64 oracle residual and its tangent dR = R_U dU + R_p . seed (seed: aero_hip.Seed or None)."""
+def residual_jvp(om: OracleMesh, cfg, U, dU=None, seed=None, frozen=None):
+    """FP64 oracle residual and its tangent dR = R_U dU + R_p . seed (seed: aero_hip.Seed or None;
+    frozen: limiter ratios [8, F] or None)."""
     assert om.dtype == np.float64
     U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
     dUa = None if dU is None else np.ascontiguousarray(dU, np.float64).reshape(om.N, 5)
     R = np.zeros((om.N, 5))
     dR = np.zeros((om.N, 5))
     lib = oracle()
+    fr = None if frozen is None else np.ascontiguousarray(frozen, np.float64)
     lib.ao_residual_jvp_f64(C.byref(om.c), C.byref(cfg), ptr(U), None if dUa is None else ptr(dUa),
-                            None if seed is None else seed.ptr(), ptr(R), ptr(dR))
+                            None if seed is None else seed.ptr(), None if fr is None else ptr(fr), ptr(R), ptr(dR))
     return R, dR
+
+
+def limiter_capture(om: OracleMesh, cfg, U):
+    U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
+    out = np.zeros((8, om.F))
+    oracle().ao_limiter_capture_f64(C.byref(om.c), C.byref(cfg), ptr(U), ptr(out))
+    return out
+
+
+def residual_frozen(om: OracleMesh, cfg, U, ratios):
+    U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
+    fr = np.ascontiguousarray(ratios, np.float64)
+    R = np.zeros((om.N, 5))
+    lam = np.zeros(om.N)
+    diag = np.zeros(3, np.int32)
+    oracle().ao_residual_frozen_f64(C.byref(om.c), C.byref(cfg), ptr(U), ptr(fr), ptr(R), ptr(lam), ptr(diag))
+    return R, lam, diag

@@ -221,6 +221,10 @@ def test_sensitivities_adjoint_tangent_and_finite_differences(hip):
             assert ti["res"] <= 1e-11 * ti["res0"], ti
             d_tan = J_p + float(np.sum(ob["dJdU"] * v))
             assert abs(d_adj - d_tan) <= 1e-8 * abs(d_adj), (param, d_adj, d_tan)
+            say(f"[sens] adjoint GMRES {param}")
+            psi_g, gi = hm.adjoint_solve(rc, ob["dJdU"], U, cfl=1.5, max_iters=3000, tol=1e-12, gmres_restart=30)
+            assert gi["res"] <= 1e-12 * gi["res0"] and gi["iterations"] < ai["iterations"], (gi, ai["iterations"])
+            assert abs(J_p - float(np.sum(psi_g * R_p)) - d_adj) <= 1e-8 * abs(d_adj)
             results[param] = d_adj
         h = {"aoa": 1e-3, "mach": 1e-4}
         for param in ("aoa", "mach"):
@@ -233,3 +237,39 @@ def test_sensitivities_adjoint_tangent_and_finite_differences(hip):
                 cls.append(o["cl"])
             fd = (cls[0] - cls[1]) / (2 * h[param])
             assert abs(results[param] - fd) <= 1e-5 * abs(fd), (param, results[param], fd)
+
+
+def test_shape_sensitivities(hip):
+    """Hicks-Henne amplitudes on a deformed 40x8 mesh (first-order Rusanov, converged to round-off):
+    adjoint dCL/da_i equals the tangent solve and central differences of converged CL over
+    re-deformed meshes."""
+    from aero_hip.adjoint import sensitivities
+    base = helpers.mesh("40x8")
+    rng = np.random.default_rng(4242)
+    amp = rng.uniform(-0.002, 0.002, 20)
+    m, tan = hip.deform_mesh(base, amp)
+    iters = 9000
+    case = _converged_case(hip, m, 2.0, 0.5, iters)
+    with hip.HipMesh(m) as hm:
+        say("[shape] primal")
+        out = hm.run_case(case)
+        assert out["history"][-1, 8] / out["history"][0, 8] < 1e-11
+        params = [2, 7, 13]
+        say("[shape] adjoint + tangent")
+        s = sensitivities(hm, case, out["conserved"], shape_tangents=tan, shape_params=params, cfl=1.5, tol=1e-12,
+                          tangent=True)
+        assert s["adjoint"]["res"] <= 1e-11 * s["adjoint"]["res0"]
+        for name in s["d"]:
+            assert abs(s["d"][name] - s["d_tangent"][name]) <= 1e-8 * max(abs(s["d"][name]), 1e-12), name
+    h = 1e-5
+    for i in params:
+        cls = []
+        for sgn in (1, -1):
+            e = np.zeros(20)
+            e[i] = sgn * h
+            mi, _ = hip.deform_mesh(base, amp + e, tangents=False)
+            say(f"[shape] finite difference a{i} {sgn:+d}")
+            with hip.HipMesh(mi) as hmi:
+                cls.append(hmi.run_case(case)["cl"])
+        fd = (cls[0] - cls[1]) / (2 * h)
+        assert abs(s["d"][f"shape{i}"] - fd) <= 1e-5 * abs(fd), (i, s["d"][f"shape{i}"], fd)
