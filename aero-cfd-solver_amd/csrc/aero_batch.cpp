@@ -37,7 +37,8 @@ struct aero_hip_batch {
   int hist_cap = 0;
   aero_case_config cfg{};
   FaceOpts base{};
-  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [second order][start buffer]
+  hipGraphExec_t graph[2][2][2] = {};  // [second order][start buffer][frozen limiter]
+  bool frozen = false;                  // limiter ratios captured (limiter_freeze_iter reached)
   std::vector<double> alpha, q_inf;
   int state_buf = 0;  // ring buffer that holds every case's current state
 };
@@ -74,8 +75,9 @@ void free_batch(aero_hip_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto& row : b->graph)
-    for (auto& g : row)
-      if (g) (void)hipGraphExecDestroy(g);
+    for (auto& col : row)
+      for (auto& g : col)
+        if (g) (void)hipGraphExecDestroy(g);
   for (void* p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -85,7 +87,7 @@ void free_batch(aero_hip_batch* b) {
 double clampd(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
 // One pseudo-time iteration of every active case, enqueued on the batch stream.
-int enqueue_iteration(aero_hip_batch* b, bool second, int start, char* err, size_t errlen) {
+int enqueue_iteration(aero_hip_batch* b, bool second, int start, bool frozen, char* err, size_t errlen) {
   const aero_case_config& c = b->cfg;
   AERO_TRY(aero_host::launch_batch_init_diag(b->d_cases, b->ncases, b->stream), "k_batch_init_diag");
   // Ring: RK3 ends in its start buffer (stage 2 updates in place: each thread reads U^n of its own cell
@@ -106,8 +108,8 @@ int enqueue_iteration(aero_hip_batch* b, bool second, int start, char* err, size
       AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
-    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, b->d_geoms, b->d_cases, b->d_ctl,
-                                          b->d_work, b->nwork, b->smax, b->base, bs, b->stream),
+    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->d_geoms, b->d_cases,
+                                          b->d_ctl, b->d_work, b->nwork, b->smax, b->base, bs, b->stream),
              "k_flux_batch");
     cur = bs.out;
     final_buf = bs.out;
@@ -139,10 +141,6 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
   }
   if (c->all_speed_staged_controller) {
     set_err(err, errlen, "aero_hip_batch_create: the staged all-speed controller is single-case only");
-    return AERO_EINVAL;
-  }
-  if (c->limiter_freeze_iter >= 0) {
-    set_err(err, errlen, "aero_hip_batch_create: the frozen limiter is single-case only");
     return AERO_EINVAL;
   }
   const int BC = meshes[0]->BC;
@@ -205,6 +203,11 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     cs.R = nullptr;
     cs.lam = nullptr;
     AERO_BSETUP(balloc(b, &cs.partials, 4 * static_cast<size_t>(m->nblk)));
+    cs.lim_ratio = nullptr;
+    if (c->limiter_freeze_iter >= 0) {  // frozen limiter (beyond the reference); boundary faces keep 0
+      AERO_BSETUP(balloc(b, &cs.lim_ratio, 8 * static_cast<size_t>(m->F)));
+      AERO_BSETUP(hipMemset(cs.lim_ratio, 0, 8 * static_cast<size_t>(m->F) * sizeof(double)));
+    }
     AERO_BSETUP(balloc(b, &cs.diag, 4));
     // freestream of this case (euler_solver.cpp:543-561) and its schedules
     const double mach = cases[i].mach;
@@ -328,28 +331,57 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   const int startup = std::max(c.startup_first_order_iters, 0);
   const int sync_every = std::max(iters_per_sync, 1);
   auto enqueue = [&](bool second, int start) -> int {
-    int rc = enqueue_iteration(b, second, start, err, errlen);
+    int rc = enqueue_iteration(b, second, start, b->frozen, err, errlen);
     if (rc) return rc;
     AERO_TRY(aero_host::launch_batch_control(b->d_cases, b->d_ctl, p, b->stream), "k_batch_control");
     return 0;
   };
   // Graphs are specific to (order, start buffer) and to the run parameters; rebuild them per run.
   for (auto& row : b->graph)
-    for (auto& gx : row)
-      if (gx) {
-        (void)hipGraphExecDestroy(gx);
-        gx = nullptr;
-      }
+    for (auto& col : row)
+      for (auto& gx : col)
+        if (gx) {
+          (void)hipGraphExecDestroy(gx);
+          gx = nullptr;
+        }
+  // Frozen limiter (beyond the reference): at the start of iteration limiter_freeze_iter (second order)
+  // every active case captures its ratios; the iterations run in chunks that end there.
+  const int freeze_at =
+    (c.limiter_freeze_iter >= 0 && !b->frozen) ? std::max(c.limiter_freeze_iter, std::max(c.startup_first_order_iters, 0))
+                                               : -1;
+  // Iterations of earlier runs; cases an earlier run stopped at its cap (not converged, not failed) resume.
+  int done_iters = 0;
+  {
+    std::vector<CaseCtl> h0(b->ncases);
+    AERO_TRY(hipMemcpy(h0.data(), b->d_ctl, h0.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost), "control readback");
+    for (auto& k : h0) {
+      done_iters = std::max(done_iters, k.iter);
+      if (!k.converged && !k.failed) k.active = 1;
+    }
+    AERO_TRY(hipMemcpy(b->d_ctl, h0.data(), h0.size() * sizeof(CaseCtl), hipMemcpyHostToDevice), "control resume");
+  }
+  p.max_iters = done_iters + max_iters;
   std::vector<CaseCtl> h(b->ncases);
   const auto t0 = std::chrono::steady_clock::now();
   int it = 0, start = b->state_buf;
   while (it < max_iters) {
-    const bool second = it >= startup;
-    const int phase_end = second ? max_iters : std::min(startup, max_iters);
+    const int g_it = done_iters + it;  // the cases' iteration index
+    const bool second = g_it >= startup;
+    if (freeze_at >= 0 && !b->frozen && g_it == freeze_at) {
+      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
+                                                b->smax, c.gamma, start, b->stream),
+               "k_gradient_batch");
+      AERO_TRY(aero_host::launch_batch_limiter_capture(b->BC, c.limiter, b->d_geoms, b->d_cases, b->d_ctl, b->d_work,
+                                                       b->nwork, c.gamma, start, b->stream),
+               "k_limiter_capture_batch");
+      b->frozen = true;
+    }
+    int phase_end = second ? max_iters : std::min(startup - done_iters, max_iters);
+    if (freeze_at >= 0 && !b->frozen && g_it < freeze_at) phase_end = std::min(phase_end, freeze_at - done_iters);
     const int n = std::min(sync_every, phase_end - it);
     for (int k = 0; k < n; ++k) {
       if (use_graph) {
-        hipGraphExec_t& gx = b->graph[second ? 1 : 0][start];
+        hipGraphExec_t& gx = b->graph[second ? 1 : 0][start][b->frozen ? 1 : 0];
         if (!gx) {
           hipGraph_t graph = nullptr;
           AERO_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

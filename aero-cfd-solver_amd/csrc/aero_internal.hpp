@@ -78,6 +78,7 @@ struct BatchCase {
   double floor_start, floor_target;         // stabilisation-floor schedule (euler_solver.cpp:141-157)
   double mach_ref, mach_min, beta_min, beta_max;  // build_precondition_config (euler_solver.cpp:341-355)
   WallArgs wall;     // wall faces of the geometry + this case's reference values
+  double* lim_ratio; // frozen limiter ratios [8][F] (beyond the reference) or null
 };
 
 // Device-resident control state of one case, advanced by k_batch_control after every iteration with the
@@ -166,10 +167,14 @@ hipError_t launch_iter_reduce(const aero_dev::DevMesh& m, const double* partials
 hipError_t launch_batch_gradient(int BC, int grad, const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
                                  const aero_dev::CaseCtl* ctl, const int2* work, int nwork, int smax, double g, int cur,
                                  hipStream_t s);
-hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, const aero_dev::DevMesh* geoms,
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const aero_dev::DevMesh* geoms,
                              const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl, const int2* work,
                              int nwork, int smax, const aero_dev::FaceOpts& base, const aero_dev::BatchStage& bs,
                              hipStream_t s);
+// Frozen limiter capture of every active case at its state U[cur] (gradients in G).
+hipError_t launch_batch_limiter_capture(int BC, int limiter, const aero_dev::DevMesh* geoms,
+                                        const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl,
+                                        const int2* work, int nwork, double g, int cur, hipStream_t s);
 hipError_t launch_batch_reduce(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl, int ncases, double g, int cur,
                                hipStream_t s);
 hipError_t launch_batch_control(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl,

@@ -454,11 +454,9 @@ __global__ __launch_bounds__(BC) void k_flux(DevMesh m, FaceOpts o, const double
 // state U and its gradients G, stored at ratios[j * F + face], j = owner rho, u, v, p, neighbour
 // rho, u, v, p.
 template <int BC, int LIMITER>
-__global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double* __restrict__ U,
-                                                        const double* __restrict__ G, double g,
-                                                        double* __restrict__ ratios) {
-  const int b = block_of(m);
-  if (b >= m.nblk) return;
+__device__ __forceinline__ void limiter_capture_body(const DevMesh& m, int b, const double* __restrict__ U,
+                                                     const double* __restrict__ G, double g,
+                                                     double* __restrict__ ratios) {
   const int s0 = m.slot_off[b];
   const int ns = m.slot_off[b + 1] - s0;
   const int ld = m.ld, F = m.nfaces;
@@ -482,6 +480,15 @@ __global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double*
       ratios[(4 + k) * F + f] = rn;
     }
   }
+}
+
+template <int BC, int LIMITER>
+__global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double* __restrict__ U,
+                                                        const double* __restrict__ G, double g,
+                                                        double* __restrict__ ratios) {
+  const int b = block_of(m);
+  if (b >= m.nblk) return;
+  limiter_capture_body<BC, LIMITER>(m, b, U, G, g, ratios);
 }
 
 // Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict
   gradient_body<BC, GK>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, lds);
 }
 
-template <int BC, int SCHEME, int LIMITER, bool SECOND>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
 __global__ __launch_bounds__(BC) void k_flux_batch(const DevMesh* __restrict__ geoms,
                                                     const BatchCase* __restrict__ cases,
                                                     const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
@@ -696,6 +703,7 @@ __global__ __launch_bounds__(BC) void k_flux_batch(const DevMesh* __restrict__ g
   o.mach_min = cs.mach_min;
   o.beta_min = cs.beta_min;
   o.beta_max = cs.beta_max;
+  o.frozen = FROZEN ? cs.lim_ratio : nullptr;
   StageArgs st;
   st.stage = bs.stage;
   st.last_stage = bs.last;
@@ -704,8 +712,19 @@ __global__ __launch_bounds__(BC) void k_flux_batch(const DevMesh* __restrict__ g
   st.Un = cs.U[bs.base];
   st.Unext = cs.U[bs.out];
   st.partials = cs.partials;
-  flux_body<BC, SCHEME, LIMITER, SECOND, 1>(geoms[cs.geom], w.y, o, cs.U[bs.cur], cs.G, nullptr, nullptr, cs.diag, st,
-                                            lds);
+  flux_body<BC, SCHEME, LIMITER, SECOND, 1, FROZEN>(geoms[cs.geom], w.y, o, cs.U[bs.cur], cs.G, nullptr, nullptr,
+                                                    cs.diag, st, lds);
+}
+
+template <int BC, int LIMITER>
+__global__ __launch_bounds__(BC) void k_limiter_capture_batch(const DevMesh* __restrict__ geoms,
+                                                              const BatchCase* __restrict__ cases,
+                                                              const CaseCtl* __restrict__ ctl,
+                                                              const int2* __restrict__ work, double g, int cur) {
+  const int2 w = work[blockIdx.x];
+  if (w.x < 0 || !ctl[w.x].active) return;
+  const BatchCase& cs = cases[w.x];
+  limiter_capture_body<BC, LIMITER>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, cs.lim_ratio);
 }
 
 // Per case: norm partials + wall force sums of the updated state into ctl[case].sums (same fixed-order
@@ -771,10 +790,9 @@ __global__ void k_batch_control(const BatchCase* __restrict__ cases, CaseCtl* __
   }
   k.iter += 1;
   if (!p.fixed_iters && k.rho_ratio <= p.rho_drop) k.converged = 1;
-  if (k.converged || k.iter >= p.max_iters) {
-    k.active = 0;
-    return;
-  }
+  // The next iteration's parameters are set even when the case stops here, so that a later run resumes
+  // it exactly where it left off.
+  if (k.converged || k.iter >= p.max_iters) k.active = 0;
   const int it = k.iter;
   double base;
   if (p.cfl_ramp <= 0) {
@@ -942,9 +960,18 @@ hipError_t launch_batch_gradient(int BC, int grad, const DevMesh* geoms, const B
 }
 
 template <int BC>
-static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, const DevMesh* geoms,
+static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
                                        const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork,
                                        size_t lds, const FaceOpts& o, const BatchStage& bs, hipStream_t s) {
+  if (second && frozen) {  // the frozen ratios replace the limiter
+    if (scheme == 0)
+      hipLaunchKernelGGL((k_flux_batch<BC, 0, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o,
+                         bs);
+    else
+      hipLaunchKernelGGL((k_flux_batch<BC, 1, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o,
+                         bs);
+    return hipGetLastError();
+  }
 #define AERO_BFLUX(SC, LI, SE)                                                                              \
   if (scheme == SC && (limiter == LI || !SE) && second == SE) {                                             \
     hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o, bs); \
@@ -960,16 +987,39 @@ static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, con
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, const DevMesh* geoms,
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
                              const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork, int smax,
                              const FaceOpts& base, const BatchStage& bs, hipStream_t s) {
   const size_t lds = std::max<size_t>(static_cast<size_t>(smax) * 5, 4 * (BC / 64)) * sizeof(double);
   switch (BC) {
-    case 128: return launch_batch_flux_bc<128>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
-    case 256: return launch_batch_flux_bc<256>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
-    case 512: return launch_batch_flux_bc<512>(scheme, limiter, second, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    case 128:
+      return launch_batch_flux_bc<128>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    case 256:
+      return launch_batch_flux_bc<256>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+    case 512:
+      return launch_batch_flux_bc<512>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_batch_limiter_capture(int BC, int limiter, const DevMesh* geoms, const BatchCase* cases,
+                                        const CaseCtl* ctl, const int2* work, int nwork, double g, int cur,
+                                        hipStream_t s) {
+#define AERO_BCAP(BCV)                                                                                             \
+  if (BC == BCV) {                                                                                                 \
+    if (limiter == 1)                                                                                              \
+      hipLaunchKernelGGL((k_limiter_capture_batch<BCV, 1>), dim3(nwork), dim3(BCV), 0, s, geoms, cases, ctl, work, g, \
+                         cur);                                                                                     \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_limiter_capture_batch<BCV, 0>), dim3(nwork), dim3(BCV), 0, s, geoms, cases, ctl, work, g, \
+                         cur);                                                                                     \
+    return hipGetLastError();                                                                                      \
+  }
+  AERO_BCAP(128)
+  AERO_BCAP(256)
+  AERO_BCAP(512)
+#undef AERO_BCAP
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_batch_reduce(const BatchCase* cases, CaseCtl* ctl, int ncases, double g, int cur, hipStream_t s) {

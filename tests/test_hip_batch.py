@@ -96,3 +96,29 @@ def test_converged_cases_are_masked(hip):
         ref = hm.run_case(case)
     assert np.array_equal(s0, ref["conserved"]), "a masked case must keep its state from the iteration it converged"
     g.close()
+
+
+def test_batch_frozen_limiter_matches_solo_runs(hip):
+    """With a freeze iteration (an option beyond the reference) the batch captures every case's limiter
+    ratios at the same iteration as the solo driver, and stays bitwise identical to the solo runs, also
+    when the batch runs in two calls around the freeze iteration."""
+    m1 = helpers.mesh("40x8")
+    m2 = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    cases = [(0, 0.3, 1.0), (1, 0.5, 2.5), (1, 0.6, -2.0)]
+    iters, freeze = 40, 17
+    base = dict(BASE, limiter_freeze_iter=freeze)
+    g1, g2 = hip.HipMesh(m1), hip.HipMesh(m2)
+    say("[batch] frozen-limiter batch (two runs around the freeze)")
+    with hip.HipBatch([g1, g2], cases, hip.CaseConfig(**base), hist_capacity=iters) as b:
+        b.run(11, fixed_iters=True, iters_per_sync=4, use_graph=True)
+        b.run(iters - 11, fixed_iters=True, iters_per_sync=5, use_graph=True)
+        states = [b.state(i) for i in range(len(cases))]
+        hists = [b.history(i) for i in range(len(cases))]
+    for i, (gi, mach, aoa) in enumerate(cases):
+        case = hip.CaseConfig(**dict(base, iterations=iters, mach=mach, aoa_deg=aoa))
+        with hip.HipMesh([m1, m2][gi]) as hm:
+            ref = hm.run_case(case)
+        assert np.array_equal(states[i], ref["conserved"]), f"case {i} state differs from its solo run"
+        assert np.array_equal(hists[i][:, 1], ref["history"][:, 8]), f"case {i} density L2 history differs"
+    g1.close()
+    g2.close()
