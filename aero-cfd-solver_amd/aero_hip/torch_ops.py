@@ -1,0 +1,80 @@
+"""torch.autograd.Function wrappers over the device residual and its discrete adjoint.
+
+* ``residual(U, hm, cfg)``: R(U) [N, 5] with backward R_U^T grad (aero_hip_residual_vjp), so the residual
+  can sit inside any torch computation.
+* ``steady_coefficients(params, base_mesh, case, ...)``: (CL, CD) of the converged solution for
+  params = [aoa_deg, mach, a_1 .. a_P] (Hicks-Henne amplitudes deforming ``base_mesh``); the backward
+  pass runs one adjoint solve for J = grad_CL * CL + grad_CD * CD and returns dJ/dparams.
+
+Tensors may live on any device; values cross the C ABI as FP64 host arrays.  Only the Python layer uses
+torch: the numerics are the HIP kernels behind ``aero_hip``.
+"""
+from __future__ import annotations
+
+from dataclasses import replace
+
+import numpy as np
+import torch
+
+from .adjoint import sensitivities
+from .mesh import deform_mesh
+from .solver import CaseConfig, HipMesh
+
+
+def _np(t: torch.Tensor) -> np.ndarray:
+    return t.detach().to("cpu", torch.float64).numpy()
+
+
+class ResidualFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U, hm, cfg):
+        R, _, _ = hm.residual(_np(U), cfg)
+        ctx.hm, ctx.cfg = hm, cfg
+        ctx.save_for_backward(U)
+        return torch.from_numpy(R).to(U.device)
+
+    @staticmethod
+    def backward(ctx, grad_R):
+        (U,) = ctx.saved_tensors
+        out = ctx.hm.residual_vjp(ctx.cfg, _np(grad_R), _np(U))
+        return torch.from_numpy(out).to(U.device, grad_R.dtype), None, None
+
+
+def residual(U: torch.Tensor, hm: HipMesh, cfg) -> torch.Tensor:
+    """Residual R(U) of ``hm``'s mesh under ``cfg`` as a differentiable torch op."""
+    return ResidualFunction.apply(U, hm, cfg)
+
+
+class SteadyCoefficients(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, base_mesh, case, shape_kw, solve_kw):
+        p = _np(params)
+        case_p = replace(case, aoa_deg=float(p[0]), mach=float(p[1]))
+        n_shape = len(p) - 2
+        tan = None
+        mesh = base_mesh
+        if n_shape > 0:
+            mesh, tan = deform_mesh(base_mesh, p[2:], bumps_per_surface=n_shape // 2, **shape_kw)
+        hm = HipMesh(mesh)
+        out = hm.run_case(case_p)
+        ctx.saved = (hm, case_p, out["conserved"], tan, n_shape, dict(solve_kw))
+        ctx.device = params.device
+        return torch.tensor([out["cl"], out["cd"]], dtype=torch.float64, device=params.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        hm, case_p, U, tan, n_shape, solve_kw = ctx.saved
+        g = _np(grad)
+        s = sensitivities(hm, case_p, U, weights=(float(g[0]), float(g[1]), 0.0), shape_tangents=tan, **solve_kw)
+        d = [s["d"]["aoa"], s["d"]["mach"]] + [s["d"][f"shape{i}"] for i in range(n_shape)]
+        hm.close()
+        ctx.saved = None
+        return torch.tensor(d, dtype=torch.float64, device=ctx.device), None, None, None, None
+
+
+def steady_coefficients(params: torch.Tensor, base_mesh, case: CaseConfig, shape_kw: dict | None = None,
+                        solve_kw: dict | None = None) -> torch.Tensor:
+    """(CL, CD) of the converged case at params = [aoa_deg, mach, amplitudes...], differentiable through
+    the discrete adjoint.  shape_kw: deform_mesh options (h_min, h_max, idw_exponent); solve_kw:
+    sensitivities options (cfl, tol, max_iters, gmres_restart, ...)."""
+    return SteadyCoefficients.apply(params, base_mesh, case, shape_kw or {}, solve_kw or {})

@@ -376,7 +376,15 @@ __device__ __forceinline__ C4T<T> char_farfield_state(const C4T<T>& in, T nx, T 
   const T rref = dmax(0.5 * (pi.r + pf.r), kFloor);
   const T aref = dmax(0.5 * (ai + af), kFloor);
   const T at = dmax(dclamp(s, T(1.0e-4), T(1.0)) * aref, 1.0e-6);
-  if (vabs(uni) >= at) return (uni >= 0.0) ? in : o.ff;
+  if (vabs(uni) >= at) {
+    C4T<T> r;  // supersonic normal flow: upwind state, selected per component
+    const bool out = uni >= 0.0;
+    r.r = out ? in.r : o.ff.r;
+    r.ru = out ? in.ru : o.ff.ru;
+    r.rv = out ? in.rv : o.ff.rv;
+    r.E = out ? in.E : o.ff.E;
+    return r;
+  }
   const T rp = uni + pi.p / (rref * at);
   const T rm = unf - pf.p / (rref * at);
   const T unb = 0.5 * (rp + rm);

@@ -102,8 +102,15 @@ template <int SCHEME, class T>
 __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
   const double g = o.g;
   const C4T<T> in = p2c(c2p(uo, g), g);
-  const T fb = wall ? face_beta(in, in, o) : face_beta(in, o.ff, o);
-  const T sc = acoustic_scale(in, wall ? in : o.ff, fb, o);
+  // The exterior state of beta and the acoustic scale: the interior state at a wall, the freestream at a
+  // farfield (selected per component, which keeps both in registers).
+  C4T<T> ext;
+  ext.r = wall ? in.r : o.ff.r;
+  ext.ru = wall ? in.ru : o.ff.ru;
+  ext.rv = wall ? in.rv : o.ff.rv;
+  ext.E = wall ? in.E : o.ff.E;
+  const T fb = face_beta(in, ext, o);
+  const T sc = acoustic_scale(in, ext, fb, o);
   FluxT<T> F;
   if (wall) {  // boundary_conditions.cpp:97-111
     const P4T<T> p = c2p(in, g);
@@ -116,7 +123,8 @@ __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T
     F.f3 = 0.0;
     F.fallback = 0;
   } else {  // boundary_conditions.cpp:113-122
-    const C4T<T> bs = o.char_bc ? char_farfield_state(in, nx, ny, sc, o) : o.ff;
+    C4T<T> bs = ext;
+    if (o.char_bc) bs = char_farfield_state(in, nx, ny, sc, o);
     F = face_flux<SCHEME>(in, bs, nx, ny, sc, o);
   }
   return F;

@@ -114,28 +114,34 @@ def run_sweep(args) -> int:
     aoas = np.linspace(-4.0, 8.0, 16)
     machs = [0.3, 0.5, 0.6, 0.7]
     cases = [(0, float(m), float(a)) for m in machs for a in aoas]
-    base = CaseConfig(flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=1.1, cfl_ramp_iters=120,
+    base = CaseConfig(flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.sweep_cfl, cfl_ramp_iters=120,
                       startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
-                      stab_ramp_iters=400, precond_experimental_enabled=0)
+                      stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter)
     cap = args.sweep_iters
-    log(f"[sweep] batched: {len(cases)} cases x {mesh.num_cells} cells, cap {cap} iterations")
+    log(f"[sweep] batched: {len(cases)} cases x {mesh.num_cells} cells, cap {cap} iterations, "
+        f"cfl {args.sweep_cfl}, limiter frozen at {args.freeze_iter}")
     with HipBatch([g], cases, base) as b:
         torch.cuda.synchronize()
-        rb = b.run(cap, 1e-8, iters_per_sync=100, use_graph=True)
+        rb = b.run(cap, 1e-8, iters_per_sync=500, use_graph=True)
     t_batch = rb["seconds"]
     nseq = min(args.sweep_seq_cases, len(cases))
     log(f"[sweep] sequential: {nseq} cases one after another")
     t_seq = 0.0
     seq_rows = []
     for i in range(nseq):
-        with HipBatch([g], [cases[i]], base) as b1:
+        j = (i * len(cases)) // nseq  # spread the timed cases over the sweep
+        log(f"[sweep] sequential case {j}")
+        with HipBatch([g], [cases[j]], base) as b1:
             r1 = b1.run(cap, 1e-8, iters_per_sync=1, use_graph=False)
         t_seq += r1["seconds"]
-        seq_rows.append(r1["cases"][0])
+        seq_rows.append((j, r1["cases"][0]))
     # each sequential case ran the same iterations as in the batch (identical per-case arithmetic)
-    same = all(seq_rows[i]["iterations"] == rb["cases"][i]["iterations"] and
-               seq_rows[i]["rho_l2"] == rb["cases"][i]["rho_l2"] for i in range(nseq))
-    t_seq_all = t_seq * len(cases) / nseq
+    same = all(r["iterations"] == rb["cases"][j]["iterations"] and r["rho_l2"] == rb["cases"][j]["rho_l2"]
+               for j, r in seq_rows)
+    # extrapolate by iterations: the timed cases' seconds per iteration times every case's iterations
+    it_timed = sum(r["iterations"] for _, r in seq_rows)
+    it_all = sum(r["iterations"] for r in rb["cases"])
+    t_seq_all = t_seq / max(it_timed, 1) * it_all
     g.close()
     rows = rb["cases"]
     value = len(cases) / t_batch * 3600.0
@@ -144,9 +150,13 @@ def run_sweep(args) -> int:
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "sweep64_mixed66k_hllc_venkat", "cells": mesh.num_cells, "cases": len(cases),
                       "mesh": mkw, "aoa_deg": [float(a) for a in aoas], "mach": machs, "iteration_cap": cap,
-                      "rho_drop_target": 1e-8},
+                      "rho_drop_target": 1e-8, "cfl_max": args.sweep_cfl, "limiter_freeze_iter": args.freeze_iter},
            "sweep": {"batched_seconds": round(t_batch, 3), "sequential_seconds": round(t_seq_all, 3),
-                     "sequential_cases_timed": nseq, "speedup_vs_sequential": round(t_seq_all / t_batch, 2),
+                     "sequential_cases_timed": nseq, "sequential_extrapolation": "seconds per iteration of the "
+                     "timed cases x all cases' iterations",
+                     "speedup_vs_sequential": round(t_seq_all / t_batch, 2),
+                     "iterations_max": max(r["iterations"] for r in rows),
+                     "iterations_median": float(np.median([r["iterations"] for r in rows])),
                      "sequential_solves_per_hr": round(len(cases) / t_seq_all * 3600.0, 1),
                      "sequential_matches_batched": same,
                      "converged_8_decades": sum(r["converged"] for r in rows),
@@ -167,8 +177,11 @@ def main() -> int:
     ap.add_argument("--no-renumber", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample length (0 = skip)")
     ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
-    ap.add_argument("--sweep-iters", type=int, default=3000, help="iteration cap per case in --sweep")
-    ap.add_argument("--sweep-seq-cases", type=int, default=64, help="cases timed sequentially in --sweep")
+    ap.add_argument("--sweep-iters", type=int, default=60000, help="iteration cap per case in --sweep")
+    ap.add_argument("--sweep-seq-cases", type=int, default=4, help="cases timed sequentially in --sweep")
+    ap.add_argument("--sweep-cfl", type=float, default=2.0, help="CFL after the ramp in --sweep")
+    ap.add_argument("--freeze-iter", type=int, default=1000,
+                    help="--sweep / --adjoint: iteration at which the limiter is frozen (-1: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
