@@ -196,6 +196,12 @@ class HipMesh:
         return R, lam, dict(first_order_fallback=d.first_order_fallback, diffusive_fallback=d.diffusive_fallback,
                             first_failure_face=d.first_failure_face)
 
+    def state(self) -> np.ndarray:
+        """The handle's current conserved state [N, 5] (reference numbering)."""
+        U = np.zeros((self.N, 5))
+        self._call("aero_hip_get_state", _pd(U))
+        return U
+
     def gradients(self) -> np.ndarray:
         g = np.zeros((self.N, 5, 2))
         self._call("aero_hip_get_gradients", _pd(g))
@@ -247,6 +253,19 @@ class HipMesh:
         out = np.zeros((8, self.F))
         self._call("aero_hip_freeze_limiter", C.byref(cfg), pu, _pd(out))
         return out
+
+    def iterate(self, cfg: _lib.ResidualConfig, n_iters: int, cfl: float, rk_stages: int = 3,
+                history: bool = True):
+        """n fixed-configuration pseudo-time iterations on the handle's state (aero_hip_iterate); returns the
+        density-residual L2 after each iteration, or None."""
+        h = np.zeros(max(n_iters, 1)) if history else None
+        self._call("aero_hip_iterate", C.byref(cfg), n_iters, rk_stages, cfl, _pd(h) if history else None)
+        return h[:n_iters] if history else None
+
+    def set_limiter_ratios(self, ratios: np.ndarray):
+        """Install frozen limiter ratios [8, F] (e.g. a batch case's)."""
+        r = np.ascontiguousarray(ratios, dtype=np.float64).reshape(8, self.F)
+        self._call("aero_hip_set_limiter_ratios", _pd(r))
 
     # ---- tangent / adjoint ------------------------------------------------------------------------
     def _state_ptr(self, U):
@@ -419,3 +438,12 @@ class HipBatch:
         _lib.check(self.lib.aero_hip_batch_history(self.h, i, _pd(out), self.hist_capacity, err, len(err)), err,
                    "batch history")
         return out[: self.hist_capacity]
+
+    def limiter_ratios(self, i: int) -> np.ndarray:
+        """Case i's frozen limiter ratios [8, F] (batches created with a freeze iteration)."""
+        mesh = self.meshes[self.cases[i][0]]
+        out = np.zeros((8, mesh.F))
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_limiter_ratios(self.h, i, _pd(out), err, len(err)), err,
+                   "batch limiter ratios")
+        return out

@@ -34,7 +34,6 @@ struct AdjointData {
   double* R2 = nullptr;    // [4][N]
   double* dR = nullptr;    // [4][N]
   double* J = nullptr;     // [96][F] face Jacobians
-  double* T = nullptr;     // [24][F]
   double* Gbar = nullptr;  // [8][N]
   double* out = nullptr;   // [4][N]
   double* psi = nullptr;   // [4][N]
@@ -43,6 +42,7 @@ struct AdjointData {
   double* partials = nullptr;
   double* hist = nullptr;  // [hist_cap] squared residual norms
   int hist_cap = 0;
+  int* hist_ctr = nullptr;  // device-side index into hist
   double* coef = nullptr;  // [wall_count]
   double* geo = nullptr;   // [5F + 3N] geometry tangents
   // GMRES(m): Krylov basis [m + 1][4N], work vectors, dot-product scratch
@@ -201,6 +201,7 @@ int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
   for (int i = 0; i < 3; ++i) AERO_TRY(aalloc(a, &a->x[i], n4), "adjoint setup");
   AERO_TRY(aalloc(a, &a->rhs, n4), "adjoint setup");
   AERO_TRY(aalloc(a, &a->partials, static_cast<size_t>(a->nparts)), "adjoint setup");
+  AERO_TRY(aalloc(a, &a->hist_ctr, 1), "adjoint setup");
   AERO_TRY(aalloc(a, &a->coef, static_cast<size_t>(std::max(m->wall_count, 1))), "adjoint setup");
   AERO_TRY(aalloc(a, &a->geo, 5 * static_cast<size_t>(F) + 3 * static_cast<size_t>(N)), "adjoint setup");
   return AERO_OK;
@@ -210,7 +211,6 @@ int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
 int ensure_jacobian_buffers(aero_hip_mesh* m, char* err, size_t errlen) {
   AdjointData* a = m->adj;
   if (!a->J) AERO_TRY(aalloc(a, &a->J, 96 * static_cast<size_t>(m->F)), "face Jacobian buffer");
-  if (!a->T) AERO_TRY(aalloc(a, &a->T, 24 * static_cast<size_t>(m->F)), "face product buffer");
   return AERO_OK;
 }
 
@@ -348,9 +348,9 @@ int enqueue_jvp(aero_hip_mesh* m, const aero_residual_config* cfg, const double*
   const size_t n8 = 8 * static_cast<size_t>(m->N);
   if (cfg->second_order) {
     AERO_TRY(aero_host::launch_tan_gradient(a->am, U, dU, gt, cfg->gamma, a->G2, a->dG, m->stream), "k_tan_gradient");
-  } else {
-    AERO_TRY(hipMemsetAsync(a->G2, 0, n8 * sizeof(double), m->stream), "zero gradient");
-    AERO_TRY(hipMemsetAsync(a->dG, 0, n8 * sizeof(double), m->stream), "zero gradient");
+  } else {  // fill kernels rather than memsets: this sequence may be captured in a graph
+    AERO_TRY(aero_host::launch_fill(static_cast<long long>(n8), 0.0, a->G2, m->stream), "zero gradient");
+    AERO_TRY(aero_host::launch_fill(static_cast<long long>(n8), 0.0, a->dG, m->stream), "zero gradient");
   }
   AERO_TRY(aero_host::launch_tan_face(a->am, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, U, dU, a->G2,
                                       a->dG, gt, o, a->fa, m->stream),
@@ -551,8 +551,7 @@ int aero_hip_residual_vjp(aero_hip_mesh* m, const aero_residual_config* cfg, con
   AdjointData* a = m->adj;
   rc = upload_internal(m, psi, a->psi, err, errlen);
   if (rc) return rc;
-  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, m->U[m->cur], a->psi, cfg->gamma, a->T, a->Gbar, a->out,
-                                         m->stream),
+  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out, m->stream),
            "adjoint product");
   return download_internal(m, a->out, out, err, errlen);
 }
@@ -652,7 +651,7 @@ namespace {
 // the parameter derivative (tangent), leaving the linear map G.  partials_out (or null) receives the
 // squared norm of the stage-0 linear residual.
 int one_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forward, bool forcing, const GeoTan& gt,
-                  const FaceOptsT<Dual>& od, const aero_linear_solve* ctl, int* start_io, double* norm_out, char* err,
+                  const FaceOptsT<Dual>& od, const aero_linear_solve* ctl, int* start_io, bool record, char* err,
                   size_t errlen) {
   AdjointData* a = m->adj;
   const FaceOpts o = face_opts(cfg);
@@ -676,17 +675,27 @@ int one_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forward
       const int rc = enqueue_jvp(m, cfg, xs, forcing ? gt : no_geo, od_plain, err, errlen);
       if (rc) return rc;
       r = a->dR;
-    } else {
-      AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, U, o, xs, a->psi, m->stream), "k_adj_scale");
-      AERO_TRY(aero_host::launch_adj_product(a->am, a->J, U, a->psi, cfg->gamma, a->T, a->Gbar, a->out, m->stream),
-               "adjoint product");
-      r = a->out;
+    } else {  // psi = P^T D xs is current (set by the previous stage, or by the caller before stage 0)
+      const bool want = st == 0 && record;
+      AERO_TRY(aero_host::launch_adj_stage(a->am, a->J, U, o, st, rk3, x0, xs, forcing ? a->rhs : nullptr, m->lam,
+                                           ctl->cfl, a->Gbar, a->out, xo, want ? a->partials : nullptr, a->psi,
+                                           m->stream),
+               "adjoint stage");
+      if (want)
+        AERO_TRY(aero_host::launch_partials_sum_counted(a->nparts, a->partials, a->hist, a->hist_ctr, a->hist_cap,
+                                                        m->stream),
+                 "k_partials_sum");
+      xs = xo;
+      continue;
     }
-    const bool want = st == 0 && norm_out;
+    const bool want = st == 0 && record;
     AERO_TRY(aero_host::launch_lin_update(a->am, st, rk3, forward, x0, xs, r, (forward || !forcing) ? nullptr : a->rhs,
                                           m->lam, ctl->cfl, U, o, xo, want ? a->partials : nullptr, m->stream),
              "k_lin_update");
-    if (want) AERO_TRY(aero_host::launch_partials_sum(a->nparts, a->partials, norm_out, m->stream), "k_partials_sum");
+    if (want)
+      AERO_TRY(aero_host::launch_partials_sum_counted(a->nparts, a->partials, a->hist, a->hist_ctr, a->hist_cap,
+                                                      m->stream),
+               "k_partials_sum");
     xs = xo;
   }
   if (!rk3) start = 1 - start;
@@ -705,15 +714,45 @@ int plain_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forward
   if (rc) return rc;
   const size_t n4 = 4 * static_cast<size_t>(m->N);
   AERO_TRY(hipMemsetAsync(a->x[0], 0, n4 * sizeof(double), m->stream), "linear solve init");
-  int start = 0;
+  AERO_TRY(hipMemsetAsync(a->hist_ctr, 0, sizeof(int), m->stream), "linear solve init");
+  // One graph per replay unit: an iteration under RK3 (its ring ends where it starts), two under RK1
+  // (which ping-pongs), so the launches of every iteration replay from the same graph.
+  const int unit = ctl->rk_stages == 3 ? 1 : 2;
+  hipGraphExec_t gx = nullptr;
+  {
+    hipGraph_t graph = nullptr;
+    int s0 = 0;
+    AERO_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    for (int u = 0; u < unit && !rc; ++u) rc = one_iteration(m, cfg, forward, true, gt, od, ctl, &s0, true, err, errlen);
+    const hipError_t ec = hipStreamEndCapture(m->stream, &graph);
+    if (rc) return rc;
+    AERO_TRY(ec, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    AERO_TRY(ei, "hipGraphInstantiate");
+  }
+  struct GraphGuard {
+    hipGraphExec_t g;
+    ~GraphGuard() {
+      if (g) (void)hipGraphExecDestroy(g);
+    }
+  } guard{gx};
+  if (!forward)  // the transposed stages read psi = P^T D x of their input
+    AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, m->U[m->cur], face_opts(cfg), a->x[0], a->psi,
+                                         m->stream),
+             "k_adj_scale");
+  int start = 0;  // after whole units the ring is back at buffer 0
   int done = 0;
   double res0 = -1.0, res = 0.0;
   std::vector<double> h(maxit);
   const auto t0 = std::chrono::steady_clock::now();
   while (done < maxit) {
-    const int batch = std::min(every, maxit - done);
-    for (int it = 0; it < batch; ++it) {
-      rc = one_iteration(m, cfg, forward, true, gt, od, ctl, &start, a->hist + done + it, err, errlen);
+    int batch = std::min(std::max(every, unit), maxit - done);
+    batch = std::max(unit, (batch / unit) * unit);
+    if (done + batch > maxit) batch = maxit - done;
+    for (int it = 0; it + unit <= batch; it += unit) AERO_TRY(hipGraphLaunch(gx, m->stream), "hipGraphLaunch");
+    for (int it = (batch / unit) * unit; it < batch; ++it) {  // a last odd RK1 iteration
+      rc = one_iteration(m, cfg, forward, true, gt, od, ctl, &start, true, err, errlen);
       if (rc) return rc;
     }
     AERO_TRY(hipMemcpyAsync(h.data() + done, a->hist + done, batch * sizeof(double), hipMemcpyDeviceToHost, m->stream),
@@ -774,9 +813,13 @@ int apply_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forwa
     AERO_TRY(hipMemcpyAsync(a->x[0], x, bytes, hipMemcpyDeviceToDevice, m->stream), "iterate in");
   else
     AERO_TRY(hipMemsetAsync(a->x[0], 0, bytes, m->stream), "iterate in");
+  if (!forward)
+    AERO_TRY(aero_host::launch_adj_scale(a->am, m->lam, ctl->cfl, m->U[m->cur], face_opts(cfg), a->x[0], a->psi,
+                                         m->stream),
+             "k_adj_scale");
   int start = 0;
   for (int k = 0; k < std::max(ctl->gmres_sweeps, 1); ++k) {
-    const int rc = one_iteration(m, cfg, forward, forcing, gt, od, ctl, &start, nullptr, err, errlen);
+    const int rc = one_iteration(m, cfg, forward, forcing, gt, od, ctl, &start, false, err, errlen);
     if (rc) return rc;
   }
   AERO_TRY(hipMemcpyAsync(y, a->x[start], bytes, hipMemcpyDeviceToDevice, m->stream), "iterate out");

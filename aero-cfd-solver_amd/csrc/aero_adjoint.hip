@@ -4,8 +4,9 @@
 // compute (face_eval.hpp, instantiated on the dual-number type), at a fixed state.
 //
 //   tangent:  dR = R_U dU + R_p dp           k_tan_gradient -> k_tan_face -> k_tan_gather
-//   adjoint:  out = R_U^T psi                 k_adj_face -> k_adj_cell -> k_adj_grad, over the face
-//             Jacobians J_f = d(F_f A_f)/d(U_o, U_n, G_o, G_n) that k_face_jacobian stores once per state
+//   adjoint:  out = R_U^T psi                 k_adj_gather -> k_adj_grad, over the face Jacobians
+//             J_f = d(F_f A_f)/d(U_o, U_n, G_o, G_n) that k_face_jacobian stores once per state; inside the
+//             pseudo-time iteration k_adj_grad_update also applies the stage update
 //
 // All sums run per cell over its faces in increasing reference face index (no atomics), so results are
 // deterministic.  Face-indexed arrays are SoA over the F faces in reference order; cell arrays are SoA
@@ -221,8 +222,9 @@ __global__ __launch_bounds__(kThreads) void k_tan_gather(AdjMesh am, const doubl
 }
 
 // ---- face Jacobians ----------------------------------------------------------------------------------
-// Thread (dir, f): column `dir` of d(F_f A_f)/d(U_o[0..3], U_n[0..3], G_o[0..7], G_n[0..7]), stored at
-// J[(dir * 4 + k) * F + f].  Boundary faces depend on U_o only; their other columns are zero.
+// Thread (f, dir): column `dir` of d(F_f A_f)/d(U_o[0..3], U_n[0..3], G_o[0..7], G_n[0..7]), stored face-major
+// at J[f * 96 + dir * 4 + k] (the 24 threads of a face write one contiguous 768-byte record).  Boundary faces
+// depend on U_o only; their other columns are zero.
 template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
 __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const double* __restrict__ U,
                                                             const double* __restrict__ G, FaceOpts o0,
@@ -230,8 +232,8 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
   const long long tid = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   const int F = am.F;
   if (tid >= 24LL * F) return;
-  const int dir = static_cast<int>(tid / F);
-  const int f = static_cast<int>(tid - static_cast<long long>(dir) * F);
+  const int f = static_cast<int>(tid / 24);
+  const int dir = static_cast<int>(tid - 24LL * f);
   const int ld = am.ld;
   const int co = am.fo[f], cn = am.fn[f];
   FaceOptsT<Dual> o;
@@ -304,38 +306,20 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
     col[3] = Fl.f3.d * A;
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) J[static_cast<size_t>(dir * 4 + k) * F + f] = col[k];
+  for (int k = 0; k < 4; ++k) J[static_cast<size_t>(f) * 96 + dir * 4 + k] = col[k];
 }
 
 // ---- adjoint product out = R_U^T psi ---------------------------------------------------------------
 // Face f adds F_f A_f to R_o and subtracts it from R_n, so its weight is w = psi_o - psi_n (psi_o on
-// boundary faces) and T[dir][f] = sum_k J[dir][k][f] w_k.
-__global__ __launch_bounds__(kThreads) void k_adj_face(AdjMesh am, const double* __restrict__ J,
-                                                       const double* __restrict__ psi, double* __restrict__ T) {
-  const int f = blockIdx.x * kThreads + threadIdx.x;
-  if (f >= am.F) return;
-  const int F = am.F, ld = am.ld;
-  const int o = am.fo[f], n = am.fn[f];
-  double w[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) w[k] = psi[k * ld + o] - (n >= 0 ? psi[k * ld + n] : 0.0);
-  const int ndir = n >= 0 ? 24 : 4;
-  for (int dir = 0; dir < 24; ++dir) {
-    double t = 0.0;
-    if (dir < ndir) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) t += J[static_cast<size_t>(dir * 4 + k) * F + f] * w[k];
-    }
-    T[static_cast<size_t>(dir) * F + f] = t;
-  }
-}
-
-// Per cell: the direct part of R_U^T psi and the gradient adjoint Gbar, over its faces in face order.
-__global__ __launch_bounds__(kThreads) void k_adj_cell(AdjMesh am, const double* __restrict__ T,
-                                                       double* __restrict__ out, double* __restrict__ Gbar) {
+// boundary faces).  Per cell, over its faces in face order, the cell's side of J_f^T w: the owner takes
+// columns 0..3 (U_o) and 8..15 (G_o), the neighbour 4..7 (U_n) and 16..23 (G_n).  Writes the direct part
+// of R_U^T psi and the gradient adjoint Gbar.
+__global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
+                                                         const double* __restrict__ psi, double* __restrict__ out,
+                                                         double* __restrict__ Gbar) {
   const int c = blockIdx.x * kThreads + threadIdx.x;
   if (c >= am.N) return;
-  const int F = am.F, ld = am.ld;
+  const int ld = am.ld;
   double u[4] = {0.0, 0.0, 0.0, 0.0};
   double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int e1 = am.cf_off[c + 1];
@@ -343,10 +327,27 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(AdjMesh am, const double*
     const int ent = am.cf_ent[e];
     const int f = ent >> 1;
     const int side = ent & 1;
+    const int o = am.fo[f], n = am.fn[f];
+    double w[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) u[k] += T[static_cast<size_t>(side * 4 + k) * F + f];
+    for (int k = 0; k < 4; ++k) w[k] = psi[k * ld + o] - (n >= 0 ? psi[k * ld + n] : 0.0);
+    const double* Jf = J + static_cast<size_t>(f) * 96;
+    const double* Ju = Jf + (side ? 4 : 0) * 4;   // columns 0..3 or 4..7
+    const double* Jg = Jf + (side ? 16 : 8) * 4;  // columns 8..15 or 16..23
 #pragma unroll
-    for (int j = 0; j < 8; ++j) gb[j] += T[static_cast<size_t>(8 + side * 8 + j) * F + f];
+    for (int d = 0; d < 4; ++d) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t += Ju[d * 4 + k] * w[k];
+      u[d] += t;
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t += Jg[d * 4 + k] * w[k];
+      gb[d] += t;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) out[k * ld + c] = u[k];
@@ -465,6 +466,113 @@ __device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, dou
   *b2m1_inv_v = (*b2 - 1.0) * inv * wp.v;
 }
 
+// One stage of the transposed pseudo-time iteration, per cell, after k_adj_gather: the gradient-adjoint
+// gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
+// x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
+// and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
+template <int GK>
+__global__ __launch_bounds__(kThreads) void k_adj_grad_update(AdjMesh am, const double* __restrict__ U, FaceOpts o,
+                                                              const double* __restrict__ Gbar,
+                                                              const double* __restrict__ direct, int stage, int rk3,
+                                                              const double* __restrict__ x0,
+                                                              const double* __restrict__ xs,
+                                                              const double* __restrict__ rhs,
+                                                              const double* __restrict__ lam, double cfl,
+                                                              double* __restrict__ xout, double* __restrict__ partials,
+                                                              double* __restrict__ psi) {
+  __shared__ double red[kThreads];
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  double sq = 0.0;
+  if (c < am.N) {
+    const int ld = am.ld;
+    double wb[4] = {0.0, 0.0, 0.0, 0.0};
+    const int e1 = am.cf_off[c + 1];
+    for (int e = am.cf_off[c]; e < e1; ++e) {
+      const int ent = am.cf_ent[e];
+      const int f = ent >> 1;
+      const int fo = am.fo[f], fn = am.fn[f];
+      if (GK == 1) {
+        if (fn < 0) continue;
+        const double wx = am.fwdx[f], wy = am.fwdy[f];
+        const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
+        const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
+          const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
+          const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
+          const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
+          const double sv = wx * ax + wy * ay;
+          wb[k] += (ent & 1) ? sv : -sv;
+        }
+      } else {
+        const double A = am.fA[f];
+        const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
+        const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
+        if (fn >= 0) {
+          const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
+            const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
+            wb[k] += 0.5 * (nax * tx + nay * ty);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
+        }
+      }
+    }
+    const C4 u0 = load_c4(U, ld, c);
+    double r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      C4T<Dual> sdu;
+      sdu.r = Dual(u0.r, j == 0 ? 1.0 : 0.0);
+      sdu.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
+      sdu.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
+      sdu.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
+      const P4T<Dual> w = c2p(sdu, o.g);
+      const double add = w.r.d * wb[0] + w.u.d * wb[1] + w.v.d * wb[2] + w.p.d * wb[3];
+      r[j] = direct[j * ld + c] + add;
+    }
+    double x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double res = r[k] - (rhs ? rhs[k * ld + c] : 0.0);
+      double v = xs[k * ld + c] - res;
+      if (rk3 && stage == 1) v = 0.75 * x0[k * ld + c] + 0.25 * v;
+      if (rk3 && stage == 2) v = (1.0 / 3.0) * x0[k * ld + c] + (2.0 / 3.0) * v;
+      x[k] = v;
+      xout[k * ld + c] = v;
+      sq += res * res;
+    }
+    const double dt = cfl / dmax(lam[c], 1.0e-8);
+    double y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) y[k] = dt * x[k];
+    if (o.precond) {
+      double au, av, b2;
+      precond_row(u0, o, &au, &av, &b2);
+      psi[c] = y[0];
+      psi[ld + c] = y[1] + au * y[3];
+      psi[2 * ld + c] = y[2] + av * y[3];
+      psi[3 * ld + c] = b2 * y[3];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) psi[k * ld + c] = y[k];
+    }
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && partials) partials[blockIdx.x] = red[0];
+}
+
 __global__ __launch_bounds__(kThreads) void k_adj_scale(AdjMesh am, const double* __restrict__ lam, double cfl,
                                                         const double* __restrict__ U, FaceOpts o,
                                                         const double* __restrict__ phi, double* __restrict__ psi) {
@@ -549,6 +657,32 @@ __global__ __launch_bounds__(kThreads) void k_partials_sum(int n, const double* 
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// The same sum stored at hist[*counter] (counter incremented, entries beyond cap dropped): the iteration
+// index lives on the device, so a captured iteration graph is replayable.
+__global__ __launch_bounds__(kThreads) void k_partials_sum_counted(int n, const double* __restrict__ partials,
+                                                                   double* __restrict__ hist, int* __restrict__ counter,
+                                                                   int cap) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) s += partials[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int i = *counter;
+    if (i < cap) hist[i] = red[0];
+    *counter = i + 1;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill(long long n, double v, double* __restrict__ y) {
+  const long long k = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k < n) y[k] = v;
 }
 
 // ---- Krylov vector kernels (flat vectors of n doubles) -------------------------------------------
@@ -699,9 +833,8 @@ hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool
 }
 
 hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
-                              double* T, double* Gbar, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_face, dim3(blocks_for(am.F)), dim3(kThreads), 0, s, am, J, psi, T);
-  hipLaunchKernelGGL(k_adj_cell, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, T, out, Gbar);
+                              double* Gbar, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_gather, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, J, psi, out, Gbar);
   if (am.grad == 1)
     hipLaunchKernelGGL(k_adj_grad<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
   else
@@ -712,6 +845,20 @@ hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* 
 hipError_t launch_objective_gradient(const AdjMesh& am, const double* U, double g, const double* coef, double* out,
                                      hipStream_t s) {
   hipLaunchKernelGGL(k_objective_gradient, dim3(1), dim3(64), 0, s, am, U, g, coef, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U, const FaceOpts& o, int stage, int rk3,
+                            const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
+                            double* Gbar, double* direct, double* xout, double* partials, double* psi,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_gather, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, J, psi, direct, Gbar);
+  if (am.grad == 1)
+    hipLaunchKernelGGL(k_adj_grad_update<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, o, Gbar, direct,
+                       stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
+  else
+    hipLaunchKernelGGL(k_adj_grad_update<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, o, Gbar, direct,
+                       stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
   return hipGetLastError();
 }
 
@@ -745,6 +892,17 @@ hipError_t launch_combine(long long n, double a, const double* V, int nv, const 
 
 hipError_t launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpby, dim3(blocks_for(n)), dim3(kThreads), 0, s, n, a, x, b, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_partials_sum_counted(int nparts, const double* partials, double* hist, int* counter, int cap,
+                                      hipStream_t s) {
+  hipLaunchKernelGGL(k_partials_sum_counted, dim3(1), dim3(kThreads), 0, s, nparts, partials, hist, counter, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(long long n, double v, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kThreads), 0, s, n, v, y);
   return hipGetLastError();
 }
 

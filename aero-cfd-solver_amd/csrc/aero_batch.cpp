@@ -468,4 +468,19 @@ int aero_hip_batch_history(aero_hip_batch* b, int32_t case_index, double* hist, 
   return AERO_OK;
 }
 
+int aero_hip_batch_limiter_ratios(aero_hip_batch* b, int32_t case_index, double* ratios, char* err, size_t errlen) {
+  if (!b || case_index < 0 || case_index >= b->ncases || !ratios) {
+    set_err(err, errlen, "aero_hip_batch_limiter_ratios: bad argument");
+    return AERO_EINVAL;
+  }
+  const BatchCase& cs = b->h_cases[case_index];
+  if (!cs.lim_ratio) {
+    set_err(err, errlen, "aero_hip_batch_limiter_ratios: the batch has no freeze iteration");
+    return AERO_EINVAL;
+  }
+  const size_t n = 8 * static_cast<size_t>(b->geoms[cs.geom]->F);
+  AERO_TRY(hipMemcpy(ratios, cs.lim_ratio, n * sizeof(double), hipMemcpyDeviceToHost), "ratios copy");
+  return AERO_OK;
+}
+
 }  // extern "C"

@@ -628,6 +628,19 @@ int aero_hip_get_residual(aero_hip_mesh* m, double* residual, double* spectral_r
   return AERO_OK;
 }
 
+int aero_hip_get_state(aero_hip_mesh* m, double* conserved, char* err, size_t errlen) {
+  if (!m || !conserved) {
+    set_err(err, errlen, "aero_hip_get_state: null argument");
+    return AERO_EINVAL;
+  }
+  std::vector<double> soa(4 * static_cast<size_t>(m->N));
+  AERO_HIP_TRY(hipMemcpyAsync(soa.data(), m->U[m->cur], soa.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+               "aero_hip_get_state: copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_get_state: sync");
+  from_internal4(m, soa, conserved);
+  return AERO_OK;
+}
+
 int aero_hip_get_gradients(aero_hip_mesh* m, double* gradients, char* err, size_t errlen) {
   if (!m || !gradients) {
     set_err(err, errlen, "aero_hip_get_gradients: null argument");
@@ -689,6 +702,69 @@ int aero_hip_freeze_limiter(aero_hip_mesh* m, const aero_residual_config* cfg, c
                  "aero_hip_freeze_limiter: copy");
   AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_freeze_limiter: sync");
   m->lim_valid = true;
+  return AERO_OK;
+}
+
+int aero_hip_set_limiter_ratios(aero_hip_mesh* m, const double* ratios, char* err, size_t errlen) {
+  if (!m || !ratios) {
+    set_err(err, errlen, "aero_hip_set_limiter_ratios: null argument");
+    return AERO_EINVAL;
+  }
+  const size_t n = 8 * static_cast<size_t>(m->F);
+  if (!m->lim_ratio) {
+    void* q = nullptr;
+    AERO_HIP_TRY(hipMalloc(&q, n * sizeof(double)), "aero_hip_set_limiter_ratios: hipMalloc");
+    m->allocs.push_back(q);
+    m->lim_ratio = static_cast<double*>(q);
+  }
+  AERO_HIP_TRY(hipMemcpyAsync(m->lim_ratio, ratios, n * sizeof(double), hipMemcpyHostToDevice, m->stream),
+               "aero_hip_set_limiter_ratios: copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_set_limiter_ratios: sync");
+  m->lim_valid = true;
+  return AERO_OK;
+}
+
+int aero_hip_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t n_iters, int32_t rk_stages, double cfl,
+                     double* rho_l2, char* err, size_t errlen) {
+  if (!m || !cfg || n_iters < 0 || (rk_stages != 1 && rk_stages != 3)) {
+    set_err(err, errlen, "aero_hip_iterate: bad argument (rk_stages must be 1 or 3)");
+    return AERO_EINVAL;
+  }
+  std::vector<double> sums;
+  if (rho_l2) sums.resize(static_cast<size_t>(n_iters));
+  double* d_sums = nullptr;
+  if (rho_l2 && n_iters > 0) {
+    void* q = nullptr;
+    AERO_HIP_TRY(hipMalloc(&q, n_iters * sizeof(double)), "aero_hip_iterate: hipMalloc");
+    d_sums = static_cast<double*>(q);
+  }
+  struct Free {
+    double* p;
+    ~Free() {
+      if (p) (void)hipFree(p);
+    }
+  } guard{d_sums};
+  WallArgs wa = m->wall;
+  for (int it = 0; it < n_iters; ++it) {
+    for (int st = 0; st < rk_stages; ++st) {
+      const int rc = aero_hip_stage(m, cfg, st, st == rk_stages - 1, cfl, 1, 0, err, errlen);
+      if (rc) return rc;
+    }
+    if (d_sums) {
+      AERO_HIP_TRY(aero_host::launch_iter_reduce(m->dm, m->partials, wa, m->U[m->cur], cfg->gamma, m->iter_out,
+                                                 nullptr, m->stream),
+                   "k_iter_reduce");
+      AERO_HIP_TRY(hipMemcpyAsync(d_sums + it, m->iter_out + 3, sizeof(double), hipMemcpyDeviceToDevice, m->stream),
+                   "aero_hip_iterate: norm");
+    }
+  }
+  if (d_sums) {
+    AERO_HIP_TRY(hipMemcpyAsync(sums.data(), d_sums, n_iters * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "aero_hip_iterate: history");
+  }
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_iterate: sync");
+  for (int it = 0; it < n_iters && rho_l2; ++it)
+    rho_l2[it] = std::sqrt(sums[it] / static_cast<double>(std::max(1, m->N)));
   return AERO_OK;
 }
 

@@ -191,7 +191,13 @@ hipError_t launch_tan_gather(const aero_dev::AdjMesh& am, const double* fa, doub
 hipError_t launch_face_jacobian(const aero_dev::AdjMesh& am, int scheme, int limiter, bool second, const double* U,
                                 const double* G, const aero_dev::FaceOpts& o, double* J, hipStream_t s);
 hipError_t launch_adj_product(const aero_dev::AdjMesh& am, const double* J, const double* U, const double* psi,
-                              double g, double* T, double* Gbar, double* out, hipStream_t s);
+                              double g, double* Gbar, double* out, hipStream_t s);
+// One stage of the transposed pseudo-time iteration (k_adj_gather + k_adj_grad_update): reads psi, writes
+// xout, the block partials (may be null) and the next stage's psi.
+hipError_t launch_adj_stage(const aero_dev::AdjMesh& am, const double* J, const double* U, const aero_dev::FaceOpts& o,
+                            int stage, int rk3, const double* x0, const double* xs, const double* rhs,
+                            const double* lam, double cfl, double* Gbar, double* direct, double* xout,
+                            double* partials, double* psi, hipStream_t s);
 hipError_t launch_objective_gradient(const aero_dev::AdjMesh& am, const double* U, double g, const double* coef,
                                      double* out, hipStream_t s);
 hipError_t launch_adj_scale(const aero_dev::AdjMesh& am, const double* lam, double cfl, const double* U,
@@ -201,6 +207,11 @@ hipError_t launch_lin_update(const aero_dev::AdjMesh& am, int stage, int rk3, in
                              const double* U, const aero_dev::FaceOpts& o, double* xout, double* partials,
                              hipStream_t s);
 hipError_t launch_partials_sum(int nparts, const double* partials, double* out, hipStream_t s);
+// The sum written to hist[*counter], counter incremented on the device (graph-replayable).
+hipError_t launch_partials_sum_counted(int nparts, const double* partials, double* hist, int* counter, int cap,
+                                      hipStream_t s);
+// y[0..n) = v (a kernel, so it may sit inside a captured graph).
+hipError_t launch_fill(long long n, double v, double* y, hipStream_t s);
 // Krylov vector operations on flat vectors of n doubles; the dot products use kDotBlocks (512) block
 // partials per vector, summed in a fixed order.
 hipError_t launch_multidot(long long n, const double* V, int nv, const double* w, double* partials, double* out,

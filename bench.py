@@ -167,6 +167,125 @@ def run_sweep(args) -> int:
     return 0
 
 
+def _frozen_continuation(hm, rc, U0, iters, cfl):
+    """Pseudo-time iterations from U0 with the handle's frozen limiter ratios (aero_hip_iterate); returns the
+    density-residual L2 after the first and the last iteration."""
+    hm.set_state(U0)
+    h = hm.iterate(rc, iters, cfl)
+    return float(h[0]), float(h[-1])
+
+
+def run_adjoint(args) -> int:
+    """Discrete adjoint on the ~66k-cell jittered mixed mesh deformed by 20 Hicks-Henne bumps (10 per
+    surface, peaks at x/c = 0.05..0.95, amplitudes U(+-0.002c) from seed 4242, IDW volume deformation);
+    M 0.6, AoA 2 deg, HLLC + venkat with the limiter frozen at --freeze-iter.  The primal runs as a
+    one-case batch (device control, hipGraph) to an 8-decade density drop; then one adjoint solve per
+    objective (CL, CD) gives dJ/d(20 amplitudes, AoA, Mach).  value = adjoint time / primal time.  Checks:
+    the tangent solve and central differences of converged solutions (same frozen ratios) for AoA, Mach and
+    two amplitudes."""
+    from dataclasses import replace
+    import numpy as np
+    import aero_hip
+    from aero_hip import CaseConfig, HipBatch, HipMesh, case_linearization, deform_mesh, generate_mesh
+    from aero_hip.adjoint import geometry_seed, sensitivities
+
+    if not aero_hip.hip_available():
+        raise SystemExit("bench.py: no gfx950 device visible")
+    mkw = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
+               quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    log("[adjoint] generating and deforming the mesh")
+    base = generate_mesh(precision=8, **mkw)
+    amp = np.random.default_rng(4242).uniform(-0.002, 0.002, 20)
+    mesh, tan = deform_mesh(base, amp)
+    cap = args.sweep_iters
+    case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.sweep_cfl,
+                      cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
+                      stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
+                      residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
+    hm = HipMesh(mesh, block_cells=args.block_cells)
+    log(f"[adjoint] primal: {mesh.num_cells} cells to an 8-decade density drop (cap {cap})")
+    with HipBatch([hm], [(0, case.mach, case.aoa_deg)], case) as b:
+        r = b.run(cap, 1e-8, iters_per_sync=500, use_graph=True)
+        U = b.state(0)
+        ratios = b.limiter_ratios(0)
+        hm.set_limiter_ratios(ratios)
+    row = r["cases"][0]
+    t_primal, iters = r["seconds"], row["iterations"]
+    done = replace(case, iterations=iters)
+    out = {}
+    for name, w in (("cl", (1.0, 0.0, 0.0)), ("cd", (0.0, 1.0, 0.0))):
+        log(f"[adjoint] adjoint solve for {name}")
+        s = sensitivities(hm, done, U, weights=w, shape_tangents=tan, tol=args.adjoint_tol, max_iters=4 * cap)
+        out[name] = s
+    t_adj = {k: v["adjoint"]["seconds"] for k, v in out.items()}
+    # checks on CL: tangent solves and central differences of converged solutions
+    rc, _, obj, _ = case_linearization(done)
+    checks = {}
+    for pname, h in (("aoa", 1e-3), ("mach", 1e-4), ("shape3", 1e-5), ("shape14", 1e-5)):
+        log(f"[adjoint] tangent solve {pname}")
+        if pname in ("aoa", "mach"):
+            _, seed, _, dq = case_linearization(done, param=pname)
+            ob = hm.objective(replace_obj(obj, 1.0), U)
+            j_p = ob["dJ_daoa"] if pname == "aoa" else ob["dJ_dq"] * dq
+        else:
+            i = int(pname[5:])
+            seed = geometry_seed(tan, i)
+            ob = hm.objective(replace_obj(obj, 1.0), U, seed=seed)
+            j_p = ob["dJ_seed"]
+        v, ti = hm.tangent_solve(rc, seed, U, cfl=case.cfl_max, max_iters=4 * cap, tol=args.adjoint_tol,
+                                 check_every=500)
+        d_tan = j_p + float(np.sum(ob["dJdU"] * v))
+        cls = []
+        for sgn in (1, -1):
+            log(f"[adjoint] finite difference {pname} {sgn:+d}")
+            if pname in ("aoa", "mach"):
+                cp = replace(done, **{("aoa_deg" if pname == "aoa" else "mach"): getattr(
+                    done, "aoa_deg" if pname == "aoa" else "mach") + sgn * h})
+                rcp, _, objp, _ = case_linearization(cp)
+                r0, r1 = _frozen_continuation(hm, rcp, U, args.fd_iters, case.cfl_max)
+                cls.append(hm.objective(replace_obj(objp, 1.0), None, gradient=False)["J"])
+                ok = r1 <= 1e-7 * r0
+            else:
+                e = np.zeros(20)
+                e[int(pname[5:])] = sgn * h
+                mi, _ = deform_mesh(base, amp + e, tangents=False)
+                with HipMesh(mi, block_cells=args.block_cells) as hmi:
+                    hmi.set_limiter_ratios(ratios)
+                    r0, r1 = _frozen_continuation(hmi, rc, U, args.fd_iters, case.cfl_max)
+                    cls.append(hmi.objective(replace_obj(obj, 1.0), None, gradient=False)["J"])
+                ok = r1 <= 1e-7 * r0
+            checks.setdefault(pname, {})[f"fd_converged_{sgn:+d}"] = bool(ok)
+        fd = (cls[0] - cls[1]) / (2 * h)
+        d_adj = out["cl"]["d"][pname]
+        checks[pname].update(adjoint=d_adj, tangent=d_tan, fd=fd, rel_adjoint_fd=abs(d_adj - fd) / abs(fd),
+                             rel_adjoint_tangent=abs(d_adj - d_tan) / abs(d_adj), tangent_iterations=ti["iterations"])
+    hm.close()
+    ratio = max(t_adj.values()) / t_primal
+    res = {"metric": METRIC, "value": round(ratio, 3), "unit": "adjoint/primal time", "n_gpus": 1, "steps": iters,
+           "warmup": 0, "ms_per_step": round(t_primal / max(iters, 1) * 1e3, 4), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "cells": mesh.num_cells, "mesh": mkw,
+                      "bumps": 20, "amplitude_range": 0.002, "seed": 4242, "mach": 0.6, "aoa_deg": 2.0,
+                      "cfl_max": args.sweep_cfl, "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
+                      "adjoint_tol": args.adjoint_tol},
+           "adjoint": {"primal_seconds": round(t_primal, 3), "primal_iterations": iters,
+                       "primal_rho_drop": row["rho_ratio"], "primal_converged": bool(row["converged"]),
+                       "adjoint_seconds": {k: round(v, 3) for k, v in t_adj.items()},
+                       "adjoint_iterations": {k: v["adjoint"]["iterations"] for k, v in out.items()},
+                       "adjoint_residual_drop": {k: v["adjoint"]["res"] / v["adjoint"]["res0"] for k, v in out.items()},
+                       "cl": out["cl"]["J"], "cd": out["cd"]["J"],
+                       "dcl": out["cl"]["d"], "dcd": out["cd"]["d"], "checks_cl": checks}}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def replace_obj(obj, w_cl, w_cd=0.0, w_cm=0.0):
+    """A copy of an aero_objective with other weights."""
+    o = type(obj).from_buffer_copy(obj)
+    o.w_cl, o.w_cd, o.w_cm = w_cl, w_cd, w_cm
+    return o
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +301,9 @@ def main() -> int:
     ap.add_argument("--sweep-cfl", type=float, default=2.0, help="CFL after the ramp in --sweep")
     ap.add_argument("--freeze-iter", type=int, default=1000,
                     help="--sweep / --adjoint: iteration at which the limiter is frozen (-1: never)")
+    ap.add_argument("--adjoint", action="store_true", help="run the discrete-adjoint workload instead")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="--adjoint: linear residual drop")
+    ap.add_argument("--fd-iters", type=int, default=60000, help="--adjoint: continuation iterations per FD run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -198,6 +320,8 @@ def main() -> int:
         return 0
     if args.sweep:
         return run_sweep(args)
+    if args.adjoint:
+        return run_adjoint(args)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the path has no "
               "multi-GPU execution in this build", file=sys.stderr)

@@ -166,6 +166,8 @@ int aero_hip_get_residual(aero_hip_mesh* mesh, double* residual, double* spectra
                           char* err, size_t errlen);
 int aero_hip_get_gradients(aero_hip_mesh* mesh, double* gradients /* [N][5][2] reference layout */, char* err,
                            size_t errlen);
+/* The handle's current conserved state, [N][5] in the reference numbering. */
+int aero_hip_get_state(aero_hip_mesh* mesh, double* conserved, char* err, size_t errlen);
 /* The handle's HIP stream (hipStream_t), for event timing by the caller. */
 void* aero_hip_stream(aero_hip_mesh* mesh);
 
@@ -195,6 +197,15 @@ int aero_hip_stage(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t
  * owner-side rho, u, v, p then neighbour-side rho, u, v, p, faces in the reference order. */
 int aero_hip_freeze_limiter(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
                             double* ratios, char* err, size_t errlen);
+/* Sets the handle's frozen limiter ratios ([8][F], the layout above), e.g. from a batch case. */
+int aero_hip_set_limiter_ratios(aero_hip_mesh* mesh, const double* ratios, char* err, size_t errlen);
+
+/* n pseudo-time iterations on the handle's state with a fixed residual configuration and CFL (local
+ * time stepping, no ramps, no monitor; the stage body of euler_solver.cpp:802-900), e.g. to continue a
+ * converged solution after a small parameter change.  rho_l2 (may be NULL): [n] density-residual L2
+ * (the last stage's residual, as the driver's norms) after each iteration.  No reference counterpart. */
+int aero_hip_iterate(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t n_iters, int32_t rk_stages,
+                     double cfl, double* rho_l2, char* err, size_t errlen);
 
 /* The full pseudo-time run of run_euler_airfoil_case (euler_solver.cpp:463-1208) with the stage
  * loop, norms and forces on the device and the ramp / CFL / convergence control on the host. */
@@ -238,6 +249,10 @@ int aero_hip_batch_get_state(aero_hip_batch* batch, int32_t case_index, double* 
                              size_t errlen);
 int aero_hip_batch_history(aero_hip_batch* batch, int32_t case_index, double* hist /* [cap][4] */, int32_t cap,
                            char* err, size_t errlen);
+/* The frozen limiter ratios a case captured ([8][F] of its mesh; zeros before the freeze iteration).
+ * Fails when the batch was created without a freeze iteration. */
+int aero_hip_batch_limiter_ratios(aero_hip_batch* batch, int32_t case_index, double* ratios, char* err,
+                                  size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
  * Discrete tangent and adjoint of the residual (no reference counterpart: the reference has no

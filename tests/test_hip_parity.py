@@ -217,3 +217,25 @@ def test_lsq_run_case_parity(hip):
         assert np.all((rel <= 1e-9) | (np.abs(ho[:, col] - hr[:, col]) <= 1e-12)), (col, rel.max())
     sU = np.abs(ref["conserved"]).max(axis=0) + 1e-300
     assert (np.abs(out["conserved"] - ref["conserved"]) / sU).max() <= 1e-9
+
+
+def test_iterate_matches_stage_loop(hip):
+    """aero_hip_iterate (fixed configuration, device loop) equals the same iterations through
+    aero_hip_stage, bitwise, and reports the density-residual L2 of each iteration."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    ff = ff64(0.5, 2.0)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.05, 17)
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff))
+    say("[iterate] device loop vs stage loop")
+    with hip.HipMesh(m) as a, hip.HipMesh(m) as b:
+        a.set_state(U)
+        h = a.iterate(cfg, 7, 0.9)
+        b.set_state(U)
+        for _ in range(7):
+            for st in range(3):
+                b.stage(cfg, st, st == 2, 0.9, True, st == 2)
+        assert np.array_equal(a.state(), b.state())
+        Rb, _, _ = b.get_residual()
+        rho_l2 = np.sqrt(np.sum((Rb[:, 0] / m.volume) ** 2) / m.num_cells)
+        assert len(h) == 7 and np.all(np.isfinite(h))
+        assert abs(h[-1] - rho_l2) <= 1e-12 * rho_l2
