@@ -5,10 +5,10 @@ Importing the package loads libaero_hip.so (the HIP kernels + C ABI); a missing 
 from . import _lib
 from . import adjoint
 from .mesh import Mesh, deform_mesh, generate_mesh
-from .solver import (CaseConfig, HipBatch, HipMesh, Seed, case_linearization, freestream, hip_available,
-                     residual_config)
+from .solver import (CaseConfig, HipBatch, HipMesh, Seed, adjoint_solve_batch, case_linearization, freestream,
+                     hip_available, residual_config)
 
-__all__ = ["Mesh", "deform_mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "Seed", "case_linearization", "freestream",
-           "hip_available", "residual_config", "adjoint", "_lib"]
+__all__ = ["Mesh", "deform_mesh", "generate_mesh", "CaseConfig", "HipBatch", "HipMesh", "Seed", "adjoint_solve_batch",
+           "case_linearization", "freestream", "hip_available", "residual_config", "adjoint", "_lib"]
 
 _lib.lib()

@@ -100,7 +100,8 @@ class MeshArrays(C.Structure):
 
 
 class BatchCaseC(C.Structure):
-    _fields_ = [("mesh_index", i32), ("mach", f64), ("aoa_deg", f64)]
+    _fields_ = [("mesh_index", i32), ("mach", f64), ("aoa_deg", f64), ("precond", i32),
+                ("all_speed_flux_fix", i32), ("config", C.c_void_p)]
 
 
 class BatchResult(C.Structure):
@@ -124,6 +125,15 @@ class LinearSolve(C.Structure):
     _fields_ = [("rk_stages", i32), ("cfl", f64), ("max_iters", i32), ("tol", f64), ("check_every", i32),
                 ("gmres_restart", i32), ("gmres_sweeps", i32), ("iterations", i32), ("res0", f64), ("res", f64),
                 ("seconds", f64)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32)
+
+
+class AdjointCaseC(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("cfg", C.c_void_p), ("conserved", C.c_void_p), ("rhs", C.c_void_p),
+                ("psi", C.c_void_p), ("cfl", f64), ("iterations", i32), ("converged", i32), ("failed", i32),
+                ("res0", f64), ("res", f64)]
 
 
 class ShapeConfig(C.Structure):
@@ -182,6 +192,8 @@ PROTOTYPES = {
                                          C.c_char_p, C.c_size_t]),
     "aero_hip_tangent_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, C.POINTER(TangentSeed),
                                          C.POINTER(LinearSolve), PD, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_adjoint_solve_batch": (C.c_int, [C.POINTER(AdjointCaseC), i32, C.POINTER(LinearSolve), i32,
+                                               PROGRESS_FN, vp, C.c_char_p, C.c_size_t]),
 }
 
 

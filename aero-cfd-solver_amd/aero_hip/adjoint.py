@@ -16,7 +16,7 @@ import time
 
 import numpy as np
 
-from .solver import CaseConfig, HipMesh, Seed, case_linearization
+from .solver import CaseConfig, HipMesh, Seed, adjoint_solve_batch, case_linearization
 
 
 def geometry_seed(tangents: dict, i: int) -> Seed:
@@ -57,3 +57,62 @@ def sensitivities(hm: HipMesh, case: CaseConfig, U: np.ndarray, weights=(1.0, 0.
             out["d_tangent"][name] = j_p + float(np.sum(ob["dJdU"] * v))
             out["tangent_info"][name] = ti
     return out
+
+
+def sensitivities_batch(items: list, objectives=(("cl", (1.0, 0.0, 0.0)), ("cd", (0.0, 1.0, 0.0))),
+                        cfl: float | None = None, max_iters: int = 20000, tol: float = 1e-10,
+                        iters_per_sync: int = 200, progress=None) -> dict:
+    """dJ/d(aoa [per degree], mach, shape_i) for every (case, objective) pair through one batched adjoint
+    solve (aero_hip_adjoint_solve_batch).
+
+    items: dicts with hm (HipMesh, holding the case's frozen limiter ratios when the case froze them),
+    case (CaseConfig whose `iterations` is the converged iteration count), U (the converged state) and
+    optionally shape_tangents (deform_mesh tangents).  Each case iterates at `cfl` (default: its own
+    cfl_max); all share rk_stages; progress(iterations, active) is called at every host check of the
+    solve.  Returns per-item results and the wall times of the solve and of the gradient assembly."""
+    if not items:
+        raise ValueError("sensitivities_batch: no cases")
+    stages = {it["case"].rk_stages for it in items}
+    if len(stages) != 1:
+        raise ValueError("sensitivities_batch: cases need one rk_stages")
+    solves, prep = [], []
+    for it in items:
+        hm, case, U = it["hm"], it["case"], it["U"]
+        rc, seed_a, obj, _ = case_linearization(case, param="aoa")
+        _, seed_m, _, dq_m = case_linearization(case, param="mach")
+        hm.set_state(U)
+        obs = {}
+        for name, w in objectives:
+            obj.w_cl, obj.w_cd, obj.w_cm = w
+            obs[name] = hm.objective(obj, None)
+            solves.append((hm, rc, obs[name]["dJdU"], None, case.cfl_max if cfl is None else cfl))
+        prep.append((rc, seed_a, seed_m, dq_m, obj, obs))
+    psis, infos, t_solve = adjoint_solve_batch(solves, rk_stages=stages.pop(), max_iters=max_iters,
+                                               tol=tol, iters_per_sync=iters_per_sync, progress=progress)
+    t0 = time.perf_counter()
+    out = []
+    k = 0
+    for it, (rc, seed_a, seed_m, dq_m, obj, obs) in zip(items, prep):
+        hm, U, tan = it["hm"], it["U"], it.get("shape_tangents")
+        hm.set_state(U)
+        params = [("aoa", seed_a, {n: o["dJ_daoa"] for n, o in obs.items()}),
+                  ("mach", seed_m, {n: o["dJ_dq"] * dq_m for n, o in obs.items()})]
+        if tan is not None:
+            for i in range(tan["area"].shape[0]):
+                s = geometry_seed(tan, i)
+                jp = {}
+                for name, w in objectives:
+                    obj.w_cl, obj.w_cd, obj.w_cm = w
+                    jp[name] = hm.objective(obj, None, seed=s, gradient=False)["dJ_seed"]
+                params.append((f"shape{i}", s, jp))
+        psi = {name: psis[k + j] for j, (name, _) in enumerate(objectives)}
+        info = {name: infos[k + j] for j, (name, _) in enumerate(objectives)}
+        k += len(objectives)
+        d = {name: {} for name, _ in objectives}
+        for pname, seed, jp in params:
+            _, r_p = hm.residual_jvp(rc, None, None, seed)
+            for name, _ in objectives:
+                d[name][pname] = jp[name] - float(np.sum(psi[name] * r_p))
+        out.append(dict(J={n: o["J"] for n, o in obs.items()}, d=d, adjoint=info))
+    return dict(cases=out, solve_seconds=t_solve, assembly_seconds=time.perf_counter() - t0)
+

@@ -330,6 +330,43 @@ class HipMesh:
                        history=hist[: ctl.iterations].copy())
 
 
+def adjoint_solve_batch(solves: list, cfl: float = 1.0, rk_stages: int = 3, max_iters: int = 5000,
+                        tol: float = 1e-10, iters_per_sync: int = 200, progress=None):
+    """Many adjoint solves advanced together (aero_hip_adjoint_solve_batch).  solves: (HipMesh, residual
+    config, rhs [N,5][, U [N,5] or None[, cfl]]); solves on one HipMesh share its linearisation; a solve's
+    own cfl replaces `cfl`.
+    progress(iterations, active) is called at every host check (a true return stops the solve).
+    Returns ([psi], [info]), info per solve, and the wall time of the iterations."""
+    lib = _lib.lib()
+    keep = []
+    arr = (_lib.AdjointCaseC * len(solves))()
+    psis = []
+    for i, sv in enumerate(solves):
+        hm, cfg, rhs = sv[0], sv[1], sv[2]
+        U = sv[3] if len(sv) > 3 else None
+        arr[i].cfl = float(sv[4]) if len(sv) > 4 and sv[4] is not None else 0.0
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(hm.N, 5)
+        psi = np.zeros((hm.N, 5))
+        keep += [rhs, cfg]
+        arr[i].mesh = hm.h.value
+        arr[i].cfg = C.addressof(cfg)
+        if U is not None:
+            U = np.ascontiguousarray(U, dtype=np.float64).reshape(hm.N, 5)
+            keep.append(U)
+            arr[i].conserved = U.ctypes.data
+        arr[i].rhs = rhs.ctypes.data
+        arr[i].psi = psi.ctypes.data
+        psis.append(psi)
+    ctl = _lib.LinearSolve(rk_stages, cfl, max_iters, tol, 1, 0, 1, 0, 0.0, 0.0, 0.0)
+    cb = _lib.PROGRESS_FN(lambda _u, it, act: int(bool(progress(it, act))) if progress else 0)
+    err = _lib.errbuf()
+    _lib.check(lib.aero_hip_adjoint_solve_batch(arr, len(solves), C.byref(ctl), iters_per_sync, cb, None, err,
+                                                len(err)), err, "aero_hip_adjoint_solve_batch")
+    infos = [dict(iterations=a.iterations, converged=bool(a.converged), failed=bool(a.failed), res0=a.res0, res=a.res)
+             for a in arr]
+    return psis, infos, ctl.seconds
+
+
 class Seed:
     """A parameter direction (aero_tangent_seed) that keeps its numpy arrays alive."""
 
@@ -381,12 +418,18 @@ class HipBatch:
     launch per RK stage over every (case, block), device-side iteration control, hipGraph replay."""
 
     def __init__(self, meshes: list, cases: list, base: CaseConfig, hist_capacity: int = 0):
+        """cases: (mesh_index, mach, aoa_deg[, precond, all_speed_flux_fix[, config]]): the switches are
+        0 = as the case's configuration, 1 = on, -1 = off; config (a CaseConfig, or None for `base`) gives
+        the case its own schedules and reference values (aero_batch_case.config)."""
         self.lib = _lib.lib()
         self.meshes = list(meshes)
-        self.cases = list(cases)
+        self.cases = [tuple(c) + (0, 0, None)[max(0, len(c) - 3):] for c in cases]
+        self._case_cfgs = [None if c[5] is None else c[5].to_c() for c in self.cases]
         arr = (C.c_void_p * len(self.meshes))(*[m.h.value for m in self.meshes])
-        cs = (_lib.BatchCaseC * len(self.cases))(*[_lib.BatchCaseC(int(i), float(mach), float(aoa))
-                                                   for (i, mach, aoa) in self.cases])
+        cs = (_lib.BatchCaseC * len(self.cases))(*[
+            _lib.BatchCaseC(int(i), float(mach), float(aoa), int(pc), int(af),
+                            None if cc is None else C.addressof(cc))
+            for (i, mach, aoa, pc, af, _), cc in zip(self.cases, self._case_cfgs)])
         self._cfg = base.to_c()
         self.hist_capacity = hist_capacity
         h = C.c_void_p()

@@ -16,6 +16,8 @@
 #include "aero_host.hpp"
 #include "aero_internal.hpp"
 
+using aero_dev::AdjBatchCtl;
+using aero_dev::AdjBatchSolve;
 using aero_dev::AdjMesh;
 using aero_dev::Dual;
 using aero_dev::FaceOpts;
@@ -1022,6 +1024,203 @@ int aero_hip_tangent_solve(aero_hip_mesh* m, const aero_residual_config* cfg, co
   rc = linear_solve(m, cfg, 1, gt, dual_opts(fo, seed), ctl, history, &x, err, errlen);
   if (rc) return rc;
   return download_internal(m, x, v, err, errlen);
+}
+
+int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linear_solve* ctl, int32_t iters_per_sync,
+                                 aero_progress_fn progress, void* user, char* err, size_t errlen) {
+  const char* who = "aero_hip_adjoint_solve_batch";
+  if (!cases || n <= 0) {
+    set_err(err, errlen, std::string(who) + ": no solves");
+    return AERO_EINVAL;
+  }
+  int rc = check_ctl(ctl, who, err, errlen);
+  if (rc) return rc;
+  if (ctl->gmres_restart > 0) {
+    set_err(err, errlen, std::string(who) + ": batched solves use the plain iteration (gmres_restart = 0)");
+    return AERO_EINVAL;
+  }
+  // Solves on one handle share its linearisation: the first solve on a handle sets it up.
+  std::vector<int> first(n, -1);
+  for (int i = 0; i < n; ++i) {
+    const aero_adjoint_case& c = cases[i];
+    if (!c.mesh || !c.rhs || !c.psi) {
+      set_err(err, errlen, std::string(who) + ": solve " + std::to_string(i) + " has a null mesh, rhs or psi");
+      return AERO_EINVAL;
+    }
+    rc = check_cfg(c.cfg, who, err, errlen);
+    if (rc) return rc;
+    if (!(c.cfl >= 0.0)) {
+      set_err(err, errlen, std::string(who) + ": solve " + std::to_string(i) + " has a negative CFL");
+      return AERO_EINVAL;
+    }
+    if (c.mesh->dm.grad != cases[0].mesh->dm.grad) {
+      set_err(err, errlen, std::string(who) + ": all meshes need the same gradient kind");
+      return AERO_EINVAL;
+    }
+    for (int j = 0; j < i; ++j)
+      if (cases[j].mesh == c.mesh) {
+        if (std::memcmp(cases[j].cfg, c.cfg, sizeof(aero_residual_config)) != 0 || cases[j].conserved != c.conserved) {
+          set_err(err, errlen, std::string(who) + ": solves " + std::to_string(j) + " and " + std::to_string(i) +
+                                 " share a mesh handle but not its configuration and state");
+          return AERO_EINVAL;
+        }
+        first[i] = j;
+        break;
+      }
+  }
+  for (int i = 0; i < n; ++i) {
+    if (first[i] >= 0) continue;
+    aero_hip_mesh* m = cases[i].mesh;
+    rc = adjoint_setup(m, err, errlen);
+    if (!rc) rc = load_state(m, cases[i].conserved, err, errlen);
+    if (!rc) rc = enqueue_spectral_radius(m, cases[i].cfg, err, errlen);
+    if (!rc) rc = enqueue_jacobian(m, cases[i].cfg, err, errlen);
+    if (rc) return rc;
+    AERO_TRY(hipStreamSynchronize(m->stream), "adjoint linearisation");
+  }
+  // Per-solve buffers: x ring (3 x 4N), psi (4N), Gbar (8N), direct (4N), rhs (4N), block partials.
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const size_t N = cases[i].mesh->N;
+    off[i + 1] = off[i] + 32 * N + (N + 255) / 256;
+  }
+  struct Res {
+    void* buf = nullptr;
+    void* tab = nullptr;
+    void* ctl = nullptr;
+    void* work = nullptr;
+    hipStream_t st = nullptr;
+    hipGraphExec_t gx = nullptr;
+    ~Res() {
+      if (st) (void)hipStreamSynchronize(st);
+      if (gx) (void)hipGraphExecDestroy(gx);
+      for (void* p : {buf, tab, ctl, work})
+        if (p) (void)hipFree(p);
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } r;
+  AERO_TRY(hipStreamCreateWithFlags(&r.st, hipStreamNonBlocking), who);
+  AERO_TRY(hipMalloc(&r.buf, off[n] * sizeof(double)), "adjoint batch buffers");
+  double* base = static_cast<double*>(r.buf);
+  std::vector<AdjBatchSolve> hs(n);
+  std::vector<AdjBatchCtl> hc(n);
+  const int rk3 = ctl->rk_stages == 3 ? 1 : 0;
+  for (int i = 0; i < n; ++i) {
+    aero_hip_mesh* m = cases[i].mesh;
+    const size_t N = m->N;
+    double* p = base + off[i];
+    AdjBatchSolve& s = hs[i];
+    s.am = m->adj->am;
+    s.J = m->adj->J;
+    s.U = m->U[m->cur];
+    s.lam = m->lam;
+    s.o = face_opts(cases[i].cfg);
+    s.cfl = cases[i].cfl > 0.0 ? cases[i].cfl : ctl->cfl;
+    for (int k = 0; k < 3; ++k) s.x[k] = p + 4 * N * k;
+    s.psi = p + 12 * N;
+    s.Gbar = p + 16 * N;
+    s.direct = p + 24 * N;
+    double* rhs = p + 28 * N;
+    s.rhs = rhs;
+    s.partials = p + 32 * N;
+    s.nparts = static_cast<int>((N + 255) / 256);
+    std::vector<double> soa;
+    to_internal_state(m, cases[i].rhs, &soa);
+    AERO_TRY(hipMemcpyAsync(rhs, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, r.st), "rhs upload");
+    AERO_TRY(aero_host::launch_fill(static_cast<long long>(4 * N), 0.0, s.x[0], r.st), "adjoint batch init");
+    AERO_TRY(aero_host::launch_fill(static_cast<long long>(4 * N), 0.0, s.psi, r.st), "adjoint batch init");
+    AERO_TRY(hipStreamSynchronize(r.st), "rhs upload");
+    hc[i] = AdjBatchCtl{1, 0, 0, 0, 0.0, 0.0};
+  }
+  // Work list: per handle, block-major / solve-minor (the solves of a block read the same face
+  // Jacobians), then spread so that each XCD (workgroup index mod 8) receives a contiguous run.
+  std::vector<int2> list;
+  for (int i = 0; i < n; ++i) {
+    if (first[i] >= 0) continue;
+    for (int blk = 0; blk < hs[i].nparts; ++blk)
+      for (int j = i; j < n; ++j)
+        if (cases[j].mesh == cases[i].mesh) list.push_back(make_int2(j, blk));
+  }
+  const int W = static_cast<int>(list.size());
+  const int per = (W + 7) / 8;
+  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
+  for (int j = 0; j < static_cast<int>(work.size()); ++j) {
+    const int src = (j & 7) * per + (j >> 3);
+    if (src < W) work[j] = list[src];
+  }
+  const int nwork = static_cast<int>(work.size());
+  AERO_TRY(hipMalloc(&r.tab, n * sizeof(AdjBatchSolve)), "adjoint batch table");
+  AERO_TRY(hipMalloc(&r.ctl, n * sizeof(AdjBatchCtl)), "adjoint batch control");
+  AERO_TRY(hipMalloc(&r.work, work.size() * sizeof(int2)), "adjoint batch work list");
+  auto* d_tab = static_cast<AdjBatchSolve*>(r.tab);
+  auto* d_ctl = static_cast<AdjBatchCtl*>(r.ctl);
+  auto* d_work = static_cast<int2*>(r.work);
+  AERO_TRY(hipMemcpy(d_tab, hs.data(), n * sizeof(AdjBatchSolve), hipMemcpyHostToDevice), "adjoint batch table");
+  AERO_TRY(hipMemcpy(d_ctl, hc.data(), n * sizeof(AdjBatchCtl), hipMemcpyHostToDevice), "adjoint batch control");
+  AERO_TRY(hipMemcpy(d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice), "adjoint batch work");
+  // One replay unit: an iteration under RK3 (its ring ends where it starts), two under RK1.
+  const int grad = cases[0].mesh->dm.grad;
+  const int unit = rk3 ? 1 : 2;
+  {
+    hipGraph_t graph = nullptr;
+    AERO_TRY(hipStreamBeginCapture(r.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e = hipSuccess;
+    for (int u = 0; u < unit && e == hipSuccess; ++u) {
+      if (rk3) {
+        const int ixs[3] = {0, 1, 2}, ixo[3] = {1, 2, 0};
+        for (int st = 0; st < 3 && e == hipSuccess; ++st)
+          e = aero_host::launch_adj_batch_stage(grad, d_tab, d_ctl, d_work, nwork, st, 1, 0, ixs[st], ixo[st],
+                                                st == 0 ? 1 : 0, r.st);
+      } else {
+        e = aero_host::launch_adj_batch_stage(grad, d_tab, d_ctl, d_work, nwork, 0, 0, u, u, 1 - u, 1, r.st);
+      }
+      if (e == hipSuccess) e = aero_host::launch_adj_batch_control(d_tab, d_ctl, n, ctl->tol, ctl->max_iters, r.st);
+    }
+    const hipError_t ec = hipStreamEndCapture(r.st, &graph);
+    AERO_TRY(e, "adjoint batch capture");
+    AERO_TRY(ec, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&r.gx, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    AERO_TRY(ei, "hipGraphInstantiate");
+  }
+  const int every = std::max(iters_per_sync, unit);
+  const auto t0 = std::chrono::steady_clock::now();
+  long long launched = 0;
+  const long long limit = static_cast<long long>(std::max(ctl->max_iters, 1)) + unit;
+  for (;;) {
+    for (int k = 0; k < every; k += unit) AERO_TRY(hipGraphLaunch(r.gx, r.st), "hipGraphLaunch");
+    launched += every;
+    AERO_TRY(hipMemcpyAsync(hc.data(), d_ctl, n * sizeof(AdjBatchCtl), hipMemcpyDeviceToHost, r.st), "control");
+    AERO_TRY(hipStreamSynchronize(r.st), "adjoint batch sync");
+    int active = 0;
+    for (const auto& k : hc) active += k.active ? 1 : 0;
+    if (!active || launched >= limit) break;
+    if (progress && progress(user, static_cast<int32_t>(launched), active) != 0) break;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  int max_it = 0;
+  for (int i = 0; i < n; ++i) {
+    aero_hip_mesh* m = cases[i].mesh;
+    const AdjBatchSolve& s = hs[i];
+    const AdjBatchCtl& k = hc[i];
+    const int fin = rk3 ? 0 : (k.iter & 1);
+    AERO_TRY(aero_host::launch_adj_scale(s.am, s.lam, s.cfl, s.U, s.o, s.x[fin], s.direct, r.st), "k_adj_scale");
+    std::vector<double> soa(4 * static_cast<size_t>(m->N));
+    AERO_TRY(hipMemcpyAsync(soa.data(), s.direct, soa.size() * sizeof(double), hipMemcpyDeviceToHost, r.st), "psi");
+    AERO_TRY(hipStreamSynchronize(r.st), "psi");
+    from_internal4(m, soa, cases[i].psi);
+    cases[i].iterations = k.iter;
+    cases[i].converged = k.converged;
+    cases[i].failed = k.failed;
+    cases[i].res0 = k.res0;
+    cases[i].res = k.res;
+    max_it = std::max(max_it, k.iter);
+  }
+  ctl->iterations = max_it;
+  ctl->res0 = 0.0;
+  ctl->res = 0.0;
+  ctl->seconds = std::chrono::duration<double>(t1 - t0).count();
+  return AERO_OK;
 }
 
 }  // extern "C"

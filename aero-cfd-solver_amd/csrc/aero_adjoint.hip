@@ -314,11 +314,9 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
 // boundary faces).  Per cell, over its faces in face order, the cell's side of J_f^T w: the owner takes
 // columns 0..3 (U_o) and 8..15 (G_o), the neighbour 4..7 (U_n) and 16..23 (G_n).  Writes the direct part
 // of R_U^T psi and the gradient adjoint Gbar.
-__global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
-                                                         const double* __restrict__ psi, double* __restrict__ out,
-                                                         double* __restrict__ Gbar) {
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= am.N) return;
+__device__ __forceinline__ void adj_gather_cell(const AdjMesh& am, const double* __restrict__ J,
+                                                const double* __restrict__ psi, double* __restrict__ out,
+                                                double* __restrict__ Gbar, int c) {
   const int ld = am.ld;
   double u[4] = {0.0, 0.0, 0.0, 0.0};
   double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -353,6 +351,14 @@ __global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const doubl
   for (int k = 0; k < 4; ++k) out[k * ld + c] = u[k];
 #pragma unroll
   for (int j = 0; j < 8; ++j) Gbar[j * ld + c] = gb[j];
+}
+
+__global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
+                                                         const double* __restrict__ psi, double* __restrict__ out,
+                                                         double* __restrict__ Gbar) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  adj_gather_cell(am, J, psi, out, Gbar, c);
 }
 
 // Per cell: Wbar from the gradient adjoint of the cell and its face neighbours, then
@@ -471,6 +477,108 @@ __device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, dou
 // x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
 // and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
 template <int GK>
+__device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const double* __restrict__ U,
+                                                       const FaceOpts& o, const double* __restrict__ Gbar,
+                                                       const double* __restrict__ direct, int stage, int rk3,
+                                                       const double* __restrict__ x0, const double* __restrict__ xs,
+                                                       const double* __restrict__ rhs, const double* __restrict__ lam,
+                                                       double cfl, double* __restrict__ xout,
+                                                       double* __restrict__ psi, int c) {
+  const int ld = am.ld;
+  double sq = 0.0;
+  double wb[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e1 = am.cf_off[c + 1];
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    const int fo = am.fo[f], fn = am.fn[f];
+    if (GK == 1) {
+      if (fn < 0) continue;
+      const double wx = am.fwdx[f], wy = am.fwdy[f];
+      const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
+      const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
+        const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
+        const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
+        const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
+        const double sv = wx * ax + wy * ay;
+        wb[k] += (ent & 1) ? sv : -sv;
+      }
+    } else {
+      const double A = am.fA[f];
+      const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
+      const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
+      if (fn >= 0) {
+        const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
+          const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
+          wb[k] += 0.5 * (nax * tx + nay * ty);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
+      }
+    }
+  }
+  const C4 u0 = load_c4(U, ld, c);
+  double r[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    C4T<Dual> sdu;
+    sdu.r = Dual(u0.r, j == 0 ? 1.0 : 0.0);
+    sdu.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
+    sdu.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
+    sdu.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
+    const P4T<Dual> w = c2p(sdu, o.g);
+    const double add = w.r.d * wb[0] + w.u.d * wb[1] + w.v.d * wb[2] + w.p.d * wb[3];
+    r[j] = direct[j * ld + c] + add;
+  }
+  double x[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double res = r[k] - (rhs ? rhs[k * ld + c] : 0.0);
+    double v = xs[k * ld + c] - res;
+    if (rk3 && stage == 1) v = 0.75 * x0[k * ld + c] + 0.25 * v;
+    if (rk3 && stage == 2) v = (1.0 / 3.0) * x0[k * ld + c] + (2.0 / 3.0) * v;
+    x[k] = v;
+    xout[k * ld + c] = v;
+    sq += res * res;
+  }
+  const double dt = cfl / dmax(lam[c], 1.0e-8);
+  double y[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y[k] = dt * x[k];
+  if (o.precond) {
+    double au, av, b2;
+    precond_row(u0, o, &au, &av, &b2);
+    psi[c] = y[0];
+    psi[ld + c] = y[1] + au * y[3];
+    psi[2 * ld + c] = y[2] + av * y[3];
+    psi[3 * ld + c] = b2 * y[3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) psi[k * ld + c] = y[k];
+  }
+  return sq;
+}
+
+// Fixed-order workgroup sum of one value per thread into *dst (thread 0 writes; dst may be null).
+__device__ __forceinline__ void block_sum_store(double v, double* red, double* dst) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && dst) *dst = red[0];
+}
+
+template <int GK>
 __global__ __launch_bounds__(kThreads) void k_adj_grad_update(AdjMesh am, const double* __restrict__ U, FaceOpts o,
                                                               const double* __restrict__ Gbar,
                                                               const double* __restrict__ direct, int stage, int rk3,
@@ -483,94 +591,70 @@ __global__ __launch_bounds__(kThreads) void k_adj_grad_update(AdjMesh am, const 
   __shared__ double red[kThreads];
   const int c = blockIdx.x * kThreads + threadIdx.x;
   double sq = 0.0;
-  if (c < am.N) {
-    const int ld = am.ld;
-    double wb[4] = {0.0, 0.0, 0.0, 0.0};
-    const int e1 = am.cf_off[c + 1];
-    for (int e = am.cf_off[c]; e < e1; ++e) {
-      const int ent = am.cf_ent[e];
-      const int f = ent >> 1;
-      const int fo = am.fo[f], fn = am.fn[f];
-      if (GK == 1) {
-        if (fn < 0) continue;
-        const double wx = am.fwdx[f], wy = am.fwdy[f];
-        const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
-        const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
-          const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
-          const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
-          const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
-          const double sv = wx * ax + wy * ay;
-          wb[k] += (ent & 1) ? sv : -sv;
-        }
-      } else {
-        const double A = am.fA[f];
-        const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
-        const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
-        if (fn >= 0) {
-          const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
-            const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
-            wb[k] += 0.5 * (nax * tx + nay * ty);
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
-        }
-      }
-    }
-    const C4 u0 = load_c4(U, ld, c);
-    double r[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      C4T<Dual> sdu;
-      sdu.r = Dual(u0.r, j == 0 ? 1.0 : 0.0);
-      sdu.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
-      sdu.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
-      sdu.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
-      const P4T<Dual> w = c2p(sdu, o.g);
-      const double add = w.r.d * wb[0] + w.u.d * wb[1] + w.v.d * wb[2] + w.p.d * wb[3];
-      r[j] = direct[j * ld + c] + add;
-    }
-    double x[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double res = r[k] - (rhs ? rhs[k * ld + c] : 0.0);
-      double v = xs[k * ld + c] - res;
-      if (rk3 && stage == 1) v = 0.75 * x0[k * ld + c] + 0.25 * v;
-      if (rk3 && stage == 2) v = (1.0 / 3.0) * x0[k * ld + c] + (2.0 / 3.0) * v;
-      x[k] = v;
-      xout[k * ld + c] = v;
-      sq += res * res;
-    }
-    const double dt = cfl / dmax(lam[c], 1.0e-8);
-    double y[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) y[k] = dt * x[k];
-    if (o.precond) {
-      double au, av, b2;
-      precond_row(u0, o, &au, &av, &b2);
-      psi[c] = y[0];
-      psi[ld + c] = y[1] + au * y[3];
-      psi[2 * ld + c] = y[2] + av * y[3];
-      psi[3 * ld + c] = b2 * y[3];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) psi[k * ld + c] = y[k];
-    }
+  if (c < am.N)
+    sq = adj_grad_update_cell<GK>(am, U, o, Gbar, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, c);
+  block_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
+}
+
+// ---- batched adjoint solves: one workgroup per (solve, block of kThreads cells) ---------------------
+// The same per-cell arithmetic as the single-solve kernels, so every solve of a batch reproduces its
+// single solve bit for bit; work items of solves that have stopped return at once.
+__global__ __launch_bounds__(kThreads) void k_adj_gather_batch(const AdjBatchSolve* __restrict__ solves,
+                                                               const AdjBatchCtl* __restrict__ ctl,
+                                                               const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  if (w.x < 0 || !ctl[w.x].active) return;
+  const AdjBatchSolve& s = solves[w.x];
+  const int c = w.y * kThreads + threadIdx.x;
+  if (c >= s.am.N) return;
+  adj_gather_cell(s.am, s.J, s.psi, s.direct, s.Gbar, c);
+}
+
+template <int GK>
+__global__ __launch_bounds__(kThreads) void k_adj_grad_update_batch(const AdjBatchSolve* __restrict__ solves,
+                                                                    const AdjBatchCtl* __restrict__ ctl,
+                                                                    const int2* __restrict__ work, int stage, int rk3,
+                                                                    int ix0, int ixs, int ixo, int want_partials) {
+  __shared__ double red[kThreads];
+  const int2 w = work[blockIdx.x];
+  if (w.x < 0 || !ctl[w.x].active) return;  // uniform over the workgroup
+  const AdjBatchSolve& s = solves[w.x];
+  const int c = w.y * kThreads + threadIdx.x;
+  double sq = 0.0;
+  if (c < s.am.N)
+    sq = adj_grad_update_cell<GK>(s.am, s.U, s.o, s.Gbar, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam,
+                                  s.cfl, s.x[ixo], s.psi, c);
+  block_sum_store(sq, red, want_partials ? s.partials + w.y : nullptr);
+}
+
+// Per solve (one workgroup each): the squared norm of the iteration's stage-0 linear residual from the
+// block partials (the fixed order of k_partials_sum), the iteration count, and the stopping tests:
+// residual <= tol * first residual, a non-finite residual, or max_iters.
+__global__ __launch_bounds__(kThreads) void k_adj_batch_control(const AdjBatchSolve* __restrict__ solves,
+                                                                AdjBatchCtl* __restrict__ ctl, double tol,
+                                                                int max_iters) {
+  __shared__ double red[kThreads];
+  AdjBatchCtl& k = ctl[blockIdx.x];
+  if (!k.active) return;
+  const AdjBatchSolve& s = solves[blockIdx.x];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < s.nparts; i += kThreads) v += s.partials[i];
+  double sum = 0.0;
+  block_sum_store(v, red, &sum);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(sum);
+  k.iter += 1;
+  if (k.iter == 1) k.res0 = res;
+  k.res = res;
+  if (!(res == res) || res > 1.7976931348623157e308) {
+    k.failed = 1;
+    k.active = 0;
+  } else if (res <= tol * k.res0) {
+    k.converged = 1;
+    k.active = 0;
+  } else if (k.iter >= max_iters) {
+    k.active = 0;
   }
-  red[threadIdx.x] = sq;
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && partials) partials[blockIdx.x] = red[0];
 }
 
 __global__ __launch_bounds__(kThreads) void k_adj_scale(AdjMesh am, const double* __restrict__ lam, double cfl,
@@ -859,6 +943,25 @@ hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U,
   else
     hipLaunchKernelGGL(k_adj_grad_update<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, o, Gbar, direct,
                        stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int2* work,
+                                  int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_gather_batch, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work);
+  if (grad == 1)
+    hipLaunchKernelGGL(k_adj_grad_update_batch<1>, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work, stage, rk3,
+                       ix0, ixs, ixo, want_partials);
+  else
+    hipLaunchKernelGGL(k_adj_grad_update_batch<0>, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work, stage, rk3,
+                       ix0, ixs, ixo, want_partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_batch_control(const AdjBatchSolve* solves, AdjBatchCtl* ctl, int nsolves, double tol,
+                                    int max_iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_batch_control, dim3(nsolves), dim3(kThreads), 0, s, solves, ctl, tol, max_iters);
   return hipGetLastError();
 }
 

@@ -36,10 +36,9 @@ struct aero_hip_batch {
   double* d_hist = nullptr;
   int hist_cap = 0;
   aero_case_config cfg{};
-  FaceOpts base{};
   hipGraphExec_t graph[2][2][2] = {};  // [second order][start buffer][frozen limiter]
   bool frozen = false;                  // limiter ratios captured (limiter_freeze_iter reached)
-  std::vector<double> alpha, q_inf;
+  std::vector<double> alpha, q_inf, chord;
   int state_buf = 0;  // ring buffer that holds every case's current state
 };
 
@@ -109,7 +108,7 @@ int enqueue_iteration(aero_hip_batch* b, bool second, int start, bool frozen, ch
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
     AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->d_geoms, b->d_cases,
-                                          b->d_ctl, b->d_work, b->nwork, b->smax, b->base, bs, b->stream),
+                                          b->d_ctl, b->d_work, b->nwork, b->smax, bs, b->stream),
              "k_flux_batch");
     cur = bs.out;
     final_buf = bs.out;
@@ -149,11 +148,27 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
       set_err(err, errlen, "aero_hip_batch_create: all meshes need the same block_cells and gradient");
       return AERO_EINVAL;
     }
-  for (int i = 0; i < n_cases; ++i)
+  for (int i = 0; i < n_cases; ++i) {
     if (cases[i].mesh_index < 0 || cases[i].mesh_index >= n_meshes) {
       set_err(err, errlen, "aero_hip_batch_create: case " + std::to_string(i) + " has a bad mesh_index");
       return AERO_EINVAL;
     }
+    if (cases[i].precond < -1 || cases[i].precond > 1 || cases[i].all_speed_flux_fix < -1 ||
+        cases[i].all_speed_flux_fix > 1) {
+      set_err(err, errlen, "aero_hip_batch_create: case " + std::to_string(i) + " has a switch outside -1..1");
+      return AERO_EINVAL;
+    }
+    const aero_case_config* cc = cases[i].config;
+    if (cc && (cc->flux_scheme != c->flux_scheme || cc->limiter != c->limiter || cc->rk_stages != c->rk_stages ||
+               cc->startup_first_order_iters != c->startup_first_order_iters ||
+               cc->limiter_freeze_iter != c->limiter_freeze_iter || !cc->local_time_stepping ||
+               cc->gamma != c->gamma || cc->all_speed_staged_controller)) {
+      set_err(err, errlen, "aero_hip_batch_create: case " + std::to_string(i) +
+                             "'s configuration differs from the base in flux scheme, limiter, rk_stages, "
+                             "startup iterations, freeze iteration, time stepping, gamma or the staged controller");
+      return AERO_EINVAL;
+    }
+  }
   aero_hip_batch* b = new aero_hip_batch();
   b->ncases = n_cases;
   b->ngeom = n_meshes;
@@ -178,21 +193,14 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
 
   AERO_BSETUP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   const double g = c->gamma;
-  const bool precond = c->precond_on && c->precond_experimental_enabled;
-  b->base.g = g;
-  b->base.low_mach_fix = c->low_mach_fix;
-  b->base.low_mach_cutoff = c->mach_cutoff;
-  b->base.char_bc = c->precond_farfield_bc;
-  b->base.precond = precond ? 1 : 0;
-  b->base.as_fix = c->all_speed_flux_fix;
-  b->base.as_cut = c->all_speed_mach_cutoff;
-  b->base.as_fmin = clampd(c->all_speed_f_min, 1.0e-6, 1.0);
   std::vector<CaseCtl> ctl(n_cases);
   b->h_cases.resize(n_cases);
   b->alpha.resize(n_cases);
   b->q_inf.resize(n_cases);
+  b->chord.resize(n_cases);
   for (int i = 0; i < n_cases; ++i) {
     aero_hip_mesh* m = meshes[cases[i].mesh_index];
+    const aero_case_config* cc = cases[i].config ? cases[i].config : c;  // this case's configuration
     BatchCase& cs = b->h_cases[i];
     cs.geom = cases[i].mesh_index;
     cs.nblk = m->nblk;
@@ -211,42 +219,61 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     AERO_BSETUP(balloc(b, &cs.diag, 4));
     // freestream of this case (euler_solver.cpp:543-561) and its schedules
     const double mach = cases[i].mach;
-    double rho = c->rho_inf;
-    if (rho <= 0.0) rho = c->p_inf / (c->gas_constant * c->t_inf);
+    double rho = cc->rho_inf;
+    if (rho <= 0.0) rho = cc->p_inf / (cc->gas_constant * cc->t_inf);
     const double pi = 3.14159265358979323846;
     const double alpha = cases[i].aoa_deg * (pi / 180.0);
-    const double a = std::sqrt(g * c->p_inf / rho);
+    const double a = std::sqrt(g * cc->p_inf / rho);
     const double v = std::max(mach, 0.0) * a;
     const double u = v * std::cos(alpha), w = v * std::sin(alpha);
     const double r = std::max(rho, 1.0e-8);
     const double ke = 0.5 * r * (u * u + w * w + 0.0 * 0.0);
-    cs.ff[0] = r;
-    cs.ff[1] = r * u;
-    cs.ff[2] = r * w;
-    cs.ff[3] = std::max(c->p_inf, 1.0e-8) / (g - 1.0) + ke;
+    FaceOpts& fo = cs.fo;
+    fo = FaceOpts{};
+    fo.ff.r = r;
+    fo.ff.ru = r * u;
+    fo.ff.rv = r * w;
+    fo.ff.E = std::max(cc->p_inf, 1.0e-8) / (g - 1.0) + ke;
+    fo.g = g;
+    fo.low_mach_fix = cc->low_mach_fix;
+    fo.low_mach_cutoff = cc->mach_cutoff;
+    fo.char_bc = cc->precond_farfield_bc;
+    const int precond = (cc->precond_on && cc->precond_experimental_enabled) ? 1 : 0;
+    fo.precond = cases[i].precond == 0 ? precond : (cases[i].precond > 0 ? 1 : 0);
+    fo.as_fix = cases[i].all_speed_flux_fix == 0 ? (cc->all_speed_flux_fix ? 1 : 0)
+                                                 : (cases[i].all_speed_flux_fix > 0 ? 1 : 0);
+    fo.as_cut = cc->all_speed_mach_cutoff;
+    fo.as_fmin = clampd(cc->all_speed_f_min, 1.0e-6, 1.0);
     const double minf = std::max(mach, 1.0e-4);
-    const double k0 = c->stab_floor_k_start >= 0.0 ? c->stab_floor_k_start : (c->stab_floor_start / minf);
-    const double k1 = c->stab_floor_k_target >= 0.0 ? c->stab_floor_k_target : (c->stab_floor_target / minf);
+    const double k0 = cc->stab_floor_k_start >= 0.0 ? cc->stab_floor_k_start : (cc->stab_floor_start / minf);
+    const double k1 = cc->stab_floor_k_target >= 0.0 ? cc->stab_floor_k_target : (cc->stab_floor_target / minf);
     cs.floor_start = clampd(std::max(k0, 0.0) * minf, 0.0, 1.0);
     cs.floor_target = clampd(std::max(k1, 0.0) * minf, 0.0, 1.0);
     const double auto_bmin = std::max(1.0e-4, std::min(1.0e-3, 0.1 * minf));
-    cs.mach_ref = std::max(c->precond_mach_ref > 0.0 ? c->precond_mach_ref : minf, 1.0e-4);
-    cs.mach_min = std::max(c->precond_mach_min > 0.0 ? c->precond_mach_min : 1.0e-6, 1.0e-6);
-    cs.beta_min = std::max(c->precond_beta_min > 0.0 ? c->precond_beta_min : auto_bmin, 1.0e-4);
-    cs.beta_max = std::max(c->precond_beta_max, cs.beta_min);
+    fo.mach_ref = std::max(cc->precond_mach_ref > 0.0 ? cc->precond_mach_ref : minf, 1.0e-4);
+    fo.mach_min = std::max(cc->precond_mach_min > 0.0 ? cc->precond_mach_min : 1.0e-6, 1.0e-6);
+    fo.beta_min = std::max(cc->precond_beta_min > 0.0 ? cc->precond_beta_min : auto_bmin, 1.0e-4);
+    fo.beta_max = std::max(cc->precond_beta_max, fo.beta_min);
+    cs.cfl_start = cc->cfl_start;
+    cs.cfl_max = cc->cfl_max;
+    cs.cfl_ramp = cc->cfl_ramp_iters;
+    cs.floor_ramp = std::max(cc->stab_ramp_iters, 0);
+    cs.as_on = std::max(cc->all_speed_ramp_start_iter, 0);
+    cs.as_len = std::max(cc->all_speed_ramp_iters, 0);
     cs.wall = m->wall;
-    cs.wall.p_inf = c->p_inf;
+    cs.wall.p_inf = cc->p_inf;
     cs.wall.q_inf = 0.5 * rho * v * v + 1.0e-12;
-    cs.wall.x_ref = c->x_ref;
-    cs.wall.y_ref = c->y_ref;
+    cs.wall.x_ref = cc->x_ref;
+    cs.wall.y_ref = cc->y_ref;
     b->alpha[i] = alpha;
     b->q_inf[i] = cs.wall.q_inf;
+    b->chord[i] = cc->chord;
     std::vector<double> u0(4 * N);
     for (size_t j = 0; j < N; ++j) {
-      u0[j] = cs.ff[0];
-      u0[N + j] = cs.ff[1];
-      u0[2 * N + j] = cs.ff[2];
-      u0[3 * N + j] = cs.ff[3];
+      u0[j] = fo.ff.r;
+      u0[N + j] = fo.ff.ru;
+      u0[2 * N + j] = fo.ff.rv;
+      u0[3 * N + j] = fo.ff.E;
     }
     AERO_BSETUP(hipMemcpy(cs.U[0], u0.data(), u0.size() * sizeof(double), hipMemcpyHostToDevice));
     AERO_BSETUP(hipMemset(cs.G, 0, 8 * N * sizeof(double)));
@@ -259,15 +286,15 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     k.l2_0 = -1.0;
     k.rho_l2_0 = -1.0;
     double base;
-    if (c->cfl_ramp_iters <= 0) {
-      base = c->cfl_max;
+    if (cc->cfl_ramp_iters <= 0) {
+      base = cc->cfl_max;
     } else {
-      const double t = std::min(1.0, 1.0 / static_cast<double>(std::max(c->cfl_ramp_iters, 1)));
-      base = c->cfl_start + t * (c->cfl_max - c->cfl_start);
+      const double t = std::min(1.0, 1.0 / static_cast<double>(std::max(cc->cfl_ramp_iters, 1)));
+      base = cc->cfl_start + t * (cc->cfl_max - cc->cfl_start);
     }
     k.cfl = base;
-    k.floor_cur = std::max(c->stab_ramp_iters, 0) <= 0 ? cs.floor_target : cs.floor_start;
-    const int as_on = std::max(c->all_speed_ramp_start_iter, 0), as_len = std::max(c->all_speed_ramp_iters, 0);
+    k.floor_cur = cs.floor_ramp <= 0 ? cs.floor_target : cs.floor_start;
+    const int as_on = cs.as_on, as_len = cs.as_len;
     k.as_w = as_len <= 0 ? (0 >= as_on ? 1.0 : 0.0) : clampd(static_cast<double>(0 - as_on) / std::max(as_len, 1), 0.0, 1.0);
   }
   // Work list: per geometry, block-major / case-minor (the cases of a block share its face data in L2),
@@ -320,12 +347,6 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   p.max_iters = max_iters;
   p.fixed_iters = fixed_iters ? 1 : 0;
   p.rho_drop = rho_drop;
-  p.cfl_start = c.cfl_start;
-  p.cfl_max = c.cfl_max;
-  p.cfl_ramp = c.cfl_ramp_iters;
-  p.floor_ramp = std::max(c.stab_ramp_iters, 0);
-  p.as_on = std::max(c.all_speed_ramp_start_iter, 0);
-  p.as_len = std::max(c.all_speed_ramp_iters, 0);
   p.hist = b->d_hist;
   p.hist_cap = b->hist_cap;
   const int startup = std::max(c.startup_first_order_iters, 0);
@@ -422,7 +443,7 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
     // forces.cpp:263-293 from the gauge sums of the last iteration
     const double fx = -k.sums[6], fy = -k.sums[7], mo = k.sums[8];
     const double ca = std::cos(b->alpha[i]), sa = std::sin(b->alpha[i]);
-    const double chord = std::max(c.chord, 1.0e-8), q = b->q_inf[i];
+    const double chord = std::max(b->chord[i], 1.0e-8), q = b->q_inf[i];
     const double drag = fx * ca + fy * sa, lift = -fx * sa + fy * ca;
     r.cd = drag / (q * chord);
     r.cl = lift / (q * chord);

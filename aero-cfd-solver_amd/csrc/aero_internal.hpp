@@ -74,9 +74,11 @@ struct BatchCase {
   double* lam;
   double* partials;  // [nblk][4]
   int* diag;         // [4]
-  double ff[4];      // freestream (rho, rhou, rhov, rhoE)
+  FaceOpts fo;       // this case's face options (freestream, preconditioner, low-Mach and all-speed
+                     // settings: euler_solver.cpp:341-355, :543-561); floor, as_w and frozen set per iteration
   double floor_start, floor_target;         // stabilisation-floor schedule (euler_solver.cpp:141-157)
-  double mach_ref, mach_min, beta_min, beta_max;  // build_precondition_config (euler_solver.cpp:341-355)
+  double cfl_start, cfl_max;                // CFL ramp (euler_solver.cpp:69-79)
+  int cfl_ramp, floor_ramp, as_on, as_len;  // ramp lengths and the all-speed ramp start
   WallArgs wall;     // wall faces of the geometry + this case's reference values
   double* lim_ratio; // frozen limiter ratios [8][F] (beyond the reference) or null
 };
@@ -99,10 +101,6 @@ struct BatchParams {
   int max_iters;
   int fixed_iters;   // 1: ignore the convergence test (run exactly max_iters)
   double rho_drop;   // converged when rho_l2 <= rho_drop * rho_l2(iteration 0)
-  double cfl_start, cfl_max;
-  int cfl_ramp;
-  int floor_ramp;
-  int as_on, as_len;
   double* hist;      // [ncases][hist_cap][4]: l2, rho_l2, cfl, cl-sum proxy (may be null)
   int hist_cap;
 };
@@ -148,6 +146,29 @@ struct GeoTan {
   const double* dvol;
 };
 
+// One solve of a batched adjoint solve: the linearisation (face Jacobians, state, spectral radius,
+// options) of its mesh handle and its own right-hand side and iteration buffers ([4][N] / [8][N], ld N).
+struct AdjBatchSolve {
+  AdjMesh am;
+  const double* J;
+  const double* U;
+  const double* lam;
+  FaceOpts o;
+  double cfl;
+  const double* rhs;
+  double* x[3];      // iterate ring
+  double* psi;       // P^T D x of the current stage input
+  double* Gbar;
+  double* direct;
+  double* partials;  // [nparts] block partials of the stage-0 squared residual
+  int nparts;
+};
+
+struct AdjBatchCtl {
+  int active, converged, failed, iter;
+  double res0, res;
+};
+
 }  // namespace aero_dev
 
 namespace aero_host {
@@ -169,8 +190,7 @@ hipError_t launch_batch_gradient(int BC, int grad, const aero_dev::DevMesh* geom
                                  hipStream_t s);
 hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const aero_dev::DevMesh* geoms,
                              const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl, const int2* work,
-                             int nwork, int smax, const aero_dev::FaceOpts& base, const aero_dev::BatchStage& bs,
-                             hipStream_t s);
+                             int nwork, int smax, const aero_dev::BatchStage& bs, hipStream_t s);
 // Frozen limiter capture of every active case at its state U[cur] (gradients in G).
 hipError_t launch_batch_limiter_capture(int BC, int limiter, const aero_dev::DevMesh* geoms,
                                         const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl,
@@ -200,6 +220,13 @@ hipError_t launch_adj_stage(const aero_dev::AdjMesh& am, const double* J, const 
                             double* partials, double* psi, hipStream_t s);
 hipError_t launch_objective_gradient(const aero_dev::AdjMesh& am, const double* U, double g, const double* coef,
                                      double* out, hipStream_t s);
+// Batched transposed stage over (solve, block) work items; the iterate ring indices select x[] of each
+// solve.  Then the per-solve control (residual norm, iteration count, stopping tests).
+hipError_t launch_adj_batch_stage(int grad, const aero_dev::AdjBatchSolve* solves, const aero_dev::AdjBatchCtl* ctl,
+                                  const int2* work, int nwork, int stage, int rk3, int ix0, int ixs, int ixo,
+                                  int want_partials, hipStream_t s);
+hipError_t launch_adj_batch_control(const aero_dev::AdjBatchSolve* solves, aero_dev::AdjBatchCtl* ctl, int nsolves,
+                                    double tol, int max_iters, hipStream_t s);
 hipError_t launch_adj_scale(const aero_dev::AdjMesh& am, const double* lam, double cfl, const double* U,
                             const aero_dev::FaceOpts& o, const double* phi, double* psi, hipStream_t s);
 hipError_t launch_lin_update(const aero_dev::AdjMesh& am, int stage, int rk3, int forward, const double* x0,

@@ -696,20 +696,16 @@ template <int BC, int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
 __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux_batch(const DevMesh* __restrict__ geoms,
                                                     const BatchCase* __restrict__ cases,
                                                     const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
-                                                    FaceOpts o, BatchStage bs) {
+                                                    BatchStage bs) {
   extern __shared__ double lds[];
   const int2 w = work[blockIdx.x];
   if (w.x < 0) return;
   const CaseCtl& k = ctl[w.x];
   if (!k.active) return;
   const BatchCase& cs = cases[w.x];
-  o.ff = C4{cs.ff[0], cs.ff[1], cs.ff[2], cs.ff[3]};
+  FaceOpts o = cs.fo;
   o.stab_floor = k.floor_cur;
   o.as_w = k.as_w;
-  o.mach_ref = cs.mach_ref;
-  o.mach_min = cs.mach_min;
-  o.beta_min = cs.beta_min;
-  o.beta_max = cs.beta_max;
   o.frozen = FROZEN ? cs.lim_ratio : nullptr;
   StageArgs st;
   st.stage = bs.stage;
@@ -802,23 +798,23 @@ __global__ void k_batch_control(const BatchCase* __restrict__ cases, CaseCtl* __
   if (k.converged || k.iter >= p.max_iters) k.active = 0;
   const int it = k.iter;
   double base;
-  if (p.cfl_ramp <= 0) {
-    base = p.cfl_max;
+  if (cs.cfl_ramp <= 0) {
+    base = cs.cfl_max;
   } else {
-    const double t = dmin(1.0, static_cast<double>(it + 1) / static_cast<double>(p.cfl_ramp > 1 ? p.cfl_ramp : 1));
-    base = p.cfl_start + t * (p.cfl_max - p.cfl_start);
+    const double t = dmin(1.0, static_cast<double>(it + 1) / static_cast<double>(cs.cfl_ramp > 1 ? cs.cfl_ramp : 1));
+    base = cs.cfl_start + t * (cs.cfl_max - cs.cfl_start);
   }
   k.cfl = base * k.cfl_scale;
-  if (p.floor_ramp <= 0) {
+  if (cs.floor_ramp <= 0) {
     k.floor_cur = cs.floor_target;
   } else {
-    const double x = dclamp(static_cast<double>(it) / static_cast<double>(p.floor_ramp > 1 ? p.floor_ramp : 1), 0.0, 1.0);
+    const double x = dclamp(static_cast<double>(it) / static_cast<double>(cs.floor_ramp > 1 ? cs.floor_ramp : 1), 0.0, 1.0);
     k.floor_cur = cs.floor_start + x * (cs.floor_target - cs.floor_start);
   }
-  if (p.as_len <= 0)
-    k.as_w = it >= p.as_on ? 1.0 : 0.0;
+  if (cs.as_len <= 0)
+    k.as_w = it >= cs.as_on ? 1.0 : 0.0;
   else
-    k.as_w = dclamp(static_cast<double>(it - p.as_on) / static_cast<double>(p.as_len > 1 ? p.as_len : 1), 0.0, 1.0);
+    k.as_w = dclamp(static_cast<double>(it - cs.as_on) / static_cast<double>(cs.as_len > 1 ? cs.as_len : 1), 0.0, 1.0);
 }
 
 }  // namespace aero_dev
@@ -969,19 +965,19 @@ hipError_t launch_batch_gradient(int BC, int grad, const DevMesh* geoms, const B
 template <int BC>
 static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
                                        const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork,
-                                       size_t lds, const FaceOpts& o, const BatchStage& bs, hipStream_t s) {
+                                       size_t lds, const BatchStage& bs, hipStream_t s) {
   if (second && frozen) {  // the frozen ratios replace the limiter
     if (scheme == 0)
-      hipLaunchKernelGGL((k_flux_batch<BC, 0, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o,
+      hipLaunchKernelGGL((k_flux_batch<BC, 0, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work,
                          bs);
     else
-      hipLaunchKernelGGL((k_flux_batch<BC, 1, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o,
+      hipLaunchKernelGGL((k_flux_batch<BC, 1, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work,
                          bs);
     return hipGetLastError();
   }
 #define AERO_BFLUX(SC, LI, SE)                                                                              \
   if (scheme == SC && (limiter == LI || !SE) && second == SE) {                                             \
-    hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, o, bs); \
+    hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, bs);    \
     return hipGetLastError();                                                                               \
   }
   AERO_BFLUX(0, 0, false)
@@ -996,15 +992,15 @@ static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, boo
 
 hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
                              const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork, int smax,
-                             const FaceOpts& base, const BatchStage& bs, hipStream_t s) {
+                             const BatchStage& bs, hipStream_t s) {
   const size_t lds = std::max<size_t>(static_cast<size_t>(smax) * 5, 4 * (BC / 64)) * sizeof(double);
   switch (BC) {
     case 128:
-      return launch_batch_flux_bc<128>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+      return launch_batch_flux_bc<128>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
     case 256:
-      return launch_batch_flux_bc<256>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+      return launch_batch_flux_bc<256>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
     case 512:
-      return launch_batch_flux_bc<512>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, base, bs, s);
+      return launch_batch_flux_bc<512>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -84,6 +84,50 @@ def cpu_baseline(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
                       f"{t:.1f} s on the host CPU"}
 
 
+def time_residual(hm, cfg, steps: int, warmup: int, stream):
+    """(total ms from HIP events on the handle's stream, wall seconds, per-kernel timing) of `steps`
+    back-to-back residual evaluations after `warmup` untimed ones."""
+    import torch
+    for _ in range(warmup):
+        hm.residual_async(cfg)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {steps} steps")
+    hm.timing_begin(steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if torch.distributed.is_available() and torch.distributed.is_initialized():
+        torch.distributed.barrier()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        hm.residual_async(cfg)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    tm = hm.timing_end()
+    return e0.elapsed_time(e1), wall, tm
+
+
+def variant_configs(flux: str, lim: str, mach: float, aoa: float):
+    """(name, mach, farfield -> residual config) of the numerics reported beside the headline workload."""
+    from aero_hip import residual_config
+    out = []
+    if (flux, lim) != ("rusanov", "minmod"):
+        out.append(("rusanov_minmod", mach, lambda ff: residual_config(ff, flux="rusanov", limiter="minmod",
+                                                                       second_order=True, stab_mach_floor=0.02)))
+    if (flux, lim) != ("hllc", "venkat"):
+        out.append(("hllc_venkat", mach, lambda ff: residual_config(ff, flux="hllc", limiter="venkat",
+                                                                    second_order=True, stab_mach_floor=0.02)))
+    # low-Mach preconditioned all-speed run (the reference's low-Mach test settings,
+    # tests/cpp/test_euler_low_mach_asymptotic.cpp:135-191, at M 0.01)
+    out.append(("lowmach_m0.01_precond_allspeed", 0.01,
+                lambda ff: residual_config(ff, flux=flux, limiter=lim, second_order=True, stab_mach_floor=0.0,
+                                           precond=True, mach_ref=0.01, mach_min=1e-3, beta_min=1e-4, beta_max=1.0,
+                                           all_speed_fix=True, all_speed_cutoff=0.25, all_speed_f_min=0.05,
+                                           all_speed_ramp=1.0)))
+    return out
+
+
 def traffic_for(workload: str):
     p = ROOT / "profiles" / "traffic.json"
     if p.exists():
@@ -163,6 +207,176 @@ def run_sweep(args) -> int:
                      "failed_nonfinite": sum(r["failed"] for r in rows),
                      "median_rho_drop": float(np.median([r["rho_ratio"] for r in rows])),
                      "cl": [round(r["cl"], 6) for r in rows], "cd": [round(r["cd"], 6) for r in rows]}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+UNEVEN_MESHES = {  # nominal size: (circumferential, radial) of a jittered mixed O-grid, 30 quad wall layers
+    "16k": (128, 78), "65k": (256, 144), "262k": (512, 271)}
+
+
+def run_uneven(args) -> int:
+    """Batched primal + adjoint over uneven meshes: --uneven-cases cases on each of the 16k, 65k and 262k
+    jittered mixed meshes, every case its own seeded Hicks-Henne geometry (20 bumps, amplitudes
+    U(+-0.005c), IDW deformation), M drawn from {0.05 (low-Mach preconditioner + all-speed HLLC fix),
+    0.4, 0.7}, AoA U(0, 6) deg; HLLC + venkat, limiter frozen at --freeze-iter.  All cases run as one
+    batch to an 8-decade density drop; then one batched adjoint solve for CL and CD of every case gives
+    dJ/d(20 amplitudes, AoA, Mach).  value = solves/hr of the primal batch; the adjoint/primal time ratio is
+    reported beside it, with a tangent-solve check of one case."""
+    from dataclasses import replace
+    import numpy as np
+    import aero_hip
+    from aero_hip import CaseConfig, HipBatch, HipMesh, case_linearization, deform_mesh, generate_mesh
+    from aero_hip.adjoint import sensitivities_batch
+
+    if not aero_hip.hip_available():
+        raise SystemExit("bench.py: no gfx950 device visible")
+    sizes = [x for x in args.uneven_meshes.split(",") if x]
+    rng = np.random.default_rng(args.uneven_seed)
+    base_cfg = CaseConfig(flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.sweep_cfl, cfl_ramp_iters=120,
+                          startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
+                          stab_ramp_iters=400, precond_on=0, precond_experimental_enabled=0, precond_mach_ref=0.0,
+                          precond_mach_min=1e-3, precond_beta_min=1e-4, precond_beta_max=1.0,
+                          all_speed_mach_cutoff=0.25, all_speed_f_min=0.05, all_speed_ramp_start_iter=100,
+                          all_speed_ramp_iters=200, limiter_freeze_iter=args.freeze_iter,
+                          residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1)
+    # the low-Mach cases' own schedules: the reference's low-Mach test settings
+    # (tests/cpp/test_euler_low_mach_asymptotic.cpp:135-191): CFL 0.1 -> 0.6 over 180 iterations, floor
+    # k 5 -> 1 over 700, preconditioner at M_ref = M, all-speed HLLC fix ramped over iterations 100..300
+    precond = args.uneven_lowmach == "precond"
+    low_cfg = replace(base_cfg, cfl_start=0.1, cfl_max=0.6, cfl_ramp_iters=180, stab_floor_k_start=5.0,
+                      stab_floor_k_target=1.0, stab_ramp_iters=700, precond_on=int(precond),
+                      precond_experimental_enabled=int(precond), all_speed_flux_fix=1)
+    if not precond:  # without the preconditioner the low-Mach cases run the plain schedules
+        low_cfg = replace(base_cfg, all_speed_flux_fix=1)
+    specs, handles, cases, mesh_cells = [], [], [], {}
+    t0 = time.time()
+    for size in sizes:
+        nc, nr = UNEVEN_MESHES[size]
+        mkw = dict(num_circumferential=nc, num_radial=nr, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
+                   quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+        base = generate_mesh(precision=8, **mkw)
+        mesh_cells[size] = base.num_cells
+        log(f"[uneven] {size}: {base.num_cells} cells, deforming {args.uneven_cases} geometries")
+        for _ in range(args.uneven_cases):
+            amp = rng.uniform(-0.005, 0.005, 20)
+            mach = float(rng.choice([0.05, 0.4, 0.7]))
+            aoa = float(rng.uniform(0.0, 6.0))
+            m, _ = deform_mesh(base, amp, tangents=False)
+            handles.append(HipMesh(m, block_cells=args.block_cells))
+            low = mach < 0.1
+            specs.append(dict(size=size, base=base, amp=amp, mach=mach, aoa=aoa, low_mach=low))
+            cases.append((len(handles) - 1, mach, aoa, 0, 0, low_cfg if low else None))
+    total_cells = sum(h.N for h in handles)
+    log(f"[uneven] {len(cases)} cases, {total_cells} cells in all ({time.time() - t0:.1f} s of setup)")
+    cap, chunk = args.uneven_iters, 5000
+    t_primal = 0.0
+    with HipBatch(handles, cases, base_cfg) as b:
+        done = 0
+        while done < cap:
+            n = min(chunk, cap - done)
+            r = b.run(n, 1e-8, iters_per_sync=500, use_graph=True)
+            t_primal += r["seconds"]
+            done += n
+            active = sum(1 for c in r["cases"] if not c["converged"] and not c["failed"] and c["iterations"] >= done)
+            log(f"[uneven] primal: {done} iterations, {active} cases still running, {t_primal:.1f} s")
+            if active == 0:
+                break
+        rows = r["cases"]
+        states = [b.state(i) for i in range(len(cases))]
+        ratios = [b.limiter_ratios(i) for i in range(len(cases))]
+    for i, (sp, row) in enumerate(zip(specs, rows)):
+        log(f"[uneven] case {i}: {sp['size']} M {sp['mach']} AoA {sp['aoa']:.2f} iterations {row['iterations']} "
+            f"rho drop {row['rho_ratio']:.2e} converged {row['converged']} failed {row['failed']}")
+    log("[uneven] adjoint set-up (converged cases): shape tangents and frozen ratios")
+    items, adj_index = [], []
+    for i, sp in enumerate(specs):
+        if not rows[i]["converged"]:
+            continue
+        adj_index.append(i)
+        handles[i].set_limiter_ratios(ratios[i])
+        _, tan = deform_mesh(sp["base"], sp["amp"])
+        case = replace(low_cfg if sp["low_mach"] else base_cfg, mach=sp["mach"], aoa_deg=sp["aoa"],
+                       iterations=rows[i]["iterations"])
+        items.append(dict(hm=handles[i], case=case, U=states[i], shape_tangents=tan))
+    if not items:
+        raise SystemExit("bench.py --uneven: no case reached the density drop; raise --uneven-iters")
+    log(f"[uneven] batched adjoint: {2 * len(items)} solves (CL, CD)")
+    def progress(it, act):
+        if it % 10000 == 0:
+            log(f"[uneven] adjoint: {it} iterations, {act} solves active")
+    sb = sensitivities_batch(items, tol=args.adjoint_tol, max_iters=4 * cap, iters_per_sync=500, progress=progress)
+    for i, c in zip(adj_index, sb["cases"]):
+        log(f"[uneven] adjoint case {i}: " + ", ".join(f"{k}: {v['iterations']} it, conv {v['converged']}, "
+                                                      f"failed {v['failed']}" for k, v in c["adjoint"].items()))
+    # dual consistency on the first adjoint case: tangent solve for AoA against the adjoint
+    it0 = items[0]
+    rc, seed, obj, _ = case_linearization(it0["case"], param="aoa")
+    obj.w_cl = 1.0
+    ob = it0["hm"].objective(obj, it0["U"])
+    log("[uneven] tangent check (case 0, AoA)")
+    try:
+        v, ti = it0["hm"].tangent_solve(rc, seed, it0["U"], cfl=it0["case"].cfl_max, max_iters=4 * cap,
+                                        tol=args.adjoint_tol, check_every=500)
+        d_tan = ob["dJ_daoa"] + float(np.sum(ob["dJdU"] * v))
+    except aero_hip._lib.HipError as e:
+        log(f"[uneven] tangent check failed: {e}")
+        ti, d_tan = {"iterations": None}, float("nan")
+    d_adj = sb["cases"][0]["d"]["cl"]["aoa"]
+    for h in handles:
+        h.close()
+    t_adj = sb["solve_seconds"]
+    adj_rows = [c["adjoint"] for c in sb["cases"]]
+    adj_of = dict(zip(adj_index, adj_rows))
+    per_size = {}
+    for i, (sp, row) in enumerate(zip(specs, rows)):
+        d = per_size.setdefault(sp["size"], dict(cells=mesh_cells[sp["size"]], cases=0, converged=0, iterations=[],
+                                                 adjoint_iterations=[], low_mach_cases=0, low_mach_converged=0))
+        d["cases"] += 1
+        d["converged"] += int(row["converged"])
+        d["low_mach_cases"] += int(sp["low_mach"])
+        d["low_mach_converged"] += int(sp["low_mach"] and row["converged"])
+        d["iterations"].append(row["iterations"])
+        if i in adj_of:
+            d["adjoint_iterations"] += [adj_of[i]["cl"]["iterations"], adj_of[i]["cd"]["iterations"]]
+    for d in per_size.values():
+        d["iterations_median"] = float(np.median(d.pop("iterations")))
+        ai = d.pop("adjoint_iterations")
+        d["adjoint_iterations_median"] = float(np.median(ai)) if ai else None
+    value = len(cases) / t_primal * 3600.0
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "solves/hr", "n_gpus": 1, "steps": done, "warmup": 0,
+           "ms_per_step": round(t_primal / max(max(r["iterations"] for r in rows), 1) * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "uneven48_hh20_mixed16k65k262k_hllc_venkat", "cases": len(cases),
+                      "meshes": {k: UNEVEN_MESHES[k] for k in sizes}, "cells_total": total_cells,
+                      "seed": args.uneven_seed, "bump_amplitude": 0.005, "mach_choices": [0.05, 0.4, 0.7],
+                      "aoa_range_deg": [0.0, 6.0],
+                      "low_mach": ("M 0.05: precond on (M_ref = M, beta_min 1e-4), all-speed HLLC fix on (ramp "
+                                   "100..300), CFL 0.1 -> 0.6, floor k 5 -> 1 over 700 iterations") if precond else
+                                  "M 0.05: preconditioner off, all-speed HLLC fix on, the other cases' schedules",
+                      "iteration_cap": cap, "rho_drop_target": 1e-8, "cfl_max": args.sweep_cfl,
+                      "limiter_freeze_iter": args.freeze_iter, "adjoint_tol": args.adjoint_tol},
+           "uneven": {"primal_seconds": round(t_primal, 3), "adjoint_seconds": round(t_adj, 3),
+                      "adjoint_solves": 2 * len(items), "adjoint_cases": adj_index,
+                      "adjoint_over_primal": round(t_adj / t_primal, 3),
+                      "adjoint_over_primal_per_objective": round(t_adj / t_primal / 2.0, 3),
+                      "gradient_assembly_seconds": round(sb["assembly_seconds"], 3),
+                      "converged_8_decades": sum(int(r["converged"]) for r in rows),
+                      "failed_nonfinite": sum(int(r["failed"]) for r in rows),
+                      "adjoint_converged": sum(int(a[k]["converged"]) for a in adj_rows for k in a),
+                      "adjoint_failed_nonfinite": sum(int(a[k]["failed"]) for a in adj_rows for k in a),
+                      "cases": [dict(mesh=sp["size"], mach=sp["mach"], aoa_deg=round(sp["aoa"], 4),
+                                     iterations=row["iterations"], rho_drop=row["rho_ratio"],
+                                     converged=bool(row["converged"]), cl=row["cl"], cd=row["cd"],
+                                     adjoint_iterations={k: v["iterations"] for k, v in adj_of[i].items()}
+                                     if i in adj_of else None,
+                                     adjoint_converged={k: v["converged"] for k, v in adj_of[i].items()}
+                                     if i in adj_of else None)
+                                for i, (sp, row) in enumerate(zip(specs, rows))],
+                      "per_mesh": per_size,
+                      "tangent_check_case0_aoa": {"adjoint": d_adj, "tangent": d_tan,
+                                                  "rel_diff": abs(d_adj - d_tan) / max(abs(d_tan), 1e-300),
+                                                  "tangent_iterations": ti["iterations"]}}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -295,6 +509,8 @@ def main() -> int:
     ap.add_argument("--block-cells", type=int, default=256)
     ap.add_argument("--no-renumber", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample length (0 = skip)")
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip the other schemes / low-Mach lines timed beside the headline workload")
     ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
     ap.add_argument("--sweep-iters", type=int, default=60000, help="iteration cap per case in --sweep")
     ap.add_argument("--sweep-seq-cases", type=int, default=4, help="cases timed sequentially in --sweep")
@@ -302,6 +518,14 @@ def main() -> int:
     ap.add_argument("--freeze-iter", type=int, default=1000,
                     help="--sweep / --adjoint: iteration at which the limiter is frozen (-1: never)")
     ap.add_argument("--adjoint", action="store_true", help="run the discrete-adjoint workload instead")
+    ap.add_argument("--uneven", action="store_true",
+                    help="run the batched primal + adjoint workload over uneven meshes instead")
+    ap.add_argument("--uneven-cases", type=int, default=16, help="--uneven: cases per mesh size")
+    ap.add_argument("--uneven-meshes", default="16k,65k,262k", help="--uneven: mesh sizes (of 16k, 65k, 262k)")
+    ap.add_argument("--uneven-iters", type=int, default=100000, help="--uneven: iteration cap of the primal batch")
+    ap.add_argument("--uneven-seed", type=int, default=5150, help="--uneven: seed of shapes, Mach and AoA")
+    ap.add_argument("--uneven-lowmach", choices=["precond", "plain"], default="precond",
+                    help="--uneven: M 0.05 cases with the low-Mach preconditioner (its long runs diverge) or without")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="--adjoint: linear residual drop")
     ap.add_argument("--fd-iters", type=int, default=60000, help="--adjoint: continuation iterations per FD run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -322,6 +546,8 @@ def main() -> int:
         return run_sweep(args)
     if args.adjoint:
         return run_adjoint(args)
+    if args.uneven:
+        return run_uneven(args)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the path has no "
               "multi-GPU execution in this build", file=sys.stderr)
@@ -354,24 +580,7 @@ def main() -> int:
     hm.set_state(U)
     stream = torch.cuda.ExternalStream(hm.stream())
     log(f"[bench] warmup {args.warmup}")
-    for _ in range(args.warmup):
-        hm.residual_async(cfg)
-    torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} steps")
-    hm.timing_begin(args.steps)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    if torch.distributed.is_available() and torch.distributed.is_initialized():
-        torch.distributed.barrier()
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        hm.residual_async(cfg)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    tm = hm.timing_end()
-    total_ms = e0.elapsed_time(e1)
+    total_ms, wall, tm = time_residual(hm, cfg, args.steps, args.warmup, stream)
     ms_per_step = total_ms / args.steps
     value = N * args.steps / (total_ms / 1e3) / 1e6
     R, lam, diag = hm.get_residual()
@@ -393,6 +602,24 @@ def main() -> int:
              "eval_hbm_frac": round(eval_bytes * args.steps / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
              "eval_algorithmic_bytes": eval_bytes, "wall_s": round(wall, 4), "layout": layout,
              "fallbacks": diag}
+    # The workload's other numerics on the same mesh: the (Rusanov, minmod) scheme pair and the low-Mach
+    # preconditioned all-speed run (M 0.01, precond on, all-speed HLLC fix f_min 0.05), as reported lines
+    # beside the headline one.
+    variants = {}
+    if args.variants:
+        for vname, vmach, vcfg in variant_configs(flux, lim, mach, aoa):
+            log(f"[bench] variant {vname}")
+            vff = freestream(vmach, aoa)[1]
+            hm.set_state(perturbed_state(mesh.cell_center, vff))
+            v_ms, _, vtm = time_residual(hm, vcfg(vff), args.steps, args.warmup, stream)
+            vR, _, vdiag = hm.get_residual()
+            vn = max(vtm["n"], 1)
+            v_flux = vtm["flux_ms"] / vn
+            variants[vname] = {"value": round(N * args.steps / (v_ms / 1e3) / 1e6, 2), "unit": "Mcell-res/s",
+                               "ms_per_step": round(v_ms / args.steps, 5), "mach": vmach,
+                               "flux_kernel_ms": round(v_flux, 5),
+                               "flux_kernel_hbm_frac": round(flux_bytes / (v_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "finite": bool(np.all(np.isfinite(vR))), "fallbacks": vdiag}
     cpu = None
     if args.cpu_seconds > 0:
         log("[bench] cpu baseline")
@@ -404,7 +631,7 @@ def main() -> int:
            "config": {"workload": args.workload, "cells": N, "faces": F, "flux": flux, "limiter": lim,
                       "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": args.block_cells,
                       "renumber": not args.no_renumber},
-           "roofline": roofline, "cpu_baseline": cpu, "detail": extra}
+           "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "detail": extra}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -226,6 +226,16 @@ typedef struct aero_batch_case {
   int32_t mesh_index; /* index into the meshes passed to aero_hip_batch_create */
   double mach;
   double aoa_deg;
+  /* Per-case numerics switches, so that one batch can mix low-Mach preconditioned cases with plain
+   * ones: 0 takes the case configuration's setting, 1 turns it on, -1 off.  precond = 1 turns the
+   * low-Mach preconditioner on without the experimental gate (the caller has passed it). */
+  int32_t precond;
+  int32_t all_speed_flux_fix;
+  /* NULL (the batch's base configuration) or this case's own: freestream reference values, CFL,
+   * stabilisation-floor and all-speed schedules, preconditioner and low-Mach settings.  It must agree
+   * with the base on flux scheme, limiter, rk_stages, startup iterations, freeze iteration, local time
+   * stepping and gamma; its mach and aoa_deg are ignored (the fields above apply). */
+  const aero_case_config* config;
 } aero_batch_case;
 
 typedef struct aero_batch_result {
@@ -233,8 +243,8 @@ typedef struct aero_batch_result {
   double l2, rho_l2, rho_ratio, cl, cd, cm;
 } aero_batch_result;
 
-/* `base` supplies everything but Mach and AoA (flux, limiter, CFL schedule, startup, precond, ...);
- * rk_stages 1 or 3, local time stepping only.  hist_capacity rows of (l2, rho_l2, cfl, linf) are kept
+/* `base` supplies everything but Mach and AoA (flux, limiter, CFL schedule, startup, precond, ...) for
+ * cases without their own configuration; rk_stages 1 or 3, local time stepping only.  hist_capacity rows of (l2, rho_l2, cfl, linf) are kept
  * per case. */
 int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const aero_batch_case* cases,
                           int32_t n_cases, const aero_case_config* base, int32_t hist_capacity,
@@ -333,6 +343,34 @@ int aero_hip_adjoint_solve(aero_hip_mesh* mesh, const aero_residual_config* cfg,
 int aero_hip_tangent_solve(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
                            const aero_tangent_seed* seed, aero_linear_solve* ctl, double* v, double* history, char* err,
                            size_t errlen);
+
+/* One solve of a batched adjoint solve. */
+typedef struct aero_adjoint_case {
+  aero_hip_mesh* mesh;             /* linearised at its state, or at `conserved` when given */
+  const aero_residual_config* cfg;
+  const double* conserved;         /* [N][5] or NULL */
+  const double* rhs;               /* [N][5] dJ/dU */
+  double* psi;                     /* [N][5] out: R_U^T psi = rhs */
+  double cfl;                      /* > 0: this solve's pseudo-time CFL; 0: ctl->cfl */
+  int32_t iterations, converged, failed; /* out */
+  double res0, res;                /* out: first and last linear residual L2 norms */
+} aero_adjoint_case;
+
+/* Many adjoint solves advanced together (e.g. CL and CD of every case of a sweep, on meshes of uneven
+ * size): one launch per transposed RK stage over every (solve, block of 256 cells), per-solve stopping
+ * on the device (the tests of aero_hip_adjoint_solve, checked after every iteration), iterations replayed
+ * from a hipGraph.  Solves on the same handle share its face Jacobians and must share cfg and
+ * conserved; all handles need the same gradient kind.  ctl gives rk_stages, cfl (unless a solve sets
+ * its own), max_iters and tol (gmres_restart must be 0); ctl->iterations and ctl->seconds receive the largest iteration count and
+ * the wall time of the iterations.  The host checks progress every iters_per_sync iterations and then
+ * calls `progress` (may be NULL) with the iterations run and the solves still active; a non-zero return
+ * stops the iterations there.  A solve whose residual turns non-finite stops with `failed` set while the
+ * others go on (the call itself succeeds).  Each solve equals aero_hip_adjoint_solve with
+ * check_every = 1 bit for bit.  No reference counterpart. */
+typedef int (*aero_progress_fn)(void* user, int32_t iterations, int32_t active);
+int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n_cases, aero_linear_solve* ctl,
+                                 int32_t iters_per_sync, aero_progress_fn progress, void* user, char* err,
+                                 size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
  * Mesh generation (host side).  The NACA-4 O-grid of make_airfoil_ogrid_mesh

@@ -273,3 +273,72 @@ def test_shape_sensitivities(hip):
                 cls.append(hmi.run_case(case)["cl"])
         fd = (cls[0] - cls[1]) / (2 * h)
         assert abs(s["d"][f"shape{i}"] - fd) <= 1e-5 * abs(fd), (i, s["d"][f"shape{i}"], fd)
+
+
+def _first_order_state(hip, hm, mach, aoa, iters, **kw):
+    case = _converged_case(hip, hm.mesh, aoa, mach, iters)
+    for k, v in kw.items():
+        setattr(case, k, v)
+    return case, hm.run_case(case)["conserved"]
+
+
+@pytest.mark.parametrize("rk_stages", [3, 1])
+def test_adjoint_batch_matches_single_solves(hip, rk_stages):
+    """Batched adjoint solves (CL and CD on two meshes of different size, one preconditioned) against
+    single solves checked after every iteration: bitwise identical under RK3; under RK1, whose single
+    solve checks every second iteration, equal to the solve tolerance."""
+    m1 = helpers.mesh("40x8")
+    m2 = _mesh64()
+    with hip.HipMesh(m1) as h1, hip.HipMesh(m2) as h2:
+        say("[adj-batch] primal states")
+        c1, U1 = _first_order_state(hip, h1, 0.5, 2.0, 2000)
+        # preconditioner settings under which the linearised iteration contracts quickly (at the defaults
+        # beta_min 0.2, mach_ref 0.15 and M 0.2 its amplification reaches 0.9997 on this mesh)
+        c2, U2 = _first_order_state(hip, h2, 0.4, 3.0, 2000, precond_on=1, precond_experimental_enabled=1,
+                                    precond_mach_ref=0.3, precond_beta_min=0.5)
+        solves = []
+        for hm, case, U, cfl in ((h1, c1, U1, None), (h2, c2, U2, 1.2)):  # the second solve has its own CFL
+            rc, _, obj, _ = hip.case_linearization(case)
+            for w in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0)):
+                obj.w_cl, obj.w_cd, obj.w_cm = w
+                rhs = hm.objective(obj, U)["dJdU"]
+                solves.append((hm, rc, rhs, U, cfl))
+        kw = dict(rk_stages=rk_stages, max_iters=20000, tol=1e-10)
+        say(f"[adj-batch] batched solve rk{rk_stages}")
+        psis, infos, _ = hip.adjoint_solve_batch(solves, cfl=1.5, iters_per_sync=150, **kw)
+        for i, (hm, rc, rhs, U, cfl) in enumerate(solves):
+            say(f"[adj-batch] single solve {i}")
+            psi, info = hm.adjoint_solve(rc, rhs, U, check_every=1, cfl=cfl or 1.5, **kw)
+            assert infos[i]["converged"] and not infos[i]["failed"], infos[i]
+            if rk_stages == 3:
+                assert infos[i]["iterations"] == info["iterations"], (i, infos[i], info["iterations"])
+                assert np.array_equal(psis[i], psi), f"solve {i} differs from its single solve"
+            else:
+                assert abs(infos[i]["iterations"] - info["iterations"]) <= 1
+                assert _maxrel(psis[i], psi) <= 1e-8
+
+
+def test_sensitivities_batch_matches_single_case_sensitivities(hip):
+    """sensitivities_batch (one batched solve for every case and objective) against sensitivities per
+    case and objective, with shape tangents on a deformed mesh."""
+    from aero_hip.adjoint import sensitivities, sensitivities_batch
+    base = helpers.mesh("40x8")
+    amp = np.random.default_rng(7).uniform(-0.002, 0.002, 20)
+    m, tan = hip.deform_mesh(base, amp)
+    with hip.HipMesh(m) as h1, hip.HipMesh(base) as h2:
+        say("[sens-batch] primal states")
+        c1, U1 = _first_order_state(hip, h1, 0.5, 2.0, 3000)
+        c2, U2 = _first_order_state(hip, h2, 0.4, 1.0, 3000)
+        items = [dict(hm=h1, case=c1, U=U1, shape_tangents=tan), dict(hm=h2, case=c2, U=U2)]
+        say("[sens-batch] batched")
+        sb = sensitivities_batch(items, cfl=1.5, max_iters=20000, tol=1e-12)
+        for it, res in zip(items, sb["cases"]):
+            for name, w in (("cl", (1.0, 0.0, 0.0)), ("cd", (0.0, 1.0, 0.0))):
+                say(f"[sens-batch] single {name}")
+                s = sensitivities(it["hm"], it["case"], it["U"], weights=w, shape_tangents=it.get("shape_tangents"),
+                                  cfl=1.5, max_iters=20000, tol=1e-12)
+                assert res["J"][name] == pytest.approx(s["J"], rel=1e-13)
+                assert set(res["d"][name]) == set(s["d"])
+                for p, v in s["d"].items():
+                    assert abs(res["d"][name][p] - v) <= 1e-8 * max(abs(v), 1e-10), (name, p, res["d"][name][p], v)
+

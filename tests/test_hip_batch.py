@@ -122,3 +122,53 @@ def test_batch_frozen_limiter_matches_solo_runs(hip):
         assert np.array_equal(hists[i][:, 1], ref["history"][:, 8]), f"case {i} density L2 history differs"
     g1.close()
     g2.close()
+
+
+def test_batch_per_case_switches_match_solo_runs(hip):
+    """Per-case low-Mach preconditioner and all-speed switches: a batch mixing preconditioned all-speed,
+    preconditioned and plain cases equals the solo runs of the same cases configured that way."""
+    m1 = helpers.mesh("40x8")
+    m2 = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    base = dict(BASE, all_speed_ramp_start_iter=5, all_speed_ramp_iters=10, precond_beta_min=1e-4,
+                precond_mach_ref=0.0)
+    # a case with its own configuration: CFL, floor and all-speed schedules, preconditioner, p_inf
+    own = dict(base, cfl_start=0.1, cfl_max=0.6, cfl_ramp_iters=18, stab_floor_k_start=5.0, stab_floor_k_target=1.0,
+               stab_ramp_iters=20, precond_on=1, precond_experimental_enabled=1, all_speed_flux_fix=1,
+               all_speed_ramp_start_iter=3, all_speed_ramp_iters=7, p_inf=90000.0, chord=1.3)
+    # (mesh, mach, aoa, precond, all-speed fix[, config]): the base has the preconditioner gated off
+    cases = [(0, 0.05, 1.0, 1, 1), (1, 0.5, 2.5, 0, 0), (1, 0.1, 2.0, 1, 0), (0, 0.3, -1.0, -1, 1),
+             (1, 0.2, 1.5, 0, 0, hip.CaseConfig(**own))]
+    iters = 30
+    g1, g2 = hip.HipMesh(m1), hip.HipMesh(m2)
+    say("[batch] per-case preconditioner / all-speed switches and configurations")
+    with hip.HipBatch([g1, g2], cases, hip.CaseConfig(**base), hist_capacity=iters) as b:
+        r = b.run(iters, fixed_iters=True, iters_per_sync=9, use_graph=True)
+        states = [b.state(i) for i in range(len(cases))]
+        hists = [b.history(i) for i in range(len(cases))]
+    for i, c in enumerate(cases):
+        gi, mach, aoa, pc, af = c[:5]
+        if len(c) > 5:
+            kw = dict(own, iterations=iters, mach=mach, aoa_deg=aoa)
+        else:
+            kw = dict(base, iterations=iters, mach=mach, aoa_deg=aoa)
+            kw["precond_experimental_enabled"] = 1 if pc == 1 else 0
+            kw["precond_on"] = 1 if pc == 1 else 0
+            kw["all_speed_flux_fix"] = 1 if af == 1 else 0
+        say(f"[batch] solo case {i}")
+        with hip.HipMesh([m1, m2][gi]) as hm:
+            ref = hm.run_case(hip.CaseConfig(**kw))
+        assert np.array_equal(states[i], ref["conserved"]), f"case {i} state differs from its solo run"
+        assert np.array_equal(hists[i][:, 1], ref["history"][:, 8]), f"case {i} density L2 history differs"
+        assert r["cases"][i]["cl"] == pytest.approx(ref["cl"], rel=1e-12, abs=1e-14)
+    g1.close()
+    g2.close()
+
+
+def test_batch_rejects_bad_switch(hip):
+    g = hip.HipMesh(helpers.mesh("40x8"))
+    with pytest.raises(RuntimeError, match="switch"):
+        hip.HipBatch([g], [(0, 0.3, 1.0, 2, 0)], hip.CaseConfig(**BASE))
+    with pytest.raises(RuntimeError, match="differs from the base"):
+        hip.HipBatch([g], [(0, 0.3, 1.0, 0, 0, hip.CaseConfig(**dict(BASE, limiter=0)))], hip.CaseConfig(**BASE))
+    g.close()
+
