@@ -1,0 +1,325 @@
+"""Run a case file on the HIP backend and write the reference runner's outputs.
+
+`run_case_file(case_path, out_dir)` plays the role of the native `run_case(case_path, out_dir, backend)`
+behind the reference's pybind module (bindings/pybind_module.cpp:18-38, numerics/residual_stub.cpp:395-785)
+for euler_airfoil_2d cases: it builds the case's O-grid with the reference's generator (FP32 point set,
+so the mesh is the reference's mesh), runs the driver on the device and writes, in the reference's
+column layouts,
+
+  residuals.csv   iter,residual_l1,residual_l2,residual_linf,residual_ratio  (euler_solver.cpp:722, :1161)
+  forces.csv      iter,cl,cd,cm,lift,drag,moment,dcl,dcd,dcm,mean_dcl,mean_dcd,mean_dcm,
+                  force_window_stable,force_mean_drift_stable,force_physical_converged  (:723-724, :1163-1169)
+  cp_wall.csv     s,x,y,Cp over the wall faces ordered from the trailing edge over the upper surface
+                  (post/forces.cpp:39-129, :177-206; euler_solver.cpp:1467-1476)
+  field_0000.vtu  cell rho, u, v, p, Mach, residual_magnitude (io/io_vtk_stub.cpp:95-166)
+  run.log         the run summary lines (residual_stub.cpp:600-700, the subset this backend knows)
+
+and returns the same summary dict as the pybind run_case.  Numbers are printed like a C++ ostream's
+default float format (six significant digits).
+"""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .case_file import euler_case_from_kv, load_case
+from .mesh import Mesh, generate_mesh
+from .solver import CaseConfig, HipMesh
+
+VERSION = "0.1.1-hip"
+
+
+def _g(v) -> str:
+    """A C++ ostream's default formatting of a float (precision 6, %g-style)."""
+    v = float(v)
+    if math.isnan(v):
+        return "nan" if math.copysign(1.0, v) > 0 else "-nan"
+    if math.isinf(v):
+        return "inf" if v > 0 else "-inf"
+    return f"{v:.6g}"
+
+
+def force_monitor(cl, cd, cm, window: int, tol: float, drift_tol: float) -> dict:
+    """Per-iteration force columns of ConvergenceMonitor (solvers/euler/convergence_monitor.cpp:21-117):
+    change from the previous iteration, stability of the last `window` values (max - min <= tol each),
+    mean drift between the last two windows and its test (<= drift_tol; tol when drift_tol <= 0)."""
+    n = len(cl)
+    w = max(int(window), 1)
+    dtol = drift_tol if drift_tol > 0.0 else tol
+    F = np.stack([np.asarray(cl, float), np.asarray(cd, float), np.asarray(cm, float)], axis=1)
+    d = np.zeros((n, 3))
+    if n > 1:
+        d[1:] = np.abs(F[1:] - F[:-1])
+    md = np.zeros((n, 3))
+    stable = np.zeros(n, bool)
+    mstable = np.zeros(n, bool)
+    for i in range(n):
+        if i + 1 >= w:
+            win = F[i + 1 - w:i + 1]
+            stable[i] = bool(np.all(win.max(axis=0) - win.min(axis=0) <= tol))
+        if i + 1 >= 2 * w:
+            prev = F[i + 1 - 2 * w:i + 1 - w].sum(axis=0)
+            curr = F[i + 1 - w:i + 1].sum(axis=0)
+            md[i] = np.abs(curr / w - prev / w)
+            mstable[i] = bool(np.all(md[i] <= dtol))
+    return dict(d=d, mean_d=md, stable=stable, mean_stable=mstable, converged=stable & mstable)
+
+
+def order_wall_faces(mesh: Mesh) -> np.ndarray:
+    """Wall faces in the order of order_wall_faces (post/forces.cpp:39-129): from the wall vertex of
+    largest x (ties within 1e-8: largest y) towards its neighbour of larger y, then along the chain;
+    by descending atan2(y, x) of the face centres when the chain does not close."""
+    faces = np.arange(mesh.wall_start, mesh.wall_start + mesh.wall_count)
+    if len(faces) <= 2:
+        return faces
+    fv = mesh.face_vertices
+    pts = mesh.points
+    edge = {}
+    nbrs: dict[int, list[int]] = {}
+    for f in faces:
+        a, b = int(fv[f, 0]), int(fv[f, 1])
+        edge[(min(a, b), max(a, b))] = int(f)
+        nbrs.setdefault(a, []).append(b)
+        nbrs.setdefault(b, []).append(a)
+    start, bx, by = -1, -1.0e30, -1.0e30
+    for v in sorted(nbrs):
+        x, y = float(pts[v, 0]), float(pts[v, 1])
+        if x > bx or (abs(x - bx) < 1.0e-8 and y > by):
+            start, bx, by = v, x, y
+    first = nbrs.get(start, [])
+    if start < 0 or len(first) < 2:
+        return faces
+    nxt = first[0]
+    if pts[first[1], 1] - pts[start, 1] > pts[first[0], 1] - pts[start, 1]:
+        nxt = first[1]
+    out = []
+    prev, cur = start, nxt
+    for _ in range(len(faces)):
+        f = edge.get((min(prev, cur), max(prev, cur)))
+        if f is None:
+            out = []
+            break
+        out.append(f)
+        cand = [c for c in nbrs[cur] if c != prev]
+        if not cand:
+            out = []
+            break
+        prev, cur = cur, cand[0]
+        if prev == start:
+            break
+    if len(out) != len(faces):
+        ang = np.arctan2(mesh.face_center[faces, 1], mesh.face_center[faces, 0])
+        return faces[np.argsort(-ang, kind="stable")]
+    return np.asarray(out)
+
+
+def _primitives(U: np.ndarray, gamma: float):
+    rho = np.maximum(U[:, 0], 1.0e-8)
+    u, v = U[:, 1] / rho, U[:, 2] / rho
+    p = np.maximum((gamma - 1.0) * (U[:, 4] - 0.5 * rho * (u * u + v * v)), 1.0e-8)
+    a = np.sqrt(gamma * np.maximum(p, 1.0e-8) / rho)
+    mach = np.sqrt(u * u + v * v) / np.maximum(a, 1.0e-8)
+    return rho, u, v, p, mach
+
+
+def write_vtu(path: Path, mesh: Mesh, U: np.ndarray, R: np.ndarray, gamma: float) -> None:
+    rho, u, v, p, mach = _primitives(U, gamma)
+    inv = 1.0 / np.maximum(mesh.volume.astype(float), 1.0e-12)
+    rmag = np.sqrt(np.sum((R * inv[:, None]) ** 2, axis=1))
+    offs = np.asarray(mesh.cell_offsets)
+    sizes = np.diff(np.concatenate([[0], offs]))
+    types = np.where(sizes == 3, 5, np.where(sizes == 4, 9, 7))  # triangle, quad, polygon
+    arr = lambda a: " ".join(_g(x) for x in np.ravel(a))
+    iarr = lambda a: " ".join(str(int(x)) for x in np.ravel(a))
+    with open(path, "w", encoding="utf-8") as o:
+        o.write('<?xml version="1.0"?>\n')
+        o.write('<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">\n')
+        o.write("  <UnstructuredGrid>\n")
+        o.write(f'    <Piece NumberOfPoints="{len(mesh.points)}" NumberOfCells="{mesh.num_cells}">\n')
+        o.write("      <Points>\n")
+        o.write(f'        <DataArray type="Float32" NumberOfComponents="3" format="ascii">{arr(mesh.points)}</DataArray>\n')
+        o.write("      </Points>\n      <Cells>\n")
+        o.write(f'        <DataArray type="Int32" Name="connectivity" format="ascii">{iarr(mesh.cell_connectivity)}</DataArray>\n')
+        o.write(f'        <DataArray type="Int32" Name="offsets" format="ascii">{iarr(offs)}</DataArray>\n')
+        o.write(f'        <DataArray type="UInt8" Name="types" format="ascii">{iarr(types)}</DataArray>\n')
+        o.write("      </Cells>\n      <PointData/>\n")
+        o.write('      <CellData Scalars="rho">\n')
+        for name, a in (("rho", rho), ("u", u), ("v", v), ("p", p), ("Mach", mach), ("residual_magnitude", rmag)):
+            o.write(f'        <DataArray type="Float32" Name="{name}" format="ascii">{arr(a)}</DataArray>\n')
+        o.write("      </CellData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n")
+
+
+def write_outputs(out_dir: Path, mesh: Mesh, case: CaseConfig, out: dict) -> dict:
+    """Write residuals.csv, forces.csv, cp_wall.csv and field_0000.vtu of one run; returns the paths."""
+    out_dir.mkdir(parents=True, exist_ok=True)
+    H = out["history"]
+    rho_inf = case.rho_inf if case.rho_inf > 0.0 else case.p_inf / (case.gas_constant * case.t_inf)
+    V = max(case.mach, 0.0) * math.sqrt(case.gamma * case.p_inf / max(rho_inf, 1.0e-12))
+    q = 0.5 * rho_inf * V * V + 1.0e-12
+    chord = max(case.chord, 1.0e-8)
+    s_ref = chord * 1.0
+    paths = dict(residuals=out_dir / "residuals.csv", forces=out_dir / "forces.csv", cp=out_dir / "cp_wall.csv",
+                 vtu=out_dir / "field_0000.vtu")
+    with open(paths["residuals"], "w", encoding="utf-8") as f:
+        f.write("iter,residual_l1,residual_l2,residual_linf,residual_ratio\n")
+        for r in H:
+            f.write(f"{int(r[_lib.H_ITER])},{_g(r[_lib.H_L1])},{_g(r[_lib.H_L2])},{_g(r[_lib.H_LINF])},"
+                    f"{_g(r[_lib.H_RATIO])}\n")
+    cl, cd, cm = H[:, _lib.H_CL], H[:, _lib.H_CD], H[:, _lib.H_CM]
+    mon = force_monitor(cl, cd, cm, case.force_stability_window, case.force_stability_tol, case.force_mean_drift_tol)
+    with open(paths["forces"], "w", encoding="utf-8") as f:
+        f.write("iter,cl,cd,cm,lift,drag,moment,dcl,dcd,dcm,mean_dcl,mean_dcd,mean_dcm,"
+                "force_window_stable,force_mean_drift_stable,force_physical_converged\n")
+        for i, r in enumerate(H):
+            vals = [cl[i], cd[i], cm[i], cl[i] * q * s_ref, cd[i] * q * s_ref, cm[i] * q * s_ref * chord,
+                    *mon["d"][i], *mon["mean_d"][i]]
+            f.write(f"{int(r[_lib.H_ITER])}," + ",".join(_g(x) for x in vals) +
+                    f",{int(mon['stable'][i])},{int(mon['mean_stable'][i])},{int(mon['converged'][i])}\n")
+    order = order_wall_faces(mesh)
+    cp = np.asarray(out["wall_cp"])
+    with open(paths["cp"], "w", encoding="utf-8") as f:
+        f.write("s,x,y,Cp\n")
+        s = 0.0
+        for face in order:
+            f.write(f"{_g(s)},{_g(mesh.face_center[face, 0])},{_g(mesh.face_center[face, 1])},"
+                    f"{_g(cp[face - mesh.wall_start])}\n")
+            s += float(mesh.area[face])
+    write_vtu(paths["vtu"], mesh, out["conserved"], out["residual"], case.gamma)
+    return paths
+
+
+def case_and_mesh(kv: dict) -> tuple[CaseConfig, Mesh]:
+    """CaseConfig and the reference generator's O-grid (FP32 point set, FP64 arrays) of native keys."""
+    case_type = kv.get("case_type", "scalar_advect_demo")
+    if case_type != "euler_airfoil_2d":
+        raise ValueError(f"the HIP backend runs euler_airfoil_2d cases (case_type={case_type})")
+    case, ms = euler_case_from_kv(kv)
+    if ms["airfoil_source"].lower() != "naca4":
+        raise ValueError("the HIP backend's mesh generator builds NACA 4-digit airfoils (airfoil_source=naca4)")
+    mesh = generate_mesh(naca=ms["naca"], num_circumferential=ms["num_circumferential"], num_radial=ms["num_radial"],
+                         farfield_radius=ms["farfield_radius"], radial_stretch=ms["radial_stretch"],
+                         chord=ms["chord"], cells="tri", precision=4).astype(np.float64)
+    return case, mesh
+
+
+def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int = 256,
+                  overrides: dict | None = None) -> dict:
+    """Run an euler_airfoil_2d case file (native key=value or YAML) on the device and write the reference
+    runner's outputs into out_dir; returns the pybind run_case summary dict.  overrides: native keys
+    replacing the file's (e.g. {"aoa_deg": "4"})."""
+    kv = load_case(case_path)
+    kv.update({k: str(v) for k, v in (overrides or {}).items()})
+    case_type = kv.get("case_type", "scalar_advect_demo")
+    case, mesh = case_and_mesh(kv)
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    with HipMesh(mesh, block_cells=block_cells) as hm:
+        out = hm.run_case(case)
+    if out["nonfinite"]:
+        raise RuntimeError("Euler pseudo-time iteration produced a non-finite state.")
+    write_outputs(out_dir, mesh, case, out)
+    H = out["history"]
+    last = H[-1]
+    summary = dict(status="ok", backend="hip", case_type=case_type, run_log=str(out_dir / "run.log"),
+                   iterations=int(len(H)), residual_l1=float(last[_lib.H_L1]), residual_l2=float(last[_lib.H_L2]),
+                   residual_linf=float(last[_lib.H_LINF]), cl=float(out["cl"]), cd=float(out["cd"]),
+                   cm=float(out["cm"]))
+    rho_inf = case.rho_inf if case.rho_inf > 0.0 else case.p_inf / (case.gas_constant * case.t_inf)
+    V = max(case.mach, 0.0) * math.sqrt(case.gamma * case.p_inf / max(rho_inf, 1.0e-12))
+    a = math.radians(case.aoa_deg)
+    f = out["forces"]
+    lines = ["AeroCFD Euler airfoil 2D", f"version={VERSION}", f"case_path={case_path}", "backend=hip",
+             f"case_type={case_type}", f"num_cells={mesh.num_cells}", f"num_faces={mesh.num_faces}",
+             f"iterations={len(H)}", f"residual_l1={_g(summary['residual_l1'])}",
+             f"residual_l2={_g(summary['residual_l2'])}", f"residual_linf={_g(summary['residual_linf'])}",
+             f"cl={_g(f[5])}", f"cd={_g(f[6])}", f"cm={_g(f[7])}", f"aoa_deg={_g(case.aoa_deg)}",
+             f"aoa_rad={_g(a)}", f"u_inf={_g(V * math.cos(a))}", f"v_inf={_g(V * math.sin(a))}",
+             f"wall_faces_integrated={mesh.wall_count}", f"q_inf={_g(0.5 * rho_inf * V * V + 1e-12)}",
+             f"V_inf={_g(V)}", f"rho_inf={_g(rho_inf)}", f"S_ref={_g(max(case.chord, 1e-8))}",
+             f"chord={_g(max(case.chord, 1e-8))}", f"Fx={_g(f[0])}", f"Fy={_g(f[1])}", f"L={_g(f[2])}",
+             f"D={_g(f[3])}", f"flux={'hllc' if case.flux_scheme else 'rusanov'}",
+             f"limiter={'venkat' if case.limiter else 'minmod'}",
+             f"precond={'on' if case.precond_on else 'off'}",
+             f"low_mach_fix={'on' if case.low_mach_fix else 'off'}",
+             f"all_speed_flux_fix={'on' if case.all_speed_flux_fix else 'off'}",
+             f"device_seconds={out['seconds']:.6g}"]
+    (out_dir / "run.log").write_text("\n".join(lines) + "\n", encoding="utf-8")
+    return summary
+
+
+def wall_cp_from_state(mesh: Mesh, case: CaseConfig, U: np.ndarray) -> np.ndarray:
+    """Cp of the wall faces (wall-patch order) from the owner cells' pressure (forces.cpp:177-206)."""
+    rho_inf = case.rho_inf if case.rho_inf > 0.0 else case.p_inf / (case.gas_constant * case.t_inf)
+    V = max(case.mach, 0.0) * math.sqrt(case.gamma * case.p_inf / max(rho_inf, 1.0e-12))
+    q = 0.5 * rho_inf * V * V + 1.0e-12
+    _, _, _, p, _ = _primitives(U, case.gamma)
+    owners = mesh.owner[mesh.wall_start:mesh.wall_start + mesh.wall_count]
+    return (p[owners] - case.p_inf) / q
+
+
+def run_sweep_file(sweep_path: str | Path, out_dir: str | Path, batched: bool = False, block_cells: int = 256,
+                   jobs: int = 1, progress=None) -> dict:
+    """A sweep YAML (base_case: a case file relative to the sweep file; sweep: aoa_deg and optionally
+    mach lists, crossed) as python/cfd/sweep.py:14-59 runs it, except that every run gets its own AoA
+    and Mach.  Serial: one device run per case with the reference's outputs in run_000, run_001, ...
+    Batched: all cases in one aero_hip_batch over the shared mesh, each stopping at a density-residual drop
+    of residual_reduction_target or at `iterations`; per run cp_wall.csv and field_0000.vtu."""
+    import yaml
+    sp = Path(sweep_path)
+    if not sp.exists():
+        raise FileNotFoundError(f"Sweep file not found: {sp}")
+    cfg = yaml.safe_load(sp.read_text(encoding="utf-8")) or {}
+    if not isinstance(cfg, dict):
+        raise ValueError("Sweep YAML must contain a top-level mapping.")
+    if not cfg.get("base_case"):
+        raise ValueError("Sweep YAML must define 'base_case'.")
+    base_path = (sp.parent / str(cfg["base_case"])).resolve()
+    kv = load_case(base_path)
+    case0, mesh = case_and_mesh(kv)
+    sweep = cfg.get("sweep", {}) or {}
+    aoas = [float(a) for a in sweep.get("aoa_deg", [case0.aoa_deg])]
+    machs = [float(m) for m in sweep.get("mach", [case0.mach])]
+    points = [(m, a) for m in machs for a in aoas]
+    root = Path(out_dir)
+    root.mkdir(parents=True, exist_ok=True)
+    runs = []
+    if not batched:
+        for i, (m, a) in enumerate(points):
+            s = run_case_file(base_path, root / f"run_{i:03d}", block_cells=block_cells,
+                              overrides={"aoa_deg": repr(a), "mach": repr(m)})
+            s.update(aoa_deg=a, mach=m)
+            runs.append(s)
+            if progress:
+                progress(i + 1, len(points))
+    else:
+        from dataclasses import replace
+        from .solver import HipBatch, case_linearization
+        with HipMesh(mesh, block_cells=block_cells) as hm:
+            with HipBatch([hm], [(0, m, a) for m, a in points], case0) as b:
+                r = b.run(case0.iterations, case0.residual_reduction_target, iters_per_sync=200)
+                states = [b.state(i) for i in range(len(points))]
+            for i, ((m, a), row) in enumerate(zip(points, r["cases"])):
+                d = root / f"run_{i:03d}"
+                d.mkdir(parents=True, exist_ok=True)
+                ci = replace(case0, mach=m, aoa_deg=a, iterations=max(row["iterations"], 1))
+                rc, _, _, _ = case_linearization(ci)  # the residual configuration of the last iteration
+                R, _, _ = hm.residual(states[i], rc)
+                cp = wall_cp_from_state(mesh, ci, states[i])
+                order = order_wall_faces(mesh)
+                with open(d / "cp_wall.csv", "w", encoding="utf-8") as f:
+                    f.write("s,x,y,Cp\n")
+                    s_acc = 0.0
+                    for face in order:
+                        f.write(f"{_g(s_acc)},{_g(mesh.face_center[face, 0])},{_g(mesh.face_center[face, 1])},"
+                                f"{_g(cp[face - mesh.wall_start])}\n")
+                        s_acc += float(mesh.area[face])
+                write_vtu(d / "field_0000.vtu", mesh, states[i], R, ci.gamma)
+                runs.append(dict(status="ok", backend="hip", case_type="euler_airfoil_2d", aoa_deg=a, mach=m,
+                                 iterations=row["iterations"], converged=row["converged"],
+                                 rho_residual_ratio=row["rho_ratio"], residual_l2=row["l2"], cl=row["cl"],
+                                 cd=row["cd"], cm=row["cm"]))
+    return dict(status="ok", jobs_requested=int(jobs), output_root=str(root), batched=bool(batched), runs=runs)
+
