@@ -32,7 +32,9 @@ struct aero_hip_batch {
   DevMesh* d_geoms = nullptr;
   BatchCase* d_cases = nullptr;
   CaseCtl* d_ctl = nullptr;
-  int2* d_work = nullptr;
+  int2* d_work = nullptr;          // work list of the active cases (capacity: every case's blocks)
+  std::vector<int2> h_list;        // every (case, block) item, per geometry block-major / case-minor
+  std::vector<char> listed;        // the cases the device work list holds
   double* d_hist = nullptr;
   int hist_cap = 0;
   aero_case_config cfg{};
@@ -116,6 +118,37 @@ int enqueue_iteration(aero_hip_batch* b, bool second, int start, bool frozen, ch
   AERO_TRY(aero_host::launch_batch_reduce(b->d_cases, b->d_ctl, b->ncases, c.gamma, final_buf, b->stream),
            "k_batch_reduce");
   return 0;
+}
+
+// The device work list: the listed cases' items of h_list, spread so that each XCD (workgroup index mod 8)
+// receives a contiguous run.  Kernels launched over it reach only these cases, so cases that stopped cost
+// nothing; the graphs, whose grids depend on the list length, are rebuilt.
+int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* err, size_t errlen) {
+  std::vector<int2> list;
+  list.reserve(b->h_list.size());
+  for (const int2& w : b->h_list)
+    if (active[w.x]) list.push_back(w);
+  const int W = static_cast<int>(list.size());
+  const int per = (W + 7) / 8;
+  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
+  for (int j = 0; j < static_cast<int>(work.size()); ++j) {
+    const int src = (j & 7) * per + (j >> 3);
+    if (src < W) work[j] = list[src];
+  }
+  if (work.empty()) work.push_back(make_int2(-1, -1));
+  AERO_TRY(hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream),
+           "work list upload");
+  AERO_TRY(hipStreamSynchronize(b->stream), "work list upload");
+  b->nwork = static_cast<int>(work.size());
+  b->listed = active;
+  for (auto& row : b->graph)
+    for (auto& col : row)
+      for (auto& gx : col)
+        if (gx) {
+          (void)hipGraphExecDestroy(gx);
+          gx = nullptr;
+        }
+  return AERO_OK;
 }
 
 }  // namespace
@@ -299,7 +332,7 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
   }
   // Work list: per geometry, block-major / case-minor (the cases of a block share its face data in L2),
   // then spread so that each XCD (workgroup index mod 8) receives a contiguous run.
-  std::vector<int2> list;
+  std::vector<int2>& list = b->h_list;
   for (int gi = 0; gi < n_meshes; ++gi)
     for (int blk = 0; blk < meshes[gi]->nblk; ++blk)
       for (int i = 0; i < n_cases; ++i)
@@ -312,6 +345,7 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     if (src < W) work[j] = list[src];
   }
   b->nwork = static_cast<int>(work.size());
+  b->listed.assign(n_cases, 1);
   std::vector<DevMesh> gtab(n_meshes);
   for (int gi = 0; gi < n_meshes; ++gi) gtab[gi] = meshes[gi]->dm;
   AERO_BSETUP(balloc(b, &b->d_geoms, gtab.size()));
@@ -351,6 +385,7 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   p.hist_cap = b->hist_cap;
   const int startup = std::max(c.startup_first_order_iters, 0);
   const int sync_every = std::max(iters_per_sync, 1);
+  const bool compact = true;  // relaunch over the active cases only, whenever the active set changes
   auto enqueue = [&](bool second, int start) -> int {
     int rc = enqueue_iteration(b, second, start, b->frozen, err, errlen);
     if (rc) return rc;
@@ -375,11 +410,18 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   {
     std::vector<CaseCtl> h0(b->ncases);
     AERO_TRY(hipMemcpy(h0.data(), b->d_ctl, h0.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost), "control readback");
-    for (auto& k : h0) {
+    std::vector<char> act(b->ncases, 0);
+    for (int i = 0; i < b->ncases; ++i) {
+      CaseCtl& k = h0[i];
       done_iters = std::max(done_iters, k.iter);
       if (!k.converged && !k.failed) k.active = 1;
+      act[i] = k.active ? 1 : 0;
     }
     AERO_TRY(hipMemcpy(b->d_ctl, h0.data(), h0.size() * sizeof(CaseCtl), hipMemcpyHostToDevice), "control resume");
+    if (act != b->listed) {
+      const int rc = upload_work_list(b, act, err, errlen);
+      if (rc) return rc;
+    }
   }
   p.max_iters = done_iters + max_iters;
   std::vector<CaseCtl> h(b->ncases);
@@ -426,9 +468,17 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
     AERO_TRY(hipMemcpyAsync(h.data(), b->d_ctl, h.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost, b->stream),
              "control readback");
     AERO_TRY(hipStreamSynchronize(b->stream), "aero_hip_batch_run: hipStreamSynchronize");
+    std::vector<char> act(b->ncases, 0);
     bool any = false;
-    for (const auto& k : h) any = any || k.active;
+    for (int i = 0; i < b->ncases; ++i) {
+      act[i] = h[i].active ? 1 : 0;
+      any = any || h[i].active;
+    }
     if (!any) break;
+    if (compact && act != b->listed) {  // drop the stopped cases from the launches
+      const int rc = upload_work_list(b, act, err, errlen);
+      if (rc) return rc;
+    }
   }
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int i = 0; i < b->ncases; ++i) {

@@ -91,10 +91,16 @@ def test_converged_cases_are_masked(hip):
         n = c0["iterations"]
         assert h[n - 1, 1] / h[0, 1] <= 1e-2 < h[n - 2, 1] / h[0, 1]
         assert c1["iterations"] > n or c1["converged"]
+        s1, n1 = b.state(1), c1["iterations"]
     case = hip.CaseConfig(**dict(first_order, iterations=n, mach=0.3, aoa_deg=0.0))
     with hip.HipMesh(m) as hm:
         ref = hm.run_case(case)
     assert np.array_equal(s0, ref["conserved"]), "a masked case must keep its state from the iteration it converged"
+    # the case still running after the other stopped (launches then cover its blocks only) is unaffected
+    case1 = hip.CaseConfig(**dict(first_order, iterations=n1, mach=0.6, aoa_deg=3.0))
+    with hip.HipMesh(m) as hm:
+        ref1 = hm.run_case(case1)
+    assert np.array_equal(s1, ref1["conserved"]), "the continuing case must equal its solo run"
     g.close()
 
 
