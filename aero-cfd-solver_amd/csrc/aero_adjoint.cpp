@@ -1132,18 +1132,35 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
     AERO_TRY(hipStreamSynchronize(r.st), "rhs upload");
     hc[i] = AdjBatchCtl{1, 0, 0, 0, 0.0, 0.0};
   }
-  // Work list: per handle, block-major / solve-minor (the solves of a block read the same face
-  // Jacobians), then spread so that each XCD (workgroup index mod 8) receives a contiguous run.
-  std::vector<int2> list;
+  // Work items: solves on one handle with the same CFL go in pairs (a, b), the rest alone (a, -1).  Per
+  // handle, block-major / item-minor (the items of a block read the same face Jacobians), then spread so
+  // that each XCD (workgroup index mod 8) receives a contiguous run.
+  std::vector<int4> list;
   for (int i = 0; i < n; ++i) {
     if (first[i] >= 0) continue;
+    std::vector<int> mine;
+    for (int j = i; j < n; ++j)
+      if (cases[j].mesh == cases[i].mesh) mine.push_back(j);
+    std::vector<int2> items;
+    std::vector<char> used(mine.size(), 0);
+    for (size_t a = 0; a < mine.size(); ++a) {
+      if (used[a]) continue;
+      used[a] = 1;
+      int partner = -1;
+      for (size_t b = a + 1; b < mine.size(); ++b)
+        if (!used[b] && hs[mine[b]].cfl == hs[mine[a]].cfl) {
+          partner = mine[b];
+          used[b] = 1;
+          break;
+        }
+      items.push_back(make_int2(mine[a], partner));
+    }
     for (int blk = 0; blk < hs[i].nparts; ++blk)
-      for (int j = i; j < n; ++j)
-        if (cases[j].mesh == cases[i].mesh) list.push_back(make_int2(j, blk));
+      for (const int2& it : items) list.push_back(make_int4(it.x, it.y, blk, 0));
   }
   const int W = static_cast<int>(list.size());
   const int per = (W + 7) / 8;
-  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
+  std::vector<int4> work(static_cast<size_t>(per) * 8, make_int4(-1, -1, -1, 0));
   for (int j = 0; j < static_cast<int>(work.size()); ++j) {
     const int src = (j & 7) * per + (j >> 3);
     if (src < W) work[j] = list[src];
@@ -1151,13 +1168,13 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
   const int nwork = static_cast<int>(work.size());
   AERO_TRY(hipMalloc(&r.tab, n * sizeof(AdjBatchSolve)), "adjoint batch table");
   AERO_TRY(hipMalloc(&r.ctl, n * sizeof(AdjBatchCtl)), "adjoint batch control");
-  AERO_TRY(hipMalloc(&r.work, work.size() * sizeof(int2)), "adjoint batch work list");
+  AERO_TRY(hipMalloc(&r.work, work.size() * sizeof(int4)), "adjoint batch work list");
   auto* d_tab = static_cast<AdjBatchSolve*>(r.tab);
   auto* d_ctl = static_cast<AdjBatchCtl*>(r.ctl);
-  auto* d_work = static_cast<int2*>(r.work);
+  auto* d_work = static_cast<int4*>(r.work);
   AERO_TRY(hipMemcpy(d_tab, hs.data(), n * sizeof(AdjBatchSolve), hipMemcpyHostToDevice), "adjoint batch table");
   AERO_TRY(hipMemcpy(d_ctl, hc.data(), n * sizeof(AdjBatchCtl), hipMemcpyHostToDevice), "adjoint batch control");
-  AERO_TRY(hipMemcpy(d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice), "adjoint batch work");
+  AERO_TRY(hipMemcpy(d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice), "adjoint batch work");
   // One replay unit: an iteration under RK3 (its ring ends where it starts), two under RK1.
   const int grad = cases[0].mesh->dm.grad;
   const int unit = rk3 ? 1 : 2;

@@ -476,17 +476,12 @@ __device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, dou
 // gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
 // x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
 // and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
+// Wbar of cell c from the gradient adjoint of its faces' cells (k_adj_grad's gather).
 template <int GK>
-__device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const double* __restrict__ U,
-                                                       const FaceOpts& o, const double* __restrict__ Gbar,
-                                                       const double* __restrict__ direct, int stage, int rk3,
-                                                       const double* __restrict__ x0, const double* __restrict__ xs,
-                                                       const double* __restrict__ rhs, const double* __restrict__ lam,
-                                                       double cfl, double* __restrict__ xout,
-                                                       double* __restrict__ psi, int c) {
+__device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __restrict__ Gbar, int c, double* wb) {
   const int ld = am.ld;
-  double sq = 0.0;
-  double wb[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wb[k] = 0.0;
   const int e1 = am.cf_off[c + 1];
   for (int e = am.cf_off[c]; e < e1; ++e) {
     const int ent = am.cf_ent[e];
@@ -525,8 +520,11 @@ __device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const 
       }
     }
   }
-  const C4 u0 = load_c4(U, ld, c);
-  double r[4];
+}
+
+// d(rho, u, v, p)/dU_j of the primitive map at u0 (euler_flux.cpp:115-127), by four dual evaluations:
+// jac[j] = (drho, du, dv, dp) / dU_j.
+__device__ __forceinline__ void c2p_jacobian(const C4& u0, double g, double (*jac)[4]) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     C4T<Dual> sdu;
@@ -534,8 +532,26 @@ __device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const 
     sdu.ru = Dual(u0.ru, j == 1 ? 1.0 : 0.0);
     sdu.rv = Dual(u0.rv, j == 2 ? 1.0 : 0.0);
     sdu.E = Dual(u0.E, j == 3 ? 1.0 : 0.0);
-    const P4T<Dual> w = c2p(sdu, o.g);
-    const double add = w.r.d * wb[0] + w.u.d * wb[1] + w.v.d * wb[2] + w.p.d * wb[3];
+    const P4T<Dual> w = c2p(sdu, g);
+    jac[j][0] = w.r.d;
+    jac[j][1] = w.u.d;
+    jac[j][2] = w.v.d;
+    jac[j][3] = w.p.d;
+  }
+}
+
+// r = direct + (dW/dU)^T Wbar, the update x* = x_s - (r - rhs) with the RK stage combination, and
+// psi = P^T D x* for the next stage; returns the squared residual |r - rhs|^2 of the cell.
+__device__ __forceinline__ double adj_update(int ld, const C4& u0, const FaceOpts& o, const double (*jac)[4],
+                                             const double* wb, const double* __restrict__ direct, int stage, int rk3,
+                                             const double* __restrict__ x0, const double* __restrict__ xs,
+                                             const double* __restrict__ rhs, const double* __restrict__ lam,
+                                             double cfl, double* __restrict__ xout, double* __restrict__ psi, int c) {
+  double sq = 0.0;
+  double r[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double add = jac[j][0] * wb[0] + jac[j][1] * wb[1] + jac[j][2] * wb[2] + jac[j][3] * wb[3];
     r[j] = direct[j * ld + c] + add;
   }
   double x[4];
@@ -567,6 +583,22 @@ __device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const 
   return sq;
 }
 
+template <int GK>
+__device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const double* __restrict__ U,
+                                                       const FaceOpts& o, const double* __restrict__ Gbar,
+                                                       const double* __restrict__ direct, int stage, int rk3,
+                                                       const double* __restrict__ x0, const double* __restrict__ xs,
+                                                       const double* __restrict__ rhs, const double* __restrict__ lam,
+                                                       double cfl, double* __restrict__ xout,
+                                                       double* __restrict__ psi, int c) {
+  double wb[4];
+  adj_wbar<GK>(am, Gbar, c, wb);
+  const C4 u0 = load_c4(U, am.ld, c);
+  double jac[4][4];
+  c2p_jacobian(u0, o.g, jac);
+  return adj_update(am.ld, u0, o, jac, wb, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, c);
+}
+
 // Fixed-order workgroup sum of one value per thread into *dst (thread 0 writes; dst may be null).
 __device__ __forceinline__ void block_sum_store(double v, double* red, double* dst) {
   red[threadIdx.x] = v;
@@ -596,35 +628,124 @@ __global__ __launch_bounds__(kThreads) void k_adj_grad_update(AdjMesh am, const 
   block_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
 }
 
-// ---- batched adjoint solves: one workgroup per (solve, block of kThreads cells) ---------------------
-// The same per-cell arithmetic as the single-solve kernels, so every solve of a batch reproduces its
-// single solve bit for bit; work items of solves that have stopped return at once.
+// ---- batched adjoint solves: one workgroup per (solve a, solve b or -1, block of kThreads cells) ------
+// Two solves of one linearisation (same face Jacobians, state, options and CFL; e.g. the CL and CD
+// adjoints of a case) share a work item: one pass over the face Jacobians and one primitive-map Jacobian
+// per cell serve both.  Each solve runs the same per-cell arithmetic as a single solve, so every solve
+// of a batch reproduces its single solve bit for bit; solves that have stopped are neither computed nor
+// written.
 __global__ __launch_bounds__(kThreads) void k_adj_gather_batch(const AdjBatchSolve* __restrict__ solves,
                                                                const AdjBatchCtl* __restrict__ ctl,
-                                                               const int2* __restrict__ work) {
-  const int2 w = work[blockIdx.x];
-  if (w.x < 0 || !ctl[w.x].active) return;
-  const AdjBatchSolve& s = solves[w.x];
-  const int c = w.y * kThreads + threadIdx.x;
-  if (c >= s.am.N) return;
-  adj_gather_cell(s.am, s.J, s.psi, s.direct, s.Gbar, c);
+                                                               const int4* __restrict__ work) {
+  const int4 w = work[blockIdx.x];
+  if (w.x < 0) return;
+  const bool aa = ctl[w.x].active != 0;
+  const bool ab = w.y >= 0 && ctl[w.y].active != 0;
+  if (!aa && !ab) return;
+  const AdjBatchSolve& sa = solves[aa ? w.x : w.y];
+  const int c = w.z * kThreads + threadIdx.x;
+  const AdjMesh& am = sa.am;
+  if (c >= am.N) return;
+  if (!(aa && ab)) {  // one running solve: the single-solve body
+    adj_gather_cell(am, sa.J, sa.psi, sa.direct, sa.Gbar, c);
+    return;
+  }
+  const AdjBatchSolve& sb = solves[w.y];
+  const int ld = am.ld;
+  const double* __restrict__ pa = sa.psi;
+  const double* __restrict__ pb = sb.psi;
+  double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
+  double ga[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int e1 = am.cf_off[c + 1];
+  for (int e = am.cf_off[c]; e < e1; ++e) {
+    const int ent = am.cf_ent[e];
+    const int f = ent >> 1;
+    const int side = ent & 1;
+    const int o = am.fo[f], n = am.fn[f];
+    double wa[4], wv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      wa[k] = pa[k * ld + o] - (n >= 0 ? pa[k * ld + n] : 0.0);
+      wv[k] = pb[k * ld + o] - (n >= 0 ? pb[k * ld + n] : 0.0);
+    }
+    const double* Jf = sa.J + static_cast<size_t>(f) * 96;
+    const double* Ju = Jf + (side ? 4 : 0) * 4;
+    const double* Jg = Jf + (side ? 16 : 8) * 4;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      double ta = 0.0, tb = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double j = Ju[d * 4 + k];
+        ta += j * wa[k];
+        tb += j * wv[k];
+      }
+      ua[d] += ta;
+      ub[d] += tb;
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      double ta = 0.0, tb = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double j = Jg[d * 4 + k];
+        ta += j * wa[k];
+        tb += j * wv[k];
+      }
+      ga[d] += ta;
+      gb[d] += tb;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sa.direct[k * ld + c] = ua[k];
+    sb.direct[k * ld + c] = ub[k];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    sa.Gbar[j * ld + c] = ga[j];
+    sb.Gbar[j * ld + c] = gb[j];
+  }
 }
 
 template <int GK>
 __global__ __launch_bounds__(kThreads) void k_adj_grad_update_batch(const AdjBatchSolve* __restrict__ solves,
                                                                     const AdjBatchCtl* __restrict__ ctl,
-                                                                    const int2* __restrict__ work, int stage, int rk3,
+                                                                    const int4* __restrict__ work, int stage, int rk3,
                                                                     int ix0, int ixs, int ixo, int want_partials) {
   __shared__ double red[kThreads];
-  const int2 w = work[blockIdx.x];
-  if (w.x < 0 || !ctl[w.x].active) return;  // uniform over the workgroup
-  const AdjBatchSolve& s = solves[w.x];
-  const int c = w.y * kThreads + threadIdx.x;
-  double sq = 0.0;
-  if (c < s.am.N)
-    sq = adj_grad_update_cell<GK>(s.am, s.U, s.o, s.Gbar, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam,
-                                  s.cfl, s.x[ixo], s.psi, c);
-  block_sum_store(sq, red, want_partials ? s.partials + w.y : nullptr);
+  const int4 w = work[blockIdx.x];
+  if (w.x < 0) return;
+  const bool aa = ctl[w.x].active != 0;
+  const bool ab = w.y >= 0 && ctl[w.y].active != 0;
+  if (!aa && !ab) return;  // uniform over the workgroup
+  const AdjBatchSolve& s0 = solves[aa ? w.x : w.y];
+  const AdjMesh& am = s0.am;
+  const int c = w.z * kThreads + threadIdx.x;
+  double sqa = 0.0, sqb = 0.0;
+  if (c < am.N) {
+    const C4 u0 = load_c4(s0.U, am.ld, c);
+    double jac[4][4];
+    c2p_jacobian(u0, s0.o.g, jac);
+    double wb[4];
+    if (aa) {
+      const AdjBatchSolve& s = solves[w.x];
+      adj_wbar<GK>(am, s.Gbar, c, wb);
+      sqa = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
+                       s.x[ixo], s.psi, c);
+    }
+    if (ab) {
+      const AdjBatchSolve& s = solves[w.y];
+      adj_wbar<GK>(am, s.Gbar, c, wb);
+      sqb = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
+                       s.x[ixo], s.psi, c);
+    }
+  }
+  if (want_partials) {
+    if (aa) block_sum_store(sqa, red, solves[w.x].partials + w.z);
+    if (aa && ab) __syncthreads();
+    if (ab) block_sum_store(sqb, red, solves[w.y].partials + w.z);
+  }
 }
 
 // Per solve (one workgroup each): the squared norm of the iteration's stage-0 linear residual from the
@@ -946,7 +1067,7 @@ hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U,
   return hipGetLastError();
 }
 
-hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int2* work,
+hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
                                   int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
                                   hipStream_t s) {
   hipLaunchKernelGGL(k_adj_gather_batch, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work);

@@ -284,9 +284,10 @@ def _first_order_state(hip, hm, mach, aoa, iters, **kw):
 
 @pytest.mark.parametrize("rk_stages", [3, 1])
 def test_adjoint_batch_matches_single_solves(hip, rk_stages):
-    """Batched adjoint solves (CL and CD on two meshes of different size, one preconditioned) against
-    single solves checked after every iteration: bitwise identical under RK3; under RK1, whose single
-    solve checks every second iteration, equal to the solve tolerance."""
+    """Batched adjoint solves (CL and CD on two meshes of different size, one preconditioned; the two solves
+    of a mesh share work items, a third solve with its own CFL runs alone) against single solves checked
+    after every iteration: bitwise identical under RK3; under RK1, whose single solve checks every second
+    iteration, equal to the solve tolerance."""
     m1 = helpers.mesh("40x8")
     m2 = _mesh64()
     with hip.HipMesh(m1) as h1, hip.HipMesh(m2) as h2:
@@ -303,6 +304,7 @@ def test_adjoint_batch_matches_single_solves(hip, rk_stages):
                 obj.w_cl, obj.w_cd, obj.w_cm = w
                 rhs = hm.objective(obj, U)["dJdU"]
                 solves.append((hm, rc, rhs, U, cfl))
+        solves.append(solves[0][:4] + (1.3,))  # a third solve on the first mesh, alone (its own CFL)
         kw = dict(rk_stages=rk_stages, max_iters=20000, tol=1e-10)
         say(f"[adj-batch] batched solve rk{rk_stages}")
         psis, infos, _ = hip.adjoint_solve_batch(solves, cfl=1.5, iters_per_sync=150, **kw)
