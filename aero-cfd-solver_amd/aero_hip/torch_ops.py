@@ -4,7 +4,10 @@
   can sit inside any torch computation.
 * ``steady_coefficients(params, base_mesh, case, ...)``: (CL, CD) of the converged solution for
   params = [aoa_deg, mach, a_1 .. a_P] (Hicks-Henne amplitudes deforming ``base_mesh``); the backward
-  pass runs one adjoint solve for J = grad_CL * CL + grad_CD * CD and returns dJ/dparams.
+  pass runs one adjoint solve for J = grad_CL * CL + grad_CD * CD and returns dJ/dparams.  The primal is
+  the device driver (``primal="driver"``: aero_hip_run_case with the case's own stopping test) or a
+  one-case device batch to a density-residual drop (``primal="batch"``, ``rho_drop``), which also carries
+  a frozen limiter's ratios over to the adjoint.
 
 Tensors may live on any device; values cross the C ABI as FP64 host arrays.  Only the Python layer uses
 torch: the numerics are the HIP kernels behind ``aero_hip``.
@@ -18,7 +21,10 @@ import torch
 
 from .adjoint import sensitivities
 from .mesh import deform_mesh
-from .solver import CaseConfig, HipMesh
+from .solver import CaseConfig, HipBatch, HipMesh
+
+# Timings of the last SteadyCoefficients forward (primal) and backward (adjoint) on this process.
+last_run: dict = {}
 
 
 def _np(t: torch.Tensor) -> np.ndarray:
@@ -56,19 +62,40 @@ class SteadyCoefficients(torch.autograd.Function):
         if n_shape > 0:
             mesh, tan = deform_mesh(base_mesh, p[2:], bumps_per_surface=n_shape // 2, **shape_kw)
         hm = HipMesh(mesh)
-        out = hm.run_case(case_p)
-        ctx.saved = (hm, case_p, out["conserved"], tan, n_shape, dict(solve_kw))
+        kw = dict(solve_kw)
+        primal = kw.pop("primal", "driver")
+        rho_drop = kw.pop("rho_drop", 1e-8)
+        if primal == "batch":
+            with HipBatch([hm], [(0, case_p.mach, case_p.aoa_deg)], case_p) as b:
+                res = b.run(case_p.iterations, rho_drop, iters_per_sync=500)
+                row = res["cases"][0]
+                last_run.update(primal_seconds=res["seconds"], primal_iterations=row["iterations"],
+                                primal_rho_ratio=row["rho_ratio"], primal_converged=bool(row["converged"]))
+                U = b.state(0)
+                if case_p.limiter_freeze_iter >= 0:
+                    hm.set_limiter_ratios(b.limiter_ratios(0))
+            case_p = replace(case_p, iterations=max(row["iterations"], 1))
+            cl, cd = row["cl"], row["cd"]
+        elif primal == "driver":
+            out = hm.run_case(case_p)
+            U, cl, cd = out["conserved"], out["cl"], out["cd"]
+            last_run.update(primal_seconds=out["seconds"], primal_iterations=out["n_iter"])
+        else:
+            raise ValueError("primal must be 'driver' or 'batch'")
+        ctx.saved = (hm, case_p, U, tan, n_shape, kw)
         ctx.device = params.device
-        return torch.tensor([out["cl"], out["cd"]], dtype=torch.float64, device=params.device)
+        return torch.tensor([cl, cd], dtype=torch.float64, device=params.device)
 
     @staticmethod
     def backward(ctx, grad):
         hm, case_p, U, tan, n_shape, solve_kw = ctx.saved
         g = _np(grad)
         s = sensitivities(hm, case_p, U, weights=(float(g[0]), float(g[1]), 0.0), shape_tangents=tan, **solve_kw)
+        last_run.update(adjoint_seconds=s["adjoint_seconds"], adjoint_iterations=s["adjoint"]["iterations"],
+                        adjoint_res0=s["adjoint"]["res0"], adjoint_res=s["adjoint"]["res"])
         d = [s["d"]["aoa"], s["d"]["mach"]] + [s["d"][f"shape{i}"] for i in range(n_shape)]
-        hm.close()
-        ctx.saved = None
+        # the device mesh stays with ctx (freed with the graph), so a retained graph can be differentiated
+        # again, e.g. once for CL and once for CD
         return torch.tensor(d, dtype=torch.float64, device=ctx.device), None, None, None, None
 
 
@@ -76,5 +103,6 @@ def steady_coefficients(params: torch.Tensor, base_mesh, case: CaseConfig, shape
                         solve_kw: dict | None = None) -> torch.Tensor:
     """(CL, CD) of the converged case at params = [aoa_deg, mach, amplitudes...], differentiable through
     the discrete adjoint.  shape_kw: deform_mesh options (h_min, h_max, idw_exponent); solve_kw:
-    sensitivities options (cfl, tol, max_iters, gmres_restart, ...)."""
+    primal ("driver" or "batch"), rho_drop (batch primal) and the sensitivities options (cfl, tol,
+    max_iters, gmres_restart, ...)."""
     return SteadyCoefficients.apply(params, base_mesh, case, shape_kw or {}, solve_kw or {})

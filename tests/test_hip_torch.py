@@ -78,3 +78,30 @@ def test_steady_coefficients_gradient(hip):
             vals.append(float(cc[0] + 0.5 * cc[1]))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(g[i] - fd) <= 1e-5 * abs(fd), (i, g[i], fd)
+
+
+def test_steady_coefficients_batch_primal_and_two_backwards(hip):
+    """The batch primal of the Function (a one-case device batch to a density drop) against the driver
+    primal, and two backward passes over one retained graph (dCL, then dCD)."""
+    import torch
+    from aero_hip.torch_ops import last_run, steady_coefficients
+    base = helpers.mesh("40x8")
+    case = hip.CaseConfig(flux_scheme=0, limiter=0, cfl_start=0.5, cfl_max=1.5, cfl_ramp_iters=50,
+                          startup_first_order_iters=10 ** 6, residual_reduction_target=0.0, force_stability_tol=0.0,
+                          min_iterations=1, iterations=9000, low_mach_fix=0)
+    p0 = np.concatenate([[2.0, 0.5], np.random.default_rng(5).uniform(-0.002, 0.002, 4)])
+    grads = {}
+    for primal in ("driver", "batch"):
+        params = torch.tensor(p0, requires_grad=True)
+        say(f"[torch] {primal} primal")
+        c = steady_coefficients(params, base, case, solve_kw=dict(primal=primal, rho_drop=1e-11, cfl=1.5,
+                                                                   tol=1e-12, max_iters=20000))
+        if primal == "batch":
+            assert last_run["primal_converged"] and last_run["primal_rho_ratio"] <= 1e-11
+        (g_cl,) = torch.autograd.grad(c[0], params, retain_graph=True)
+        (g_cd,) = torch.autograd.grad(c[1], params)
+        grads[primal] = (c.detach().numpy(), g_cl.numpy(), g_cd.numpy())
+    (cd_, gl_d, gd_d), (cb, gl_b, gd_b) = grads["driver"], grads["batch"]
+    assert np.allclose(cb, cd_, rtol=1e-9)
+    assert np.allclose(gl_b, gl_d, rtol=1e-6, atol=1e-9) and np.allclose(gd_b, gd_d, rtol=1e-6, atol=1e-9)
+    assert not np.allclose(gl_b, gd_b)
