@@ -7,7 +7,8 @@
   pass runs one adjoint solve for J = grad_CL * CL + grad_CD * CD and returns dJ/dparams.  The primal is
   the device driver (``primal="driver"``: aero_hip_run_case with the case's own stopping test) or a
   one-case device batch to a density-residual drop (``primal="batch"``, ``rho_drop``), which also carries
-  a frozen limiter's ratios over to the adjoint.
+  a frozen limiter's ratios over to the adjoint.  With ``jacobian=True`` the first backward solves the CL
+  and CD adjoints as one batched pair and keeps both gradient rows for further backward passes.
 
 Tensors may live on any device; values cross the C ABI as FP64 host arrays.  Only the Python layer uses
 torch: the numerics are the HIP kernels behind ``aero_hip``.
@@ -19,7 +20,7 @@ from dataclasses import replace
 import numpy as np
 import torch
 
-from .adjoint import sensitivities
+from .adjoint import sensitivities, sensitivities_batch
 from .mesh import deform_mesh
 from .solver import CaseConfig, HipBatch, HipMesh
 
@@ -90,10 +91,28 @@ class SteadyCoefficients(torch.autograd.Function):
     def backward(ctx, grad):
         hm, case_p, U, tan, n_shape, solve_kw = ctx.saved
         g = _np(grad)
-        s = sensitivities(hm, case_p, U, weights=(float(g[0]), float(g[1]), 0.0), shape_tangents=tan, **solve_kw)
+        names = ["aoa", "mach"] + [f"shape{i}" for i in range(n_shape)]
+        if solve_kw.get("jacobian", False):
+            # Both rows of d(CL, CD)/dparams from one batched solve of the two adjoints (they share the
+            # face Jacobians), kept on ctx: a later backward over the same graph needs no solve.
+            if getattr(ctx, "jac", None) is None:
+                kw = {k: v for k, v in solve_kw.items() if k in ("cfl", "max_iters", "tol")}
+                sb = sensitivities_batch([dict(hm=hm, case=case_p, U=U, shape_tangents=tan)], **kw)
+                c = sb["cases"][0]
+                ctx.jac = np.array([[c["d"]["cl"][n] for n in names], [c["d"]["cd"][n] for n in names]])
+                last_run.update(adjoint_seconds=sb["solve_seconds"],
+                                adjoint_iterations=max(a["iterations"] for a in c["adjoint"].values()),
+                                adjoint_res0=c["adjoint"]["cl"]["res0"], adjoint_res=c["adjoint"]["cl"]["res"],
+                                adjoint_solves="cl+cd")
+            else:
+                last_run.update(adjoint_seconds=0.0, adjoint_solves="cached")
+            d = (g[0] * ctx.jac[0] + g[1] * ctx.jac[1]).tolist()
+            return torch.tensor(d, dtype=torch.float64, device=ctx.device), None, None, None, None
+        kw = {k: v for k, v in solve_kw.items() if k != "jacobian"}
+        s = sensitivities(hm, case_p, U, weights=(float(g[0]), float(g[1]), 0.0), shape_tangents=tan, **kw)
         last_run.update(adjoint_seconds=s["adjoint_seconds"], adjoint_iterations=s["adjoint"]["iterations"],
-                        adjoint_res0=s["adjoint"]["res0"], adjoint_res=s["adjoint"]["res"])
-        d = [s["d"]["aoa"], s["d"]["mach"]] + [s["d"][f"shape{i}"] for i in range(n_shape)]
+                        adjoint_res0=s["adjoint"]["res0"], adjoint_res=s["adjoint"]["res"], adjoint_solves="weighted")
+        d = [s["d"][n] for n in names]
         # the device mesh stays with ctx (freed with the graph), so a retained graph can be differentiated
         # again, e.g. once for CL and once for CD
         return torch.tensor(d, dtype=torch.float64, device=ctx.device), None, None, None, None
@@ -103,6 +122,7 @@ def steady_coefficients(params: torch.Tensor, base_mesh, case: CaseConfig, shape
                         solve_kw: dict | None = None) -> torch.Tensor:
     """(CL, CD) of the converged case at params = [aoa_deg, mach, amplitudes...], differentiable through
     the discrete adjoint.  shape_kw: deform_mesh options (h_min, h_max, idw_exponent); solve_kw:
-    primal ("driver" or "batch"), rho_drop (batch primal) and the sensitivities options (cfl, tol,
+    primal ("driver" or "batch"), rho_drop (batch primal), jacobian (True: the first backward solves the
+    CL and CD adjoints together and later backwards reuse them) and the sensitivities options (cfl, tol,
     max_iters, gmres_restart, ...)."""
     return SteadyCoefficients.apply(params, base_mesh, case, shape_kw or {}, solve_kw or {})

@@ -91,17 +91,21 @@ def test_steady_coefficients_batch_primal_and_two_backwards(hip):
                           min_iterations=1, iterations=9000, low_mach_fix=0)
     p0 = np.concatenate([[2.0, 0.5], np.random.default_rng(5).uniform(-0.002, 0.002, 4)])
     grads = {}
-    for primal in ("driver", "batch"):
+    for primal, jac in (("driver", False), ("batch", False), ("batch", True)):
         params = torch.tensor(p0, requires_grad=True)
-        say(f"[torch] {primal} primal")
+        say(f"[torch] {primal} primal, jacobian={jac}")
         c = steady_coefficients(params, base, case, solve_kw=dict(primal=primal, rho_drop=1e-11, cfl=1.5,
-                                                                   tol=1e-12, max_iters=20000))
+                                                                   tol=1e-12, max_iters=20000, jacobian=jac))
         if primal == "batch":
             assert last_run["primal_converged"] and last_run["primal_rho_ratio"] <= 1e-11
         (g_cl,) = torch.autograd.grad(c[0], params, retain_graph=True)
         (g_cd,) = torch.autograd.grad(c[1], params)
-        grads[primal] = (c.detach().numpy(), g_cl.numpy(), g_cd.numpy())
-    (cd_, gl_d, gd_d), (cb, gl_b, gd_b) = grads["driver"], grads["batch"]
+        if jac:
+            assert last_run["adjoint_solves"] == "cached"  # the second row came from the first solve
+        grads[(primal, jac)] = (c.detach().numpy(), g_cl.numpy(), g_cd.numpy())
+    (cd_, gl_d, gd_d), (cb, gl_b, gd_b) = grads[("driver", False)], grads[("batch", False)]
     assert np.allclose(cb, cd_, rtol=1e-9)
     assert np.allclose(gl_b, gl_d, rtol=1e-6, atol=1e-9) and np.allclose(gd_b, gd_d, rtol=1e-6, atol=1e-9)
     assert not np.allclose(gl_b, gd_b)
+    _, gl_j, gd_j = grads[("batch", True)]
+    assert np.allclose(gl_j, gl_b, rtol=1e-8, atol=1e-11) and np.allclose(gd_j, gd_b, rtol=1e-8, atol=1e-11)
