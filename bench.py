@@ -417,7 +417,7 @@ def run_adjoint(args) -> int:
     amp = np.random.default_rng(4242).uniform(-0.002, 0.002, 20)
     mesh, tan = deform_mesh(base, amp)
     cap = args.sweep_iters
-    case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.sweep_cfl,
+    case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.adjoint_cfl,
                       cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
                       stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
                       residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
@@ -474,7 +474,7 @@ def run_adjoint(args) -> int:
                 cp = replace(done, **{("aoa_deg" if pname == "aoa" else "mach"): getattr(
                     done, "aoa_deg" if pname == "aoa" else "mach") + sgn * h})
                 rcp, _, objp, _ = case_linearization(cp)
-                r0, r1 = _frozen_continuation(hm, rcp, U, args.fd_iters, case.cfl_max)
+                r0, r1 = _frozen_continuation(hm, rcp, U, args.fd_iters, args.fd_cfl)
                 cls.append(hm.objective(replace_obj(objp, 1.0), None, gradient=False)["J"])
                 ok = r1 <= 1e-7 * r0
             else:
@@ -483,7 +483,7 @@ def run_adjoint(args) -> int:
                 mi, _ = deform_mesh(base, amp + e, tangents=False)
                 with HipMesh(mi, block_cells=args.block_cells) as hmi:
                     hmi.set_limiter_ratios(ratios)
-                    r0, r1 = _frozen_continuation(hmi, rc, U, args.fd_iters, case.cfl_max)
+                    r0, r1 = _frozen_continuation(hmi, rc, U, args.fd_iters, args.fd_cfl)
                     cls.append(hmi.objective(replace_obj(obj, 1.0), None, gradient=False)["J"])
                 ok = r1 <= 1e-7 * r0
             checks.setdefault(pname, {})[f"fd_converged_{sgn:+d}"] = bool(ok)
@@ -499,7 +499,7 @@ def run_adjoint(args) -> int:
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "cells": mesh.num_cells, "mesh": mkw,
                       "bumps": 20, "amplitude_range": 0.002, "seed": 4242, "mach": 0.6, "aoa_deg": 2.0,
-                      "cfl_max": args.sweep_cfl, "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
+                      "cfl_max": args.adjoint_cfl, "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
                       "adjoint_tol": args.adjoint_tol},
            "adjoint": {"gradients": "torch.autograd.grad through aero_hip.torch_ops.steady_coefficients",
                        "primal_seconds": round(t_primal, 3), "primal_iterations": t_iters,
@@ -536,7 +536,9 @@ def main() -> int:
     ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
     ap.add_argument("--sweep-iters", type=int, default=60000, help="iteration cap per case in --sweep")
     ap.add_argument("--sweep-seq-cases", type=int, default=4, help="cases timed sequentially in --sweep")
-    ap.add_argument("--sweep-cfl", type=float, default=2.0, help="CFL after the ramp in --sweep")
+    ap.add_argument("--sweep-cfl", type=float, default=2.4,
+                    help="CFL after the ramp in --sweep / --uneven (2.4: the most cases converged of 2.0, 2.4, "
+                         "2.8 and 3.2 on the sweep)")
     ap.add_argument("--freeze-iter", type=int, default=1000,
                     help="--sweep / --adjoint: iteration at which the limiter is frozen (-1: never)")
     ap.add_argument("--adjoint", action="store_true", help="run the discrete-adjoint workload instead")
@@ -549,7 +551,12 @@ def main() -> int:
     ap.add_argument("--uneven-lowmach", choices=["precond", "plain"], default="precond",
                     help="--uneven: M 0.05 cases with the low-Mach preconditioner (its long runs diverge) or without")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="--adjoint: linear residual drop")
-    ap.add_argument("--fd-iters", type=int, default=60000, help="--adjoint: continuation iterations per FD run")
+    ap.add_argument("--adjoint-cfl", type=float, default=2.0,
+                    help="--adjoint: CFL of the primal and the adjoint (at 2.4 the AoA / Mach finite-difference "
+                         "continuations no longer converge to 1e-7)")
+    ap.add_argument("--fd-iters", type=int, default=120000, help="--adjoint: continuation iterations per FD run")
+    ap.add_argument("--fd-cfl", type=float, default=2.0,
+                    help="--adjoint: fixed CFL of the FD continuation runs (they have no CFL adaptation)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
