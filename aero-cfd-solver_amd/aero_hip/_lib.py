@@ -130,6 +130,11 @@ class LinearSolve(C.Structure):
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32)
 
 
+class FreestreamRef(C.Structure):
+    _fields_ = [("rho_inf", f64), ("p_inf", f64), ("aoa_deg", f64), ("speed_inf", f64), ("chord", f64),
+                ("x_ref", f64), ("y_ref", f64)]
+
+
 class AdjointCaseC(C.Structure):
     _fields_ = [("mesh", C.c_void_p), ("cfg", C.c_void_p), ("conserved", C.c_void_p), ("rhs", C.c_void_p),
                 ("psi", C.c_void_p), ("cfl", f64), ("iterations", i32), ("converged", i32), ("failed", i32),
@@ -192,6 +197,7 @@ PROTOTYPES = {
                                          C.c_char_p, C.c_size_t]),
     "aero_hip_tangent_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, C.POINTER(TangentSeed),
                                          C.POINTER(LinearSolve), PD, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_forces": (C.c_int, [vp, PD, C.POINTER(FreestreamRef), i32, f64, PD, PD, C.c_char_p, C.c_size_t]),
     "aero_hip_adjoint_solve_batch": (C.c_int, [C.POINTER(AdjointCaseC), i32, C.POINTER(LinearSolve), i32,
                                                PROGRESS_FN, vp, C.c_char_p, C.c_size_t]),
 }

@@ -246,6 +246,18 @@ class HipMesh:
                     wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)
 
 
+    def forces(self, rho_inf: float, p_inf: float, aoa_deg: float, speed_inf: float, chord: float = 1.0,
+               x_ref: float = 0.25, y_ref: float = 0.0, gauge: bool = True, gamma: float = 1.4, U=None) -> dict:
+        """Wall pressure forces and Cp at U (None: the handle's state), integrate_pressure_forces /
+        extract_wall_cp (aero_hip_forces)."""
+        _, pu = self._state_ptr(U)
+        ref = _lib.FreestreamRef(rho_inf, p_inf, aoa_deg, speed_inf, chord, x_ref, y_ref)
+        f = np.zeros(8)
+        cp = np.zeros(max(self.mesh.wall_count, 1))
+        self._call("aero_hip_forces", pu, C.byref(ref), int(bool(gauge)), gamma, _pd(f), _pd(cp))
+        return dict(fx=f[0], fy=f[1], lift=f[2], drag=f[3], moment=f[4], cl=f[5], cd=f[6], cm=f[7],
+                    wall_cp=cp[: self.mesh.wall_count])
+
     def freeze_limiter(self, cfg: _lib.ResidualConfig, U=None) -> np.ndarray:
         """Capture the limiter ratios at U (None: the handle's state); residual configs with
         limiter_frozen = 1 use them.  Returns the [8, F] ratios."""

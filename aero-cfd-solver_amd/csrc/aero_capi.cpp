@@ -641,6 +641,65 @@ int aero_hip_get_state(aero_hip_mesh* m, double* conserved, char* err, size_t er
   return AERO_OK;
 }
 
+int aero_hip_forces(aero_hip_mesh* m, const double* conserved, const aero_freestream_ref* ref, int32_t gauge,
+                    double gamma, double* forces, double* wall_cp, char* err, size_t errlen) {
+  if (!m || !ref || !forces) {
+    set_err(err, errlen, "aero_hip_forces: null argument");
+    return AERO_EINVAL;
+  }
+  if (!(gamma > 1.0)) {
+    set_err(err, errlen, "aero_hip_forces: gamma must be > 1");
+    return AERO_EINVAL;
+  }
+  std::vector<double> own;
+  const double* U = conserved;
+  if (!U) {
+    own.resize(5 * static_cast<size_t>(m->N));
+    const int rc = aero_hip_get_state(m, own.data(), err, errlen);
+    if (rc) return rc;
+    U = own.data();
+  }
+  // owner-cell pressure of conservative_to_primitive (euler_flux.cpp:115-127)
+  auto pressure = [&](int cell) {
+    const double* u = U + 5 * static_cast<size_t>(cell);
+    const double rho = u[0] < 1.0e-8 ? 1.0e-8 : u[0];
+    const double inv = 1.0 / rho;
+    const double vx = u[1] * inv, vy = u[2] * inv;
+    const double ke = 0.5 * rho * (vx * vx + vy * vy);
+    const double p = (gamma - 1.0) * (u[4] - ke);
+    return p < 1.0e-8 ? 1.0e-8 : p;
+  };
+  // integrate_pressure_forces (forces.cpp:250-294) over the wall patch in face order; nA = -n A
+  const double q_inf = 0.5 * ref->rho_inf * ref->speed_inf * ref->speed_inf + 1.0e-12;
+  double fx = 0.0, fy = 0.0, mom = 0.0;
+  for (int i = 0; i < m->wall_count; ++i) {
+    const int f = m->wall_start + i;
+    const double p = pressure(m->h_owner[f]);
+    const double pp = gauge ? p - ref->p_inf : p;
+    const double nax = -m->h_nx[f] * m->h_area[f], nay = -m->h_ny[f] * m->h_area[f];
+    fx += -pp * nax;
+    fy += -pp * nay;
+    const double dfx = -pp * nax, dfy = -pp * nay;
+    mom += (m->h_fx[f] - ref->x_ref) * dfy - (m->h_fy[f] - ref->y_ref) * dfx;
+    if (wall_cp) wall_cp[i] = (p - ref->p_inf) / q_inf;  // extract_wall_cp (forces.cpp:177-206)
+  }
+  const double pi = 3.14159265358979323846;
+  const double a = ref->aoa_deg * (pi / 180.0);
+  const double ca = std::cos(a), sa = std::sin(a);
+  const double chord = std::max(ref->chord, 1.0e-8);
+  const double s_ref = chord * 1.0;
+  const double drag = fx * ca + fy * sa, lift = -fx * sa + fy * ca;
+  forces[0] = fx;
+  forces[1] = fy;
+  forces[2] = lift;
+  forces[3] = drag;
+  forces[4] = mom;
+  forces[5] = lift / (q_inf * s_ref);
+  forces[6] = drag / (q_inf * s_ref);
+  forces[7] = mom / (q_inf * s_ref * chord);
+  return AERO_OK;
+}
+
 int aero_hip_get_gradients(aero_hip_mesh* m, double* gradients, char* err, size_t errlen) {
   if (!m || !gradients) {
     set_err(err, errlen, "aero_hip_get_gradients: null argument");

@@ -207,6 +207,19 @@ int aero_hip_set_limiter_ratios(aero_hip_mesh* mesh, const double* ratios, char*
 int aero_hip_iterate(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t n_iters, int32_t rk_stages,
                      double cfl, double* rho_l2, char* err, size_t errlen);
 
+/* FreestreamReference (post/forces.hpp:9-17). */
+typedef struct aero_freestream_ref {
+  double rho_inf, p_inf, aoa_deg, speed_inf, chord, x_ref, y_ref;
+} aero_freestream_ref;
+
+/* Pressure forces on the wall patch from the owner cells' pressure, integrate_pressure_forces
+ * (post/forces.cpp:250-294; gauge = 1 subtracts p_inf, as the driver's coefficients do), summed over the
+ * wall faces in face order: forces[8] = fx, fy, lift, drag, moment, cl, cd, cm.  wall_cp (may be NULL):
+ * [wall_count] Cp per wall face in wall-patch order (extract_wall_cp, forces.cpp:177-206, which then
+ * orders the faces along the surface).  conserved: [N][5] or NULL for the handle's state. */
+int aero_hip_forces(aero_hip_mesh* mesh, const double* conserved, const aero_freestream_ref* ref, int32_t gauge,
+                    double gamma, double* forces, double* wall_cp, char* err, size_t errlen);
+
 /* The full pseudo-time run of run_euler_airfoil_case (euler_solver.cpp:463-1208) with the stage
  * loop, norms and forces on the device and the ramp / CFL / convergence control on the host. */
 int aero_hip_run_case(aero_hip_mesh* mesh, const aero_case_config* cfg, aero_run_output* out, char* err,

@@ -92,3 +92,33 @@ def test_sweep_runs_each_aoa(hip, tmp_path):
     assert (tmp_path / "batched" / "run_000" / "cp_wall.csv").exists()
     # same arithmetic as the serial runs until the serial driver's own stopping test: 60 iterations both
     assert rb["runs"][1]["cl"] == pytest.approx(cls[1], rel=1e-12)
+
+
+def test_forces_match_oracle(hip):
+    """aero_hip_forces (gauge and absolute) against the FP64 oracle's integrate_pressure_forces on the owner
+    pressures of the same state, and equal to the driver's final coefficients."""
+    import ctypes as C
+    import oracle_lib as ol
+    m = helpers.mesh("40x8")
+    om = ol.OracleMesh(m, np.float64)
+    case = hip.CaseConfig(mach=0.5, aoa_deg=1.25, flux_scheme=1, limiter=1, iterations=40, min_iterations=1,
+                          residual_reduction_target=0.0)
+    with hip.HipMesh(m) as hm:
+        out = hm.run_case(case)
+        U = out["conserved"]
+        rho_inf = case.p_inf / (case.gas_constant * case.t_inf)
+        V = case.mach * np.sqrt(case.gamma * case.p_inf / rho_inf)
+        for gauge in (True, False):
+            f = hm.forces(rho_inf, case.p_inf, case.aoa_deg, V, gauge=gauge)
+            rho = np.maximum(U[:, 0], 1e-8)
+            inv = 1.0 / rho
+            u, v = U[:, 1] * inv, U[:, 2] * inv
+            p = np.maximum((1.4 - 1.0) * (U[:, 4] - 0.5 * rho * (u * u + v * v)), 1e-8)
+            ref7 = np.array([rho_inf, case.p_inf, case.aoa_deg, V, 1.0, 0.25, 0.0])
+            o = np.zeros(8)
+            ol.oracle().ao_forces_f64(C.byref(om.c), ol.ptr(p), ol.ptr(ref7), int(gauge), ol.ptr(o))
+            got = np.array([f[k] for k in ("fx", "fy", "lift", "drag", "moment", "cl", "cd", "cm")])
+            assert np.allclose(got, o, rtol=1e-13, atol=1e-9 * np.abs(o).max()), (gauge, got, o)
+            if gauge:
+                assert f["cl"] == pytest.approx(out["cl"], rel=1e-12)
+                assert np.allclose(f["wall_cp"], out["wall_cp"], rtol=1e-12, atol=1e-14)
