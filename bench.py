@@ -436,7 +436,8 @@ def run_adjoint(args) -> int:
     import torch
     from aero_hip.torch_ops import last_run, steady_coefficients
     params = torch.tensor(np.concatenate([[case.aoa_deg, case.mach], amp]), requires_grad=True)
-    solve_kw = dict(primal="batch", rho_drop=1e-8, tol=args.adjoint_tol, max_iters=4 * cap, jacobian=True)
+    solve_kw = dict(primal="batch", rho_drop=1e-8, tol=args.adjoint_tol, max_iters=4 * cap, jacobian=True,
+                    cfl=args.adjoint_solve_cfl)
     log("[adjoint] torch Function forward (primal)")
     coef = steady_coefficients(params, base, case, solve_kw=solve_kw)
     t_primal, t_iters = last_run["primal_seconds"], last_run["primal_iterations"]
@@ -464,7 +465,7 @@ def run_adjoint(args) -> int:
             seed = geometry_seed(tan, i)
             ob = hm.objective(replace_obj(obj, 1.0), U, seed=seed)
             j_p = ob["dJ_seed"]
-        v, ti = hm.tangent_solve(rc, seed, U, cfl=case.cfl_max, max_iters=4 * cap, tol=args.adjoint_tol,
+        v, ti = hm.tangent_solve(rc, seed, U, cfl=args.adjoint_solve_cfl, max_iters=4 * cap, tol=args.adjoint_tol,
                                  check_every=500)
         d_tan = j_p + float(np.sum(ob["dJdU"] * v))
         cls = []
@@ -499,7 +500,8 @@ def run_adjoint(args) -> int:
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "cells": mesh.num_cells, "mesh": mkw,
                       "bumps": 20, "amplitude_range": 0.002, "seed": 4242, "mach": 0.6, "aoa_deg": 2.0,
-                      "cfl_max": args.adjoint_cfl, "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
+                      "cfl_max": args.adjoint_cfl, "linear_solve_cfl": args.adjoint_solve_cfl,
+                      "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
                       "adjoint_tol": args.adjoint_tol},
            "adjoint": {"gradients": "torch.autograd.grad through aero_hip.torch_ops.steady_coefficients",
                        "primal_seconds": round(t_primal, 3), "primal_iterations": t_iters,
@@ -551,6 +553,9 @@ def main() -> int:
     ap.add_argument("--uneven-lowmach", choices=["precond", "plain"], default="precond",
                     help="--uneven: M 0.05 cases with the low-Mach preconditioner (its long runs diverge) or without")
     ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="--adjoint: linear residual drop")
+    ap.add_argument("--adjoint-solve-cfl", type=float, default=2.4,
+                    help="--adjoint: CFL of the linear (adjoint / tangent) pseudo-time solves; with the limiter "
+                         "frozen the linearised iteration on this case converges at 2.5 and diverges from 3.0")
     ap.add_argument("--adjoint-cfl", type=float, default=2.0,
                     help="--adjoint: CFL of the primal and the adjoint (at 2.4 the AoA / Mach finite-difference "
                          "continuations no longer converge to 1e-7)")
