@@ -23,19 +23,22 @@ using aero_dev::Dual;
 using aero_dev::FaceOpts;
 using aero_dev::FaceOptsT;
 using aero_dev::GeoTan;
+using aero_dev::kEntryTile;
+using aero_dev::kJacCols;
 using namespace aero_hostutil;
 
 struct AdjointData {
   AdjMesh am{};
   std::vector<void*> allocs;
   int nparts = 0;
+  size_t nslots = 0;       // face-entry slots (kEntryTile-cell tiles)
   double* G2 = nullptr;    // [8][N] gradient values of the tangent evaluation
   double* dG = nullptr;    // [8][N]
   double* dU = nullptr;    // [4][N]
   double* fa = nullptr;    // [8][F]
   double* R2 = nullptr;    // [4][N]
   double* dR = nullptr;    // [4][N]
-  double* J = nullptr;     // [96][F] face Jacobians
+  double* J = nullptr;     // [tiles][kJacCols][kEntryTile] face Jacobians per face-entry slot
   double* Gbar = nullptr;  // [8][N]
   double* out = nullptr;   // [4][N]
   double* psi = nullptr;   // [4][N]
@@ -91,9 +94,8 @@ template <class T> hipError_t aupload(AdjointData* a, T** p, const std::vector<T
   return e;
 }
 
-// Builds the workspace of handle m.
-int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
-  if (m->adj) return AERO_OK;
+// Builds the workspace of handle m (adjoint_setup releases it again when any step fails).
+int adjoint_setup_steps(aero_hip_mesh* m, char* err, size_t errlen) {
   const int N = m->N, F = m->F;
   AdjointData* a = new AdjointData();
   m->adj = a;
@@ -150,6 +152,52 @@ int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
       H[2 * static_cast<size_t>(N) + c] = aa * inv;
     }
   }
+  // Face entries in tiles of kEntryTile cells (the AdjMesh e_* layout): per group of cells as many tiles as
+  // its largest face count; per slot the other cell and side, the gradient-transpose face weights and the
+  // reconstruction vector; per face the slots of its two sides.
+  const int ngroups = (N + kEntryTile - 1) / kEntryTile;
+  std::vector<int> e_tile(ngroups + 1, 0);
+  for (int b = 0; b < ngroups; ++b) {
+    int deg = 0;
+    for (int c = b * kEntryTile; c < std::min(N, (b + 1) * kEntryTile); ++c) deg = std::max(deg, cf_off[c + 1] - cf_off[c]);
+    e_tile[b + 1] = e_tile[b] + deg;
+  }
+  if (static_cast<long long>(e_tile[ngroups]) * kEntryTile > 0x7fffffffLL) {
+    set_err(err, errlen, "adjoint setup: too many face entries for 32-bit slot indices");
+    return AERO_EINVAL;
+  }
+  const size_t nslots = static_cast<size_t>(e_tile[ngroups]) * kEntryTile;
+  std::vector<int> e_nb(nslots, -2), f_slot(2 * static_cast<size_t>(F), -1);
+  std::vector<double> e_gx(nslots, 0.0), e_gy(nslots, 0.0), e_dx(nslots, 0.0), e_dy(nslots, 0.0);
+  for (int c = 0; c < N; ++c) {
+    const int rc = m->perm[c];
+    for (int e = cf_off[c]; e < cf_off[c + 1]; ++e) {
+      const int f = cf_ent[e] >> 1, side = cf_ent[e] & 1;
+      const size_t sl = static_cast<size_t>(e_tile[c / kEntryTile] + (e - cf_off[c])) * kEntryTile + c % kEntryTile;
+      const int other = side ? fo[f] : fn[f];
+      e_nb[sl] = other >= 0 ? (other << 1) | side : -2;
+      if (m->dm.grad == 1) {
+        e_gx[sl] = wdx[f];
+        e_gy[sl] = wdy[f];
+      } else {
+        e_gx[sl] = m->h_nx[f] * m->h_area[f];
+        e_gy[sl] = m->h_ny[f] * m->h_area[f];
+      }
+      e_dx[sl] = m->h_fx[f] - m->h_ccx[rc];
+      e_dy[sl] = m->h_fy[f] - m->h_ccy[rc];
+      f_slot[static_cast<size_t>(side) * F + f] = static_cast<int>(sl);
+    }
+  }
+  a->nslots = nslots;
+  int *d_etile = nullptr, *d_enb = nullptr, *d_fslot = nullptr;
+  double *d_egx = nullptr, *d_egy = nullptr, *d_edx = nullptr, *d_edy = nullptr;
+  AERO_TRY(aupload(a, &d_etile, e_tile), "adjoint setup");
+  AERO_TRY(aupload(a, &d_enb, e_nb), "adjoint setup");
+  AERO_TRY(aupload(a, &d_fslot, f_slot), "adjoint setup");
+  AERO_TRY(aupload(a, &d_egx, e_gx), "adjoint setup");
+  AERO_TRY(aupload(a, &d_egy, e_gy), "adjoint setup");
+  AERO_TRY(aupload(a, &d_edx, e_dx), "adjoint setup");
+  AERO_TRY(aupload(a, &d_edy, e_dy), "adjoint setup");
   const size_t n4 = 4 * static_cast<size_t>(N), n8 = 8 * static_cast<size_t>(N);
   int *d_fo = nullptr, *d_fn = nullptr, *d_bc = nullptr, *d_off = nullptr, *d_ent = nullptr;
   double *d_nx = nullptr, *d_ny = nullptr, *d_A = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_wx = nullptr,
@@ -188,6 +236,13 @@ int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
   am.ccx = m->dm.ccx;
   am.ccy = m->dm.ccy;
   am.H = d_H;
+  am.e_tile = d_etile;
+  am.e_nb = d_enb;
+  am.e_gx = d_egx;
+  am.e_gy = d_egy;
+  am.e_dx = d_edx;
+  am.e_dy = d_edy;
+  am.f_slot = d_fslot;
   am.wall_start = m->wall_start;
   am.wall_count = m->wall_count;
   a->nparts = (N + 255) / 256;
@@ -209,10 +264,17 @@ int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
   return AERO_OK;
 }
 
-// Face Jacobian and per-face product buffers (the largest: 96 + 24 doubles per face), on demand.
+int adjoint_setup(aero_hip_mesh* m, char* err, size_t errlen) {
+  if (m->adj) return AERO_OK;
+  const int rc = adjoint_setup_steps(m, err, errlen);
+  if (rc) adjoint_free(m);
+  return rc;
+}
+
+// The stored face Jacobians (kJacCols doubles per face-entry slot, the largest adjoint buffer), on demand.
 int ensure_jacobian_buffers(aero_hip_mesh* m, char* err, size_t errlen) {
   AdjointData* a = m->adj;
-  if (!a->J) AERO_TRY(aalloc(a, &a->J, 96 * static_cast<size_t>(m->F)), "face Jacobian buffer");
+  if (!a->J) AERO_TRY(aalloc(a, &a->J, kJacCols * a->nslots), "face Jacobian buffer");
   return AERO_OK;
 }
 

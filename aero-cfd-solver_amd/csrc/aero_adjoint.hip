@@ -5,7 +5,8 @@
 //
 //   tangent:  dR = R_U dU + R_p dp           k_tan_gradient -> k_tan_face -> k_tan_gather
 //   adjoint:  out = R_U^T psi                 k_adj_gather -> k_adj_grad, over the face Jacobians
-//             J_f = d(F_f A_f)/d(U_o, U_n, G_o, G_n) that k_face_jacobian stores once per state; inside the
+//             J_f = d(F_f A_f)/d(U_o, U_n, a_o, a_n) (a: the reconstruction increments) that
+//             k_face_jacobian stores once per state, each side in its cell's face entry; inside the
 //             pseudo-time iteration k_adj_grad_update also applies the stage update
 //
 // All sums run per cell over its faces in increasing reference face index (no atomics), so results are
@@ -222,18 +223,23 @@ __global__ __launch_bounds__(kThreads) void k_tan_gather(AdjMesh am, const doubl
 }
 
 // ---- face Jacobians ----------------------------------------------------------------------------------
-// Thread (f, dir): column `dir` of d(F_f A_f)/d(U_o[0..3], U_n[0..3], G_o[0..7], G_n[0..7]), stored face-major
-// at J[f * 96 + dir * 4 + k] (the 24 threads of a face write one contiguous 768-byte record).  Boundary faces
-// depend on U_o only; their other columns are zero.
+// Thread (f, dir): one column of d(F_f A_f) / d(U_o[0..3], U_n[0..3], a_o[0..3], a_n[0..3]), where a_o / a_n
+// are the owner's and neighbour's reconstruction increments of (rho, u, v, p) (interior_face_incr: the flux
+// depends on the gradients only through them).  Columns of the owner's variables go to the face's
+// owner-side slot, the neighbour's to its neighbour-side slot (kJacCols layout).  Boundary faces depend on
+// U_o only; their increment columns are zero.
 template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
 __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const double* __restrict__ U,
                                                             const double* __restrict__ G, FaceOpts o0,
                                                             double* __restrict__ J) {
   const long long tid = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   const int F = am.F;
-  if (tid >= 24LL * F) return;
-  const int f = static_cast<int>(tid / 24);
-  const int dir = static_cast<int>(tid - 24LL * f);
+  if (tid >= 16LL * F) return;
+  const int f = static_cast<int>(tid >> 4);
+  const int dir = static_cast<int>(tid & 15);
+  const int side = (dir >> 2) & 1;  // 0..3 and 8..11: owner; 4..7 and 12..15: neighbour
+  const int slot = am.f_slot[side * F + f];
+  if (slot < 0) return;  // the missing neighbour side of a boundary face
   const int ld = am.ld;
   const int co = am.fo[f], cn = am.fn[f];
   FaceOptsT<Dual> o;
@@ -258,29 +264,26 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
   double col[4] = {0.0, 0.0, 0.0, 0.0};
   if (cn >= 0) {
     C4T<Dual> uo = load_dual_u(U, nullptr, ld, co), un = load_dual_u(U, nullptr, ld, cn);
-    Dual go[8], gn[8];
-    Dual dox(0.0), doy(0.0), dnx(0.0), dny(0.0);
+    Dual ao[4], an[4];
     if (SECOND) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        go[j] = Dual(G[j * ld + co]);
-        gn[j] = Dual(G[j * ld + cn]);
-      }
       const double fx = am.ffx[f], fy = am.ffy[f];
-      dox = Dual(fx - am.ccx[co]);
-      doy = Dual(fy - am.ccy[co]);
-      dnx = Dual(fx - am.ccx[cn]);
-      dny = Dual(fy - am.ccy[cn]);
+      const double dox = fx - am.ccx[co], doy = fy - am.ccy[co];
+      const double dnx = fx - am.ccx[cn], dny = fy - am.ccy[cn];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ao[k] = Dual(recon_incr(G[(2 * k) * ld + co], G[(2 * k + 1) * ld + co], dox, doy));
+        an[k] = Dual(recon_incr(G[(2 * k) * ld + cn], G[(2 * k + 1) * ld + cn], dnx, dny));
+      }
     }
     bool active = true;
     if (dir < 4) {
       (dir == 0 ? uo.r : dir == 1 ? uo.ru : dir == 2 ? uo.rv : uo.E).d = 1.0;
     } else if (dir < 8) {
       (dir == 4 ? un.r : dir == 5 ? un.ru : dir == 6 ? un.rv : un.E).d = 1.0;
-    } else if (dir < 16) {
-      if (SECOND) go[dir - 8].d = 1.0; else active = false;
+    } else if (dir < 12) {
+      if (SECOND) ao[dir - 8].d = 1.0; else active = false;
     } else {
-      if (SECOND) gn[dir - 16].d = 1.0; else active = false;
+      if (SECOND) an[dir - 12].d = 1.0; else active = false;
     }
     if (active) {
       double fr[8];
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
       }
       bool reverted;
       const FluxT<Dual> Fl =
-          interior_face<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, go, gn, nx, ny, dox, doy, dnx, dny, o, &reverted, fr);
+          interior_face_incr<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, ao, an, nx, ny, o, &reverted, fr);
       col[0] = Fl.f0.d * A;
       col[1] = Fl.f1.d * A;
       col[2] = Fl.f2.d * A;
@@ -305,46 +308,61 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
     col[2] = Fl.f2.d * A;
     col[3] = Fl.f3.d * A;
   }
+  const int q0 = (dir >= 8 ? 16 : 0) + (dir & 3) * 4;
+  double* t = J + static_cast<size_t>(slot / kEntryTile) * (kJacCols * kEntryTile) + (slot % kEntryTile);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) J[static_cast<size_t>(f) * 96 + dir * 4 + k] = col[k];
+  for (int k = 0; k < 4; ++k) t[(q0 + k) * kEntryTile] = col[k];
 }
 
 // ---- adjoint product out = R_U^T psi ---------------------------------------------------------------
 // Face f adds F_f A_f to R_o and subtracts it from R_n, so its weight is w = psi_o - psi_n (psi_o on
-// boundary faces).  Per cell, over its faces in face order, the cell's side of J_f^T w: the owner takes
-// columns 0..3 (U_o) and 8..15 (G_o), the neighbour 4..7 (U_n) and 16..23 (G_n).  Writes the direct part
-// of R_U^T psi and the gradient adjoint Gbar.
+// boundary faces).  Per cell, over its face entries (increasing face index), its side of J_f^T w: the
+// conserved-state columns give the direct part of R_U^T psi, the increment columns b_d = (dF/da_d)^T w
+// give the gradient adjoint Gbar[d] += b_d (face centre - cell centre).
+__device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi, int ld, int c, const double* pc,
+                                                 int nb, double* w) {
+  const int other = nb >> 1;  // -1 on boundary faces
+  if (nb & 1) {               // this cell is the face's neighbour: w = psi_other - psi_c
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = psi[k * ld + other] - pc[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = pc[k] - (other >= 0 ? psi[k * ld + other] : 0.0);
+  }
+}
+
 __device__ __forceinline__ void adj_gather_cell(const AdjMesh& am, const double* __restrict__ J,
                                                 const double* __restrict__ psi, double* __restrict__ out,
                                                 double* __restrict__ Gbar, int c) {
   const int ld = am.ld;
+  const int lane = c % kEntryTile;
+  const int t0 = am.e_tile[c / kEntryTile];
+  const int ne = am.cf_off[c + 1] - am.cf_off[c];
+  double pc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pc[k] = psi[k * ld + c];
   double u[4] = {0.0, 0.0, 0.0, 0.0};
   double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int e1 = am.cf_off[c + 1];
-  for (int e = am.cf_off[c]; e < e1; ++e) {
-    const int ent = am.cf_ent[e];
-    const int f = ent >> 1;
-    const int side = ent & 1;
-    const int o = am.fo[f], n = am.fn[f];
+  for (int i = 0; i < ne; ++i) {
+    const int s = (t0 + i) * kEntryTile + lane;
     double w[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) w[k] = psi[k * ld + o] - (n >= 0 ? psi[k * ld + n] : 0.0);
-    const double* Jf = J + static_cast<size_t>(f) * 96;
-    const double* Ju = Jf + (side ? 4 : 0) * 4;   // columns 0..3 or 4..7
-    const double* Jg = Jf + (side ? 16 : 8) * 4;  // columns 8..15 or 16..23
+    adj_entry_weight(psi, ld, c, pc, am.e_nb[s], w);
+    const double* Jt = J + static_cast<size_t>(t0 + i) * (kJacCols * kEntryTile) + lane;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
       double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) t += Ju[d * 4 + k] * w[k];
+      for (int k = 0; k < 4; ++k) t += Jt[(d * 4 + k) * kEntryTile] * w[k];
       u[d] += t;
     }
+    const double dx = am.e_dx[s], dy = am.e_dy[s];
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
+    for (int d = 0; d < 4; ++d) {
       double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) t += Jg[d * 4 + k] * w[k];
-      gb[d] += t;
+      for (int k = 0; k < 4; ++k) t += Jt[(16 + d * 4 + k) * kEntryTile] * w[k];
+      gb[2 * d] += t * dx;
+      gb[2 * d + 1] += t * dy;
     }
   }
 #pragma unroll
@@ -361,58 +379,71 @@ __global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const doubl
   adj_gather_cell(am, J, psi, out, Gbar, c);
 }
 
-// Per cell: Wbar from the gradient adjoint of the cell and its face neighbours, then
-// out += (dW/dU)^T Wbar.
+// Wbar of cell c from the gradient adjoint of the cell and its face neighbours, over its face entries
+// (increasing face index):
 //   Green-Gauss: G_c[k] = iv_c sum_f +-q_f nA_f, q_f = (W_o + W_n) / 2 (W_o on boundary faces), so both
 //     cells of an interior face get (1/2) nA . (iv_o Gbar_o[k] - iv_n Gbar_n[k]); a boundary face gives
 //     its owner nA . iv_o Gbar_o[k].
 //   Least squares: G_c[k] = H_c sum_f (w d)_f (W_n - W_o)_k, so s = (w d)_f . (H_o Gbar_o[k] +
 //     H_n Gbar_n[k]) goes to the neighbour with + and to the owner with -.
 template <int GK>
-__global__ __launch_bounds__(kThreads) void k_adj_grad(AdjMesh am, const double* __restrict__ U, double g,
-                                                       const double* __restrict__ Gbar, double* __restrict__ out) {
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= am.N) return;
+__device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __restrict__ Gbar, int c, double* wb) {
   const int ld = am.ld;
-  double wb[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = am.cf_off[c + 1];
-  for (int e = am.cf_off[c]; e < e1; ++e) {
-    const int ent = am.cf_ent[e];
-    const int f = ent >> 1;
-    const int o = am.fo[f], n = am.fn[f];
+  const int lane = c % kEntryTile;
+  const int t0 = am.e_tile[c / kEntryTile];
+  const int ne = am.cf_off[c + 1] - am.cf_off[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wb[k] = 0.0;
+  for (int i = 0; i < ne; ++i) {
+    const int s = (t0 + i) * kEntryTile + lane;
+    const int nb = am.e_nb[s];
+    const int side = nb & 1;
+    const int other = nb >> 1;
+    const int fo = side ? other : c;
+    const int fn = side ? c : other;
     if (GK == 1) {
-      if (n < 0) continue;
-      const double wx = am.fwdx[f], wy = am.fwdy[f];
-      const double hxo = am.H[o], hyo = am.H[ld + o], hzo = am.H[2 * ld + o];
-      const double hxn = am.H[n], hyn = am.H[ld + n], hzn = am.H[2 * ld + n];
+      if (fn < 0) continue;
+      const double wx = am.e_gx[s], wy = am.e_gy[s];
+      const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
+      const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double gxo = Gbar[(2 * k) * ld + o], gyo = Gbar[(2 * k + 1) * ld + o];
-        const double gxn = Gbar[(2 * k) * ld + n], gyn = Gbar[(2 * k + 1) * ld + n];
+        const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
+        const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
         const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
         const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
-        const double s = wx * ax + wy * ay;
-        wb[k] += (ent & 1) ? s : -s;
+        const double sv = wx * ax + wy * ay;
+        wb[k] += side ? sv : -sv;
       }
     } else {
-      const double A = am.fA[f];
-      const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
-      const double ivo = 1.0 / dmax(am.vol[o], 1.0e-12);
-      if (n >= 0) {
-        const double ivn = 1.0 / dmax(am.vol[n], 1.0e-12);
+      const double nax = am.e_gx[s], nay = am.e_gy[s];
+      const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
+      if (fn >= 0) {
+        const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const double tx = ivo * Gbar[(2 * k) * ld + o] - ivn * Gbar[(2 * k) * ld + n];
-          const double ty = ivo * Gbar[(2 * k + 1) * ld + o] - ivn * Gbar[(2 * k + 1) * ld + n];
+          const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
+          const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
           wb[k] += 0.5 * (nax * tx + nay * ty);
         }
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + o]) + nay * (ivo * Gbar[(2 * k + 1) * ld + o]);
+          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
       }
     }
   }
+}
+
+// Per cell: Wbar (adj_wbar), then out += (dW/dU)^T Wbar.
+template <int GK>
+__global__ __launch_bounds__(kThreads) void k_adj_grad(AdjMesh am, const double* __restrict__ U, double g,
+                                                       const double* __restrict__ Gbar, double* __restrict__ out) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= am.N) return;
+  const int ld = am.ld;
+  double wb[4];
+  adj_wbar<GK>(am, Gbar, c, wb);
   // (dW/dU)^T Wbar by four dual evaluations of the primitive map (euler_flux.cpp:115-127).
   const C4 u0 = load_c4(U, ld, c);
   double add[4];
@@ -476,52 +507,6 @@ __device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, dou
 // gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
 // x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
 // and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
-// Wbar of cell c from the gradient adjoint of its faces' cells (k_adj_grad's gather).
-template <int GK>
-__device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __restrict__ Gbar, int c, double* wb) {
-  const int ld = am.ld;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) wb[k] = 0.0;
-  const int e1 = am.cf_off[c + 1];
-  for (int e = am.cf_off[c]; e < e1; ++e) {
-    const int ent = am.cf_ent[e];
-    const int f = ent >> 1;
-    const int fo = am.fo[f], fn = am.fn[f];
-    if (GK == 1) {
-      if (fn < 0) continue;
-      const double wx = am.fwdx[f], wy = am.fwdy[f];
-      const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
-      const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
-        const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
-        const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
-        const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
-        const double sv = wx * ax + wy * ay;
-        wb[k] += (ent & 1) ? sv : -sv;
-      }
-    } else {
-      const double A = am.fA[f];
-      const double nax = am.fnx[f] * A, nay = am.fny[f] * A;
-      const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
-      if (fn >= 0) {
-        const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
-          const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
-          wb[k] += 0.5 * (nax * tx + nay * ty);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
-      }
-    }
-  }
-}
-
 // d(rho, u, v, p)/dU_j of the primitive map at u0 (euler_flux.cpp:115-127), by four dual evaluations:
 // jac[j] = (drho, du, dv, dp) / dU_j.
 __device__ __forceinline__ void c2p_jacobian(const C4& u0, double g, double (*jac)[4]) {
@@ -654,46 +639,50 @@ __global__ __launch_bounds__(kThreads) void k_adj_gather_batch(const AdjBatchSol
   const int ld = am.ld;
   const double* __restrict__ pa = sa.psi;
   const double* __restrict__ pb = sb.psi;
+  const int lane = c % kEntryTile;
+  const int t0 = am.e_tile[c / kEntryTile];
+  const int ne = am.cf_off[c + 1] - am.cf_off[c];
+  double pca[4], pcb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    pca[k] = pa[k * ld + c];
+    pcb[k] = pb[k * ld + c];
+  }
   double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
   double ga[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int e1 = am.cf_off[c + 1];
-  for (int e = am.cf_off[c]; e < e1; ++e) {
-    const int ent = am.cf_ent[e];
-    const int f = ent >> 1;
-    const int side = ent & 1;
-    const int o = am.fo[f], n = am.fn[f];
+  for (int i = 0; i < ne; ++i) {
+    const int s = (t0 + i) * kEntryTile + lane;
+    const int nb = am.e_nb[s];
     double wa[4], wv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      wa[k] = pa[k * ld + o] - (n >= 0 ? pa[k * ld + n] : 0.0);
-      wv[k] = pb[k * ld + o] - (n >= 0 ? pb[k * ld + n] : 0.0);
-    }
-    const double* Jf = sa.J + static_cast<size_t>(f) * 96;
-    const double* Ju = Jf + (side ? 4 : 0) * 4;
-    const double* Jg = Jf + (side ? 16 : 8) * 4;
+    adj_entry_weight(pa, ld, c, pca, nb, wa);
+    adj_entry_weight(pb, ld, c, pcb, nb, wv);
+    const double* Jt = sa.J + static_cast<size_t>(t0 + i) * (kJacCols * kEntryTile) + lane;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
       double ta = 0.0, tb = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double j = Ju[d * 4 + k];
+        const double j = Jt[(d * 4 + k) * kEntryTile];
         ta += j * wa[k];
         tb += j * wv[k];
       }
       ua[d] += ta;
       ub[d] += tb;
     }
+    const double dx = am.e_dx[s], dy = am.e_dy[s];
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
+    for (int d = 0; d < 4; ++d) {
       double ta = 0.0, tb = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double j = Jg[d * 4 + k];
+        const double j = Jt[(16 + d * 4 + k) * kEntryTile];
         ta += j * wa[k];
         tb += j * wv[k];
       }
-      ga[d] += ta;
-      gb[d] += tb;
+      ga[2 * d] += ta * dx;
+      ga[2 * d + 1] += ta * dy;
+      gb[2 * d] += tb * dx;
+      gb[2 * d + 1] += tb * dy;
     }
   }
 #pragma unroll
@@ -1010,7 +999,7 @@ hipError_t launch_tan_gather(const AdjMesh& am, const double* fa, double* R, dou
 template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
 static void face_jac_t(const AdjMesh& am, const double* U, const double* G, const FaceOpts& o, double* J,
                        hipStream_t s) {
-  hipLaunchKernelGGL((k_face_jacobian<SCHEME, LIMITER, SECOND, FROZEN>), dim3(blocks_for(24LL * am.F)), dim3(kThreads),
+  hipLaunchKernelGGL((k_face_jacobian<SCHEME, LIMITER, SECOND, FROZEN>), dim3(blocks_for(16LL * am.F)), dim3(kThreads),
                      0, s, am, U, G, o, J);
 }
 

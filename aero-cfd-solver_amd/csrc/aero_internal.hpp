@@ -132,7 +132,25 @@ struct AdjMesh {
   const double* ccy;
   const double* H;       // [3][N] least-squares inverse normal matrix (hxx, hxy, hyy); zero when singular
   int wall_start, wall_count;
+  // Per-cell face entries in tiles of kEntryTile cells: cell c (group c / kEntryTile, lane
+  // c % kEntryTile) keeps its i-th face, in increasing reference face index, in slot
+  // (e_tile[group] + i) * kEntryTile + lane, so a wavefront of consecutive cells reads consecutive
+  // addresses.  A group has as many tiles as its cells' largest face count.
+  const int* e_tile;     // [groups + 1] first tile of each group
+  const int* e_nb;       // [slots] other cell << 1 | side (side 1: this cell is the face's neighbour); -2 boundary
+  const double* e_gx;    // [slots] gradient-transpose face weights: nx A (Green-Gauss) or w dx (least squares)
+  const double* e_gy;
+  const double* e_dx;    // [slots] face centre - this cell's centre (the reconstruction vector)
+  const double* e_dy;
+  const int* f_slot;     // [2][F] owner-side and neighbour-side slot of each face, -1 where none
 };
+
+// Cells per entry tile (one workgroup of the adjoint kernels).
+constexpr int kEntryTile = 256;
+// Stored face-Jacobian doubles per slot, tile-major [tile][kJacCols][kEntryTile]: column d * 4 + k is
+// d(F_k A) / d(U_d) of this side's conserved state, 16 + d * 4 + k is d(F_k A) / d(a_d) of this side's
+// reconstruction increment a_d = g_d . (face centre - cell centre) of primitive d (rho, u, v, p).
+constexpr int kJacCols = 32;
 
 // Geometry tangents (any pointer may be null = zero): faces in reference order, cells internal.
 struct GeoTan {

@@ -8,32 +8,54 @@
 
 namespace aero_dev {
 
-// MUSCL reconstruction of one primitive variable (reconstruction.cpp:194-213).
-template <int LIMITER, class T>
-__device__ __forceinline__ void recon_var(T qo, T qn, T gox, T goy, T gnx, T gny, T dox, T doy, T dnx, T dny, T* ql,
-                                          T* qr) {
+// Unlimited reconstruction increment g . d of one primitive variable (reconstruction.cpp:194-213).
+template <class T>
+__device__ __forceinline__ T recon_incr(T gx, T gy, T dx, T dy) {
+  T a = 0.0;
+  a += gx * dx;
+  a += gy * dy;
+  return a;
+}
+
+// Sources of the increments of variable k (0..3 = rho, u, v, p) on the owner (da) and neighbour (db)
+// sides: from the cells' gradients and centre-to-face vectors, or given directly.
+template <class T>
+struct GradIncr {
+  const T* go;
+  const T* gn;
+  T dox, doy, dnx, dny;
+  __device__ __forceinline__ void operator()(int k, T* da, T* db) const {
+    *da = recon_incr(go[2 * k], go[2 * k + 1], dox, doy);
+    *db = recon_incr(gn[2 * k], gn[2 * k + 1], dnx, dny);
+  }
+};
+template <class T>
+struct GivenIncr {
+  const T* ao;
+  const T* an;
+  __device__ __forceinline__ void operator()(int k, T* da, T* db) const {
+    *da = ao[k];
+    *db = an[k];
+  }
+};
+
+// MUSCL reconstruction of primitive variable k (reconstruction.cpp:194-213).
+template <int LIMITER, class T, class Src>
+__device__ __forceinline__ void recon_var(T qo, T qn, int k, const Src& src, T* ql, T* qr) {
   const T dc = qn - qo;
-  T da = 0.0;
-  da += gox * dox;
-  da += goy * doy;
-  T db = 0.0;
-  db += gnx * dnx;
-  db += gny * dny;
+  T da, db;
+  src(k, &da, &db);
   *ql = qo + limiter<LIMITER>(da, dc);
   *qr = qn + limiter<LIMITER>(db, -dc);
 }
 
 // Frozen limiter (beyond the reference): the limited increment is ratio * a with the ratio
 // phi(a, b) / a captured once (0 where a == 0), a smooth function of the state.
-template <class T>
-__device__ __forceinline__ void recon_var_frozen(T qo, T qn, T gox, T goy, T gnx, T gny, T dox, T doy, T dnx, T dny,
-                                                 double ro, double rn, T* ql, T* qr) {
-  T da = 0.0;
-  da += gox * dox;
-  da += goy * doy;
-  T db = 0.0;
-  db += gnx * dnx;
-  db += gny * dny;
+template <class T, class Src>
+__device__ __forceinline__ void recon_var_frozen(T qo, T qn, int k, const Src& src, double ro, double rn, T* ql,
+                                                 T* qr) {
+  T da, db;
+  src(k, &da, &db);
   *ql = qo + ro * da;
   *qr = qn + rn * db;
 }
@@ -53,14 +75,12 @@ __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, 
   *rn = db != 0.0 ? limiter<LIMITER>(db, -dc) / db : 0.0;
 }
 
-// Interior face: reconstruction with the first-order fallback (residual_assembly.cpp:107-137).
-// go / gn are the cells' primitive gradients (rho_x, rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the
-// vectors from the cell centres to the face centre.  *reverted reports the fallback to cell values.
+// Interior face with the first-order fallback (residual_assembly.cpp:107-137), the reconstruction
+// increments taken from src.  *reverted reports the fallback to cell values.
 // FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, class T>
-__device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
-                                                  T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
-                                                  bool* reverted, const double* fr = nullptr) {
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, class T, class Src>
+__device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4T<T>& un, const Src& src, T nx, T ny,
+                                                      const FaceOptsT<T>& o, bool* reverted, const double* fr) {
   const double g = o.g;
   P4T<T> L = c2p(uo, g);
   P4T<T> Rt = c2p(un, g);
@@ -68,15 +88,15 @@ __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>
   if (SECOND) {
     P4T<T> l2, r2;
     if (FROZEN) {
-      recon_var_frozen(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, fr[0], fr[4], &l2.r, &r2.r);
-      recon_var_frozen(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, fr[1], fr[5], &l2.u, &r2.u);
-      recon_var_frozen(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, fr[2], fr[6], &l2.v, &r2.v);
-      recon_var_frozen(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, fr[3], fr[7], &l2.p, &r2.p);
+      recon_var_frozen(L.r, Rt.r, 0, src, fr[0], fr[4], &l2.r, &r2.r);
+      recon_var_frozen(L.u, Rt.u, 1, src, fr[1], fr[5], &l2.u, &r2.u);
+      recon_var_frozen(L.v, Rt.v, 2, src, fr[2], fr[6], &l2.v, &r2.v);
+      recon_var_frozen(L.p, Rt.p, 3, src, fr[3], fr[7], &l2.p, &r2.p);
     } else {
-      recon_var<LIMITER>(L.r, Rt.r, go[0], go[1], gn[0], gn[1], dox, doy, dnx, dny, &l2.r, &r2.r);
-      recon_var<LIMITER>(L.u, Rt.u, go[2], go[3], gn[2], gn[3], dox, doy, dnx, dny, &l2.u, &r2.u);
-      recon_var<LIMITER>(L.v, Rt.v, go[4], go[5], gn[4], gn[5], dox, doy, dnx, dny, &l2.v, &r2.v);
-      recon_var<LIMITER>(L.p, Rt.p, go[6], go[7], gn[6], gn[7], dox, doy, dnx, dny, &l2.p, &r2.p);
+      recon_var<LIMITER>(L.r, Rt.r, 0, src, &l2.r, &r2.r);
+      recon_var<LIMITER>(L.u, Rt.u, 1, src, &l2.u, &r2.u);
+      recon_var<LIMITER>(L.v, Rt.v, 2, src, &l2.v, &r2.v);
+      recon_var<LIMITER>(L.p, Rt.p, 3, src, &l2.p, &r2.p);
     }
     l2.r = dmax(l2.r, kFloor);
     r2.r = dmax(r2.r, kFloor);
@@ -94,6 +114,27 @@ __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>
   const T fb = face_beta(ul, ur, o);
   const T sc = acoustic_scale(ul, ur, fb, o);
   return face_flux<SCHEME>(ul, ur, nx, ny, sc, o);
+}
+
+// Interior face (residual_assembly.cpp:107-137).  go / gn are the cells' primitive gradients (rho_x,
+// rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the vectors from the cell centres to the face centre.
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, class T>
+__device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
+                                                  T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
+                                                  bool* reverted, const double* fr = nullptr) {
+  const GradIncr<T> src{go, gn, dox, doy, dnx, dny};
+  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, src, nx, ny, o, reverted, fr);
+}
+
+// The same face from its reconstruction increments ao / an (owner / neighbour g . d of rho, u, v, p):
+// the flux depends on the cell gradients only through these eight values, which is the form the stored
+// face Jacobians take.
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, class T>
+__device__ __forceinline__ FluxT<T> interior_face_incr(const C4T<T>& uo, const C4T<T>& un, const T* ao, const T* an,
+                                                       T nx, T ny, const FaceOptsT<T>& o, bool* reverted,
+                                                       const double* fr = nullptr) {
+  const GivenIncr<T> src{ao, an};
+  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, src, nx, ny, o, reverted, fr);
 }
 
 // Boundary face: slip wall or farfield on the owner's first-order state (residual_assembly.cpp:138-166,
