@@ -319,8 +319,16 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
 // boundary faces).  Per cell, over its face entries (increasing face index), its side of J_f^T w: the
 // conserved-state columns give the direct part of R_U^T psi, the increment columns b_d = (dF/da_d)^T w
 // give the gradient adjoint Gbar[d] += b_d (face centre - cell centre).
-__device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi, int ld, int c, const double* pc,
-                                                 int nb, double* w) {
+//
+// The per-cell kernels run kWide threads per cell (kWideThreads per workgroup: entry lane `sub` of the
+// cells of one entry group handles entries sub, sub + kWide, ...), so a small mesh still fills the chip.
+// wide_accumulate adds the entries' values into LDS one entry lane at a time, in increasing entry order,
+// so every per-cell sum is that of a serial loop over the cell's entries, bit for bit.
+constexpr int kWide = 4;
+constexpr int kWideThreads = kWide * kEntryTile;
+
+__device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi, int ld, const double* pc, int nb,
+                                                 double* w) {
   const int other = nb >> 1;  // -1 on boundary faces
   if (nb & 1) {               // this cell is the face's neighbour: w = psi_other - psi_c
 #pragma unroll
@@ -331,106 +339,198 @@ __device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi,
   }
 }
 
-__device__ __forceinline__ void adj_gather_cell(const AdjMesh& am, const double* __restrict__ J,
-                                                const double* __restrict__ psi, double* __restrict__ out,
-                                                double* __restrict__ Gbar, int c) {
-  const int ld = am.ld;
-  const int lane = c % kEntryTile;
-  const int t0 = am.e_tile[c / kEntryTile];
-  const int ne = am.cf_off[c + 1] - am.cf_off[c];
-  double pc[4];
+// Thread layout of the wide kernels over entry group `grp`.
+struct WideCell {
+  int lane, sub, c, ne, t0, tiles;
+  bool live;
+};
+
+__device__ __forceinline__ WideCell wide_cell(const AdjMesh& am, int grp) {
+  WideCell w;
+  w.lane = threadIdx.x % kEntryTile;
+  w.sub = threadIdx.x / kEntryTile;
+  w.c = grp * kEntryTile + w.lane;
+  w.live = w.c < am.N;
+  w.ne = w.live ? am.cf_off[w.c + 1] - am.cf_off[w.c] : 0;
+  w.t0 = am.e_tile[grp];
+  w.tiles = am.e_tile[grp + 1] - w.t0;  // uniform over the workgroup
+  return w;
+}
+
+// acc[v * kEntryTile + lane] = sum over the cell's entries i (in increasing order) of the values that
+// entry(i, val) reports (entry returns false for an entry that adds nothing).  All threads of the
+// workgroup must call it; acc is ready for every thread on return.
+template <int NV, class Fn>
+__device__ __forceinline__ void wide_accumulate(double* acc, const WideCell& w, Fn&& entry) {
+  for (int v = threadIdx.x; v < NV * kEntryTile; v += kWideThreads) acc[v] = 0.0;
+  __syncthreads();
+  for (int r = 0; r < w.tiles; r += kWide) {
+    const int i = r + w.sub;
+    double val[NV];
+    bool add = false;
+    if (i < w.ne) add = entry(i, val);
+    for (int j = 0; j < kWide; ++j) {
+      if (w.sub == j && add) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) pc[k] = psi[k * ld + c];
-  double u[4] = {0.0, 0.0, 0.0, 0.0};
-  double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < ne; ++i) {
-    const int s = (t0 + i) * kEntryTile + lane;
-    double w[4];
-    adj_entry_weight(psi, ld, c, pc, am.e_nb[s], w);
-    const double* Jt = J + static_cast<size_t>(t0 + i) * (kJacCols * kEntryTile) + lane;
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      double t = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) t += Jt[(d * 4 + k) * kEntryTile] * w[k];
-      u[d] += t;
-    }
-    const double dx = am.e_dx[s], dy = am.e_dy[s];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      double t = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) t += Jt[(16 + d * 4 + k) * kEntryTile] * w[k];
-      gb[2 * d] += t * dx;
-      gb[2 * d + 1] += t * dy;
+        for (int v = 0; v < NV; ++v) acc[v * kEntryTile + w.lane] += val[v];
+      }
+      __syncthreads();
     }
   }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) out[k * ld + c] = u[k];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) Gbar[j * ld + c] = gb[j];
 }
 
-__global__ __launch_bounds__(kThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
-                                                         const double* __restrict__ psi, double* __restrict__ out,
-                                                         double* __restrict__ Gbar) {
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= am.N) return;
-  adj_gather_cell(am, J, psi, out, Gbar, c);
+// One face entry of the gather for NS solves sharing the face Jacobians: val[12 s + d] (d < 4) the direct
+// part, val[12 s + 4 + j] the gradient adjoint of solve s.
+template <int NS>
+__device__ __forceinline__ bool adj_gather_entry(const AdjMesh& am, const double* __restrict__ J,
+                                                 const double* const* psi, const double (*pc)[4], const WideCell& w,
+                                                 int i, double* val) {
+  const int s = (w.t0 + i) * kEntryTile + w.lane;
+  const int nb = am.e_nb[s];
+  double wt[NS][4];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) adj_entry_weight(psi[q], am.ld, pc[q], nb, wt[q]);
+  const double* Jt = J + static_cast<size_t>(w.t0 + i) * (kJacCols * kEntryTile) + w.lane;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    double t[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) t[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double j = Jt[(d * 4 + k) * kEntryTile];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) t[q] += j * wt[q][k];
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) val[12 * q + d] = t[q];
+  }
+  const double dx = am.e_dx[s], dy = am.e_dy[s];
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    double t[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) t[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double j = Jt[(16 + d * 4 + k) * kEntryTile];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) t[q] += j * wt[q][k];
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      val[12 * q + 4 + 2 * d] = t[q] * dx;
+      val[12 * q + 4 + 2 * d + 1] = t[q] * dy;
+    }
+  }
+  return true;
 }
 
-// Wbar of cell c from the gradient adjoint of the cell and its face neighbours, over its face entries
-// (increasing face index):
+// The gather of entry group grp for NS solves: direct[q] = the direct part of R_U^T psi[q], Gbar[q] its
+// gradient adjoint.  acc: 12 NS kEntryTile doubles of LDS.
+template <int NS>
+__device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double* __restrict__ J,
+                                                 const double* const* psi, double* const* direct,
+                                                 double* const* Gbar, int grp, double* acc) {
+  const WideCell w = wide_cell(am, grp);
+  const int ld = am.ld;
+  double pc[NS][4];
+#pragma unroll
+  for (int q = 0; q < NS; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pc[q][k] = w.live ? psi[q][k * ld + w.c] : 0.0;
+  wide_accumulate<12 * NS>(acc, w, [&](int i, double* val) {
+    return adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
+  });
+  if (!w.live) return;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const int v = 12 * q + j;
+      if (v % kWide != w.sub) continue;  // the entry lanes share the stores
+      const double x = acc[v * kEntryTile + w.lane];
+      if (j < 4)
+        direct[q][j * ld + w.c] = x;
+      else
+        Gbar[q][(j - 4) * ld + w.c] = x;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWideThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
+                                                             const double* __restrict__ psi, double* __restrict__ out,
+                                                             double* __restrict__ Gbar) {
+  __shared__ double acc[12 * kEntryTile];
+  const double* p[1] = {psi};
+  double* o[1] = {out};
+  double* g[1] = {Gbar};
+  adj_gather_group<1>(am, J, p, o, g, blockIdx.x, acc);
+}
+
+// The gradient-adjoint term of one face entry of cell c, val[k] for primitive k (returns false where the
+// face adds nothing):
 //   Green-Gauss: G_c[k] = iv_c sum_f +-q_f nA_f, q_f = (W_o + W_n) / 2 (W_o on boundary faces), so both
 //     cells of an interior face get (1/2) nA . (iv_o Gbar_o[k] - iv_n Gbar_n[k]); a boundary face gives
 //     its owner nA . iv_o Gbar_o[k].
 //   Least squares: G_c[k] = H_c sum_f (w d)_f (W_n - W_o)_k, so s = (w d)_f . (H_o Gbar_o[k] +
 //     H_n Gbar_n[k]) goes to the neighbour with + and to the owner with -.
 template <int GK>
-__device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __restrict__ Gbar, int c, double* wb) {
+__device__ __forceinline__ bool adj_wbar_entry(const AdjMesh& am, const double* __restrict__ Gbar, int c, int s,
+                                               double* val) {
   const int ld = am.ld;
+  const int nb = am.e_nb[s];
+  const int side = nb & 1;
+  const int other = nb >> 1;
+  const int fo = side ? other : c;
+  const int fn = side ? c : other;
+  if (GK == 1) {
+    if (fn < 0) return false;
+    const double wx = am.e_gx[s], wy = am.e_gy[s];
+    const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
+    const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
+      const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
+      const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
+      const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
+      const double sv = wx * ax + wy * ay;
+      val[k] = side ? sv : -sv;
+    }
+  } else {
+    const double nax = am.e_gx[s], nay = am.e_gy[s];
+    const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
+    if (fn >= 0) {
+      const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
+        const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
+        val[k] = 0.5 * (nax * tx + nay * ty);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        val[k] = nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
+    }
+  }
+  return true;
+}
+
+// Wbar of cell c from the gradient adjoint of the cell and its face neighbours (one thread per cell).
+template <int GK>
+__device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __restrict__ Gbar, int c, double* wb) {
   const int lane = c % kEntryTile;
   const int t0 = am.e_tile[c / kEntryTile];
   const int ne = am.cf_off[c + 1] - am.cf_off[c];
 #pragma unroll
   for (int k = 0; k < 4; ++k) wb[k] = 0.0;
   for (int i = 0; i < ne; ++i) {
-    const int s = (t0 + i) * kEntryTile + lane;
-    const int nb = am.e_nb[s];
-    const int side = nb & 1;
-    const int other = nb >> 1;
-    const int fo = side ? other : c;
-    const int fn = side ? c : other;
-    if (GK == 1) {
-      if (fn < 0) continue;
-      const double wx = am.e_gx[s], wy = am.e_gy[s];
-      const double hxo = am.H[fo], hyo = am.H[ld + fo], hzo = am.H[2 * ld + fo];
-      const double hxn = am.H[fn], hyn = am.H[ld + fn], hzn = am.H[2 * ld + fn];
+    double val[4];
+    if (adj_wbar_entry<GK>(am, Gbar, c, (t0 + i) * kEntryTile + lane, val)) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double gxo = Gbar[(2 * k) * ld + fo], gyo = Gbar[(2 * k + 1) * ld + fo];
-        const double gxn = Gbar[(2 * k) * ld + fn], gyn = Gbar[(2 * k + 1) * ld + fn];
-        const double ax = hxo * gxo + hyo * gyo + hxn * gxn + hyn * gyn;
-        const double ay = hyo * gxo + hzo * gyo + hyn * gxn + hzn * gyn;
-        const double sv = wx * ax + wy * ay;
-        wb[k] += side ? sv : -sv;
-      }
-    } else {
-      const double nax = am.e_gx[s], nay = am.e_gy[s];
-      const double ivo = 1.0 / dmax(am.vol[fo], 1.0e-12);
-      if (fn >= 0) {
-        const double ivn = 1.0 / dmax(am.vol[fn], 1.0e-12);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double tx = ivo * Gbar[(2 * k) * ld + fo] - ivn * Gbar[(2 * k) * ld + fn];
-          const double ty = ivo * Gbar[(2 * k + 1) * ld + fo] - ivn * Gbar[(2 * k + 1) * ld + fn];
-          wb[k] += 0.5 * (nax * tx + nay * ty);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          wb[k] += nax * (ivo * Gbar[(2 * k) * ld + fo]) + nay * (ivo * Gbar[(2 * k + 1) * ld + fo]);
-      }
+      for (int k = 0; k < 4; ++k) wb[k] += val[k];
     }
   }
 }
@@ -568,172 +668,142 @@ __device__ __forceinline__ double adj_update(int ld, const C4& u0, const FaceOpt
   return sq;
 }
 
-template <int GK>
-__device__ __forceinline__ double adj_grad_update_cell(const AdjMesh& am, const double* __restrict__ U,
-                                                       const FaceOpts& o, const double* __restrict__ Gbar,
-                                                       const double* __restrict__ direct, int stage, int rk3,
-                                                       const double* __restrict__ x0, const double* __restrict__ xs,
-                                                       const double* __restrict__ rhs, const double* __restrict__ lam,
-                                                       double cfl, double* __restrict__ xout,
-                                                       double* __restrict__ psi, int c) {
-  double wb[4];
-  adj_wbar<GK>(am, Gbar, c, wb);
-  const C4 u0 = load_c4(U, am.ld, c);
-  double jac[4][4];
-  c2p_jacobian(u0, o.g, jac);
-  return adj_update(am.ld, u0, o, jac, wb, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, c);
-}
-
-// Fixed-order workgroup sum of one value per thread into *dst (thread 0 writes; dst may be null).
-__device__ __forceinline__ void block_sum_store(double v, double* red, double* dst) {
-  red[threadIdx.x] = v;
+// Fixed-order sum of one value per cell of the workgroup's entry group (threads below kEntryTile) into
+// *dst (thread 0 writes; dst may be null): the tree of a kEntryTile-thread workgroup.  All threads call it.
+__device__ __forceinline__ void group_sum_store(double v, double* red, double* dst) {
+  if (threadIdx.x < kEntryTile) red[threadIdx.x] = v;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
+  for (int w = kEntryTile / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0 && dst) *dst = red[0];
 }
 
-template <int GK>
-__global__ __launch_bounds__(kThreads) void k_adj_grad_update(AdjMesh am, const double* __restrict__ U, FaceOpts o,
-                                                              const double* __restrict__ Gbar,
-                                                              const double* __restrict__ direct, int stage, int rk3,
-                                                              const double* __restrict__ x0,
-                                                              const double* __restrict__ xs,
-                                                              const double* __restrict__ rhs,
-                                                              const double* __restrict__ lam, double cfl,
-                                                              double* __restrict__ xout, double* __restrict__ partials,
-                                                              double* __restrict__ psi) {
-  __shared__ double red[kThreads];
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  double sq = 0.0;
-  if (c < am.N)
-    sq = adj_grad_update_cell<GK>(am, U, o, Gbar, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, c);
-  block_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
+// Wbar of NS solves over entry group grp into acc[(4 q + k) kEntryTile + lane].
+template <int GK, int NS>
+__device__ __forceinline__ void adj_wbar_group(const AdjMesh& am, const double* const* Gbar, const WideCell& w,
+                                               double* acc) {
+  wide_accumulate<4 * NS>(acc, w, [&](int i, double* val) {
+    const int s = (w.t0 + i) * kEntryTile + w.lane;
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) any = adj_wbar_entry<GK>(am, Gbar[q], w.c, s, val + 4 * q) || any;
+    return any;
+  });
 }
 
-// ---- batched adjoint solves: one workgroup per (solve a, solve b or -1, block of kThreads cells) ------
+template <int GK>
+__global__ __launch_bounds__(kWideThreads) void k_adj_grad_update(AdjMesh am, const double* __restrict__ U,
+                                                                  FaceOpts o, const double* __restrict__ Gbar,
+                                                                  const double* __restrict__ direct, int stage,
+                                                                  int rk3, const double* __restrict__ x0,
+                                                                  const double* __restrict__ xs,
+                                                                  const double* __restrict__ rhs,
+                                                                  const double* __restrict__ lam, double cfl,
+                                                                  double* __restrict__ xout,
+                                                                  double* __restrict__ partials,
+                                                                  double* __restrict__ psi) {
+  __shared__ double acc[4 * kEntryTile];
+  __shared__ double red[kEntryTile];
+  const WideCell w = wide_cell(am, blockIdx.x);
+  const double* g[1] = {Gbar};
+  adj_wbar_group<GK, 1>(am, g, w, acc);
+  double sq = 0.0;
+  if (w.sub == 0 && w.live) {
+    double wb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wb[k] = acc[k * kEntryTile + w.lane];
+    const C4 u0 = load_c4(U, am.ld, w.c);
+    double jac[4][4];
+    c2p_jacobian(u0, o.g, jac);
+    sq = adj_update(am.ld, u0, o, jac, wb, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, w.c);
+  }
+  group_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
+}
+
+// ---- batched adjoint solves: one workgroup per (solve a, solve b or -1, entry group) -----------------
 // Two solves of one linearisation (same face Jacobians, state, options and CFL; e.g. the CL and CD
 // adjoints of a case) share a work item: one pass over the face Jacobians and one primitive-map Jacobian
 // per cell serve both.  Each solve runs the same per-cell arithmetic as a single solve, so every solve
 // of a batch reproduces its single solve bit for bit; solves that have stopped are neither computed nor
 // written.
-__global__ __launch_bounds__(kThreads) void k_adj_gather_batch(const AdjBatchSolve* __restrict__ solves,
-                                                               const AdjBatchCtl* __restrict__ ctl,
-                                                               const int4* __restrict__ work) {
-  const int4 w = work[blockIdx.x];
-  if (w.x < 0) return;
-  const bool aa = ctl[w.x].active != 0;
-  const bool ab = w.y >= 0 && ctl[w.y].active != 0;
-  if (!aa && !ab) return;
-  const AdjBatchSolve& sa = solves[aa ? w.x : w.y];
-  const int c = w.z * kThreads + threadIdx.x;
-  const AdjMesh& am = sa.am;
-  if (c >= am.N) return;
-  if (!(aa && ab)) {  // one running solve: the single-solve body
-    adj_gather_cell(am, sa.J, sa.psi, sa.direct, sa.Gbar, c);
-    return;
-  }
-  const AdjBatchSolve& sb = solves[w.y];
-  const int ld = am.ld;
-  const double* __restrict__ pa = sa.psi;
-  const double* __restrict__ pb = sb.psi;
-  const int lane = c % kEntryTile;
-  const int t0 = am.e_tile[c / kEntryTile];
-  const int ne = am.cf_off[c + 1] - am.cf_off[c];
-  double pca[4], pcb[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    pca[k] = pa[k * ld + c];
-    pcb[k] = pb[k * ld + c];
-  }
-  double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
-  double ga[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < ne; ++i) {
-    const int s = (t0 + i) * kEntryTile + lane;
-    const int nb = am.e_nb[s];
-    double wa[4], wv[4];
-    adj_entry_weight(pa, ld, c, pca, nb, wa);
-    adj_entry_weight(pb, ld, c, pcb, nb, wv);
-    const double* Jt = sa.J + static_cast<size_t>(t0 + i) * (kJacCols * kEntryTile) + lane;
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      double ta = 0.0, tb = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double j = Jt[(d * 4 + k) * kEntryTile];
-        ta += j * wa[k];
-        tb += j * wv[k];
-      }
-      ua[d] += ta;
-      ub[d] += tb;
-    }
-    const double dx = am.e_dx[s], dy = am.e_dy[s];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      double ta = 0.0, tb = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double j = Jt[(16 + d * 4 + k) * kEntryTile];
-        ta += j * wa[k];
-        tb += j * wv[k];
-      }
-      ga[2 * d] += ta * dx;
-      ga[2 * d + 1] += ta * dy;
-      gb[2 * d] += tb * dx;
-      gb[2 * d + 1] += tb * dy;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    sa.direct[k * ld + c] = ua[k];
-    sb.direct[k * ld + c] = ub[k];
-  }
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    sa.Gbar[j * ld + c] = ga[j];
-    sb.Gbar[j * ld + c] = gb[j];
+__global__ __launch_bounds__(kWideThreads) void k_adj_gather_batch(const AdjBatchSolve* __restrict__ solves,
+                                                                   const AdjBatchCtl* __restrict__ ctl,
+                                                                   const int4* __restrict__ work) {
+  __shared__ double acc[24 * kEntryTile];
+  const int4 wk = work[blockIdx.x];
+  if (wk.x < 0) return;
+  const bool aa = ctl[wk.x].active != 0;
+  const bool ab = wk.y >= 0 && ctl[wk.y].active != 0;
+  if (!aa && !ab) return;  // uniform over the workgroup
+  const AdjBatchSolve& sa = solves[aa ? wk.x : wk.y];
+  if (aa && ab) {
+    const AdjBatchSolve& sb = solves[wk.y];
+    const double* p[2] = {sa.psi, sb.psi};
+    double* o[2] = {sa.direct, sb.direct};
+    double* g[2] = {sa.Gbar, sb.Gbar};
+    adj_gather_group<2>(sa.am, sa.J, p, o, g, wk.z, acc);
+  } else {
+    const double* p[1] = {sa.psi};
+    double* o[1] = {sa.direct};
+    double* g[1] = {sa.Gbar};
+    adj_gather_group<1>(sa.am, sa.J, p, o, g, wk.z, acc);
   }
 }
 
 template <int GK>
-__global__ __launch_bounds__(kThreads) void k_adj_grad_update_batch(const AdjBatchSolve* __restrict__ solves,
-                                                                    const AdjBatchCtl* __restrict__ ctl,
-                                                                    const int4* __restrict__ work, int stage, int rk3,
-                                                                    int ix0, int ixs, int ixo, int want_partials) {
-  __shared__ double red[kThreads];
-  const int4 w = work[blockIdx.x];
-  if (w.x < 0) return;
-  const bool aa = ctl[w.x].active != 0;
-  const bool ab = w.y >= 0 && ctl[w.y].active != 0;
+__global__ __launch_bounds__(kWideThreads) void k_adj_grad_update_batch(const AdjBatchSolve* __restrict__ solves,
+                                                                        const AdjBatchCtl* __restrict__ ctl,
+                                                                        const int4* __restrict__ work, int stage,
+                                                                        int rk3, int ix0, int ixs, int ixo,
+                                                                        int want_partials) {
+  __shared__ double acc[8 * kEntryTile];
+  __shared__ double red[kEntryTile];
+  const int4 wk = work[blockIdx.x];
+  if (wk.x < 0) return;
+  const bool aa = ctl[wk.x].active != 0;
+  const bool ab = wk.y >= 0 && ctl[wk.y].active != 0;
   if (!aa && !ab) return;  // uniform over the workgroup
-  const AdjBatchSolve& s0 = solves[aa ? w.x : w.y];
+  const AdjBatchSolve& s0 = solves[aa ? wk.x : wk.y];
   const AdjMesh& am = s0.am;
-  const int c = w.z * kThreads + threadIdx.x;
+  const WideCell w = wide_cell(am, wk.z);
+  // acc rows 0..3: the first running solve's Wbar, rows 4..7 the second's when both run
+  if (aa && ab) {
+    const double* g[2] = {solves[wk.x].Gbar, solves[wk.y].Gbar};
+    adj_wbar_group<GK, 2>(am, g, w, acc);
+  } else {
+    const double* g[1] = {s0.Gbar};
+    adj_wbar_group<GK, 1>(am, g, w, acc);
+  }
   double sqa = 0.0, sqb = 0.0;
-  if (c < am.N) {
+  if (w.sub == 0 && w.live) {
+    const int c = w.c;
     const C4 u0 = load_c4(s0.U, am.ld, c);
     double jac[4][4];
     c2p_jacobian(u0, s0.o.g, jac);
     double wb[4];
+    int row = 0;
     if (aa) {
-      const AdjBatchSolve& s = solves[w.x];
-      adj_wbar<GK>(am, s.Gbar, c, wb);
+      const AdjBatchSolve& s = solves[wk.x];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wb[k] = acc[k * kEntryTile + w.lane];
       sqa = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
                        s.x[ixo], s.psi, c);
+      row = 4;
     }
     if (ab) {
-      const AdjBatchSolve& s = solves[w.y];
-      adj_wbar<GK>(am, s.Gbar, c, wb);
+      const AdjBatchSolve& s = solves[wk.y];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wb[k] = acc[(row + k) * kEntryTile + w.lane];
       sqb = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
                        s.x[ixo], s.psi, c);
     }
   }
   if (want_partials) {
-    if (aa) block_sum_store(sqa, red, solves[w.x].partials + w.z);
+    if (aa) group_sum_store(sqa, red, solves[wk.x].partials + wk.z);
     if (aa && ab) __syncthreads();
-    if (ab) block_sum_store(sqb, red, solves[w.y].partials + w.z);
+    if (ab) group_sum_store(sqb, red, solves[wk.y].partials + wk.z);
   }
 }
 
@@ -750,7 +820,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_batch_control(const AdjBatchSo
   double v = 0.0;
   for (int i = threadIdx.x; i < s.nparts; i += kThreads) v += s.partials[i];
   double sum = 0.0;
-  block_sum_store(v, red, &sum);
+  static_assert(kThreads == kEntryTile, "the control sum uses the entry-group tree");
+  group_sum_store(v, red, &sum);
   if (threadIdx.x != 0) return;
   const double res = sqrt(sum);
   k.iter += 1;
@@ -949,6 +1020,8 @@ using namespace aero_dev;
 
 namespace {
 inline unsigned blocks_for(long long n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+// Workgroups of the wide per-cell kernels: one per entry group of kEntryTile cells.
+inline unsigned entry_groups(int n) { return static_cast<unsigned>((n + kEntryTile - 1) / kEntryTile); }
 }  // namespace
 
 hipError_t launch_tan_gradient(const AdjMesh& am, const double* U, const double* dU, const GeoTan& gt, double g,
@@ -1028,7 +1101,7 @@ hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool
 
 hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
                               double* Gbar, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, J, psi, out, Gbar);
+  hipLaunchKernelGGL(k_adj_gather, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, out, Gbar);
   if (am.grad == 1)
     hipLaunchKernelGGL(k_adj_grad<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
   else
@@ -1046,12 +1119,12 @@ hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U,
                             const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
                             double* Gbar, double* direct, double* xout, double* partials, double* psi,
                             hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, J, psi, direct, Gbar);
+  hipLaunchKernelGGL(k_adj_gather, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, direct, Gbar);
   if (am.grad == 1)
-    hipLaunchKernelGGL(k_adj_grad_update<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, o, Gbar, direct,
+    hipLaunchKernelGGL(k_adj_grad_update<1>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, U, o, Gbar, direct,
                        stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
   else
-    hipLaunchKernelGGL(k_adj_grad_update<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, o, Gbar, direct,
+    hipLaunchKernelGGL(k_adj_grad_update<0>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, U, o, Gbar, direct,
                        stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
   return hipGetLastError();
 }
@@ -1059,12 +1132,12 @@ hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U,
 hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
                                   int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
                                   hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather_batch, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work);
+  hipLaunchKernelGGL(k_adj_gather_batch, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work);
   if (grad == 1)
-    hipLaunchKernelGGL(k_adj_grad_update_batch<1>, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work, stage, rk3,
+    hipLaunchKernelGGL(k_adj_grad_update_batch<1>, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work, stage, rk3,
                        ix0, ixs, ixo, want_partials);
   else
-    hipLaunchKernelGGL(k_adj_grad_update_batch<0>, dim3(nwork), dim3(kThreads), 0, s, solves, ctl, work, stage, rk3,
+    hipLaunchKernelGGL(k_adj_grad_update_batch<0>, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work, stage, rk3,
                        ix0, ixs, ixo, want_partials);
   return hipGetLastError();
 }

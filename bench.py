@@ -548,7 +548,7 @@ def main() -> int:
                     help="run the batched primal + adjoint workload over uneven meshes instead")
     ap.add_argument("--uneven-cases", type=int, default=16, help="--uneven: cases per mesh size")
     ap.add_argument("--uneven-meshes", default="16k,65k,262k", help="--uneven: mesh sizes (of 16k, 65k, 262k)")
-    ap.add_argument("--uneven-iters", type=int, default=100000, help="--uneven: iteration cap of the primal batch")
+    ap.add_argument("--uneven-iters", type=int, default=120000, help="--uneven: iteration cap of the primal batch")
     ap.add_argument("--uneven-seed", type=int, default=5150, help="--uneven: seed of shapes, Mach and AoA")
     ap.add_argument("--uneven-lowmach", choices=["precond", "plain"], default="precond",
                     help="--uneven: M 0.05 cases with the low-Mach preconditioner (its long runs diverge) or without")
