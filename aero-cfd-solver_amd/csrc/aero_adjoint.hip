@@ -320,12 +320,15 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
 // conserved-state columns give the direct part of R_U^T psi, the increment columns b_d = (dF/da_d)^T w
 // give the gradient adjoint Gbar[d] += b_d (face centre - cell centre).
 //
-// The per-cell kernels run kWide threads per cell (kWideThreads per workgroup: entry lane `sub` of the
-// cells of one entry group handles entries sub, sub + kWide, ...), so a small mesh still fills the chip.
-// wide_accumulate adds the entries' values into LDS one entry lane at a time, in increasing entry order,
-// so every per-cell sum is that of a serial loop over the cell's entries, bit for bit.
+// The per-cell kernels come in two launch shapes with the same sums.  One workgroup covers one entry
+// group of kEntryTile cells, either with one thread per cell (serial over the cell's entries), or, WIDE,
+// with kWide threads per cell (kWideThreads per workgroup: entry lane `sub` handles entries sub,
+// sub + kWide, ...) so that a launch of few groups still fills the chip.  wide_accumulate adds the
+// entries' values into LDS one entry lane at a time, in increasing entry order, so every per-cell sum is
+// that of the serial loop, bit for bit.  The launchers take WIDE below kWideBelow groups.
 constexpr int kWide = 4;
 constexpr int kWideThreads = kWide * kEntryTile;
+constexpr int kWideBelow = 1024;
 
 __device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi, int ld, const double* pc, int nb,
                                                  double* w) {
@@ -339,16 +342,17 @@ __device__ __forceinline__ void adj_entry_weight(const double* __restrict__ psi,
   }
 }
 
-// Thread layout of the wide kernels over entry group `grp`.
+// The thread's cell and entry lane in entry group `grp` (sub = 0 in the one-thread-per-cell shape).
 struct WideCell {
   int lane, sub, c, ne, t0, tiles;
   bool live;
 };
 
+template <bool WIDE>
 __device__ __forceinline__ WideCell wide_cell(const AdjMesh& am, int grp) {
   WideCell w;
-  w.lane = threadIdx.x % kEntryTile;
-  w.sub = threadIdx.x / kEntryTile;
+  w.lane = WIDE ? threadIdx.x % kEntryTile : threadIdx.x;
+  w.sub = WIDE ? threadIdx.x / kEntryTile : 0;
   w.c = grp * kEntryTile + w.lane;
   w.live = w.c < am.N;
   w.ne = w.live ? am.cf_off[w.c + 1] - am.cf_off[w.c] : 0;
@@ -427,45 +431,68 @@ __device__ __forceinline__ bool adj_gather_entry(const AdjMesh& am, const double
 }
 
 // The gather of entry group grp for NS solves: direct[q] = the direct part of R_U^T psi[q], Gbar[q] its
-// gradient adjoint.  acc: 12 NS kEntryTile doubles of LDS.
-template <int NS>
+// gradient adjoint.  acc (WIDE): 12 NS kEntryTile doubles of LDS.
+template <bool WIDE, int NS>
 __device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double* __restrict__ J,
                                                  const double* const* psi, double* const* direct,
                                                  double* const* Gbar, int grp, double* acc) {
-  const WideCell w = wide_cell(am, grp);
+  const WideCell w = wide_cell<WIDE>(am, grp);
   const int ld = am.ld;
   double pc[NS][4];
 #pragma unroll
   for (int q = 0; q < NS; ++q)
 #pragma unroll
     for (int k = 0; k < 4; ++k) pc[q][k] = w.live ? psi[q][k * ld + w.c] : 0.0;
-  wide_accumulate<12 * NS>(acc, w, [&](int i, double* val) {
-    return adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
-  });
-  if (!w.live) return;
+  if (WIDE) {
+    wide_accumulate<12 * NS>(acc, w, [&](int i, double* val) {
+      return adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
+    });
+    if (!w.live) return;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
+    for (int q = 0; q < NS; ++q) {
 #pragma unroll
-    for (int j = 0; j < 12; ++j) {
-      const int v = 12 * q + j;
-      if (v % kWide != w.sub) continue;  // the entry lanes share the stores
-      const double x = acc[v * kEntryTile + w.lane];
-      if (j < 4)
-        direct[q][j * ld + w.c] = x;
-      else
-        Gbar[q][(j - 4) * ld + w.c] = x;
+      for (int j = 0; j < 12; ++j) {
+        const int v = 12 * q + j;
+        if (v % kWide != w.sub) continue;  // the entry lanes share the stores
+        const double x = acc[v * kEntryTile + w.lane];
+        if (j < 4)
+          direct[q][j * ld + w.c] = x;
+        else
+          Gbar[q][(j - 4) * ld + w.c] = x;
+      }
+    }
+  } else {
+    if (!w.live) return;
+    double sum[12 * NS];
+#pragma unroll
+    for (int v = 0; v < 12 * NS; ++v) sum[v] = 0.0;
+    for (int i = 0; i < w.ne; ++i) {
+      double val[12 * NS];
+      adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
+#pragma unroll
+      for (int v = 0; v < 12 * NS; ++v) sum[v] += val[v];
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) direct[q][j * ld + w.c] = sum[12 * q + j];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Gbar[q][j * ld + w.c] = sum[12 * q + 4 + j];
     }
   }
 }
 
-__global__ __launch_bounds__(kWideThreads) void k_adj_gather(AdjMesh am, const double* __restrict__ J,
-                                                             const double* __restrict__ psi, double* __restrict__ out,
-                                                             double* __restrict__ Gbar) {
-  __shared__ double acc[12 * kEntryTile];
+template <bool WIDE>
+__global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather(AdjMesh am,
+                                                                                 const double* __restrict__ J,
+                                                                                 const double* __restrict__ psi,
+                                                                                 double* __restrict__ out,
+                                                                                 double* __restrict__ Gbar) {
+  __shared__ double acc[WIDE ? 12 * kEntryTile : 1];
   const double* p[1] = {psi};
   double* o[1] = {out};
   double* g[1] = {Gbar};
-  adj_gather_group<1>(am, J, p, o, g, blockIdx.x, acc);
+  adj_gather_group<WIDE, 1>(am, J, p, o, g, blockIdx.x, acc);
 }
 
 // The gradient-adjoint term of one face entry of cell c, val[k] for primitive k (returns false where the
@@ -680,44 +707,63 @@ __device__ __forceinline__ void group_sum_store(double v, double* red, double* d
   if (threadIdx.x == 0 && dst) *dst = red[0];
 }
 
-// Wbar of NS solves over entry group grp into acc[(4 q + k) kEntryTile + lane].
-template <int GK, int NS>
-__device__ __forceinline__ void adj_wbar_group(const AdjMesh& am, const double* const* Gbar, const WideCell& w,
-                                               double* acc) {
-  wide_accumulate<4 * NS>(acc, w, [&](int i, double* val) {
-    const int s = (w.t0 + i) * kEntryTile + w.lane;
-    bool any = false;
+// Wbar of NS solves for the thread's cell, wb[q] (set where w.sub == 0 and the cell is live).  All threads
+// of the workgroup call it.  acc (WIDE): 4 NS kEntryTile doubles of LDS.
+template <bool WIDE, int GK, int NS>
+__device__ __forceinline__ void adj_wbar_cells(const AdjMesh& am, const double* const* Gbar, const WideCell& w,
+                                               double* acc, double (*wb)[4]) {
+  if (WIDE) {
+    wide_accumulate<4 * NS>(acc, w, [&](int i, double* val) {
+      const int s = (w.t0 + i) * kEntryTile + w.lane;
+      bool any = false;  // the same for every solve: it depends on the face only
 #pragma unroll
-    for (int q = 0; q < NS; ++q) any = adj_wbar_entry<GK>(am, Gbar[q], w.c, s, val + 4 * q) || any;
-    return any;
-  });
+      for (int q = 0; q < NS; ++q) any = adj_wbar_entry<GK>(am, Gbar[q], w.c, s, val + 4 * q) || any;
+      return any;
+    });
+    if (w.sub == 0 && w.live) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wb[q][k] = acc[(4 * q + k) * kEntryTile + w.lane];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wb[q][k] = 0.0;
+    if (!w.live) return;
+    for (int i = 0; i < w.ne; ++i) {
+      const int s = (w.t0 + i) * kEntryTile + w.lane;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        double val[4];
+        if (adj_wbar_entry<GK>(am, Gbar[q], w.c, s, val)) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wb[q][k] += val[k];
+        }
+      }
+    }
+  }
 }
 
-template <int GK>
-__global__ __launch_bounds__(kWideThreads) void k_adj_grad_update(AdjMesh am, const double* __restrict__ U,
-                                                                  FaceOpts o, const double* __restrict__ Gbar,
-                                                                  const double* __restrict__ direct, int stage,
-                                                                  int rk3, const double* __restrict__ x0,
-                                                                  const double* __restrict__ xs,
-                                                                  const double* __restrict__ rhs,
-                                                                  const double* __restrict__ lam, double cfl,
-                                                                  double* __restrict__ xout,
-                                                                  double* __restrict__ partials,
-                                                                  double* __restrict__ psi) {
-  __shared__ double acc[4 * kEntryTile];
+template <bool WIDE, int GK>
+__global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_update(
+    AdjMesh am, const double* __restrict__ U, FaceOpts o, const double* __restrict__ Gbar,
+    const double* __restrict__ direct, int stage, int rk3, const double* __restrict__ x0,
+    const double* __restrict__ xs, const double* __restrict__ rhs, const double* __restrict__ lam, double cfl,
+    double* __restrict__ xout, double* __restrict__ partials, double* __restrict__ psi) {
+  __shared__ double acc[WIDE ? 4 * kEntryTile : 1];
   __shared__ double red[kEntryTile];
-  const WideCell w = wide_cell(am, blockIdx.x);
+  const WideCell w = wide_cell<WIDE>(am, blockIdx.x);
   const double* g[1] = {Gbar};
-  adj_wbar_group<GK, 1>(am, g, w, acc);
+  double wb[1][4];
+  adj_wbar_cells<WIDE, GK, 1>(am, g, w, acc, wb);
   double sq = 0.0;
   if (w.sub == 0 && w.live) {
-    double wb[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) wb[k] = acc[k * kEntryTile + w.lane];
     const C4 u0 = load_c4(U, am.ld, w.c);
     double jac[4][4];
     c2p_jacobian(u0, o.g, jac);
-    sq = adj_update(am.ld, u0, o, jac, wb, direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, w.c);
+    sq = adj_update(am.ld, u0, o, jac, wb[0], direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, w.c);
   }
   group_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
 }
@@ -728,10 +774,10 @@ __global__ __launch_bounds__(kWideThreads) void k_adj_grad_update(AdjMesh am, co
 // per cell serve both.  Each solve runs the same per-cell arithmetic as a single solve, so every solve
 // of a batch reproduces its single solve bit for bit; solves that have stopped are neither computed nor
 // written.
-__global__ __launch_bounds__(kWideThreads) void k_adj_gather_batch(const AdjBatchSolve* __restrict__ solves,
-                                                                   const AdjBatchCtl* __restrict__ ctl,
-                                                                   const int4* __restrict__ work) {
-  __shared__ double acc[24 * kEntryTile];
+template <bool WIDE>
+__global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather_batch(
+    const AdjBatchSolve* __restrict__ solves, const AdjBatchCtl* __restrict__ ctl, const int4* __restrict__ work) {
+  __shared__ double acc[WIDE ? 24 * kEntryTile : 1];
   const int4 wk = work[blockIdx.x];
   if (wk.x < 0) return;
   const bool aa = ctl[wk.x].active != 0;
@@ -743,22 +789,20 @@ __global__ __launch_bounds__(kWideThreads) void k_adj_gather_batch(const AdjBatc
     const double* p[2] = {sa.psi, sb.psi};
     double* o[2] = {sa.direct, sb.direct};
     double* g[2] = {sa.Gbar, sb.Gbar};
-    adj_gather_group<2>(sa.am, sa.J, p, o, g, wk.z, acc);
+    adj_gather_group<WIDE, 2>(sa.am, sa.J, p, o, g, wk.z, acc);
   } else {
     const double* p[1] = {sa.psi};
     double* o[1] = {sa.direct};
     double* g[1] = {sa.Gbar};
-    adj_gather_group<1>(sa.am, sa.J, p, o, g, wk.z, acc);
+    adj_gather_group<WIDE, 1>(sa.am, sa.J, p, o, g, wk.z, acc);
   }
 }
 
-template <int GK>
-__global__ __launch_bounds__(kWideThreads) void k_adj_grad_update_batch(const AdjBatchSolve* __restrict__ solves,
-                                                                        const AdjBatchCtl* __restrict__ ctl,
-                                                                        const int4* __restrict__ work, int stage,
-                                                                        int rk3, int ix0, int ixs, int ixo,
-                                                                        int want_partials) {
-  __shared__ double acc[8 * kEntryTile];
+template <bool WIDE, int GK>
+__global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_update_batch(
+    const AdjBatchSolve* __restrict__ solves, const AdjBatchCtl* __restrict__ ctl, const int4* __restrict__ work,
+    int stage, int rk3, int ix0, int ixs, int ixo, int want_partials) {
+  __shared__ double acc[WIDE ? 8 * kEntryTile : 1];
   __shared__ double red[kEntryTile];
   const int4 wk = work[blockIdx.x];
   if (wk.x < 0) return;
@@ -767,14 +811,15 @@ __global__ __launch_bounds__(kWideThreads) void k_adj_grad_update_batch(const Ad
   if (!aa && !ab) return;  // uniform over the workgroup
   const AdjBatchSolve& s0 = solves[aa ? wk.x : wk.y];
   const AdjMesh& am = s0.am;
-  const WideCell w = wide_cell(am, wk.z);
-  // acc rows 0..3: the first running solve's Wbar, rows 4..7 the second's when both run
+  const WideCell w = wide_cell<WIDE>(am, wk.z);
+  // wb[0]: the first running solve's Wbar, wb[1] the second's when both run
+  double wb[2][4];
   if (aa && ab) {
     const double* g[2] = {solves[wk.x].Gbar, solves[wk.y].Gbar};
-    adj_wbar_group<GK, 2>(am, g, w, acc);
+    adj_wbar_cells<WIDE, GK, 2>(am, g, w, acc, wb);
   } else {
     const double* g[1] = {s0.Gbar};
-    adj_wbar_group<GK, 1>(am, g, w, acc);
+    adj_wbar_cells<WIDE, GK, 1>(am, g, w, acc, wb);
   }
   double sqa = 0.0, sqb = 0.0;
   if (w.sub == 0 && w.live) {
@@ -782,22 +827,15 @@ __global__ __launch_bounds__(kWideThreads) void k_adj_grad_update_batch(const Ad
     const C4 u0 = load_c4(s0.U, am.ld, c);
     double jac[4][4];
     c2p_jacobian(u0, s0.o.g, jac);
-    double wb[4];
-    int row = 0;
     if (aa) {
       const AdjBatchSolve& s = solves[wk.x];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) wb[k] = acc[k * kEntryTile + w.lane];
-      sqa = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
+      sqa = adj_update(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
                        s.x[ixo], s.psi, c);
-      row = 4;
     }
     if (ab) {
       const AdjBatchSolve& s = solves[wk.y];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) wb[k] = acc[(row + k) * kEntryTile + w.lane];
-      sqb = adj_update(am.ld, u0, s.o, jac, wb, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
-                       s.x[ixo], s.psi, c);
+      sqb = adj_update(am.ld, u0, s.o, jac, wb[aa ? 1 : 0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+                       s.lam, s.cfl, s.x[ixo], s.psi, c);
     }
   }
   if (want_partials) {
@@ -1101,7 +1139,10 @@ hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool
 
 hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
                               double* Gbar, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, out, Gbar);
+  if (entry_groups(am.N) < kWideBelow)
+    hipLaunchKernelGGL(k_adj_gather<true>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, out, Gbar);
+  else
+    hipLaunchKernelGGL(k_adj_gather<false>, dim3(entry_groups(am.N)), dim3(kEntryTile), 0, s, am, J, psi, out, Gbar);
   if (am.grad == 1)
     hipLaunchKernelGGL(k_adj_grad<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
   else
@@ -1115,30 +1156,53 @@ hipError_t launch_objective_gradient(const AdjMesh& am, const double* U, double 
   return hipGetLastError();
 }
 
+template <bool WIDE>
+static void adj_stage_t(const AdjMesh& am, const double* J, const double* U, const FaceOpts& o, int stage, int rk3,
+                        const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
+                        double* Gbar, double* direct, double* xout, double* partials, double* psi, hipStream_t s) {
+  const dim3 grid(entry_groups(am.N)), block(WIDE ? kWideThreads : kEntryTile);
+  hipLaunchKernelGGL(k_adj_gather<WIDE>, grid, block, 0, s, am, J, psi, direct, Gbar);
+  if (am.grad == 1)
+    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 1>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0, xs,
+                       rhs, lam, cfl, xout, partials, psi);
+  else
+    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 0>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0, xs,
+                       rhs, lam, cfl, xout, partials, psi);
+}
+
 hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U, const FaceOpts& o, int stage, int rk3,
                             const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
                             double* Gbar, double* direct, double* xout, double* partials, double* psi,
                             hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, direct, Gbar);
-  if (am.grad == 1)
-    hipLaunchKernelGGL(k_adj_grad_update<1>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, U, o, Gbar, direct,
-                       stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
+  if (entry_groups(am.N) < kWideBelow)
+    adj_stage_t<true>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
   else
-    hipLaunchKernelGGL(k_adj_grad_update<0>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, U, o, Gbar, direct,
-                       stage, rk3, x0, xs, rhs, lam, cfl, xout, partials, psi);
+    adj_stage_t<false>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
   return hipGetLastError();
 }
 
+template <bool WIDE>
+static void adj_batch_stage_t(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
+                              int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
+                              hipStream_t s) {
+  const dim3 grid(nwork), block(WIDE ? kWideThreads : kEntryTile);
+  hipLaunchKernelGGL(k_adj_gather_batch<WIDE>, grid, block, 0, s, solves, ctl, work);
+  if (grad == 1)
+    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 1>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0, ixs,
+                       ixo, want_partials);
+  else
+    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 0>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0, ixs,
+                       ixo, want_partials);
+}
+
+// Work items are (solve pair, entry group): the four-lanes-per-cell shape below kWideBelow of them.
 hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
                                   int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
                                   hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather_batch, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work);
-  if (grad == 1)
-    hipLaunchKernelGGL(k_adj_grad_update_batch<1>, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work, stage, rk3,
-                       ix0, ixs, ixo, want_partials);
+  if (nwork < kWideBelow)
+    adj_batch_stage_t<true>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
   else
-    hipLaunchKernelGGL(k_adj_grad_update_batch<0>, dim3(nwork), dim3(kWideThreads), 0, s, solves, ctl, work, stage, rk3,
-                       ix0, ixs, ixo, want_partials);
+    adj_batch_stage_t<false>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
   return hipGetLastError();
 }
 

@@ -524,6 +524,40 @@ def replace_obj(obj, w_cl, w_cd=0.0, w_cm=0.0):
     return o
 
 
+def run_vjp(args) -> int:
+    """The adjoint product R_U^T psi (aero_hip_residual_vjp: face Jacobians, then the gather and gradient
+    transpose kernels) on the jittered mixed mesh of --vjp's size, HLLC + venkat at a perturbed M 0.6
+    state; --vjp-calls calls.  Meant to run under rocprofv3 for the per-kernel times; the line reports
+    the wall time per call."""
+    import numpy as np
+    import aero_hip
+    from aero_hip import HipMesh, freestream, generate_mesh, residual_config
+
+    if not aero_hip.hip_available():
+        raise SystemExit("bench.py: no gfx950 device visible")
+    nc, nr = UNEVEN_MESHES[args.vjp]
+    m = generate_mesh(precision=8, num_circumferential=nc, num_radial=nr, farfield_radius=20.0,
+                      radial_stretch=200.0, cells="mixed", quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    ff = freestream(0.6, 2.0)[1]
+    U = perturbed_state(m.cell_center, ff)
+    cfg = residual_config(ff, flux="hllc", limiter="venkat", second_order=True, stab_mach_floor=0.02)
+    psi = np.random.default_rng(1).standard_normal((m.num_cells, 5))
+    walls = []
+    with HipMesh(m, block_cells=args.block_cells) as hm:
+        hm.set_state(U)
+        for i in range(args.vjp_calls):
+            t0 = time.time()
+            out = hm.residual_vjp(cfg, psi)
+            walls.append(time.time() - t0)
+            log(f"[vjp] call {i}: {walls[-1] * 1e3:.2f} ms")
+    assert np.all(np.isfinite(out))
+    print(json.dumps({"metric": "adjoint_product_ms", "value": round(1e3 * float(np.median(walls)), 4), "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": f"vjp_mixed{args.vjp}_hllc_venkat", "cells": m.num_cells,
+                                 "faces": m.num_faces, "calls": args.vjp_calls}}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -562,6 +596,10 @@ def main() -> int:
     ap.add_argument("--fd-iters", type=int, default=120000, help="--adjoint: continuation iterations per FD run")
     ap.add_argument("--fd-cfl", type=float, default=2.0,
                     help="--adjoint: fixed CFL of the FD continuation runs (they have no CFL adaptation)")
+    ap.add_argument("--vjp", choices=sorted(UNEVEN_MESHES), default=None,
+                    help="instead of the benchmark, time --vjp-calls adjoint products on that mixed mesh (for "
+                         "rocprofv3 kernel statistics)")
+    ap.add_argument("--vjp-calls", type=int, default=12)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -576,6 +614,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
         return 0
+    if args.vjp:
+        return run_vjp(args)
     if args.sweep:
         return run_sweep(args)
     if args.adjoint:
