@@ -159,7 +159,8 @@ int adjoint_setup_steps(aero_hip_mesh* m, char* err, size_t errlen) {
   std::vector<int> e_tile(ngroups + 1, 0);
   for (int b = 0; b < ngroups; ++b) {
     int deg = 0;
-    for (int c = b * kEntryTile; c < std::min(N, (b + 1) * kEntryTile); ++c) deg = std::max(deg, cf_off[c + 1] - cf_off[c]);
+    for (int c = b * kEntryTile; c < std::min(N, (b + 1) * kEntryTile); ++c)
+      deg = std::max(deg, cf_off[c + 1] - cf_off[c]);
     e_tile[b + 1] = e_tile[b] + deg;
   }
   if (static_cast<long long>(e_tile[ngroups]) * kEntryTile > 0x7fffffffLL) {
@@ -173,7 +174,8 @@ int adjoint_setup_steps(aero_hip_mesh* m, char* err, size_t errlen) {
     const int rc = m->perm[c];
     for (int e = cf_off[c]; e < cf_off[c + 1]; ++e) {
       const int f = cf_ent[e] >> 1, side = cf_ent[e] & 1;
-      const size_t sl = static_cast<size_t>(e_tile[c / kEntryTile] + (e - cf_off[c])) * kEntryTile + c % kEntryTile;
+      const size_t sl =
+          static_cast<size_t>(e_tile[c / kEntryTile] + (e - cf_off[c])) * kEntryTile + c % kEntryTile;
       const int other = side ? fo[f] : fn[f];
       e_nb[sl] = other >= 0 ? (other << 1) | side : -2;
       if (m->dm.grad == 1) {
