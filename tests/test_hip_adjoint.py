@@ -320,6 +320,60 @@ def test_adjoint_batch_matches_single_solves(hip, rk_stages):
                 assert _maxrel(psis[i], psi) <= 1e-8
 
 
+def _mixed(nc, nr):
+    return helpers.mesh("256x64", num_circumferential=nc, num_radial=nr, farfield_radius=20.0, radial_stretch=200.0,
+                        cells="mixed", quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+
+
+@pytest.mark.parametrize("gradient", ["green_gauss", "least_squares"])
+def test_adjoint_launch_shapes_agree(hip, gradient):
+    """The adjoint kernels run four lanes per cell below 1024 work items and one thread per cell from 1024
+    up, with the same per-cell sums.  Four solves with their own CFLs on a 66k-cell mesh (4 x 258 work
+    items: one thread per cell) against the same solves alone (258 entry groups: four lanes per cell),
+    bitwise after 300 RK3 iterations."""
+    m = _mixed(256, 144)
+    assert m.num_cells == 66048
+    ff, U = _state(m, 0.5, 1.25, 0.02, 20260101)
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 0, 0, 0, 0), ff))
+    rhs = np.random.default_rng(5).standard_normal(U.shape)
+    rhs[:, 3] = 0.0
+    kw = dict(rk_stages=3, max_iters=300, tol=1e-30)
+    with hip.HipMesh(m, gradient=gradient) as hm:
+        solves = [(hm, cfg, rhs, U, cfl) for cfl in (0.5, 0.55, 0.6, 0.65)]
+        say(f"[adj-shapes] {gradient}: batched, one thread per cell")
+        psis, infos, _ = hip.adjoint_solve_batch(solves, iters_per_sync=100, **kw)
+        for i, (_, _, _, _, cfl) in enumerate(solves):
+            say(f"[adj-shapes] {gradient}: single solve at CFL {cfl}, four lanes per cell")
+            psi, info = hm.adjoint_solve(cfg, rhs, U, cfl=cfl, check_every=1, **kw)
+            assert not infos[i]["failed"] and infos[i]["iterations"] == info["iterations"] == 300, (infos[i], info)
+            assert np.all(np.isfinite(psi))
+            assert np.array_equal(psis[i], psi), f"solve {i}: launch shapes differ"
+
+
+@pytest.mark.parametrize("gradient", ["green_gauss", "least_squares"])
+def test_adjoint_product_one_thread_per_cell(hip, gradient):
+    """The adjoint product in its one-thread-per-cell shape (262k cells: 1024 entry groups) against the
+    tangent product: psi . (R_U v) = (R_U^T psi) . v."""
+    m = _mixed(512, 271)
+    assert m.num_cells == 262144
+    ff, U = _state(m, 0.5, 1.25, 0.02, 20260101)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(U.shape) * np.abs(U).max(axis=0) * 1e-3
+    v[:, 3] = 0.0
+    psi = rng.standard_normal(U.shape)
+    psi[:, 3] = 0.0
+    with hip.HipMesh(m, gradient=gradient) as hm:
+        for combo in [(1, 1, 1, 0, 0, 0, 0), (0, 0, 1, 1, 1, 1, 1)]:
+            cfg = helpers.res_cfg(cases.residual_cfg_values(combo, ff))
+            say(f"[adj-262k] {gradient} combo={combo}")
+            _, Jv = hm.residual_jvp(cfg, U, v)
+            JTpsi = hm.residual_vjp(cfg, psi, U)
+            a = float(np.sum(psi * Jv))
+            b = float(np.sum(JTpsi * v))
+            scale = float(np.sum(np.abs(psi * Jv)))
+            assert abs(a - b) <= 1e-12 * scale, (combo, a, b)
+
+
 def test_sensitivities_batch_matches_single_case_sensitivities(hip):
     """sensitivities_batch (one batched solve for every case and objective) against sensitivities per
     case and objective, with shape tangents on a deformed mesh."""
