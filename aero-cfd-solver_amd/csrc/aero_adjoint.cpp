@@ -352,6 +352,7 @@ FaceOptsT<Dual> dual_opts(const FaceOpts& o, const aero_tangent_seed* seed) {
   d.as_fmin = o.as_fmin;
   d.as_w = o.as_w;
   d.frozen = o.frozen;
+  d.frozen_slots = o.frozen_slots;
   return d;
 }
 

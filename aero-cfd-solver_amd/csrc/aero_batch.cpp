@@ -245,9 +245,12 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     cs.lam = nullptr;
     AERO_BSETUP(balloc(b, &cs.partials, 4 * static_cast<size_t>(m->nblk)));
     cs.lim_ratio = nullptr;
+    cs.lim_slot = nullptr;
     if (c->limiter_freeze_iter >= 0) {  // frozen limiter (beyond the reference); boundary faces keep 0
       AERO_BSETUP(balloc(b, &cs.lim_ratio, 8 * static_cast<size_t>(m->F)));
       AERO_BSETUP(hipMemset(cs.lim_ratio, 0, 8 * static_cast<size_t>(m->F) * sizeof(double)));
+      AERO_BSETUP(balloc(b, &cs.lim_slot, 8 * static_cast<size_t>(m->S)));
+      AERO_BSETUP(hipMemset(cs.lim_slot, 0, 8 * static_cast<size_t>(m->S) * sizeof(double)));
     }
     AERO_BSETUP(balloc(b, &cs.diag, 4));
     // freestream of this case (euler_solver.cpp:543-561) and its schedules
@@ -437,6 +440,10 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
       AERO_TRY(aero_host::launch_batch_limiter_capture(b->BC, c.limiter, b->d_geoms, b->d_cases, b->d_ctl, b->d_work,
                                                        b->nwork, c.gamma, start, b->stream),
                "k_limiter_capture_batch");
+      for (const BatchCase& cs : b->h_cases)  // the flux kernels' slot-order copy of each case's ratios
+        if (cs.lim_ratio)
+          AERO_TRY(aero_host::launch_ratio_slots(b->geoms[cs.geom]->dm, cs.lim_ratio, cs.lim_slot, b->stream),
+                   "k_ratio_slots");
       b->frozen = true;
     }
     int phase_end = second ? max_iters : std::min(startup - done_iters, max_iters);

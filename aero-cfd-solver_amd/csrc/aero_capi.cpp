@@ -115,6 +115,23 @@ template <class T> hipError_t upload(aero_hip_mesh* m, T** p, const std::vector<
   return e;
 }
 
+// The frozen-ratio buffers of a handle: face order [8][F] and the flux kernels' slot order [8][S].
+int ensure_ratio_buffers(aero_hip_mesh* m, const char* who, char* err, size_t errlen) {
+  for (auto [pp, n] : {std::pair<double**, size_t>{&m->lim_ratio, 8 * static_cast<size_t>(m->F)},
+                       std::pair<double**, size_t>{&m->lim_slot, 8 * static_cast<size_t>(m->S)}}) {
+    if (*pp) continue;
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, n * sizeof(double));
+    if (e != hipSuccess) {
+      set_err(err, errlen, std::string(who) + ": hipMalloc: " + hipGetErrorString(e));
+      return AERO_EHIP;
+    }
+    m->allocs.push_back(q);
+    *pp = static_cast<double*>(q);
+  }
+  return AERO_OK;
+}
+
 void free_events(aero_hip_mesh* m) {
   for (hipEvent_t e : m->tev) (void)hipEventDestroy(e);
   m->tev.clear();
@@ -163,6 +180,7 @@ FaceOpts face_opts(const aero_residual_config* c) {
   o.as_fmin = c->all_speed_f_min;
   o.as_w = c->all_speed_ramp;
   o.frozen = nullptr;
+  o.frozen_slots = nullptr;
   return o;
 }
 
@@ -174,6 +192,7 @@ int mesh_face_opts(const aero_hip_mesh* m, const aero_residual_config* c, FaceOp
       return AERO_EINVAL;
     }
     o->frozen = m->lim_ratio;
+    o->frozen_slots = m->lim_slot;
   }
   return AERO_OK;
 }
@@ -470,6 +489,7 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   dm.smax = std::max(smax, 1);
   dm.xcd_remap = renumber ? 1 : 0;
   dm.grad = grad;
+  dm.nslots = m->S;
   dm.grid = dm.xcd_remap ? ((m->nblk + 7) / 8) * 8 : m->nblk;
   dm.slot_off = p_slot_off;
   dm.s_o = p_so;
@@ -745,17 +765,13 @@ int aero_hip_freeze_limiter(aero_hip_mesh* m, const aero_residual_config* cfg, c
     if (rc) return rc;
   }
   const size_t n = 8 * static_cast<size_t>(m->F);
-  if (!m->lim_ratio) {
-    void* q = nullptr;
-    AERO_HIP_TRY(hipMalloc(&q, n * sizeof(double)), "aero_hip_freeze_limiter: hipMalloc");
-    m->allocs.push_back(q);
-    m->lim_ratio = static_cast<double*>(q);
-  }
+  if (const int rc = ensure_ratio_buffers(m, "aero_hip_freeze_limiter", err, errlen)) return rc;
   const double* U = m->U[m->cur];
   AERO_HIP_TRY(hipMemsetAsync(m->lim_ratio, 0, n * sizeof(double), m->stream), "aero_hip_freeze_limiter");
   AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, cfg->gamma, m->stream), "k_gradient");
   AERO_HIP_TRY(aero_host::launch_limiter_capture(m->dm, cfg->limiter, U, m->G, cfg->gamma, m->lim_ratio, m->stream),
                "k_limiter_capture");
+  AERO_HIP_TRY(aero_host::launch_ratio_slots(m->dm, m->lim_ratio, m->lim_slot, m->stream), "k_ratio_slots");
   if (ratios)
     AERO_HIP_TRY(hipMemcpyAsync(ratios, m->lim_ratio, n * sizeof(double), hipMemcpyDeviceToHost, m->stream),
                  "aero_hip_freeze_limiter: copy");
@@ -770,14 +786,10 @@ int aero_hip_set_limiter_ratios(aero_hip_mesh* m, const double* ratios, char* er
     return AERO_EINVAL;
   }
   const size_t n = 8 * static_cast<size_t>(m->F);
-  if (!m->lim_ratio) {
-    void* q = nullptr;
-    AERO_HIP_TRY(hipMalloc(&q, n * sizeof(double)), "aero_hip_set_limiter_ratios: hipMalloc");
-    m->allocs.push_back(q);
-    m->lim_ratio = static_cast<double*>(q);
-  }
+  if (const int rc = ensure_ratio_buffers(m, "aero_hip_set_limiter_ratios", err, errlen)) return rc;
   AERO_HIP_TRY(hipMemcpyAsync(m->lim_ratio, ratios, n * sizeof(double), hipMemcpyHostToDevice, m->stream),
                "aero_hip_set_limiter_ratios: copy");
+  AERO_HIP_TRY(aero_host::launch_ratio_slots(m->dm, m->lim_ratio, m->lim_slot, m->stream), "k_ratio_slots");
   AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_set_limiter_ratios: sync");
   m->lim_valid = true;
   return AERO_OK;

@@ -24,6 +24,7 @@ struct DevMesh {
   int grid;      // workgroups launched (>= nblk when XCD-remapped)
   int xcd_remap;
   int grad;      // AERO_GRAD_*: 0 Green-Gauss, 1 weighted least squares
+  int nslots;    // face slots S over all blocks
   const int* slot_off;  // [nblk + 1]
   const int* s_o;       // [S] owner (internal id)
   const int* s_n;       // [S] neighbour (internal id) or -1
@@ -81,6 +82,7 @@ struct BatchCase {
   int cfl_ramp, floor_ramp, as_on, as_len;  // ramp lengths and the all-speed ramp start
   WallArgs wall;     // wall faces of the geometry + this case's reference values
   double* lim_ratio; // frozen limiter ratios [8][F] (beyond the reference) or null
+  double* lim_slot;  // the same per face slot of the geometry [8][S] (the flux kernel's coalesced copy)
 };
 
 // Device-resident control state of one case, advanced by k_batch_control after every iteration with the
@@ -195,6 +197,8 @@ hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, 
                        int mode, const double* U, const double* G, double* R, double* lam, int* diag,
                        const aero_dev::StageArgs& st, hipStream_t s);
 // Frozen limiter ratios [8][F] at U with gradients G (k_limiter_capture).
+// Frozen limiter ratios [8][F] -> per face slot [8][S] (k_ratio_slots).
+hipError_t launch_ratio_slots(const aero_dev::DevMesh& m, const double* ratios, double* slots, hipStream_t s);
 hipError_t launch_limiter_capture(const aero_dev::DevMesh& m, int limiter, const double* U, const double* G, double g,
                                   double* ratios, hipStream_t s);
 hipError_t launch_global_update(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, const double* U,

@@ -272,10 +272,9 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
         dny = fy - m.ccy[cn];
       }
       double fr[8];
-      if (FROZEN && SECOND) {
-        const int f = m.s_face[s];
+      if (FROZEN && SECOND) {  // slot order: consecutive slots, consecutive addresses
 #pragma unroll
-        for (int j = 0; j < 8; ++j) fr[j] = o.frozen[j * m.nfaces + f];
+        for (int j = 0; j < 8; ++j) fr[j] = o.frozen_slots[j * m.nslots + s];
       }
       bool reverted;
       F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
@@ -499,6 +498,17 @@ __global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double*
 }
 
 // Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
+// Frozen limiter ratios from face order [8][F] to slot order [8][S], once per capture or install: the flux
+// kernels then read them coalesced (the faces of a block are scattered in reference numbering).
+__global__ __launch_bounds__(256) void k_ratio_slots(DevMesh m, const double* __restrict__ fr,
+                                                     double* __restrict__ out) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= m.nslots) return;
+  const int f = m.s_face[s];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) out[static_cast<size_t>(j) * m.nslots + s] = fr[static_cast<size_t>(j) * m.nfaces + f];
+}
+
 __global__ void k_min_dt(int N, const double* __restrict__ lam, double cfl, double* __restrict__ out) {
   __shared__ double red[256];
   double v = 1.79769313486231570815e+308;
@@ -707,6 +717,7 @@ __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux_batch(const
   o.stab_floor = k.floor_cur;
   o.as_w = k.as_w;
   o.frozen = FROZEN ? cs.lim_ratio : nullptr;
+  o.frozen_slots = FROZEN ? cs.lim_slot : nullptr;
   StageArgs st;
   st.stage = bs.stage;
   st.last_stage = bs.last;
@@ -921,6 +932,11 @@ hipError_t launch_flux(const DevMesh& m, const FaceOpts& o, int scheme, int limi
                                : launch_flux_bc<512, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_ratio_slots(const DevMesh& m, const double* ratios, double* slots, hipStream_t s) {
+  hipLaunchKernelGGL(k_ratio_slots, dim3((m.nslots + 255) / 256), dim3(256), 0, s, m, ratios, slots);
+  return hipGetLastError();
 }
 
 hipError_t launch_global_update(const DevMesh& m, const FaceOpts& o, const double* U, const double* R,

@@ -72,6 +72,7 @@ template <class T> struct FaceOptsT {
   int as_fix;
   double as_cut, as_fmin, as_w;
   const double* frozen;  // frozen limiter ratios [8][F] (beyond the reference) or null
+  const double* frozen_slots;  // the same ratios per face slot [8][S], as the flux kernels read them
 };
 using FaceOpts = FaceOptsT<double>;
 

@@ -166,7 +166,7 @@ def run_sweep(args) -> int:
         f"cfl {args.sweep_cfl}, limiter frozen at {args.freeze_iter}")
     with HipBatch([g], cases, base) as b:
         torch.cuda.synchronize()
-        rb = b.run(cap, 1e-8, iters_per_sync=500, use_graph=True)
+        rb = b.run(cap, 1e-8, iters_per_sync=500, use_graph=not args.sweep_no_graph)
     t_batch = rb["seconds"]
     nseq = min(args.sweep_seq_cases, len(cases))
     log(f"[sweep] sequential: {nseq} cases one after another")
@@ -200,10 +200,10 @@ def run_sweep(args) -> int:
                      "cases_per_hr": round(len(cases) / t_batch * 3600.0, 1),
                      "sequential_cases_timed": nseq, "sequential_extrapolation": "seconds per iteration of the "
                      "timed cases x all cases' iterations",
-                     "speedup_vs_sequential": round(t_seq_all / t_batch, 2),
+                     "speedup_vs_sequential": round(t_seq_all / t_batch, 2) if t_seq_all > 0 else None,
                      "iterations_max": max(r["iterations"] for r in rows),
                      "iterations_median": float(np.median([r["iterations"] for r in rows])),
-                     "sequential_solves_per_hr": round(n_conv / t_seq_all * 3600.0, 1),
+                     "sequential_solves_per_hr": round(n_conv / t_seq_all * 3600.0, 1) if t_seq_all > 0 else None,
                      "sequential_matches_batched": same,
                      "converged_8_decades": sum(r["converged"] for r in rows),
                      "failed_nonfinite": sum(r["failed"] for r in rows),
@@ -572,6 +572,9 @@ def main() -> int:
     ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
     ap.add_argument("--sweep-iters", type=int, default=60000, help="iteration cap per case in --sweep")
     ap.add_argument("--sweep-seq-cases", type=int, default=4, help="cases timed sequentially in --sweep")
+    ap.add_argument("--sweep-no-graph", action="store_true",
+                    help="--sweep: launch the batch's kernels directly instead of replaying hipGraphs (for "
+                         "rocprofv3 kernel statistics; slower)")
     ap.add_argument("--sweep-cfl", type=float, default=2.4,
                     help="CFL after the ramp in --sweep / --uneven (2.4: the most cases converged of 2.0, 2.4, "
                          "2.8 and 3.2 on the sweep)")
