@@ -630,10 +630,6 @@ __device__ __forceinline__ void precond_row(const C4& us, const FaceOpts& o, dou
   *b2m1_inv_v = (*b2 - 1.0) * inv * wp.v;
 }
 
-// One stage of the transposed pseudo-time iteration, per cell, after k_adj_gather: the gradient-adjoint
-// gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
-// x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
-// and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
 // d(rho, u, v, p)/dU_j of the primitive map at u0 (euler_flux.cpp:115-127), by four dual evaluations:
 // jac[j] = (drho, du, dv, dp) / dU_j.
 __device__ __forceinline__ void c2p_jacobian(const C4& u0, double g, double (*jac)[4]) {
@@ -746,6 +742,10 @@ __device__ __forceinline__ void adj_wbar_cells(const AdjMesh& am, const double* 
   }
 }
 
+// One stage of the transposed pseudo-time iteration, per cell, after k_adj_gather: the gradient-adjoint
+// gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
+// x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
+// and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
 template <bool WIDE, int GK>
 __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_update(
     AdjMesh am, const double* __restrict__ U, FaceOpts o, const double* __restrict__ Gbar,
