@@ -10,7 +10,8 @@ Synthetic state: freestream with a seeded smooth perturbation.  One step = one r
 Prints one JSON line (rank 0).  `roofline` uses the algorithmic bytes of the dominant (flux) kernel,
 48 B per face + 152 B per cell, over its average duration from HIP events recorded on the kernel's
 stream inside the timed region.  `cpu_baseline` times the FP64 CPU oracle (a serial restatement of the
-reference CPU path) on a bounded sample of the same workload.
+reference CPU path) on a bounded sample of the same workload; `cpu_reference` times the compiled
+reference's own CPU residual (FP32) on the same mesh and state, where oracle/_ref is built.
 """
 from __future__ import annotations
 
@@ -82,6 +83,38 @@ def cpu_baseline(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
             "sample": f"{n} back-to-back FP64 oracle residual evaluations (primitive + Green-Gauss + assembly, "
                       f"serial like the reference CPU path) on the same {mesh.num_cells}-cell mesh, "
                       f"{t:.1f} s on the host CPU"}
+
+
+def cpu_reference(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
+    """The compiled reference's own CPU residual (oracle/_ref: its primitive conversion, Green-Gauss gradient
+    and residual assembly, FP32, serial) on the same mesh and state, built from the mesh arrays."""
+    try:
+        sys.path.insert(0, str(ROOT / "tests"))
+        sys.path.insert(0, str(ROOT / "tests" / "golden"))
+        import numpy as np
+        import oracle_lib as ol
+        ref = ol.ref()
+    except Exception as e:
+        log(f"reference cpu baseline unavailable: {e}")
+        return None
+    if ref is None:
+        log("reference cpu baseline unavailable: oracle/_ref/libaero_ref.so is not built")
+        return None
+    U32 = np.ascontiguousarray(U, np.float32)
+    with ol.RefMesh.from_mesh(ref, mesh) as rm:
+        cfg = ol.ref_residual_cfg(cfg_vals)
+        t0 = time.perf_counter()
+        rm.residual(cfg, U32)
+        t1 = time.perf_counter() - t0
+        n = max(1, int(seconds_target / max(t1, 1e-6)))
+        t0 = time.perf_counter()
+        for _ in range(n):
+            rm.residual(cfg, U32)
+        t = time.perf_counter() - t0
+    return {"value": round(mesh.num_cells * n / t / 1e6, 4), "unit": "Mcell-res/s", "cores": 1, "kind": "reference",
+            "dtype": "f32", "sample": f"{n} back-to-back residual evaluations of the compiled reference (FP32, its "
+                                      f"driver's stage body: primitives, Green-Gauss gradient, assembly) on the same "
+                                      f"{mesh.num_cells}-cell mesh, {t:.1f} s on the host CPU"}
 
 
 def time_residual(hm, cfg, steps: int, warmup: int, stream):
@@ -697,10 +730,12 @@ def main() -> int:
                                "flux_kernel_ms": round(v_flux, 5),
                                "flux_kernel_hbm_frac": round(flux_bytes / (v_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                "finite": bool(np.all(np.isfinite(vR))), "fallbacks": vdiag}
-    cpu = None
+    cpu = cpu_ref = None
     if args.cpu_seconds > 0:
         log("[bench] cpu baseline")
         cpu = cpu_baseline(mesh, cfg_vals, U, args.cpu_seconds)
+        log("[bench] reference cpu path")
+        cpu_ref = cpu_reference(mesh, cfg_vals, U, args.cpu_seconds)
     hm.close()
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcell-res/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
@@ -708,7 +743,8 @@ def main() -> int:
            "config": {"workload": args.workload, "cells": N, "faces": F, "flux": flux, "limiter": lim,
                       "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": args.block_cells,
                       "renumber": not args.no_renumber},
-           "roofline": roofline, "cpu_baseline": cpu, "variants": variants, "detail": extra}
+           "roofline": roofline, "cpu_baseline": cpu, "cpu_reference": cpu_ref, "variants": variants,
+           "detail": extra}
     print(json.dumps(out), flush=True)
     return 0
 

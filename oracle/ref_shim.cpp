@@ -2,9 +2,10 @@
 //
 // extern "C" wrapper over the compiled FP32 reference (AstaWasTaken/Aero-CFD-Solver v0.1.1), linked
 // with the reference's own sources where they lie (see oracle/Makefile, target `ref`), into
-// oracle/_ref/libaero_ref.so.  It is used in this container only, by tests/golden/make_golden.py
-// (to write golden vectors) and by tests/test_oracle_vs_ref.py (when the library exists).  Nothing
-// here is reference source: it only calls the reference's public C++ API.
+// oracle/_ref/libaero_ref.so.  It is used by tests/golden/make_golden.py (to write golden vectors), by
+// tests/test_oracle_vs_ref.py (when the library exists) and by bench.py's CPU-baseline leg, which times
+// the reference's residual beside the device kernels.  Nothing here is reference source: it only calls
+// the reference's public C++ API.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -209,10 +210,12 @@ void ref_gradients(const ref_mesh_cfg* c, const float* U, float g, float* grad) 
   std::memcpy(grad, gr.values.data(), gr.values.size() * sizeof(float));
 }
 
+}  // extern "C"
+
+namespace {
 // One residual evaluation exactly as the driver's stage body (euler_solver.cpp:803-869).
-void ref_residual(const ref_mesh_cfg* c, const ref_residual_cfg* rc, const float* U, float* res, float* lam,
-                  int* diag) {
-  const UnstructuredMesh m = make_mesh(c);
+void residual_on(const UnstructuredMesh& m, const ref_residual_cfg* rc, const float* U, float* res, float* lam,
+                 int* diag) {
   std::vector<PrimitiveState> prim(static_cast<std::size_t>(m.num_cells));
   for (int i = 0; i < m.num_cells; ++i) prim[i] = conservative_to_primitive(cons_at(U, i), rc->gamma);
   PrimitiveGradients gr;
@@ -231,6 +234,48 @@ void ref_residual(const ref_mesh_cfg* c, const ref_residual_cfg* rc, const float
   diag[0] = d.num_faces_first_order_fallback;
   diag[1] = d.num_faces_diffusive_fallback;
   diag[2] = d.first_failure_face;
+}
+}  // namespace
+
+extern "C" {
+
+void ref_residual(const ref_mesh_cfg* c, const ref_residual_cfg* rc, const float* U, float* res, float* lam,
+                  int* diag) {
+  residual_on(make_mesh(c), rc, U, res, lam, diag);
+}
+
+// A reference mesh from caller arrays, for meshes the reference's generator does not build (the mixed,
+// jittered meshes the HIP backend is timed on): unit normals and face / cell centres xyz-interleaved, the
+// wall patch [wall_start, wall_start + wall_count), every other boundary face farfield.  Freed with
+// ref_mesh_delete.
+void* ref_mesh_new(int n_cells, int n_faces, const int* owner, const int* neighbor, const float* normal,
+                   const float* face_center, const float* area, const float* volume, const float* cell_center,
+                   int wall_start, int wall_count) {
+  auto* m = new UnstructuredMesh();
+  m->num_cells = n_cells;
+  m->num_faces = n_faces;
+  m->face_owner.assign(owner, owner + n_faces);
+  m->face_neighbor.assign(neighbor, neighbor + n_faces);
+  m->face_vertices.assign(2 * static_cast<std::size_t>(n_faces), 0);
+  m->face_normal.assign(normal, normal + 3 * static_cast<std::size_t>(n_faces));
+  m->face_center.assign(face_center, face_center + 3 * static_cast<std::size_t>(n_faces));
+  m->face_area.assign(area, area + n_faces);
+  m->cell_volume.assign(volume, volume + n_cells);
+  m->cell_center.assign(cell_center, cell_center + 3 * static_cast<std::size_t>(n_cells));
+  BoundaryPatchRange wall;
+  wall.name = "wall";
+  wall.start_face = wall_start;
+  wall.face_count = wall_count;
+  m->boundary_patches.push_back(wall);
+  return m;
+}
+
+void ref_mesh_delete(void* mesh) { delete static_cast<UnstructuredMesh*>(mesh); }
+
+// ref_residual on a mesh from ref_mesh_new.
+void ref_residual_mesh(const void* mesh, const ref_residual_cfg* rc, const float* U, float* res, float* lam,
+                       int* diag) {
+  residual_on(*static_cast<const UnstructuredMesh*>(mesh), rc, U, res, lam, diag);
 }
 
 void ref_stage_tail(int n, int stage, const float* un, const float* us, const float* res, const float* lam, float cfl,

@@ -170,3 +170,69 @@ def residual_frozen(om: OracleMesh, cfg, U, ratios):
     diag = np.zeros(3, np.int32)
     oracle().ao_residual_frozen_f64(C.byref(om.c), C.byref(cfg), ptr(U), ptr(fr), ptr(R), ptr(lam), ptr(diag))
     return R, lam, diag
+
+
+# ---- the compiled reference (oracle/_ref/libaero_ref.so, FP32) on caller meshes --------------------
+class RefMeshCfg(C.Structure):
+    """ref_mesh_cfg of oracle/ref_shim.cpp (the reference's O-grid generator inputs)."""
+    _fields_ = [("num_circumferential", C.c_int), ("num_radial", C.c_int), ("farfield_radius", C.c_float),
+                ("radial_stretch", C.c_float), ("chord", C.c_float), ("naca_code", C.c_char * 8)]
+
+
+class RefResidualCfg(C.Structure):
+    """ref_residual_cfg of oracle/ref_shim.cpp."""
+    _fields_ = [("farfield", C.c_float * 5), ("gamma", C.c_float), ("scheme", C.c_int), ("limiter", C.c_int),
+                ("second_order", C.c_int), ("low_mach_fix", C.c_int), ("low_mach_cutoff", C.c_float),
+                ("stab_mach_floor", C.c_float), ("precond_farfield_bc", C.c_int), ("precond_enabled", C.c_int),
+                ("mach_ref", C.c_float), ("mach_min", C.c_float), ("beta_min", C.c_float),
+                ("beta_max", C.c_float), ("as_fix", C.c_int), ("as_cutoff", C.c_float), ("as_fmin", C.c_float),
+                ("as_ramp", C.c_float)]
+
+
+def ref_residual_cfg(vals: dict) -> RefResidualCfg:
+    """The reference's residual configuration from the C ABI's field names (aero_residual_config)."""
+    return RefResidualCfg((C.c_float * 5)(*vals["farfield"]), vals["gamma"], vals["flux_scheme"], vals["limiter"],
+                          vals["second_order"], vals["low_mach_fix"], vals["low_mach_cutoff"],
+                          vals["stab_mach_floor"], vals["precond_farfield_bc"], vals["precond_enabled"],
+                          vals["precond_mach_ref"], vals["precond_mach_min"], vals["precond_beta_min"],
+                          vals["precond_beta_max"], vals["all_speed_fix"], vals["all_speed_cutoff"],
+                          vals["all_speed_f_min"], vals["all_speed_ramp"])
+
+
+class RefMesh:
+    """A reference UnstructuredMesh built from mesh arrays (ref_mesh_new), FP32 like the reference."""
+
+    def __init__(self, ref: C.CDLL, num_cells, num_faces, owner, neighbor, normal, face_center, area, volume,
+                 cell_center, wall_start, wall_count):
+        self.ref = ref
+        self.N, self.F = int(num_cells), int(num_faces)
+        self._keep = [np.ascontiguousarray(owner, np.int32), np.ascontiguousarray(neighbor, np.int32),
+                      np.ascontiguousarray(normal, np.float32), np.ascontiguousarray(face_center, np.float32),
+                      np.ascontiguousarray(area, np.float32), np.ascontiguousarray(volume, np.float32),
+                      np.ascontiguousarray(cell_center, np.float32)]
+        ref.ref_mesh_new.restype = C.c_void_p
+        self.h = ref.ref_mesh_new(self.N, self.F, *[ptr(a) for a in self._keep], int(wall_start), int(wall_count))
+
+    @classmethod
+    def from_mesh(cls, ref: C.CDLL, mesh) -> "RefMesh":
+        return cls(ref, mesh.num_cells, mesh.num_faces, mesh.owner, mesh.neighbor, mesh.normal, mesh.face_center,
+                   mesh.area, mesh.volume, mesh.cell_center, mesh.wall_start, mesh.wall_count)
+
+    def residual(self, cfg: RefResidualCfg, U):
+        U = np.ascontiguousarray(U, np.float32).reshape(self.N, 5)
+        R = np.zeros((self.N, 5), np.float32)
+        lam = np.zeros(self.N, np.float32)
+        diag = np.zeros(3, np.int32)
+        self.ref.ref_residual_mesh(C.c_void_p(self.h), C.byref(cfg), ptr(U), ptr(R), ptr(lam), ptr(diag))
+        return R, lam, diag
+
+    def close(self):
+        if self.h:
+            self.ref.ref_mesh_delete(C.c_void_p(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
