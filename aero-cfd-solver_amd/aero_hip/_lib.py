@@ -179,6 +179,8 @@ PROTOTYPES = {
     "aero_mesh_generate": (C.c_int, [C.POINTER(MeshGenConfig), i32, C.POINTER(MeshArrays), C.c_char_p,
                                      C.c_size_t]),
     "aero_mesh_free": (None, [C.POINTER(MeshArrays)]),
+    "aero_mesh_write": (C.c_int, [C.POINTER(MeshArrays), C.c_char_p, C.c_char_p, C.c_size_t]),
+    "aero_mesh_read": (C.c_int, [C.c_char_p, C.POINTER(MeshArrays), C.c_char_p, C.c_size_t]),
     "aero_mesh_deform": (C.c_int, [C.POINTER(MeshArrays), C.POINTER(ShapeConfig), PD, C.POINTER(MeshArrays),
                                    C.POINTER(GeometryTangents), C.c_char_p, C.c_size_t]),
     "aero_hip_freeze_limiter": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.c_char_p, C.c_size_t]),

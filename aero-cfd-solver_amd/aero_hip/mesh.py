@@ -95,15 +95,17 @@ def _from_arrays(out: "_lib.MeshArrays", meta: dict) -> Mesh:
                 out.farfield_start, out.farfield_count, meta)
 
 
-def _to_arrays(m: Mesh):
-    """aero_mesh_arrays over an FP64 mesh's arrays (returns the struct and the arrays to keep alive)."""
-    keep = [np.ascontiguousarray(a) for a in (m.owner, m.neighbor, m.bc, m.face_vertices)]
-    keep += [np.ascontiguousarray(a, dtype=np.float64) for a in (m.normal, m.face_center, m.area, m.volume,
-                                                                  m.cell_center, m.points)]
+def _to_arrays(m: Mesh, precision: int = 8):
+    """aero_mesh_arrays over a mesh's arrays at `precision` (4 or 8) (returns the struct and the arrays to
+    keep alive)."""
+    ft = np.float32 if precision == 4 else np.float64
+    keep = [np.ascontiguousarray(a, dtype=np.int32) for a in (m.owner, m.neighbor, m.bc, m.face_vertices)]
+    keep += [np.ascontiguousarray(a, dtype=ft) for a in (m.normal, m.face_center, m.area, m.volume,
+                                                          m.cell_center, m.points)]
     keep += [np.ascontiguousarray(a, dtype=np.int32) for a in (m.cell_offsets, m.cell_connectivity)]
     PI = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    arr = _lib.MeshArrays(8, m.num_cells, m.num_faces, len(m.points), PI(keep[0]), PI(keep[1]), PI(keep[2]),
+    arr = _lib.MeshArrays(precision, m.num_cells, m.num_faces, len(m.points), PI(keep[0]), PI(keep[1]), PI(keep[2]),
                           PI(keep[3]), vp(keep[4]), vp(keep[5]), vp(keep[6]), vp(keep[7]), vp(keep[8]), vp(keep[9]),
                           PI(keep[10]), PI(keep[11]), m.wall_start, m.wall_count, m.farfield_start, m.farfield_count)
     return arr, keep
@@ -144,3 +146,23 @@ def deform_mesh(mesh: Mesh, amplitudes, bumps_per_surface: int = 10, h_min: floa
         lib.aero_mesh_free(C.byref(out))
     del keep
     return m, tan
+
+
+def save_mesh(mesh: Mesh, path) -> None:
+    """Write `mesh` to a binary mesh file (aero_mesh_write), at the mesh's own precision."""
+    precision = 4 if np.asarray(mesh.area).dtype == np.float32 else 8
+    arr, _keep = _to_arrays(mesh, precision)
+    err = _lib.errbuf()
+    _lib.check(_lib.lib().aero_mesh_write(C.byref(arr), str(path).encode(), err, len(err)), err, "aero_mesh_write")
+
+
+def load_mesh(path) -> Mesh:
+    """Read a binary mesh file (aero_mesh_read); the file's indices and ranges are validated."""
+    out = _lib.MeshArrays()
+    lib = _lib.lib()
+    err = _lib.errbuf()
+    _lib.check(lib.aero_mesh_read(str(path).encode(), C.byref(out), err, len(err)), err, "aero_mesh_read")
+    try:
+        return _from_arrays(out, dict(file=str(path)))
+    finally:
+        lib.aero_mesh_free(C.byref(out))

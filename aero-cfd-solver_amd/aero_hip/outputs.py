@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 from .case_file import euler_case_from_kv, load_case
-from .mesh import Mesh, generate_mesh
+from .mesh import Mesh, generate_mesh, load_mesh
 from .solver import CaseConfig, HipMesh
 
 VERSION = "0.1.1-hip"
@@ -191,12 +191,15 @@ def write_outputs(out_dir: Path, mesh: Mesh, case: CaseConfig, out: dict) -> dic
     return paths
 
 
-def case_and_mesh(kv: dict) -> tuple[CaseConfig, Mesh]:
-    """CaseConfig and the reference generator's O-grid (FP32 point set, FP64 arrays) of native keys."""
+def case_and_mesh(kv: dict, mesh_path=None) -> tuple[CaseConfig, Mesh]:
+    """CaseConfig and the reference generator's O-grid (FP32 point set, FP64 arrays) of native keys, or
+    the mesh of a binary mesh file (save_mesh) when mesh_path is given."""
     case_type = kv.get("case_type", "scalar_advect_demo")
     if case_type != "euler_airfoil_2d":
         raise ValueError(f"the HIP backend runs euler_airfoil_2d cases (case_type={case_type})")
     case, ms = euler_case_from_kv(kv)
+    if mesh_path is not None:
+        return case, load_mesh(mesh_path).astype(np.float64)
     if ms["airfoil_source"].lower() != "naca4":
         raise ValueError("the HIP backend's mesh generator builds NACA 4-digit airfoils (airfoil_source=naca4)")
     mesh = generate_mesh(naca=ms["naca"], num_circumferential=ms["num_circumferential"], num_radial=ms["num_radial"],
@@ -206,14 +209,15 @@ def case_and_mesh(kv: dict) -> tuple[CaseConfig, Mesh]:
 
 
 def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int = 256,
-                  overrides: dict | None = None) -> dict:
+                  overrides: dict | None = None, mesh_path=None) -> dict:
     """Run an euler_airfoil_2d case file (native key=value or YAML) on the device and write the reference
     runner's outputs into out_dir; returns the pybind run_case summary dict.  overrides: native keys
-    replacing the file's (e.g. {"aoa_deg": "4"})."""
+    replacing the file's (e.g. {"aoa_deg": "4"}); mesh_path: a binary mesh file to run on instead of the
+    generated O-grid."""
     kv = load_case(case_path)
     kv.update({k: str(v) for k, v in (overrides or {}).items()})
     case_type = kv.get("case_type", "scalar_advect_demo")
-    case, mesh = case_and_mesh(kv)
+    case, mesh = case_and_mesh(kv, mesh_path)
     out_dir = Path(out_dir)
     out_dir.mkdir(parents=True, exist_ok=True)
     with HipMesh(mesh, block_cells=block_cells) as hm:
@@ -261,7 +265,7 @@ def wall_cp_from_state(mesh: Mesh, case: CaseConfig, U: np.ndarray) -> np.ndarra
 
 
 def run_sweep_file(sweep_path: str | Path, out_dir: str | Path, batched: bool = False, block_cells: int = 256,
-                   jobs: int = 1, progress=None) -> dict:
+                   jobs: int = 1, progress=None, mesh_path=None) -> dict:
     """A sweep YAML (base_case: a case file relative to the sweep file; sweep: aoa_deg and optionally
     mach lists, crossed) as python/cfd/sweep.py:14-59 runs it, except that every run gets its own AoA
     and Mach.  Serial: one device run per case with the reference's outputs in run_000, run_001, ...
@@ -278,7 +282,7 @@ def run_sweep_file(sweep_path: str | Path, out_dir: str | Path, batched: bool = 
         raise ValueError("Sweep YAML must define 'base_case'.")
     base_path = (sp.parent / str(cfg["base_case"])).resolve()
     kv = load_case(base_path)
-    case0, mesh = case_and_mesh(kv)
+    case0, mesh = case_and_mesh(kv, mesh_path)
     sweep = cfg.get("sweep", {}) or {}
     aoas = [float(a) for a in sweep.get("aoa_deg", [case0.aoa_deg])]
     machs = [float(m) for m in sweep.get("mach", [case0.mach])]
@@ -289,7 +293,7 @@ def run_sweep_file(sweep_path: str | Path, out_dir: str | Path, batched: bool = 
     if not batched:
         for i, (m, a) in enumerate(points):
             s = run_case_file(base_path, root / f"run_{i:03d}", block_cells=block_cells,
-                              overrides={"aoa_deg": repr(a), "mach": repr(m)})
+                              overrides={"aoa_deg": repr(a), "mach": repr(m)}, mesh_path=mesh_path)
             s.update(aoa_deg=a, mach=m)
             runs.append(s)
             if progress:

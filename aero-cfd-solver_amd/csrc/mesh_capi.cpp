@@ -1,4 +1,6 @@
 // mesh_capi.cpp -- extern "C" export of the host-side O-grid generator (aero_mesh_generate).
+#include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -77,6 +79,153 @@ extern "C" void aero_mesh_free(aero_mesh_arrays* a) {
                   a->face_area,  a->cell_volume,   a->cell_center, a->points,      a->cell_offsets, a->cell_connectivity};
   for (void* p : ptrs) std::free(p);
   std::memset(a, 0, sizeof(*a));
+}
+
+// ---- binary mesh file ------------------------------------------------------------------------------
+// Layout: the 8-byte magic, int32 precision, num_cells, num_faces, num_points, connectivity length,
+// wall_start, wall_count, farfield_start, farfield_count, then the arrays of aero_mesh_arrays in
+// declaration order, the floating ones at the file's precision, all in the host's byte order (little
+// endian on x86-64 and the GPU hosts).
+namespace {
+
+constexpr char kMeshMagic[8] = {'A', 'E', 'R', 'O', 'M', 'S', 'H', '1'};
+
+struct MeshFileHead {
+  int32_t precision, num_cells, num_faces, num_points, conn_len, wall_start, wall_count, far_start, far_count;
+};
+
+// Element counts of the twelve arrays in declaration order.
+void mesh_counts(const MeshFileHead& h, size_t n[12]) {
+  const size_t N = h.num_cells, F = h.num_faces, P = h.num_points;
+  const size_t c[12] = {F, F, F, 2 * F, 3 * F, 3 * F, F, N, 3 * N, 3 * P, N, static_cast<size_t>(h.conn_len)};
+  for (int i = 0; i < 12; ++i) n[i] = c[i];
+}
+
+bool mesh_is_float(int i) { return i >= 4 && i <= 9; }
+
+void set_msg(char* err, size_t errlen, const std::string& m) {
+  if (err && errlen) std::snprintf(err, errlen, "%s", m.c_str());
+}
+
+// Every index in range and every count consistent, so that a mesh read from a file can go to the
+// device kernels without further checks.
+std::string mesh_invalid(const MeshFileHead& h, const aero_mesh_arrays& a) {
+  const int N = h.num_cells, F = h.num_faces, P = h.num_points;
+  for (int f = 0; f < F; ++f) {
+    if (a.face_owner[f] < 0 || a.face_owner[f] >= N) return "face owner out of range";
+    if (a.face_neighbor[f] < -1 || a.face_neighbor[f] >= N) return "face neighbour out of range";
+    if (a.face_bc[f] < 0 || a.face_bc[f] > 2) return "face boundary code out of range";
+    if ((a.face_neighbor[f] < 0) != (a.face_bc[f] != AERO_BC_INTERIOR)) return "face boundary code mismatch";
+    for (int k = 0; k < 2; ++k)
+      if (a.face_vertices[2 * f + k] < 0 || a.face_vertices[2 * f + k] >= P) return "face vertex out of range";
+  }
+  int prev = 0;
+  for (int c = 0; c < N; ++c) {
+    if (a.cell_offsets[c] < prev) return "cell offsets decrease";
+    prev = a.cell_offsets[c];
+  }
+  if (prev != h.conn_len) return "cell offsets do not end at the connectivity length";
+  for (int i = 0; i < h.conn_len; ++i)
+    if (a.cell_connectivity[i] < 0 || a.cell_connectivity[i] >= P) return "cell connectivity out of range";
+  auto range_ok = [F](int s, int n) { return s >= 0 && n >= 0 && s <= F && n <= F - s; };
+  if (!range_ok(h.wall_start, h.wall_count) || !range_ok(h.far_start, h.far_count)) return "patch range invalid";
+  return "";
+}
+
+}  // namespace
+
+extern "C" int aero_mesh_write(const aero_mesh_arrays* a, const char* path, char* err, size_t errlen) {
+  if (!a || !path || (a->precision != 4 && a->precision != 8) || a->num_cells < 0 || a->num_faces < 0 ||
+      a->num_points < 0) {
+    set_msg(err, errlen, "aero_mesh_write: bad arguments");
+    return AERO_EINVAL;
+  }
+  MeshFileHead h{a->precision, a->num_cells, a->num_faces, a->num_points,
+                 a->num_cells > 0 ? a->cell_offsets[a->num_cells - 1] : 0, a->wall_start, a->wall_count,
+                 a->farfield_start, a->farfield_count};
+  std::FILE* fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_msg(err, errlen, std::string("aero_mesh_write: cannot open ") + path);
+    return AERO_EINVAL;
+  }
+  size_t n[12];
+  mesh_counts(h, n);
+  const void* arrays[12] = {a->face_owner, a->face_neighbor, a->face_bc, a->face_vertices, a->face_normal,
+                            a->face_center, a->face_area, a->cell_volume, a->cell_center, a->points,
+                            a->cell_offsets, a->cell_connectivity};
+  bool ok = std::fwrite(kMeshMagic, 1, 8, fp) == 8 && std::fwrite(&h, sizeof(h), 1, fp) == 1;
+  for (int i = 0; ok && i < 12; ++i) {
+    const size_t es = mesh_is_float(i) ? static_cast<size_t>(h.precision) : sizeof(int32_t);
+    ok = n[i] == 0 || std::fwrite(arrays[i], es, n[i], fp) == n[i];
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok) {
+    set_msg(err, errlen, std::string("aero_mesh_write: write failed: ") + path);
+    return AERO_EINVAL;
+  }
+  return AERO_OK;
+}
+
+extern "C" int aero_mesh_read(const char* path, aero_mesh_arrays* out, char* err, size_t errlen) {
+  if (!path || !out) {
+    set_msg(err, errlen, "aero_mesh_read: null argument");
+    return AERO_EINVAL;
+  }
+  std::memset(out, 0, sizeof(*out));
+  std::FILE* fp = std::fopen(path, "rb");
+  if (!fp) {
+    set_msg(err, errlen, std::string("aero_mesh_read: cannot open ") + path);
+    return AERO_EINVAL;
+  }
+  char magic[8];
+  MeshFileHead h{};
+  const bool head_ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, kMeshMagic, 8) == 0 &&
+                       std::fread(&h, sizeof(h), 1, fp) == 1 && (h.precision == 4 || h.precision == 8) &&
+                       h.num_cells >= 0 && h.num_faces >= 0 && h.num_points >= 0 && h.conn_len >= 0;
+  if (!head_ok) {
+    std::fclose(fp);
+    set_msg(err, errlen, std::string("aero_mesh_read: not a mesh file (or an unsupported version): ") + path);
+    return AERO_EINVAL;
+  }
+  size_t n[12];
+  mesh_counts(h, n);
+  void* arrays[12] = {};
+  bool ok = true;
+  for (int i = 0; ok && i < 12; ++i) {
+    const size_t es = mesh_is_float(i) ? static_cast<size_t>(h.precision) : sizeof(int32_t);
+    arrays[i] = std::malloc(std::max<size_t>(n[i], 1) * es);
+    ok = arrays[i] && (n[i] == 0 || std::fread(arrays[i], es, n[i], fp) == n[i]);
+  }
+  char extra;
+  ok = ok && std::fread(&extra, 1, 1, fp) == 0;  // nothing after the last array
+  std::fclose(fp);
+  out->precision = h.precision;
+  out->num_cells = h.num_cells;
+  out->num_faces = h.num_faces;
+  out->num_points = h.num_points;
+  out->face_owner = static_cast<int32_t*>(arrays[0]);
+  out->face_neighbor = static_cast<int32_t*>(arrays[1]);
+  out->face_bc = static_cast<int32_t*>(arrays[2]);
+  out->face_vertices = static_cast<int32_t*>(arrays[3]);
+  out->face_normal = arrays[4];
+  out->face_center = arrays[5];
+  out->face_area = arrays[6];
+  out->cell_volume = arrays[7];
+  out->cell_center = arrays[8];
+  out->points = arrays[9];
+  out->cell_offsets = static_cast<int32_t*>(arrays[10]);
+  out->cell_connectivity = static_cast<int32_t*>(arrays[11]);
+  out->wall_start = h.wall_start;
+  out->wall_count = h.wall_count;
+  out->farfield_start = h.far_start;
+  out->farfield_count = h.far_count;
+  std::string why = ok ? mesh_invalid(h, *out) : std::string("truncated or oversized file");
+  if (!why.empty()) {
+    aero_mesh_free(out);
+    set_msg(err, errlen, std::string("aero_mesh_read: ") + why + ": " + path);
+    return AERO_EINVAL;
+  }
+  return AERO_OK;
 }
 
 // ---- shape deformation ------------------------------------------------------------------------------

@@ -427,6 +427,16 @@ int aero_mesh_generate(const aero_mesh_gen_config* cfg, int32_t precision, aero_
                        size_t errlen);
 void aero_mesh_free(aero_mesh_arrays* arrays);
 
+/* Binary mesh file (no reference counterpart: the reference regenerates its mesh on every run,
+ * euler_solver.cpp:474; the file holds the UnstructuredMesh fields of mesh.hpp:14-36 plus this ABI's
+ * boundary codes and patch ranges).  Layout: the magic "AEROMSH1", int32 precision (4 or 8), num_cells,
+ * num_faces, num_points, connectivity length, wall_start, wall_count, farfield_start, farfield_count, then
+ * the arrays of aero_mesh_arrays in declaration order (floating arrays at the file's precision), in the
+ * host's byte order.  aero_mesh_read allocates `out` (free with aero_mesh_free) and rejects a file whose
+ * indices, counts or patch ranges are out of range, so its meshes are safe to hand to the device. */
+int aero_mesh_write(const aero_mesh_arrays* mesh, const char* path, char* err, size_t errlen);
+int aero_mesh_read(const char* path, aero_mesh_arrays* out, char* err, size_t errlen);
+
 /* Shape parameters (no reference counterpart): Hicks-Henne bumps b_i(x) = sin^3(pi x^(ln 0.5 / ln h_i)),
  * h_i = linspace(h_min, h_max, bumps_per_surface) in x/c, on the upper surface (parameters
  * 0..n-1) then the lower surface (n..2n-1); amplitude a_i moves that surface's wall nodes by a_i b_i(x)

@@ -76,6 +76,22 @@ def test_run_case_file_outputs(hip, tmp_path):
     assert "backend=hip" in (out / "run.log").read_text()
 
 
+def test_run_case_on_a_mesh_file(hip, tmp_path):
+    """The runner on a binary mesh file of the case's O-grid (`--mesh`) reproduces the generated-mesh run:
+    the same summary and the same output files, byte for byte."""
+    from aero_hip.__main__ import main
+    cfg = tmp_path / "native_case.cfg"
+    cfg.write_text(CASE)
+    mfile = tmp_path / "case.amsh"
+    assert main(["mesh", str(cfg), str(mfile)]) == 0
+    say("[runner] generated mesh vs mesh file")
+    a = hip.run_case_file(cfg, tmp_path / "gen")
+    b = hip.run_case_file(cfg, tmp_path / "file", mesh_path=mfile)
+    assert (a["cl"], a["cd"], a["cm"], a["iterations"]) == (b["cl"], b["cd"], b["cm"], b["iterations"])
+    for name in ("residuals.csv", "forces.csv", "cp_wall.csv", "field_0000.vtu"):
+        assert (tmp_path / "gen" / name).read_bytes() == (tmp_path / "file" / name).read_bytes(), name
+
+
 def test_sweep_runs_each_aoa(hip, tmp_path):
     (tmp_path / "base.cfg").write_text(CASE)
     (tmp_path / "sweep.yaml").write_text("base_case: base.cfg\nsweep:\n  aoa_deg: [0.0, 2.0]\n")
