@@ -79,10 +79,20 @@ def cpu_baseline(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
     t1 = ol.time_residual(om, cfg, U, 1)
     n = max(1, int(seconds_target / max(t1, 1e-6)))
     t = ol.time_residual(om, cfg, U, n)
-    return {"value": round(mesh.num_cells * n / t / 1e6, 4), "unit": "Mcell-res/s", "cores": 1, "kind": "port",
-            "sample": f"{n} back-to-back FP64 oracle residual evaluations (primitive + Green-Gauss + assembly, "
-                      f"serial like the reference CPU path) on the same {mesh.num_cells}-cell mesh, "
-                      f"{t:.1f} s on the host CPU"}
+    out = {"value": round(mesh.num_cells * n / t / 1e6, 4), "unit": "Mcell-res/s", "cores": 1, "kind": "port",
+           "sample": f"{n} back-to-back FP64 oracle residual evaluations (primitive + Green-Gauss + assembly, "
+                     f"serial like the reference CPU path) on the same {mesh.num_cells}-cell mesh, "
+                     f"{t:.1f} s on the host CPU"}
+    # the same evaluation on all OpenMP threads (per-face values, per-cell sums in face order: the serial
+    # result bit for bit)
+    threads = ol.max_threads()
+    t1 = ol.time_residual_threaded(om, cfg, U, 1)
+    n = max(1, int(seconds_target / max(t1, 1e-6)))
+    t = ol.time_residual_threaded(om, cfg, U, n)
+    out["threaded"] = {"value": round(mesh.num_cells * n / t / 1e6, 4), "unit": "Mcell-res/s", "cores": threads,
+                       "kind": "port", "sample": f"{n} evaluations of the same FP64 oracle on {threads} OpenMP "
+                                                 f"threads, {t:.1f} s"}
+    return out
 
 
 def cpu_reference(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
@@ -115,6 +125,41 @@ def cpu_reference(mesh, cfg_vals, U, seconds_target: float) -> dict | None:
             "dtype": "f32", "sample": f"{n} back-to-back residual evaluations of the compiled reference (FP32, its "
                                       f"driver's stage body: primitives, Green-Gauss gradient, assembly) on the same "
                                       f"{mesh.num_cells}-cell mesh, {t:.1f} s on the host CPU"}
+
+
+def copy_ceiling(mib: int = 1024, reps: int = 20) -> float:
+    """Achieved bandwidth of a plain device-to-device copy of `mib` MiB (read + write bytes over the time of
+    `reps` copies, HIP events): the practical HBM ceiling of this GPU, beside the datasheet peak."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * n * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def host_cpu() -> dict:
+    """The host CPU the CPU baselines ran on: model name and logical CPUs."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo", encoding="utf-8"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
 
 
 def time_residual(hm, cfg, steps: int, warmup: int, stream):
@@ -703,10 +748,12 @@ def main() -> int:
     flux_ms = tm["flux_ms"] / n
     grad_ms = tm["gradient_ms"] / n
     achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    ceiling = copy_ceiling()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(args.workload),
                 "kernel": "k_flux (reconstruction + Riemann flux + face->cell gather)",
-                "algorithmic_bytes_per_launch": flux_bytes, "avg_launch_ms": round(flux_ms, 5)}
+                "algorithmic_bytes_per_launch": flux_bytes, "avg_launch_ms": round(flux_ms, 5),
+                "copy_ceiling_gbs": round(ceiling, 1), "frac_of_copy_ceiling": round(achieved / ceiling, 4)}
     extra = {"gradient_kernel_ms": round(grad_ms, 5),
              "gradient_kernel_gbs": round(grad_bytes / (grad_ms * 1e-3) / 1e9, 1) if grad_ms > 0 else None,
              "eval_hbm_frac": round(eval_bytes * args.steps / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -743,7 +790,8 @@ def main() -> int:
            "config": {"workload": args.workload, "cells": N, "faces": F, "flux": flux, "limiter": lim,
                       "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": args.block_cells,
                       "renumber": not args.no_renumber},
-           "roofline": roofline, "cpu_baseline": cpu, "cpu_reference": cpu_ref, "variants": variants,
+           "roofline": roofline, "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_host": host_cpu(),
+           "variants": variants,
            "detail": extra}
     print(json.dumps(out), flush=True)
     return 0

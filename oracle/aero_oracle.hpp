@@ -699,6 +699,54 @@ void limiter_ratios(const MeshView<R>& m, const std::vector<Prim<R>>& prim, cons
   }
 }
 
+// One face of the assembly loop (residual_assembly.cpp:97-166): its flux F and wave speed w (before the
+// area), whether the reconstruction fell back to first order (*fo) and whether the flux used its
+// diffusive fallback (*fbk).
+template <class R>
+void face_contribution(const MeshView<R>& m, const AssemblyCfg<R>& c, const std::vector<Prim<R>>& prim,
+                       const std::vector<R>& grad, int f, A5<R>* Fout, R* wout, bool* fo, bool* fbk) {
+  const R g = c.gamma;
+  const int o = m.owner[f];
+  const int nb = m.neighbor[f];
+  const A3<R> n = normal_of(m, f);
+  A5<R> F{};
+  R w = R(0);
+  *fo = false;
+  *fbk = false;
+  if (nb >= 0) {
+    Prim<R> L = prim[o], Rt = prim[nb];
+    if (c.second_order) {
+      reconstruct(m, prim, grad, f, c.limiter, &L, &Rt, c.frozen);
+      if (!valid_prim(L) || !valid_prim(Rt)) {
+        *fo = true;
+        L = prim[o];
+        Rt = prim[nb];
+      }
+    }
+    const Cons<R> ul = prim_to_cons(L, g);
+    const Cons<R> ur = prim_to_cons(Rt, g);
+    const R fb = face_beta(ul, ur, g, c.precond);
+    const R s = acoustic_scale(c, ul, ur, fb);
+    F = face_flux(ul, ur, n, g, c.scheme, &w, s, c.as_fix, c.as_cutoff, c.as_fmin, c.as_ramp, fbk);
+  } else {
+    const Cons<R> in = prim_to_cons(prim[o], g);
+    const int bc = m.bc[f];
+    Cons<R> ex = c.farfield;
+    R fb;
+    if (bc == kSlipWall) {
+      fb = face_beta(in, in, g, c.precond);
+      ex = in;
+    } else {
+      fb = face_beta(in, c.farfield, g, c.precond);
+    }
+    const R s = acoustic_scale(c, in, ex, fb);
+    F = boundary_flux(bc, in, c.farfield, n, c.scheme, g, s, c.precond_farfield_bc, c.as_fix, c.as_cutoff,
+                      c.as_fmin, c.as_ramp, &w, fbk);
+  }
+  *Fout = F;
+  *wout = w;
+}
+
 template <class R>
 void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std::vector<Prim<R>>& prim,
                        const std::vector<R>& grad, std::vector<R>* res, std::vector<R>* lam,
@@ -708,49 +756,19 @@ void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std:
   res->assign(static_cast<std::size_t>(m.num_cells) * 5, R(0));
   lam->assign(static_cast<std::size_t>(m.num_cells), R(0));
   if (diag) *diag = AssemblyDiag{};
-  const R g = c.gamma;
   for (int f = 0; f < m.num_faces; ++f) {
     const int o = m.owner[f];
     const int nb = m.neighbor[f];
-    const A3<R> n = normal_of(m, f);
     const R A = m.area[f];
-    A5<R> F{};
-    R w = R(0);
-    if (nb >= 0) {
-      Prim<R> L = prim[o], Rt = prim[nb];
-      if (c.second_order) {
-        reconstruct(m, prim, grad, f, c.limiter, &L, &Rt, c.frozen);
-        if (!valid_prim(L) || !valid_prim(Rt)) {
-          if (diag && diag->first_failure_face < 0) diag->first_failure_face = f;
-          L = prim[o];
-          Rt = prim[nb];
-          if (diag) ++diag->first_order_fallback;
-        }
-      }
-      const Cons<R> ul = prim_to_cons(L, g);
-      const Cons<R> ur = prim_to_cons(Rt, g);
-      const R fb = face_beta(ul, ur, g, c.precond);
-      const R s = acoustic_scale(c, ul, ur, fb);
-      bool fbk = false;
-      F = face_flux(ul, ur, n, g, c.scheme, &w, s, c.as_fix, c.as_cutoff, c.as_fmin, c.as_ramp, &fbk);
-      if (diag && fbk) ++diag->diffusive_fallback;
-    } else {
-      const Cons<R> in = prim_to_cons(prim[o], g);
-      const int bc = m.bc[f];
-      Cons<R> ex = c.farfield;
-      R fb;
-      if (bc == kSlipWall) {
-        fb = face_beta(in, in, g, c.precond);
-        ex = in;
-      } else {
-        fb = face_beta(in, c.farfield, g, c.precond);
-      }
-      const R s = acoustic_scale(c, in, ex, fb);
-      bool fbk = false;
-      F = boundary_flux(bc, in, c.farfield, n, c.scheme, g, s, c.precond_farfield_bc, c.as_fix, c.as_cutoff,
-                        c.as_fmin, c.as_ramp, &w, &fbk);
-      if (diag && fbk) ++diag->diffusive_fallback;
+    A5<R> F;
+    R w;
+    bool fo, fbk;
+    face_contribution(m, c, prim, grad, f, &F, &w, &fo, &fbk);
+    if (diag && fo) {
+      if (diag->first_failure_face < 0) diag->first_failure_face = f;
+      ++diag->first_order_fallback;
     }
+    if (diag && fbk) ++diag->diffusive_fallback;
     const R wave = w * A;
     (*lam)[o] += wave;
     for (int k = 0; k < 5; ++k) {

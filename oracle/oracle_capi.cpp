@@ -4,6 +4,9 @@
 // one per scalar type (_f32 reproduces the reference's FP32 arithmetic, _f64 is the FP64 parity oracle).
 // Loaded by tests/ (ctypes) and by bench.py's cpu_baseline leg only.
 #include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstring>
 
 #include "aero_oracle.hpp"
@@ -365,7 +368,146 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// ---- the same evaluation on all host cores (the CPU baseline's threaded line) ------------------------
+// Per-face values in parallel, then per cell the sum over its faces in increasing face index: the serial
+// loops' accumulation order, so the results are the serial ones bit for bit.  Green-Gauss only.
+struct CellFaces {
+  std::vector<int> off, ent;  // per cell, its faces in increasing index: face << 1 | (cell is the neighbour)
+};
+
+template <class R> CellFaces cell_faces(const MeshView<R>& m) {
+  CellFaces cf;
+  cf.off.assign(static_cast<std::size_t>(m.num_cells) + 1, 0);
+  for (int f = 0; f < m.num_faces; ++f) {
+    ++cf.off[m.owner[f] + 1];
+    if (m.neighbor[f] >= 0) ++cf.off[m.neighbor[f] + 1];
+  }
+  for (int c = 0; c < m.num_cells; ++c) cf.off[c + 1] += cf.off[c];
+  cf.ent.resize(cf.off[m.num_cells]);
+  std::vector<int> fill(cf.off.begin(), cf.off.end() - 1);
+  for (int f = 0; f < m.num_faces; ++f) {
+    cf.ent[fill[m.owner[f]]++] = f << 1;
+    if (m.neighbor[f] >= 0) cf.ent[fill[m.neighbor[f]]++] = (f << 1) | 1;
+  }
+  return cf;
+}
+
+template <class R>
+void residual_threaded(const MeshView<R>& v, const AssemblyCfg<R>& a, const CellFaces& cf, const R* U,
+                       std::vector<Prim<R>>* p, std::vector<R>* qf, std::vector<R>* gr, std::vector<R>* fa,
+                       std::vector<R>* r, std::vector<R>* l, AssemblyDiag* d) {
+  const int N = v.num_cells, F = v.num_faces;
+  p->resize(N);
+  qf->resize(static_cast<std::size_t>(F) * 5);
+  gr->assign(static_cast<std::size_t>(N) * 10, R(0));
+  fa->resize(static_cast<std::size_t>(F) * 6);
+  r->resize(static_cast<std::size_t>(N) * 5);
+  l->resize(N);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) (*p)[c] = cons_to_prim(cons_at(U, c), a.gamma);
+  if (a.second_order && v.gradient == 1) {
+    cell_gradients(v, *p, gr);  // least squares: the serial routine
+  } else if (a.second_order) {
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < F; ++f)
+      for (int k = 0; k < 5; ++k) {
+        const R qo = prim_var((*p)[v.owner[f]], k);
+        R q = qo;
+        if (v.neighbor[f] >= 0) q = AO_K(0.5) * (qo + prim_var((*p)[v.neighbor[f]], k));
+        (*qf)[static_cast<std::size_t>(5 * f + k)] = q;
+      }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < N; ++c) {
+      const R iv = AO_K(1.0) / smax(v.volume[c], AO_K(1.0e-12));
+      for (int k = 0; k < 5; ++k)
+        for (int dd = 0; dd < 2; ++dd) {
+          R acc = R(0);
+          for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
+            const int f = cf.ent[e] >> 1;
+            const R nA = v.normal[3 * f + dd] * v.area[f];
+            const R t = (*qf)[static_cast<std::size_t>(5 * f + k)] * nA;
+            if (cf.ent[e] & 1) acc -= t; else acc += t;
+          }
+          (*gr)[static_cast<std::size_t>((c * 5 + k) * 2 + dd)] = acc * iv;
+        }
+    }
+  }
+  int n_fo = 0, n_diff = 0, first = F;
+#pragma omp parallel for schedule(static) reduction(+ : n_fo, n_diff) reduction(min : first)
+  for (int f = 0; f < F; ++f) {
+    A5<R> Fl;
+    R w;
+    bool fo, fbk;
+    face_contribution(v, a, *p, *gr, f, &Fl, &w, &fo, &fbk);
+    const R A = v.area[f];
+    for (int k = 0; k < 5; ++k) (*fa)[static_cast<std::size_t>(6 * f + k)] = Fl[k] * A;
+    (*fa)[static_cast<std::size_t>(6 * f + 5)] = w * A;
+    if (fo) {
+      ++n_fo;
+      first = std::min(first, f);
+    }
+    if (fbk) ++n_diff;
+  }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) {
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
+    R w = R(0);
+    for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
+      const std::size_t f = static_cast<std::size_t>(cf.ent[e] >> 1);
+      for (int k = 0; k < 5; ++k) {
+        if (cf.ent[e] & 1) acc[k] -= (*fa)[6 * f + k]; else acc[k] += (*fa)[6 * f + k];
+      }
+      w += (*fa)[6 * f + 5];
+    }
+    for (int k = 0; k < 5; ++k) (*r)[static_cast<std::size_t>(5 * c + k)] = acc[k];
+    (*l)[c] = w;
+  }
+  d->first_order_fallback = n_fo;
+  d->diffusive_fallback = n_diff;
+  d->first_failure_face = first < F ? first : -1;
+}
+
+template <class R>
+void residual_threaded_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, R* res, R* lam, int* diag) {
+  const MeshView<R> v = view<R>(m);
+  const AssemblyCfg<R> a = acfg<R>(c);
+  const CellFaces cf = cell_faces(v);
+  std::vector<Prim<R>> p;
+  std::vector<R> qf, gr, fa, r, l;
+  AssemblyDiag d;
+  residual_threaded(v, a, cf, U, &p, &qf, &gr, &fa, &r, &l, &d);
+  std::memcpy(res, r.data(), r.size() * sizeof(R));
+  std::memcpy(lam, l.data(), l.size() * sizeof(R));
+  if (diag) {
+    diag[0] = d.first_order_fallback;
+    diag[1] = d.diffusive_fallback;
+    diag[2] = d.first_failure_face;
+  }
+}
+
+template <class R>
+double time_residual_threaded_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, int n_evals) {
+  const MeshView<R> v = view<R>(m);
+  const AssemblyCfg<R> a = acfg<R>(c);
+  const CellFaces cf = cell_faces(v);
+  std::vector<Prim<R>> p;
+  std::vector<R> qf, gr, fa, r, l;
+  AssemblyDiag d;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0; it < n_evals; ++it) residual_threaded(v, a, cf, U, &p, &qf, &gr, &fa, &r, &l, &d);
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
 }  // namespace
+
+extern "C" int ao_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 #define AO_EXPORT_ALL(SUF, T)                                                                                    \
   extern "C" void ao_limiter_##SUF(int lim, const T* a, const T* b, T* out, int n) { limiter_t<T>(lim, a, b, out, n); } \
@@ -399,6 +541,14 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
   }                                                                                                                      \
   extern "C" double ao_time_residual_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, int n_evals) {      \
     return time_residual_t<T>(m, c, U, n_evals);                                                                         \
+  }                                                                                                                      \
+  extern "C" void ao_residual_threaded_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, T* res, T* lam,    \
+                                             int* diag) {                                                                \
+    residual_threaded_t<T>(m, c, U, res, lam, diag);                                                                     \
+  }                                                                                                                      \
+  extern "C" double ao_time_residual_threaded_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U,             \
+                                                    int n_evals) {                                                       \
+    return time_residual_threaded_t<T>(m, c, U, n_evals);                                                                \
   }
 
 AO_EXPORT_ALL(f32, float)

@@ -55,6 +55,7 @@ def oracle() -> C.CDLL:
         assert lib.ao_hist_fields() == HIST_FIELDS
         for suf in ("f32", "f64"):
             getattr(lib, f"ao_time_residual_{suf}").restype = C.c_double
+            getattr(lib, f"ao_time_residual_threaded_{suf}").restype = C.c_double
         _ORACLE = lib
     return _ORACLE
 
@@ -138,6 +139,30 @@ def time_residual(om: OracleMesh, cfg, U, n_evals: int) -> float:
     dt = om.dtype
     U = np.ascontiguousarray(U, dt).reshape(om.N, 5)
     return getattr(oracle(), f"ao_time_residual_{suffix(dt)}")(C.byref(om.c), C.byref(cfg), ptr(U), n_evals)
+
+
+def residual_threaded(om: OracleMesh, cfg, U):
+    """The residual on all OpenMP threads (per-face values, then per-cell sums in face order: the serial
+    result bit for bit) -> (R[N,5], lam[N], diag[3])."""
+    dt = om.dtype
+    U = np.ascontiguousarray(U, dt).reshape(om.N, 5)
+    R = np.zeros((om.N, 5), dt)
+    lam = np.zeros(om.N, dt)
+    diag = np.zeros(3, np.int32)
+    getattr(oracle(), f"ao_residual_threaded_{suffix(dt)}")(C.byref(om.c), C.byref(cfg), ptr(U), ptr(R), ptr(lam),
+                                                            ptr(diag))
+    return R, lam, diag
+
+
+def time_residual_threaded(om: OracleMesh, cfg, U, n_evals: int) -> float:
+    dt = om.dtype
+    U = np.ascontiguousarray(U, dt).reshape(om.N, 5)
+    return getattr(oracle(), f"ao_time_residual_threaded_{suffix(dt)}")(C.byref(om.c), C.byref(cfg), ptr(U), n_evals)
+
+
+def max_threads() -> int:
+    """OpenMP threads the threaded oracle uses (OMP_NUM_THREADS, else the host's cores)."""
+    return int(oracle().ao_max_threads())
 
 
 def residual_jvp(om: OracleMesh, cfg, U, dU=None, seed=None, frozen=None):

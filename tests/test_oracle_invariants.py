@@ -291,3 +291,19 @@ def test_lsq_second_order_residual_preserves_freestream(oracle_lib):
     off_wall[m.owner[m.wall_start: m.wall_start + m.wall_count]] = False
     comp = [0, 1, 2, 4]
     assert (np.abs(R[off_wall][:, comp]) / scale[off_wall][:, comp]).max() < 1e-13
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_threaded_oracle_equals_serial(dtype):
+    """The all-core oracle (the CPU baseline's threaded line) gives the serial oracle's residual, spectral
+    radius and fallback counters bit for bit, for every option combination on a jittered mixed mesh."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=9, rcm=True)
+    om = ol.OracleMesh(m, dtype)
+    ff = np.asarray(__import__("aero_hip").freestream(0.5, 1.25)[1])
+    for amp, seed in ((0.05, 3), (0.9, 4)):
+        U = helpers.perturbed_state(m.cell_center, ff, amp, seed)
+        for combo in cases.RESIDUAL_COMBOS:
+            cfg = helpers.res_cfg(cases.residual_cfg_values(combo, ff), dtype)
+            a = ol.residual(om, cfg, U)
+            b = ol.residual_threaded(om, cfg, U)
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), (combo, amp)
