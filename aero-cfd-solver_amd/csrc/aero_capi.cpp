@@ -403,7 +403,8 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   }
   m->S = slot_off[m->nblk];
   const int grad = opts ? opts->gradient : AERO_GRAD_GREEN_GAUSS;
-  const size_t lds_per_slot = grad == AERO_GRAD_LEAST_SQUARES ? 8 : 6;  // the gradient kernels' staging
+  // doubles staged per slot: the larger of the gradient kernel's (LSQ 8, Green-Gauss 4) and the flux kernel's (5)
+  const size_t lds_per_slot = grad == AERO_GRAD_LEAST_SQUARES ? 8 : 5;
   if (static_cast<size_t>(smax) * lds_per_slot * sizeof(double) > 64 * 1024) {
     delete m;
     set_err(err, errlen, "aero_hip_mesh_create: a block touches " + std::to_string(smax) +

@@ -133,12 +133,12 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   const int s0 = m.slot_off[b];
   const int ns = m.slot_off[b + 1] - s0;
   const int smax = m.smax;
+  // Phase 1 stages the four face values only (4 doubles per slot: more workgroups fit the LDS); phase 2
+  // forms n A from the slot stream again, the same product.
   double* q0 = lds;
   double* q1 = lds + smax;
   double* q2 = lds + 2 * smax;
   double* q3 = lds + 3 * smax;
-  double* nax = lds + 4 * smax;
-  double* nay = lds + 5 * smax;
   const int ld = m.ld;
   for (int i = threadIdx.x; i < ns; i += BC) {
     const int s = s0 + i;
@@ -153,13 +153,10 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
       a2 = 0.5 * (wo.v + wn.v);
       a3 = 0.5 * (wo.p + wn.p);
     }
-    const double A = m.s_A[s];
     q0[i] = a0;
     q1[i] = a1;
     q2[i] = a2;
     q3[i] = a3;
-    nax[i] = m.s_nx[s] * A;
-    nay[i] = m.s_ny[s] * A;
   }
   __syncthreads();
   const int c = b * BC + threadIdx.x;
@@ -169,7 +166,8 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   for (int e = m.c_off[c]; e < e1; ++e) {
     const int ent = m.c_ent[e];
     const int l = ent >> 1;
-    const double x = nax[l], y = nay[l];
+    const double A = m.s_A[s0 + l];
+    const double x = m.s_nx[s0 + l] * A, y = m.s_ny[s0 + l] * A;
     const double v0 = q0[l], v1 = q1[l], v2 = q2[l], v3 = q3[l];
     if (ent & 1) {
       gx0 -= v0 * x; gy0 -= v0 * y;
@@ -204,7 +202,7 @@ __device__ __forceinline__ void gradient_body(const DevMesh& m, int b, const dou
 }
 
 // LDS doubles per face slot of the gradient kernels.
-constexpr int grad_lds_per_slot(int gk) { return gk == 1 ? 8 : 6; }
+constexpr int grad_lds_per_slot(int gk) { return gk == 1 ? 8 : 4; }
 
 template <int BC, int GK>
 __global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
