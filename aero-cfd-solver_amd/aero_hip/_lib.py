@@ -18,6 +18,9 @@ BC_INTERIOR, BC_FARFIELD, BC_SLIP_WALL = 0, 1, 2
 CELLS = {"tri": 0, "quad": 1, "mixed": 2}
 HIST_FIELDS = 12
 H_ITER, H_L1, H_L2, H_LINF, H_RATIO, H_CL, H_CD, H_CM, H_RHO_L2, H_CFL, H_FO, H_DIFF = range(12)
+# per-face scaling record columns (AERO_FS_*) of aero_hip_face_scales
+(FS_MACH, FS_UN_ABS, FS_SOUND, FS_BETA, FS_LOW_MACH, FS_ACOUSTIC_BASE, FS_STAB, FS_ACOUSTIC, FS_ALL_SPEED,
+ FS_SMAX_PHYSICAL, FS_SMAX_DISSIPATION, FS_WALL_MASS_FLUX, FS_FIELDS) = range(13)
 
 i32, f64, vp = C.c_int32, C.c_double, C.c_void_p
 PI = C.POINTER(C.c_int32)
@@ -81,7 +84,7 @@ class CaseConfigC(C.Structure):
 class RunOutput(C.Structure):
     _fields_ = [("hist_capacity", i32), ("n_iter", i32), ("nonfinite", i32), ("history", PD),
                 ("conserved", PD), ("residual", PD), ("spectral_radius", PD), ("forces", PD),
-                ("wall_cp", PD), ("device_seconds", f64)]
+                ("wall_cp", PD), ("device_seconds", f64), ("final_config", C.POINTER(ResidualConfig))]
 
 
 class MeshGenConfig(C.Structure):
@@ -200,6 +203,7 @@ PROTOTYPES = {
     "aero_hip_tangent_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, C.POINTER(TangentSeed),
                                          C.POINTER(LinearSolve), PD, PD, C.c_char_p, C.c_size_t]),
     "aero_hip_forces": (C.c_int, [vp, PD, C.POINTER(FreestreamRef), i32, f64, PD, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_face_scales": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, PD, C.c_char_p, C.c_size_t]),
     "aero_hip_adjoint_solve_batch": (C.c_int, [C.POINTER(AdjointCaseC), i32, C.POINTER(LinearSolve), i32,
                                                PROGRESS_FN, vp, C.c_char_p, C.c_size_t]),
 }

@@ -12,7 +12,11 @@ column layouts,
   cp_wall.csv     s,x,y,Cp over the wall faces ordered from the trailing edge over the upper surface
                   (post/forces.cpp:39-129, :177-206; euler_solver.cpp:1467-1476)
   field_0000.vtu  cell rho, u, v, p, Mach, residual_magnitude (io/io_vtk_stub.cpp:95-166)
-  run.log         the run summary lines (residual_stub.cpp:600-700, the subset this backend knows)
+  wall_flux.csv   face,x,y,mass_flux_n over the slip-wall faces (euler_solver.cpp:1304-1331)
+  dissipation_debug.csv  the scaling record of ten wall-adjacent interior and ten farfield faces
+                  (:1333-1465)
+  run.log         the run summary lines (residual_stub.cpp:600-762, the subset this backend knows, with
+                  the beta / local-Mach / acoustic-scale statistics of the final state)
 
 and returns the same summary dict as the pybind run_case.  Numbers are printed like a C++ ostream's
 default float format (six significant digits).
@@ -191,6 +195,90 @@ def write_outputs(out_dir: Path, mesh: Mesh, case: CaseConfig, out: dict) -> dic
     return paths
 
 
+DISSIPATION_REGIONS = ("wall_adjacent_interior", "farfield_boundary")
+
+
+def _quantile_sorted(v: np.ndarray, q: float) -> float:
+    """linear_quantile_from_sorted (euler_solver.cpp:380-390)."""
+    if v.size == 0:
+        return 1.0
+    idx = min(max(q, 0.0), 1.0) * (v.size - 1)
+    i0 = int(math.floor(idx))
+    i1 = min(i0 + 1, v.size - 1)
+    t = idx - i0
+    return float((1.0 - t) * v[i0] + t * v[i1])
+
+
+def _wall_adjacent_faces(mesh: Mesh) -> np.ndarray:
+    """Interior faces with a cell that owns a slip-wall face, in face order (euler_solver.cpp:1334-1355)."""
+    wall = np.flatnonzero(mesh.bc == _lib.BC_SLIP_WALL)
+    mask = np.zeros(mesh.num_cells, bool)
+    mask[mesh.owner[wall]] = True
+    nb = mesh.neighbor
+    interior = nb >= 0
+    touch = np.zeros(mesh.num_faces, bool)
+    touch[interior] = mask[mesh.owner[interior]] | mask[nb[interior]]
+    return np.flatnonzero(touch)
+
+
+def scaling_statistics(mesh: Mesh, fs: np.ndarray, cs: np.ndarray) -> dict:
+    """run.log's beta, local-Mach and acoustic-scale statistics of a state (EulerRunResult fields,
+    euler_solver.cpp:1217-1402) from aero_hip_face_scales records: fs [FS_FIELDS][F], cs [2][N]."""
+    st = {}
+    beta, mach = cs[0], cs[1]
+    if mesh.num_cells > 0:
+        st.update(beta_min=beta.min(), beta_max=beta.max(), beta_mean=beta.mean(), mach_local_min=mach.min(),
+                  mach_local_max=mach.max(), mach_local_mean=mach.mean())
+    else:
+        st.update(beta_min=1.0, beta_max=1.0, beta_mean=1.0, mach_local_min=0.0, mach_local_max=0.0,
+                  mach_local_mean=0.0)
+    base, final = fs[_lib.FS_ACOUSTIC_BASE], fs[_lib.FS_ACOUSTIC]
+    for key, v in (("acoustic_scale", base), ("acoustic_scale_final", final)):
+        st.update({f"{key}_min": v.min() if v.size else 1.0, f"{key}_max": v.max() if v.size else 1.0,
+                   f"{key}_mean": v.mean() if v.size else 1.0})
+    wa = _wall_adjacent_faces(mesh)
+    for key, col in (("wall_adjacent_acoustic_scale", _lib.FS_ACOUSTIC_BASE),
+                     ("wall_adjacent_stabilization_scale", _lib.FS_STAB),
+                     ("wall_adjacent_acoustic_scale_final", _lib.FS_ACOUSTIC)):
+        v = np.sort(fs[col, wa])
+        st.update({f"{key}_min": float(v[0]) if v.size else 1.0, f"{key}_p01": _quantile_sorted(v, 0.01),
+                   f"{key}_p50": _quantile_sorted(v, 0.50)})
+    wall = mesh.bc == _lib.BC_SLIP_WALL
+    st["max_wall_mass_flux"] = float(np.abs(fs[_lib.FS_WALL_MASS_FLUX, wall]).max()) if wall.any() else 0.0
+    return {k: float(v) for k, v in st.items()}
+
+
+def dissipation_faces(mesh: Mesh) -> list[tuple[int, str]]:
+    """The faces of dissipation_debug.csv: the first ten wall-adjacent interior faces, then the first ten
+    farfield faces, in face order (euler_solver.cpp:1445-1463)."""
+    far = np.flatnonzero(mesh.bc == _lib.BC_FARFIELD)
+    return ([(int(f), DISSIPATION_REGIONS[0]) for f in _wall_adjacent_faces(mesh)[:10]] +
+            [(int(f), DISSIPATION_REGIONS[1]) for f in far[:10]])
+
+
+def write_post_files(out_dir: Path, mesh: Mesh, cfg: _lib.ResidualConfig, fs: np.ndarray) -> dict:
+    """wall_flux.csv (euler_solver.cpp:1304-1331) and dissipation_debug.csv (:1406-1463) from the
+    aero_hip_face_scales records fs of the final state; returns the paths."""
+    paths = dict(wall_flux=out_dir / "wall_flux.csv", dissipation=out_dir / "dissipation_debug.csv")
+    with open(paths["wall_flux"], "w", encoding="utf-8") as f:
+        f.write("face,x,y,mass_flux_n\n")
+        for face in np.flatnonzero(mesh.bc == _lib.BC_SLIP_WALL):
+            f.write(f"{face},{_g(mesh.face_center[face, 0])},{_g(mesh.face_center[face, 1])},"
+                    f"{_g(fs[_lib.FS_WALL_MASS_FLUX, face])}\n")
+    m0 = min(max(cfg.stab_mach_floor, 0.0), 1.0)
+    cols = (_lib.FS_MACH, _lib.FS_UN_ABS, _lib.FS_SOUND, _lib.FS_BETA, _lib.FS_LOW_MACH, _lib.FS_ACOUSTIC_BASE,
+            _lib.FS_STAB, _lib.FS_ACOUSTIC)
+    with open(paths["dissipation"], "w", encoding="utf-8") as f:
+        f.write("face,region,x,y,area,mach_local,un_abs,a,beta,low_mach_scale,acoustic_scale,s_stab,"
+                "acoustic_scale_final,stabilization_m0,all_speed_pressure_scale,smax_physical,smax_dissipation\n")
+        for face, region in dissipation_faces(mesh):
+            vals = [mesh.face_center[face, 0], mesh.face_center[face, 1], mesh.area[face],
+                    *(fs[c, face] for c in cols), m0, fs[_lib.FS_ALL_SPEED, face], fs[_lib.FS_SMAX_PHYSICAL, face],
+                    fs[_lib.FS_SMAX_DISSIPATION, face]]
+            f.write(f"{face},{region}," + ",".join(_g(v) for v in vals) + "\n")
+    return paths
+
+
 def case_and_mesh(kv: dict, mesh_path=None) -> tuple[CaseConfig, Mesh]:
     """CaseConfig and the reference generator's O-grid (FP32 point set, FP64 arrays) of native keys, or
     the mesh of a binary mesh file (save_mesh) when mesh_path is given."""
@@ -222,9 +310,12 @@ def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int =
     out_dir.mkdir(parents=True, exist_ok=True)
     with HipMesh(mesh, block_cells=block_cells) as hm:
         out = hm.run_case(case)
-    if out["nonfinite"]:
-        raise RuntimeError("Euler pseudo-time iteration produced a non-finite state.")
+        if out["nonfinite"]:
+            raise RuntimeError("Euler pseudo-time iteration produced a non-finite state.")
+        fs, cs = hm.face_scales(out["final_config"], out["conserved"])
     write_outputs(out_dir, mesh, case, out)
+    post = write_post_files(out_dir, mesh, out["final_config"], fs)
+    stats = scaling_statistics(mesh, fs, cs)
     H = out["history"]
     last = H[-1]
     summary = dict(status="ok", backend="hip", case_type=case_type, run_log=str(out_dir / "run.log"),
@@ -249,6 +340,9 @@ def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int =
              f"precond={'on' if case.precond_on else 'off'}",
              f"low_mach_fix={'on' if case.low_mach_fix else 'off'}",
              f"all_speed_flux_fix={'on' if case.all_speed_flux_fix else 'off'}",
+             f"stabilization_mach_floor_final={_g(out['final_config'].stab_mach_floor)}",
+             *(f"{k}={_g(v)}" for k, v in stats.items() if k != "max_wall_mass_flux"),
+             f"dissipation_debug_csv={post['dissipation']}",
              f"device_seconds={out['seconds']:.6g}"]
     (out_dir / "run.log").write_text("\n".join(lines) + "\n", encoding="utf-8")
     return summary

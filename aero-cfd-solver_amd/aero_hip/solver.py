@@ -237,13 +237,24 @@ class HipMesh:
         lam = np.zeros(self.N)
         forces = np.zeros(8)
         cp = np.zeros(max(self.mesh.wall_count, 1))
-        out = _lib.RunOutput(cap, 0, 0, _pd(hist), _pd(U), _pd(R), _pd(lam), _pd(forces), _pd(cp), 0.0)
+        final = _lib.ResidualConfig()
+        out = _lib.RunOutput(cap, 0, 0, _pd(hist), _pd(U), _pd(R), _pd(lam), _pd(forces), _pd(cp), 0.0,
+                             C.pointer(final))
         cc = case.to_c()
         self._call("aero_hip_run_case", C.byref(cc), C.byref(out))
         n = out.n_iter
         return dict(history=hist[:n].copy(), n_iter=n, nonfinite=bool(out.nonfinite), conserved=U, residual=R,
                     spectral_radius=lam, forces=forces, cl=forces[5], cd=forces[6], cm=forces[7],
-                    wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds)
+                    wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds, final_config=final)
+
+    def face_scales(self, cfg: _lib.ResidualConfig, U=None) -> tuple[np.ndarray, np.ndarray]:
+        """The post-run scaling diagnostics at U (None: the handle's state) under cfg (aero_hip_face_scales):
+        per-face records [FS_FIELDS][F] in face order and per-cell (beta, local Mach) [2][N]."""
+        _, pu = self._state_ptr(U)
+        fs = np.zeros((_lib.FS_FIELDS, self.mesh.num_faces))
+        cs = np.zeros((2, self.N))
+        self._call("aero_hip_face_scales", C.byref(cfg), pu, _pd(fs), _pd(cs))
+        return fs, cs
 
 
     def forces(self, rho_inf: float, p_inf: float, aoa_deg: float, speed_inf: float, chord: float = 1.0,

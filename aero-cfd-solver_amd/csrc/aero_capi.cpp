@@ -721,6 +721,57 @@ int aero_hip_forces(aero_hip_mesh* m, const double* conserved, const aero_freest
   return AERO_OK;
 }
 
+static_assert(aero_dev::kFsFields == AERO_FS_FIELDS && aero_dev::kFsWallMassFlux == AERO_FS_WALL_MASS_FLUX &&
+                  aero_dev::kFsAcoustic == AERO_FS_ACOUSTIC,
+              "face-scale columns of the kernel and the public header differ");
+
+int aero_hip_face_scales(aero_hip_mesh* m, const aero_residual_config* cfg, const double* conserved,
+                         double* face_scales, double* cell_scales, char* err, size_t errlen) {
+  if (!m || !cfg || !face_scales) {
+    set_err(err, errlen, "aero_hip_face_scales: null argument");
+    return AERO_EINVAL;
+  }
+  if (!(cfg->gamma > 1.0)) {
+    set_err(err, errlen, "aero_hip_face_scales: gamma must be > 1");
+    return AERO_EINVAL;
+  }
+  const size_t F = m->F, N = m->N;
+  const size_t nf = AERO_FS_FIELDS * F, nc = 2 * N, nu = conserved ? 4 * N : 0;
+  void* q = nullptr;
+  AERO_HIP_TRY(hipMalloc(&q, (nf + nc + nu) * sizeof(double)), "aero_hip_face_scales: hipMalloc");
+  struct Free {
+    void* p;
+    ~Free() { (void)hipFree(p); }
+  } guard{q};
+  double* d_faces = static_cast<double*>(q);
+  double* d_cells = d_faces + nf;
+  const double* U = m->U[m->cur];
+  if (conserved) {  // evaluated on a copy: the handle's state stays as it is
+    std::vector<double> soa;
+    to_internal_state(m, conserved, &soa);
+    double* d_u = d_cells + nc;
+    AERO_HIP_TRY(hipMemcpyAsync(d_u, soa.data(), nu * sizeof(double), hipMemcpyHostToDevice, m->stream),
+                 "aero_hip_face_scales: state copy");
+    U = d_u;
+  }
+  const FaceOpts o = face_opts(cfg);
+  AERO_HIP_TRY(aero_host::launch_face_scales(m->dm, o, U, d_faces, d_cells, m->stream), "k_face_scales");
+  AERO_HIP_TRY(hipMemcpyAsync(face_scales, d_faces, nf * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+               "aero_hip_face_scales: copy");
+  std::vector<double> cells;
+  if (cell_scales) {
+    cells.resize(nc);
+    AERO_HIP_TRY(hipMemcpyAsync(cells.data(), d_cells, nc * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "aero_hip_face_scales: cell copy");
+  }
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_face_scales: hipStreamSynchronize");
+  for (size_t c = 0; c < (cell_scales ? N : 0); ++c) {
+    cell_scales[m->perm[c]] = cells[c];
+    cell_scales[N + m->perm[c]] = cells[N + c];
+  }
+  return AERO_OK;
+}
+
 int aero_hip_get_gradients(aero_hip_mesh* m, double* gradients, char* err, size_t errlen) {
   if (!m || !gradients) {
     set_err(err, errlen, "aero_hip_get_gradients: null argument");
@@ -1277,5 +1328,6 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     const double f8[8] = {last.fx, last.fy, last.lift, last.drag, last.moment, last.cl, last.cd, last.cm};
     std::memcpy(out->forces, f8, sizeof(f8));
   }
+  if (out->final_config) *out->final_config = rc;
   return AERO_OK;
 }

@@ -13,6 +13,12 @@ namespace aero_dev {
 constexpr int kMetaBcMask = 3;
 constexpr int kMetaPrimary = 4;
 
+// Columns of the per-face scaling record (k_face_scales; AERO_FS_* in the public header).
+enum : int {
+  kFsMach = 0, kFsUnAbs, kFsSound, kFsBeta, kFsLowMach, kFsAcousticBase, kFsStab, kFsAcoustic, kFsAllSpeed,
+  kFsSmaxPhysical, kFsSmaxDissipation, kFsWallMassFlux, kFsFields
+};
+
 // Device view of a mesh in internal (block) order.  Passed by value to kernels.
 struct DevMesh {
   int N;         // cells
@@ -199,6 +205,8 @@ hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, 
 // Frozen limiter ratios [8][F] at U with gradients G (k_limiter_capture).
 // Frozen limiter ratios [8][F] -> per face slot [8][S] (k_ratio_slots).
 hipError_t launch_ratio_slots(const aero_dev::DevMesh& m, const double* ratios, double* slots, hipStream_t s);
+hipError_t launch_face_scales(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, const double* U, double* faces,
+                              double* cells, hipStream_t s);
 hipError_t launch_limiter_capture(const aero_dev::DevMesh& m, int limiter, const double* U, const double* G, double g,
                                   double* ratios, hipStream_t s);
 hipError_t launch_global_update(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, const double* U,

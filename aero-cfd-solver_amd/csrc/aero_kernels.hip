@@ -507,6 +507,60 @@ __global__ __launch_bounds__(256) void k_ratio_slots(DevMesh m, const double* __
   for (int j = 0; j < 8; ++j) out[static_cast<size_t>(j) * m.nslots + s] = fr[static_cast<size_t>(j) * m.nfaces + f];
 }
 
+// Per-face acoustic scaling record of a state, the post-processing diagnostics of the reference driver:
+// compute_acoustic_scaling_info (euler_solver.cpp:392-416) plus the dissipation row's speeds and
+// all-speed scale (:1412-1437).  The exterior state is the interior one at a slip wall and the freestream
+// at a farfield (:1270-1279); cell states are taken as stored, since every quantity converts to
+// primitives with the floors.  out[col][face] in the reference's face numbering, columns kFs*.
+__global__ __launch_bounds__(256) void k_face_scales(DevMesh m, FaceOpts o, const double* __restrict__ U,
+                                                     double* __restrict__ out) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= m.nslots) return;
+  const int meta = m.s_meta[s];
+  if (!(meta & kMetaPrimary)) return;
+  const int f = m.s_face[s];
+  const int cn = m.s_n[s];
+  const bool wall = cn < 0 && (meta & kMetaBcMask) == 2;
+  const double g = o.g;
+  const double nx = m.s_nx[s], ny = m.s_ny[s];
+  const C4 l = load_u(U, m.ld, m.s_o[s]);
+  const C4 r = cn >= 0 ? load_u(U, m.ld, cn) : (wall ? l : o.ff);
+  const P4 pl = c2p(l, g), pr = c2p(r, g);
+  const double unl = pl.u * nx + pl.v * ny, unr = pr.u * nx + pr.v * ny;
+  const double al = sound(pl, g), ar = sound(pr, g);
+  const double fm = 0.5 * (local_mach(pl, g) + local_mach(pr, g));
+  const double beta = face_beta(l, r, o);
+  double base = o.precond ? dclamp(beta, 1.0e-4, 1.0) : 1.0;
+  const double lms = o.low_mach_fix ? low_mach_scale(l, r, o) : 1.0;
+  base = dclamp(dmin(base, lms), 1.0e-4, 1.0);
+  const double stab = dclamp(smooth_max(fm, dclamp(o.stab_floor, 0.0, 1.0)), 1.0e-4, 1.0);
+  const double sc = dmax(base, stab);
+  const size_t F = static_cast<size_t>(m.nfaces);
+  out[kFsMach * F + f] = fm;
+  out[kFsUnAbs * F + f] = 0.5 * (fabs(unl) + fabs(unr));
+  out[kFsSound * F + f] = 0.5 * (al + ar);
+  out[kFsBeta * F + f] = beta;
+  out[kFsLowMach * F + f] = lms;
+  out[kFsAcousticBase * F + f] = base;
+  out[kFsStab * F + f] = stab;
+  out[kFsAcoustic * F + f] = sc;
+  out[kFsAllSpeed * F + f] = all_speed_scale(pl, pr, nx, ny, o);
+  out[kFsSmaxPhysical * F + f] = dmax(fabs(unl) + al, fabs(unr) + ar);
+  out[kFsSmaxDissipation * F + f] = dmax(pspeeds(unl, al, sc).sr, pspeeds(unr, ar, sc).sr);
+  // mass component of the slip-wall flux evaluated at the cell state (euler_solver.cpp:1312-1324)
+  out[kFsWallMassFlux * F + f] = wall ? boundary_face<0>(l, true, nx, ny, o).f0 : 0.0;
+}
+
+// Per-cell preconditioner beta and local Mach (euler_solver.cpp:1229-1246), out[2][N] in internal order.
+__global__ __launch_bounds__(256) void k_cell_scales(DevMesh m, FaceOpts o, const double* __restrict__ U,
+                                                     double* __restrict__ out) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= m.N) return;
+  const P4 p = c2p(load_u(U, m.ld, c), o.g);
+  out[c] = o.precond ? cell_beta(p, o) : 1.0;
+  out[static_cast<size_t>(m.N) + c] = local_mach(p, o.g);
+}
+
 __global__ void k_min_dt(int N, const double* __restrict__ lam, double cfl, double* __restrict__ out) {
   __shared__ double red[256];
   double v = 1.79769313486231570815e+308;
@@ -930,6 +984,15 @@ hipError_t launch_flux(const DevMesh& m, const FaceOpts& o, int scheme, int limi
                                : launch_flux_bc<512, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_face_scales(const DevMesh& m, const FaceOpts& o, const double* U, double* faces, double* cells,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_face_scales, dim3((m.nslots + 255) / 256), dim3(256), 0, s, m, o, U, faces);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cell_scales, dim3((m.N + 255) / 256), dim3(256), 0, s, m, o, U, cells);
+  return hipGetLastError();
 }
 
 hipError_t launch_ratio_slots(const DevMesh& m, const double* ratios, double* slots, hipStream_t s) {

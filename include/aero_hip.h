@@ -134,6 +134,8 @@ typedef struct aero_run_output {
   double* forces;          /* [8]: fx, fy, lift, drag, moment, cl, cd, cm (gauge) */
   double* wall_cp;         /* [wall_count] Cp in wall-patch order */
   double device_seconds;   /* wall time of the iteration loop */
+  aero_residual_config* final_config; /* may be NULL: the residual configuration of the last iteration
+                                         (ramped floors and all-speed weights, effective preconditioner) */
 } aero_run_output;
 
 typedef struct aero_hip_mesh aero_hip_mesh; /* opaque device mesh + state + stream */
@@ -219,6 +221,35 @@ typedef struct aero_freestream_ref {
  * orders the faces along the surface).  conserved: [N][5] or NULL for the handle's state. */
 int aero_hip_forces(aero_hip_mesh* mesh, const double* conserved, const aero_freestream_ref* ref, int32_t gauge,
                     double gamma, double* forces, double* wall_cp, char* err, size_t errlen);
+
+/* Columns of the per-face scaling record (AcousticScalingInfo, euler_solver.cpp:392-416, and the
+ * dissipation_debug.csv row, :1412-1437). */
+enum {
+  AERO_FS_MACH = 0,        /* face Mach, mean of the two sides' |v| / max(a, 1e-8) */
+  AERO_FS_UN_ABS,          /* mean |u.n| */
+  AERO_FS_SOUND,           /* mean speed of sound */
+  AERO_FS_BETA,            /* face preconditioner beta (1 when the preconditioner is off) */
+  AERO_FS_LOW_MACH,        /* low-Mach dissipation scale (1 when low_mach_fix is off) */
+  AERO_FS_ACOUSTIC_BASE,   /* acoustic_scale_base */
+  AERO_FS_STAB,            /* stabilization_scale */
+  AERO_FS_ACOUSTIC,        /* acoustic_scale (the one the flux uses) */
+  AERO_FS_ALL_SPEED,       /* all-speed HLLC pressure scale */
+  AERO_FS_SMAX_PHYSICAL,   /* max(|u_n| + a) */
+  AERO_FS_SMAX_DISSIPATION,/* max of the preconditioned spectral radii at acoustic_scale */
+  AERO_FS_WALL_MASS_FLUX,  /* slip walls: mass component of the wall flux; 0 elsewhere */
+  AERO_FS_FIELDS
+};
+
+/* The post-run scaling diagnostics of run_euler_airfoil_case (euler_solver.cpp:1217-1437), evaluated on
+ * the device at conserved ([N][5], or NULL for the handle's state) under cfg (the residual configuration
+ * of the last iteration, as aero_run_output.final_config returns it; the limiter fields are unused):
+ * face_scales[AERO_FS_FIELDS][F] per face in the reference's face numbering, with the exterior state the
+ * interior one at a slip wall and the freestream at a farfield (:1270-1279); cell_scales (may be NULL)
+ * [2][N]: cell preconditioner beta, then local Mach, in the reference's cell numbering (:1229-1246).
+ * The host writes dissipation_debug.csv, wall_flux.csv and run.log's beta / Mach / acoustic-scale
+ * statistics from these. */
+int aero_hip_face_scales(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
+                         double* face_scales, double* cell_scales, char* err, size_t errlen);
 
 /* The full pseudo-time run of run_euler_airfoil_case (euler_solver.cpp:463-1208) with the stage
  * loop, norms and forces on the device and the ramp / CFL / convergence control on the host. */

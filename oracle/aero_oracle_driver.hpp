@@ -67,6 +67,7 @@ template <class R> struct RunOut {
   std::vector<R> lam;    // last stage spectral radius [N]
   Coeffs<R> forces;      // gauge forces of the final state
   std::vector<R> cp;     // wall Cp in wall-patch order
+  AssemblyCfg<R> cfg;    // residual configuration of the last stage (frozen ratios not kept)
 };
 
 template <class R> R compute_cfl(const CaseCfg<R>& c, int iter, R scale) {  // euler_solver.cpp:69-79
@@ -264,6 +265,8 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
       ac.as_ramp = as_w;
       AssemblyDiag d;
       assemble_residual(m, ac, prim, grad, &res, &lam, &d);
+      out.cfg = ac;
+      out.cfg.frozen = nullptr;
       iter_diag.first_order_fallback += d.first_order_fallback;
       iter_diag.diffusive_fallback += d.diffusive_fallback;
       if (iter_diag.first_failure_face < 0 && d.first_failure_face >= 0) iter_diag.first_failure_face = d.first_failure_face;

@@ -348,6 +348,40 @@ int run_case_t(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, R* hist, R*
   }
 }
 
+// The residual configuration of a run's last stage (the ramped stabilisation floor, the all-speed
+// weights and the effective preconditioner), the configuration the post-run diagnostics use.
+template <class R> int run_case_cfg_t(const ao_mesh* m, const ao_case_cfg* c, ao_residual_cfg* out) {
+  try {
+    const AssemblyCfg<R> a = run_case(view<R>(m), ccfg<R>(c)).cfg;
+    *out = ao_residual_cfg{};
+    out->farfield[0] = a.farfield.rho;
+    out->farfield[1] = a.farfield.rhou;
+    out->farfield[2] = a.farfield.rhov;
+    out->farfield[3] = a.farfield.rhow;
+    out->farfield[4] = a.farfield.rhoE;
+    out->gamma = a.gamma;
+    out->scheme = a.scheme;
+    out->limiter = a.limiter;
+    out->second_order = a.second_order;
+    out->low_mach_fix = a.low_mach_fix;
+    out->low_mach_cutoff = a.low_mach_cutoff;
+    out->stab_mach_floor = a.stab_mach_floor;
+    out->precond_farfield_bc = a.precond_farfield_bc;
+    out->precond_enabled = a.precond.enabled;
+    out->mach_ref = a.precond.mach_ref;
+    out->mach_min = a.precond.mach_min;
+    out->beta_min = a.precond.beta_min;
+    out->beta_max = a.precond.beta_max;
+    out->as_fix = a.as_fix;
+    out->as_cutoff = a.as_cutoff;
+    out->as_fmin = a.as_fmin;
+    out->as_ramp = a.as_ramp;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 // CPU baseline timing: `n_evals` back-to-back residual evaluations (primitive + gradient + assembly),
 // timed with steady_clock around the loop only.  Returns seconds.
 template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cfg* c, const R* U, int n_evals) {
@@ -539,6 +573,9 @@ extern "C" int ao_max_threads() {
                                    T* lam, T* forces8, T* cp, int* nonfinite, char* err, int errlen) {                   \
     return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen);                        \
   }                                                                                                                      \
+  extern "C" int ao_run_case_cfg_##SUF(const ao_mesh* m, const ao_case_cfg* c, ao_residual_cfg* out) {              \
+    return run_case_cfg_t<T>(m, c, out);                                                                                 \
+  }                                                                                                                      \
   extern "C" double ao_time_residual_##SUF(const ao_mesh* m, const ao_residual_cfg* c, const T* U, int n_evals) {      \
     return time_residual_t<T>(m, c, U, n_evals);                                                                         \
   }                                                                                                                      \
@@ -586,6 +623,73 @@ extern "C" void ao_residual_frozen_f64(const ao_mesh* m, const ao_residual_cfg* 
     diag[0] = d.first_order_fallback;
     diag[1] = d.diffusive_fallback;
     diag[2] = d.first_failure_face;
+  }
+}
+
+// ---- post-run scaling diagnostics (FP64) ----------------------------------------------------------
+// The reference driver's per-face AcousticScalingInfo (euler_solver.cpp:392-416) with the exterior state
+// of its post-processing loop (:1270-1279: interior at a slip wall, freestream at a farfield), the
+// dissipation_debug.csv row's speeds and all-speed scale (:1412-1437) and the wall_flux.csv mass flux
+// (:1312-1324), out[12][F] in the column order of the device's record; per-cell beta and local Mach
+// (:1229-1246), cells[2][N].
+extern "C" void ao_face_scales_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, double* out,
+                                   double* cells) {
+  const MeshView<double> v = view<double>(m);
+  const AssemblyCfg<double> a = acfg<double>(c);
+  const double g = a.gamma;
+  auto cons = [&](int i) {
+    const double* u = U + 5 * static_cast<std::size_t>(i);
+    Cons<double> s;
+    s.rho = u[0];
+    s.rhou = u[1];
+    s.rhov = u[2];
+    s.rhow = u[3];
+    s.rhoE = u[4];
+    return s;
+  };
+  const std::size_t F = static_cast<std::size_t>(v.num_faces);
+  for (int f = 0; f < v.num_faces; ++f) {
+    const Cons<double> l = cons(v.owner[f]);
+    const int nb = v.neighbor[f];
+    const bool wall = nb < 0 && v.bc[f] == kSlipWall;
+    const Cons<double> r = nb >= 0 ? cons(nb) : (wall ? l : a.farfield);
+    const A3<double> n = normal_of(v, f);
+    const Prim<double> pl = cons_to_prim(l, g), pr = cons_to_prim(r, g);
+    const double unl = pl.u * n[0] + pl.v * n[1] + pl.w * n[2];
+    const double unr = pr.u * n[0] + pr.v * n[1] + pr.w * n[2];
+    const double al = speed_of_sound(pl, g), ar = speed_of_sound(pr, g);
+    // compute_acoustic_scaling_info, step by step (:399-415)
+    const double face_m = 0.5 * (local_mach(pl, g) + local_mach(pr, g));
+    const double beta = face_beta(l, r, g, a.precond);
+    double base = a.precond.enabled ? sclamp(beta, 1.0e-4, 1.0) : 1.0;
+    const double lms =
+        a.low_mach_fix ? low_mach_dissipation_scale(l, r, g, true, a.low_mach_cutoff, a.precond.mach_ref) : 1.0;
+    base = sclamp(smin(base, lms), 1.0e-4, 1.0);
+    const double stab = sclamp(smooth_max(face_m, sclamp(a.stab_mach_floor, 0.0, 1.0)), 1.0e-4, 1.0);
+    const double s = smax(base, stab);
+    double* o = out + f;
+    o[0 * F] = face_m;  // also the row's mach_local: each side's |v| / max(a, 1e-8)
+    o[1 * F] = 0.5 * (ao_abs(unl) + ao_abs(unr));
+    o[2 * F] = 0.5 * (al + ar);
+    o[3 * F] = beta;
+    o[4 * F] = lms;
+    o[5 * F] = base;
+    o[6 * F] = stab;
+    o[7 * F] = s;
+    o[8 * F] = all_speed_pressure_scale(l, r, n, g, a.as_fix, a.as_cutoff, a.as_fmin, a.as_ramp);
+    o[9 * F] = smax(ao_abs(unl) + al, ao_abs(unr) + ar);
+    o[10 * F] = smax(precond_spectral_radius(unl, al, s), precond_spectral_radius(unr, ar, s));
+    double lam = 0.0;
+    o[11 * F] = wall ? boundary_flux(static_cast<int>(kSlipWall), l, a.farfield, n, a.scheme, g, s, false, false, 0.0,
+                                     0.0, 0.0, &lam, nullptr)[0]
+                     : 0.0;
+  }
+  if (cells) {
+    for (int i = 0; i < v.num_cells; ++i) {
+      const Prim<double> p = cons_to_prim(cons(i), g);
+      cells[i] = cell_beta(p, g, a.precond);
+      cells[static_cast<std::size_t>(v.num_cells) + i] = local_mach(p, g);
+    }
   }
 }
 

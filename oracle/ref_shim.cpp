@@ -307,6 +307,9 @@ void ref_forces(const ref_mesh_cfg* c, const float* p, const float* ref7, int su
   std::memcpy(out8, o, sizeof(o));
 }
 
+constexpr int kRunPost = 23;
+static float g_run_post[kRunPost] = {};
+
 // Full reference run (run_euler_airfoil_case_cpu).  History rows: iter, l1, l2, linf, cl, cd, cm.
 // The precond gate reads AERO_ENABLE_EXPERIMENTAL_PRECOND from the process environment.
 int ref_run_case(const ref_mesh_cfg* mc, const ref_case_cfg* c, const char* out_dir, int hist_cap, float* hist,
@@ -410,7 +413,37 @@ int ref_run_case(const ref_mesh_cfg* mc, const ref_case_cfg* c, const char* out_
     cp_ordered_xyc[4 * i + 2] = samples[i].y;
     cp_ordered_xyc[4 * i + 3] = samples[i].cp;
   }
+  const float post[kRunPost] = {r.max_wall_mass_flux,
+                                r.beta_min,
+                                r.beta_max,
+                                r.beta_mean,
+                                r.mach_local_min,
+                                r.mach_local_max,
+                                r.mach_local_mean,
+                                r.acoustic_scale_min,
+                                r.acoustic_scale_max,
+                                r.acoustic_scale_mean,
+                                r.acoustic_scale_final_min,
+                                r.acoustic_scale_final_max,
+                                r.acoustic_scale_final_mean,
+                                r.wall_adjacent_acoustic_scale_min,
+                                r.wall_adjacent_acoustic_scale_p01,
+                                r.wall_adjacent_acoustic_scale_p50,
+                                r.wall_adjacent_stabilization_scale_min,
+                                r.wall_adjacent_stabilization_scale_p01,
+                                r.wall_adjacent_stabilization_scale_p50,
+                                r.wall_adjacent_acoustic_scale_final_min,
+                                r.wall_adjacent_acoustic_scale_final_p01,
+                                r.wall_adjacent_acoustic_scale_final_p50,
+                                r.stabilization_mach_floor_final};
+  std::memcpy(g_run_post, post, sizeof(post));
   return rows;
 }
+
+// The post-processing scalars of the last ref_run_case (EulerRunResult, euler_solver.hpp:101-141):
+// max_wall_mass_flux, beta min/max/mean, mach_local min/max/mean, acoustic_scale min/max/mean,
+// acoustic_scale_final min/max/mean, the wall-adjacent acoustic / stabilization / final scales'
+// min/p01/p50, stabilization_mach_floor_final.
+void ref_last_run_post(float* out) { std::memcpy(out, g_run_post, sizeof(g_run_post)); }
 
 }  // extern "C"

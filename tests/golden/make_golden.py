@@ -284,6 +284,25 @@ def residuals(ref) -> dict:
     return out
 
 
+DISSIPATION_REGIONS = ("wall_adjacent_interior", "farfield_boundary")
+
+
+def post_csvs(out_dir: Path) -> tuple[np.ndarray, np.ndarray]:
+    """The reference run's dissipation_debug.csv rows as [face, region (index in DISSIPATION_REGIONS), x, y,
+    area, mach_local, un_abs, a, beta, low_mach_scale, acoustic_scale, s_stab, acoustic_scale_final,
+    stabilization_m0, all_speed_pressure_scale, smax_physical, smax_dissipation] and its wall_flux.csv rows
+    as [face, x, y, mass_flux_n] (euler_solver.cpp:1304-1465), as printed (six significant digits)."""
+    rows = []
+    lines = (out_dir / "dissipation_debug.csv").read_text().splitlines()
+    for line in lines[1:]:
+        f = line.split(",")
+        rows.append([float(f[0]), float(DISSIPATION_REGIONS.index(f[1]))] + [float(x) for x in f[2:]])
+    dissipation = np.array(rows, np.float64).reshape(-1, 17)
+    lines = (out_dir / "wall_flux.csv").read_text().splitlines()
+    wall_flux = np.array([[float(x) for x in line.split(",")] for line in lines[1:]], np.float64).reshape(-1, 4)
+    return dissipation, wall_flux
+
+
 def runs(ref) -> dict:
     out = {}
     for name, run in cases.RUNS.items():
@@ -307,6 +326,9 @@ def runs(ref) -> dict:
         with tempfile.TemporaryDirectory() as td:
             rows = ref.ref_run_case(C.byref(mesh_cfg(run["mesh"])), C.byref(rc), td.encode(), cap, f32p(hist),
                                     f32p(U), f32p(R), f32p(lam), f32p(forces), f32p(cp), f32p(cpo), C.byref(ncpo))
+            post = np.zeros(23, np.float32)
+            ref.ref_last_run_post(f32p(post))
+            dissipation, wall_flux = post_csvs(Path(td))
         os.environ.pop("AERO_ENABLE_EXPERIMENTAL_PRECOND", None)
         out[f"run_{name}_hist"] = hist[:rows]
         out[f"run_{name}_U"] = U
@@ -315,6 +337,9 @@ def runs(ref) -> dict:
         out[f"run_{name}_forces"] = forces
         out[f"run_{name}_cp"] = cp
         out[f"run_{name}_cp_ordered"] = cpo[: ncpo.value]
+        out[f"run_{name}_post"] = post
+        out[f"run_{name}_dissipation"] = dissipation
+        out[f"run_{name}_wall_flux"] = wall_flux
         print(f"  run {name}: {rows} iterations, cl={forces[5]:.6f} cd={forces[6]:.6f}")
     return out
 

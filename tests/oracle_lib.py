@@ -180,6 +180,25 @@ def residual_jvp(om: OracleMesh, cfg, U, dU=None, seed=None, frozen=None):
     return R, dR
 
 
+def run_case_cfg(om: OracleMesh, case: CaseCfg) -> ResidualCfg:
+    """The residual configuration of the oracle driver's last stage for a case (ramped floors, all-speed
+    weights, effective preconditioner)."""
+    out = ResidualCfg()
+    if getattr(oracle(), f"ao_run_case_cfg_{suffix(om.dtype)}")(C.byref(om.c), C.byref(case), C.byref(out)) != 0:
+        raise RuntimeError("ao_run_case_cfg failed")
+    return out
+
+
+def face_scales(om: OracleMesh, cfg, U):
+    """Post-run scaling record of the FP64 oracle: per face [12][F] (the device record's columns) and per
+    cell (beta, local Mach) [2][N]."""
+    U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
+    fs = np.zeros((12, om.F))
+    cs = np.zeros((2, om.N))
+    oracle().ao_face_scales_f64(C.byref(om.c), C.byref(cfg), ptr(U), ptr(fs), ptr(cs))
+    return fs, cs
+
+
 def limiter_capture(om: OracleMesh, cfg, U):
     U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
     out = np.zeros((8, om.F))
