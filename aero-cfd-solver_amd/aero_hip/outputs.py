@@ -313,6 +313,8 @@ def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int =
         if out["nonfinite"]:
             raise RuntimeError("Euler pseudo-time iteration produced a non-finite state.")
         fs, cs = hm.face_scales(out["final_config"], out["conserved"])
+        fa = hm.forces(rho_inf_of(case), case.p_inf, case.aoa_deg, speed_inf_of(case), case.chord, case.x_ref,
+                       case.y_ref, gauge=False, gamma=case.gamma, U=out["conserved"])
     write_outputs(out_dir, mesh, case, out)
     post = write_post_files(out_dir, mesh, out["final_config"], fs)
     stats = scaling_statistics(mesh, fs, cs)
@@ -326,26 +328,79 @@ def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int =
     V = max(case.mach, 0.0) * math.sqrt(case.gamma * case.p_inf / max(rho_inf, 1.0e-12))
     a = math.radians(case.aoa_deg)
     f = out["forces"]
+    onoff = lambda v: "on" if v else "off"
+    c = case
+    wall = np.flatnonzero(mesh.bc == _lib.BC_SLIP_WALL)
+    nax = float(np.sum(-mesh.normal[wall, 0] * mesh.area[wall]))  # face_nA = -n A (forces.cpp:233-238)
+    nay = float(np.sum(-mesh.normal[wall, 1] * mesh.area[wall]))
+    mon = force_monitor(H[:, _lib.H_CL], H[:, _lib.H_CD], H[:, _lib.H_CM], c.force_stability_window,
+                        c.force_stability_tol, c.force_mean_drift_tol)
+    # residual_stub.cpp:609-762, in its order; the keys this backend has values for
     lines = ["AeroCFD Euler airfoil 2D", f"version={VERSION}", f"case_path={case_path}", "backend=hip",
              f"case_type={case_type}", f"num_cells={mesh.num_cells}", f"num_faces={mesh.num_faces}",
              f"iterations={len(H)}", f"residual_l1={_g(summary['residual_l1'])}",
              f"residual_l2={_g(summary['residual_l2'])}", f"residual_linf={_g(summary['residual_linf'])}",
-             f"cl={_g(f[5])}", f"cd={_g(f[6])}", f"cm={_g(f[7])}", f"aoa_deg={_g(case.aoa_deg)}",
+             f"cl={_g(f[5])}", f"cd={_g(f[6])}", f"cm={_g(f[7])}", f"aoa_deg={_g(c.aoa_deg)}",
              f"aoa_rad={_g(a)}", f"u_inf={_g(V * math.cos(a))}", f"v_inf={_g(V * math.sin(a))}",
-             f"wall_faces_integrated={mesh.wall_count}", f"q_inf={_g(0.5 * rho_inf * V * V + 1e-12)}",
-             f"V_inf={_g(V)}", f"rho_inf={_g(rho_inf)}", f"S_ref={_g(max(case.chord, 1e-8))}",
-             f"chord={_g(max(case.chord, 1e-8))}", f"Fx={_g(f[0])}", f"Fy={_g(f[1])}", f"L={_g(f[2])}",
-             f"D={_g(f[3])}", f"flux={'hllc' if case.flux_scheme else 'rusanov'}",
-             f"limiter={'venkat' if case.limiter else 'minmod'}",
-             f"precond={'on' if case.precond_on else 'off'}",
-             f"low_mach_fix={'on' if case.low_mach_fix else 'off'}",
-             f"all_speed_flux_fix={'on' if case.all_speed_flux_fix else 'off'}",
+             f"wall_faces_integrated={len(wall)}", "normal_is_unit=1", f"sum_nA_x={_g(nax)}", f"sum_nA_y={_g(nay)}",
+             f"sum_nA_mag={_g(math.hypot(nax, nay))}", f"Fx_abs={_g(fa['fx'])}", f"Fy_abs={_g(fa['fy'])}",
+             f"Fx_gauge={_g(f[0])}", f"Fy_gauge={_g(f[1])}", f"q_inf={_g(0.5 * rho_inf * V * V + 1e-12)}",
+             f"V_inf={_g(V)}", f"rho_inf={_g(rho_inf)}", f"S_ref={_g(max(c.chord, 1e-8))}",
+             f"chord={_g(max(c.chord, 1e-8))}", f"Fx={_g(f[0])}", f"Fy={_g(f[1])}", f"L={_g(f[2])}",
+             f"D={_g(f[3])}", f"cl_abs={_g(fa['cl'])}", f"cd_abs={_g(fa['cd'])}", f"cm_abs={_g(fa['cm'])}",
+             f"flux={'hllc' if c.flux_scheme else 'rusanov'}", f"limiter={'venkat' if c.limiter else 'minmod'}",
+             f"low_mach_fix={onoff(c.low_mach_fix)}", f"mach_cutoff={_g(c.mach_cutoff)}",
+             f"all_speed_flux_fix={onoff(c.all_speed_flux_fix)}", f"all_speed_mach_cutoff={_g(c.all_speed_mach_cutoff)}",
+             f"all_speed_f_min={_g(c.all_speed_f_min)}", f"all_speed_ramp_start_iter={c.all_speed_ramp_start_iter}",
+             f"all_speed_ramp_iters={c.all_speed_ramp_iters}",
+             f"all_speed_staged_controller={onoff(c.all_speed_staged_controller)}",
+             f"all_speed_stage_a_min_iters={c.all_speed_stage_a_min_iters}",
+             f"all_speed_stage_a_max_iters={c.all_speed_stage_a_max_iters}",
+             f"all_speed_stage_f_min_high={_g(c.all_speed_stage_f_min_high)}",
+             f"all_speed_stage_f_min_mid={_g(c.all_speed_stage_f_min_mid)}",
+             f"all_speed_stage_f_min_low={_g(c.all_speed_stage_f_min_low)}",
+             f"all_speed_pjump_wall_target={_g(c.all_speed_pjump_wall_target)}",
+             f"all_speed_pjump_spike_factor={_g(c.all_speed_pjump_spike_factor)}",
+             f"all_speed_pjump_hold_iters={c.all_speed_pjump_hold_iters}",
+             f"all_speed_freeze_iters={c.all_speed_freeze_iters}",
+             f"all_speed_cfl_drop_factor={_g(c.all_speed_cfl_drop_factor)}",
+             f"all_speed_cfl_restore_factor={_g(c.all_speed_cfl_restore_factor)}",
+             f"all_speed_cfl_min_scale={_g(c.all_speed_cfl_min_scale)}",
+             f"aoa0_symmetry_enforce={onoff(c.aoa0_symmetry_enforce)}",
+             f"aoa0_symmetry_enforce_interval={c.aoa0_symmetry_enforce_interval}",
+             f"precond={onoff(c.precond_on)}", f"precond_mach_ref={_g(c.precond_mach_ref)}",
+             f"precond_mach_min={_g(c.precond_mach_min)}", f"precond_beta_min={_g(c.precond_beta_min)}",
+             f"precond_beta_max={_g(c.precond_beta_max)}", f"precond_farfield_bc={onoff(c.precond_farfield_bc)}",
+             f"stabilization_mach_floor_start={_g(c.stab_floor_start)}",
+             f"stabilization_mach_floor_target={_g(c.stab_floor_target)}",
+             f"stabilization_mach_floor_k_start={_g(c.stab_floor_k_start)}",
+             f"stabilization_mach_floor_k_target={_g(c.stab_floor_k_target)}",
+             f"stabilization_ramp_iters={c.stab_ramp_iters}",
              f"stabilization_mach_floor_final={_g(out['final_config'].stab_mach_floor)}",
              *(f"{k}={_g(v)}" for k, v in stats.items() if k != "max_wall_mass_flux"),
              f"dissipation_debug_csv={post['dissipation']}",
+             f"startup_first_order_iters={c.startup_first_order_iters}", f"rk_stages={c.rk_stages}",
+             f"local_time_stepping={onoff(c.local_time_stepping)}",
+             f"force_stability_window={c.force_stability_window}",
+             f"force_mean_drift_tol={_g(c.force_mean_drift_tol)}",
+             f"force_window_stable_final={int(mon['stable'][-1])}",
+             f"force_mean_drift_stable_final={int(mon['mean_stable'][-1])}",
+             f"force_physical_converged_final={int(mon['converged'][-1])}",
+             f"force_mean_dcl_final={_g(mon['mean_d'][-1][0])}", f"force_mean_dcd_final={_g(mon['mean_d'][-1][1])}",
+             f"force_mean_dcm_final={_g(mon['mean_d'][-1][2])}",
              f"device_seconds={out['seconds']:.6g}"]
     (out_dir / "run.log").write_text("\n".join(lines) + "\n", encoding="utf-8")
     return summary
+
+
+def rho_inf_of(case: CaseConfig) -> float:
+    """Freestream density: rho_inf, or p_inf / (R T_inf) when it is not set (euler_solver.cpp:543-549)."""
+    return case.rho_inf if case.rho_inf > 0.0 else case.p_inf / (case.gas_constant * case.t_inf)
+
+
+def speed_inf_of(case: CaseConfig) -> float:
+    """Freestream speed M a_inf (euler_solver.cpp:551-553)."""
+    return max(case.mach, 0.0) * math.sqrt(case.gamma * case.p_inf / max(rho_inf_of(case), 1.0e-12))
 
 
 def wall_cp_from_state(mesh: Mesh, case: CaseConfig, U: np.ndarray) -> np.ndarray:

@@ -95,3 +95,12 @@ def test_runner_post_files(hip, tmp_path):
     st = outputs.scaling_statistics(m, ofs, ocs)
     for key in ("beta_min", "mach_local_max", "acoustic_scale_final_mean", "wall_adjacent_acoustic_scale_p01"):
         assert float(log[key]) == pytest.approx(st[key], rel=1e-5), key
+    fa = None
+    with hip.HipMesh(m) as hm:
+        fa = hm.forces(outputs.rho_inf_of(case), case.p_inf, case.aoa_deg, outputs.speed_inf_of(case), case.chord,
+                       case.x_ref, case.y_ref, gauge=False, gamma=case.gamma, U=U)
+    assert float(log["cl_abs"]) == pytest.approx(fa["cl"], rel=1e-5)
+    assert float(log["Fx_abs"]) == pytest.approx(fa["fx"], rel=1e-5)
+    wall_nA = -(m.normal[wall, :2] * m.area[wall, None]).sum(axis=0)
+    assert float(log["sum_nA_mag"]) == pytest.approx(float(np.hypot(*wall_nA)), rel=1e-5, abs=1e-12)
+    assert log["force_physical_converged_final"] in ("0", "1") and log["rk_stages"] == "3"
