@@ -552,12 +552,15 @@ int aero_hip_residual_async(aero_hip_mesh* m, const aero_residual_config* cfg, c
   }
   FaceOpts o;
   if (const int rc = mesh_face_opts(m, cfg, &o, err, errlen)) return rc;
-  hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
-  AERO_HIP_TRY(hipGetLastError(), "aero_hip_residual_async: k_init_diag");
+  if (!cfg->second_order) {  // at second order the gradient kernel resets the counters
+    hipLaunchKernelGGL(k_init_diag, dim3(1), dim3(64), 0, m->stream, m->diag);
+    AERO_HIP_TRY(hipGetLastError(), "aero_hip_residual_async: k_init_diag");
+  }
   const double* U = m->U[m->cur];
   const bool timed = m->tcount < m->tcap;
   if (timed) AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 0], m->stream), "hipEventRecord");
-  if (cfg->second_order) AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream), "k_gradient");
+  if (cfg->second_order)
+    AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream, m->diag), "k_gradient");
   if (timed) AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 1], m->stream), "hipEventRecord");
   StageArgs st{};
   AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 0, U, m->G,

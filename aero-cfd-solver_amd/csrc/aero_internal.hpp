@@ -198,7 +198,9 @@ struct AdjBatchCtl {
 }  // namespace aero_dev
 
 namespace aero_host {
-hipError_t launch_gradient(const aero_dev::DevMesh& m, const double* U, double* G, double g, hipStream_t s);
+// diag (optional): residual counters [4] the next flux launch accumulates, reset by the gradient kernel.
+hipError_t launch_gradient(const aero_dev::DevMesh& m, const double* U, double* G, double g, hipStream_t s,
+                           int* diag = nullptr);
 hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, int scheme, int limiter, bool second,
                        int mode, const double* U, const double* G, double* R, double* lam, int* diag,
                        const aero_dev::StageArgs& st, hipStream_t s);

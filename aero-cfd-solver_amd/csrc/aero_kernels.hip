@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "aero_internal.hpp"
 #include "face_eval.hpp"
@@ -206,8 +207,16 @@ constexpr int grad_lds_per_slot(int gk) { return gk == 1 ? 8 : 4; }
 
 template <int BC, int GK>
 __global__ __launch_bounds__(BC) void k_gradient(DevMesh m, const double* __restrict__ U, double* __restrict__ G,
-                                                  double g) {
+                                                  double g, int* __restrict__ diag) {
   extern __shared__ double lds[];
+  // Optional reset of the residual counters that the flux kernel launched after this one accumulates
+  // (first-order fallback, diffusive fallback, first failing face, non-finite flag).
+  if (diag && blockIdx.x == 0 && threadIdx.x == 0) {
+    diag[0] = 0;
+    diag[1] = 0;
+    diag[2] = INT_MAX;
+    diag[3] = 0;
+  }
   const int b = block_of(m);
   if (b >= m.nblk) return;
   gradient_body<BC, GK>(m, b, U, G, g, lds);
@@ -890,20 +899,21 @@ namespace aero_host {
 using namespace aero_dev;
 
 template <int BC>
-static hipError_t launch_gradient_bc(const DevMesh& m, const double* U, double* G, double g, hipStream_t s) {
+static hipError_t launch_gradient_bc(const DevMesh& m, const double* U, double* G, double g, int* diag,
+                                     hipStream_t s) {
   const size_t lds = static_cast<size_t>(m.smax) * grad_lds_per_slot(m.grad) * sizeof(double);
   if (m.grad == 1)
-    hipLaunchKernelGGL((k_gradient<BC, 1>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
+    hipLaunchKernelGGL((k_gradient<BC, 1>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g, diag);
   else
-    hipLaunchKernelGGL((k_gradient<BC, 0>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g);
+    hipLaunchKernelGGL((k_gradient<BC, 0>), dim3(m.grid), dim3(BC), lds, s, m, U, G, g, diag);
   return hipGetLastError();
 }
 
-hipError_t launch_gradient(const DevMesh& m, const double* U, double* G, double g, hipStream_t s) {
+hipError_t launch_gradient(const DevMesh& m, const double* U, double* G, double g, hipStream_t s, int* diag) {
   switch (m.BC) {
-    case 128: return launch_gradient_bc<128>(m, U, G, g, s);
-    case 256: return launch_gradient_bc<256>(m, U, G, g, s);
-    case 512: return launch_gradient_bc<512>(m, U, G, g, s);
+    case 128: return launch_gradient_bc<128>(m, U, G, g, diag, s);
+    case 256: return launch_gradient_bc<256>(m, U, G, g, diag, s);
+    case 512: return launch_gradient_bc<512>(m, U, G, g, diag, s);
     default: return hipErrorInvalidValue;
   }
 }
