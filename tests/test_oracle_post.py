@@ -95,3 +95,27 @@ def test_statistics_reduce_to_closed_forms(oracle_lib):
     assert np.allclose(cs[1], mach, rtol=1e-14)
     stab = 0.5 * (mach + np.sqrt(mach * mach + 1e-12))
     assert np.allclose(fs[hl.FS_STAB], stab, rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["regression", "case_file_numerics", "precond_staged"])
+def test_writer_reproduces_reference_files(golden, oracle_lib, name, tmp_path):
+    """The runner's writer (aero_hip.outputs.write_post_files), fed the oracle's record, writes the
+    reference's wall_flux.csv and dissipation_debug.csv: the same faces, regions and geometry columns and
+    the same values to the printed digits."""
+    g, m, cfg, fs, _ = _setup(golden, name)
+    paths = outputs.write_post_files(tmp_path, m, cfg, fs)
+    lines = paths["dissipation"].read_text().splitlines()
+    assert lines[0] == ("face,region,x,y,area,mach_local,un_abs,a,beta,low_mach_scale,acoustic_scale,s_stab,"
+                        "acoustic_scale_final,stabilization_m0,all_speed_pressure_scale,smax_physical,"
+                        "smax_dissipation")
+    ref = g[f"run_{name}_dissipation"]
+    got = np.array([[float(f[0]), float(outputs.DISSIPATION_REGIONS.index(f[1]))] + [float(x) for x in f[2:]]
+                    for f in (line.split(",") for line in lines[1:])])
+    assert got.shape == ref.shape
+    assert np.array_equal(got[:, :2], ref[:, :2])
+    for k in range(2, ref.shape[1]):
+        _close(got[:, k], ref[:, k], f"{name} dissipation column {k}")
+    wf = np.loadtxt(paths["wall_flux"], delimiter=",", skiprows=1, ndmin=2)
+    ref_wf = g[f"run_{name}_wall_flux"]
+    assert np.array_equal(wf[:, [0, 3]], ref_wf[:, [0, 3]])
+    _close(wf[:, 1:3], ref_wf[:, 1:3], f"{name} wall_flux x, y")
