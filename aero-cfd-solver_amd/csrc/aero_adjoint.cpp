@@ -792,10 +792,11 @@ int plain_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int forward
     AERO_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     for (int u = 0; u < unit && !rc; ++u) rc = one_iteration(m, cfg, forward, true, gt, od, ctl, &s0, true, err, errlen);
     const hipError_t ec = hipStreamEndCapture(m->stream, &graph);
+    hipError_t ei = hipSuccess;
+    if (!rc && ec == hipSuccess) ei = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);  // on every path, the failed ones included
     if (rc) return rc;
     AERO_TRY(ec, "hipStreamEndCapture");
-    const hipError_t ei = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
     AERO_TRY(ei, "hipGraphInstantiate");
   }
   struct GraphGuard {
@@ -1259,10 +1260,11 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
       if (e == hipSuccess) e = aero_host::launch_adj_batch_control(d_tab, d_ctl, n, ctl->tol, ctl->max_iters, r.st);
     }
     const hipError_t ec = hipStreamEndCapture(r.st, &graph);
+    hipError_t ei = hipSuccess;
+    if (e == hipSuccess && ec == hipSuccess) ei = hipGraphInstantiate(&r.gx, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);  // on every path, the failed ones included
     AERO_TRY(e, "adjoint batch capture");
     AERO_TRY(ec, "hipStreamEndCapture");
-    const hipError_t ei = hipGraphInstantiate(&r.gx, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
     AERO_TRY(ei, "hipGraphInstantiate");
   }
   const int every = std::max(iters_per_sync, unit);

@@ -38,10 +38,9 @@ struct aero_hip_batch {
   double* d_hist = nullptr;
   int hist_cap = 0;
   aero_case_config cfg{};
-  hipGraphExec_t graph[2][2][2] = {};  // [second order][start buffer][frozen limiter]
-  bool frozen = false;                  // limiter ratios captured (limiter_freeze_iter reached)
+  hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
+  bool frozen = false;               // limiter ratios captured (limiter_freeze_iter reached)
   std::vector<double> alpha, q_inf, chord;
-  int state_buf = 0;  // ring buffer that holds every case's current state
 };
 
 namespace {
@@ -76,9 +75,8 @@ void free_batch(aero_hip_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto& row : b->graph)
-    for (auto& col : row)
-      for (auto& g : col)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   for (void* p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -87,12 +85,16 @@ void free_batch(aero_hip_batch* b) {
 
 double clampd(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
+// The ring buffer that holds a case's current state between iterations (aero_internal.hpp kRingParity).
+int state_sel(const aero_case_config& c) { return c.rk_stages == 3 ? 0 : aero_dev::kRingParity; }
+
 // One pseudo-time iteration of every active case, enqueued on the batch stream.
-int enqueue_iteration(aero_hip_batch* b, bool second, int start, bool frozen, char* err, size_t errlen) {
+int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, char* err, size_t errlen) {
   const aero_case_config& c = b->cfg;
   AERO_TRY(aero_host::launch_batch_init_diag(b->d_cases, b->ncases, b->stream), "k_batch_init_diag");
-  // Ring: RK3 ends in its start buffer (stage 2 updates in place: each thread reads U^n of its own cell
-  // before writing it); RK1 ping-pongs between two buffers.
+  // Ring: RK3 starts and ends in buffer 0 (stage 2 updates in place: each thread reads U^n of its own cell
+  // before writing it); RK1 ping-pongs between buffers 0 and 1 by each case's own iteration count.
+  const int start = state_sel(c);
   int cur = start;
   int final_buf = start;
   for (int st = 0; st < c.rk_stages; ++st) {
@@ -102,9 +104,9 @@ int enqueue_iteration(aero_hip_batch* b, bool second, int start, bool frozen, ch
     bs.cur = cur;
     bs.base = start;
     if (c.rk_stages == 3)
-      bs.out = st == 0 ? (start + 1) % 3 : (st == 1 ? (start + 2) % 3 : start);
+      bs.out = st == 0 ? 1 : (st == 1 ? 2 : 0);
     else
-      bs.out = 1 - start;
+      bs.out = aero_dev::kRingParityNext;
     if (second)
       AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
                                                 b->smax, c.gamma, cur, b->stream),
@@ -142,12 +144,28 @@ int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* e
   b->nwork = static_cast<int>(work.size());
   b->listed = active;
   for (auto& row : b->graph)
-    for (auto& col : row)
-      for (auto& gx : col)
-        if (gx) {
-          (void)hipGraphExecDestroy(gx);
-          gx = nullptr;
-        }
+    for (auto& gx : row)
+      if (gx) {
+        (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+      }
+  return AERO_OK;
+}
+
+// Captures one iteration (enqueue) on the batch stream into an executable graph.  The captured graph is
+// destroyed on every path, including a failed enqueue, end of capture or instantiation.
+template <class Enqueue>
+int capture_iteration(aero_hip_batch* b, Enqueue&& enqueue, hipGraphExec_t* gx, char* err, size_t errlen) {
+  AERO_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  const int rc = enqueue();
+  hipGraph_t captured = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->stream, &captured);
+  hipError_t ei = hipSuccess;
+  if (!rc && ec == hipSuccess) ei = hipGraphInstantiate(gx, captured, nullptr, nullptr, 0);
+  if (captured) (void)hipGraphDestroy(captured);
+  if (rc) return rc;
+  AERO_TRY(ec, "hipStreamEndCapture");
+  AERO_TRY(ei, "hipGraphInstantiate");
   return AERO_OK;
 }
 
@@ -389,20 +407,19 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   const int startup = std::max(c.startup_first_order_iters, 0);
   const int sync_every = std::max(iters_per_sync, 1);
   const bool compact = true;  // relaunch over the active cases only, whenever the active set changes
-  auto enqueue = [&](bool second, int start) -> int {
-    int rc = enqueue_iteration(b, second, start, b->frozen, err, errlen);
+  auto enqueue = [&](bool second) -> int {
+    int rc = enqueue_iteration(b, second, b->frozen, err, errlen);
     if (rc) return rc;
     AERO_TRY(aero_host::launch_batch_control(b->d_cases, b->d_ctl, p, b->stream), "k_batch_control");
     return 0;
   };
-  // Graphs are specific to (order, start buffer) and to the run parameters; rebuild them per run.
+  // Graphs are specific to (order, frozen limiter) and to the run parameters; rebuild them per run.
   for (auto& row : b->graph)
-    for (auto& col : row)
-      for (auto& gx : col)
-        if (gx) {
-          (void)hipGraphExecDestroy(gx);
-          gx = nullptr;
-        }
+    for (auto& gx : row)
+      if (gx) {
+        (void)hipGraphExecDestroy(gx);
+        gx = nullptr;
+      }
   // Frozen limiter (beyond the reference): at the start of iteration limiter_freeze_iter (second order)
   // every active case captures its ratios; the iterations run in chunks that end there.
   const int freeze_at =
@@ -429,7 +446,8 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   p.max_iters = done_iters + max_iters;
   std::vector<CaseCtl> h(b->ncases);
   const auto t0 = std::chrono::steady_clock::now();
-  int it = 0, start = b->state_buf;
+  const int start = state_sel(c);
+  int it = 0;
   while (it < max_iters) {
     const int g_it = done_iters + it;  // the cases' iteration index
     const bool second = g_it >= startup;
@@ -451,25 +469,16 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
     const int n = std::min(sync_every, phase_end - it);
     for (int k = 0; k < n; ++k) {
       if (use_graph) {
-        hipGraphExec_t& gx = b->graph[second ? 1 : 0][start][b->frozen ? 1 : 0];
+        hipGraphExec_t& gx = b->graph[second ? 1 : 0][b->frozen ? 1 : 0];
         if (!gx) {
-          hipGraph_t graph = nullptr;
-          AERO_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-          const int rc = enqueue(second, start);
-          hipGraph_t captured = nullptr;
-          const hipError_t ec = hipStreamEndCapture(b->stream, &captured);
+          const int rc = capture_iteration(b, [&] { return enqueue(second); }, &gx, err, errlen);
           if (rc) return rc;
-          AERO_TRY(ec, "hipStreamEndCapture");
-          graph = captured;
-          AERO_TRY(hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-          (void)hipGraphDestroy(graph);
         }
         AERO_TRY(hipGraphLaunch(gx, b->stream), "hipGraphLaunch");
       } else {
-        const int rc = enqueue(second, start);
+        const int rc = enqueue(second);
         if (rc) return rc;
       }
-      if (c.rk_stages == 1) start = 1 - start;
     }
     it += n;
     AERO_TRY(hipMemcpyAsync(h.data(), b->d_ctl, h.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost, b->stream),
@@ -506,7 +515,6 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
     r.cl = lift / (q * chord);
     r.cm = mo / (q * chord * chord);
   }
-  b->state_buf = start;  // where every case's state is after this run
   return AERO_OK;
 }
 
@@ -519,7 +527,12 @@ int aero_hip_batch_get_state(aero_hip_batch* b, int32_t case_index, double* cons
   aero_hip_mesh* m = b->geoms[cs.geom];
   const size_t N = cs.N;
   std::vector<double> u(4 * N);
-  const int buf = b->state_buf;
+  int buf = 0;  // RK3: buffer 0; RK1: the parity of this case's own iteration count
+  if (b->cfg.rk_stages == 1) {
+    CaseCtl k{};
+    AERO_TRY(hipMemcpy(&k, b->d_ctl + case_index, sizeof(CaseCtl), hipMemcpyDeviceToHost), "control readback");
+    buf = k.iter & 1;
+  }
   AERO_TRY(hipMemcpy(u.data(), cs.U[buf], u.size() * sizeof(double), hipMemcpyDeviceToHost), "state copy");
   for (size_t ic = 0; ic < N; ++ic) {
     double* o = conserved + 5 * static_cast<size_t>(m->perm[ic]);

@@ -113,6 +113,14 @@ struct BatchParams {
   int hist_cap;
 };
 
+// Buffer selectors of the batched kernels (BatchStage cur / base / out, and the `cur` arguments).
+// sel >= 0 names a ring buffer directly: RK3 ends every iteration in its start buffer, so every case's
+// state stays in one buffer.  RK1 ping-pongs between buffers 0 and 1, and a case that has stopped no longer
+// advances, so its buffer follows its own iteration count: kRingParity is buffer iter & 1 (the state after
+// `iter` iterations), kRingParityNext the other one (the output of the running iteration).
+constexpr int kRingParity = -1;
+constexpr int kRingParityNext = -2;
+
 struct BatchStage {
   int stage, last, cur, base, out;
 };

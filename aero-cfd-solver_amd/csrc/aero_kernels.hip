@@ -751,6 +751,11 @@ __global__ __launch_bounds__(256) void k_iter_reduce(int nblk, const double* __r
 }
 
 // ---- batched kernels: one workgroup per (case, block) work item --------------------------------
+// A case's state-ring buffer for a selector sel (kRingParity / kRingParityNext: aero_internal.hpp).
+__device__ __forceinline__ int ring_buf(int sel, int iter) {
+  return sel >= 0 ? sel : (sel == kRingParity ? (iter & 1) : 1 - (iter & 1));
+}
+
 template <int BC, int GK>
 __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict__ geoms,
                                                         const BatchCase* __restrict__ cases,
@@ -760,7 +765,7 @@ __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict
   const int2 w = work[blockIdx.x];
   if (w.x < 0 || !ctl[w.x].active) return;
   const BatchCase& cs = cases[w.x];
-  gradient_body<BC, GK>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, lds);
+  gradient_body<BC, GK>(geoms[cs.geom], w.y, cs.U[ring_buf(cur, ctl[w.x].iter)], cs.G, g, lds);
 }
 
 template <int BC, int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
@@ -784,11 +789,11 @@ __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux_batch(const
   st.last_stage = bs.last;
   st.write_res = 0;
   st.cfl = k.cfl;
-  st.Un = cs.U[bs.base];
-  st.Unext = cs.U[bs.out];
+  st.Un = cs.U[ring_buf(bs.base, k.iter)];
+  st.Unext = cs.U[ring_buf(bs.out, k.iter)];
   st.partials = cs.partials;
-  flux_body<BC, SCHEME, LIMITER, SECOND, 1, FROZEN>(geoms[cs.geom], w.y, o, cs.U[bs.cur], cs.G, nullptr, nullptr,
-                                                    cs.diag, st, lds);
+  flux_body<BC, SCHEME, LIMITER, SECOND, 1, FROZEN>(geoms[cs.geom], w.y, o, cs.U[ring_buf(bs.cur, k.iter)], cs.G,
+                                                    nullptr, nullptr, cs.diag, st, lds);
 }
 
 template <int BC, int LIMITER>
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(BC) void k_limiter_capture_batch(const DevMesh* __r
   const int2 w = work[blockIdx.x];
   if (w.x < 0 || !ctl[w.x].active) return;
   const BatchCase& cs = cases[w.x];
-  limiter_capture_body<BC, LIMITER>(geoms[cs.geom], w.y, cs.U[cur], cs.G, g, cs.lim_ratio);
+  limiter_capture_body<BC, LIMITER>(geoms[cs.geom], w.y, cs.U[ring_buf(cur, ctl[w.x].iter)], cs.G, g, cs.lim_ratio);
 }
 
 // Per case: norm partials + wall force sums of the updated state into ctl[case].sums (same fixed-order
@@ -810,7 +815,8 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchCase* __restric
   const int c = blockIdx.x;
   if (!ctl[c].active) return;
   const BatchCase& cs = cases[c];
-  iter_reduce_body(cs.nblk, cs.partials, cs.wall, cs.U[cur], cs.N, g, ctl[c].sums, nullptr, red);
+  iter_reduce_body(cs.nblk, cs.partials, cs.wall, cs.U[ring_buf(cur, ctl[c].iter)], cs.N, g, ctl[c].sums, nullptr,
+                   red);
 }
 
 __global__ void k_batch_init_diag(const BatchCase* __restrict__ cases, int ncases) {

@@ -784,6 +784,114 @@ void assemble_residual(const MeshView<R>& m, const AssemblyCfg<R>& c, const std:
   }
 }
 
+// ---- the same evaluation on several host threads (OpenMP; serial without -fopenmp) ----------------
+// Per-face values in parallel, then per cell the sum over its faces in increasing face index: the serial
+// loops' accumulation order (reconstruction.cpp:85-107, residual_assembly.cpp:97-181), so the results are
+// the serial ones bit for bit.
+struct CellFaces {
+  std::vector<int> off, ent;  // per cell, its faces in increasing index: face << 1 | (cell is the neighbour)
+};
+
+template <class R> CellFaces cell_faces(const MeshView<R>& m) {
+  CellFaces cf;
+  cf.off.assign(static_cast<std::size_t>(m.num_cells) + 1, 0);
+  for (int f = 0; f < m.num_faces; ++f) {
+    ++cf.off[m.owner[f] + 1];
+    if (m.neighbor[f] >= 0) ++cf.off[m.neighbor[f] + 1];
+  }
+  for (int c = 0; c < m.num_cells; ++c) cf.off[c + 1] += cf.off[c];
+  cf.ent.resize(cf.off[m.num_cells]);
+  std::vector<int> fill(cf.off.begin(), cf.off.end() - 1);
+  for (int f = 0; f < m.num_faces; ++f) {
+    cf.ent[fill[m.owner[f]]++] = f << 1;
+    if (m.neighbor[f] >= 0) cf.ent[fill[m.neighbor[f]]++] = (f << 1) | 1;
+  }
+  return cf;
+}
+
+// Gradients of the second-order reconstruction: Green-Gauss face values in parallel, then the ordered
+// per-cell sums; least squares runs the serial routine.  qf is scratch.
+template <class R>
+void cell_gradients_threaded(const MeshView<R>& v, const CellFaces& cf, const std::vector<Prim<R>>& p,
+                             std::vector<R>* qf, std::vector<R>* gr) {
+  if (v.gradient == 1) {
+    cell_gradients(v, p, gr);
+    return;
+  }
+  const int N = v.num_cells, F = v.num_faces;
+  qf->resize(static_cast<std::size_t>(F) * 5);
+  gr->assign(static_cast<std::size_t>(N) * 10, R(0));
+#pragma omp parallel for schedule(static)
+  for (int f = 0; f < F; ++f)
+    for (int k = 0; k < 5; ++k) {
+      const R qo = prim_var(p[v.owner[f]], k);
+      R q = qo;
+      if (v.neighbor[f] >= 0) q = AO_K(0.5) * (qo + prim_var(p[v.neighbor[f]], k));
+      (*qf)[static_cast<std::size_t>(5 * f + k)] = q;
+    }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) {
+    const R iv = AO_K(1.0) / smax(v.volume[c], AO_K(1.0e-12));
+    for (int k = 0; k < 5; ++k)
+      for (int dd = 0; dd < 2; ++dd) {
+        R acc = R(0);
+        for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
+          const int f = cf.ent[e] >> 1;
+          const R nA = v.normal[3 * f + dd] * v.area[f];
+          const R t = (*qf)[static_cast<std::size_t>(5 * f + k)] * nA;
+          if (cf.ent[e] & 1) acc -= t; else acc += t;
+        }
+        (*gr)[static_cast<std::size_t>((c * 5 + k) * 2 + dd)] = acc * iv;
+      }
+  }
+}
+
+// assemble_residual with the face loop in parallel and the per-cell sums in face order.  fa is scratch.
+template <class R>
+void assemble_residual_threaded(const MeshView<R>& v, const AssemblyCfg<R>& a, const CellFaces& cf,
+                                const std::vector<Prim<R>>& p, const std::vector<R>& gr, std::vector<R>* fa,
+                                std::vector<R>* r, std::vector<R>* l, AssemblyDiag* d) {
+  const int N = v.num_cells, F = v.num_faces;
+  fa->resize(static_cast<std::size_t>(F) * 6);
+  r->resize(static_cast<std::size_t>(N) * 5);
+  l->resize(N);
+  int n_fo = 0, n_diff = 0, first = F;
+#pragma omp parallel for schedule(static) reduction(+ : n_fo, n_diff) reduction(min : first)
+  for (int f = 0; f < F; ++f) {
+    A5<R> Fl;
+    R w;
+    bool fo, fbk;
+    face_contribution(v, a, p, gr, f, &Fl, &w, &fo, &fbk);
+    const R A = v.area[f];
+    for (int k = 0; k < 5; ++k) (*fa)[static_cast<std::size_t>(6 * f + k)] = Fl[k] * A;
+    (*fa)[static_cast<std::size_t>(6 * f + 5)] = w * A;
+    if (fo) {
+      ++n_fo;
+      first = std::min(first, f);
+    }
+    if (fbk) ++n_diff;
+  }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) {
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
+    R w = R(0);
+    for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
+      const std::size_t f = static_cast<std::size_t>(cf.ent[e] >> 1);
+      for (int k = 0; k < 5; ++k) {
+        if (cf.ent[e] & 1) acc[k] -= (*fa)[6 * f + k]; else acc[k] += (*fa)[6 * f + k];
+      }
+      w += (*fa)[6 * f + 5];
+    }
+    for (int k = 0; k < 5; ++k) (*r)[static_cast<std::size_t>(5 * c + k)] = acc[k];
+    (*l)[c] = w;
+  }
+  if (d) {
+    d->first_order_fallback = n_fo;
+    d->diffusive_fallback = n_diff;
+    d->first_failure_face = first < F ? first : -1;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Time integration (src/cfd_core/solvers/euler/time_integrator.cpp:31-104)
 // ---------------------------------------------------------------------------------------------

@@ -108,8 +108,9 @@ template <class R> PrecondCfg<R> build_precond(const CaseCfg<R>& c, bool enabled
 }
 
 // The driver.  `m` supplies the mesh; its normals are re-normalised into a private copy first
-// (euler_solver.cpp:475-478).
-template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
+// (euler_solver.cpp:475-478).  threaded: the stage gradients and residuals run on all OpenMP threads
+// (cell_gradients_threaded / assemble_residual_threaded: the serial results bit for bit).
+template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool threaded = false) {
   if (c.iterations <= 0) throw std::invalid_argument("Euler iterations must be positive.");
   if (c.rk_stages != 1 && c.rk_stages != 3) throw std::invalid_argument("Euler pseudo-time rk_stages must be 1 or 3.");
   std::vector<R> normals(m.normal, m.normal + 3 * static_cast<std::size_t>(m.num_faces));
@@ -205,7 +206,9 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
   int b_progress = 0, below = 0, freeze = 0;
 
   std::vector<Prim<R>> prim(static_cast<std::size_t>(N));
-  std::vector<R> grad, res, resp, lam, dt, un, us, nxt;
+  std::vector<R> grad, res, resp, lam, dt, un, us, nxt, scratch_q, scratch_f;
+  CellFaces cf;
+  if (threaded) cf = cell_faces(m);
   std::vector<R> cell_p(static_cast<std::size_t>(N), c.p_inf);
   std::vector<double> frozen;  // frozen limiter ratios once captured
 
@@ -243,7 +246,9 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
         Cons<R> s{us[5 * i + 0], us[5 * i + 1], us[5 * i + 2], us[5 * i + 3], us[5 * i + 4]};
         prim[i] = cons_to_prim(s, g);
       }
-      if (second)
+      if (second && threaded)
+        cell_gradients_threaded(m, cf, prim, &scratch_q, &grad);
+      else if (second)
         cell_gradients(m, prim, &grad);
       else
         grad.assign(static_cast<std::size_t>(N) * 10, R(0));
@@ -264,7 +269,10 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c) {
       ac.as_fmin = as_fmin_eff;
       ac.as_ramp = as_w;
       AssemblyDiag d;
-      assemble_residual(m, ac, prim, grad, &res, &lam, &d);
+      if (threaded)
+        assemble_residual_threaded(m, ac, cf, prim, grad, &scratch_f, &res, &lam, &d);
+      else
+        assemble_residual(m, ac, prim, grad, &res, &lam, &d);
       out.cfg = ac;
       out.cfg.frozen = nullptr;
       iter_diag.first_order_fallback += d.first_order_fallback;

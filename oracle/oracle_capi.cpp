@@ -321,11 +321,12 @@ void forces_t(const ao_mesh* m, const R* p, const R* ref7, int subtract, R* out8
 }
 
 // Full run; returns the number of history rows written (<= hist_cap rows), -1 on error.
+// threaded: the stage residuals on all OpenMP threads (the serial run's results bit for bit).
 template <class R>
 int run_case_t(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, R* hist, R* U, R* res, R* lam, R* forces8,
-               R* cp, int* nonfinite, char* err, int errlen) {
+               R* cp, int* nonfinite, char* err, int errlen, bool threaded = false) {
   try {
-    const RunOut<R> o = run_case(view<R>(m), ccfg<R>(c));
+    const RunOut<R> o = run_case(view<R>(m), ccfg<R>(c), threaded);
     const int rows = std::min(o.n_iter, hist_cap);
     std::memcpy(hist, o.hist.data(), static_cast<std::size_t>(rows) * kHistFields * sizeof(R));
     if (U) std::memcpy(U, o.U.data(), o.U.size() * sizeof(R));
@@ -403,102 +404,21 @@ template <class R> double time_residual_t(const ao_mesh* m, const ao_residual_cf
 }
 
 // ---- the same evaluation on all host cores (the CPU baseline's threaded line) ------------------------
-// Per-face values in parallel, then per cell the sum over its faces in increasing face index: the serial
-// loops' accumulation order, so the results are the serial ones bit for bit.  Green-Gauss only.
-struct CellFaces {
-  std::vector<int> off, ent;  // per cell, its faces in increasing index: face << 1 | (cell is the neighbour)
-};
-
-template <class R> CellFaces cell_faces(const MeshView<R>& m) {
-  CellFaces cf;
-  cf.off.assign(static_cast<std::size_t>(m.num_cells) + 1, 0);
-  for (int f = 0; f < m.num_faces; ++f) {
-    ++cf.off[m.owner[f] + 1];
-    if (m.neighbor[f] >= 0) ++cf.off[m.neighbor[f] + 1];
-  }
-  for (int c = 0; c < m.num_cells; ++c) cf.off[c + 1] += cf.off[c];
-  cf.ent.resize(cf.off[m.num_cells]);
-  std::vector<int> fill(cf.off.begin(), cf.off.end() - 1);
-  for (int f = 0; f < m.num_faces; ++f) {
-    cf.ent[fill[m.owner[f]]++] = f << 1;
-    if (m.neighbor[f] >= 0) cf.ent[fill[m.neighbor[f]]++] = (f << 1) | 1;
-  }
-  return cf;
-}
-
+// Per-face values in parallel, then per cell the sum over its faces in increasing face index (CellFaces,
+// aero_oracle.hpp): the serial result bit for bit.
 template <class R>
 void residual_threaded(const MeshView<R>& v, const AssemblyCfg<R>& a, const CellFaces& cf, const R* U,
                        std::vector<Prim<R>>* p, std::vector<R>* qf, std::vector<R>* gr, std::vector<R>* fa,
                        std::vector<R>* r, std::vector<R>* l, AssemblyDiag* d) {
-  const int N = v.num_cells, F = v.num_faces;
+  const int N = v.num_cells;
   p->resize(N);
-  qf->resize(static_cast<std::size_t>(F) * 5);
-  gr->assign(static_cast<std::size_t>(N) * 10, R(0));
-  fa->resize(static_cast<std::size_t>(F) * 6);
-  r->resize(static_cast<std::size_t>(N) * 5);
-  l->resize(N);
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < N; ++c) (*p)[c] = cons_to_prim(cons_at(U, c), a.gamma);
-  if (a.second_order && v.gradient == 1) {
-    cell_gradients(v, *p, gr);  // least squares: the serial routine
-  } else if (a.second_order) {
-#pragma omp parallel for schedule(static)
-    for (int f = 0; f < F; ++f)
-      for (int k = 0; k < 5; ++k) {
-        const R qo = prim_var((*p)[v.owner[f]], k);
-        R q = qo;
-        if (v.neighbor[f] >= 0) q = AO_K(0.5) * (qo + prim_var((*p)[v.neighbor[f]], k));
-        (*qf)[static_cast<std::size_t>(5 * f + k)] = q;
-      }
-#pragma omp parallel for schedule(static)
-    for (int c = 0; c < N; ++c) {
-      const R iv = AO_K(1.0) / smax(v.volume[c], AO_K(1.0e-12));
-      for (int k = 0; k < 5; ++k)
-        for (int dd = 0; dd < 2; ++dd) {
-          R acc = R(0);
-          for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
-            const int f = cf.ent[e] >> 1;
-            const R nA = v.normal[3 * f + dd] * v.area[f];
-            const R t = (*qf)[static_cast<std::size_t>(5 * f + k)] * nA;
-            if (cf.ent[e] & 1) acc -= t; else acc += t;
-          }
-          (*gr)[static_cast<std::size_t>((c * 5 + k) * 2 + dd)] = acc * iv;
-        }
-    }
-  }
-  int n_fo = 0, n_diff = 0, first = F;
-#pragma omp parallel for schedule(static) reduction(+ : n_fo, n_diff) reduction(min : first)
-  for (int f = 0; f < F; ++f) {
-    A5<R> Fl;
-    R w;
-    bool fo, fbk;
-    face_contribution(v, a, *p, *gr, f, &Fl, &w, &fo, &fbk);
-    const R A = v.area[f];
-    for (int k = 0; k < 5; ++k) (*fa)[static_cast<std::size_t>(6 * f + k)] = Fl[k] * A;
-    (*fa)[static_cast<std::size_t>(6 * f + 5)] = w * A;
-    if (fo) {
-      ++n_fo;
-      first = std::min(first, f);
-    }
-    if (fbk) ++n_diff;
-  }
-#pragma omp parallel for schedule(static)
-  for (int c = 0; c < N; ++c) {
-    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
-    R w = R(0);
-    for (int e = cf.off[c]; e < cf.off[c + 1]; ++e) {
-      const std::size_t f = static_cast<std::size_t>(cf.ent[e] >> 1);
-      for (int k = 0; k < 5; ++k) {
-        if (cf.ent[e] & 1) acc[k] -= (*fa)[6 * f + k]; else acc[k] += (*fa)[6 * f + k];
-      }
-      w += (*fa)[6 * f + 5];
-    }
-    for (int k = 0; k < 5; ++k) (*r)[static_cast<std::size_t>(5 * c + k)] = acc[k];
-    (*l)[c] = w;
-  }
-  d->first_order_fallback = n_fo;
-  d->diffusive_fallback = n_diff;
-  d->first_failure_face = first < F ? first : -1;
+  if (a.second_order)
+    cell_gradients_threaded(v, cf, *p, qf, gr);
+  else
+    gr->assign(static_cast<std::size_t>(N) * 10, R(0));
+  assemble_residual_threaded(v, a, cf, *p, *gr, fa, r, l, d);
 }
 
 template <class R>
@@ -572,6 +492,11 @@ extern "C" int ao_max_threads() {
   extern "C" int ao_run_case_##SUF(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, T* hist, T* U, T* res,        \
                                    T* lam, T* forces8, T* cp, int* nonfinite, char* err, int errlen) {                   \
     return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen);                        \
+  }                                                                                                                      \
+  extern "C" int ao_run_case_threaded_##SUF(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, T* hist, T* U,      \
+                                            T* res, T* lam, T* forces8, T* cp, int* nonfinite, char* err,                \
+                                            int errlen) {                                                                \
+    return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen, true);                 \
   }                                                                                                                      \
   extern "C" int ao_run_case_cfg_##SUF(const ao_mesh* m, const ao_case_cfg* c, ao_residual_cfg* out) {              \
     return run_case_cfg_t<T>(m, c, out);                                                                                 \

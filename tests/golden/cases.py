@@ -11,6 +11,7 @@ from __future__ import annotations
 
 MESHES = {
     "40x8": dict(num_circumferential=40, num_radial=8, farfield_radius=10.0, radial_stretch=1.25),
+    "48x10": dict(num_circumferential=48, num_radial=10, farfield_radius=10.0, radial_stretch=1.25),
     "64x14": dict(num_circumferential=64, num_radial=14, farfield_radius=12.0, radial_stretch=1.35),
     "72x18": dict(num_circumferential=72, num_radial=18, farfield_radius=12.0, radial_stretch=1.35),
     "160x42": dict(num_circumferential=160, num_radial=42, farfield_radius=15.0, radial_stretch=1.45),
@@ -56,6 +57,23 @@ RUNS = {
         cfl_start=0.2, cfl_max=1.1, cfl_ramp_iters=120, startup_first_order_iters=25, mach=0.5, aoa_deg=1.25,
         residual_reduction_target=1.0e-3, force_stability_tol=2.0e-5, force_stability_window=6,
         force_mean_drift_tol=2.0e-5)),
+    # The reference's CPU / CUDA backend parity case (tests/cpp/test_euler_parity.cpp:15-34): RK1, zero CFL,
+    # first order throughout, gas constant 1, p_inf 0.05.
+    "parity_rk1": dict(mesh="48x10", env_precond=False, cfg=dict(
+        gamma=1.4, gas_constant=1.0, p_inf=0.05, t_inf=1.0, rho_inf=1.0, iterations=26, min_iterations=26,
+        cfl_start=0.0, cfl_max=0.0, cfl_ramp_iters=1, residual_reduction_target=0.0, force_stability_tol=0.0,
+        precond_on=0, rk_stages=1, startup_first_order_iters=1000)),
+    # The global pseudo-time step (compute_pseudo_time_step_over_volume's non-local branch,
+    # time_integrator.cpp:49-58) under RK1 and under SSP-RK3, through the first-order startup into second
+    # order, with the CFL ramp and adaptation active.
+    "rk1_global": dict(mesh="40x8", env_precond=False, cfg=dict(
+        iterations=60, min_iterations=60, rk_stages=1, local_time_stepping=0, cfl_start=0.3, cfl_max=0.6,
+        cfl_ramp_iters=20, startup_first_order_iters=20, residual_reduction_target=0.0, force_stability_tol=0.0,
+        flux_scheme=1, limiter=1, mach=0.3, aoa_deg=2.0)),
+    "rk3_global": dict(mesh="64x14", env_precond=False, cfg=dict(
+        iterations=50, min_iterations=50, rk_stages=3, local_time_stepping=0, cfl_start=0.2, cfl_max=1.2,
+        cfl_ramp_iters=30, startup_first_order_iters=10, residual_reduction_target=0.0, force_stability_tol=0.0,
+        flux_scheme=0, limiter=0, mach=0.4, aoa_deg=1.0)),
     # Preconditioning actually on (env gate set), characteristic farfield, staged all-speed controller.
     "precond_staged": dict(mesh="64x14", env_precond=True, cfg=dict(_LOW_MACH, mach=0.05, precond_on=1,
                                                                     precond_mach_ref=0.05, precond_farfield_bc=1,

@@ -116,7 +116,9 @@ def gradients(om: OracleMesh, U, gamma=1.4):
     return G
 
 
-def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int):
+def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int, threaded: bool = False):
+    """The oracle driver; threaded runs the stage residuals on all OpenMP threads (bit for bit the serial
+    run)."""
     dt = om.dtype
     hist = np.zeros((hist_cap, HIST_FIELDS), dt)
     U = np.zeros((om.N, 5), dt)
@@ -126,7 +128,8 @@ def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int):
     cp = np.zeros(max(om.wall_count, 1), dt)
     nonfinite = C.c_int(0)
     err = C.create_string_buffer(512)
-    n = getattr(oracle(), f"ao_run_case_{suffix(dt)}")(C.byref(om.c), C.byref(case), hist_cap, ptr(hist), ptr(U),
+    entry = "ao_run_case_threaded" if threaded else "ao_run_case"
+    n = getattr(oracle(), f"{entry}_{suffix(dt)}")(C.byref(om.c), C.byref(case), hist_cap, ptr(hist), ptr(U),
                                                        ptr(R), ptr(lam), ptr(forces), ptr(cp), C.byref(nonfinite),
                                                        err, len(err))
     if n < 0:

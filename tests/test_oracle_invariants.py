@@ -307,3 +307,18 @@ def test_threaded_oracle_equals_serial(dtype):
             a = ol.residual(om, cfg, U)
             b = ol.residual_threaded(om, cfg, U)
             assert all(np.array_equal(x, y) for x, y in zip(a, b)), (combo, amp)
+
+
+@pytest.mark.parametrize("name", ["case_file_numerics", "rk3_global", "precond_staged"])
+def test_threaded_oracle_driver_equals_serial(name):
+    """The oracle driver with its stage residuals on all threads (used to keep long GPU parity runs short)
+    reproduces the serial driver's history, state and forces bit for bit."""
+    run = cases.RUNS[name]
+    m = helpers.mesh(run["mesh"])
+    om = ol.OracleMesh(m, np.float64)
+    cfg = helpers.case_cfg(run["cfg"], run["env_precond"])
+    n = run["cfg"]["iterations"]
+    a = ol.run_case(om, cfg, n)
+    b = ol.run_case(om, cfg, n, threaded=True)
+    for k in ("history", "conserved", "residual", "spectral_radius", "forces", "cp"):
+        assert np.array_equal(a[k], b[k]), (name, k)
