@@ -76,7 +76,7 @@ def test_graph_replay_equals_eager(hip):
 
 @pytest.mark.parametrize("rk", [3, 1])
 def test_converged_cases_are_masked(hip, rk):
-    """A case that reaches its drop stops iterating while the others continue; its state is frozen, and the
+    """A case that reaches its drop stops iterating while the other continues; its state is frozen, and the
     batch returns it bitwise equal to the solo run of as many iterations.  Under RK1 (two ring buffers)
     the stopped case's state must be found whatever the parity of the iterations the other case runs after
     it stopped: the batch is rerun to one and to two iterations past the convergence."""
@@ -85,35 +85,39 @@ def test_converged_cases_are_masked(hip, rk):
     first_order = dict(BASE, startup_first_order_iters=100000, rk_stages=rk)
     if rk == 1:
         first_order["cfl_max"] = 0.8
-    # the first case converges (first order, M 0.3) long before the second (M 0.6) is allowed to stop
+    # first order at M 0.3 and M 0.6: the two cases reach the drop hundreds of iterations apart
     cases = [(0, 0.3, 0.0), (0, 0.6, 3.0)]
     cap = 1500
     say(f"[batch] masked convergence rk={rk}")
     with hip.HipBatch([g], cases, hip.CaseConfig(**first_order), hist_capacity=cap) as b:
         r = b.run(cap, rho_drop=1e-2, iters_per_sync=20)
-        c0, c1 = r["cases"]
-        assert c0["converged"] and c0["rho_ratio"] <= 1e-2 and c0["iterations"] < cap
-        s0 = b.state(0)
-        h = b.history(0)
-        n = c0["iterations"]
+        its = [c["iterations"] for c in r["cases"]]
+        a = int(np.argmin(its))  # the case that stopped first
+        o = 1 - a
+        ca = r["cases"][a]
+        assert ca["converged"] and ca["rho_ratio"] <= 1e-2 and ca["iterations"] < cap
+        n = ca["iterations"]
+        h = b.history(a)
         assert h[n - 1, 1] / h[0, 1] <= 1e-2 < h[n - 2, 1] / h[0, 1]
-        assert c1["iterations"] > n + 2, "the second case must still run after the first converged"
-        s1, n1 = b.state(1), c1["iterations"]
-    case = hip.CaseConfig(**dict(first_order, iterations=n, mach=0.3, aoa_deg=0.0))
-    with hip.HipMesh(m) as hm:
-        ref = hm.run_case(case)
-    assert np.array_equal(s0, ref["conserved"]), "a masked case must keep its state from the iteration it converged"
+        assert its[o] > n + 2, "the other case must still run after the first converged"
+        sa, so, no = b.state(a), b.state(o), its[o]
+    say(f"[batch] case {a} stopped at {n}, case {o} at {no}")
+
+    def solo_state(i, iters):
+        case = hip.CaseConfig(**dict(first_order, iterations=iters, mach=cases[i][1], aoa_deg=cases[i][2]))
+        with hip.HipMesh(m) as hm:
+            return hm.run_case(case)["conserved"]
+
+    ref = solo_state(a, n)
+    assert np.array_equal(sa, ref), "a masked case must keep its state from the iteration it converged"
     # the case still running after the other stopped (launches then cover its blocks only) is unaffected
-    case1 = hip.CaseConfig(**dict(first_order, iterations=n1, mach=0.6, aoa_deg=3.0))
-    with hip.HipMesh(m) as hm:
-        ref1 = hm.run_case(case1)
-    assert np.array_equal(s1, ref1["conserved"]), "the continuing case must equal its solo run"
+    assert np.array_equal(so, solo_state(o, no)), "the continuing case must equal its solo run"
     # one and two iterations of the other case past the convergence, with graph replay and without
     for extra, graph in ((1, True), (2, False)):
         with hip.HipBatch([g], cases, hip.CaseConfig(**first_order)) as b:
             r = b.run(n + extra, rho_drop=1e-2, iters_per_sync=7, use_graph=graph)
-            assert r["cases"][0]["iterations"] == n and r["cases"][1]["iterations"] == n + extra
-            assert np.array_equal(b.state(0), ref["conserved"]), (rk, extra)
+            assert r["cases"][a]["iterations"] == n and r["cases"][o]["iterations"] == n + extra
+            assert np.array_equal(b.state(a), ref), (rk, extra)
     g.close()
 
 
