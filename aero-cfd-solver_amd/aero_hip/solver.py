@@ -278,11 +278,13 @@ class HipMesh:
         return out
 
     def iterate(self, cfg: _lib.ResidualConfig, n_iters: int, cfl: float, rk_stages: int = 3,
-                history: bool = True):
+                history: bool = True, noalloc: bool = False):
         """n fixed-configuration pseudo-time iterations on the handle's state (aero_hip_iterate); returns the
-        density-residual L2 after each iteration, or None."""
+        density-residual L2 after each iteration, or None.  noalloc: the allocation-free entry point
+        (aero_hip_iterate_noalloc, at most AERO_NOALLOC_MAX_ITERS iterations with a history)."""
         h = np.zeros(max(n_iters, 1)) if history else None
-        self._call("aero_hip_iterate", C.byref(cfg), n_iters, rk_stages, cfl, _pd(h) if history else None)
+        name = "aero_hip_iterate_noalloc" if noalloc else "aero_hip_iterate"
+        self._call(name, C.byref(cfg), n_iters, rk_stages, cfl, _pd(h) if history else None)
         return h[:n_iters] if history else None
 
     def set_limiter_ratios(self, ratios: np.ndarray):

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -891,6 +892,82 @@ int aero_hip_iterate(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t 
   AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_iterate: sync");
   for (int it = 0; it < n_iters && rho_l2; ++it)
     rho_l2[it] = std::sqrt(sums[it] / static_cast<double>(std::max(1, m->N)));
+  return AERO_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free variant of aero_hip_iterate -----------------------------------------------------
+// The per-iteration density-residual sums go to a file-scope device array instead of a hipMalloc'd
+// buffer.  In host code `scratch` names a host-side stand-in, never the device memory: the device
+// address comes from hipGetSymbolAddress, taken once per device with that device current and kept here.
+namespace {
+constexpr int kNoAllocMaxIters = AERO_NOALLOC_MAX_ITERS;
+__device__ __align__(256) double scratch[kNoAllocMaxIters];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (taken on first use on that device).
+int scratch_on(int device, double** out, char* err, size_t errlen) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_err(err, errlen, "aero_hip_iterate_noalloc: device index out of range");
+    return AERO_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int prev = 0;
+    AERO_HIP_TRY(hipGetDevice(&prev), "aero_hip_iterate_noalloc: hipGetDevice");
+    AERO_HIP_TRY(hipSetDevice(device), "aero_hip_iterate_noalloc: hipSetDevice");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    AERO_HIP_TRY(e, "aero_hip_iterate_noalloc: hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return AERO_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int aero_hip_iterate_noalloc(aero_hip_mesh* m, const aero_residual_config* cfg, int32_t n_iters, int32_t rk_stages,
+                             double cfl, double* rho_l2, char* err, size_t errlen) {
+  if (!m || !cfg || n_iters < 0 || (rk_stages != 1 && rk_stages != 3)) {
+    set_err(err, errlen, "aero_hip_iterate_noalloc: bad argument (rk_stages must be 1 or 3)");
+    return AERO_EINVAL;
+  }
+  if (rho_l2 && n_iters > kNoAllocMaxIters) {
+    set_err(err, errlen, "aero_hip_iterate_noalloc: n_iters " + std::to_string(n_iters) +
+                           " exceeds the history scratch of " + std::to_string(kNoAllocMaxIters) +
+                           " iterations (AERO_NOALLOC_MAX_ITERS)");
+    return AERO_EINVAL;
+  }
+  double* d_sums = nullptr;
+  if (rho_l2 && n_iters > 0) {
+    if (const int rc = scratch_on(m->device, &d_sums, err, errlen)) return rc;
+  }
+  const WallArgs wa = m->wall;
+  for (int it = 0; it < n_iters; ++it) {
+    for (int st = 0; st < rk_stages; ++st) {
+      const int rc = aero_hip_stage(m, cfg, st, st == rk_stages - 1, cfl, 1, 0, err, errlen);
+      if (rc) return rc;
+    }
+    if (d_sums) {
+      AERO_HIP_TRY(aero_host::launch_iter_reduce(m->dm, m->partials, wa, m->U[m->cur], cfg->gamma, m->iter_out,
+                                                 nullptr, m->stream),
+                   "k_iter_reduce");
+      AERO_HIP_TRY(hipMemcpyAsync(d_sums + it, m->iter_out + 3, sizeof(double), hipMemcpyDeviceToDevice, m->stream),
+                   "aero_hip_iterate_noalloc: norm");
+    }
+  }
+  if (d_sums)  // every entry read back was written by this call
+    AERO_HIP_TRY(hipMemcpyAsync(rho_l2, d_sums, n_iters * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+                 "aero_hip_iterate_noalloc: history");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_iterate_noalloc: sync");
+  for (int it = 0; it < n_iters && rho_l2; ++it)
+    rho_l2[it] = std::sqrt(rho_l2[it] / static_cast<double>(std::max(1, m->N)));
   return AERO_OK;
 }
 

@@ -209,6 +209,17 @@ int aero_hip_set_limiter_ratios(aero_hip_mesh* mesh, const double* ratios, char*
 int aero_hip_iterate(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t n_iters, int32_t rk_stages,
                      double cfl, double* rho_l2, char* err, size_t errlen);
 
+/* aero_hip_iterate without any allocation: no hipMalloc / hipFree or other allocating call, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid it).  No
+ * counterpart in the reference.  The per-iteration density-residual sums are staged in a static device
+ * array of AERO_NOALLOC_MAX_ITERS doubles (512 KiB) that each GPU holds until the process ends; with
+ * rho_l2 != NULL, n_iters above that limit is rejected with AERO_EINVAL.  Every call on one GPU shares
+ * that GPU's array, so calls that overlap on one GPU (from several host threads or streams) race; calls
+ * in sequence do not (each call writes every entry it reads back).  Same results as aero_hip_iterate. */
+#define AERO_NOALLOC_MAX_ITERS 65536
+int aero_hip_iterate_noalloc(aero_hip_mesh* mesh, const aero_residual_config* cfg, int32_t n_iters,
+                             int32_t rk_stages, double cfl, double* rho_l2, char* err, size_t errlen);
+
 /* FreestreamReference (post/forces.hpp:9-17). */
 typedef struct aero_freestream_ref {
   double rho_inf, p_inf, aoa_deg, speed_inf, chord, x_ref, y_ref;
