@@ -264,28 +264,26 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
     const double A = m.s_A[s];
     Flux F;
     if (cn >= 0) {
-      double go[8], gn[8];
-      double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
+      // Reconstruction increments g . d of rho, u, v, p per side, formed as the gradients are loaded (the
+      // flux depends on the gradients only through them), times the frozen ratios when frozen.
+      double ao[4] = {0.0, 0.0, 0.0, 0.0}, an[4] = {0.0, 0.0, 0.0, 0.0};
       if (SECOND) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          go[j] = G[j * ld + co];
-          gn[j] = G[j * ld + cn];
-        }
         const double fx = m.s_fx[s], fy = m.s_fy[s];
-        dox = fx - m.ccx[co];
-        doy = fy - m.ccy[co];
-        dnx = fx - m.ccx[cn];
-        dny = fy - m.ccy[cn];
-      }
-      double fr[8];
-      if (FROZEN && SECOND) {  // slot order: consecutive slots, consecutive addresses
+        const double dox = fx - m.ccx[co], doy = fy - m.ccy[co];
+        const double dnx = fx - m.ccx[cn], dny = fy - m.ccy[cn];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) fr[j] = o.frozen_slots[j * m.nslots + s];
+        for (int k = 0; k < 4; ++k) {
+          ao[k] = recon_incr(G[(2 * k) * ld + co], G[(2 * k + 1) * ld + co], dox, doy);
+          an[k] = recon_incr(G[(2 * k) * ld + cn], G[(2 * k + 1) * ld + cn], dnx, dny);
+          if (FROZEN) {  // slot order: consecutive slots, consecutive addresses
+            ao[k] = o.frozen_slots[k * m.nslots + s] * ao[k];
+            an[k] = o.frozen_slots[(4 + k) * m.nslots + s] * an[k];
+          }
+        }
       }
       bool reverted;
-      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
-                                                         dnx, dny, o, &reverted, fr);
+      F = interior_face_loaded<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), ao, an, nx, ny, o,
+                                                                &reverted);
       if (reverted && (meta & kMetaPrimary)) {
         ++n_fo;
         first_fail = min(first_fail, m.s_face[s]);
