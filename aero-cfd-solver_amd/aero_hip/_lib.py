@@ -198,6 +198,10 @@ PROTOTYPES = {
     "aero_hip_residual_jvp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(TangentSeed), PD, PD,
                                         C.c_char_p, C.c_size_t]),
     "aero_hip_residual_vjp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_residual_device": (C.c_int, [vp, C.POINTER(ResidualConfig), vp, vp, vp, C.POINTER(ResidualDiag),
+                                           C.c_char_p, C.c_size_t]),
+    "aero_hip_residual_vjp_device": (C.c_int, [vp, C.POINTER(ResidualConfig), vp, vp, vp, C.c_char_p,
+                                               C.c_size_t]),
     "aero_hip_objective": (C.c_int, [vp, C.POINTER(Objective), PD, PD, PD, PD, PD, C.POINTER(TangentSeed), PD,
                                      C.c_char_p, C.c_size_t]),
     "aero_hip_adjoint_solve": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(LinearSolve), PD, PD,

@@ -181,6 +181,20 @@ class HipMesh:
         return R, lam, dict(first_order_fallback=d.first_order_fallback, diffusive_fallback=d.diffusive_fallback,
                             first_failure_face=d.first_failure_face)
 
+    def residual_device(self, U_ptr: int, cfg: _lib.ResidualConfig, R_ptr: int, lam_ptr: int = 0) -> dict:
+        """The residual of device arrays (aero_hip_residual_device): U_ptr, R_ptr (and lam_ptr, 0 = none)
+        are device addresses of [N, 5] / [N] FP64 arrays on this handle's GPU, inputs ready."""
+        d = _lib.ResidualDiag()
+        self._call("aero_hip_residual_device", C.byref(cfg), C.c_void_p(U_ptr), C.c_void_p(R_ptr),
+                   C.c_void_p(lam_ptr) if lam_ptr else None, C.byref(d))
+        return dict(first_order_fallback=d.first_order_fallback, diffusive_fallback=d.diffusive_fallback,
+                    first_failure_face=d.first_failure_face)
+
+    def residual_vjp_device(self, cfg: _lib.ResidualConfig, psi_ptr: int, U_ptr: int, out_ptr: int):
+        """R_U^T psi of device arrays (aero_hip_residual_vjp_device), [N, 5] FP64 device addresses."""
+        self._call("aero_hip_residual_vjp_device", C.byref(cfg), C.c_void_p(U_ptr), C.c_void_p(psi_ptr),
+                   C.c_void_p(out_ptr))
+
     def set_state(self, U: np.ndarray):
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.N, 5)
         self._call("aero_hip_set_state", _pd(U))

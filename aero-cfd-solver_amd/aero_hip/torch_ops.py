@@ -10,8 +10,10 @@
   a frozen limiter's ratios over to the adjoint.  With ``jacobian=True`` the first backward solves the CL
   and CD adjoints as one batched pair and keeps both gradient rows for further backward passes.
 
-Tensors may live on any device; values cross the C ABI as FP64 host arrays.  Only the Python layer uses
-torch: the numerics are the HIP kernels behind ``aero_hip``.
+``residual`` keeps GPU tensors on the GPU: their FP64 storage goes to the C ABI's device-pointer entry
+points (aero_hip_residual_device / aero_hip_residual_vjp_device) after the caller's stream is
+synchronised; CPU tensors cross as FP64 host arrays.  ``steady_coefficients`` takes its few parameters on
+any device.  Only the Python layer uses torch: the numerics are the HIP kernels behind ``aero_hip``.
 """
 from __future__ import annotations
 
@@ -32,17 +34,35 @@ def _np(t: torch.Tensor) -> np.ndarray:
     return t.detach().to("cpu", torch.float64).numpy()
 
 
+def _dev64(t: torch.Tensor) -> torch.Tensor:
+    """t as a contiguous FP64 tensor on its GPU, its producers finished (the C ABI's device entry points
+    run on the mesh handle's own stream)."""
+    t = t.detach().to(torch.float64).contiguous()
+    torch.cuda.current_stream(t.device).synchronize()
+    return t
+
+
 class ResidualFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U, hm, cfg):
-        R, _, _ = hm.residual(_np(U), cfg)
         ctx.hm, ctx.cfg = hm, cfg
         ctx.save_for_backward(U)
+        if U.is_cuda:
+            u = _dev64(U)
+            R = torch.empty_like(u)
+            hm.residual_device(u.data_ptr(), cfg, R.data_ptr())
+            return R.to(U.dtype)
+        R, _, _ = hm.residual(_np(U), cfg)
         return torch.from_numpy(R).to(U.device)
 
     @staticmethod
     def backward(ctx, grad_R):
         (U,) = ctx.saved_tensors
+        if U.is_cuda:
+            u, g = _dev64(U), _dev64(grad_R)
+            out = torch.empty_like(u)
+            ctx.hm.residual_vjp_device(ctx.cfg, g.data_ptr(), u.data_ptr(), out.data_ptr())
+            return out.to(grad_R.dtype), None, None
         out = ctx.hm.residual_vjp(ctx.cfg, _np(grad_R), _np(U))
         return torch.from_numpy(out).to(U.device, grad_R.dtype), None, None
 

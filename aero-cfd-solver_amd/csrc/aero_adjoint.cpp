@@ -623,6 +623,31 @@ int aero_hip_residual_vjp(aero_hip_mesh* m, const aero_residual_config* cfg, con
   return download_internal(m, a->out, out, err, errlen);
 }
 
+int aero_hip_residual_vjp_device(aero_hip_mesh* m, const aero_residual_config* cfg, const double* d_conserved,
+                                 const double* d_psi, double* d_out, char* err, size_t errlen) {
+  if (!m || !d_conserved || !d_psi || !d_out) {
+    set_err(err, errlen, "aero_hip_residual_vjp_device: null argument");
+    return AERO_EINVAL;
+  }
+  int rc = check_cfg(cfg, "aero_hip_residual_vjp_device", err, errlen);
+  if (!rc) rc = adjoint_setup(m, err, errlen);
+  if (rc) return rc;
+  m->cur = m->base = 0;
+  AERO_TRY(aero_host::launch_aos_to_internal(m->N, m->d_perm, d_conserved, m->U[0], m->stream),
+           "aero_hip_residual_vjp_device: state");
+  rc = enqueue_jacobian(m, cfg, err, errlen);
+  if (rc) return rc;
+  AdjointData* a = m->adj;
+  AERO_TRY(aero_host::launch_aos_to_internal(m->N, m->d_perm, d_psi, a->psi, m->stream),
+           "aero_hip_residual_vjp_device: psi");
+  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out, m->stream),
+           "adjoint product");
+  AERO_TRY(aero_host::launch_internal_to_aos(m->N, m->d_perm, a->out, d_out, nullptr, nullptr, m->stream),
+           "aero_hip_residual_vjp_device: output");
+  AERO_TRY(hipStreamSynchronize(m->stream), "aero_hip_residual_vjp_device: sync");
+  return AERO_OK;
+}
+
 int aero_hip_objective(aero_hip_mesh* m, const aero_objective* obj, const double* conserved, double* J, double* dJdU,
                        double* dJ_daoa, double* dJ_dq, const aero_tangent_seed* seed, double* dJ_seed, char* err,
                        size_t errlen) {

@@ -66,6 +66,29 @@ __global__ void k_aoa0_symmetry(int N, int ld, double* U, double g) {
   U[3 * ld + c] = y.E;
 }
 
+// Wall pressure-jump energy of the staged all-speed controller (euler_solver.cpp:1026-1047): over the
+// interior faces that touch a wall-adjacent cell, in increasing face index, sum (p_o - p_n)^2 of the
+// cells' primitive pressures.  The per-face terms are formed in parallel; one thread then adds them in
+// face order, the serial loop's summation order.  out[0] = the sum.  One workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_pjump_sum(int n, const int2* __restrict__ faces, const double* __restrict__ U,
+                                                   int ld, double g, double* __restrict__ terms,
+                                                   double* __restrict__ out) {
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int2 f = faces[i];
+    const aero_dev::C4 so{U[f.x], U[ld + f.x], U[2 * ld + f.x], U[3 * ld + f.x]};
+    const aero_dev::C4 sn{U[f.y], U[ld + f.y], U[2 * ld + f.y], U[3 * ld + f.y]};
+    const double dp = aero_dev::c2p(so, g).p - aero_dev::c2p(sn, g).p;
+    terms[i] = dp * dp;
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += terms[i];
+    out[0] = acc;
+  }
+}
+
 // Recursive coordinate bisection of cell centres into blocks of BC cells (every block full except the
 // last); cells inside a block keep increasing reference order.
 void rcb(std::vector<int>& idx, int lo, int hi, const double* cc, int BC) {
@@ -467,6 +490,7 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   AERO_SETUP(upload(m, &p_wA, w_A));
   AERO_SETUP(upload(m, &p_wfx, w_fx));
   AERO_SETUP(upload(m, &p_wfy, w_fy));
+  AERO_SETUP(upload(m, &m->d_perm, m->perm));
   for (int k = 0; k < 3; ++k) AERO_SETUP(dev_alloc(m, &m->U[k], static_cast<size_t>(4) * N));
   AERO_SETUP(dev_alloc(m, &m->G, static_cast<size_t>(8) * N));
   AERO_SETUP(dev_alloc(m, &m->R, static_cast<size_t>(4) * N));
@@ -797,6 +821,31 @@ int aero_hip_get_gradients(aero_hip_mesh* m, double* gradients, char* err, size_
     o[7] = 0.0;
     o[8] = g[6 * N + c];
     o[9] = g[7 * N + c];
+  }
+  return AERO_OK;
+}
+
+int aero_hip_residual_device(aero_hip_mesh* m, const aero_residual_config* cfg, const double* d_conserved,
+                             double* d_residual, double* d_spectral_radius, aero_residual_diag* diag, char* err,
+                             size_t errlen) {
+  if (!m || !cfg || !d_conserved || !d_residual) {
+    set_err(err, errlen, "aero_hip_residual_device: null argument");
+    return AERO_EINVAL;
+  }
+  m->cur = m->base = 0;
+  AERO_HIP_TRY(aero_host::launch_aos_to_internal(m->N, m->d_perm, d_conserved, m->U[0], m->stream),
+               "aero_hip_residual_device: state");
+  if (const int rc = aero_hip_residual_async(m, cfg, err, errlen)) return rc;
+  AERO_HIP_TRY(aero_host::launch_internal_to_aos(m->N, m->d_perm, m->R, d_residual, m->lam, d_spectral_radius,
+                                                 m->stream),
+               "aero_hip_residual_device: residual");
+  int dg[4];
+  AERO_HIP_TRY(hipMemcpyAsync(dg, m->diag, sizeof(dg), hipMemcpyDeviceToHost, m->stream), "diag copy");
+  AERO_HIP_TRY(hipStreamSynchronize(m->stream), "aero_hip_residual_device: hipStreamSynchronize");
+  if (diag) {
+    diag->first_order_fallback = dg[0];
+    diag->diffusive_fallback = dg[1];
+    diag->first_failure_face = dg[2] == INT_MAX ? -1 : dg[2];
   }
   return AERO_OK;
 }
@@ -1188,10 +1237,37 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     k.cm = k.moment / (q_inf * sref * chord);
     return k;
   };
-  std::vector<unsigned char> wall_adj;
+  // Faces of the wall pressure-jump energy (staged controller): interior faces touching a wall-adjacent
+  // cell, in increasing face index, as internal (owner, neighbour) ids; the sum runs on the device.
+  int n_pj = 0;
+  int2* d_pj_faces = nullptr;
+  double* d_pj_terms = nullptr;
+  struct PjFree {
+    int2* f;
+    double* t;
+    ~PjFree() {
+      if (f) (void)hipFree(f);
+      if (t) (void)hipFree(t);
+    }
+  } pj_guard{nullptr, nullptr};
   if (staged) {
-    wall_adj.assign(m->N, 0u);
+    std::vector<unsigned char> wall_adj(m->N, 0u);
     for (int i = 0; i < m->wall_count; ++i) wall_adj[m->h_owner[m->wall_start + i]] = 1u;
+    std::vector<int2> pf;
+    for (int f = 0; f < m->F; ++f) {
+      const int o = m->h_owner[f], nb = m->h_neighbor[f];
+      if (nb >= 0 && (wall_adj[o] || wall_adj[nb])) pf.push_back(make_int2(m->iperm[o], m->iperm[nb]));
+    }
+    n_pj = static_cast<int>(pf.size());
+    if (n_pj > 0) {
+      void* q = nullptr;
+      AERO_HIP_TRY(hipMalloc(&q, pf.size() * sizeof(int2)), "aero_hip_run_case: hipMalloc");
+      d_pj_faces = pj_guard.f = static_cast<int2*>(q);
+      AERO_HIP_TRY(hipMalloc(&q, pf.size() * sizeof(double)), "aero_hip_run_case: hipMalloc");
+      d_pj_terms = pj_guard.t = static_cast<double*>(q);
+      AERO_HIP_TRY(hipMemcpy(d_pj_faces, pf.data(), pf.size() * sizeof(int2), hipMemcpyHostToDevice),
+                   "aero_hip_run_case: p-jump faces");
+    }
   }
 
   Monitor mon(c->residual_reduction_target, c->force_stability_tol, c->force_stability_window, c->min_iterations,
@@ -1278,7 +1354,13 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     }
     AERO_HIP_TRY(aero_host::launch_iter_reduce(m->dm, m->partials, wa, m->U[m->cur], g, m->iter_out, nullptr, m->stream),
                  "k_iter_reduce");
-    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter, m->iter_out, 9 * sizeof(double), hipMemcpyDeviceToHost, m->stream),
+    if (n_pj > 0) {  // iter_out[9]: the wall pressure-jump sum of the new state
+      hipLaunchKernelGGL(k_pjump_sum, dim3(1), dim3(256), 0, m->stream, n_pj, d_pj_faces, m->U[m->cur], m->dm.ld, g,
+                         d_pj_terms, m->iter_out + 9);
+      AERO_HIP_TRY(hipGetLastError(), "k_pjump_sum");
+    }
+    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter, m->iter_out, (n_pj > 0 ? 10 : 9) * sizeof(double), hipMemcpyDeviceToHost,
+                                m->stream),
                  "iteration scalars");
     AERO_HIP_TRY(hipMemcpyAsync(m->h_iter + 10, m->diag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream),
                  "iteration counters");
@@ -1304,28 +1386,9 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     last = fg;
     mon.update_forces(fg);
     const bool window_stable = mon.stable();
-    if (staged) {  // p-jump diagnostic + staged all-speed controller (:1026-1139)
-      std::vector<double> u(static_cast<size_t>(4) * m->N);
-      AERO_HIP_TRY(hipMemcpy(u.data(), m->U[m->cur], u.size() * sizeof(double), hipMemcpyDeviceToHost), "state copy");
-      std::vector<double> p(m->N);
-      for (int ic = 0; ic < m->N; ++ic) {
-        const aero_dev::C4 s{u[ic], u[m->N + ic], u[2 * static_cast<size_t>(m->N) + ic],
-                             u[3 * static_cast<size_t>(m->N) + ic]};
-        const double r = std::max(s.r, 1.0e-8), inv = 1.0 / r, uu = s.ru * inv, vv = s.rv * inv;
-        const double ke = 0.5 * r * (uu * uu + vv * vv);
-        p[m->perm[ic]] = std::max((g - 1.0) * (s.E - ke), 1.0e-8);
-      }
-      double pj = 0.0;
-      int pn = 0;
-      for (int f = 0; f < m->F; ++f) {
-        const int o = m->h_owner[f], nb = m->h_neighbor[f];
-        if (nb < 0) continue;
-        if (wall_adj[o] || wall_adj[nb]) {
-          const double dp = p[o] - p[nb];
-          pj += dp * dp;
-          ++pn;
-        }
-      }
+    if (staged) {  // p-jump diagnostic (device sum, h[9]) + staged all-speed controller (:1026-1139)
+      const double pj = n_pj > 0 ? h[9] : 0.0;
+      const int pn = n_pj;
       const double pref2 = pressure_ref * pressure_ref + 1.0e-24;
       const double pjw = pn > 0 ? (pj / static_cast<double>(pn)) / pref2 : 0.0;
       if (stage_ab == 1) {

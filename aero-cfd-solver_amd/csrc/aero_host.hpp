@@ -15,6 +15,7 @@ struct aero_hip_mesh {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<int> perm;   // internal -> reference cell
+  int* d_perm = nullptr;   // the same on the device (the device-pointer entry points)
   std::vector<int> iperm;  // reference -> internal cell
   std::vector<void*> allocs;
   aero_dev::DevMesh dm{};

@@ -212,6 +212,11 @@ hipError_t launch_gradient(const aero_dev::DevMesh& m, const double* U, double* 
 hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, int scheme, int limiter, bool second,
                        int mode, const double* U, const double* G, double* R, double* lam, int* diag,
                        const aero_dev::StageArgs& st, hipStream_t s);
+// Caller device arrays in the reference layout ([N][5], reference numbering; perm: internal -> reference)
+// to / from the internal SoA [4][N]; lam (internal order) goes to lam_ref (reference order) when both are set.
+hipError_t launch_aos_to_internal(int N, const int* perm, const double* aos5, double* soa, hipStream_t s);
+hipError_t launch_internal_to_aos(int N, const int* perm, const double* soa, double* aos5, const double* lam,
+                                  double* lam_ref, hipStream_t s);
 // Frozen limiter ratios [8][F] at U with gradients G (k_limiter_capture).
 // Frozen limiter ratios [8][F] -> per face slot [8][S] (k_ratio_slots).
 hipError_t launch_ratio_slots(const aero_dev::DevMesh& m, const double* ratios, double* slots, hipStream_t s);

@@ -568,6 +568,34 @@ __global__ __launch_bounds__(256) void k_cell_scales(DevMesh m, FaceOpts o, cons
   out[static_cast<size_t>(m.N) + c] = local_mach(p, o.g);
 }
 
+// Layout conversions between the caller's device arrays (reference layout: [N][5] conserved in the
+// reference cell numbering, w dropped / written as 0) and the internal SoA [4][N]; perm maps an internal
+// cell to its reference index.
+__global__ __launch_bounds__(256) void k_aos_to_internal(int N, const int* __restrict__ perm,
+                                                         const double* __restrict__ aos5, double* __restrict__ soa) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= N) return;
+  const double* u = aos5 + 5 * static_cast<size_t>(perm[c]);
+  soa[c] = u[0];
+  soa[static_cast<size_t>(N) + c] = u[1];
+  soa[2 * static_cast<size_t>(N) + c] = u[2];
+  soa[3 * static_cast<size_t>(N) + c] = u[4];
+}
+
+__global__ __launch_bounds__(256) void k_internal_to_aos(int N, const int* __restrict__ perm,
+                                                         const double* __restrict__ soa, double* __restrict__ aos5,
+                                                         const double* __restrict__ lam, double* __restrict__ lam_ref) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= N) return;
+  double* o = aos5 + 5 * static_cast<size_t>(perm[c]);
+  o[0] = soa[c];
+  o[1] = soa[static_cast<size_t>(N) + c];
+  o[2] = soa[2 * static_cast<size_t>(N) + c];
+  o[3] = 0.0;
+  o[4] = soa[3 * static_cast<size_t>(N) + c];
+  if (lam_ref) lam_ref[perm[c]] = lam[c];
+}
+
 __global__ void k_min_dt(int N, const double* __restrict__ lam, double cfl, double* __restrict__ out) {
   __shared__ double red[256];
   double v = 1.79769313486231570815e+308;
@@ -1006,6 +1034,17 @@ hipError_t launch_face_scales(const DevMesh& m, const FaceOpts& o, const double*
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_cell_scales, dim3((m.N + 255) / 256), dim3(256), 0, s, m, o, U, cells);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_to_internal(int N, const int* perm, const double* aos5, double* soa, hipStream_t s) {
+  hipLaunchKernelGGL(k_aos_to_internal, dim3((N + 255) / 256), dim3(256), 0, s, N, perm, aos5, soa);
+  return hipGetLastError();
+}
+
+hipError_t launch_internal_to_aos(int N, const int* perm, const double* soa, double* aos5, const double* lam,
+                                  double* lam_ref, hipStream_t s) {
+  hipLaunchKernelGGL(k_internal_to_aos, dim3((N + 255) / 256), dim3(256), 0, s, N, perm, soa, aos5, lam, lam_ref);
   return hipGetLastError();
 }
 

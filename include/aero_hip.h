@@ -384,6 +384,18 @@ int aero_hip_residual_jvp(aero_hip_mesh* mesh, const aero_residual_config* cfg, 
 /* out = R_U^T psi, through the stored face Jacobians. */
 int aero_hip_residual_vjp(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* conserved,
                           const double* psi, double* out, char* err, size_t errlen);
+
+/* Device-pointer forms of aero_hip_residual (euler_residual_cuda, cuda_backend.hpp:49-52) and
+ * aero_hip_residual_vjp, for callers whose arrays already live on the handle's GPU (the torch layer):
+ * the same layouts ([N][5] conserved / residual / psi in the reference numbering, spectral radius [N],
+ * which may be NULL), as device pointers; the arrays never cross to the host.  Work runs on the handle's
+ * stream: the caller makes its inputs ready first (e.g. synchronises its own stream); the call returns
+ * when the outputs are written.  They set the handle's state to d_conserved. */
+int aero_hip_residual_device(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* d_conserved,
+                             double* d_residual, double* d_spectral_radius, aero_residual_diag* diag, char* err,
+                             size_t errlen);
+int aero_hip_residual_vjp_device(aero_hip_mesh* mesh, const aero_residual_config* cfg, const double* d_conserved,
+                                 const double* d_psi, double* d_out, char* err, size_t errlen);
 /* Objective value, dJ/dU [N][5] (may be NULL) and the explicit partials: dJ/d(aoa, per degree) and
  * dJ/dq_inf at fixed state, and the derivative along the geometry of `seed` (may be NULL). */
 int aero_hip_objective(aero_hip_mesh* mesh, const aero_objective* obj, const double* conserved, double* J,

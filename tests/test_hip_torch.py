@@ -49,6 +49,32 @@ def test_residual_autograd(hip):
         assert abs(an - fd) <= 1e-6 * abs(fd), (an, fd)
 
 
+def test_residual_autograd_on_device_tensors(hip):
+    """With the state on the GPU the Function keeps every array there (the device-pointer entry points) and
+    gives the host path's residual and R_U^T w bit for bit."""
+    import torch
+    from aero_hip.torch_ops import residual
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    from aero_hip import freestream
+    ff = freestream(0.5, 2.0)[1]
+    U0 = helpers.perturbed_state(m.cell_center, ff, 0.05, 17)
+    cfg = helpers.res_cfg(cases.residual_cfg_values((1, 1, 1, 1, 0, 0, 0), ff))
+    rng = np.random.default_rng(4)
+    w0 = rng.standard_normal(U0.shape)
+    with hip.HipMesh(m) as hm:
+        say("[torch] residual forward / backward on cuda tensors")
+        U = torch.from_numpy(U0.copy()).to("cuda").requires_grad_(True)
+        w = torch.from_numpy(w0).to("cuda")
+        R = residual(U, hm, cfg)
+        assert R.is_cuda
+        (w * R).sum().backward()
+        assert U.grad.is_cuda
+        Rh, _, _ = hm.residual(U0, cfg)
+        ref = hm.residual_vjp(cfg, w0, U0)
+        assert np.array_equal(R.detach().cpu().numpy(), Rh)
+        assert np.array_equal(U.grad.cpu().numpy(), ref)
+
+
 def test_steady_coefficients_gradient(hip):
     """d(CL + 0.5 CD)/d(aoa, mach, amplitudes) through the autograd Function against central differences of
     the converged coefficients (first-order Rusanov on the 40x8 mesh, converged to round-off)."""
