@@ -268,7 +268,7 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
       AERO_BSETUP(balloc(b, &cs.lim_ratio, 8 * static_cast<size_t>(m->F)));
       AERO_BSETUP(hipMemset(cs.lim_ratio, 0, 8 * static_cast<size_t>(m->F) * sizeof(double)));
       AERO_BSETUP(balloc(b, &cs.lim_slot, 8 * static_cast<size_t>(m->S)));
-      AERO_BSETUP(hipMemset(cs.lim_slot, 0, 8 * static_cast<size_t>(m->S) * sizeof(double)));
+      AERO_BSETUP(hipMemset(cs.lim_slot, 0, 8 * static_cast<size_t>(m->S) * sizeof(unsigned short)));
     }
     AERO_BSETUP(balloc(b, &cs.diag, 4));
     // freestream of this case (euler_solver.cpp:543-561) and its schedules

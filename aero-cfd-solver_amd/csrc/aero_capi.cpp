@@ -139,19 +139,21 @@ template <class T> hipError_t upload(aero_hip_mesh* m, T** p, const std::vector<
   return e;
 }
 
-// The frozen-ratio buffers of a handle: face order [8][F] and the flux kernels' slot order [8][S].
+// The frozen-ratio buffers of a handle: face order [8][F] (FP64) and the flux kernels' slot order [S][8] (16-bit).
 int ensure_ratio_buffers(aero_hip_mesh* m, const char* who, char* err, size_t errlen) {
-  for (auto [pp, n] : {std::pair<double**, size_t>{&m->lim_ratio, 8 * static_cast<size_t>(m->F)},
-                       std::pair<double**, size_t>{&m->lim_slot, 8 * static_cast<size_t>(m->S)}}) {
-    if (*pp) continue;
+  void** bufs[2] = {reinterpret_cast<void**>(&m->lim_ratio), reinterpret_cast<void**>(&m->lim_slot)};
+  const size_t bytes[2] = {8 * static_cast<size_t>(m->F) * sizeof(double),
+                           8 * static_cast<size_t>(m->S) * sizeof(unsigned short)};
+  for (int k = 0; k < 2; ++k) {
+    if (*bufs[k]) continue;
     void* q = nullptr;
-    const hipError_t e = hipMalloc(&q, n * sizeof(double));
+    const hipError_t e = hipMalloc(&q, bytes[k]);
     if (e != hipSuccess) {
       set_err(err, errlen, std::string(who) + ": hipMalloc: " + hipGetErrorString(e));
       return AERO_EHIP;
     }
     m->allocs.push_back(q);
-    *pp = static_cast<double*>(q);
+    *bufs[k] = q;
   }
   return AERO_OK;
 }

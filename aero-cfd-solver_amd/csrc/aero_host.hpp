@@ -40,7 +40,7 @@ struct aero_hip_mesh {
   int wall_start = 0, wall_count = 0;
   AdjointData* adj = nullptr;
   double* lim_ratio = nullptr;  // frozen limiter ratios [8][F] (aero_hip_freeze_limiter)
-  double* lim_slot = nullptr;   // the same per face slot [8][S] (read by the flux kernels)
+  unsigned short* lim_slot = nullptr;  // the same per face slot [S][8] in units of 2^-15 (flux kernels)
   bool lim_valid = false;
 };
 

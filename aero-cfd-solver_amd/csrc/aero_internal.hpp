@@ -88,7 +88,8 @@ struct BatchCase {
   int cfl_ramp, floor_ramp, as_on, as_len;  // ramp lengths and the all-speed ramp start
   WallArgs wall;     // wall faces of the geometry + this case's reference values
   double* lim_ratio; // frozen limiter ratios [8][F] (beyond the reference) or null
-  double* lim_slot;  // the same per face slot of the geometry [8][S] (the flux kernel's coalesced copy)
+  unsigned short* lim_slot;  // the same per face slot of the geometry [S][8] in units of 2^-15 (the flux
+                             // kernel's coalesced copy)
 };
 
 // Device-resident control state of one case, advanced by k_batch_control after every iteration with the
@@ -218,8 +219,8 @@ hipError_t launch_aos_to_internal(int N, const int* perm, const double* aos5, do
 hipError_t launch_internal_to_aos(int N, const int* perm, const double* soa, double* aos5, const double* lam,
                                   double* lam_ref, hipStream_t s);
 // Frozen limiter ratios [8][F] at U with gradients G (k_limiter_capture).
-// Frozen limiter ratios [8][F] -> per face slot [8][S] (k_ratio_slots).
-hipError_t launch_ratio_slots(const aero_dev::DevMesh& m, const double* ratios, double* slots, hipStream_t s);
+// Frozen limiter ratios [8][F] -> per face slot [S][8] in units of 2^-15 (k_ratio_slots).
+hipError_t launch_ratio_slots(const aero_dev::DevMesh& m, double* ratios, unsigned short* slots, hipStream_t s);
 hipError_t launch_face_scales(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, const double* U, double* faces,
                               double* cells, hipStream_t s);
 hipError_t launch_limiter_capture(const aero_dev::DevMesh& m, int limiter, const double* U, const double* G, double g,

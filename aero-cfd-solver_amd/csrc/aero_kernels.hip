@@ -264,26 +264,33 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
     const double A = m.s_A[s];
     Flux F;
     if (cn >= 0) {
-      // Reconstruction increments g . d of rho, u, v, p per side, formed as the gradients are loaded (the
-      // flux depends on the gradients only through them), times the frozen ratios when frozen.
-      double ao[4] = {0.0, 0.0, 0.0, 0.0}, an[4] = {0.0, 0.0, 0.0, 0.0};
+      double go[8], gn[8];
+      double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
       if (SECOND) {
-        const double fx = m.s_fx[s], fy = m.s_fy[s];
-        const double dox = fx - m.ccx[co], doy = fy - m.ccy[co];
-        const double dnx = fx - m.ccx[cn], dny = fy - m.ccy[cn];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          ao[k] = recon_incr(G[(2 * k) * ld + co], G[(2 * k + 1) * ld + co], dox, doy);
-          an[k] = recon_incr(G[(2 * k) * ld + cn], G[(2 * k + 1) * ld + cn], dnx, dny);
-          if (FROZEN) {  // slot order: consecutive slots, consecutive addresses
-            ao[k] = o.frozen_slots[k * m.nslots + s] * ao[k];
-            an[k] = o.frozen_slots[(4 + k) * m.nslots + s] * an[k];
-          }
+        for (int j = 0; j < 8; ++j) {
+          go[j] = G[j * ld + co];
+          gn[j] = G[j * ld + cn];
+        }
+        const double fx = m.s_fx[s], fy = m.s_fy[s];
+        dox = fx - m.ccx[co];
+        doy = fy - m.ccy[co];
+        dnx = fx - m.ccx[cn];
+        dny = fy - m.ccy[cn];
+      }
+      double fr[8];
+      if (FROZEN && SECOND) {  // one 16-byte load per slot: eight 16-bit ratios in units of 2^-15
+        const uint4 q4 = reinterpret_cast<const uint4*>(o.frozen_slots)[s];
+        const unsigned int w[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          fr[2 * j] = static_cast<double>(w[j] & 0xffffu) * kRatioStep;
+          fr[2 * j + 1] = static_cast<double>(w[j] >> 16) * kRatioStep;
         }
       }
       bool reverted;
-      F = interior_face_loaded<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), ao, an, nx, ny, o,
-                                                                &reverted);
+      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
+                                                         dnx, dny, o, &reverted, fr);
       if (reverted && (meta & kMetaPrimary)) {
         ++n_fo;
         first_fail = min(first_fail, m.s_face[s]);
@@ -502,16 +509,24 @@ __global__ __launch_bounds__(BC) void k_limiter_capture(DevMesh m, const double*
   limiter_capture_body<BC, LIMITER>(m, b, U, G, g, ratios);
 }
 
-// Non-LTS stage tail (global dt, time_integrator.cpp:276-285): dt = min over cells, then update.
-// Frozen limiter ratios from face order [8][F] to slot order [8][S], once per capture or install: the flux
-// kernels then read them coalesced (the faces of a block are scattered in reference numbering).
-__global__ __launch_bounds__(256) void k_ratio_slots(DevMesh m, const double* __restrict__ fr,
-                                                     double* __restrict__ out) {
+// Frozen limiter ratios from face order [8][F] to slot order [S][8] (16-bit, units of 2^-15), once per
+// capture or install: the flux kernels then read one coalesced 16-byte record per slot (the faces of a
+// block are scattered in reference numbering).
+__global__ __launch_bounds__(256) void k_ratio_slots(DevMesh m, double* __restrict__ fr,
+                                                     unsigned short* __restrict__ out) {
+  // Also brings ratios installed from outside (aero_hip_set_limiter_ratios) to their 2^-15 resolution in
+  // the face-order copy, which the tangent and adjoint kernels read: the face's primary slot writes it.
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= m.nslots) return;
   const int f = m.s_face[s];
+  const bool primary = (m.s_meta[s] & kMetaPrimary) != 0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) out[static_cast<size_t>(j) * m.nslots + s] = fr[static_cast<size_t>(j) * m.nfaces + f];
+  for (int j = 0; j < 8; ++j) {
+    const size_t at = static_cast<size_t>(j) * m.nfaces + f;
+    const double q = quantize_ratio(fr[at]);
+    out[static_cast<size_t>(s) * 8 + j] = static_cast<unsigned short>(q * kRatioUnits);
+    if (primary) fr[at] = q;
+  }
 }
 
 // Per-face acoustic scaling record of a state, the post-processing diagnostics of the reference driver:
@@ -1048,7 +1063,7 @@ hipError_t launch_internal_to_aos(int N, const int* perm, const double* soa, dou
   return hipGetLastError();
 }
 
-hipError_t launch_ratio_slots(const DevMesh& m, const double* ratios, double* slots, hipStream_t s) {
+hipError_t launch_ratio_slots(const DevMesh& m, double* ratios, unsigned short* slots, hipStream_t s) {
   hipLaunchKernelGGL(k_ratio_slots, dim3((m.nslots + 255) / 256), dim3(256), 0, s, m, ratios, slots);
   return hipGetLastError();
 }

@@ -72,8 +72,18 @@ template <class T> struct FaceOptsT {
   int as_fix;
   double as_cut, as_fmin, as_w;
   const double* frozen;  // frozen limiter ratios [8][F] (beyond the reference) or null
-  const double* frozen_slots;  // the same ratios per face slot [8][S], as the flux kernels read them
+  const unsigned short* frozen_slots;  // the same per face slot [S][8] in units of kRatioStep (flux kernels)
 };
+
+// Frozen limiter ratios (beyond the reference) are held at a resolution of 2^-15: a captured ratio
+// r in [0, 1] is stored as the nearest multiple of 2^-15 (ties to even), so the flux kernels can stream it
+// as a 16-bit integer q with ratio = q * 2^-15 exactly.
+constexpr double kRatioUnits = 32768.0;
+constexpr double kRatioStep = 1.0 / 32768.0;
+__host__ __device__ __forceinline__ double quantize_ratio(double r) {
+  const double c = (r < 0.0) ? 0.0 : (1.0 < r) ? 1.0 : r;  // NaN -> 0
+  return (c == c ? rint(c * kRatioUnits) : 0.0) * kRatioStep;
+}
 using FaceOpts = FaceOptsT<double>;
 
 template <class T> __device__ __forceinline__ C4T<T> p2c(const P4T<T>& w, double g) {  // euler_flux.cpp:101-113

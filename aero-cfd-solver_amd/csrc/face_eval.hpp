@@ -60,17 +60,7 @@ __device__ __forceinline__ void recon_var_frozen(T qo, T qn, int k, const Src& s
   *qr = qn + rn * db;
 }
 
-// Limited increments given directly (the frozen limiter's ratio * a, formed where the increments are
-// loaded): the reconstruction only adds them.
-template <class T, class Src>
-__device__ __forceinline__ void recon_var_applied(T qo, T qn, int k, const Src& src, T* ql, T* qr) {
-  T da, db;
-  src(k, &da, &db);
-  *ql = qo + da;
-  *qr = qn + db;
-}
-
-// The capture: ratio phi(a, b) / a of both sides of one variable.
+// The capture: ratio phi(a, b) / a of both sides of one variable, held at 2^-15 (quantize_ratio).
 template <int LIMITER>
 __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, double goy, double gnx, double gny,
                                               double dox, double doy, double dnx, double dny, double* ro, double* rn) {
@@ -81,15 +71,14 @@ __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, 
   double db = 0.0;
   db += gnx * dnx;
   db += gny * dny;
-  *ro = da != 0.0 ? limiter<LIMITER>(da, dc) / da : 0.0;
-  *rn = db != 0.0 ? limiter<LIMITER>(db, -dc) / db : 0.0;
+  *ro = quantize_ratio(da != 0.0 ? limiter<LIMITER>(da, dc) / da : 0.0);
+  *rn = quantize_ratio(db != 0.0 ? limiter<LIMITER>(db, -dc) / db : 0.0);
 }
 
 // Interior face with the first-order fallback (residual_assembly.cpp:107-137), the reconstruction
 // increments taken from src.  *reverted reports the fallback to cell values.
 // FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
-// APPLIED: src gives the limited increments themselves (frozen ratio * g . d).
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, class T, class Src, bool APPLIED = false>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, class T, class Src>
 __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4T<T>& un, const Src& src, T nx, T ny,
                                                       const FaceOptsT<T>& o, bool* reverted, const double* fr) {
   const double g = o.g;
@@ -98,12 +87,7 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
   *reverted = false;
   if (SECOND) {
     P4T<T> l2, r2;
-    if (APPLIED) {
-      recon_var_applied(L.r, Rt.r, 0, src, &l2.r, &r2.r);
-      recon_var_applied(L.u, Rt.u, 1, src, &l2.u, &r2.u);
-      recon_var_applied(L.v, Rt.v, 2, src, &l2.v, &r2.v);
-      recon_var_applied(L.p, Rt.p, 3, src, &l2.p, &r2.p);
-    } else if (FROZEN) {
+    if (FROZEN) {
       recon_var_frozen(L.r, Rt.r, 0, src, fr[0], fr[4], &l2.r, &r2.r);
       recon_var_frozen(L.u, Rt.u, 1, src, fr[1], fr[5], &l2.u, &r2.u);
       recon_var_frozen(L.v, Rt.v, 2, src, fr[2], fr[6], &l2.v, &r2.v);
@@ -151,18 +135,6 @@ __device__ __forceinline__ FluxT<T> interior_face_incr(const C4T<T>& uo, const C
                                                        const double* fr = nullptr) {
   const GivenIncr<T> src{ao, an};
   return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, src, nx, ny, o, reverted, fr);
-}
-
-// The primal kernels' form: the eight increments ao / an formed as the gradients are loaded (g . d of
-// rho, u, v, p, in recon_incr's operation order; times the frozen ratios when FROZEN), so the gradients
-// themselves are not held across the flux.
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
-__device__ __forceinline__ Flux interior_face_loaded(const C4& uo, const C4& un, const double* ao, const double* an,
-                                                     double nx, double ny, const FaceOpts& o, bool* reverted) {
-  const GivenIncr<double> src{ao, an};
-  if (FROZEN) return interior_face_src<SCHEME, LIMITER, SECOND, false, double, GivenIncr<double>, true>(
-      uo, un, src, nx, ny, o, reverted, nullptr);
-  return interior_face_src<SCHEME, LIMITER, SECOND, false>(uo, un, src, nx, ny, o, reverted, nullptr);
 }
 
 // Boundary face: slip wall or farfield on the owner's first-order state (residual_assembly.cpp:138-166,

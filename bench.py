@@ -11,7 +11,12 @@ Prints one JSON line (rank 0).  `roofline` uses the algorithmic bytes of the dom
 48 B per face + 152 B per cell, over its average duration from HIP events recorded on the kernel's
 stream inside the timed region.  `cpu_baseline` times the FP64 CPU oracle (a serial restatement of the
 reference CPU path) on a bounded sample of the same workload; `cpu_reference` times the compiled
-reference's own CPU residual (FP32) on the same mesh and state, where oracle/_ref is built.
+reference's own CPU residual (FP32) on the same mesh and state, where oracle/_ref is built.  The same line
+carries the other two headline quantities, measured after the timed residual region: `sweep` (the 64-case
+batched AoA x Mach sweep to an 8-decade drop, solves/hr and the speed-up over sequential solves timed on
+--default-seq-cases of the cases) and `adjoint` (adjoint / primal time of the dCL, dCD gradients on the
+Hicks-Henne-deformed 66k mesh).  Other modes: --sweep (every sequential case timed), --adjoint (with
+finite-difference and tangent checks), --uneven, --vjp.
 """
 from __future__ import annotations
 
@@ -206,21 +211,41 @@ def variant_configs(flux: str, lim: str, mach: float, aoa: float):
     return out
 
 
+FLUX_KERNEL_SOURCES = ("aero_kernels.hip", "face_eval.hpp", "euler_device.hpp", "aero_internal.hpp")
+
+
+def kernel_source_digest() -> str:
+    """sha256 of the flux kernel's sources, which a committed traffic measurement is keyed to."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in FLUX_KERNEL_SOURCES:
+        h.update((ROOT / "aero-cfd-solver_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()
+
+
 def traffic_for(workload: str):
+    """HBM bytes per k_flux launch from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration), only when it was measured on the current kernel sources; else None."""
     p = ROOT / "profiles" / "traffic.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get(workload)
-        except Exception:
-            return None
-    return None
+    try:
+        rec = json.loads(p.read_text()).get(workload)
+    except Exception:
+        return None
+    if not isinstance(rec, dict) or rec.get("kernel_sources_sha256") != kernel_source_digest():
+        return None
+    return rec.get("bytes")
 
 
-def run_sweep(args) -> int:
+SWEEP_MESH = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
+                  quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+
+
+def measure_sweep(args, seq_cases: int) -> dict:
     """64-case AoA x Mach sweep on one ~66k-cell jittered mixed mesh: batched (one launch per RK stage over
-    all cases, device-side control, hipGraph replay, host sync every 100 iterations) against the same cases
-    solved one after another (batch of 1, no graph, host sync every iteration).  A case stops at an 8-decade
-    density-residual drop or at the iteration cap."""
+    all cases, device-side control, hipGraph replay, host sync every 500 iterations) against the same cases
+    solved one after another (batch of 1, no graph, host sync every iteration), `seq_cases` of them timed
+    (all 64: measured; fewer: spread over the sweep and extrapolated by iteration count).  A case stops at an
+    8-decade density-residual drop or at the iteration cap.  Returns the line's fields."""
     import numpy as np
     import torch
     import aero_hip
@@ -228,8 +253,7 @@ def run_sweep(args) -> int:
 
     if not aero_hip.hip_available():
         raise SystemExit("bench.py: no gfx950 device visible")
-    mkw = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
-               quad_layers=30, jitter=0.1, seed=12345, rcm=True)
+    mkw = SWEEP_MESH
     log("[sweep] generating mesh")
     mesh = generate_mesh(precision=8, **mkw)
     g = HipMesh(mesh, block_cells=args.block_cells)
@@ -246,49 +270,103 @@ def run_sweep(args) -> int:
         torch.cuda.synchronize()
         rb = b.run(cap, 1e-8, iters_per_sync=500, use_graph=not args.sweep_no_graph)
     t_batch = rb["seconds"]
-    nseq = min(args.sweep_seq_cases, len(cases))
+    log(f"[sweep] batched: {t_batch:.1f} s")
+    nseq = min(seq_cases, len(cases))
     log(f"[sweep] sequential: {nseq} cases one after another")
     t_seq = 0.0
     seq_rows = []
     for i in range(nseq):
         j = (i * len(cases)) // nseq  # spread the timed cases over the sweep
-        log(f"[sweep] sequential case {j}")
         with HipBatch([g], [cases[j]], base) as b1:
             r1 = b1.run(cap, 1e-8, iters_per_sync=1, use_graph=False)
         t_seq += r1["seconds"]
         seq_rows.append((j, r1["cases"][0]))
+        log(f"[sweep] sequential case {j}: {r1['seconds']:.2f} s, {r1['cases'][0]['iterations']} iterations")
     # each sequential case ran the same iterations as in the batch (identical per-case arithmetic)
     same = all(r["iterations"] == rb["cases"][j]["iterations"] and r["rho_l2"] == rb["cases"][j]["rho_l2"]
                for j, r in seq_rows)
-    # extrapolate by iterations: the timed cases' seconds per iteration times every case's iterations
     it_timed = sum(r["iterations"] for _, r in seq_rows)
     it_all = sum(r["iterations"] for r in rb["cases"])
-    t_seq_all = t_seq / max(it_timed, 1) * it_all
+    measured = nseq == len(cases)
+    t_seq_all = t_seq if measured else t_seq / max(it_timed, 1) * it_all
     g.close()
     rows = rb["cases"]
     n_conv = sum(int(r["converged"]) for r in rows)
-    value = n_conv / t_batch * 3600.0  # solves = cases that reached the 8-decade drop
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "solves/hr", "n_gpus": 1, "steps": cap,
-           "warmup": 0, "ms_per_step": round(t_batch / max(max(r["iterations"] for r in rows), 1) * 1e3, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "sweep64_mixed66k_hllc_venkat", "cells": mesh.num_cells, "cases": len(cases),
-                      "mesh": mkw, "aoa_deg": [float(a) for a in aoas], "mach": machs, "iteration_cap": cap,
-                      "rho_drop_target": 1e-8, "cfl_max": args.sweep_cfl, "limiter_freeze_iter": args.freeze_iter},
-           "sweep": {"batched_seconds": round(t_batch, 3), "sequential_seconds": round(t_seq_all, 3),
-                     "cases_per_hr": round(len(cases) / t_batch * 3600.0, 1),
-                     "sequential_cases_timed": nseq, "sequential_extrapolation": "seconds per iteration of the "
-                     "timed cases x all cases' iterations",
-                     "speedup_vs_sequential": round(t_seq_all / t_batch, 2) if t_seq_all > 0 else None,
-                     "iterations_max": max(r["iterations"] for r in rows),
-                     "iterations_median": float(np.median([r["iterations"] for r in rows])),
-                     "sequential_solves_per_hr": round(n_conv / t_seq_all * 3600.0, 1) if t_seq_all > 0 else None,
-                     "sequential_matches_batched": same,
-                     "converged_8_decades": sum(r["converged"] for r in rows),
-                     "failed_nonfinite": sum(r["failed"] for r in rows),
-                     "median_rho_drop": float(np.median([r["rho_ratio"] for r in rows])),
-                     "cl": [round(r["cl"], 6) for r in rows], "cd": [round(r["cd"], 6) for r in rows]}}
+    return {"value": round(n_conv / t_batch * 3600.0, 1), "unit": "solves/hr", "cap": cap, "cells": mesh.num_cells,
+            "ms_per_step": round(t_batch / max(max(r["iterations"] for r in rows), 1) * 1e3, 4),
+            "config": {"workload": "sweep64_mixed66k_hllc_venkat", "cells": mesh.num_cells, "cases": len(cases),
+                       "mesh": mkw, "aoa_deg": [float(a) for a in aoas], "mach": machs, "iteration_cap": cap,
+                       "rho_drop_target": 1e-8, "cfl_max": args.sweep_cfl, "limiter_freeze_iter": args.freeze_iter,
+                       "frozen_ratio_resolution": 2.0 ** -15},
+            "sweep": {"solves_per_hr": round(n_conv / t_batch * 3600.0, 1),
+                      "batched_seconds": round(t_batch, 3), "sequential_seconds": round(t_seq_all, 3),
+                      "cases_per_hr": round(len(cases) / t_batch * 3600.0, 1),
+                      "sequential_cases_timed": nseq,
+                      "sequential_baseline": ("measured: every case solved alone, one after another" if measured else
+                                              "seconds per iteration of the timed cases x all cases' iterations"),
+                      "speedup_vs_sequential": round(t_seq_all / t_batch, 2) if t_seq_all > 0 else None,
+                      "iterations_max": max(r["iterations"] for r in rows),
+                      "iterations_median": float(np.median([r["iterations"] for r in rows])),
+                      "sequential_solves_per_hr": round(n_conv / t_seq_all * 3600.0, 1) if t_seq_all > 0 else None,
+                      "sequential_matches_batched": same,
+                      "converged_8_decades": n_conv,
+                      "failed_nonfinite": sum(r["failed"] for r in rows),
+                      "median_rho_drop": float(np.median([r["rho_ratio"] for r in rows])),
+                      "cl": [round(r["cl"], 6) for r in rows], "cd": [round(r["cd"], 6) for r in rows]}}
+
+
+def run_sweep(args) -> int:
+    """The sweep as its own line (measure_sweep with --sweep-seq-cases sequential cases, default all 64)."""
+    m = measure_sweep(args, args.sweep_seq_cases)
+    out = {"metric": METRIC, "value": m["value"], "unit": m["unit"], "n_gpus": 1, "steps": m["cap"], "warmup": 0,
+           "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic", "config": m["config"], "sweep": m["sweep"]}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_adjoint_pair(args) -> dict:
+    """The config-4 workload's gradients without the finite-difference checks: the deformed 66k mesh's
+    primal through the torch Function's forward (a one-case batch to an 8-decade drop) and one backward that
+    solves the CL and CD adjoints as a batched pair (a second backward reuses them).  ratio = adjoint time
+    for both gradients / 2 / primal time."""
+    import numpy as np
+    import torch
+    import aero_hip
+    from aero_hip import CaseConfig, generate_mesh
+    from aero_hip.torch_ops import last_run, steady_coefficients
+
+    if not aero_hip.hip_available():
+        raise SystemExit("bench.py: no gfx950 device visible")
+    log("[adjoint] generating the mesh")
+    base = generate_mesh(precision=8, **SWEEP_MESH)
+    amp = np.random.default_rng(4242).uniform(-0.002, 0.002, 20)
+    cap = args.sweep_iters
+    case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.adjoint_cfl,
+                      cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
+                      stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
+                      residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
+    params = torch.tensor(np.concatenate([[case.aoa_deg, case.mach], amp]), requires_grad=True)
+    solve_kw = dict(primal="batch", rho_drop=1e-8, tol=args.adjoint_tol, max_iters=4 * cap, jacobian=True,
+                    cfl=args.adjoint_solve_cfl)
+    log("[adjoint] torch Function forward (primal)")
+    coef = steady_coefficients(params, base, case, solve_kw=solve_kw)
+    run = dict(last_run)
+    log(f"[adjoint] primal {run['primal_seconds']:.2f} s, {run['primal_iterations']} iterations; backward (CL, CD)")
+    (g_cl,) = torch.autograd.grad(coef[0], params, retain_graph=True)
+    t_adj = last_run["adjoint_seconds"]
+    adj_iters = last_run["adjoint_iterations"]
+    (g_cd,) = torch.autograd.grad(coef[1], params)
+    names = ["aoa", "mach"] + [f"shape{i}" for i in range(len(amp))]
+    return {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "ratio_per_gradient": round(t_adj / 2.0 / run["primal_seconds"], 3),
+            "unit": "adjoint/primal time per gradient", "primal_seconds": round(run["primal_seconds"], 3),
+            "primal_iterations": run["primal_iterations"], "primal_converged": bool(run["primal_converged"]),
+            "primal_rho_drop": run["primal_rho_ratio"], "adjoint_seconds_both": round(t_adj, 3),
+            "adjoint_iterations_max": adj_iters, "adjoint_tol": args.adjoint_tol,
+            "linear_solve_cfl": args.adjoint_solve_cfl, "cfl_max": args.adjoint_cfl,
+            "cl": float(coef[0]), "cd": float(coef[1]),
+            "dcl": dict(zip(names, [round(x, 10) for x in g_cl.tolist()])),
+            "dcd": dict(zip(names, [round(x, 10) for x in g_cd.tolist()]))}
 
 
 UNEVEN_MESHES = {  # nominal size: (circumferential, radial) of a jittered mixed O-grid, 30 quad wall layers
@@ -649,7 +727,14 @@ def main() -> int:
                     help="skip the other schemes / low-Mach lines timed beside the headline workload")
     ap.add_argument("--sweep", action="store_true", help="run the 64-case batched sweep instead")
     ap.add_argument("--sweep-iters", type=int, default=60000, help="iteration cap per case in --sweep")
-    ap.add_argument("--sweep-seq-cases", type=int, default=4, help="cases timed sequentially in --sweep")
+    ap.add_argument("--sweep-seq-cases", type=int, default=64,
+                    help="cases timed sequentially in --sweep (64: every case, no extrapolation)")
+    ap.add_argument("--no-sweep", dest="with_sweep", action="store_false",
+                    help="default mode: skip the batched-sweep object")
+    ap.add_argument("--no-adjoint", dest="with_adjoint", action="store_false",
+                    help="default mode: skip the adjoint object")
+    ap.add_argument("--default-seq-cases", type=int, default=4,
+                    help="default mode: sequential cases timed for the sweep object's baseline")
     ap.add_argument("--sweep-no-graph", action="store_true",
                     help="--sweep: launch the batch's kernels directly instead of replaying hipGraphs (for "
                          "rocprofv3 kernel statistics; slower)")
@@ -777,13 +862,23 @@ def main() -> int:
                                "flux_kernel_ms": round(v_flux, 5),
                                "flux_kernel_hbm_frac": round(flux_bytes / (v_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                "finite": bool(np.all(np.isfinite(vR))), "fallbacks": vdiag}
+    hm.close()
+    # The other two headline quantities, bounded so that the default run stays within a few minutes: the
+    # 64-case batched sweep (its sequential baseline on --default-seq-cases spread cases) and the adjoint /
+    # primal time of the config-4 gradients (no finite-difference checks).
+    sweep = adjoint = None
+    if args.with_sweep:
+        m = measure_sweep(args, args.default_seq_cases)
+        sweep = dict(m["sweep"], value=m["value"], unit=m["unit"], config=m["config"])
+        sweep.pop("cl"), sweep.pop("cd")
+    if args.with_adjoint:
+        adjoint = measure_adjoint_pair(args)
     cpu = cpu_ref = None
     if args.cpu_seconds > 0:
         log("[bench] cpu baseline")
         cpu = cpu_baseline(mesh, cfg_vals, U, args.cpu_seconds)
         log("[bench] reference cpu path")
         cpu_ref = cpu_reference(mesh, cfg_vals, U, args.cpu_seconds)
-    hm.close()
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcell-res/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -791,7 +886,7 @@ def main() -> int:
                       "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": args.block_cells,
                       "renumber": not args.no_renumber},
            "roofline": roofline, "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_host": host_cpu(),
-           "variants": variants,
+           "variants": variants, "sweep": sweep, "adjoint": adjoint,
            "detail": extra}
     print(json.dumps(out), flush=True)
     return 0

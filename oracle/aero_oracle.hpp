@@ -687,16 +687,26 @@ struct AssemblyDiag {
   int first_failure_face = -1;
 };
 
-// Limiter ratios of every interior face at a state (see reconstruct); boundary faces keep 0.
+// The frozen limiter's storage resolution (no reference counterpart): a captured ratio r in [0, 1] is kept
+// as the nearest multiple of 2^-15, ties to even (NaN -> 0), so it can be held as a 16-bit integer.
+inline double quantize_ratio(double r) {
+  const double c = (r < 0.0) ? 0.0 : (1.0 < r) ? 1.0 : r;
+  return (c == c ? std::nearbyint(c * 32768.0) : 0.0) * (1.0 / 32768.0);
+}
+
+// Limiter ratios of every interior face at a state (see reconstruct); boundary faces keep 0.  quantized:
+// held at the frozen limiter's 2^-15 resolution (its definition); false gives the exact ratios.
 template <class R>
 void limiter_ratios(const MeshView<R>& m, const std::vector<Prim<R>>& prim, const std::vector<R>& grad, int lim,
-                    std::vector<double>* out) {
+                    std::vector<double>* out, bool quantized = true) {
   out->assign(8 * static_cast<std::size_t>(m.num_faces), 0.0);
   for (int f = 0; f < m.num_faces; ++f) {
     if (m.neighbor[f] < 0) continue;
     Prim<R> L, Rt;
     reconstruct(m, prim, grad, f, lim, &L, &Rt, nullptr, out->data());
   }
+  if (quantized)
+    for (double& r : *out) r = quantize_ratio(r);
 }
 
 // One face of the assembly loop (residual_assembly.cpp:97-166): its flux F and wave speed w (before the

@@ -517,15 +517,23 @@ AO_EXPORT_ALL(f32, float)
 AO_EXPORT_ALL(f64, double)
 
 // ---- frozen limiter (FP64) ------------------------------------------------------------------------
-// Limiter ratios [8][F] at the state U (gradients of U, second order).
-extern "C" void ao_limiter_capture_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, double* ratios) {
+// Limiter ratios [8][F] at the state U (gradients of U, second order), at the frozen limiter's 2^-15
+// resolution; the _exact entry gives them unrounded (the pinning tests compare the two).
+static void limiter_capture(const ao_mesh* m, const ao_residual_cfg* c, const double* U, double* ratios, bool q) {
   const MeshView<double> v = view<double>(m);
   const AssemblyCfg<double> a = acfg<double>(c);
   const std::vector<Prim<double>> p = prims(v.num_cells, U, a.gamma);
   std::vector<double> gr, out;
   cell_gradients(v, p, &gr);
-  limiter_ratios(v, p, gr, a.limiter, &out);
+  limiter_ratios(v, p, gr, a.limiter, &out, q);
   std::memcpy(ratios, out.data(), out.size() * sizeof(double));
+}
+extern "C" void ao_limiter_capture_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U, double* ratios) {
+  limiter_capture(m, c, U, ratios, true);
+}
+extern "C" void ao_limiter_capture_exact_f64(const ao_mesh* m, const ao_residual_cfg* c, const double* U,
+                                             double* ratios) {
+  limiter_capture(m, c, U, ratios, false);
 }
 
 // Residual with frozen limiter ratios (second order).

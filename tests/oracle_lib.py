@@ -202,10 +202,12 @@ def face_scales(om: OracleMesh, cfg, U):
     return fs, cs
 
 
-def limiter_capture(om: OracleMesh, cfg, U):
+def limiter_capture(om: OracleMesh, cfg, U, exact: bool = False):
+    """Frozen limiter ratios [8, F] at U, at the frozen limiter's 2^-15 resolution (exact: unrounded)."""
     U = np.ascontiguousarray(U, np.float64).reshape(om.N, 5)
     out = np.zeros((8, om.F))
-    oracle().ao_limiter_capture_f64(C.byref(om.c), C.byref(cfg), ptr(U), ptr(out))
+    entry = "ao_limiter_capture_exact_f64" if exact else "ao_limiter_capture_f64"
+    getattr(oracle(), entry)(C.byref(om.c), C.byref(cfg), ptr(U), ptr(out))
     return out
 
 

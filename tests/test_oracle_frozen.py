@@ -1,6 +1,8 @@
 """Frozen limiter (an option beyond the reference) in the oracle: the captured ratios are the limiter's
-phi(a, b) / a, the frozen residual equals the limited one at the capture state, its tangent matches
-central differences, and a driver run with a freeze iteration follows the reference run up to it."""
+phi(a, b) / a held at 2^-15 resolution (the nearest multiple, ties to even), the frozen residual with the
+unrounded ratios equals the limited one at the capture state and with the held ratios differs from it
+by no more than the rounding of the increments allows, its tangent matches central differences, and a
+driver run with a freeze iteration follows the reference run up to it."""
 import numpy as np
 import pytest
 
@@ -23,7 +25,7 @@ def test_capture_and_frozen_residual(oracle_lib, limiter):
     m, om, ff, U = _setup()
     for scheme in (0, 1):
         cfg = helpers.res_cfg(cases.residual_cfg_values((scheme, limiter, 1, 1, 0, 0, 0), ff))
-        r = ol.limiter_capture(om, cfg, U)
+        r = ol.limiter_capture(om, cfg, U, exact=True)
         assert r.shape == (8, m.num_faces)
         assert np.all(r >= 0.0) and np.all(r <= 1.0 + 1e-15)
         assert np.all(r[:, m.neighbor < 0] == 0.0)
@@ -34,6 +36,16 @@ def test_capture_and_frozen_residual(oracle_lib, limiter):
         comp = [0, 1, 2, 4]
         assert (np.abs(Rf - R0)[:, comp] / (scale[:, comp] + 1e-300)).max() <= 1e-13
         assert np.array_equal(lf, l0)
+        # the held ratios: the nearest multiple of 2^-15 of the exact ones (numpy's rint rounds ties to even)
+        rq = ol.limiter_capture(om, cfg, U)
+        assert np.array_equal(rq, np.rint(np.clip(r, 0.0, 1.0) * 32768.0) / 32768.0)
+        assert np.all(np.abs(rq - r) <= 2.0 ** -16)
+        assert np.array_equal(rq * 32768.0, np.floor(rq * 32768.0))
+        # with the held ratios the residual moves by the rounding of the increments only: a few 1e-5 of the
+        # face-flux scale, nowhere near the limited-vs-unlimited difference
+        Rq, lq, _ = ol.residual_frozen(om, cfg, U, rq)
+        dq = (np.abs(Rq - R0)[:, comp] / (scale[:, comp] + 1e-300)).max()
+        assert 0.0 < dq <= 1e-4, dq
 
 
 def test_frozen_tangent_matches_central_difference(oracle_lib):
