@@ -16,7 +16,10 @@ import time
 
 import numpy as np
 
-from .solver import CaseConfig, HipMesh, Seed, adjoint_solve_batch, case_linearization
+from dataclasses import replace
+
+from .mesh import Mesh, deform_mesh
+from .solver import CaseConfig, HipBatch, HipMesh, Seed, adjoint_solve_batch, case_linearization
 
 
 def geometry_seed(tangents: dict, i: int) -> Seed:
@@ -113,6 +116,70 @@ def sensitivities_batch(items: list, objectives=(("cl", (1.0, 0.0, 0.0)), ("cd",
             _, r_p = hm.residual_jvp(rc, None, None, seed)
             for name, _ in objectives:
                 d[name][pname] = jp[name] - float(np.sum(psi[name] * r_p))
-        out.append(dict(J={n: o["J"] for n, o in obs.items()}, d=d, adjoint=info))
+        out.append(dict(J={n: o["J"] for n, o in obs.items()}, d=d, adjoint=info, psi=psi))
     return dict(cases=out, solve_seconds=t_solve, assembly_seconds=time.perf_counter() - t0)
 
+
+
+FD_STEPS = {"aoa": 1e-3, "mach": 1e-4, "shape": 1e-5}
+
+
+def fd_sensitivities_batch(base_mesh: Mesh, amplitudes, case: CaseConfig, U: np.ndarray, ratios: np.ndarray,
+                           params=None, steps=None, cfl: float | None = None, max_iters: int = 120000,
+                           rho_drop: float = 1e-7, block_cells: int = 256, shape_kw: dict | None = None,
+                           iters_per_sync: int = 500) -> dict:
+    """Central differences of the converged (CL, CD) of the frozen-limiter discretisation, every perturbed
+    solution continued from the converged state U in one batch (aero_hip_batch_set_state /
+    aero_hip_batch_set_limiter_ratios): for p in params ("aoa" [per degree], "mach", "shape<i>"), the cases
+    p + h and p - h with h from steps (FD_STEPS by default), all with the frozen ratios of U's run.
+
+    base_mesh / amplitudes: the undeformed mesh and the Hicks-Henne amplitudes of the converged case (None
+    amplitudes: the case's mesh is base_mesh itself, no shape parameters).  The continuation keeps the
+    converged run's final configuration (first-order startup, CFL, floor and all-speed ramps off; fixed
+    CFL `cfl`, default the case's cfl_max) and stops a case at a `rho_drop` density-residual drop relative
+    to its first iteration.  Returns {param: {"cl", "cd", "converged", "iterations", "h"}} and the batch
+    seconds."""
+    shape_kw = shape_kw or {}
+    amp = None if amplitudes is None else np.asarray(amplitudes, dtype=np.float64)
+    n_shape = 0 if amp is None else len(amp)
+    if params is None:
+        params = ["aoa", "mach"] + [f"shape{i}" for i in range(n_shape)]
+    steps = dict(FD_STEPS, **(steps or {}))
+    mesh0 = base_mesh if amp is None else deform_mesh(base_mesh, amp, bumps_per_surface=n_shape // 2,
+                                                      tangents=False, **shape_kw)[0]
+    handles = [HipMesh(mesh0, block_cells=block_cells)]
+    cases, hs = [], {}
+    for p in params:
+        kind = "shape" if p.startswith("shape") else p
+        h = steps[kind]
+        hs[p] = h
+        for sgn in (1.0, -1.0):
+            if kind == "aoa":
+                cases.append((0, case.mach, case.aoa_deg + sgn * h))
+            elif kind == "mach":
+                cases.append((0, case.mach + sgn * h, case.aoa_deg))
+            else:
+                e = np.zeros(n_shape)
+                e[int(p[5:])] = sgn * h
+                mi = deform_mesh(base_mesh, amp + e, bumps_per_surface=n_shape // 2, tangents=False, **shape_kw)[0]
+                handles.append(HipMesh(mi, block_cells=block_cells))
+                cases.append((len(handles) - 1, case.mach, case.aoa_deg))
+    cont = replace(case, startup_first_order_iters=0, cfl_ramp_iters=0, cfl_max=case.cfl_max if cfl is None else cfl,
+                   stab_ramp_iters=0, all_speed_ramp_iters=0, all_speed_ramp_start_iter=0,
+                   limiter_freeze_iter=max(case.limiter_freeze_iter, 0))
+    try:
+        with HipBatch(handles, cases, cont) as b:
+            for i in range(len(cases)):
+                b.set_state(i, U)
+                b.set_limiter_ratios(i, ratios)
+            r = b.run(max_iters, rho_drop, iters_per_sync=iters_per_sync)
+    finally:
+        for hm in handles:
+            hm.close()
+    out = {}
+    for k, p in enumerate(params):
+        a, m = r["cases"][2 * k], r["cases"][2 * k + 1]
+        h = hs[p]
+        out[p] = dict(cl=(a["cl"] - m["cl"]) / (2 * h), cd=(a["cd"] - m["cd"]) / (2 * h), h=h,
+                      converged=[a["converged"], m["converged"]], iterations=[a["iterations"], m["iterations"]])
+    return dict(d=out, seconds=r["seconds"])

@@ -521,6 +521,22 @@ class HipBatch:
                    "batch history")
         return out[: self.hist_capacity]
 
+    def set_state(self, i: int, U: np.ndarray):
+        """Warm start: case i's next iteration starts from U [N, 5] (aero_hip_batch_set_state)."""
+        g = self.meshes[self.cases[i][0]]
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(g.N, 5)
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_set_state(self.h, i, _pd(U), err, len(err)), err, "batch set_state")
+
+    def set_limiter_ratios(self, i: int, ratios: np.ndarray):
+        """Install case i's frozen limiter ratios [8, F]; the batch counts as frozen from its next iteration
+        (aero_hip_batch_set_limiter_ratios: install them for every case)."""
+        mesh = self.meshes[self.cases[i][0]]
+        r = np.ascontiguousarray(ratios, dtype=np.float64).reshape(8, mesh.F)
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_set_limiter_ratios(self.h, i, _pd(r), err, len(err)), err,
+                   "batch set_limiter_ratios")
+
     def limiter_ratios(self, i: int) -> np.ndarray:
         """Case i's frozen limiter ratios [8, F] (batches created with a freeze iteration)."""
         mesh = self.meshes[self.cases[i][0]]

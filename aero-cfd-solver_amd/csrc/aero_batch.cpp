@@ -559,6 +559,47 @@ int aero_hip_batch_history(aero_hip_batch* b, int32_t case_index, double* hist, 
   return AERO_OK;
 }
 
+int aero_hip_batch_set_state(aero_hip_batch* b, int32_t case_index, const double* conserved, char* err,
+                             size_t errlen) {
+  if (!b || case_index < 0 || case_index >= b->ncases || !conserved) {
+    set_err(err, errlen, "aero_hip_batch_set_state: bad argument");
+    return AERO_EINVAL;
+  }
+  const BatchCase& cs = b->h_cases[case_index];
+  aero_hip_mesh* m = b->geoms[cs.geom];
+  std::vector<double> soa;
+  aero_hostutil::to_internal_state(m, conserved, &soa);
+  int buf = 0;  // the buffer the case's next iteration starts from (see aero_hip_batch_get_state)
+  if (b->cfg.rk_stages == 1) {
+    CaseCtl k{};
+    AERO_TRY(hipMemcpy(&k, b->d_ctl + case_index, sizeof(CaseCtl), hipMemcpyDeviceToHost), "control readback");
+    buf = k.iter & 1;
+  }
+  AERO_TRY(hipMemcpy(cs.U[buf], soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice), "state copy");
+  return AERO_OK;
+}
+
+int aero_hip_batch_set_limiter_ratios(aero_hip_batch* b, int32_t case_index, const double* ratios, char* err,
+                                      size_t errlen) {
+  if (!b || case_index < 0 || case_index >= b->ncases || !ratios) {
+    set_err(err, errlen, "aero_hip_batch_set_limiter_ratios: bad argument");
+    return AERO_EINVAL;
+  }
+  const BatchCase& cs = b->h_cases[case_index];
+  if (!cs.lim_ratio) {
+    set_err(err, errlen, "aero_hip_batch_set_limiter_ratios: the batch has no freeze iteration");
+    return AERO_EINVAL;
+  }
+  aero_hip_mesh* m = b->geoms[cs.geom];
+  const size_t n = 8 * static_cast<size_t>(m->F);
+  AERO_TRY(hipMemcpyAsync(cs.lim_ratio, ratios, n * sizeof(double), hipMemcpyHostToDevice, b->stream),
+           "ratios copy");
+  AERO_TRY(aero_host::launch_ratio_slots(m->dm, cs.lim_ratio, cs.lim_slot, b->stream), "k_ratio_slots");
+  AERO_TRY(hipStreamSynchronize(b->stream), "aero_hip_batch_set_limiter_ratios: sync");
+  b->frozen = true;  // no capture at the freeze iteration from here on
+  return AERO_OK;
+}
+
 int aero_hip_batch_limiter_ratios(aero_hip_batch* b, int32_t case_index, double* ratios, char* err, size_t errlen) {
   if (!b || case_index < 0 || case_index >= b->ncases || !ratios) {
     set_err(err, errlen, "aero_hip_batch_limiter_ratios: bad argument");

@@ -318,6 +318,18 @@ int aero_hip_batch_history(aero_hip_batch* batch, int32_t case_index, double* hi
  * Fails when the batch was created without a freeze iteration. */
 int aero_hip_batch_limiter_ratios(aero_hip_batch* batch, int32_t case_index, double* ratios, char* err,
                                   size_t errlen);
+/* Warm starts for batched continuations (no reference counterpart), e.g. the perturbed solutions of
+ * finite-difference sensitivities started from a converged state: set_state makes conserved ([N][5] of the
+ * case's mesh, reference numbering) the state the case's next iteration starts from.  set_limiter_ratios
+ * installs frozen limiter ratios ([8][F], aero_hip_freeze_limiter's layout; held at 2^-15) for the case
+ * and marks the whole batch frozen from its next iteration, so no capture happens at limiter_freeze_iter:
+ * install them for every case.  It needs a batch created with limiter_freeze_iter >= 0.  A continuation
+ * batch typically sets startup_first_order_iters = 0, cfl_ramp_iters = 0 and stab_ramp_iters = 0 so that
+ * its first iteration already uses the converged run's final configuration. */
+int aero_hip_batch_set_state(aero_hip_batch* batch, int32_t case_index, const double* conserved, char* err,
+                             size_t errlen);
+int aero_hip_batch_set_limiter_ratios(aero_hip_batch* batch, int32_t case_index, const double* ratios, char* err,
+                                      size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
  * Discrete tangent and adjoint of the residual (no reference counterpart: the reference has no
