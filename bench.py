@@ -556,8 +556,10 @@ def run_adjoint(args) -> int:
     forward runs the primal as a one-case batch (device control, hipGraph) to an 8-decade density drop;
     the backward of CL solves the CL and CD adjoints together (one batched pair) and the backward of CD
     reuses them, giving d(CL, CD)/d(20 amplitudes, AoA, Mach).  value = adjoint time for both gradients /
-    2 / primal time.  Checks (on a second primal of the same case): the tangent solve and central
-    differences of converged solutions (same frozen ratios) for AoA, Mach and two amplitudes."""
+    2 / primal time.  Checks (at a second primal of the same case, the same state): central differences of
+    converged solutions for all 22 parameters and both objectives (the 44 perturbed solutions continued
+    from the converged state in one batch with the same frozen ratios, aero_hip.adjoint.
+    fd_sensitivities_batch) and tangent solves for AoA, Mach and two amplitudes."""
     from dataclasses import replace
     import numpy as np
     import aero_hip
@@ -607,47 +609,45 @@ def run_adjoint(args) -> int:
                          adjoint=dict(iterations=last_run["adjoint_iterations"], solves=last_run["adjoint_solves"]))
     del coef
     t_both = sum(t_adj.values())  # the first backward solves both adjoints (paired), the second reuses them
-    # checks on CL: tangent solves and central differences of converged solutions
+    # checks: central differences of converged solutions for all 22 parameters and both objectives (the 44
+    # perturbed solutions continued from the converged state in one batch, with the same frozen ratios),
+    # and tangent solves for four parameters (one solve serves CL and CD)
+    from aero_hip.adjoint import fd_sensitivities_batch
+    log("[adjoint] 44 finite-difference solutions in one batch")
+    fd = fd_sensitivities_batch(base, amp, done, U, ratios, cfl=args.fd_cfl, max_iters=args.fd_iters)
+    checks = {"fd_seconds": round(fd["seconds"], 3), "fd_iterations": fd["iterations"], "per_parameter": {}}
+    worst = {"cl": 0.0, "cd": 0.0}
+    for pname, v in fd["d"].items():
+        prow = {"fd_converged": v["converged"], "h": v["h"]}
+        for obj in ("cl", "cd"):
+            a = out[obj]["d"][pname]
+            rel = abs(a - v[obj]) / abs(v[obj])
+            worst[obj] = max(worst[obj], rel)
+            prow[obj] = dict(adjoint=a, fd=v[obj], rel_adjoint_fd=rel)
+        checks["per_parameter"][pname] = prow
+    checks["max_rel_adjoint_fd"] = worst
     rc, _, obj, _ = case_linearization(done)
-    checks = {}
-    for pname, h in (("aoa", 1e-3), ("mach", 1e-4), ("shape3", 1e-5), ("shape14", 1e-5)):
+    tworst = {"cl": 0.0, "cd": 0.0}
+    for pname in ("aoa", "mach", "shape3", "shape14"):
         log(f"[adjoint] tangent solve {pname}")
         if pname in ("aoa", "mach"):
             _, seed, _, dq = case_linearization(done, param=pname)
-            ob = hm.objective(replace_obj(obj, 1.0), U)
-            j_p = ob["dJ_daoa"] if pname == "aoa" else ob["dJ_dq"] * dq
+            seed_geo = None
         else:
-            i = int(pname[5:])
-            seed = geometry_seed(tan, i)
-            ob = hm.objective(replace_obj(obj, 1.0), U, seed=seed)
-            j_p = ob["dJ_seed"]
+            seed = seed_geo = geometry_seed(tan, int(pname[5:]))
         v, ti = hm.tangent_solve(rc, seed, U, cfl=args.adjoint_solve_cfl, max_iters=4 * cap, tol=args.adjoint_tol,
                                  check_every=500)
-        d_tan = j_p + float(np.sum(ob["dJdU"] * v))
-        cls = []
-        for sgn in (1, -1):
-            log(f"[adjoint] finite difference {pname} {sgn:+d}")
-            if pname in ("aoa", "mach"):
-                cp = replace(done, **{("aoa_deg" if pname == "aoa" else "mach"): getattr(
-                    done, "aoa_deg" if pname == "aoa" else "mach") + sgn * h})
-                rcp, _, objp, _ = case_linearization(cp)
-                r0, r1 = _frozen_continuation(hm, rcp, U, args.fd_iters, args.fd_cfl)
-                cls.append(hm.objective(replace_obj(objp, 1.0), None, gradient=False)["J"])
-                ok = r1 <= 1e-7 * r0
-            else:
-                e = np.zeros(20)
-                e[int(pname[5:])] = sgn * h
-                mi, _ = deform_mesh(base, amp + e, tangents=False)
-                with HipMesh(mi, block_cells=args.block_cells) as hmi:
-                    hmi.set_limiter_ratios(ratios)
-                    r0, r1 = _frozen_continuation(hmi, rc, U, args.fd_iters, args.fd_cfl)
-                    cls.append(hmi.objective(replace_obj(obj, 1.0), None, gradient=False)["J"])
-                ok = r1 <= 1e-7 * r0
-            checks.setdefault(pname, {})[f"fd_converged_{sgn:+d}"] = bool(ok)
-        fd = (cls[0] - cls[1]) / (2 * h)
-        d_adj = out["cl"]["d"][pname]
-        checks[pname].update(adjoint=d_adj, tangent=d_tan, fd=fd, rel_adjoint_fd=abs(d_adj - fd) / abs(fd),
-                             rel_adjoint_tangent=abs(d_adj - d_tan) / abs(d_adj), tangent_iterations=ti["iterations"])
+        prow = checks["per_parameter"][pname]
+        prow["tangent_iterations"] = ti["iterations"]
+        for name, w in (("cl", 1.0), ("cd", 0.0)):
+            ob = hm.objective(replace_obj(obj, w, 1.0 - w), U, seed=seed_geo)
+            j_p = ob["dJ_seed"] if seed_geo is not None else (ob["dJ_daoa"] if pname == "aoa" else ob["dJ_dq"] * dq)
+            d_tan = j_p + float(np.sum(ob["dJdU"] * v))
+            a = out[name]["d"][pname]
+            rel = abs(a - d_tan) / abs(a)
+            tworst[name] = max(tworst[name], rel)
+            prow[name].update(tangent=d_tan, rel_adjoint_tangent=rel)
+    checks["max_rel_adjoint_tangent"] = tworst
     hm.close()
     ratio = t_both / 2.0 / t_primal  # per gradient: dCL and dCD come from one paired solve
     res = {"metric": METRIC, "value": round(ratio, 3), "unit": "adjoint/primal time per gradient", "n_gpus": 1,
@@ -668,7 +668,7 @@ def run_adjoint(args) -> int:
                        "adjoint_seconds": {k: round(v, 3) for k, v in t_adj.items()},
                        "adjoint_iterations_max": out["cl"]["adjoint"]["iterations"],
                        "cl": out["cl"]["J"], "cd": out["cd"]["J"],
-                       "dcl": out["cl"]["d"], "dcd": out["cd"]["d"], "checks_cl": checks}}
+                       "dcl": out["cl"]["d"], "dcd": out["cd"]["d"], "checks": checks}}
     print(json.dumps(res), flush=True)
     return 0
 
@@ -759,7 +759,8 @@ def main() -> int:
     ap.add_argument("--adjoint-cfl", type=float, default=2.0,
                     help="--adjoint: CFL of the primal and the adjoint (at 2.4 the AoA / Mach finite-difference "
                          "continuations no longer converge to 1e-7)")
-    ap.add_argument("--fd-iters", type=int, default=120000, help="--adjoint: continuation iterations per FD run")
+    ap.add_argument("--fd-iters", type=int, default=200000,
+                    help="--adjoint: iteration cap of the batched finite-difference continuation")
     ap.add_argument("--fd-cfl", type=float, default=2.0,
                     help="--adjoint: fixed CFL of the FD continuation runs (they have no CFL adaptation)")
     ap.add_argument("--vjp", choices=sorted(UNEVEN_MESHES), default=None,

@@ -4,7 +4,8 @@ IDW), M 0.6, AoA 2 deg, HLLC + venkat with the limiter frozen at iteration 1000.
 (AoA, Mach, the 20 amplitudes) and both CL and CD, the batched adjoint gradient must agree to 1e-6 with:
 
 * central differences of converged solutions (aero_hip.adjoint.fd_sensitivities_batch: every perturbed
-  case continued from the converged state in one batch, with the same frozen ratios);
+  case continued from the converged state in one batch, with the same frozen ratios, until no CL or CD
+  moves by more than 1e-14 over 2000 iterations; steps 1e-3 deg, 1e-4 and 2.5e-6 chords);
 * the CPU tangent evaluation: the FP64 oracle's dual-number residual derivative R_p along each
   parameter's direction and its force integration (central differences of the wall forces, exact for
   these bilinear sums up to round-off), assembled with the device adjoint state as J_p - psi . R_p.
@@ -81,8 +82,7 @@ def test_adjoint_matches_central_differences(workload):
     from aero_hip.adjoint import fd_sensitivities_batch
     w = workload
     say("[config4] 44 perturbed solutions in one batch")
-    fd = fd_sensitivities_batch(w["base"], w["amp"], w["case"], w["U"], w["ratios"], cfl=2.0, rho_drop=1e-8,
-                                max_iters=200000)
+    fd = fd_sensitivities_batch(w["base"], w["amp"], w["case"], w["U"], w["ratios"], cfl=2.0, max_iters=200000)
     for p, v in fd["d"].items():
         assert all(v["converged"]), (p, v)
     ref = {obj: {p: v[obj] for p, v in fd["d"].items()} for obj in ("cl", "cd")}
