@@ -341,14 +341,14 @@ def measure_adjoint_pair(args) -> dict:
     log("[adjoint] generating the mesh")
     base = generate_mesh(precision=8, **SWEEP_MESH)
     amp = np.random.default_rng(4242).uniform(-0.002, 0.002, 20)
-    cap = args.sweep_iters
+    cap = args.adjoint_iters
     case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.adjoint_cfl,
                       cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
                       stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
                       residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
     params = torch.tensor(np.concatenate([[case.aoa_deg, case.mach], amp]), requires_grad=True)
-    solve_kw = dict(primal="batch", rho_drop=1e-8, tol=args.adjoint_tol, max_iters=4 * cap, jacobian=True,
-                    cfl=args.adjoint_solve_cfl)
+    solve_kw = dict(primal="batch", rho_drop=args.adjoint_primal_drop, tol=args.adjoint_tol, max_iters=4 * cap,
+                    jacobian=True, cfl=args.adjoint_solve_cfl)
     log("[adjoint] torch Function forward (primal)")
     coef = steady_coefficients(params, base, case, solve_kw=solve_kw)
     run = dict(last_run)
@@ -358,13 +358,14 @@ def measure_adjoint_pair(args) -> dict:
     adj_iters = last_run["adjoint_iterations"]
     (g_cd,) = torch.autograd.grad(coef[1], params)
     names = ["aoa", "mach"] + [f"shape{i}" for i in range(len(amp))]
-    return {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "ratio_per_gradient": round(t_adj / 2.0 / run["primal_seconds"], 3),
-            "unit": "adjoint/primal time per gradient", "primal_seconds": round(run["primal_seconds"], 3),
+    ratio = t_adj / 2.0 / run["primal_seconds"]
+    return {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "ratio_per_gradient": round(ratio, 3),
+            "primal_rho_drop_target": args.adjoint_primal_drop, "unit": "adjoint/primal time per gradient", "primal_seconds": round(run["primal_seconds"], 3),
             "primal_iterations": run["primal_iterations"], "primal_converged": bool(run["primal_converged"]),
             "primal_rho_drop": run["primal_rho_ratio"], "adjoint_seconds_both": round(t_adj, 3),
             "adjoint_iterations_max": adj_iters, "adjoint_tol": args.adjoint_tol,
             "linear_solve_cfl": args.adjoint_solve_cfl, "cfl_max": args.adjoint_cfl,
-            "cl": float(coef[0]), "cd": float(coef[1]),
+            "cl": float(coef[0].detach()), "cd": float(coef[1].detach()),
             "dcl": dict(zip(names, [round(x, 10) for x in g_cl.tolist()])),
             "dcd": dict(zip(names, [round(x, 10) for x in g_cd.tolist()]))}
 
@@ -574,15 +575,15 @@ def run_adjoint(args) -> int:
     base = generate_mesh(precision=8, **mkw)
     amp = np.random.default_rng(4242).uniform(-0.002, 0.002, 20)
     mesh, tan = deform_mesh(base, amp)
-    cap = args.sweep_iters
+    cap = args.adjoint_iters
     case = CaseConfig(mach=0.6, aoa_deg=2.0, flux_scheme=1, limiter=1, cfl_start=0.2, cfl_max=args.adjoint_cfl,
                       cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
                       stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
                       residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
     hm = HipMesh(mesh, block_cells=args.block_cells)
-    log(f"[adjoint] primal: {mesh.num_cells} cells to an 8-decade density drop (cap {cap})")
+    log(f"[adjoint] primal: {mesh.num_cells} cells to a {args.adjoint_primal_drop:.0e} density drop (cap {cap})")
     with HipBatch([hm], [(0, case.mach, case.aoa_deg)], case) as b:
-        r = b.run(cap, 1e-8, iters_per_sync=500, use_graph=True)
+        r = b.run(cap, args.adjoint_primal_drop, iters_per_sync=500, use_graph=True)
         U = b.state(0)
         ratios = b.limiter_ratios(0)
         hm.set_limiter_ratios(ratios)
@@ -594,8 +595,8 @@ def run_adjoint(args) -> int:
     import torch
     from aero_hip.torch_ops import last_run, steady_coefficients
     params = torch.tensor(np.concatenate([[case.aoa_deg, case.mach], amp]), requires_grad=True)
-    solve_kw = dict(primal="batch", rho_drop=1e-8, tol=args.adjoint_tol, max_iters=4 * cap, jacobian=True,
-                    cfl=args.adjoint_solve_cfl)
+    solve_kw = dict(primal="batch", rho_drop=args.adjoint_primal_drop, tol=args.adjoint_tol, max_iters=4 * cap,
+                    jacobian=True, cfl=args.adjoint_solve_cfl)
     log("[adjoint] torch Function forward (primal)")
     coef = steady_coefficients(params, base, case, solve_kw=solve_kw)
     t_primal, t_iters = last_run["primal_seconds"], last_run["primal_iterations"]
@@ -657,7 +658,7 @@ def run_adjoint(args) -> int:
            "config": {"workload": "adjoint_mixed66k_hh20_hllc_venkat", "cells": mesh.num_cells, "mesh": mkw,
                       "bumps": 20, "amplitude_range": 0.002, "seed": 4242, "mach": 0.6, "aoa_deg": 2.0,
                       "cfl_max": args.adjoint_cfl, "linear_solve_cfl": args.adjoint_solve_cfl,
-                      "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": 1e-8,
+                      "limiter_freeze_iter": args.freeze_iter, "rho_drop_target": args.adjoint_primal_drop,
                       "adjoint_tol": args.adjoint_tol},
            "adjoint": {"gradients": "torch.autograd.grad through aero_hip.torch_ops.steady_coefficients",
                        "primal_seconds": round(t_primal, 3), "primal_iterations": t_iters,
@@ -752,7 +753,12 @@ def main() -> int:
     ap.add_argument("--uneven-seed", type=int, default=5150, help="--uneven: seed of shapes, Mach and AoA")
     ap.add_argument("--uneven-lowmach", choices=["precond", "plain"], default="precond",
                     help="--uneven: M 0.05 cases with the low-Mach preconditioner (its long runs diverge) or without")
-    ap.add_argument("--adjoint-tol", type=float, default=1e-8, help="--adjoint: linear residual drop")
+    ap.add_argument("--adjoint-tol", type=float, default=1e-11,
+                    help="--adjoint: linear residual drop of the adjoint and tangent solves")
+    ap.add_argument("--adjoint-iters", type=int, default=150000, help="--adjoint: iteration cap of the primal")
+    ap.add_argument("--adjoint-primal-drop", type=float, default=1e-10,
+                    help="--adjoint: density-residual drop of the primal the gradients are taken at (1e-6 "
+                         "gradients need a primal converged beyond 8 decades)")
     ap.add_argument("--adjoint-solve-cfl", type=float, default=2.4,
                     help="--adjoint: CFL of the linear (adjoint / tangent) pseudo-time solves; with the limiter "
                          "frozen the linearised iteration on this case converges at 2.5 and diverges from 3.0")
