@@ -34,7 +34,7 @@ class MeshDesc(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("renumber", i32), ("block_cells", i32), ("gradient", i32)]
+    _fields_ = [("renumber", i32), ("block_cells", i32), ("gradient", i32), ("accumulation", i32)]
 
 
 class ResidualConfig(C.Structure):
@@ -194,6 +194,7 @@ PROTOTYPES = {
                                            C.c_size_t]),
     "aero_hip_batch_limiter_ratios": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
     "aero_hip_batch_set_state": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
+    "aero_hip_layout_probe": (C.c_int, [vp, PD]),
     "aero_hip_batch_set_limiter_ratios": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
     "aero_case_linearization": (C.c_int, [C.POINTER(CaseConfigC), i32, i32, C.POINTER(ResidualConfig),
                                           C.POINTER(TangentSeed), C.POINTER(Objective), PD, C.c_char_p, C.c_size_t]),

@@ -135,15 +135,23 @@ class HipMesh:
 
     GRADIENTS = {"green_gauss": 0, "least_squares": 1}
 
-    def __init__(self, mesh: Mesh, renumber: bool = True, block_cells: int = 256, gradient: str = "green_gauss"):
+    ACCUMULATIONS = {"auto": 0, "gather": 1, "colour": 2}
+
+    def __init__(self, mesh: Mesh, renumber: bool = True, block_cells: int = 256, gradient: str = "green_gauss",
+                 accumulation: str = "auto"):
+        """block_cells 128, 256 or 512, or 0 for the layout probe (aero_hip_options); accumulation "gather",
+        "colour" or "auto" (gather, or the probe's choice with block_cells = 0)."""
         self.lib = _lib.lib()
         self.mesh = mesh
         self.N, self.F = mesh.num_cells, mesh.num_faces
         self._desc = mesh.desc()
         if gradient not in self.GRADIENTS:
             raise ValueError(f"gradient must be one of {sorted(self.GRADIENTS)}")
+        if accumulation not in self.ACCUMULATIONS:
+            raise ValueError(f"accumulation must be one of {sorted(self.ACCUMULATIONS)}")
         self.gradient = gradient
-        opts = _lib.Options(1 if renumber else 0, block_cells, self.GRADIENTS[gradient])
+        opts = _lib.Options(1 if renumber else 0, block_cells, self.GRADIENTS[gradient],
+                            self.ACCUMULATIONS[accumulation])
         h = C.c_void_p()
         err = _lib.errbuf()
         _lib.check(self.lib.aero_hip_mesh_create(C.byref(self._desc), C.byref(opts), C.byref(h), err, len(err)),
@@ -233,7 +241,16 @@ class HipMesh:
     def layout(self) -> dict:
         a = (C.c_int64 * 8)()
         self.lib.aero_hip_layout(self.h, a)
-        return dict(cells=a[0], faces=a[1], blocks=a[2], block_cells=a[3], slots=a[4], max_slots=a[5], grid=a[7])
+        acc = {1: "gather", 2: "colour"}.get(a[6], "gather")
+        return dict(cells=a[0], faces=a[1], blocks=a[2], block_cells=a[3], slots=a[4], max_slots=a[5], grid=a[7],
+                    accumulation=acc)
+
+    def layout_probe(self) -> dict:
+        """The layout probe's ms per residual evaluation (block_cells = 0), None where not probed."""
+        ms = np.zeros(4)
+        self.lib.aero_hip_layout_probe(self.h, _pd(ms))
+        keys = ("gather128", "gather256", "gather512", "colour")
+        return {k: (round(float(v), 5) if v >= 0 else None) for k, v in zip(keys, ms)}
 
     def stream(self) -> int:
         return self.lib.aero_hip_stream(self.h) or 0

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -248,6 +249,208 @@ void from_internal4(const aero_hip_mesh* m, const std::vector<double>& soa, doub
 
 }  // namespace aero_hostutil
 
+namespace {
+
+int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, int accum, aero_hip_mesh** out,
+                       char* err, size_t errlen);
+
+// The face-coloured layout (AERO_ACCUM_COLOUR): faces in internal-owner order coloured greedily so that no
+// two faces of a colour share a cell, then grouped by colour (owner order kept inside a colour).
+int build_colour_layout(aero_hip_mesh* m, const aero_mesh_desc* d, char* err, size_t errlen) {
+  const int N = m->N, F = m->F;
+  std::vector<int> order(F);
+  for (int f = 0; f < F; ++f) order[f] = f;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return m->iperm[d->face_owner[a]] < m->iperm[d->face_owner[b]]; });
+  std::vector<uint32_t> used(N, 0u);
+  std::vector<int> colour(F, 0);
+  int nc = 0;
+  for (int f : order) {
+    const int o = m->iperm[d->face_owner[f]];
+    const int n = d->face_neighbor[f] >= 0 ? m->iperm[d->face_neighbor[f]] : -1;
+    const uint32_t taken = used[o] | (n >= 0 ? used[n] : 0u);
+    if (taken == 0xffffffffu) {
+      set_err(err, errlen, "aero_hip_mesh_create: a cell has more than 32 faces (face colouring)");
+      return AERO_EINVAL;
+    }
+    const int c = __builtin_ctz(~taken);
+    colour[f] = c;
+    used[o] |= 1u << c;
+    if (n >= 0) used[n] |= 1u << c;
+    nc = std::max(nc, c + 1);
+  }
+  m->colour_off.assign(nc + 1, 0);
+  for (int f = 0; f < F; ++f) ++m->colour_off[colour[f] + 1];
+  for (int c = 0; c < nc; ++c) m->colour_off[c + 1] += m->colour_off[c];
+  std::vector<int> fill(m->colour_off.begin(), m->colour_off.end() - 1), co(F), cn(F), cmeta(F), cface(F);
+  std::vector<double> nx(F), ny(F), A(F), fx(F), fy(F);
+  for (int f : order) {
+    const int i = fill[colour[f]]++;
+    co[i] = m->iperm[d->face_owner[f]];
+    cn[i] = d->face_neighbor[f] >= 0 ? m->iperm[d->face_neighbor[f]] : -1;
+    cmeta[i] = m->h_bc[f];
+    cface[i] = f;
+    nx[i] = m->h_nx[f];
+    ny[i] = m->h_ny[f];
+    A[i] = d->face_area[f];
+    fx[i] = m->h_fx[f];
+    fy[i] = m->h_fy[f];
+  }
+  aero_dev::ColourFaces& cf = m->cfaces;
+  int *p_o, *p_n, *p_meta, *p_face;
+  double *p_nx, *p_ny, *p_A, *p_fx, *p_fy;
+  for (const hipError_t e : {upload(m, &p_o, co), upload(m, &p_n, cn), upload(m, &p_meta, cmeta),
+                             upload(m, &p_face, cface), upload(m, &p_nx, nx), upload(m, &p_ny, ny), upload(m, &p_A, A),
+                             upload(m, &p_fx, fx), upload(m, &p_fy, fy)})
+    if (e != hipSuccess) {
+      set_err(err, errlen, std::string("aero_hip_mesh_create: colour layout: ") + hipGetErrorString(e));
+      return AERO_EHIP;
+    }
+  cf = aero_dev::ColourFaces{p_o, p_n, p_meta, p_face, p_nx, p_ny, p_A, p_fx, p_fy};
+  m->ncolours = nc;
+  m->colour = true;
+  return AERO_OK;
+}
+
+// The layout probe behind block_cells = 0: the residual (HLLC + venkat, second order, on a smooth
+// synthetic state) timed with HIP events in each candidate layout -- gather with 128, 256 and 512 cells per
+// block, and the face-coloured layout -- and the fastest kept, per mesh (keyed by a hash of its arrays and
+// options) for the life of the process.
+struct Tuned {
+  int BC = 256, accum = AERO_ACCUM_GATHER;
+  double ms[4] = {-1.0, -1.0, -1.0, -1.0};  // gather 128, 256, 512, colour (-1: not probed)
+};
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
+uint64_t mesh_fingerprint(const aero_mesh_desc* d, const aero_hip_options* opts, int accum) {
+  uint64_t h = 1469598103934665603ull;
+  const int32_t head[7] = {d->num_cells, d->num_faces, d->wall_start, d->wall_count, opts->renumber, opts->gradient,
+                           accum};
+  h = fnv1a(h, head, sizeof(head));
+  const size_t F = static_cast<size_t>(d->num_faces), N = static_cast<size_t>(d->num_cells);
+  h = fnv1a(h, d->face_owner, F * sizeof(int32_t));
+  h = fnv1a(h, d->face_neighbor, F * sizeof(int32_t));
+  if (d->face_bc) h = fnv1a(h, d->face_bc, F * sizeof(int32_t));
+  h = fnv1a(h, d->face_normal, 3 * F * sizeof(double));
+  h = fnv1a(h, d->face_center, 3 * F * sizeof(double));
+  h = fnv1a(h, d->face_area, F * sizeof(double));
+  h = fnv1a(h, d->cell_volume, N * sizeof(double));
+  h = fnv1a(h, d->cell_center, 3 * N * sizeof(double));
+  return h;
+}
+
+std::mutex g_tune_mu;
+std::map<uint64_t, Tuned> g_tuned;
+
+// Mean ms per residual evaluation of handle m (2 untimed, then 6 timed evaluations).
+int probe_residual(aero_hip_mesh* m, double* ms, char* err, size_t errlen) {
+  const int N = m->N;
+  const double g = 1.4, p0 = 101325.0, r0 = p0 / (287.05 * 288.15), a0 = std::sqrt(g * p0 / r0), V = 0.5 * a0;
+  const double al = 2.0 * 3.14159265358979323846 / 180.0, u0 = V * std::cos(al), v0 = V * std::sin(al);
+  std::vector<double> U(static_cast<size_t>(N) * 5);
+  for (int c = 0; c < N; ++c) {
+    const double x = m->h_ccx[c], y = m->h_ccy[c];
+    const double s = std::sin(2.0 * 3.14159265358979323846 * x) * std::cos(2.0 * 3.14159265358979323846 * y);
+    const double r = r0 * (1.0 + 0.02 * s), u = u0 + 0.02 * V * s, v = v0 - 0.02 * V * s, p = p0 * (1.0 + 0.02 * s);
+    double* q = &U[5 * static_cast<size_t>(c)];
+    q[0] = r;
+    q[1] = r * u;
+    q[2] = r * v;
+    q[3] = 0.0;
+    q[4] = p / (g - 1.0) + 0.5 * r * (u * u + v * v);
+  }
+  if (const int rc = aero_hip_set_state(m, U.data(), err, errlen)) return rc;
+  aero_residual_config c{};
+  c.farfield[0] = r0;
+  c.farfield[1] = r0 * u0;
+  c.farfield[2] = r0 * v0;
+  c.farfield[4] = p0 / (g - 1.0) + 0.5 * r0 * V * V;
+  c.gamma = g;
+  c.flux_scheme = AERO_FLUX_HLLC;
+  c.limiter = AERO_LIMITER_VENKAT;
+  c.second_order = 1;
+  c.low_mach_cutoff = 0.3;
+  c.stab_mach_floor = 0.02;
+  c.precond_mach_ref = 0.15;
+  c.precond_mach_min = 1e-3;
+  c.precond_beta_min = 0.2;
+  c.precond_beta_max = 1.0;
+  c.all_speed_cutoff = 0.25;
+  c.all_speed_f_min = 0.05;
+  c.all_speed_ramp = 1.0;
+  for (int i = 0; i < 2; ++i)
+    if (const int rc = aero_hip_residual_async(m, &c, err, errlen)) return rc;
+  hipEvent_t e0, e1;
+  AERO_HIP_TRY(hipEventCreate(&e0), "layout probe: hipEventCreate");
+  AERO_HIP_TRY(hipEventCreate(&e1), "layout probe: hipEventCreate");
+  int rc = AERO_OK;
+  const int reps = 6;
+  if (hipEventRecord(e0, m->stream) != hipSuccess) rc = AERO_EHIP;
+  for (int i = 0; i < reps && !rc; ++i) rc = aero_hip_residual_async(m, &c, err, errlen);
+  float t = 0.0f;
+  if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&t, e0, e1) != hipSuccess)) {
+    set_err(err, errlen, "layout probe: event timing failed");
+    rc = AERO_EHIP;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = static_cast<double>(t) / reps;
+  return rc;
+}
+
+int tuned_layout(const aero_mesh_desc* d, const aero_hip_options* opts, int accum, Tuned* out, char* err,
+                 size_t errlen) {
+  const uint64_t key = mesh_fingerprint(d, opts, accum);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) {
+    *out = it->second;
+    return AERO_OK;
+  }
+  Tuned t;
+  double best = 1e300;
+  const int bcs[3] = {128, 256, 512};
+  for (int k = 0; k < 4; ++k) {
+    const int mode = k < 3 ? AERO_ACCUM_GATHER : AERO_ACCUM_COLOUR;
+    if (accum != AERO_ACCUM_AUTO && accum != mode) continue;
+    aero_hip_options o = *opts;
+    o.block_cells = k < 3 ? bcs[k] : 256;
+    aero_hip_mesh* m = nullptr;
+    char why[512] = {0};
+    int rc = mesh_create_layout(d, &o, mode, &m, why, sizeof(why));
+    if (rc == AERO_EINVAL && k < 3) continue;  // a block too large for LDS at this size: not a candidate
+    if (rc) {
+      set_err(err, errlen, std::string("layout probe: ") + why);
+      return rc;
+    }
+    double ms = 0.0;
+    rc = probe_residual(m, &ms, err, errlen);
+    aero_hip_mesh_destroy(m);
+    if (rc) return rc;
+    t.ms[k] = ms;
+    if (ms < best) {
+      best = ms;
+      t.BC = o.block_cells;
+      t.accum = mode;
+    }
+  }
+  if (best >= 1e300) {
+    set_err(err, errlen, "layout probe: no candidate layout could be built");
+    return AERO_EINVAL;
+  }
+  g_tuned[key] = t;
+  *out = t;
+  return AERO_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int aero_hip_available(char* err, size_t errlen) {
@@ -269,8 +472,14 @@ int aero_hip_available(char* err, size_t errlen) {
   return 1;
 }
 
-int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, aero_hip_mesh** out, char* err,
-                         size_t errlen) {
+}  // extern "C"
+
+namespace {
+
+// A mesh handle in one fixed layout: opts->block_cells is 128, 256 or 512; accum AERO_ACCUM_GATHER or
+// AERO_ACCUM_COLOUR (the residual evaluations of the handle then use the face-coloured layout).
+int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, int accum, aero_hip_mesh** out,
+                              char* err, size_t errlen) {
   if (!d || !out) {
     set_err(err, errlen, "aero_hip_mesh_create: null argument");
     return AERO_EINVAL;
@@ -293,9 +502,9 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
     set_err(err, errlen, "aero_hip_mesh_create: wall patch range out of bounds");
     return AERO_EINVAL;
   }
-  int BC = opts && opts->block_cells > 0 ? opts->block_cells : 256;
+  const int BC = opts && opts->block_cells > 0 ? opts->block_cells : 256;
   if (BC != 128 && BC != 256 && BC != 512) {
-    set_err(err, errlen, "aero_hip_mesh_create: block_cells must be 128, 256 or 512");
+    set_err(err, errlen, "aero_hip_mesh_create: block_cells must be 0, 128, 256 or 512");
     return AERO_EINVAL;
   }
   const bool renumber = opts ? opts->renumber != 0 : true;
@@ -542,7 +751,51 @@ int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, 
   m->wall.fx = p_wfx;
   m->wall.fy = p_wfy;
   m->cur = m->base = 0;
+  if (accum == AERO_ACCUM_COLOUR) {
+    const int rc = build_colour_layout(m, d, err, errlen);
+    if (rc) {
+      free_all(m);
+      delete m;
+      return rc;
+    }
+  }
   *out = m;
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_hip_mesh_create(const aero_mesh_desc* d, const aero_hip_options* opts, aero_hip_mesh** out, char* err,
+                         size_t errlen) {
+  if (!d || !out) {
+    set_err(err, errlen, "aero_hip_mesh_create: null argument");
+    return AERO_EINVAL;
+  }
+  const int accum = opts ? opts->accumulation : AERO_ACCUM_AUTO;
+  if (accum != AERO_ACCUM_AUTO && accum != AERO_ACCUM_GATHER && accum != AERO_ACCUM_COLOUR) {
+    set_err(err, errlen, "aero_hip_mesh_create: accumulation must be AERO_ACCUM_AUTO, _GATHER or _COLOUR");
+    return AERO_EINVAL;
+  }
+  if (!opts || opts->block_cells != 0)
+    return mesh_create_layout(d, opts, accum == AERO_ACCUM_COLOUR ? AERO_ACCUM_COLOUR : AERO_ACCUM_GATHER, out, err,
+                              errlen);
+  // block_cells = 0: the layout probe picks block size and accumulation for this mesh (cached per mesh)
+  Tuned t;
+  int rc = tuned_layout(d, opts, accum, &t, err, errlen);
+  if (rc) return rc;
+  aero_hip_options o = *opts;
+  o.block_cells = t.BC;
+  rc = mesh_create_layout(d, &o, t.accum, out, err, errlen);
+  if (rc) return rc;
+  std::memcpy((*out)->probe_ms, t.ms, sizeof(t.ms));
+  return AERO_OK;
+}
+
+int aero_hip_layout_probe(aero_hip_mesh* m, double* ms4) {
+  if (!m || !ms4) return AERO_EINVAL;
+  std::memcpy(ms4, m->probe_ms, sizeof(m->probe_ms));
   return AERO_OK;
 }
 
@@ -590,9 +843,15 @@ int aero_hip_residual_async(aero_hip_mesh* m, const aero_residual_config* cfg, c
     AERO_HIP_TRY(aero_host::launch_gradient(m->dm, U, m->G, o.g, m->stream, m->diag), "k_gradient");
   if (timed) AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 1], m->stream), "hipEventRecord");
   StageArgs st{};
-  AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 0, U, m->G,
-                                      m->R, m->lam, m->diag, st, m->stream),
-               "k_flux");
+  if (m->colour)
+    AERO_HIP_TRY(aero_host::launch_flux_colour(m->dm, m->cfaces, m->colour_off.data(), m->ncolours, o,
+                                               cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, U, m->G, m->R,
+                                               m->lam, m->diag, m->stream),
+                 "k_flux_colour");
+  else
+    AERO_HIP_TRY(aero_host::launch_flux(m->dm, o, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, 0, U, m->G,
+                                        m->R, m->lam, m->diag, st, m->stream),
+                 "k_flux");
   if (timed) {
     AERO_HIP_TRY(hipEventRecord(m->tev[3 * m->tcount + 2], m->stream), "hipEventRecord");
     ++m->tcount;
@@ -650,7 +909,7 @@ int aero_hip_layout(aero_hip_mesh* m, int64_t* out8) {
   out8[3] = m->BC;
   out8[4] = m->S;
   out8[5] = m->dm.smax;
-  out8[6] = 0;
+  out8[6] = m->colour ? AERO_ACCUM_COLOUR : AERO_ACCUM_GATHER;
   out8[7] = m->dm.grid;
   return AERO_OK;
 }

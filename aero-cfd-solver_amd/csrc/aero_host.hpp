@@ -42,6 +42,12 @@ struct aero_hip_mesh {
   double* lim_ratio = nullptr;  // frozen limiter ratios [8][F] (aero_hip_freeze_limiter)
   unsigned short* lim_slot = nullptr;  // the same per face slot [S][8] in units of 2^-15 (flux kernels)
   bool lim_valid = false;
+  // face-coloured layout of the residual evaluations (AERO_ACCUM_COLOUR)
+  bool colour = false;
+  int ncolours = 0;
+  std::vector<int> colour_off;  // [ncolours + 1] offsets into cfaces
+  aero_dev::ColourFaces cfaces{};
+  double probe_ms[4] = {-1.0, -1.0, -1.0, -1.0};  // layout probe (block_cells = 0): gather 128/256/512, colour
 };
 
 namespace aero_hostutil {

@@ -48,6 +48,23 @@ struct DevMesh {
   const double* ccy;
 };
 
+// Face-coloured layout (the alternative to the per-block gather that the layout probe can choose for
+// residual evaluations): faces grouped by colour so that no two faces of one colour share a cell, and
+// within a colour ordered by owner cell.  One launch per colour adds each face's F A to its owner and
+// subtracts it from its neighbour directly, without floating-point atomics; a cell's contributions then
+// arrive in colour order instead of reference face order (the same sums to rounding).
+struct ColourFaces {
+  const int* o;       // [F] owner (internal id), colour-major
+  const int* n;       // [F] neighbour (internal id) or -1
+  const int* meta;    // [F] bc code
+  const int* face;    // [F] reference face index
+  const double* nx;   // [F]
+  const double* ny;
+  const double* A;
+  const double* fx;   // face centre
+  const double* fy;
+};
+
 struct StageArgs {
   int stage;
   int last_stage;
@@ -213,6 +230,11 @@ hipError_t launch_gradient(const aero_dev::DevMesh& m, const double* U, double* 
 hipError_t launch_flux(const aero_dev::DevMesh& m, const aero_dev::FaceOpts& o, int scheme, int limiter, bool second,
                        int mode, const double* U, const double* G, double* R, double* lam, int* diag,
                        const aero_dev::StageArgs& st, hipStream_t s);
+// The residual R, lam of U on the face-coloured layout (colour_off: [ncolours + 1] host offsets into cf):
+// zeroes R and lam, then one launch per colour.  G: the gradients (second order).
+hipError_t launch_flux_colour(const aero_dev::DevMesh& m, const aero_dev::ColourFaces& cf, const int* colour_off,
+                              int ncolours, const aero_dev::FaceOpts& o, int scheme, int limiter, bool second,
+                              const double* U, const double* G, double* R, double* lam, int* diag, hipStream_t s);
 // Caller device arrays in the reference layout ([N][5], reference numbering; perm: internal -> reference)
 // to / from the internal SoA [4][N]; lam (internal order) goes to lam_ref (reference order) when both are set.
 hipError_t launch_aos_to_internal(int N, const int* perm, const double* aos5, double* soa, hipStream_t s);

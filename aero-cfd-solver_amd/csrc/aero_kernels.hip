@@ -467,6 +467,82 @@ __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux(DevMesh m, 
   flux_body<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN>(m, b, o, U, G, R, lam, diag, st, lds);
 }
 
+// The residual on the face-coloured layout: one thread per face of colour range [begin, end); the face
+// flux of flux_body's phase 1, then F A added to the owner's and subtracted from the neighbour's R (and
+// lam A to both) in place: no two faces of a colour share a cell.
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
+__global__ __launch_bounds__(256) void k_flux_colour(DevMesh m, ColourFaces cf, int begin, int end, FaceOpts o,
+                                                     const double* __restrict__ U, const double* __restrict__ G,
+                                                     double* __restrict__ R, double* __restrict__ lam,
+                                                     int* __restrict__ diag) {
+  const int i = begin + blockIdx.x * 256 + threadIdx.x;
+  const int ld = m.ld;
+  int n_fo = 0, n_diff = 0, first_fail = 0x7fffffff;
+  if (i < end) {
+    const int co = cf.o[i], cn = cf.n[i];
+    const double nx = cf.nx[i], ny = cf.ny[i], A = cf.A[i];
+    Flux F;
+    if (cn >= 0) {
+      double go[8], gn[8];
+      double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
+      if (SECOND) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          go[j] = G[j * ld + co];
+          gn[j] = G[j * ld + cn];
+        }
+        const double fx = cf.fx[i], fy = cf.fy[i];
+        dox = fx - m.ccx[co];
+        doy = fy - m.ccy[co];
+        dnx = fx - m.ccx[cn];
+        dny = fy - m.ccy[cn];
+      }
+      double fr[8];
+      if (FROZEN && SECOND) {  // face-order ratios, already held at 2^-15
+        const int f = cf.face[i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fr[j] = o.frozen[static_cast<size_t>(j) * m.nfaces + f];
+      }
+      bool reverted;
+      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
+                                                         dnx, dny, o, &reverted, fr);
+      if (reverted) {
+        ++n_fo;
+        first_fail = cf.face[i];
+      }
+    } else {
+      F = boundary_face<SCHEME>(load_u(U, ld, co), (cf.meta[i] & kMetaBcMask) == 2, nx, ny, o);
+    }
+    if (F.fallback) ++n_diff;
+    const double a0 = F.f0 * A, a1 = F.f1 * A, a2 = F.f2 * A, a3 = F.f3 * A, w = F.lam * A;
+    R[co] += a0;
+    R[ld + co] += a1;
+    R[2 * ld + co] += a2;
+    R[3 * ld + co] += a3;
+    lam[co] += w;
+    if (cn >= 0) {
+      R[cn] -= a0;
+      R[ld + cn] -= a1;
+      R[2 * ld + cn] -= a2;
+      R[3 * ld + cn] -= a3;
+      lam[cn] += w;
+    }
+  }
+  const int a = wave_sum_i(n_fo);
+  const int d = wave_sum_i(n_diff);
+  const int ff = wave_min_i(first_fail);
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicAdd(&diag[0], a);
+    if (d) atomicAdd(&diag[1], d);
+    if (ff != 0x7fffffff) atomicMin(&diag[2], ff);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero(long long n, double* __restrict__ y) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) y[i] = 0.0;
+}
+
 // Frozen limiter capture (beyond the reference): per interior face (taken once, from the block that owns
 // its owner cell), the ratios phi(a, b) / a of both sides and the four primitive variables at the
 // state U and its gradients G, stored at ratios[j * F + face], j = owner rho, u, v, p, neighbour
@@ -1041,6 +1117,45 @@ hipError_t launch_flux(const DevMesh& m, const FaceOpts& o, int scheme, int limi
                                : launch_flux_bc<512, 1>(m, o, scheme, limiter, second, U, G, R, lam, diag, st, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN>
+static hipError_t launch_colours_t(const DevMesh& m, const ColourFaces& cf, const int* off, int nc, const FaceOpts& o,
+                                   const double* U, const double* G, double* R, double* lam, int* diag,
+                                   hipStream_t s) {
+  for (int c = 0; c < nc; ++c) {
+    const int n = off[c + 1] - off[c];
+    if (n <= 0) continue;
+    hipLaunchKernelGGL((k_flux_colour<SCHEME, LIMITER, SECOND, FROZEN>), dim3((n + 255) / 256), dim3(256), 0, s, m, cf,
+                       off[c], off[c + 1], o, U, G, R, lam, diag);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_flux_colour(const DevMesh& m, const ColourFaces& cf, const int* off, int nc, const FaceOpts& o,
+                              int scheme, int limiter, bool second, const double* U, const double* G, double* R,
+                              double* lam, int* diag, hipStream_t s) {
+  const long long nr = 4LL * m.N;
+  hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>((nr + 255) / 256)), dim3(256), 0, s, nr, R);
+  hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>((m.N + 255) / 256)), dim3(256), 0, s,
+                     static_cast<long long>(m.N), lam);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (second && o.frozen) {
+    if (scheme == 0) return launch_colours_t<0, 0, true, true>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+    return launch_colours_t<1, 0, true, true>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  }
+  if (!second) {
+    if (scheme == 0) return launch_colours_t<0, 0, false, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+    return launch_colours_t<1, 0, false, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  }
+  if (scheme == 0 && limiter == 0) return launch_colours_t<0, 0, true, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  if (scheme == 0 && limiter == 1) return launch_colours_t<0, 1, true, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  if (scheme == 1 && limiter == 0) return launch_colours_t<1, 0, true, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  if (scheme == 1 && limiter == 1) return launch_colours_t<1, 1, true, false>(m, cf, off, nc, o, U, G, R, lam, diag, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_face_scales(const DevMesh& m, const FaceOpts& o, const double* U, double* faces, double* cells,

@@ -256,7 +256,7 @@ def measure_sweep(args, seq_cases: int) -> dict:
     mkw = SWEEP_MESH
     log("[sweep] generating mesh")
     mesh = generate_mesh(precision=8, **mkw)
-    g = HipMesh(mesh, block_cells=args.block_cells)
+    g = HipMesh(mesh, block_cells=args.batch_block_cells)
     aoas = np.linspace(-4.0, 8.0, 16)
     machs = [0.3, 0.5, 0.6, 0.7]
     cases = [(0, float(m), float(a)) for m in machs for a in aoas]
@@ -422,7 +422,7 @@ def run_uneven(args) -> int:
             mach = float(rng.choice([0.05, 0.4, 0.7]))
             aoa = float(rng.uniform(0.0, 6.0))
             m, _ = deform_mesh(base, amp, tangents=False)
-            handles.append(HipMesh(m, block_cells=args.block_cells))
+            handles.append(HipMesh(m, block_cells=args.batch_block_cells))
             low = mach < 0.1
             specs.append(dict(size=size, base=base, amp=amp, mach=mach, aoa=aoa, low_mach=low))
             cases.append((len(handles) - 1, mach, aoa, 0, 0, low_cfg if low else None))
@@ -580,7 +580,7 @@ def run_adjoint(args) -> int:
                       cfl_ramp_iters=120, startup_first_order_iters=25, stab_floor_start=0.1, stab_floor_target=0.02,
                       stab_ramp_iters=400, precond_experimental_enabled=0, limiter_freeze_iter=args.freeze_iter,
                       residual_reduction_target=0.0, force_stability_tol=0.0, min_iterations=1, iterations=cap)
-    hm = HipMesh(mesh, block_cells=args.block_cells)
+    hm = HipMesh(mesh, block_cells=args.batch_block_cells)
     log(f"[adjoint] primal: {mesh.num_cells} cells to a {args.adjoint_primal_drop:.0e} density drop (cap {cap})")
     with HipBatch([hm], [(0, case.mach, case.aoa_deg)], case) as b:
         r = b.run(cap, args.adjoint_primal_drop, iters_per_sync=500, use_graph=True)
@@ -700,7 +700,7 @@ def run_vjp(args) -> int:
     cfg = residual_config(ff, flux="hllc", limiter="venkat", second_order=True, stab_mach_floor=0.02)
     psi = np.random.default_rng(1).standard_normal((m.num_cells, 5))
     walls = []
-    with HipMesh(m, block_cells=args.block_cells) as hm:
+    with HipMesh(m, block_cells=args.batch_block_cells) as hm:
         hm.set_state(U)
         for i in range(args.vjp_calls):
             t0 = time.time()
@@ -721,7 +721,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="mixed1m_hllc_venkat", choices=sorted(WORKLOADS))
-    ap.add_argument("--block-cells", type=int, default=256)
+    ap.add_argument("--block-cells", type=int, default=0,
+                    help="cells per block of the headline residual workload (0: the layout probe chooses block size "
+                         "and accumulation for the mesh)")
+    ap.add_argument("--batch-block-cells", type=int, default=256,
+                    help="cells per block of the batched workloads (sweep, adjoint, uneven)")
     ap.add_argument("--no-renumber", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample length (0 = skip)")
     ap.add_argument("--no-variants", dest="variants", action="store_false",
@@ -822,8 +826,12 @@ def main() -> int:
                     all_speed_cutoff=0.25, all_speed_f_min=0.05, all_speed_ramp=1.0)
     cfg = residual_config(ff, flux=flux, limiter=lim, second_order=True, stab_mach_floor=0.02)
 
+    t_probe = time.time()
     hm = HipMesh(mesh, renumber=not args.no_renumber, block_cells=args.block_cells)
     layout = hm.layout()
+    layout["probe_ms"] = hm.layout_probe()
+    layout["setup_seconds"] = round(time.time() - t_probe, 2)
+    log(f"[bench] layout {layout['block_cells']} cells per block, {layout['accumulation']} (probe {layout['probe_ms']})")
     hm.set_state(U)
     stream = torch.cuda.ExternalStream(hm.stream())
     log(f"[bench] warmup {args.warmup}")
@@ -843,7 +851,9 @@ def main() -> int:
     ceiling = copy_ceiling()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(args.workload),
-                "kernel": "k_flux (reconstruction + Riemann flux + face->cell gather)",
+                "kernel": ("k_flux (reconstruction + Riemann flux + face->cell gather)"
+                           if layout["accumulation"] == "gather" else
+                           "k_flux_colour (every colour's launch of one evaluation, and the zero fill of R)"),
                 "algorithmic_bytes_per_launch": flux_bytes, "avg_launch_ms": round(flux_ms, 5),
                 "copy_ceiling_gbs": round(ceiling, 1), "frac_of_copy_ceiling": round(achieved / ceiling, 4)}
     extra = {"gradient_kernel_ms": round(grad_ms, 5),
@@ -890,7 +900,7 @@ def main() -> int:
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": args.workload, "cells": N, "faces": F, "flux": flux, "limiter": lim,
-                      "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": args.block_cells,
+                      "mach": mach, "aoa_deg": aoa, "mesh": mkw, "block_cells": layout["block_cells"],
                       "renumber": not args.no_renumber},
            "roofline": roofline, "cpu_baseline": cpu, "cpu_reference": cpu_ref, "cpu_host": host_cpu(),
            "variants": variants, "sweep": sweep, "adjoint": adjoint,

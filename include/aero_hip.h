@@ -48,10 +48,25 @@ typedef struct aero_mesh_desc {
 } aero_mesh_desc;
 
 /* Device layout options for aero_hip_mesh_create. */
+enum { AERO_ACCUM_AUTO = 0, AERO_ACCUM_GATHER = 1, AERO_ACCUM_COLOUR = 2 };
 typedef struct aero_hip_options {
   int32_t renumber;      /* 1: renumber cells into compact blocks (recursive coordinate bisection); 0: keep order */
-  int32_t block_cells;   /* cells per workgroup block (64..1024, multiple of 64); 0 = autotuned default */
+  int32_t block_cells;   /* cells per workgroup block: 128, 256 or 512; 0 = chosen by the layout probe below */
   int32_t gradient;      /* AERO_GRAD_* */
+  /* Face->cell accumulation of residual evaluations (no reference counterpart; the reference's serial loop
+   * sums in face order, residual_assembly.cpp:97-181):
+   *  AERO_ACCUM_GATHER: per-block face slots staged in LDS, each cell summing its faces in reference face
+   *    order (the reference's summation order: results bit for bit the FP64 oracle's);
+   *  AERO_ACCUM_COLOUR: faces coloured so that no two faces of a colour share a cell, one launch per
+   *    colour updating both cells in place (contributions arrive in colour order: the same sums to
+   *    rounding, within 1e-12 of the face-flux scale);
+   *  AERO_ACCUM_AUTO: gather, unless block_cells = 0.
+   * With block_cells = 0 the layout probe times one residual evaluation (HLLC + venkat, second order, on a
+   * smooth synthetic state) in every candidate -- gather at 128, 256 and 512 cells per block and, unless
+   * accumulation = GATHER, the coloured layout -- and keeps the fastest, cached per mesh for the process
+   * (aero_hip_layout_probe reports the times).  Stage updates (aero_hip_stage, _iterate, _run_case and the
+   * batches) always use the gather layout at the chosen block size. */
+  int32_t accumulation;
 } aero_hip_options;
 
 /* Per-evaluation scalars of EulerResidualAssemblyConfig (solvers/euler/residual_assembly.hpp:12-31). */
@@ -170,6 +185,10 @@ int aero_hip_get_gradients(aero_hip_mesh* mesh, double* gradients /* [N][5][2] r
                            size_t errlen);
 /* The handle's current conserved state, [N][5] in the reference numbering. */
 int aero_hip_get_state(aero_hip_mesh* mesh, double* conserved, char* err, size_t errlen);
+/* The layout probe's timings for this handle (block_cells = 0): ms per residual evaluation with gather at
+ * 128, 256 and 512 cells per block and with the coloured layout; -1 where a candidate was not probed
+ * (explicit block_cells, or not buildable). */
+int aero_hip_layout_probe(aero_hip_mesh* mesh, double* ms4);
 /* The handle's HIP stream (hipStream_t), for event timing by the caller. */
 void* aero_hip_stream(aero_hip_mesh* mesh);
 
