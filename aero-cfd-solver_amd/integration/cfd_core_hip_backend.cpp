@@ -88,7 +88,7 @@ bool init_euler_hip_mesh(const UnstructuredMesh& mesh, const std::vector<EulerBo
   const std::vector<double> cc(mesh.cell_center.begin(), mesh.cell_center.end());
   aero_mesh_desc d{mesh.num_cells, mesh.num_faces, mesh.face_owner.data(), mesh.face_neighbor.data(),
                    bc.data(), n.data(), fc.data(), area.data(), vol.data(), cc.data(), wall_start, wall_count};
-  const aero_hip_options opts{1, 0, AERO_GRAD_GREEN_GAUSS};
+  const aero_hip_options opts{1, 0, AERO_GRAD_GREEN_GAUSS, AERO_ACCUM_AUTO};  // block size and accumulation: the layout probe
   char err[512];
   if (aero_hip_mesh_create(&d, &opts, &guard->handle, err, sizeof err) != AERO_OK) {
     if (error_message) *error_message = err;
