@@ -84,7 +84,18 @@ class CaseConfigC(C.Structure):
 class RunOutput(C.Structure):
     _fields_ = [("hist_capacity", i32), ("n_iter", i32), ("nonfinite", i32), ("history", PD),
                 ("conserved", PD), ("residual", PD), ("spectral_radius", PD), ("forces", PD),
-                ("wall_cp", PD), ("device_seconds", f64), ("final_config", C.POINTER(ResidualConfig))]
+                ("wall_cp", PD), ("device_seconds", f64), ("final_config", C.POINTER(ResidualConfig)),
+                ("diagnostics", PD)]
+
+
+DIAG_COLUMNS = ["max_wall_un", "max_farfield_abs_dp", "max_farfield_abs_drho", "max_farfield_abs_du",
+                "max_farfield_abs_dv", "wall_cp_min", "wall_cp_max", "wall_cp_nonfinite", "Fx_gauge", "Fy_gauge",
+                "Fx_abs", "Fy_abs", "cl_abs", "cd_abs", "cm_abs", "L", "D", "p_jump_energy_nd",
+                "p_jump_energy_wall_nd", "p_fluct_nd_max", "p_fluct_nd_wall", "p_fluct_nd_farfield",
+                "all_speed_stage", "all_speed_flux_fix_effective", "all_speed_f_min_effective",
+                "all_speed_ramp_weight_effective", "all_speed_freeze_counter", "all_speed_controller_action",
+                "cfl_scale_effective", "first_failure_face"]
+DIAG_FIELDS = len(DIAG_COLUMNS)
 
 
 class MeshGenConfig(C.Structure):

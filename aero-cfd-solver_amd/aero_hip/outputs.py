@@ -49,26 +49,36 @@ def _g(v) -> str:
 def force_monitor(cl, cd, cm, window: int, tol: float, drift_tol: float) -> dict:
     """Per-iteration force columns of ConvergenceMonitor (solvers/euler/convergence_monitor.cpp:21-117):
     change from the previous iteration, stability of the last `window` values (max - min <= tol each),
-    mean drift between the last two windows and its test (<= drift_tol; tol when drift_tol <= 0)."""
-    n = len(cl)
+    mean drift between the last two windows and its test (<= drift_tol; tol when drift_tol <= 0).  The
+    arithmetic is the monitor's, in the coefficients' own precision: window sums accumulated in order,
+    means as sum * (1 / window)."""
+    F = np.stack([np.asarray(cl), np.asarray(cd), np.asarray(cm)], axis=1)
+    if F.dtype not in (np.float32, np.float64):
+        F = F.astype(np.float64)
+    t = F.dtype.type
+    n = len(F)
     w = max(int(window), 1)
-    dtol = drift_tol if drift_tol > 0.0 else tol
-    F = np.stack([np.asarray(cl, float), np.asarray(cd, float), np.asarray(cm, float)], axis=1)
-    d = np.zeros((n, 3))
+    tol_t = t(tol)
+    dtol = t(drift_tol) if drift_tol > 0.0 else tol_t
+    d = np.zeros((n, 3), F.dtype)
     if n > 1:
         d[1:] = np.abs(F[1:] - F[:-1])
-    md = np.zeros((n, 3))
+    md = np.zeros((n, 3), F.dtype)
     stable = np.zeros(n, bool)
     mstable = np.zeros(n, bool)
-    for i in range(n):
-        if i + 1 >= w:
-            win = F[i + 1 - w:i + 1]
-            stable[i] = bool(np.all(win.max(axis=0) - win.min(axis=0) <= tol))
-        if i + 1 >= 2 * w:
-            prev = F[i + 1 - 2 * w:i + 1 - w].sum(axis=0)
-            curr = F[i + 1 - w:i + 1].sum(axis=0)
-            md[i] = np.abs(curr / w - prev / w)
-            mstable[i] = bool(np.all(md[i] <= dtol))
+    if n >= w:
+        win = np.lib.stride_tricks.sliding_window_view(F, w, axis=0)  # [n - w + 1, 3, w]: F[e+1-w .. e]
+        stable[w - 1:] = np.all(win.max(axis=-1) - win.min(axis=-1) <= tol_t, axis=1)
+    if n >= 2 * w:
+        ends = np.arange(2 * w - 1, n)
+        prev = np.zeros((len(ends), 3), F.dtype)
+        curr = np.zeros((len(ends), 3), F.dtype)
+        for k in range(w):  # in order, oldest first
+            prev = prev + F[ends + 1 - 2 * w + k]
+            curr = curr + F[ends + 1 - w + k]
+        inv = t(1.0) / t(w)
+        md[ends] = np.abs(curr * inv - prev * inv)
+        mstable[ends] = np.all(md[ends] <= dtol, axis=1)
     return dict(d=d, mean_d=md, stable=stable, mean_stable=mstable, converged=stable & mstable)
 
 
@@ -195,6 +205,78 @@ def write_outputs(out_dir: Path, mesh: Mesh, case: CaseConfig, out: dict) -> dic
     return paths
 
 
+DIAGNOSTICS_HEADER = (
+    "iter,residual_ratio,force_window_stable,integrated_wall_faces,wall_patch_faces,wall_patch_bc_mismatch,"
+    "sum_wall_ds,sum_nA_x,sum_nA_y,sum_nA_mag,max_wall_un,max_wall_mass_flux_n,max_farfield_abs_dp,"
+    "max_farfield_abs_drho,max_farfield_abs_du,max_farfield_abs_dv,max_farfield_abs_dw,wall_cp_min,wall_cp_max,"
+    "wall_cp_nonfinite,q_inf,rho_inf,V_inf,S_ref,chord,Fx_gauge,Fy_gauge,Fx_abs,Fy_abs,L,D,Cl,Cd,Cm,cl_abs,cd_abs,"
+    "cm_abs,dcl,dcd,dcm,force_mean_drift_stable,force_physical_converged,mean_dcl,mean_dcd,mean_dcm,cd_sign_flip,"
+    "p_jump_energy_nd,p_jump_energy_wall_nd,all_speed_stage,all_speed_flux_fix_effective,all_speed_f_min_effective,"
+    "all_speed_ramp_weight_effective,all_speed_freeze_counter,all_speed_controller_action,cfl_scale_effective,"
+    "cd_times_mach,p_fluct_nd_max,p_fluct_nd_wall,p_fluct_nd_farfield,num_faces_first_order_fallback,"
+    "num_faces_diffusive_fallback,first_failure_face,first_failure_owner,first_failure_neighbor")
+STAGE_NAMES = ("single", "A", "B")
+
+
+def diagnostics_rows(mesh: Mesh, case: CaseConfig, out: dict) -> list[list]:
+    """The rows of euler_diagnostics.csv (euler_solver.cpp:1161-1203) from a run's history, its per-iteration
+    diagnostics (aero_run_output.diagnostics) and the mesh: one list of values per iteration, in the
+    header's order (all_speed_stage as the reference's stage name).  The wall-patch sums run in patch
+    order in the mesh's precision (summarize_face_geometry, compute_pressure_force_diagnostics:
+    euler_solver.cpp:197-228, post/forces.cpp:208-246)."""
+    H, D = out["history"], out["diagnostics"]
+    rho_inf = rho_inf_of(case)
+    V = speed_inf_of(case)
+    q = 0.5 * rho_inf * V * V + 1.0e-12
+    chord = max(case.chord, 1.0e-8)
+    wall_patch = np.arange(mesh.wall_start, mesh.wall_start + mesh.wall_count)
+    wall_bc = np.flatnonzero(mesh.bc == _lib.BC_SLIP_WALL)
+    mismatch = len(np.setxor1d(wall_patch, wall_bc))
+    area = mesh.area[wall_patch]
+    sum_ds = float(np.cumsum(area)[-1]) if len(area) else 0.0
+    nA = -mesh.normal[wall_patch, :2] * area[:, None]  # face_nA: the body-outward normal times the area
+    nax = float(np.cumsum(nA[:, 0])[-1]) if len(area) else 0.0
+    nay = float(np.cumsum(nA[:, 1])[-1]) if len(area) else 0.0
+    nam = float(np.sqrt(np.asarray(nax * nax + nay * nay, area.dtype)))
+    cl, cd, cm = H[:, _lib.H_CL], H[:, _lib.H_CD], H[:, _lib.H_CM]
+    mon = force_monitor(cl, cd, cm, case.force_stability_window, case.force_stability_tol, case.force_mean_drift_tol)
+    col = {n: i for i, n in enumerate(_lib.DIAG_COLUMNS)}
+    mach = H.dtype.type(max(case.mach, 0.0))
+    rows = []
+    for i, (h, d) in enumerate(zip(H, D)):
+        ff = int(d[col["first_failure_face"]])
+        flip = i > 0 and ((cd[i - 1] >= 0.0) != (cd[i] >= 0.0))
+        rows.append([
+            int(h[_lib.H_ITER]), h[_lib.H_RATIO], int(mon["stable"][i]), mesh.wall_count, mesh.wall_count, mismatch,
+            sum_ds, nax, nay, nam, d[col["max_wall_un"]], 0.0,  # the slip-wall flux carries no mass
+            d[col["max_farfield_abs_dp"]], d[col["max_farfield_abs_drho"]], d[col["max_farfield_abs_du"]],
+            d[col["max_farfield_abs_dv"]], 0.0, d[col["wall_cp_min"]], d[col["wall_cp_max"]],
+            int(d[col["wall_cp_nonfinite"]]), q, rho_inf, V, chord, chord, d[col["Fx_gauge"]], d[col["Fy_gauge"]],
+            d[col["Fx_abs"]], d[col["Fy_abs"]], d[col["L"]], d[col["D"]], cl[i], cd[i], cm[i], d[col["cl_abs"]],
+            d[col["cd_abs"]], d[col["cm_abs"]], *mon["d"][i], int(mon["mean_stable"][i]), int(mon["converged"][i]),
+            *mon["mean_d"][i], int(flip), d[col["p_jump_energy_nd"]], d[col["p_jump_energy_wall_nd"]],
+            STAGE_NAMES[int(d[col["all_speed_stage"]])], int(d[col["all_speed_flux_fix_effective"]]),
+            d[col["all_speed_f_min_effective"]], d[col["all_speed_ramp_weight_effective"]],
+            int(d[col["all_speed_freeze_counter"]]), int(d[col["all_speed_controller_action"]]),
+            d[col["cfl_scale_effective"]], cd[i] * mach, d[col["p_fluct_nd_max"]],
+            d[col["p_fluct_nd_wall"]], d[col["p_fluct_nd_farfield"]], int(h[_lib.H_FO]),
+            int(h[_lib.H_DIFF]), ff, int(mesh.owner[ff]) if ff >= 0 else -1,
+            int(mesh.neighbor[ff]) if ff >= 0 else -1])
+    return rows
+
+
+def diagnostics_lines(mesh: Mesh, case: CaseConfig, out: dict) -> list[str]:
+    """euler_diagnostics.csv as lines (euler_solver.cpp:717-738 header, :1161-1203 rows)."""
+    fmt = lambda v: v if isinstance(v, str) else (str(v) if isinstance(v, (int, np.integer)) else _g(v))
+    return [DIAGNOSTICS_HEADER] + [",".join(fmt(v) for v in r) for r in diagnostics_rows(mesh, case, out)]
+
+
+def write_diagnostics(path: Path, mesh: Mesh, case: CaseConfig, out: dict) -> None:
+    """Write euler_diagnostics.csv of a run."""
+    with open(path, "w", encoding="utf-8") as f:
+        f.write("\n".join(diagnostics_lines(mesh, case, out)) + "\n")
+
+
 DISSIPATION_REGIONS = ("wall_adjacent_interior", "farfield_boundary")
 
 
@@ -309,13 +391,14 @@ def run_case_file(case_path: str | Path, out_dir: str | Path, block_cells: int =
     out_dir = Path(out_dir)
     out_dir.mkdir(parents=True, exist_ok=True)
     with HipMesh(mesh, block_cells=block_cells) as hm:
-        out = hm.run_case(case)
+        out = hm.run_case(case, diagnostics=True)
         if out["nonfinite"]:
             raise RuntimeError("Euler pseudo-time iteration produced a non-finite state.")
         fs, cs = hm.face_scales(out["final_config"], out["conserved"])
         fa = hm.forces(rho_inf_of(case), case.p_inf, case.aoa_deg, speed_inf_of(case), case.chord, case.x_ref,
                        case.y_ref, gauge=False, gamma=case.gamma, U=out["conserved"])
     write_outputs(out_dir, mesh, case, out)
+    write_diagnostics(out_dir / "euler_diagnostics.csv", mesh, case, out)
     post = write_post_files(out_dir, mesh, out["final_config"], fs)
     stats = scaling_statistics(mesh, fs, cs)
     H = out["history"]

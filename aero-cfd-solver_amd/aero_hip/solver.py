@@ -259,10 +259,12 @@ class HipMesh:
               write_residual: bool = False):
         self._call("aero_hip_stage", C.byref(cfg), stage, int(last_stage), cfl, int(lts), int(write_residual))
 
-    def run_case(self, case: CaseConfig, hist_capacity: int | None = None) -> dict:
-        """run_euler_airfoil_case on the device (euler_solver.cpp:463-1208)."""
+    def run_case(self, case: CaseConfig, hist_capacity: int | None = None, diagnostics: bool = False) -> dict:
+        """run_euler_airfoil_case on the device (euler_solver.cpp:463-1208); diagnostics: also the
+        per-iteration device diagnostics of euler_diagnostics.csv ([n_iter, DIAG_FIELDS])."""
         cap = hist_capacity or case.iterations
         hist = np.zeros((cap, _lib.HIST_FIELDS))
+        diag = np.zeros((cap, _lib.DIAG_FIELDS)) if diagnostics else None
         U = np.zeros((self.N, 5))
         R = np.zeros((self.N, 5))
         lam = np.zeros(self.N)
@@ -270,13 +272,14 @@ class HipMesh:
         cp = np.zeros(max(self.mesh.wall_count, 1))
         final = _lib.ResidualConfig()
         out = _lib.RunOutput(cap, 0, 0, _pd(hist), _pd(U), _pd(R), _pd(lam), _pd(forces), _pd(cp), 0.0,
-                             C.pointer(final))
+                             C.pointer(final), _pd(diag) if diagnostics else None)
         cc = case.to_c()
         self._call("aero_hip_run_case", C.byref(cc), C.byref(out))
         n = out.n_iter
         return dict(history=hist[:n].copy(), n_iter=n, nonfinite=bool(out.nonfinite), conserved=U, residual=R,
                     spectral_radius=lam, forces=forces, cl=forces[5], cd=forces[6], cm=forces[7],
-                    wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds, final_config=final)
+                    wall_cp=cp[: self.mesh.wall_count], seconds=out.device_seconds, final_config=final,
+                    diagnostics=diag[:n].copy() if diagnostics else None)
 
     def face_scales(self, cfg: _lib.ResidualConfig, U=None) -> tuple[np.ndarray, np.ndarray]:
         """The post-run scaling diagnostics at U (None: the handle's state) under cfg (aero_hip_face_scales):

@@ -50,6 +50,7 @@ __global__ void k_init_diag(int* d) {
     d[1] = 0;
     d[2] = INT_MAX;
     d[3] = 0;
+    d[4] = INT_MAX;
   }
 }
 
@@ -88,6 +89,121 @@ __global__ __launch_bounds__(256) void k_pjump_sum(int n, const int2* __restrict
     for (int i = 0; i < n; ++i) acc += terms[i];
     out[0] = acc;
   }
+}
+
+// The first failing face of an iteration's first stage with one (residual_assembly.hpp:33-41 per stage, the
+// driver keeping the first stage's, euler_solver.cpp:870-900): after each stage d[4] latches d[2] once and
+// d[2] is cleared for the next stage.
+__global__ void k_latch_first_failure(int* d) {
+  if (threadIdx.x == 0) {
+    if (d[4] == INT_MAX && d[2] != INT_MAX) d[4] = d[2];
+    d[2] = INT_MAX;
+  }
+}
+
+// Per-iteration diagnostics of the reference driver's euler_diagnostics.csv (euler_solver.cpp:921-1060) from
+// the iteration's final state: two passes, block partials then one fixed-order reduction.  Slots:
+enum : int {
+  kDgMaxUn = 0, kDgFfDp, kDgFfDrho, kDgFfDu, kDgFfDv,  // max |u.n| at wall owners; farfield-owner deviations
+  kDgCpMin, kDgCpMax, kDgCpNonfinite,                   // wall Cp
+  kDgMomentAbs,                                         // moment of the absolute-pressure wall forces
+  kDgPjAll, kDgPjWall,                                  // sum (p_o - p_n)^2, all / wall-adjacent interior faces
+  kDgPfMax, kDgPfWall,                                  // max |p - p_inf|, all / wall-adjacent cells
+  kDgSlots
+};
+struct DiagArgs {
+  int N, ld, nint;
+  const int2* faces;                // interior faces (owner, neighbour), internal ids
+  const unsigned char* wall_adj;    // [N] internal order: 1 for wall-face owner cells
+  aero_dev::WallArgs wall;          // wall faces + reference values
+  int nff;
+  const int* ff_owner;              // farfield-face owner cells, internal ids
+  double g, rho_inf, u_inf, v_inf;
+};
+
+__device__ __forceinline__ bool dg_is_min(int k) { return k == kDgCpMin; }
+__device__ __forceinline__ bool dg_is_sum(int k) {
+  return k == kDgCpNonfinite || k == kDgMomentAbs || k == kDgPjAll || k == kDgPjWall;
+}
+__device__ __forceinline__ double dg_init(int k) {
+  return k == kDgCpMin ? 1.79769313486231570815e+308 : (k == kDgCpMax ? -1.79769313486231570815e+308 : 0.0);
+}
+__device__ __forceinline__ double dg_join(int k, double a, double b) {
+  if (dg_is_sum(k)) return a + b;
+  if (dg_is_min(k)) return aero_dev::dmin(a, b);
+  return aero_dev::dmax(a, b);
+}
+
+__device__ __forceinline__ double dg_pressure(const double* U, int ld, int c, double g) {
+  return aero_dev::c2p(aero_dev::C4{U[c], U[ld + c], U[2 * ld + c], U[3 * ld + c]}, g).p;
+}
+
+__device__ void dg_block_reduce(double (*v)[kDgSlots], double* acc) {
+  const int t = threadIdx.x;
+  for (int k = 0; k < kDgSlots; ++k) v[t][k] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int k = 0; k < kDgSlots; ++k) v[t][k] = dg_join(k, v[t][k], v[t + w][k]);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_partials(DiagArgs a, const double* __restrict__ U,
+                                                       double* __restrict__ partials) {
+  __shared__ double v[256][kDgSlots];
+  double acc[kDgSlots];
+  for (int k = 0; k < kDgSlots; ++k) acc[k] = dg_init(k);
+  const double pinf = a.wall.p_inf;
+  const int stride = gridDim.x * 256;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.N; i += stride) {
+    const double dp = fabs(dg_pressure(U, a.ld, i, a.g) - pinf);
+    acc[kDgPfMax] = aero_dev::dmax(acc[kDgPfMax], dp);
+    if (a.wall_adj[i]) acc[kDgPfWall] = aero_dev::dmax(acc[kDgPfWall], dp);
+  }
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.nint; i += stride) {
+    const int2 f = a.faces[i];
+    const double dp = dg_pressure(U, a.ld, f.x, a.g) - dg_pressure(U, a.ld, f.y, a.g);
+    const double e = dp * dp;
+    acc[kDgPjAll] += e;
+    if (a.wall_adj[f.x] || a.wall_adj[f.y]) acc[kDgPjWall] += e;
+  }
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.wall.n; i += stride) {
+    const int c = a.wall.owner[i];
+    const aero_dev::P4 q = aero_dev::c2p(aero_dev::C4{U[c], U[a.ld + c], U[2 * a.ld + c], U[3 * a.ld + c]}, a.g);
+    acc[kDgMaxUn] = aero_dev::dmax(acc[kDgMaxUn], fabs(q.u * a.wall.nx[i] + q.v * a.wall.ny[i]));
+    const double cp = (q.p - pinf) / a.wall.q_inf;
+    if (isfinite(cp)) {
+      acc[kDgCpMin] = aero_dev::dmin(acc[kDgCpMin], cp);
+      acc[kDgCpMax] = aero_dev::dmax(acc[kDgCpMax], cp);
+    } else {
+      acc[kDgCpNonfinite] += 1.0;
+    }
+    const double nA0 = -a.wall.nx[i] * a.wall.A[i], nA1 = -a.wall.ny[i] * a.wall.A[i];
+    const double dfx = -q.p * nA0, dfy = -q.p * nA1;
+    acc[kDgMomentAbs] += (a.wall.fx[i] - a.wall.x_ref) * dfy - (a.wall.fy[i] - a.wall.y_ref) * dfx;
+  }
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.nff; i += stride) {
+    const int c = a.ff_owner[i];
+    const aero_dev::P4 q = aero_dev::c2p(aero_dev::C4{U[c], U[a.ld + c], U[2 * a.ld + c], U[3 * a.ld + c]}, a.g);
+    acc[kDgFfDp] = aero_dev::dmax(acc[kDgFfDp], fabs(q.p - pinf));
+    acc[kDgFfDrho] = aero_dev::dmax(acc[kDgFfDrho], fabs(q.r - a.rho_inf));
+    acc[kDgFfDu] = aero_dev::dmax(acc[kDgFfDu], fabs(q.u - a.u_inf));
+    acc[kDgFfDv] = aero_dev::dmax(acc[kDgFfDv], fabs(q.v - a.v_inf));
+  }
+  dg_block_reduce(v, acc);
+  if (threadIdx.x < kDgSlots) partials[blockIdx.x * kDgSlots + threadIdx.x] = v[0][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(int nparts, const double* __restrict__ partials,
+                                                    double* __restrict__ out) {
+  __shared__ double v[256][kDgSlots];
+  double acc[kDgSlots];
+  for (int k = 0; k < kDgSlots; ++k) acc[k] = dg_init(k);
+  for (int b = threadIdx.x; b < nparts; b += 256)
+    for (int k = 0; k < kDgSlots; ++k) acc[k] = dg_join(k, acc[k], partials[b * kDgSlots + k]);
+  dg_block_reduce(v, acc);
+  if (threadIdx.x < kDgSlots) out[threadIdx.x] = v[0][threadIdx.x];
 }
 
 // Recursive coordinate bisection of cell centres into blocks of BC cells (every block full except the
@@ -706,12 +822,12 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
   AERO_SETUP(dev_alloc(m, &m->G, static_cast<size_t>(8) * N));
   AERO_SETUP(dev_alloc(m, &m->R, static_cast<size_t>(4) * N));
   AERO_SETUP(dev_alloc(m, &m->lam, static_cast<size_t>(N)));
-  AERO_SETUP(dev_alloc(m, &m->diag, 4));
+  AERO_SETUP(dev_alloc(m, &m->diag, 8));  // counters [0..3], the iteration's first failing face [4]
   AERO_SETUP(dev_alloc(m, &m->partials, static_cast<size_t>(4) * m->nblk));
   AERO_SETUP(dev_alloc(m, &m->dt_scratch, 1));
   AERO_SETUP(dev_alloc(m, &m->iter_out, 16));
   AERO_SETUP(dev_alloc(m, &m->cp, std::max(1, d->wall_count)));
-  AERO_SETUP(hipHostMalloc(reinterpret_cast<void**>(&m->h_iter), 32 * sizeof(double), hipHostMallocDefault));
+  AERO_SETUP(hipHostMalloc(reinterpret_cast<void**>(&m->h_iter), 64 * sizeof(double), hipHostMallocDefault));
   AERO_SETUP(hipMemsetAsync(m->U[0], 0, static_cast<size_t>(4) * N * sizeof(double), m->stream));
   AERO_SETUP(hipMemsetAsync(m->G, 0, static_cast<size_t>(8) * N * sizeof(double), m->stream));
   AERO_SETUP(hipStreamSynchronize(m->stream));
@@ -1531,6 +1647,67 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     }
   }
 
+  // Per-iteration diagnostics (out->diagnostics): the interior faces, wall-adjacent cells and farfield
+  // owners of the reductions, in internal numbering.
+  DiagArgs da{};
+  int n_parts = 0, n_wall_int = 0;
+  double* d_diag_parts = nullptr;
+  double* d_diag_out = nullptr;
+  std::vector<void*> diag_allocs;
+  struct DiagFree {
+    std::vector<void*>* v;
+    ~DiagFree() {
+      for (void* p : *v) (void)hipFree(p);
+    }
+  } diag_guard{&diag_allocs};
+  if (out->diagnostics) {
+    std::vector<unsigned char> adj(m->N, 0u);
+    for (int i = 0; i < m->wall_count; ++i) adj[m->iperm[m->h_owner[m->wall_start + i]]] = 1u;
+    std::vector<int2> faces;
+    std::vector<int> ffo;
+    for (int f = 0; f < m->F; ++f) {
+      const int o = m->iperm[m->h_owner[f]];
+      if (m->h_neighbor[f] >= 0) {
+        const int n = m->iperm[m->h_neighbor[f]];
+        faces.push_back(make_int2(o, n));
+        if (adj[o] || adj[n]) ++n_wall_int;
+      } else if (m->h_bc[f] == AERO_BC_FARFIELD) {
+        ffo.push_back(o);
+      }
+    }
+    auto dalloc = [&](void** p, size_t bytes) -> hipError_t {
+      const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+      if (e == hipSuccess) diag_allocs.push_back(*p);
+      return e;
+    };
+    void *pf = nullptr, *pa = nullptr, *po = nullptr, *pp = nullptr, *pd = nullptr;
+    da.N = m->N;
+    da.ld = m->dm.ld;
+    da.nint = static_cast<int>(faces.size());
+    da.nff = static_cast<int>(ffo.size());
+    n_parts = std::min(256, std::max(1, (std::max(m->N, da.nint) + 255) / 256));
+    AERO_HIP_TRY(dalloc(&pf, faces.size() * sizeof(int2)), "diagnostics: hipMalloc");
+    AERO_HIP_TRY(dalloc(&pa, adj.size()), "diagnostics: hipMalloc");
+    AERO_HIP_TRY(dalloc(&po, ffo.size() * sizeof(int)), "diagnostics: hipMalloc");
+    AERO_HIP_TRY(dalloc(&pp, static_cast<size_t>(n_parts) * kDgSlots * sizeof(double)), "diagnostics: hipMalloc");
+    AERO_HIP_TRY(dalloc(&pd, kDgSlots * sizeof(double)), "diagnostics: hipMalloc");
+    if (!faces.empty())
+      AERO_HIP_TRY(hipMemcpy(pf, faces.data(), faces.size() * sizeof(int2), hipMemcpyHostToDevice), "diagnostics");
+    AERO_HIP_TRY(hipMemcpy(pa, adj.data(), adj.size(), hipMemcpyHostToDevice), "diagnostics");
+    if (!ffo.empty())
+      AERO_HIP_TRY(hipMemcpy(po, ffo.data(), ffo.size() * sizeof(int), hipMemcpyHostToDevice), "diagnostics");
+    da.faces = static_cast<const int2*>(pf);
+    da.wall_adj = static_cast<const unsigned char*>(pa);
+    da.ff_owner = static_cast<const int*>(po);
+    da.wall = wa;
+    da.g = g;
+    da.rho_inf = ff[0];
+    da.u_inf = u_inf;
+    da.v_inf = vv_inf;
+    d_diag_parts = static_cast<double*>(pp);
+    d_diag_out = static_cast<double*>(pd);
+  }
+
   Monitor mon(c->residual_reduction_target, c->force_stability_tol, c->force_stability_window, c->min_iterations,
               c->force_mean_drift_tol);
   aero_residual_config rc{};
@@ -1606,6 +1783,8 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     for (int st = 0; st < c->rk_stages; ++st) {
       const int rcode = aero_hip_stage(m, &rc, st, st == c->rk_stages - 1, cfl, c->local_time_stepping, 1, err, errlen);
       if (rcode) return rcode;
+      hipLaunchKernelGGL(k_latch_first_failure, dim3(1), dim3(64), 0, m->stream, m->diag);
+      AERO_HIP_TRY(hipGetLastError(), "k_latch_first_failure");
     }
     if (c->aoa0_symmetry_enforce && c->aoa0_symmetry_enforce_interval > 0 && !(std::abs(c->aoa_deg) > 1.0e-12) &&
         ((iter + 1) % c->aoa0_symmetry_enforce_interval) == 0) {
@@ -1623,8 +1802,16 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     AERO_HIP_TRY(hipMemcpyAsync(m->h_iter, m->iter_out, (n_pj > 0 ? 10 : 9) * sizeof(double), hipMemcpyDeviceToHost,
                                 m->stream),
                  "iteration scalars");
-    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter + 10, m->diag, 4 * sizeof(int), hipMemcpyDeviceToHost, m->stream),
+    AERO_HIP_TRY(hipMemcpyAsync(m->h_iter + 10, m->diag, 5 * sizeof(int), hipMemcpyDeviceToHost, m->stream),
                  "iteration counters");
+    if (d_diag_out) {  // h_iter[16 ..]: the diagnostics reductions of the new state
+      hipLaunchKernelGGL(k_diag_partials, dim3(n_parts), dim3(256), 0, m->stream, da, m->U[m->cur], d_diag_parts);
+      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, m->stream, n_parts, d_diag_parts, d_diag_out);
+      AERO_HIP_TRY(hipGetLastError(), "diagnostics reduction");
+      AERO_HIP_TRY(hipMemcpyAsync(m->h_iter + 16, d_diag_out, kDgSlots * sizeof(double), hipMemcpyDeviceToHost,
+                                  m->stream),
+                   "diagnostics copy");
+    }
     AERO_HIP_TRY(hipStreamSynchronize(m->stream), "iteration sync");
     const int* hd = reinterpret_cast<const int*>(m->h_iter + 10);
     if (hd[3]) {  // non-finite trap (:908-946)
@@ -1647,6 +1834,7 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
     last = fg;
     mon.update_forces(fg);
     const bool window_stable = mon.stable();
+    int action = 0;  // the controller's action code this iteration (euler_solver.cpp:1070-1139)
     if (staged) {  // p-jump diagnostic (device sum, h[9]) + staged all-speed controller (:1026-1139)
       const double pj = n_pj > 0 ? h[9] : 0.0;
       const int pn = n_pj;
@@ -1659,6 +1847,7 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
           stage_ab = 2;
           b_fmin = f_high;
           b_progress = below = freeze = 0;
+          action = ready ? 10 : 11;
         }
       } else if (stage_ab == 2) {
         const double thr = pjump_target > 0.0 ? pjump_target * spike_factor : std::numeric_limits<double>::max();
@@ -1669,6 +1858,7 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
           below = 0;
           b_fmin = std::min(f_high, b_fmin + 0.05);
           cfl_scale = std::max(cfl_scale * drop, cfl_min_scale);
+          action = 20;
         } else {
           if (freeze > 0) --freeze;
           below = good ? below + 1 : 0;
@@ -1676,19 +1866,60 @@ extern "C" int aero_hip_run_case(aero_hip_mesh* m, const aero_case_config* c, ae
             if (b_fmin > f_mid + 1.0e-6 && below >= hold_iters) {
               b_fmin = f_mid;
               below = 0;
+              action = 31;
             } else if (b_fmin > f_low + 1.0e-6 && below >= hold_iters) {
               b_fmin = f_low;
               below = 0;
+              action = 32;
             } else if (b_fmin <= f_low + 1.0e-6 && good) {
               ++b_progress;
+              if (action == 0) action = 33;
             }
-            if (good)
+            if (good) {
               cfl_scale = std::min(cfl_scale * restore, 1.0);
-            else if (pjump_target > 0.0 && pjw > pjump_target)
+            } else if (pjump_target > 0.0 && pjw > pjump_target) {
               cfl_scale = std::max(cfl_scale * drop, cfl_min_scale);
+              if (action == 0) action = 21;
+            }
           }
         }
       }
+    }
+    if (out->diagnostics && out->n_iter < out->hist_capacity) {  // euler_solver.cpp:921-1060, :1161-1203
+      double* dr = out->diagnostics + static_cast<size_t>(out->n_iter) * AERO_DIAG_FIELDS;
+      const double* dg = h + 16;
+      const double pref = pressure_ref, pref2 = pressure_ref * pressure_ref + 1.0e-24;
+      dr[AERO_D_MAX_WALL_UN] = dg[kDgMaxUn];
+      dr[AERO_D_MAX_FF_DP] = dg[kDgFfDp];
+      dr[AERO_D_MAX_FF_DRHO] = dg[kDgFfDrho];
+      dr[AERO_D_MAX_FF_DU] = dg[kDgFfDu];
+      dr[AERO_D_MAX_FF_DV] = dg[kDgFfDv];
+      dr[AERO_D_WALL_CP_MIN] = m->wall_count > 0 ? dg[kDgCpMin] : 0.0;
+      dr[AERO_D_WALL_CP_MAX] = m->wall_count > 0 ? dg[kDgCpMax] : 0.0;
+      dr[AERO_D_WALL_CP_NONFINITE] = dg[kDgCpNonfinite];
+      dr[AERO_D_FX_GAUGE] = fg.fx;
+      dr[AERO_D_FY_GAUGE] = fg.fy;
+      const double fxa = -h[4], fya = -h[5];
+      dr[AERO_D_FX_ABS] = fxa;
+      dr[AERO_D_FY_ABS] = fya;
+      dr[AERO_D_CL_ABS] = (-fxa * sa + fya * ca) / (q_inf * sref);
+      dr[AERO_D_CD_ABS] = (fxa * ca + fya * sa) / (q_inf * sref);
+      dr[AERO_D_CM_ABS] = dg[kDgMomentAbs] / (q_inf * sref * chord);
+      dr[AERO_D_LIFT] = fg.lift;
+      dr[AERO_D_DRAG] = fg.drag;
+      dr[AERO_D_PJUMP_ND] = da.nint > 0 ? (dg[kDgPjAll] / static_cast<double>(da.nint)) / pref2 : 0.0;
+      dr[AERO_D_PJUMP_WALL_ND] = n_wall_int > 0 ? (dg[kDgPjWall] / static_cast<double>(n_wall_int)) / pref2 : 0.0;
+      dr[AERO_D_PFLUCT_MAX] = dg[kDgPfMax] / pref;
+      dr[AERO_D_PFLUCT_WALL] = dg[kDgPfWall] / pref;
+      dr[AERO_D_PFLUCT_FARFIELD] = dg[kDgFfDp] / pref;
+      dr[AERO_D_AS_STAGE] = stage_ab;
+      dr[AERO_D_AS_FIX] = rc.all_speed_fix;
+      dr[AERO_D_AS_FMIN] = rc.all_speed_f_min;
+      dr[AERO_D_AS_RAMP] = rc.all_speed_ramp;
+      dr[AERO_D_AS_FREEZE] = freeze;
+      dr[AERO_D_AS_ACTION] = action;
+      dr[AERO_D_CFL_SCALE] = cfl_scale;
+      dr[AERO_D_FIRST_FAILURE_FACE] = hd[4] == INT_MAX ? -1 : hd[4];
     }
     if (out->history && out->n_iter < out->hist_capacity) {
       double* row = out->history + static_cast<size_t>(out->n_iter) * AERO_HIST_FIELDS;

@@ -30,7 +30,7 @@ struct aero_hip_mesh {
   double* iter_out = nullptr;
   double* cp = nullptr;
   aero_dev::WallArgs wall{};
-  double* h_iter = nullptr;  // pinned: 9 doubles + 4 ints
+  double* h_iter = nullptr;  // pinned [64]: iteration scalars [0..9], counters (int) at [10..12], diagnostics [16..]
   std::vector<hipEvent_t> tev;  // per-kernel timing events (3 per residual evaluation)
   int tcap = 0, tcount = 0;
   // Host copies in the reference numbering (unit normals as renormalised at setup), for host-side

@@ -151,7 +151,38 @@ typedef struct aero_run_output {
   double device_seconds;   /* wall time of the iteration loop */
   aero_residual_config* final_config; /* may be NULL: the residual configuration of the last iteration
                                          (ramped floors and all-speed weights, effective preconditioner) */
+  double* diagnostics;     /* may be NULL: [hist_capacity][AERO_DIAG_FIELDS] per-iteration diagnostics, the
+                              state- and controller-dependent columns of euler_diagnostics.csv (below) */
 } aero_run_output;
+
+/* Columns of aero_run_output.diagnostics, one row per history row: the per-iteration quantities of the
+ * reference's euler_diagnostics.csv (euler_solver.cpp:717-738, :921-1139, :1161-1203) that depend on the
+ * state or the all-speed controller, reduced on the device from the iteration's final state.  The other
+ * columns of that file follow from the history rows and the mesh (the runner writes the whole file). */
+enum {
+  AERO_D_MAX_WALL_UN = 0,     /* max |u . n| of the wall-face owner cells */
+  AERO_D_MAX_FF_DP,           /* max |p - p_inf| over farfield-face owner cells */
+  AERO_D_MAX_FF_DRHO,         /* max |rho - rho_inf| */
+  AERO_D_MAX_FF_DU,           /* max |u - u_inf| */
+  AERO_D_MAX_FF_DV,           /* max |v - v_inf| */
+  AERO_D_WALL_CP_MIN, AERO_D_WALL_CP_MAX, AERO_D_WALL_CP_NONFINITE,
+  AERO_D_FX_GAUGE, AERO_D_FY_GAUGE, AERO_D_FX_ABS, AERO_D_FY_ABS,  /* compute_pressure_force_diagnostics */
+  AERO_D_CL_ABS, AERO_D_CD_ABS, AERO_D_CM_ABS,                      /* integrate_pressure_forces, absolute p */
+  AERO_D_LIFT, AERO_D_DRAG,                                         /* gauge */
+  AERO_D_PJUMP_ND,            /* mean (p_o - p_n)^2 over interior faces / (rho_inf a_inf^2)^2 */
+  AERO_D_PJUMP_WALL_ND,       /* the same over interior faces touching a wall-adjacent cell */
+  AERO_D_PFLUCT_MAX,          /* max |p - p_inf| / (rho_inf a_inf^2) over all cells */
+  AERO_D_PFLUCT_WALL,         /* ... over wall-adjacent cells */
+  AERO_D_PFLUCT_FARFIELD,     /* max_ff_dp / (rho_inf a_inf^2) */
+  AERO_D_AS_STAGE,            /* all-speed controller stage: 0 single, 1 A, 2 B */
+  AERO_D_AS_FIX, AERO_D_AS_FMIN, AERO_D_AS_RAMP,  /* effective all-speed fix, f_min, ramp weight */
+  AERO_D_AS_FREEZE,           /* freeze counter */
+  AERO_D_AS_ACTION,           /* controller action code (euler_solver.cpp:1070-1139) */
+  AERO_D_CFL_SCALE,           /* adaptive CFL scale after this iteration */
+  AERO_D_FIRST_FAILURE_FACE,  /* first face that fell back to first order in the iteration's first stage with
+                                 one (-1 if none) */
+  AERO_DIAG_FIELDS
+};
 
 typedef struct aero_hip_mesh aero_hip_mesh; /* opaque device mesh + state + stream */
 

@@ -58,8 +58,19 @@ enum HistField : int {
   kHistFields
 };
 
+// Per-iteration diagnostics row of euler_diagnostics.csv: the columns that depend on the iteration's state
+// or on the controller (euler_solver.cpp:971-1139, printed at :1171-1203), in the order of the device
+// library's AERO_D_* fields.
+enum DiagField : int {
+  kDMaxWallUn = 0, kDFfDp, kDFfDrho, kDFfDu, kDFfDv, kDCpMin, kDCpMax, kDCpNonfinite, kDFxGauge, kDFyGauge,
+  kDFxAbs, kDFyAbs, kDClAbs, kDCdAbs, kDCmAbs, kDLift, kDDrag, kDPjump, kDPjumpWall, kDPfluctMax, kDPfluctWall,
+  kDPfluctFarfield, kDAsStage, kDAsFix, kDAsFmin, kDAsRamp, kDAsFreeze, kDAsAction, kDCflScale, kDFirstFailure,
+  kDiagFields
+};
+
 template <class R> struct RunOut {
   std::vector<R> hist;   // [n_iter][kHistFields]
+  std::vector<R> diag;   // [n_iter][kDiagFields] when the run was asked for diagnostics
   int n_iter = 0;
   bool nonfinite = false;
   std::vector<R> U;      // final conserved AoS [N][5]
@@ -110,7 +121,8 @@ template <class R> PrecondCfg<R> build_precond(const CaseCfg<R>& c, bool enabled
 // The driver.  `m` supplies the mesh; its normals are re-normalised into a private copy first
 // (euler_solver.cpp:475-478).  threaded: the stage gradients and residuals run on all OpenMP threads
 // (cell_gradients_threaded / assemble_residual_threaded: the serial results bit for bit).
-template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool threaded = false) {
+template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool threaded = false,
+                                      bool diagnostics = false) {
   if (c.iterations <= 0) throw std::invalid_argument("Euler iterations must be positive.");
   if (c.rk_stages != 1 && c.rk_stages != 3) throw std::invalid_argument("Euler pseudo-time rk_stages must be 1 or 3.");
   std::vector<R> normals(m.normal, m.normal + 3 * static_cast<std::size_t>(m.num_faces));
@@ -340,6 +352,10 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool t
     out.forces = fg;
     monitor.update_forces(fg);
     const bool window_stable = monitor.forces_stable();
+    const bool stage_row_fix = as_fix_eff;  // the effective settings of this iteration, printed after the
+    const R stage_row_fmin = as_fmin_eff;   // controller has moved on (:1193-1196)
+    const R stage_row_ramp = as_w;
+    int action = 0;
 
     if (staged) {  // :1026-1139
       double pj_wall = 0.0;
@@ -365,6 +381,7 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool t
           b_progress = 0;
           below = 0;
           freeze = 0;
+          action = ready ? 10 : 11;
         }
       } else if (stage_ab == 2) {
         const R spike_thr = pjump_target > R(0) ? pjump_target * spike_factor : std::numeric_limits<R>::max();
@@ -375,6 +392,7 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool t
           below = 0;
           b_fmin = smin(f_high, b_fmin + AO_K(0.05));
           cfl_scale = smax(cfl_scale * drop, cfl_min_scale);
+          action = 20;
         } else {
           if (freeze > 0) --freeze;
           if (good)
@@ -385,19 +403,99 @@ template <class R> RunOut<R> run_case(MeshView<R> m, const CaseCfg<R>& c, bool t
             if (b_fmin > f_mid + AO_K(1.0e-6) && below >= hold_iters) {
               b_fmin = f_mid;
               below = 0;
+              action = 31;
             } else if (b_fmin > f_low + AO_K(1.0e-6) && below >= hold_iters) {
               b_fmin = f_low;
               below = 0;
+              action = 32;
             } else if (b_fmin <= f_low + AO_K(1.0e-6) && good) {
               ++b_progress;
+              action = 33;
             }
-            if (good)
+            if (good) {
               cfl_scale = smin(cfl_scale * restore, AO_K(1.0));
-            else if (pjump_target > R(0) && pj > pjump_target)
+            } else if (pjump_target > R(0) && pj > pjump_target) {
               cfl_scale = smax(cfl_scale * drop, cfl_min_scale);
+              if (action == 0) action = 21;
+            }
           }
         }
       }
+    }
+
+    if (diagnostics) {  // :971-1066 over the wall, farfield and interior faces of the updated state
+      R d[kDiagFields] = {};
+      const R q = AO_K(0.5) * ref.rho_inf * ref.speed_inf * ref.speed_inf + AO_K(1.0e-12);
+      R cp_min = std::numeric_limits<R>::max(), cp_max = std::numeric_limits<R>::lowest();
+      int cp_bad = 0;
+      for (int i = 0; i < m.wall_count; ++i) {
+        const int f = m.wall_start + i;
+        const Prim<R>& p = prim[m.owner[f]];
+        d[kDMaxWallUn] = smax(d[kDMaxWallUn], std::abs(p.u * m.normal[3 * f] + p.v * m.normal[3 * f + 1]));
+        const R cp = (p.p - c.p_inf) / q;
+        if (std::isfinite(cp)) {
+          cp_min = smin(cp_min, cp);
+          cp_max = smax(cp_max, cp);
+        } else {
+          ++cp_bad;
+        }
+      }
+      d[kDCpMin] = cp_min;
+      d[kDCpMax] = cp_max;
+      d[kDCpNonfinite] = static_cast<R>(cp_bad);
+      for (int f = 0; f < m.num_faces; ++f) {
+        if (m.bc[f] != kFarfield) continue;
+        const Prim<R>& p = prim[m.owner[f]];
+        d[kDFfDp] = smax(d[kDFfDp], std::abs(p.p - c.p_inf));
+        d[kDFfDrho] = smax(d[kDFfDrho], std::abs(p.rho - rho_inf));
+        d[kDFfDu] = smax(d[kDFfDu], std::abs(p.u - pinf.u));
+        d[kDFfDv] = smax(d[kDFfDv], std::abs(p.v - pinf.v));
+      }
+      const Coeffs<R> fa = integrate_forces(m, cell_p, ref, false);
+      d[kDFxGauge] = fg.fx;
+      d[kDFyGauge] = fg.fy;
+      d[kDFxAbs] = fa.fx;
+      d[kDFyAbs] = fa.fy;
+      d[kDClAbs] = fa.cl;
+      d[kDCdAbs] = fa.cd;
+      d[kDCmAbs] = fa.cm;
+      d[kDLift] = fg.lift;
+      d[kDDrag] = fg.drag;
+      double pj_all = 0.0, pj_wall = 0.0;
+      int n_all = 0, n_wall = 0;
+      for (int f = 0; f < m.num_faces; ++f) {
+        const int o = m.owner[f], nb = m.neighbor[f];
+        if (nb < 0) continue;
+        const R dp = prim[o].p - prim[nb].p;
+        const double e = static_cast<double>(dp) * static_cast<double>(dp);
+        pj_all += e;
+        ++n_all;
+        if (wall_adj[o] != 0u || wall_adj[nb] != 0u) {
+          pj_wall += e;
+          ++n_wall;
+        }
+      }
+      const double pref2 = static_cast<double>(pressure_ref) * static_cast<double>(pressure_ref) + 1.0e-24;
+      d[kDPjump] = n_all > 0 ? static_cast<R>((pj_all / n_all) / pref2) : R(0);
+      d[kDPjumpWall] = n_wall > 0 ? static_cast<R>((pj_wall / n_wall) / pref2) : R(0);
+      R dp_all = 0, dp_wall = 0;
+      for (int i = 0; i < N; ++i) {
+        const R dp = std::abs(prim[i].p - c.p_inf);
+        dp_all = smax(dp_all, dp);
+        if (wall_adj[i] != 0u) dp_wall = smax(dp_wall, dp);
+      }
+      d[kDPfluctMax] = dp_all / pressure_ref;
+      d[kDPfluctWall] = dp_wall / pressure_ref;
+      d[kDPfluctFarfield] = d[kDFfDp] / pressure_ref;
+      d[kDAsStage] = static_cast<R>(stage_ab);
+      d[kDAsFix] = stage_row_fix ? R(1) : R(0);
+      d[kDAsFmin] = stage_row_fmin;
+      d[kDAsRamp] = stage_row_ramp;
+      d[kDAsFreeze] = static_cast<R>(freeze);
+      d[kDAsAction] = static_cast<R>(action);
+      d[kDCflScale] = cfl_scale;
+      d[kDFirstFailure] = static_cast<R>(iter_diag.first_failure_face);
+      out.diag.insert(out.diag.end(), d, d + kDiagFields);
     }
 
     R row[kHistFields];

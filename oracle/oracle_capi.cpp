@@ -324,11 +324,12 @@ void forces_t(const ao_mesh* m, const R* p, const R* ref7, int subtract, R* out8
 // threaded: the stage residuals on all OpenMP threads (the serial run's results bit for bit).
 template <class R>
 int run_case_t(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, R* hist, R* U, R* res, R* lam, R* forces8,
-               R* cp, int* nonfinite, char* err, int errlen, bool threaded = false) {
+               R* cp, int* nonfinite, char* err, int errlen, bool threaded = false, R* diag = nullptr) {
   try {
-    const RunOut<R> o = run_case(view<R>(m), ccfg<R>(c), threaded);
+    const RunOut<R> o = run_case(view<R>(m), ccfg<R>(c), threaded, diag != nullptr);
     const int rows = std::min(o.n_iter, hist_cap);
     std::memcpy(hist, o.hist.data(), static_cast<std::size_t>(rows) * kHistFields * sizeof(R));
+    if (diag) std::memcpy(diag, o.diag.data(), static_cast<std::size_t>(rows) * kDiagFields * sizeof(R));
     if (U) std::memcpy(U, o.U.data(), o.U.size() * sizeof(R));
     if (res && !o.res.empty()) std::memcpy(res, o.res.data(), o.res.size() * sizeof(R));
     if (lam && !o.lam.empty()) std::memcpy(lam, o.lam.data(), o.lam.size() * sizeof(R));
@@ -497,6 +498,10 @@ extern "C" int ao_max_threads() {
                                             T* res, T* lam, T* forces8, T* cp, int* nonfinite, char* err,                \
                                             int errlen) {                                                                \
     return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen, true);                 \
+  }                                                                                                                      \
+  extern "C" int ao_run_case_diag_##SUF(const ao_mesh* m, const ao_case_cfg* c, int hist_cap, T* hist, T* U, T* res,   \
+                                        T* lam, T* forces8, T* cp, int* nonfinite, T* diag, char* err, int errlen) {     \
+    return run_case_t<T>(m, c, hist_cap, hist, U, res, lam, forces8, cp, nonfinite, err, errlen, true, diag);           \
   }                                                                                                                      \
   extern "C" int ao_run_case_cfg_##SUF(const ao_mesh* m, const ao_case_cfg* c, ao_residual_cfg* out) {              \
     return run_case_cfg_t<T>(m, c, out);                                                                                 \

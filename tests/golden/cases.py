@@ -83,7 +83,30 @@ RUNS = {
                                                                     all_speed_pjump_hold_iters=20,
                                                                     all_speed_freeze_iters=10, iterations=160,
                                                                     min_iterations=100)),
+    # The staged controller's other branches: stage A ends on its iteration cap, and a wall p-jump target
+    # near the run's own p-jump level gives spikes (freeze, f_min raise, CFL drop) and CFL drops.
+    "staged_spike": dict(mesh="64x14", env_precond=True, cfg=dict(_LOW_MACH, mach=0.05, precond_on=1,
+                                                                  precond_mach_ref=0.05, precond_farfield_bc=1,
+                                                                  all_speed_flux_fix=1,
+                                                                  all_speed_staged_controller=1,
+                                                                  all_speed_stage_a_min_iters=1000,
+                                                                  all_speed_stage_a_max_iters=30,
+                                                                  all_speed_pjump_wall_target=5.0e-11,
+                                                                  all_speed_pjump_spike_factor=1.1,
+                                                                  all_speed_pjump_hold_iters=20,
+                                                                  all_speed_freeze_iters=10, iterations=160,
+                                                                  min_iterations=100)),
+    # A run past its stability limit: reconstruction failures (first-order fallbacks, the first failing
+    # face) and then the driver's non-finite trap (euler_solver.cpp:908-946).
+    "trap_rk1": dict(mesh="40x8", env_precond=False, cfg=dict(
+        iterations=200, min_iterations=200, rk_stages=1, cfl_start=20.0, cfl_max=20.0, cfl_ramp_iters=0,
+        residual_reduction_target=0.0, force_stability_tol=0.0, flux_scheme=1, limiter=1,
+        startup_first_order_iters=5, mach=0.5, aoa_deg=2.0)),
 }
+
+# Runs that end in the driver's non-finite trap: their final state is not a flow field, so the post-run
+# records of that state are not compared.
+TRAPPED = {"trap_rk1"}
 
 # Residual-evaluation option combinations for the single-evaluation golden vectors:
 # (scheme, limiter, second_order, low_mach_fix, precond, all_speed_fix, char_farfield_bc)

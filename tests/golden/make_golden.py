@@ -329,6 +329,7 @@ def runs(ref) -> dict:
             post = np.zeros(23, np.float32)
             ref.ref_last_run_post(f32p(post))
             dissipation, wall_flux = post_csvs(Path(td))
+            diagnostics = np.array((Path(td) / "euler_diagnostics.csv").read_text().splitlines())
         os.environ.pop("AERO_ENABLE_EXPERIMENTAL_PRECOND", None)
         out[f"run_{name}_hist"] = hist[:rows]
         out[f"run_{name}_U"] = U
@@ -340,6 +341,7 @@ def runs(ref) -> dict:
         out[f"run_{name}_post"] = post
         out[f"run_{name}_dissipation"] = dissipation
         out[f"run_{name}_wall_flux"] = wall_flux
+        out[f"run_{name}_diagnostics"] = diagnostics  # euler_diagnostics.csv as written: header, then one line a row
         print(f"  run {name}: {rows} iterations, cl={forces[5]:.6f} cd={forces[6]:.6f}")
     return out
 

@@ -116,9 +116,13 @@ def gradients(om: OracleMesh, U, gamma=1.4):
     return G
 
 
-def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int, threaded: bool = False):
+DIAG_FIELDS = 30  # the per-iteration diagnostics row (the device library's AERO_D_* fields)
+
+
+def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int, threaded: bool = False, diagnostics: bool = False):
     """The oracle driver; threaded runs the stage residuals on all OpenMP threads (bit for bit the serial
-    run)."""
+    run); diagnostics adds the per-iteration diagnostics rows ("diagnostics", [n_iter, DIAG_FIELDS]) and
+    runs threaded."""
     dt = om.dtype
     hist = np.zeros((hist_cap, HIST_FIELDS), dt)
     U = np.zeros((om.N, 5), dt)
@@ -128,14 +132,23 @@ def run_case(om: OracleMesh, case: CaseCfg, hist_cap: int, threaded: bool = Fals
     cp = np.zeros(max(om.wall_count, 1), dt)
     nonfinite = C.c_int(0)
     err = C.create_string_buffer(512)
-    entry = "ao_run_case_threaded" if threaded else "ao_run_case"
-    n = getattr(oracle(), f"{entry}_{suffix(dt)}")(C.byref(om.c), C.byref(case), hist_cap, ptr(hist), ptr(U),
-                                                       ptr(R), ptr(lam), ptr(forces), ptr(cp), C.byref(nonfinite),
-                                                       err, len(err))
+    if diagnostics:
+        diag = np.zeros((hist_cap, DIAG_FIELDS), dt)
+        n = getattr(oracle(), f"ao_run_case_diag_{suffix(dt)}")(C.byref(om.c), C.byref(case), hist_cap, ptr(hist),
+                                                                ptr(U), ptr(R), ptr(lam), ptr(forces), ptr(cp),
+                                                                C.byref(nonfinite), ptr(diag), err, len(err))
+    else:
+        entry = "ao_run_case_threaded" if threaded else "ao_run_case"
+        n = getattr(oracle(), f"{entry}_{suffix(dt)}")(C.byref(om.c), C.byref(case), hist_cap, ptr(hist), ptr(U),
+                                                           ptr(R), ptr(lam), ptr(forces), ptr(cp), C.byref(nonfinite),
+                                                           err, len(err))
     if n < 0:
         raise RuntimeError(err.value.decode())
-    return dict(history=hist[:n], n_iter=n, conserved=U, residual=R, spectral_radius=lam, forces=forces,
-                cp=cp[: om.wall_count], nonfinite=bool(nonfinite.value))
+    out = dict(history=hist[:n], n_iter=n, conserved=U, residual=R, spectral_radius=lam, forces=forces,
+               cp=cp[: om.wall_count], nonfinite=bool(nonfinite.value))
+    if diagnostics:
+        out["diagnostics"] = diag[:n]
+    return out
 
 
 def time_residual(om: OracleMesh, cfg, U, n_evals: int) -> float:

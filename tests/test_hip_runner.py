@@ -44,7 +44,7 @@ def _csv(path):
     return lines[0].split(","), np.array([[float(x) for x in l.split(",")] for l in lines[1:]])
 
 
-def test_run_case_file_outputs(hip, tmp_path):
+def test_run_case_file_outputs(hip, tmp_path, golden):
     from aero_hip.case_file import euler_case_from_kv, read_native_case
     cfg = tmp_path / "native_case.cfg"
     cfg.write_text(CASE)
@@ -74,6 +74,13 @@ def test_run_case_file_outputs(hip, tmp_path):
     assert np.allclose(cp[:, 3], ref["wall_cp"][order - mesh.wall_start], rtol=1e-5, atol=1e-6)
     assert np.allclose(cp[:, 1], mesh.face_center[order, 0], rtol=1e-5, atol=1e-6)
     assert "backend=hip" in (out / "run.log").read_text()
+    dlines = (out / "euler_diagnostics.csv").read_text().splitlines()
+    golden_header = str(golden("runs")["run_regression_diagnostics"][0])  # the reference's header
+    assert dlines[0] == golden_header and len(dlines) == 61
+    dh, dg = dlines[0].split(","), [l.split(",") for l in dlines[1:]]
+    assert [int(r[0]) for r in dg] == list(range(60))
+    assert all(r[dh.index("all_speed_stage")] == "single" for r in dg)
+    assert np.allclose([float(r[dh.index("Cl")]) for r in dg], ref["history"][:, 5], rtol=1e-5, atol=1e-7)
 
 
 def test_run_case_on_a_mesh_file(hip, tmp_path):

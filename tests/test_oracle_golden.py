@@ -139,3 +139,30 @@ def test_driver_history(golden, oracle_lib, name):
     assert _eq(out["spectral_radius"], g[f"run_{name}_lam"])
     assert _eq(out["forces"], g[f"run_{name}_forces"])
     assert _eq(out["cp"], g[f"run_{name}_cp"])
+
+
+@pytest.mark.parametrize("name", list(cases.RUNS))
+def test_driver_diagnostics_csv(golden, oracle_lib, name):
+    """The FP32 oracle driver's per-iteration diagnostics, assembled into rows and printed by the runner
+    (aero_hip.outputs.diagnostics_lines), reproduce the reference run's euler_diagnostics.csv: the header
+    and every line, character for character."""
+    from aero_hip import CaseConfig
+    from aero_hip import outputs
+    g = golden("runs")
+    run = cases.RUNS[name]
+    m = helpers.mesh(run["mesh"], precision=4)
+    om = ol.OracleMesh(m, F32)
+    out = ol.run_case(om, helpers.case_cfg(run["cfg"], run["env_precond"], F32), run["cfg"]["iterations"],
+                      diagnostics=True)
+    case = CaseConfig(**run["cfg"])
+    lines = outputs.diagnostics_lines(m, case, out)
+    ref = [str(x) for x in g[f"run_{name}_diagnostics"]]
+    assert lines[0] == ref[0], "header"
+    assert len(lines) == len(ref), (len(lines), len(ref))
+    header = ref[0].split(",")
+    bad = {}
+    for i, (a, b) in enumerate(zip(lines[1:], ref[1:])):
+        for k, (x, y) in enumerate(zip(a.split(","), b.split(","))):
+            if x != y:
+                bad.setdefault(header[k], (i, x, y))
+    assert not bad, bad
