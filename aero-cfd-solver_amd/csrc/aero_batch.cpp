@@ -40,6 +40,7 @@ struct aero_hip_batch {
   aero_case_config cfg{};
   hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
   bool frozen = false;               // limiter ratios captured (limiter_freeze_iter reached)
+  bool plain = true;                 // every case has plain face options (flux kernels' PLAIN instantiation)
   std::vector<double> alpha, q_inf, chord;
 };
 
@@ -111,8 +112,8 @@ int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, char* err, si
       AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
-    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->d_geoms, b->d_cases,
-                                          b->d_ctl, b->d_work, b->nwork, b->smax, bs, b->stream),
+    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->plain, b->d_geoms,
+                                          b->d_cases, b->d_ctl, b->d_work, b->nwork, b->smax, bs, b->stream),
              "k_flux_batch");
     cur = bs.out;
     final_buf = bs.out;
@@ -308,6 +309,7 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     fo.mach_min = std::max(cc->precond_mach_min > 0.0 ? cc->precond_mach_min : 1.0e-6, 1.0e-6);
     fo.beta_min = std::max(cc->precond_beta_min > 0.0 ? cc->precond_beta_min : auto_bmin, 1.0e-4);
     fo.beta_max = std::max(cc->precond_beta_max, fo.beta_min);
+    b->plain = b->plain && aero_dev::plain_options(fo);
     cs.cfl_start = cc->cfl_start;
     cs.cfl_max = cc->cfl_max;
     cs.cfl_ramp = cc->cfl_ramp_iters;

@@ -256,9 +256,11 @@ hipError_t launch_iter_reduce(const aero_dev::DevMesh& m, const double* partials
 hipError_t launch_batch_gradient(int BC, int grad, const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
                                  const aero_dev::CaseCtl* ctl, const int2* work, int nwork, int smax, double g, int cur,
                                  hipStream_t s);
-hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const aero_dev::DevMesh* geoms,
-                             const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl, const int2* work,
-                             int nwork, int smax, const aero_dev::BatchStage& bs, hipStream_t s);
+// plain: every case's face options satisfy plain_options (the flux kernels' PLAIN instantiation).
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, bool plain,
+                             const aero_dev::DevMesh* geoms, const aero_dev::BatchCase* cases,
+                             const aero_dev::CaseCtl* ctl, const int2* work, int nwork, int smax,
+                             const aero_dev::BatchStage& bs, hipStream_t s);
 // Frozen limiter capture of every active case at its state U[cur] (gradients in G).
 hipError_t launch_batch_limiter_capture(int BC, int limiter, const aero_dev::DevMesh* geoms,
                                         const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl,

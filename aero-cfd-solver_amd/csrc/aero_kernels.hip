@@ -238,7 +238,8 @@ __device__ __forceinline__ int wave_min_i(int v) {
 // Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
 // MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
 // preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
-template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false>
+// PLAIN: the options have the preconditioner, the low-Mach fix and the all-speed fix off (plain_options).
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false, bool PLAIN = false>
 __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpts& o, const double* __restrict__ U,
                                           const double* __restrict__ G, double* __restrict__ R,
                                           double* __restrict__ lam, int* __restrict__ diag, const StageArgs& st,
@@ -289,14 +290,14 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
         }
       }
       bool reverted;
-      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny, dox, doy,
-                                                         dnx, dny, o, &reverted, fr);
+      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN, PLAIN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny,
+                                                                dox, doy, dnx, dny, o, &reverted, fr);
       if (reverted && (meta & kMetaPrimary)) {
         ++n_fo;
         first_fail = min(first_fail, m.s_face[s]);
       }
     } else {
-      F = boundary_face<SCHEME>(load_u(U, ld, co), (meta & kMetaBcMask) == 2, nx, ny, o);
+      F = boundary_face<SCHEME, PLAIN>(load_u(U, ld, co), (meta & kMetaBcMask) == 2, nx, ny, o);
     }
     if (F.fallback && (meta & kMetaPrimary)) ++n_diff;
     f0[i] = F.f0 * A;
@@ -361,7 +362,7 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
       }
       const C4 us = load_u(U, ld, c);
       double p3 = r3;
-      if (o.precond) {  // preconditioning.cpp:186-199, on the stage state's primitive
+      if (!PLAIN && o.precond) {  // preconditioning.cpp:186-199, on the stage state's primitive
         const P4 wp = c2p(us, g);
         const double bt = cell_beta(wp, o);
         const double b2 = bt * bt;
@@ -457,14 +458,14 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
 #define AERO_FLUX_WAVES_PER_EU 4
 #endif
 
-template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, int MODE, bool FROZEN = false, bool PLAIN = false>
 __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux(DevMesh m, FaceOpts o, const double* __restrict__ U,
                                               const double* __restrict__ G, double* __restrict__ R,
                                               double* __restrict__ lam, int* __restrict__ diag, StageArgs st) {
   extern __shared__ double lds[];
   const int b = block_of(m);
   if (b >= m.nblk) return;
-  flux_body<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN>(m, b, o, U, G, R, lam, diag, st, lds);
+  flux_body<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN, PLAIN>(m, b, o, U, G, R, lam, diag, st, lds);
 }
 
 // The residual on the face-coloured layout: one thread per face of colour range [begin, end); the face
@@ -885,7 +886,7 @@ __global__ __launch_bounds__(BC) void k_gradient_batch(const DevMesh* __restrict
   gradient_body<BC, GK>(geoms[cs.geom], w.y, cs.U[ring_buf(cur, ctl[w.x].iter)], cs.G, g, lds);
 }
 
-template <int BC, int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false>
+template <int BC, int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, bool PLAIN = false>
 __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux_batch(const DevMesh* __restrict__ geoms,
                                                     const BatchCase* __restrict__ cases,
                                                     const CaseCtl* __restrict__ ctl, const int2* __restrict__ work,
@@ -909,8 +910,8 @@ __global__ __launch_bounds__(BC, AERO_FLUX_WAVES_PER_EU) void k_flux_batch(const
   st.Un = cs.U[ring_buf(bs.base, k.iter)];
   st.Unext = cs.U[ring_buf(bs.out, k.iter)];
   st.partials = cs.partials;
-  flux_body<BC, SCHEME, LIMITER, SECOND, 1, FROZEN>(geoms[cs.geom], w.y, o, cs.U[ring_buf(bs.cur, k.iter)], cs.G,
-                                                    nullptr, nullptr, cs.diag, st, lds);
+  flux_body<BC, SCHEME, LIMITER, SECOND, 1, FROZEN, PLAIN>(geoms[cs.geom], w.y, o, cs.U[ring_buf(bs.cur, k.iter)],
+                                                           cs.G, nullptr, nullptr, cs.diag, st, lds);
 }
 
 template <int BC, int LIMITER>
@@ -1046,8 +1047,12 @@ static hipError_t launch_flux_t(const DevMesh& m, const FaceOpts& o, const doubl
                                 double* lam, int* diag, const StageArgs& st, hipStream_t s) {
   // The fused stage tail reuses the slot staging area for 4 doubles per wavefront.
   const size_t lds = static_cast<size_t>(std::max(m.smax * 5, 4 * (BC / 64))) * sizeof(double);
-  hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN>), dim3(m.grid), dim3(BC), lds, s, m, o, U, G,
-                     R, lam, diag, st);
+  if (plain_options(o))
+    hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN, true>), dim3(m.grid), dim3(BC), lds, s, m, o,
+                       U, G, R, lam, diag, st);
+  else
+    hipLaunchKernelGGL((k_flux<BC, SCHEME, LIMITER, SECOND, MODE, FROZEN, false>), dim3(m.grid), dim3(BC), lds, s, m,
+                       o, U, G, R, lam, diag, st);
   return hipGetLastError();
 }
 
@@ -1222,22 +1227,23 @@ hipError_t launch_batch_gradient(int BC, int grad, const DevMesh* geoms, const B
   return launch_batch_gradient_gk<0>(BC, geoms, cases, ctl, work, nwork, smax, g, cur, s);
 }
 
-template <int BC>
+template <int BC, bool PLAIN>
 static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
                                        const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork,
                                        size_t lds, const BatchStage& bs, hipStream_t s) {
   if (second && frozen) {  // the frozen ratios replace the limiter
     if (scheme == 0)
-      hipLaunchKernelGGL((k_flux_batch<BC, 0, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work,
-                         bs);
+      hipLaunchKernelGGL((k_flux_batch<BC, 0, 0, true, true, PLAIN>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl,
+                         work, bs);
     else
-      hipLaunchKernelGGL((k_flux_batch<BC, 1, 0, true, true>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work,
-                         bs);
+      hipLaunchKernelGGL((k_flux_batch<BC, 1, 0, true, true, PLAIN>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl,
+                         work, bs);
     return hipGetLastError();
   }
 #define AERO_BFLUX(SC, LI, SE)                                                                              \
   if (scheme == SC && (limiter == LI || !SE) && second == SE) {                                             \
-    hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, work, bs);    \
+    hipLaunchKernelGGL((k_flux_batch<BC, SC, LI, SE, false, PLAIN>), dim3(nwork), dim3(BC), lds, s, geoms, cases, ctl, \
+                       work, bs);                                                                           \
     return hipGetLastError();                                                                               \
   }
   AERO_BFLUX(0, 0, false)
@@ -1250,19 +1256,23 @@ static hipError_t launch_batch_flux_bc(int scheme, int limiter, bool second, boo
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, const DevMesh* geoms,
-                             const BatchCase* cases, const CaseCtl* ctl, const int2* work, int nwork, int smax,
-                             const BatchStage& bs, hipStream_t s) {
+hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool frozen, bool plain,
+                             const DevMesh* geoms, const BatchCase* cases, const CaseCtl* ctl, const int2* work,
+                             int nwork, int smax, const BatchStage& bs, hipStream_t s) {
   const size_t lds = std::max<size_t>(static_cast<size_t>(smax) * 5, 4 * (BC / 64)) * sizeof(double);
+#define AERO_BFLUX_BC(BCV)                                                                                         \
+  case BCV:                                                                                                        \
+    return plain ? launch_batch_flux_bc<BCV, true>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, \
+                                                   lds, bs, s)                                                     \
+                 : launch_batch_flux_bc<BCV, false>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, \
+                                                    lds, bs, s);
   switch (BC) {
-    case 128:
-      return launch_batch_flux_bc<128>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
-    case 256:
-      return launch_batch_flux_bc<256>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
-    case 512:
-      return launch_batch_flux_bc<512>(scheme, limiter, second, frozen, geoms, cases, ctl, work, nwork, lds, bs, s);
+    AERO_BFLUX_BC(128)
+    AERO_BFLUX_BC(256)
+    AERO_BFLUX_BC(512)
     default: return hipErrorInvalidValue;
   }
+#undef AERO_BFLUX_BC
 }
 
 hipError_t launch_batch_limiter_capture(int BC, int limiter, const DevMesh* geoms, const BatchCase* cases,

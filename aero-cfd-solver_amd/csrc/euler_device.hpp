@@ -86,6 +86,13 @@ __host__ __device__ __forceinline__ double quantize_ratio(double r) {
 }
 using FaceOpts = FaceOptsT<double>;
 
+// The face functions below take a compile-time PLAIN switch: PLAIN instantiations are the ones for
+// options with the low-Mach preconditioner, the low-Mach dissipation fix and the all-speed HLLC fix all
+// off (plain_options), where those branches are known not taken; the arithmetic is the same.
+template <class T> __host__ __device__ __forceinline__ bool plain_options(const FaceOptsT<T>& o) {
+  return !o.precond && !o.low_mach_fix && !o.as_fix;
+}
+
 template <class T> __device__ __forceinline__ C4T<T> p2c(const P4T<T>& w, double g) {  // euler_flux.cpp:101-113
   C4T<T> c;
   c.r = dmax(w.r, kFloor);
@@ -154,9 +161,9 @@ __device__ __forceinline__ T cell_beta(const P4T<T>& p, const FaceOptsT<T>& o) {
   return dclamp(t, bmin, bmax);
 }
 
-template <class T>
+template <bool PLAIN = false, class T>
 __device__ __forceinline__ T face_beta(const C4T<T>& l, const C4T<T>& r, const FaceOptsT<T>& o) {  // :158-173
-  if (!o.precond) return T(1.0);
+  if (PLAIN || !o.precond) return T(1.0);
   const P4T<T> pl = c2p(l, o.g);
   const P4T<T> pr = c2p(r, o.g);
   P4T<T> a;
@@ -190,11 +197,11 @@ template <class T> __device__ __forceinline__ T smooth_max(T a, T b) {  // resid
 }
 
 // Face acoustic scale (residual_assembly.cpp:31-51).
-template <class T>
+template <bool PLAIN = false, class T>
 __device__ __forceinline__ T acoustic_scale(const C4T<T>& l, const C4T<T>& r, T fb, const FaceOptsT<T>& o) {
   T s = 1.0;
-  if (o.precond) s = dclamp(fb, T(1.0e-4), T(1.0));
-  if (o.low_mach_fix) s = dmin(s, low_mach_scale(l, r, o));
+  if (!PLAIN && o.precond) s = dclamp(fb, T(1.0e-4), T(1.0));
+  if (!PLAIN && o.low_mach_fix) s = dmin(s, low_mach_scale(l, r, o));
   s = dclamp(s, T(1.0e-4), T(1.0));
   const P4T<T> pl = c2p(l, o.g);
   const P4T<T> pr = c2p(r, o.g);
@@ -206,9 +213,9 @@ __device__ __forceinline__ T acoustic_scale(const C4T<T>& l, const C4T<T>& r, T 
 }
 
 // All-speed HLLC pressure scale (euler_flux.cpp:81-98, :177-201).
-template <class T>
+template <bool PLAIN = false, class T>
 __device__ __forceinline__ T all_speed_scale(const P4T<T>& pl, const P4T<T>& pr, T nx, T ny, const FaceOptsT<T>& o) {
-  if (!o.as_fix) return T(1.0);
+  if (PLAIN || !o.as_fix) return T(1.0);
   const T unl = pl.u * nx + pl.v * ny;
   const T unr = pr.u * nx + pr.v * ny;
   const T al = dmax(sound(pl, o.g), kFloor);
@@ -265,7 +272,7 @@ template <class T> __device__ __forceinline__ T sign_floor(T v) {  // euler_flux
   return (v >= 0.0) ? T(kFloor) : T(-kFloor);
 }
 
-template <class T>
+template <bool PLAIN = false, class T>
 __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx, T ny, T s,
                                          const FaceOptsT<T>& o) {  // euler_flux.cpp:241-364
   const double g = o.g;
@@ -275,7 +282,7 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
   const T unr = pr.u * nx + pr.v * ny;
   const T al = sound(pl, g);
   const T ar = sound(pr, g);
-  const T ps = all_speed_scale(pl, pr, nx, ny, o);
+  const T ps = all_speed_scale<PLAIN>(pl, pr, nx, ny, o);
   const T pavg = 0.5 * (pl.p + pr.p);
   const T plc = pavg + ps * (pl.p - pavg);
   const T prc = pavg + ps * (pr.p - pavg);
@@ -367,10 +374,10 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
   return F;
 }
 
-template <int SCHEME, class T>
+template <int SCHEME, bool PLAIN = false, class T>
 __device__ __forceinline__ FluxT<T> face_flux(const C4T<T>& l, const C4T<T>& r, T nx, T ny, T s,
                                               const FaceOptsT<T>& o) {  // euler_flux.cpp:366-390
-  if (SCHEME == 1) return hllc(l, r, nx, ny, s, o);
+  if (SCHEME == 1) return hllc<PLAIN>(l, r, nx, ny, s, o);
   return rusanov(l, r, nx, ny, s, o.g);
 }
 

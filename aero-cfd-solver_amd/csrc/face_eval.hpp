@@ -78,7 +78,7 @@ __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, 
 // Interior face with the first-order fallback (residual_assembly.cpp:107-137), the reconstruction
 // increments taken from src.  *reverted reports the fallback to cell values.
 // FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, class T, class Src>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, bool PLAIN, class T, class Src>
 __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4T<T>& un, const Src& src, T nx, T ny,
                                                       const FaceOptsT<T>& o, bool* reverted, const double* fr) {
   const double g = o.g;
@@ -111,19 +111,19 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
   }
   const C4T<T> ul = p2c(L, g);
   const C4T<T> ur = p2c(Rt, g);
-  const T fb = face_beta(ul, ur, o);
-  const T sc = acoustic_scale(ul, ur, fb, o);
-  return face_flux<SCHEME>(ul, ur, nx, ny, sc, o);
+  const T fb = face_beta<PLAIN>(ul, ur, o);
+  const T sc = acoustic_scale<PLAIN>(ul, ur, fb, o);
+  return face_flux<SCHEME, PLAIN>(ul, ur, nx, ny, sc, o);
 }
 
 // Interior face (residual_assembly.cpp:107-137).  go / gn are the cells' primitive gradients (rho_x,
 // rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the vectors from the cell centres to the face centre.
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, class T>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, bool PLAIN = false, class T>
 __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
                                                   T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
                                                   bool* reverted, const double* fr = nullptr) {
   const GradIncr<T> src{go, gn, dox, doy, dnx, dny};
-  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, src, nx, ny, o, reverted, fr);
+  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN, PLAIN>(uo, un, src, nx, ny, o, reverted, fr);
 }
 
 // The same face from its reconstruction increments ao / an (owner / neighbour g . d of rho, u, v, p):
@@ -134,12 +134,12 @@ __device__ __forceinline__ FluxT<T> interior_face_incr(const C4T<T>& uo, const C
                                                        T nx, T ny, const FaceOptsT<T>& o, bool* reverted,
                                                        const double* fr = nullptr) {
   const GivenIncr<T> src{ao, an};
-  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN>(uo, un, src, nx, ny, o, reverted, fr);
+  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN, false>(uo, un, src, nx, ny, o, reverted, fr);
 }
 
 // Boundary face: slip wall or farfield on the owner's first-order state (residual_assembly.cpp:138-166,
 // boundary_conditions.cpp:81-123).
-template <int SCHEME, class T>
+template <int SCHEME, bool PLAIN = false, class T>
 __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
   const double g = o.g;
   const C4T<T> in = p2c(c2p(uo, g), g);
@@ -150,8 +150,8 @@ __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T
   ext.ru = wall ? in.ru : o.ff.ru;
   ext.rv = wall ? in.rv : o.ff.rv;
   ext.E = wall ? in.E : o.ff.E;
-  const T fb = face_beta(in, ext, o);
-  const T sc = acoustic_scale(in, ext, fb, o);
+  const T fb = face_beta<PLAIN>(in, ext, o);
+  const T sc = acoustic_scale<PLAIN>(in, ext, fb, o);
   FluxT<T> F;
   if (wall) {  // boundary_conditions.cpp:97-111
     const P4T<T> p = c2p(in, g);
@@ -166,7 +166,7 @@ __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T
   } else {  // boundary_conditions.cpp:113-122
     C4T<T> bs = ext;
     if (o.char_bc) bs = char_farfield_state(in, nx, ny, sc, o);
-    F = face_flux<SCHEME>(in, bs, nx, ny, sc, o);
+    F = face_flux<SCHEME, PLAIN>(in, bs, nx, ny, sc, o);
   }
   return F;
 }
