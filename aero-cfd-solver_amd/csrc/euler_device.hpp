@@ -199,6 +199,9 @@ template <class T> __device__ __forceinline__ T smooth_max(T a, T b) {  // resid
 // Face acoustic scale (residual_assembly.cpp:31-51).
 template <bool PLAIN = false, class T>
 __device__ __forceinline__ T acoustic_scale(const C4T<T>& l, const C4T<T>& r, T fb, const FaceOptsT<T>& o) {
+  // With the preconditioner and the low-Mach fix off, s stays 1 and max(1, clamp(m, 1e-4, 1)) is 1 (a NaN
+  // m also gives 1): the stabilisation floor acts only through those two options.
+  if (PLAIN) return T(1.0);
   T s = 1.0;
   if (!PLAIN && o.precond) s = dclamp(fb, T(1.0e-4), T(1.0));
   if (!PLAIN && o.low_mach_fix) s = dmin(s, low_mach_scale(l, r, o));
