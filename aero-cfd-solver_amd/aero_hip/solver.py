@@ -557,6 +557,15 @@ class HipBatch:
         _lib.check(self.lib.aero_hip_batch_set_limiter_ratios(self.h, i, _pd(r), err, len(err)), err,
                    "batch set_limiter_ratios")
 
+    def set_group_bytes(self, group_bytes: int = 0) -> int:
+        """Iterate the active cases in groups of at most group_bytes hot bytes (0: the default budget sized to
+        the Infinity Cache); returns the number of groups (aero_hip_batch_set_group_bytes)."""
+        n = C.c_int32()
+        err = _lib.errbuf()
+        _lib.check(self.lib.aero_hip_batch_set_group_bytes(self.h, int(group_bytes), C.byref(n), err, len(err)), err,
+                   "batch set_group_bytes")
+        return n.value
+
     def limiter_ratios(self, i: int) -> np.ndarray:
         """Case i's frozen limiter ratios [8, F] (batches created with a freeze iteration)."""
         mesh = self.meshes[self.cases[i][0]]

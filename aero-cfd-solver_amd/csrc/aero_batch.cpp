@@ -22,8 +22,22 @@ using aero_dev::CaseCtl;
 using aero_dev::DevMesh;
 using aero_dev::FaceOpts;
 
+// Cases iterate in groups sized to the last-level cache: a group's hot per-case data (state ring, gradients,
+// frozen ratios) stays in MI355X's 256 MB Infinity Cache while the group's kernels follow one another, where
+// one pass over every case of a large batch would stream all of it from HBM at every launch.  Groups run
+// one after another over the same iterations; a case's arithmetic does not depend on the other cases, so
+// results are bitwise the same for any grouping.
+struct CaseGroup {
+  int w0 = 0, nw = 0;  // range of the device work list (padded to a multiple of 8 items)
+  int c0 = 0, nc = 0;  // range of the device case list
+  hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
+};
+
+// Default budget of a group's hot bytes: the Infinity Cache less room for the shared geometry.
+constexpr size_t kGroupBytesDefault = static_cast<size_t>(192) << 20;
+
 struct aero_hip_batch {
-  int ncases = 0, ngeom = 0, BC = 256, grad = 0, smax = 1, nwork = 0;
+  int ncases = 0, ngeom = 0, BC = 256, grad = 0, smax = 1, nwork = 0, work_cap = 0;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   std::vector<aero_hip_mesh*> geoms;
@@ -32,13 +46,14 @@ struct aero_hip_batch {
   DevMesh* d_geoms = nullptr;
   BatchCase* d_cases = nullptr;
   CaseCtl* d_ctl = nullptr;
-  int2* d_work = nullptr;          // work list of the active cases (capacity: every case's blocks)
-  std::vector<int2> h_list;        // every (case, block) item, per geometry block-major / case-minor
+  int2* d_work = nullptr;          // work list of the active cases, group-major (capacity work_cap)
+  int* d_gcases = nullptr;         // case indices of the groups, group-major
+  std::vector<CaseGroup> groups;   // the active cases' groups
+  size_t group_bytes = kGroupBytesDefault;
   std::vector<char> listed;        // the cases the device work list holds
   double* d_hist = nullptr;
   int hist_cap = 0;
   aero_case_config cfg{};
-  hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
   bool frozen = false;               // limiter ratios captured (limiter_freeze_iter reached)
   bool plain = true;                 // every case has plain face options (flux kernels' PLAIN instantiation)
   std::vector<double> alpha, q_inf, chord;
@@ -72,12 +87,20 @@ template <class T> hipError_t balloc(aero_hip_batch* b, T** p, size_t n) {
   return e;
 }
 
+void destroy_graphs(aero_hip_batch* b) {
+  for (CaseGroup& gr : b->groups)
+    for (auto& row : gr.graph)
+      for (auto& gx : row)
+        if (gx) {
+          (void)hipGraphExecDestroy(gx);
+          gx = nullptr;
+        }
+}
+
 void free_batch(aero_hip_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (auto& row : b->graph)
-    for (auto& g : row)
-      if (g) (void)hipGraphExecDestroy(g);
+  destroy_graphs(b);
   for (void* p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -89,10 +112,12 @@ double clampd(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) 
 // The ring buffer that holds a case's current state between iterations (aero_internal.hpp kRingParity).
 int state_sel(const aero_case_config& c) { return c.rk_stages == 3 ? 0 : aero_dev::kRingParity; }
 
-// One pseudo-time iteration of every active case, enqueued on the batch stream.
-int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, char* err, size_t errlen) {
+// One pseudo-time iteration of the active cases of group gr, enqueued on the batch stream.
+int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, const CaseGroup& gr, char* err, size_t errlen) {
   const aero_case_config& c = b->cfg;
-  AERO_TRY(aero_host::launch_batch_init_diag(b->d_cases, b->ncases, b->stream), "k_batch_init_diag");
+  const int* list = b->d_gcases + gr.c0;
+  const int2* work = b->d_work + gr.w0;
+  AERO_TRY(aero_host::launch_batch_init_diag(b->d_cases, list, gr.nc, b->stream), "k_batch_init_diag");
   // Ring: RK3 starts and ends in buffer 0 (stage 2 updates in place: each thread reads U^n of its own cell
   // before writing it); RK1 ping-pongs between buffers 0 and 1 by each case's own iteration count.
   const int start = state_sel(c);
@@ -109,47 +134,90 @@ int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, char* err, si
     else
       bs.out = aero_dev::kRingParityNext;
     if (second)
-      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, b->d_work, b->nwork,
+      AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, work, gr.nw,
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
     AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->plain, b->d_geoms,
-                                          b->d_cases, b->d_ctl, b->d_work, b->nwork, b->smax, bs, b->stream),
+                                          b->d_cases, b->d_ctl, work, gr.nw, b->smax, bs, b->stream),
              "k_flux_batch");
     cur = bs.out;
     final_buf = bs.out;
   }
-  AERO_TRY(aero_host::launch_batch_reduce(b->d_cases, b->d_ctl, b->ncases, c.gamma, final_buf, b->stream),
+  AERO_TRY(aero_host::launch_batch_reduce(b->d_cases, b->d_ctl, list, gr.nc, c.gamma, final_buf, b->stream),
            "k_batch_reduce");
   return 0;
 }
 
-// The device work list: the listed cases' items of h_list, spread so that each XCD (workgroup index mod 8)
-// receives a contiguous run.  Kernels launched over it reach only these cases, so cases that stopped cost
-// nothing; the graphs, whose grids depend on the list length, are rebuilt.
+// Hot bytes of case i while it iterates: three state buffers and the gradients per cell, the frozen ratios'
+// 16-byte record per face slot.
+size_t case_hot_bytes(const aero_hip_batch* b, int i) {
+  const aero_hip_mesh* m = b->geoms[b->spec[i].mesh_index];
+  return static_cast<size_t>(m->N) * (3 * 32 + 64) + static_cast<size_t>(m->S) * 16;
+}
+
+// Work-list items of every case, the most a grouping can hold: each group pads to a multiple of 8.
+int work_capacity(const aero_hip_batch* b) {
+  size_t w = 0;
+  for (int i = 0; i < b->ncases; ++i) w += static_cast<size_t>(b->geoms[b->spec[i].mesh_index]->nblk);
+  return static_cast<int>(w + 8 * static_cast<size_t>(b->ncases) + 8);
+}
+
+// The device work list of the active cases: the cases, in index order, form groups of at most group_bytes
+// hot bytes (a case larger than that is a group of its own); within a group, items are per geometry
+// block-major / case-minor (the cases of a block share its face data in L2), then spread so that each XCD
+// (workgroup index mod 8) receives a contiguous run.  Kernels launched over a group reach only its cases,
+// so cases that stopped cost nothing; the graphs, whose grids depend on the groups, are rebuilt.
 int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* err, size_t errlen) {
-  std::vector<int2> list;
-  list.reserve(b->h_list.size());
-  for (const int2& w : b->h_list)
-    if (active[w.x]) list.push_back(w);
-  const int W = static_cast<int>(list.size());
-  const int per = (W + 7) / 8;
-  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
-  for (int j = 0; j < static_cast<int>(work.size()); ++j) {
-    const int src = (j & 7) * per + (j >> 3);
-    if (src < W) work[j] = list[src];
+  destroy_graphs(b);
+  b->groups.clear();
+  std::vector<int> gcases;
+  std::vector<int2> work;
+  std::vector<int> members;
+  size_t bytes = 0;
+  auto close_group = [&]() {
+    if (members.empty()) return;
+    CaseGroup gr;
+    gr.c0 = static_cast<int>(gcases.size());
+    gr.nc = static_cast<int>(members.size());
+    gcases.insert(gcases.end(), members.begin(), members.end());
+    std::vector<int2> items;
+    for (int gi = 0; gi < b->ngeom; ++gi)
+      for (int blk = 0; blk < b->geoms[gi]->nblk; ++blk)
+        for (const int i : members)
+          if (b->spec[i].mesh_index == gi) items.push_back(make_int2(i, blk));
+    const int W = static_cast<int>(items.size());
+    const int per = (W + 7) / 8;
+    gr.w0 = static_cast<int>(work.size());
+    gr.nw = per * 8;
+    for (int j = 0; j < gr.nw; ++j) {
+      const int src = (j & 7) * per + (j >> 3);
+      work.push_back(src < W ? items[src] : make_int2(-1, -1));
+    }
+    b->groups.push_back(gr);
+    members.clear();
+    bytes = 0;
+  };
+  for (int i = 0; i < b->ncases; ++i) {
+    if (!active[i]) continue;
+    const size_t cb = case_hot_bytes(b, i);
+    if (!members.empty() && bytes + cb > b->group_bytes) close_group();
+    members.push_back(i);
+    bytes += cb;
   }
+  close_group();
   if (work.empty()) work.push_back(make_int2(-1, -1));
+  if (gcases.empty()) gcases.push_back(0);
+  if (static_cast<int>(work.size()) > b->work_cap || static_cast<int>(gcases.size()) > b->ncases) {
+    set_err(err, errlen, "upload_work_list: work list exceeds its capacity");
+    return AERO_EINVAL;
+  }
   AERO_TRY(hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream),
            "work list upload");
+  AERO_TRY(hipMemcpyAsync(b->d_gcases, gcases.data(), gcases.size() * sizeof(int), hipMemcpyHostToDevice, b->stream),
+           "case list upload");
   AERO_TRY(hipStreamSynchronize(b->stream), "work list upload");
   b->nwork = static_cast<int>(work.size());
   b->listed = active;
-  for (auto& row : b->graph)
-    for (auto& gx : row)
-      if (gx) {
-        (void)hipGraphExecDestroy(gx);
-        gx = nullptr;
-      }
   return AERO_OK;
 }
 
@@ -353,22 +421,6 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     const int as_on = cs.as_on, as_len = cs.as_len;
     k.as_w = as_len <= 0 ? (0 >= as_on ? 1.0 : 0.0) : clampd(static_cast<double>(0 - as_on) / std::max(as_len, 1), 0.0, 1.0);
   }
-  // Work list: per geometry, block-major / case-minor (the cases of a block share its face data in L2),
-  // then spread so that each XCD (workgroup index mod 8) receives a contiguous run.
-  std::vector<int2>& list = b->h_list;
-  for (int gi = 0; gi < n_meshes; ++gi)
-    for (int blk = 0; blk < meshes[gi]->nblk; ++blk)
-      for (int i = 0; i < n_cases; ++i)
-        if (cases[i].mesh_index == gi) list.push_back(make_int2(i, blk));
-  const int W = static_cast<int>(list.size());
-  const int per = (W + 7) / 8;
-  std::vector<int2> work(static_cast<size_t>(per) * 8, make_int2(-1, -1));
-  for (int j = 0; j < static_cast<int>(work.size()); ++j) {
-    const int src = (j & 7) * per + (j >> 3);
-    if (src < W) work[j] = list[src];
-  }
-  b->nwork = static_cast<int>(work.size());
-  b->listed.assign(n_cases, 1);
   std::vector<DevMesh> gtab(n_meshes);
   for (int gi = 0; gi < n_meshes; ++gi) gtab[gi] = meshes[gi]->dm;
   AERO_BSETUP(balloc(b, &b->d_geoms, gtab.size()));
@@ -377,10 +429,19 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
   AERO_BSETUP(hipMemcpy(b->d_cases, b->h_cases.data(), b->h_cases.size() * sizeof(BatchCase), hipMemcpyHostToDevice));
   AERO_BSETUP(balloc(b, &b->d_ctl, ctl.size()));
   AERO_BSETUP(hipMemcpy(b->d_ctl, ctl.data(), ctl.size() * sizeof(CaseCtl), hipMemcpyHostToDevice));
-  AERO_BSETUP(balloc(b, &b->d_work, work.size()));
-  AERO_BSETUP(hipMemcpy(b->d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice));
+  b->work_cap = work_capacity(b);
+  AERO_BSETUP(balloc(b, &b->d_work, static_cast<size_t>(b->work_cap)));
+  AERO_BSETUP(balloc(b, &b->d_gcases, static_cast<size_t>(n_cases)));
   if (b->hist_cap > 0) AERO_BSETUP(balloc(b, &b->d_hist, static_cast<size_t>(n_cases) * b->hist_cap * 4));
 #undef AERO_BSETUP
+  {  // every case starts active
+    const int rc = upload_work_list(b, std::vector<char>(n_cases, 1), err, errlen);
+    if (rc) {
+      free_batch(b);
+      delete b;
+      return rc;
+    }
+  }
   *out = b;
   return AERO_OK;
 }
@@ -409,19 +470,15 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
   const int startup = std::max(c.startup_first_order_iters, 0);
   const int sync_every = std::max(iters_per_sync, 1);
   const bool compact = true;  // relaunch over the active cases only, whenever the active set changes
-  auto enqueue = [&](bool second) -> int {
-    int rc = enqueue_iteration(b, second, b->frozen, err, errlen);
+  auto enqueue = [&](bool second, const CaseGroup& gr) -> int {
+    int rc = enqueue_iteration(b, second, b->frozen, gr, err, errlen);
     if (rc) return rc;
-    AERO_TRY(aero_host::launch_batch_control(b->d_cases, b->d_ctl, p, b->stream), "k_batch_control");
+    AERO_TRY(aero_host::launch_batch_control(b->d_cases, b->d_ctl, b->d_gcases + gr.c0, gr.nc, p, b->stream),
+             "k_batch_control");
     return 0;
   };
   // Graphs are specific to (order, frozen limiter) and to the run parameters; rebuild them per run.
-  for (auto& row : b->graph)
-    for (auto& gx : row)
-      if (gx) {
-        (void)hipGraphExecDestroy(gx);
-        gx = nullptr;
-      }
+  destroy_graphs(b);
   // Frozen limiter (beyond the reference): at the start of iteration limiter_freeze_iter (second order)
   // every active case captures its ratios; the iterations run in chunks that end there.
   const int freeze_at =
@@ -469,19 +526,20 @@ int aero_hip_batch_run(aero_hip_batch* b, int32_t max_iters, double rho_drop, in
     int phase_end = second ? max_iters : std::min(startup - done_iters, max_iters);
     if (freeze_at >= 0 && !b->frozen && g_it < freeze_at) phase_end = std::min(phase_end, freeze_at - done_iters);
     const int n = std::min(sync_every, phase_end - it);
-    for (int k = 0; k < n; ++k) {
-      if (use_graph) {
-        hipGraphExec_t& gx = b->graph[second ? 1 : 0][b->frozen ? 1 : 0];
-        if (!gx) {
-          const int rc = capture_iteration(b, [&] { return enqueue(second); }, &gx, err, errlen);
+    for (CaseGroup& gr : b->groups)  // each group runs the chunk's n iterations in turn
+      for (int k = 0; k < n; ++k) {
+        if (use_graph) {
+          hipGraphExec_t& gx = gr.graph[second ? 1 : 0][b->frozen ? 1 : 0];
+          if (!gx) {
+            const int rc = capture_iteration(b, [&] { return enqueue(second, gr); }, &gx, err, errlen);
+            if (rc) return rc;
+          }
+          AERO_TRY(hipGraphLaunch(gx, b->stream), "hipGraphLaunch");
+        } else {
+          const int rc = enqueue(second, gr);
           if (rc) return rc;
         }
-        AERO_TRY(hipGraphLaunch(gx, b->stream), "hipGraphLaunch");
-      } else {
-        const int rc = enqueue(second);
-        if (rc) return rc;
       }
-    }
     it += n;
     AERO_TRY(hipMemcpyAsync(h.data(), b->d_ctl, h.size() * sizeof(CaseCtl), hipMemcpyDeviceToHost, b->stream),
              "control readback");
@@ -599,6 +657,19 @@ int aero_hip_batch_set_limiter_ratios(aero_hip_batch* b, int32_t case_index, con
   AERO_TRY(aero_host::launch_ratio_slots(m->dm, cs.lim_ratio, cs.lim_slot, b->stream), "k_ratio_slots");
   AERO_TRY(hipStreamSynchronize(b->stream), "aero_hip_batch_set_limiter_ratios: sync");
   b->frozen = true;  // no capture at the freeze iteration from here on
+  return AERO_OK;
+}
+
+int aero_hip_batch_set_group_bytes(aero_hip_batch* b, int64_t group_bytes, int32_t* n_groups, char* err,
+                                   size_t errlen) {
+  if (!b || group_bytes < 0) {
+    set_err(err, errlen, "aero_hip_batch_set_group_bytes: bad argument");
+    return AERO_EINVAL;
+  }
+  b->group_bytes = group_bytes == 0 ? kGroupBytesDefault : static_cast<size_t>(group_bytes);
+  const int rc = upload_work_list(b, b->listed, err, errlen);
+  if (rc) return rc;
+  if (n_groups) *n_groups = static_cast<int32_t>(b->groups.size());
   return AERO_OK;
 }
 

@@ -265,11 +265,12 @@ hipError_t launch_batch_flux(int BC, int scheme, int limiter, bool second, bool 
 hipError_t launch_batch_limiter_capture(int BC, int limiter, const aero_dev::DevMesh* geoms,
                                         const aero_dev::BatchCase* cases, const aero_dev::CaseCtl* ctl,
                                         const int2* work, int nwork, double g, int cur, hipStream_t s);
-hipError_t launch_batch_reduce(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl, int ncases, double g, int cur,
-                               hipStream_t s);
-hipError_t launch_batch_control(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl,
+// Per-iteration control of the n cases listed in list[0..n) (device array of case indices).
+hipError_t launch_batch_reduce(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl, const int* list, int n,
+                               double g, int cur, hipStream_t s);
+hipError_t launch_batch_control(const aero_dev::BatchCase* cases, aero_dev::CaseCtl* ctl, const int* list, int n,
                                 const aero_dev::BatchParams& p, hipStream_t s);
-hipError_t launch_batch_init_diag(const aero_dev::BatchCase* cases, int ncases, hipStream_t s);
+hipError_t launch_batch_init_diag(const aero_dev::BatchCase* cases, const int* list, int n, hipStream_t s);
 
 // tangent / adjoint (aero_adjoint.hip)
 hipError_t launch_tan_gradient(const aero_dev::AdjMesh& am, const double* U, const double* dU,

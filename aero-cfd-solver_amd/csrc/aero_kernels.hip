@@ -927,20 +927,21 @@ __global__ __launch_bounds__(BC) void k_limiter_capture_batch(const DevMesh* __r
 
 // Per case: norm partials + wall force sums of the updated state into ctl[case].sums (same fixed-order
 // reduction as the single-case path, so a case's history is identical to a solo run).
+// The per-iteration control kernels act on the cases of a list (one case group of the batch).
 __global__ __launch_bounds__(256) void k_batch_reduce(const BatchCase* __restrict__ cases, CaseCtl* __restrict__ ctl,
-                                                       double g, int cur) {
+                                                       const int* __restrict__ list, double g, int cur) {
   __shared__ double red[9][256];
-  const int c = blockIdx.x;
+  const int c = list[blockIdx.x];
   if (!ctl[c].active) return;
   const BatchCase& cs = cases[c];
   iter_reduce_body(cs.nblk, cs.partials, cs.wall, cs.U[ring_buf(cur, ctl[c].iter)], cs.N, g, ctl[c].sums, nullptr,
                    red);
 }
 
-__global__ void k_batch_init_diag(const BatchCase* __restrict__ cases, int ncases) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncases) return;
-  int* d = cases[c].diag;
+__global__ void k_batch_init_diag(const BatchCase* __restrict__ cases, const int* __restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int* d = cases[list[i]].diag;
   d[0] = 0;
   d[1] = 0;
   d[2] = 0x7fffffff;
@@ -951,9 +952,11 @@ __global__ void k_batch_init_diag(const BatchCase* __restrict__ cases, int ncase
 // residual ratio (convergence_monitor.cpp:13-19), adaptive CFL (euler_solver.cpp:951-958), then the CFL,
 // stabilisation-floor and all-speed ramps of the next iteration (:69-114).  Convergence is the density
 // residual drop rho_l2 <= rho_drop * rho_l2(iteration 0).
-__global__ void k_batch_control(const BatchCase* __restrict__ cases, CaseCtl* __restrict__ ctl, BatchParams p) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= p.ncases) return;
+__global__ void k_batch_control(const BatchCase* __restrict__ cases, CaseCtl* __restrict__ ctl,
+                                const int* __restrict__ list, int n, BatchParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = list[i];
   CaseCtl& k = ctl[c];
   if (!k.active) return;
   const BatchCase& cs = cases[c];
@@ -1295,18 +1298,20 @@ hipError_t launch_batch_limiter_capture(int BC, int limiter, const DevMesh* geom
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_reduce(const BatchCase* cases, CaseCtl* ctl, int ncases, double g, int cur, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_reduce, dim3(ncases), dim3(256), 0, s, cases, ctl, g, cur);
+hipError_t launch_batch_reduce(const BatchCase* cases, CaseCtl* ctl, const int* list, int n, double g, int cur,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reduce, dim3(n), dim3(256), 0, s, cases, ctl, list, g, cur);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_control(const BatchCase* cases, CaseCtl* ctl, const BatchParams& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_control, dim3((p.ncases + 63) / 64), dim3(64), 0, s, cases, ctl, p);
+hipError_t launch_batch_control(const BatchCase* cases, CaseCtl* ctl, const int* list, int n, const BatchParams& p,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_control, dim3((n + 63) / 64), dim3(64), 0, s, cases, ctl, list, n, p);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_init_diag(const BatchCase* cases, int ncases, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_init_diag, dim3((ncases + 63) / 64), dim3(64), 0, s, cases, ncases);
+hipError_t launch_batch_init_diag(const BatchCase* cases, const int* list, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_init_diag, dim3((n + 63) / 64), dim3(64), 0, s, cases, list, n);
   return hipGetLastError();
 }
 

@@ -147,6 +147,40 @@ def test_batch_frozen_limiter_matches_solo_runs(hip):
     g2.close()
 
 
+def test_case_groups_are_bitwise_one_group(hip):
+    """Iterating the active cases in cache-sized groups (each group runs a sync interval's iterations in
+    turn) gives every case the same states, histories and stopping iterations as one group: with one case
+    per group, with pairs, and with every case in one group; across a freeze, a convergence stop that
+    regroups the rest, and graph replay and eager launches."""
+    m1 = helpers.mesh("40x8")
+    m2 = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=3, rcm=True)
+    cases = [(0, 0.3, 1.0), (1, 0.5, 2.5), (1, 0.6, -2.0), (0, 0.45, 0.0), (1, 0.3, 3.0)]
+    cap, freeze = 300, 30  # three cases reach the 0.1 drop between iterations 210 and 255, two run to the cap
+    base = hip.CaseConfig(**dict(BASE, limiter_freeze_iter=freeze, precond_on=0, low_mach_fix=0))
+    g1, g2 = hip.HipMesh(m1), hip.HipMesh(m2)
+    pair = 2 * max(g.N * 160 + g.layout()["slots"] * 16 for g in (g1, g2))
+    out = {}
+    for label, budget, graph in (("one", 1 << 40, True), ("single", 1, True), ("pairs", pair, False),
+                                 ("default", 0, True)):
+        say(f"[batch] case groups: {label}")
+        with hip.HipBatch([g1, g2], cases, base, hist_capacity=cap) as b:
+            ng = b.set_group_bytes(budget)
+            r = b.run(cap, rho_drop=0.1, iters_per_sync=13, use_graph=graph)
+            out[label] = (ng, [c["iterations"] for c in r["cases"]], [b.state(i) for i in range(len(cases))],
+                          [b.history(i) for i in range(len(cases))])
+    assert out["one"][0] == 1 and out["single"][0] == len(cases) and out["pairs"][0] == 3
+    its = out["one"][1]
+    assert min(its) < max(its), "the test needs a case that stops before another"
+    for label in ("single", "pairs", "default"):
+        ng, it2, st2, h2 = out[label]
+        assert it2 == its, label
+        for i in range(len(cases)):
+            assert np.array_equal(st2[i], out["one"][2][i]), (label, i)
+            assert np.array_equal(h2[i], out["one"][3][i]), (label, i)
+    g1.close()
+    g2.close()
+
+
 def test_batch_per_case_switches_match_solo_runs(hip):
     """Per-case low-Mach preconditioner and all-speed switches: a batch mixing preconditioned all-speed,
     preconditioned and plain cases equals the solo runs of the same cases configured that way."""
