@@ -207,7 +207,7 @@ PROTOTYPES = {
     "aero_hip_batch_set_state": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
     "aero_hip_layout_probe": (C.c_int, [vp, PD]),
     "aero_hip_batch_set_limiter_ratios": (C.c_int, [vp, i32, PD, C.c_char_p, C.c_size_t]),
-    "aero_hip_batch_set_group_bytes": (C.c_int, [vp, C.c_int64, PI, C.c_char_p, C.c_size_t]),
+    "aero_hip_batch_set_groups": (C.c_int, [vp, C.c_int64, i32, PI, C.c_char_p, C.c_size_t]),
     "aero_case_linearization": (C.c_int, [C.POINTER(CaseConfigC), i32, i32, C.POINTER(ResidualConfig),
                                           C.POINTER(TangentSeed), C.POINTER(Objective), PD, C.c_char_p, C.c_size_t]),
     "aero_hip_residual_jvp": (C.c_int, [vp, C.POINTER(ResidualConfig), PD, PD, C.POINTER(TangentSeed), PD, PD,

@@ -557,13 +557,14 @@ class HipBatch:
         _lib.check(self.lib.aero_hip_batch_set_limiter_ratios(self.h, i, _pd(r), err, len(err)), err,
                    "batch set_limiter_ratios")
 
-    def set_group_bytes(self, group_bytes: int = 0) -> int:
-        """Iterate the active cases in groups of at most group_bytes hot bytes (0: the default budget sized to
-        the Infinity Cache); returns the number of groups (aero_hip_batch_set_group_bytes)."""
+    def set_groups(self, group_bytes: int = 0, group_items: int = 0) -> int:
+        """Iterate the active cases in groups of about group_bytes hot bytes with at least group_items work
+        items each (0: the defaults, sized to the Infinity Cache and the CU count); returns the number of
+        groups (aero_hip_batch_set_groups)."""
         n = C.c_int32()
         err = _lib.errbuf()
-        _lib.check(self.lib.aero_hip_batch_set_group_bytes(self.h, int(group_bytes), C.byref(n), err, len(err)), err,
-                   "batch set_group_bytes")
+        _lib.check(self.lib.aero_hip_batch_set_groups(self.h, int(group_bytes), int(group_items), C.byref(n), err,
+                                                      len(err)), err, "batch set_groups")
         return n.value
 
     def limiter_ratios(self, i: int) -> np.ndarray:

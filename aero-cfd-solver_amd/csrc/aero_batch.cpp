@@ -33,8 +33,12 @@ struct CaseGroup {
   hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
 };
 
-// Default budget of a group's hot bytes: the Infinity Cache less room for the shared geometry.
-constexpr size_t kGroupBytesDefault = static_cast<size_t>(192) << 20;
+// Default budget of a group's hot bytes: the size of the Infinity Cache (measured on the 64-case 66k-cell
+// sweep: 3 groups of about 270 MB ran its frozen iterations in 1.382 ms, 4 groups of about 200 MB in
+// 1.394 ms, one group of 64 in 1.56 ms); and the default least number of work items (workgroups) per group,
+// about 16 per CU, below which a launch no longer fills the 256 CUs (8 groups of 8 cases: 1.70 ms).
+constexpr size_t kGroupBytesDefault = static_cast<size_t>(256) << 20;
+constexpr int kGroupItemsDefault = 4096;
 
 struct aero_hip_batch {
   int ncases = 0, ngeom = 0, BC = 256, grad = 0, smax = 1, nwork = 0, work_cap = 0;
@@ -50,6 +54,7 @@ struct aero_hip_batch {
   int* d_gcases = nullptr;         // case indices of the groups, group-major
   std::vector<CaseGroup> groups;   // the active cases' groups
   size_t group_bytes = kGroupBytesDefault;
+  int group_items = kGroupItemsDefault;
   std::vector<char> listed;        // the cases the device work list holds
   double* d_hist = nullptr;
   int hist_cap = 0;
@@ -162,8 +167,10 @@ int work_capacity(const aero_hip_batch* b) {
   return static_cast<int>(w + 8 * static_cast<size_t>(b->ncases) + 8);
 }
 
-// The device work list of the active cases: the cases, in index order, form groups of at most group_bytes
-// hot bytes (a case larger than that is a group of its own); within a group, items are per geometry
+// The device work list of the active cases.  The cases, in index order, form k groups of about equal hot
+// bytes: k = ceil(hot bytes / group_bytes), at most one per group_items work items and one per case; a
+// case goes to group floor(k (bytes before it + half its own) / hot bytes).  Within a group, items are per
+// geometry
 // block-major / case-minor (the cases of a block share its face data in L2), then spread so that each XCD
 // (workgroup index mod 8) receives a contiguous run.  Kernels launched over a group reach only its cases,
 // so cases that stopped cost nothing; the graphs, whose grids depend on the groups, are rebuilt.
@@ -173,7 +180,6 @@ int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* e
   std::vector<int> gcases;
   std::vector<int2> work;
   std::vector<int> members;
-  size_t bytes = 0;
   auto close_group = [&]() {
     if (members.empty()) return;
     CaseGroup gr;
@@ -195,14 +201,31 @@ int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* e
     }
     b->groups.push_back(gr);
     members.clear();
-    bytes = 0;
   };
+  double total = 0.0;
+  size_t items = 0;
+  int nact = 0;
+  for (int i = 0; i < b->ncases; ++i)
+    if (active[i]) {
+      total += static_cast<double>(case_hot_bytes(b, i));
+      items += static_cast<size_t>(b->geoms[b->spec[i].mesh_index]->nblk);
+      ++nact;
+    }
+  size_t k = static_cast<size_t>(std::ceil(total / static_cast<double>(std::max<size_t>(b->group_bytes, 1))));
+  k = std::min(k, std::max<size_t>(1, items / static_cast<size_t>(std::max(b->group_items, 1))));
+  k = std::max<size_t>(1, std::min(k, static_cast<size_t>(std::max(nact, 1))));
+  double before = 0.0;
+  size_t cur = 0;
   for (int i = 0; i < b->ncases; ++i) {
     if (!active[i]) continue;
-    const size_t cb = case_hot_bytes(b, i);
-    if (!members.empty() && bytes + cb > b->group_bytes) close_group();
+    const double cb = static_cast<double>(case_hot_bytes(b, i));
+    size_t gi = k == static_cast<size_t>(nact) ? cur + (members.empty() ? 0 : 1)
+                                               : static_cast<size_t>(static_cast<double>(k) * (before + 0.5 * cb) / total);
+    gi = std::min(gi, k - 1);
+    if (gi != cur && !members.empty()) close_group();
+    cur = gi;
     members.push_back(i);
-    bytes += cb;
+    before += cb;
   }
   close_group();
   if (work.empty()) work.push_back(make_int2(-1, -1));
@@ -660,13 +683,14 @@ int aero_hip_batch_set_limiter_ratios(aero_hip_batch* b, int32_t case_index, con
   return AERO_OK;
 }
 
-int aero_hip_batch_set_group_bytes(aero_hip_batch* b, int64_t group_bytes, int32_t* n_groups, char* err,
-                                   size_t errlen) {
-  if (!b || group_bytes < 0) {
-    set_err(err, errlen, "aero_hip_batch_set_group_bytes: bad argument");
+int aero_hip_batch_set_groups(aero_hip_batch* b, int64_t group_bytes, int32_t group_items, int32_t* n_groups,
+                              char* err, size_t errlen) {
+  if (!b || group_bytes < 0 || group_items < 0) {
+    set_err(err, errlen, "aero_hip_batch_set_groups: bad argument");
     return AERO_EINVAL;
   }
   b->group_bytes = group_bytes == 0 ? kGroupBytesDefault : static_cast<size_t>(group_bytes);
+  b->group_items = group_items == 0 ? kGroupItemsDefault : group_items;
   const int rc = upload_work_list(b, b->listed, err, errlen);
   if (rc) return rc;
   if (n_groups) *n_groups = static_cast<int32_t>(b->groups.size());

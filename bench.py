@@ -267,6 +267,7 @@ def measure_sweep(args, seq_cases: int) -> dict:
     log(f"[sweep] batched: {len(cases)} cases x {mesh.num_cells} cells, cap {cap} iterations, "
         f"cfl {args.sweep_cfl}, limiter frozen at {args.freeze_iter}")
     with HipBatch([g], cases, base) as b:
+        n_groups = b.set_groups()  # the default Infinity-Cache-sized case groups
         torch.cuda.synchronize()
         rb = b.run(cap, 1e-8, iters_per_sync=500, use_graph=not args.sweep_no_graph)
     t_batch = rb["seconds"]
@@ -302,6 +303,7 @@ def measure_sweep(args, seq_cases: int) -> dict:
                       "batched_seconds": round(t_batch, 3), "sequential_seconds": round(t_seq_all, 3),
                       "cases_per_hr": round(len(cases) / t_batch * 3600.0, 1),
                       "sequential_cases_timed": nseq,
+                      "case_groups": n_groups,
                       "sequential_baseline": ("measured: every case solved alone, one after another" if measured else
                                               "seconds per iteration of the timed cases x all cases' iterations"),
                       "speedup_vs_sequential": round(t_seq_all / t_batch, 2) if t_seq_all > 0 else None,

@@ -380,14 +380,15 @@ int aero_hip_batch_set_state(aero_hip_batch* batch, int32_t case_index, const do
                              size_t errlen);
 int aero_hip_batch_set_limiter_ratios(aero_hip_batch* batch, int32_t case_index, const double* ratios, char* err,
                                       size_t errlen);
-/* Case groups (no reference counterpart): the batch iterates its active cases in groups whose hot data
- * (state buffers, gradients, frozen ratios: 160 B per cell + 16 B per face slot) fits group_bytes, each
- * group running a sync interval's iterations in turn, so that a group's data stays in the last-level cache
- * between its kernels.  group_bytes = 0 restores the default (192 MiB: MI355X's 256 MB Infinity Cache less
- * room for the geometry); a case larger than the budget forms a group of its own.  Results are bitwise the
+/* Case groups (no reference counterpart): the batch iterates its active cases in groups of about equal
+ * hot data (state buffers, gradients, frozen ratios: 160 B per cell + 16 B per face slot), each group
+ * running a sync interval's iterations in turn, so that a group's data stays in the last-level cache between
+ * its kernels.  The number of groups is ceil(hot bytes / group_bytes), at most one per group_items work
+ * items (workgroups) and one per case.  0 restores a default: group_bytes 256 MiB (the size of MI355X's
+ * Infinity Cache), group_items 4096 (about 16 workgroups per CU).  Results are bitwise the
  * same for any grouping.  *n_groups (may be NULL) receives the number of groups of the active cases. */
-int aero_hip_batch_set_group_bytes(aero_hip_batch* batch, int64_t group_bytes, int32_t* n_groups, char* err,
-                                   size_t errlen);
+int aero_hip_batch_set_groups(aero_hip_batch* batch, int64_t group_bytes, int32_t group_items, int32_t* n_groups,
+                              char* err, size_t errlen);
 
 /* ------------------------------------------------------------------------------------------------
  * Discrete tangent and adjoint of the residual (no reference counterpart: the reference has no

@@ -164,11 +164,11 @@ def test_case_groups_are_bitwise_one_group(hip):
                                  ("default", 0, True)):
         say(f"[batch] case groups: {label}")
         with hip.HipBatch([g1, g2], cases, base, hist_capacity=cap) as b:
-            ng = b.set_group_bytes(budget)
+            ng = b.set_groups(budget, 1)
             r = b.run(cap, rho_drop=0.1, iters_per_sync=13, use_graph=graph)
             out[label] = (ng, [c["iterations"] for c in r["cases"]], [b.state(i) for i in range(len(cases))],
                           [b.history(i) for i in range(len(cases))])
-    assert out["one"][0] == 1 and out["single"][0] == len(cases) and out["pairs"][0] == 3
+    assert out["one"][0] == 1 and out["single"][0] == len(cases) and 1 < out["pairs"][0] < len(cases)
     its = out["one"][1]
     assert min(its) < max(its), "the test needs a case that stops before another"
     for label in ("single", "pairs", "default"):
