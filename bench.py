@@ -223,9 +223,10 @@ def kernel_source_digest() -> str:
     return h.hexdigest()
 
 
-def traffic_for(workload: str):
+def traffic_for(workload: str, layout: dict):
     """HBM bytes per k_flux launch from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration), only when it was measured on the current kernel sources; else None."""
+    the copy-kernel calibration) for the gather layout's block size, only when it was measured on the
+    current kernel sources; else None."""
     p = ROOT / "profiles" / "traffic.json"
     try:
         rec = json.loads(p.read_text()).get(workload)
@@ -233,7 +234,9 @@ def traffic_for(workload: str):
         return None
     if not isinstance(rec, dict) or rec.get("kernel_sources_sha256") != kernel_source_digest():
         return None
-    return rec.get("bytes")
+    if layout.get("accumulation") != "gather":
+        return None
+    return rec.get("by_block_cells", {}).get(str(layout.get("block_cells")), {}).get("k_flux_bytes")
 
 
 SWEEP_MESH = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
@@ -852,7 +855,7 @@ def main() -> int:
     achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
     ceiling = copy_ceiling()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(args.workload),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(args.workload, layout),
                 "kernel": ("k_flux (reconstruction + Riemann flux + face->cell gather)"
                            if layout["accumulation"] == "gather" else
                            "k_flux_colour (every colour's launch of one evaluation, and the zero fill of R)"),
