@@ -30,6 +30,7 @@ using aero_dev::FaceOpts;
 struct CaseGroup {
   int w0 = 0, nw = 0;  // range of the device work list (padded to a multiple of 8 items)
   int c0 = 0, nc = 0;  // range of the device case list
+  bool plain = false;  // every case of the group has plain face options (the flux kernels' PLAIN instantiation)
   hipGraphExec_t graph[2][2] = {};  // [second order][frozen limiter]
 };
 
@@ -60,7 +61,6 @@ struct aero_hip_batch {
   int hist_cap = 0;
   aero_case_config cfg{};
   bool frozen = false;               // limiter ratios captured (limiter_freeze_iter reached)
-  bool plain = true;                 // every case has plain face options (flux kernels' PLAIN instantiation)
   std::vector<double> alpha, q_inf, chord;
 };
 
@@ -142,7 +142,7 @@ int enqueue_iteration(aero_hip_batch* b, bool second, bool frozen, const CaseGro
       AERO_TRY(aero_host::launch_batch_gradient(b->BC, b->grad, b->d_geoms, b->d_cases, b->d_ctl, work, gr.nw,
                                                 b->smax, c.gamma, cur, b->stream),
                "k_gradient_batch");
-    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, b->plain, b->d_geoms,
+    AERO_TRY(aero_host::launch_batch_flux(b->BC, c.flux_scheme, c.limiter, second, frozen, gr.plain, b->d_geoms,
                                           b->d_cases, b->d_ctl, work, gr.nw, b->smax, bs, b->stream),
              "k_flux_batch");
     cur = bs.out;
@@ -180,11 +180,13 @@ int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* e
   std::vector<int> gcases;
   std::vector<int2> work;
   std::vector<int> members;
+  bool group_plain = false;
   auto close_group = [&]() {
     if (members.empty()) return;
     CaseGroup gr;
     gr.c0 = static_cast<int>(gcases.size());
     gr.nc = static_cast<int>(members.size());
+    gr.plain = group_plain;
     gcases.insert(gcases.end(), members.begin(), members.end());
     std::vector<int2> items;
     for (int gi = 0; gi < b->ngeom; ++gi)
@@ -202,32 +204,38 @@ int upload_work_list(aero_hip_batch* b, const std::vector<char>& active, char* e
     b->groups.push_back(gr);
     members.clear();
   };
-  double total = 0.0;
-  size_t items = 0;
-  int nact = 0;
-  for (int i = 0; i < b->ncases; ++i)
-    if (active[i]) {
-      total += static_cast<double>(case_hot_bytes(b, i));
-      items += static_cast<size_t>(b->geoms[b->spec[i].mesh_index]->nblk);
-      ++nact;
+  // Cases with plain face options and the others are grouped apart, so that the plain groups run the
+  // flux kernels' PLAIN instantiation.
+  for (const bool want_plain : {true, false}) {
+    std::vector<int> part;
+    double total = 0.0;
+    size_t items = 0;
+    for (int i = 0; i < b->ncases; ++i)
+      if (active[i] && aero_dev::plain_options(b->h_cases[i].fo) == want_plain) {
+        part.push_back(i);
+        total += static_cast<double>(case_hot_bytes(b, i));
+        items += static_cast<size_t>(b->geoms[b->spec[i].mesh_index]->nblk);
+      }
+    if (part.empty()) continue;
+    const size_t np = part.size();
+    size_t k = static_cast<size_t>(std::ceil(total / static_cast<double>(std::max<size_t>(b->group_bytes, 1))));
+    k = std::min(k, std::max<size_t>(1, items / static_cast<size_t>(std::max(b->group_items, 1))));
+    k = std::max<size_t>(1, std::min(k, np));
+    group_plain = want_plain;
+    double before = 0.0;
+    size_t cur = 0;
+    for (const int i : part) {
+      const double cb = static_cast<double>(case_hot_bytes(b, i));
+      size_t gi = k == np ? cur + (members.empty() ? 0 : 1)
+                          : static_cast<size_t>(static_cast<double>(k) * (before + 0.5 * cb) / total);
+      gi = std::min(gi, k - 1);
+      if (gi != cur && !members.empty()) close_group();
+      cur = gi;
+      members.push_back(i);
+      before += cb;
     }
-  size_t k = static_cast<size_t>(std::ceil(total / static_cast<double>(std::max<size_t>(b->group_bytes, 1))));
-  k = std::min(k, std::max<size_t>(1, items / static_cast<size_t>(std::max(b->group_items, 1))));
-  k = std::max<size_t>(1, std::min(k, static_cast<size_t>(std::max(nact, 1))));
-  double before = 0.0;
-  size_t cur = 0;
-  for (int i = 0; i < b->ncases; ++i) {
-    if (!active[i]) continue;
-    const double cb = static_cast<double>(case_hot_bytes(b, i));
-    size_t gi = k == static_cast<size_t>(nact) ? cur + (members.empty() ? 0 : 1)
-                                               : static_cast<size_t>(static_cast<double>(k) * (before + 0.5 * cb) / total);
-    gi = std::min(gi, k - 1);
-    if (gi != cur && !members.empty()) close_group();
-    cur = gi;
-    members.push_back(i);
-    before += cb;
+    close_group();
   }
-  close_group();
   if (work.empty()) work.push_back(make_int2(-1, -1));
   if (gcases.empty()) gcases.push_back(0);
   if (static_cast<int>(work.size()) > b->work_cap || static_cast<int>(gcases.size()) > b->ncases) {
@@ -400,7 +408,6 @@ int aero_hip_batch_create(aero_hip_mesh* const* meshes, int32_t n_meshes, const 
     fo.mach_min = std::max(cc->precond_mach_min > 0.0 ? cc->precond_mach_min : 1.0e-6, 1.0e-6);
     fo.beta_min = std::max(cc->precond_beta_min > 0.0 ? cc->precond_beta_min : auto_bmin, 1.0e-4);
     fo.beta_max = std::max(cc->precond_beta_max, fo.beta_min);
-    b->plain = b->plain && aero_dev::plain_options(fo);
     cs.cfl_start = cc->cfl_start;
     cs.cfl_max = cc->cfl_max;
     cs.cfl_ramp = cc->cfl_ramp_iters;
