@@ -32,6 +32,8 @@ sys.path.insert(0, str(ROOT / "aero-cfd-solver_amd"))
 
 METRIC = "FP64 Mcell-res/s (%HBM peak); sweep solves/hr; adjoint/primal time ratio"
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024          # MI355X: 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9      # peak engine clock (FP64 vector peak 78.6 TFLOP/s = 1024 x 16 lanes x 2 x 2.4 GHz)
 
 WORKLOADS = {
     # name: (mesh kwargs, flux, limiter, mach, aoa)
@@ -223,20 +225,20 @@ def kernel_source_digest() -> str:
     return h.hexdigest()
 
 
-def traffic_for(workload: str, layout: dict):
-    """HBM bytes per k_flux launch from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration) for the gather layout's block size, only when it was measured on the
-    current kernel sources; else None."""
+def counters_for(workload: str, layout: dict) -> dict:
+    """The k_flux counter record of profiles/traffic.json for the gather layout's block size: HBM bytes per
+    launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and VALU wave
+    instructions per launch (SQ_INSTS_VALU), only when measured on the current kernel sources; else {}."""
     p = ROOT / "profiles" / "traffic.json"
     try:
         rec = json.loads(p.read_text()).get(workload)
     except Exception:
-        return None
+        return {}
     if not isinstance(rec, dict) or rec.get("kernel_sources_sha256") != kernel_source_digest():
-        return None
+        return {}
     if layout.get("accumulation") != "gather":
-        return None
-    return rec.get("by_block_cells", {}).get(str(layout.get("block_cells")), {}).get("k_flux_bytes")
+        return {}
+    return rec.get("by_block_cells", {}).get(str(layout.get("block_cells")), {})
 
 
 SWEEP_MESH = dict(num_circumferential=256, num_radial=144, farfield_radius=20.0, radial_stretch=200.0, cells="mixed",
@@ -858,13 +860,19 @@ def main() -> int:
     grad_ms = tm["gradient_ms"] / n
     achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
     ceiling = copy_ceiling()
+    counters = counters_for(args.workload, layout)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(args.workload, layout),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": counters.get("k_flux_bytes"),
                 "kernel": ("k_flux (reconstruction + Riemann flux + face->cell gather)"
                            if layout["accumulation"] == "gather" else
                            "k_flux_colour (every colour's launch of one evaluation, and the zero fill of R)"),
                 "algorithmic_bytes_per_launch": flux_bytes, "avg_launch_ms": round(flux_ms, 5),
                 "copy_ceiling_gbs": round(ceiling, 1), "frac_of_copy_ceiling": round(achieved / ceiling, 4)}
+    if counters.get("k_flux_valu_wave_insts"):
+        # FP64 VALU issue: a wave64 FP64 instruction holds a 16-lane SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz
+        issue = counters["k_flux_valu_wave_insts"] * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * flux_ms * 1e-3)
+        roofline["valu_issue_frac"] = round(issue, 4)
+        roofline["valu_wave_insts_per_launch"] = counters["k_flux_valu_wave_insts"]
     extra = {"gradient_kernel_ms": round(grad_ms, 5),
              "gradient_kernel_gbs": round(grad_bytes / (grad_ms * 1e-3) / 1e9, 1) if grad_ms > 0 else None,
              "eval_hbm_frac": round(eval_bytes * args.steps / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
