@@ -39,6 +39,7 @@ struct AdjointData {
   double* R2 = nullptr;    // [4][N]
   double* dR = nullptr;    // [4][N]
   double* J = nullptr;     // [tiles][kJacCols][kEntryTile] face Jacobians per face-entry slot
+  bool jfact = false;      // J in the factored form (jacobian_factored of its linearisation)
   double* Gbar = nullptr;  // [8][N]
   double* out = nullptr;   // [4][N]
   double* psi = nullptr;   // [4][N]
@@ -440,6 +441,7 @@ int enqueue_jacobian(aero_hip_mesh* m, const aero_residual_config* cfg, char* er
   AERO_TRY(aero_host::launch_face_jacobian(a->am, cfg->flux_scheme, cfg->limiter, cfg->second_order != 0, U, m->G, o,
                                            a->J, m->stream),
            "k_face_jacobian");
+  a->jfact = aero_dev::jacobian_factored(cfg->second_order != 0, o.frozen != nullptr);
   return AERO_OK;
 }
 
@@ -618,7 +620,8 @@ int aero_hip_residual_vjp(aero_hip_mesh* m, const aero_residual_config* cfg, con
   AdjointData* a = m->adj;
   rc = upload_internal(m, psi, a->psi, err, errlen);
   if (rc) return rc;
-  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out, m->stream),
+  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, a->jfact, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out,
+                                         m->stream),
            "adjoint product");
   return download_internal(m, a->out, out, err, errlen);
 }
@@ -640,7 +643,8 @@ int aero_hip_residual_vjp_device(aero_hip_mesh* m, const aero_residual_config* c
   AdjointData* a = m->adj;
   AERO_TRY(aero_host::launch_aos_to_internal(m->N, m->d_perm, d_psi, a->psi, m->stream),
            "aero_hip_residual_vjp_device: psi");
-  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out, m->stream),
+  AERO_TRY(aero_host::launch_adj_product(a->am, a->J, a->jfact, m->U[m->cur], a->psi, cfg->gamma, a->Gbar, a->out,
+                                         m->stream),
            "adjoint product");
   AERO_TRY(aero_host::launch_internal_to_aos(m->N, m->d_perm, a->out, d_out, nullptr, nullptr, m->stream),
            "aero_hip_residual_vjp_device: output");
@@ -769,7 +773,7 @@ int one_iteration(aero_hip_mesh* m, const aero_residual_config* cfg, int forward
       r = a->dR;
     } else {  // psi = P^T D xs is current (set by the previous stage, or by the caller before stage 0)
       const bool want = st == 0 && record;
-      AERO_TRY(aero_host::launch_adj_stage(a->am, a->J, U, o, st, rk3, x0, xs, forcing ? a->rhs : nullptr, m->lam,
+      AERO_TRY(aero_host::launch_adj_stage(a->am, a->J, a->jfact, U, o, st, rk3, x0, xs, forcing ? a->rhs : nullptr, m->lam,
                                            ctl->cfl, a->Gbar, a->out, xo, want ? a->partials : nullptr, a->psi,
                                            m->stream),
                "adjoint stage");
@@ -1203,6 +1207,7 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
     AdjBatchSolve& s = hs[i];
     s.am = m->adj->am;
     s.J = m->adj->J;
+    s.fact = m->adj->jfact ? 1 : 0;
     s.U = m->U[m->cur];
     s.lam = m->lam;
     s.o = face_opts(cases[i].cfg);

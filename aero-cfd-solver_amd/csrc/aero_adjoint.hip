@@ -314,6 +314,116 @@ __global__ __launch_bounds__(kThreads) void k_face_jacobian(AdjMesh am, const do
   for (int k = 0; k < 4; ++k) t[(q0 + k) * kEntryTile] = col[k];
 }
 
+// Factored face Jacobians (the frozen limiter, or first order).  There the flux of face f depends on a
+// side's conserved state U and increments a only through the side's face state W = floors(c2p(U) + r a),
+// or c2p(U) where the face reverts to first order (r: the frozen ratios; no increments at first order).
+// So d(F A)/dU = (d(F A)/dW) M C and d(F A)/da = (d(F A)/dW) M diag(r) with C = dc2p/dU of the cell, the
+// same for all its faces, and M the 0/1 mask of the rho and p floors of the reconstruction (identity on a
+// reverted face).  Thread (f, dir), dir < 8: side dir >> 2, primitive var dir & 3; it stores column
+// var * 4 + k = d(F_k A)/dW_var times M_var, and column 16 + var = r_eff (r of this side and variable,
+// 0 on a reverted face, at first order and on boundary faces).  The gather multiplies by r_eff and, once
+// per cell, by C^T (k_adj_grad / adj_update): 20 of the 32 columns are read, 160 B per face side.
+template <int SCHEME, bool SECOND>
+__global__ __launch_bounds__(kThreads) void k_face_jacobian_w(AdjMesh am, const double* __restrict__ U,
+                                                              const double* __restrict__ G, FaceOpts o0,
+                                                              double* __restrict__ J) {
+  const long long tid = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  const int F = am.F;
+  if (tid >= 8LL * F) return;
+  const int f = static_cast<int>(tid >> 3);
+  const int dir = static_cast<int>(tid & 7);
+  const int side = dir >> 2;
+  const int var = dir & 3;
+  const int slot = am.f_slot[side * F + f];
+  if (slot < 0) return;  // the missing neighbour side of a boundary face
+  const int ld = am.ld;
+  const double g = o0.g;
+  const int co = am.fo[f], cn = am.fn[f];
+  FaceOptsT<Dual> o;
+  o.ff.r = o0.ff.r; o.ff.ru = o0.ff.ru; o.ff.rv = o0.ff.rv; o.ff.E = o0.ff.E;
+  o.g = g;
+  o.low_mach_fix = o0.low_mach_fix;
+  o.low_mach_cutoff = o0.low_mach_cutoff;
+  o.stab_floor = o0.stab_floor;
+  o.char_bc = o0.char_bc;
+  o.precond = o0.precond;
+  o.mach_ref = o0.mach_ref;
+  o.mach_min = o0.mach_min;
+  o.beta_min = o0.beta_min;
+  o.beta_max = o0.beta_max;
+  o.as_fix = o0.as_fix;
+  o.as_cut = o0.as_cut;
+  o.as_fmin = o0.as_fmin;
+  o.as_w = o0.as_w;
+  o.frozen = o0.frozen;
+  const double A = am.fA[f];
+  const Dual nx(am.fnx[f]), ny(am.fny[f]);
+  double col[4] = {0.0, 0.0, 0.0, 0.0};
+  double reff = 0.0;
+  const auto seeded = [&](const P4& w, bool seed) {  // w as duals, d/dW_var = 1 where seeded
+    P4T<Dual> d;
+    d.r = Dual(w.r, seed && var == 0 ? 1.0 : 0.0);
+    d.u = Dual(w.u, seed && var == 1 ? 1.0 : 0.0);
+    d.v = Dual(w.v, seed && var == 2 ? 1.0 : 0.0);
+    d.p = Dual(w.p, seed && var == 3 ? 1.0 : 0.0);
+    return d;
+  };
+  if (cn >= 0) {
+    // the face states of interior_face_src on doubles: reconstruction, floors, validity fallback
+    P4 L = c2p(load_c4(U, ld, co), g), Rt = c2p(load_c4(U, ld, cn), g);
+    bool mask = true;  // this side's variable passes its floor (always for u, v and on a reverted face)
+    if (SECOND) {
+      const double fx = am.ffx[f], fy = am.ffy[f];
+      const double dox = fx - am.ccx[co], doy = fy - am.ccy[co];
+      const double dnx = fx - am.ccx[cn], dny = fy - am.ccy[cn];
+      const double* fr = o0.frozen;
+      P4 l2, r2;
+      double* pl[4] = {&l2.r, &l2.u, &l2.v, &l2.p};
+      double* pr[4] = {&r2.r, &r2.u, &r2.v, &r2.p};
+      const double ql0[4] = {L.r, L.u, L.v, L.p}, qr0[4] = {Rt.r, Rt.u, Rt.v, Rt.p};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double ao = recon_incr(G[(2 * k) * ld + co], G[(2 * k + 1) * ld + co], dox, doy);
+        const double an = recon_incr(G[(2 * k) * ld + cn], G[(2 * k + 1) * ld + cn], dnx, dny);
+        *pl[k] = ql0[k] + fr[k * F + f] * ao;
+        *pr[k] = qr0[k] + fr[(4 + k) * F + f] * an;
+      }
+      const P4& q = side ? r2 : l2;
+      mask = !((var == 0 && q.r < kFloor) || (var == 3 && q.p < kFloor));
+      l2.r = dmax(l2.r, kFloor);
+      r2.r = dmax(r2.r, kFloor);
+      l2.p = dmax(l2.p, kFloor);
+      r2.p = dmax(r2.p, kFloor);
+      if (valid_prim(l2) && valid_prim(r2)) {
+        L = l2;
+        Rt = r2;
+        reff = fr[(4 * side + var) * F + f];
+      } else {
+        mask = true;
+      }
+    }
+    if (mask) {
+      const FluxT<Dual> Fl =
+          flux_of_states<SCHEME, false>(seeded(L, side == 0), seeded(Rt, side == 1), nx, ny, o);
+      col[0] = Fl.f0.d * A;
+      col[1] = Fl.f1.d * A;
+      col[2] = Fl.f2.d * A;
+      col[3] = Fl.f3.d * A;
+    }
+  } else {
+    const FluxT<Dual> Fl = boundary_face_prim<SCHEME>(seeded(c2p(load_c4(U, ld, co), g), true), am.fbc[f] == 2, nx,
+                                                      ny, o);
+    col[0] = Fl.f0.d * A;
+    col[1] = Fl.f1.d * A;
+    col[2] = Fl.f2.d * A;
+    col[3] = Fl.f3.d * A;
+  }
+  double* t = J + static_cast<size_t>(slot / kEntryTile) * (kJacCols * kEntryTile) + (slot % kEntryTile);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t[(var * 4 + k) * kEntryTile] = col[k];
+  t[(16 + var) * kEntryTile] = reff;
+}
+
 // ---- adjoint product out = R_U^T psi ---------------------------------------------------------------
 // Face f adds F_f A_f to R_o and subtracts it from R_n, so its weight is w = psi_o - psi_n (psi_o on
 // boundary faces).  Per cell, over its face entries (increasing face index), its side of J_f^T w: the
@@ -384,8 +494,10 @@ __device__ __forceinline__ void wide_accumulate(double* acc, const WideCell& w, 
 }
 
 // One face entry of the gather for NS solves sharing the face Jacobians: val[12 s + d] (d < 4) the direct
-// part, val[12 s + 4 + j] the gradient adjoint of solve s.
-template <int NS>
+// part, val[12 s + 4 + j] the gradient adjoint of solve s.  FACT (factored Jacobians, k_face_jacobian_w):
+// the direct part is in the primitive variables (C^T is applied per cell afterwards), and the increment
+// columns are the direct ones times r_eff.
+template <int NS, bool FACT>
 __device__ __forceinline__ bool adj_gather_entry(const AdjMesh& am, const double* __restrict__ J,
                                                  const double* const* psi, const double (*pc)[4], const WideCell& w,
                                                  int i, double* val) {
@@ -410,6 +522,19 @@ __device__ __forceinline__ bool adj_gather_entry(const AdjMesh& am, const double
     for (int q = 0; q < NS; ++q) val[12 * q + d] = t[q];
   }
   const double dx = am.e_dx[s], dy = am.e_dy[s];
+  if (FACT) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const double r = Jt[(16 + d) * kEntryTile];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const double t = r * val[12 * q + d];
+        val[12 * q + 4 + 2 * d] = t * dx;
+        val[12 * q + 4 + 2 * d + 1] = t * dy;
+      }
+    }
+    return true;
+  }
 #pragma unroll
   for (int d = 0; d < 4; ++d) {
     double t[NS];
@@ -432,7 +557,7 @@ __device__ __forceinline__ bool adj_gather_entry(const AdjMesh& am, const double
 
 // The gather of entry group grp for NS solves: direct[q] = the direct part of R_U^T psi[q], Gbar[q] its
 // gradient adjoint.  acc (WIDE): 12 NS kEntryTile doubles of LDS.
-template <bool WIDE, int NS>
+template <bool WIDE, int NS, bool FACT>
 __device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double* __restrict__ J,
                                                  const double* const* psi, double* const* direct,
                                                  double* const* Gbar, int grp, double* acc) {
@@ -445,7 +570,7 @@ __device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double
     for (int k = 0; k < 4; ++k) pc[q][k] = w.live ? psi[q][k * ld + w.c] : 0.0;
   if (WIDE) {
     wide_accumulate<12 * NS>(acc, w, [&](int i, double* val) {
-      return adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
+      return adj_gather_entry<NS, FACT>(am, J, psi, pc, w, i, val);
     });
     if (!w.live) return;
 #pragma unroll
@@ -468,7 +593,7 @@ __device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double
     for (int v = 0; v < 12 * NS; ++v) sum[v] = 0.0;
     for (int i = 0; i < w.ne; ++i) {
       double val[12 * NS];
-      adj_gather_entry<NS>(am, J, psi, pc, w, i, val);
+      adj_gather_entry<NS, FACT>(am, J, psi, pc, w, i, val);
 #pragma unroll
       for (int v = 0; v < 12 * NS; ++v) sum[v] += val[v];
     }
@@ -482,7 +607,7 @@ __device__ __forceinline__ void adj_gather_group(const AdjMesh& am, const double
   }
 }
 
-template <bool WIDE>
+template <bool WIDE, bool FACT>
 __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather(AdjMesh am,
                                                                                  const double* __restrict__ J,
                                                                                  const double* __restrict__ psi,
@@ -492,7 +617,7 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather
   const double* p[1] = {psi};
   double* o[1] = {out};
   double* g[1] = {Gbar};
-  adj_gather_group<WIDE, 1>(am, J, p, o, g, blockIdx.x, acc);
+  adj_gather_group<WIDE, 1, FACT>(am, J, p, o, g, blockIdx.x, acc);
 }
 
 // The gradient-adjoint term of one face entry of cell c, val[k] for primitive k (returns false where the
@@ -562,8 +687,9 @@ __device__ __forceinline__ void adj_wbar(const AdjMesh& am, const double* __rest
   }
 }
 
-// Per cell: Wbar (adj_wbar), then out += (dW/dU)^T Wbar.
-template <int GK>
+// Per cell: Wbar (adj_wbar), then out += (dW/dU)^T Wbar; FACT: out holds the gather's primitive-variable
+// part s, and out = (dW/dU)^T (Wbar + s).
+template <int GK, bool FACT>
 __global__ __launch_bounds__(kThreads) void k_adj_grad(AdjMesh am, const double* __restrict__ U, double g,
                                                        const double* __restrict__ Gbar, double* __restrict__ out) {
   const int c = blockIdx.x * kThreads + threadIdx.x;
@@ -571,6 +697,13 @@ __global__ __launch_bounds__(kThreads) void k_adj_grad(AdjMesh am, const double*
   const int ld = am.ld;
   double wb[4];
   adj_wbar<GK>(am, Gbar, c, wb);
+  if (FACT) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      wb[k] += out[k * ld + c];
+      out[k * ld + c] = 0.0;
+    }
+  }
   // (dW/dU)^T Wbar by four dual evaluations of the primitive map (euler_flux.cpp:115-127).
   const C4 u0 = load_c4(U, ld, c);
   double add[4];
@@ -650,6 +783,8 @@ __device__ __forceinline__ void c2p_jacobian(const C4& u0, double g, double (*ja
 
 // r = direct + (dW/dU)^T Wbar, the update x* = x_s - (r - rhs) with the RK stage combination, and
 // psi = P^T D x* for the next stage; returns the squared residual |r - rhs|^2 of the cell.
+// FACT: direct holds the gather's primitive-variable part s, and r = (dW/dU)^T (Wbar + s).
+template <bool FACT>
 __device__ __forceinline__ double adj_update(int ld, const C4& u0, const FaceOpts& o, const double (*jac)[4],
                                              const double* wb, const double* __restrict__ direct, int stage, int rk3,
                                              const double* __restrict__ x0, const double* __restrict__ xs,
@@ -657,10 +792,18 @@ __device__ __forceinline__ double adj_update(int ld, const C4& u0, const FaceOpt
                                              double cfl, double* __restrict__ xout, double* __restrict__ psi, int c) {
   double sq = 0.0;
   double r[4];
+  if (FACT) {
+    double v[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const double add = jac[j][0] * wb[0] + jac[j][1] * wb[1] + jac[j][2] * wb[2] + jac[j][3] * wb[3];
-    r[j] = direct[j * ld + c] + add;
+    for (int k = 0; k < 4; ++k) v[k] = wb[k] + direct[k * ld + c];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = jac[j][0] * v[0] + jac[j][1] * v[1] + jac[j][2] * v[2] + jac[j][3] * v[3];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double add = jac[j][0] * wb[0] + jac[j][1] * wb[1] + jac[j][2] * wb[2] + jac[j][3] * wb[3];
+      r[j] = direct[j * ld + c] + add;
+    }
   }
   double x[4];
 #pragma unroll
@@ -746,7 +889,7 @@ __device__ __forceinline__ void adj_wbar_cells(const AdjMesh& am, const double* 
 // gather and (dW/dU)^T of k_adj_grad (same arithmetic), r = R_U^T psi, the update
 // x* = x_s - (r - rhs) with the RK stage combination, the squared norm of r - rhs into the block partial,
 // and psi = P^T D x* for the next stage (k_adj_scale's arithmetic).
-template <bool WIDE, int GK>
+template <bool WIDE, int GK, bool FACT>
 __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_update(
     AdjMesh am, const double* __restrict__ U, FaceOpts o, const double* __restrict__ Gbar,
     const double* __restrict__ direct, int stage, int rk3, const double* __restrict__ x0,
@@ -763,7 +906,7 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_u
     const C4 u0 = load_c4(U, am.ld, w.c);
     double jac[4][4];
     c2p_jacobian(u0, o.g, jac);
-    sq = adj_update(am.ld, u0, o, jac, wb[0], direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, w.c);
+    sq = adj_update<FACT>(am.ld, u0, o, jac, wb[0], direct, stage, rk3, x0, xs, rhs, lam, cfl, xout, psi, w.c);
   }
   group_sum_store(sq, red, partials ? partials + blockIdx.x : nullptr);
 }
@@ -784,17 +927,23 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather
   const bool ab = wk.y >= 0 && ctl[wk.y].active != 0;
   if (!aa && !ab) return;  // uniform over the workgroup
   const AdjBatchSolve& sa = solves[aa ? wk.x : wk.y];
-  if (aa && ab) {
+  if (aa && ab) {  // paired solves share one linearisation, so one Jacobian form
     const AdjBatchSolve& sb = solves[wk.y];
     const double* p[2] = {sa.psi, sb.psi};
     double* o[2] = {sa.direct, sb.direct};
     double* g[2] = {sa.Gbar, sb.Gbar};
-    adj_gather_group<WIDE, 2>(sa.am, sa.J, p, o, g, wk.z, acc);
+    if (sa.fact)
+      adj_gather_group<WIDE, 2, true>(sa.am, sa.J, p, o, g, wk.z, acc);
+    else
+      adj_gather_group<WIDE, 2, false>(sa.am, sa.J, p, o, g, wk.z, acc);
   } else {
     const double* p[1] = {sa.psi};
     double* o[1] = {sa.direct};
     double* g[1] = {sa.Gbar};
-    adj_gather_group<WIDE, 1>(sa.am, sa.J, p, o, g, wk.z, acc);
+    if (sa.fact)
+      adj_gather_group<WIDE, 1, true>(sa.am, sa.J, p, o, g, wk.z, acc);
+    else
+      adj_gather_group<WIDE, 1, false>(sa.am, sa.J, p, o, g, wk.z, acc);
   }
 }
 
@@ -829,13 +978,18 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_u
     c2p_jacobian(u0, s0.o.g, jac);
     if (aa) {
       const AdjBatchSolve& s = solves[wk.x];
-      sqa = adj_update(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs, s.lam, s.cfl,
-                       s.x[ixo], s.psi, c);
+      sqa = s.fact ? adj_update<true>(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+                                      s.lam, s.cfl, s.x[ixo], s.psi, c)
+                   : adj_update<false>(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+                                       s.lam, s.cfl, s.x[ixo], s.psi, c);
     }
     if (ab) {
       const AdjBatchSolve& s = solves[wk.y];
-      sqb = adj_update(am.ld, u0, s.o, jac, wb[aa ? 1 : 0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
-                       s.lam, s.cfl, s.x[ixo], s.psi, c);
+      const double* wq = wb[aa ? 1 : 0];
+      sqb = s.fact ? adj_update<true>(am.ld, u0, s.o, jac, wq, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+                                      s.lam, s.cfl, s.x[ixo], s.psi, c)
+                   : adj_update<false>(am.ld, u0, s.o, jac, wq, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+                                       s.lam, s.cfl, s.x[ixo], s.psi, c);
     }
   }
   if (want_partials) {
@@ -1116,11 +1270,16 @@ static void face_jac_t(const AdjMesh& am, const double* U, const double* G, cons
 
 hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool second, const double* U,
                                 const double* G, const FaceOpts& o, double* J, hipStream_t s) {
-  if (second && o.frozen) {  // the frozen ratios replace the limiter
-    if (scheme)
-      face_jac_t<1, 0, true, true>(am, U, G, o, J, s);
+  if (jacobian_factored(second, o.frozen != nullptr)) {  // frozen ratios or first order: d(F A)/dW per side
+    const dim3 grid(blocks_for(8LL * am.F)), block(kThreads);
+    if (scheme && second)
+      hipLaunchKernelGGL((k_face_jacobian_w<1, true>), grid, block, 0, s, am, U, G, o, J);
+    else if (scheme)
+      hipLaunchKernelGGL((k_face_jacobian_w<1, false>), grid, block, 0, s, am, U, G, o, J);
+    else if (second)
+      hipLaunchKernelGGL((k_face_jacobian_w<0, true>), grid, block, 0, s, am, U, G, o, J);
     else
-      face_jac_t<0, 0, true, true>(am, U, G, o, J, s);
+      hipLaunchKernelGGL((k_face_jacobian_w<0, false>), grid, block, 0, s, am, U, G, o, J);
     return hipGetLastError();
   }
   const int key = (scheme ? 4 : 0) | (limiter ? 2 : 0) | (second ? 1 : 0);
@@ -1137,16 +1296,27 @@ hipError_t launch_face_jacobian(const AdjMesh& am, int scheme, int limiter, bool
   return hipGetLastError();
 }
 
-hipError_t launch_adj_product(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
-                              double* Gbar, double* out, hipStream_t s) {
+template <bool FACT>
+static void adj_product_t(const AdjMesh& am, const double* J, const double* U, const double* psi, double g,
+                          double* Gbar, double* out, hipStream_t s) {
   if (entry_groups(am.N) < kWideBelow)
-    hipLaunchKernelGGL(k_adj_gather<true>, dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, out, Gbar);
+    hipLaunchKernelGGL((k_adj_gather<true, FACT>), dim3(entry_groups(am.N)), dim3(kWideThreads), 0, s, am, J, psi, out,
+                       Gbar);
   else
-    hipLaunchKernelGGL(k_adj_gather<false>, dim3(entry_groups(am.N)), dim3(kEntryTile), 0, s, am, J, psi, out, Gbar);
+    hipLaunchKernelGGL((k_adj_gather<false, FACT>), dim3(entry_groups(am.N)), dim3(kEntryTile), 0, s, am, J, psi, out,
+                       Gbar);
   if (am.grad == 1)
-    hipLaunchKernelGGL(k_adj_grad<1>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
+    hipLaunchKernelGGL((k_adj_grad<1, FACT>), dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
   else
-    hipLaunchKernelGGL(k_adj_grad<0>, dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
+    hipLaunchKernelGGL((k_adj_grad<0, FACT>), dim3(blocks_for(am.N)), dim3(kThreads), 0, s, am, U, g, Gbar, out);
+}
+
+hipError_t launch_adj_product(const AdjMesh& am, const double* J, bool fact, const double* U, const double* psi,
+                              double g, double* Gbar, double* out, hipStream_t s) {
+  if (fact)
+    adj_product_t<true>(am, J, U, psi, g, Gbar, out, s);
+  else
+    adj_product_t<false>(am, J, U, psi, g, Gbar, out, s);
   return hipGetLastError();
 }
 
@@ -1156,28 +1326,33 @@ hipError_t launch_objective_gradient(const AdjMesh& am, const double* U, double 
   return hipGetLastError();
 }
 
-template <bool WIDE>
+template <bool WIDE, bool FACT>
 static void adj_stage_t(const AdjMesh& am, const double* J, const double* U, const FaceOpts& o, int stage, int rk3,
                         const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
                         double* Gbar, double* direct, double* xout, double* partials, double* psi, hipStream_t s) {
   const dim3 grid(entry_groups(am.N)), block(WIDE ? kWideThreads : kEntryTile);
-  hipLaunchKernelGGL(k_adj_gather<WIDE>, grid, block, 0, s, am, J, psi, direct, Gbar);
+  hipLaunchKernelGGL((k_adj_gather<WIDE, FACT>), grid, block, 0, s, am, J, psi, direct, Gbar);
   if (am.grad == 1)
-    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 1>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0, xs,
-                       rhs, lam, cfl, xout, partials, psi);
+    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 1, FACT>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0,
+                       xs, rhs, lam, cfl, xout, partials, psi);
   else
-    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 0>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0, xs,
-                       rhs, lam, cfl, xout, partials, psi);
+    hipLaunchKernelGGL((k_adj_grad_update<WIDE, 0, FACT>), grid, block, 0, s, am, U, o, Gbar, direct, stage, rk3, x0,
+                       xs, rhs, lam, cfl, xout, partials, psi);
 }
 
-hipError_t launch_adj_stage(const AdjMesh& am, const double* J, const double* U, const FaceOpts& o, int stage, int rk3,
-                            const double* x0, const double* xs, const double* rhs, const double* lam, double cfl,
-                            double* Gbar, double* direct, double* xout, double* partials, double* psi,
-                            hipStream_t s) {
-  if (entry_groups(am.N) < kWideBelow)
-    adj_stage_t<true>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
+hipError_t launch_adj_stage(const AdjMesh& am, const double* J, bool fact, const double* U, const FaceOpts& o,
+                            int stage, int rk3, const double* x0, const double* xs, const double* rhs,
+                            const double* lam, double cfl, double* Gbar, double* direct, double* xout,
+                            double* partials, double* psi, hipStream_t s) {
+  const bool wide = entry_groups(am.N) < kWideBelow;
+  if (wide && fact)
+    adj_stage_t<true, true>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
+  else if (wide)
+    adj_stage_t<true, false>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
+  else if (fact)
+    adj_stage_t<false, true>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
   else
-    adj_stage_t<false>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
+    adj_stage_t<false, false>(am, J, U, o, stage, rk3, x0, xs, rhs, lam, cfl, Gbar, direct, xout, partials, psi, s);
   return hipGetLastError();
 }
 

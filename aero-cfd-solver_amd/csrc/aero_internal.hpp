@@ -200,9 +200,15 @@ struct GeoTan {
 
 // One solve of a batched adjoint solve: the linearisation (face Jacobians, state, spectral radius,
 // options) of its mesh handle and its own right-hand side and iteration buffers ([4][N] / [8][N], ld N).
+// Whether a linearisation's face Jacobians take the factored form of k_face_jacobian_w (d(F A)/dW per
+// side, with the frozen ratios and the cells' primitive-map Jacobians applied in the gather): the frozen
+// limiter, or first order.  With the limiter live, a side's face state depends on the other cell too.
+__host__ __device__ inline bool jacobian_factored(bool second, bool frozen) { return !second || frozen; }
+
 struct AdjBatchSolve {
   AdjMesh am;
   const double* J;
+  int fact;          // J in the factored form (jacobian_factored)
   const double* U;
   const double* lam;
   FaceOpts o;
@@ -281,14 +287,15 @@ hipError_t launch_tan_face(const aero_dev::AdjMesh& am, int scheme, int limiter,
 hipError_t launch_tan_gather(const aero_dev::AdjMesh& am, const double* fa, double* R, double* dR, hipStream_t s);
 hipError_t launch_face_jacobian(const aero_dev::AdjMesh& am, int scheme, int limiter, bool second, const double* U,
                                 const double* G, const aero_dev::FaceOpts& o, double* J, hipStream_t s);
-hipError_t launch_adj_product(const aero_dev::AdjMesh& am, const double* J, const double* U, const double* psi,
-                              double g, double* Gbar, double* out, hipStream_t s);
+// fact: J was built in the factored form (jacobian_factored of the linearisation).
+hipError_t launch_adj_product(const aero_dev::AdjMesh& am, const double* J, bool fact, const double* U,
+                              const double* psi, double g, double* Gbar, double* out, hipStream_t s);
 // One stage of the transposed pseudo-time iteration (k_adj_gather + k_adj_grad_update): reads psi, writes
 // xout, the block partials (may be null) and the next stage's psi.
-hipError_t launch_adj_stage(const aero_dev::AdjMesh& am, const double* J, const double* U, const aero_dev::FaceOpts& o,
-                            int stage, int rk3, const double* x0, const double* xs, const double* rhs,
-                            const double* lam, double cfl, double* Gbar, double* direct, double* xout,
-                            double* partials, double* psi, hipStream_t s);
+hipError_t launch_adj_stage(const aero_dev::AdjMesh& am, const double* J, bool fact, const double* U,
+                            const aero_dev::FaceOpts& o, int stage, int rk3, const double* x0, const double* xs,
+                            const double* rhs, const double* lam, double cfl, double* Gbar, double* direct,
+                            double* xout, double* partials, double* psi, hipStream_t s);
 hipError_t launch_objective_gradient(const aero_dev::AdjMesh& am, const double* U, double g, const double* coef,
                                      double* out, hipStream_t s);
 // Batched transposed stage over (solve a, solve b or -1, block) work items (two solves of one

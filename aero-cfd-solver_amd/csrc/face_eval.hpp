@@ -75,6 +75,19 @@ __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, 
   *rn = quantize_ratio(db != 0.0 ? limiter<LIMITER>(db, -dc) / db : 0.0);
 }
 
+// The face flux from the two face states' primitives (residual_assembly.cpp:122-137): conservative maps,
+// face beta, acoustic scale and the Riemann flux.  The stored face Jacobians of the frozen adjoint
+// differentiate this function of (L, R).
+template <int SCHEME, bool PLAIN, class T>
+__device__ __forceinline__ FluxT<T> flux_of_states(const P4T<T>& L, const P4T<T>& Rt, T nx, T ny,
+                                                   const FaceOptsT<T>& o) {
+  const C4T<T> ul = p2c(L, o.g);
+  const C4T<T> ur = p2c(Rt, o.g);
+  const T fb = face_beta<PLAIN>(ul, ur, o);
+  const T sc = acoustic_scale<PLAIN>(ul, ur, fb, o);
+  return face_flux<SCHEME, PLAIN>(ul, ur, nx, ny, sc, o);
+}
+
 // Interior face with the first-order fallback (residual_assembly.cpp:107-137), the reconstruction
 // increments taken from src.  *reverted reports the fallback to cell values.
 // FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
@@ -109,11 +122,7 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
       *reverted = true;
     }
   }
-  const C4T<T> ul = p2c(L, g);
-  const C4T<T> ur = p2c(Rt, g);
-  const T fb = face_beta<PLAIN>(ul, ur, o);
-  const T sc = acoustic_scale<PLAIN>(ul, ur, fb, o);
-  return face_flux<SCHEME, PLAIN>(ul, ur, nx, ny, sc, o);
+  return flux_of_states<SCHEME, PLAIN>(L, Rt, nx, ny, o);
 }
 
 // Interior face (residual_assembly.cpp:107-137).  go / gn are the cells' primitive gradients (rho_x,
@@ -139,10 +148,12 @@ __device__ __forceinline__ FluxT<T> interior_face_incr(const C4T<T>& uo, const C
 
 // Boundary face: slip wall or farfield on the owner's first-order state (residual_assembly.cpp:138-166,
 // boundary_conditions.cpp:81-123).
+// The same from the owner's primitive state W = c2p(uo) (the frozen adjoint's stored Jacobians
+// differentiate it in W).
 template <int SCHEME, bool PLAIN = false, class T>
-__device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
+__device__ __forceinline__ FluxT<T> boundary_face_prim(const P4T<T>& W, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
   const double g = o.g;
-  const C4T<T> in = p2c(c2p(uo, g), g);
+  const C4T<T> in = p2c(W, g);
   // The exterior state of beta and the acoustic scale: the interior state at a wall, the freestream at a
   // farfield (selected per component, which keeps both in registers).
   C4T<T> ext;
@@ -169,6 +180,11 @@ __device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T
     F = face_flux<SCHEME, PLAIN>(in, bs, nx, ny, sc, o);
   }
   return F;
+}
+
+template <int SCHEME, bool PLAIN = false, class T>
+__device__ __forceinline__ FluxT<T> boundary_face(const C4T<T>& uo, bool wall, T nx, T ny, const FaceOptsT<T>& o) {
+  return boundary_face_prim<SCHEME, PLAIN>(c2p(uo, o.g), wall, nx, ny, o);
 }
 
 }  // namespace aero_dev

@@ -692,8 +692,9 @@ def replace_obj(obj, w_cl, w_cd=0.0, w_cm=0.0):
 def run_vjp(args) -> int:
     """The adjoint product R_U^T psi (aero_hip_residual_vjp: face Jacobians, then the gather and gradient
     transpose kernels) on the jittered mixed mesh of --vjp's size, HLLC + venkat at a perturbed M 0.6
-    state; --vjp-calls calls.  Meant to run under rocprofv3 for the per-kernel times; the line reports
-    the wall time per call."""
+    state; --vjp-calls calls (--vjp-frozen: with the limiter frozen at that state, so that the face
+    Jacobians take their factored form).  Meant to run under rocprofv3 for the per-kernel times; the line
+    reports the wall time per call."""
     import numpy as np
     import aero_hip
     from aero_hip import HipMesh, freestream, generate_mesh, residual_config
@@ -710,6 +711,10 @@ def run_vjp(args) -> int:
     walls = []
     with HipMesh(m, block_cells=args.batch_block_cells) as hm:
         hm.set_state(U)
+        if args.vjp_frozen:
+            hm.freeze_limiter(cfg)
+            cfg = residual_config(ff, flux="hllc", limiter="venkat", second_order=True, stab_mach_floor=0.02,
+                                  limiter_frozen=True)
         for i in range(args.vjp_calls):
             t0 = time.time()
             out = hm.residual_vjp(cfg, psi)
@@ -718,7 +723,8 @@ def run_vjp(args) -> int:
     assert np.all(np.isfinite(out))
     print(json.dumps({"metric": "adjoint_product_ms", "value": round(1e3 * float(np.median(walls)), 4), "unit": "ms",
                       "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
-                      "config": {"workload": f"vjp_mixed{args.vjp}_hllc_venkat", "cells": m.num_cells,
+                      "config": {"workload": f"vjp_mixed{args.vjp}_hllc_venkat" + ("_frozen" if args.vjp_frozen else ""),
+                                 "cells": m.num_cells,
                                  "faces": m.num_faces, "calls": args.vjp_calls}}))
     return 0
 
@@ -788,6 +794,7 @@ def main() -> int:
                     help="instead of the benchmark, time --vjp-calls adjoint products on that mixed mesh (for "
                          "rocprofv3 kernel statistics)")
     ap.add_argument("--vjp-calls", type=int, default=12)
+    ap.add_argument("--vjp-frozen", action="store_true", help="--vjp: with the limiter frozen at the state")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
