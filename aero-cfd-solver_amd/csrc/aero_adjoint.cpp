@@ -1271,9 +1271,14 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
   AERO_TRY(hipMemcpy(d_tab, hs.data(), n * sizeof(AdjBatchSolve), hipMemcpyHostToDevice), "adjoint batch table");
   AERO_TRY(hipMemcpy(d_ctl, hc.data(), n * sizeof(AdjBatchCtl), hipMemcpyHostToDevice), "adjoint batch control");
   AERO_TRY(hipMemcpy(d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice), "adjoint batch work");
-  // One replay unit: an iteration under RK3 (its ring ends where it starts), two under RK1.
+  // One replay unit: four iterations under RK3 (its ring ends where it starts), two under RK1 (whose ring
+  // alternates).  Several iterations per graph amortise the launch of the graph; a solve that stops inside
+  // a unit is skipped by the rest of it, so iteration counts and results do not depend on the unit.
   const int grad = cases[0].mesh->dm.grad;
-  const int unit = rk3 ? 1 : 2;
+  const int unit = rk3 ? 4 : 2;
+  int n_fact = 0;
+  for (const AdjBatchSolve& s : hs) n_fact += s.fact ? 1 : 0;
+  const int fact_mode = n_fact == 0 ? 0 : (n_fact == n ? 1 : 2);
   {
     hipGraph_t graph = nullptr;
     AERO_TRY(hipStreamBeginCapture(r.st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -1282,10 +1287,10 @@ int aero_hip_adjoint_solve_batch(aero_adjoint_case* cases, int32_t n, aero_linea
       if (rk3) {
         const int ixs[3] = {0, 1, 2}, ixo[3] = {1, 2, 0};
         for (int st = 0; st < 3 && e == hipSuccess; ++st)
-          e = aero_host::launch_adj_batch_stage(grad, d_tab, d_ctl, d_work, nwork, st, 1, 0, ixs[st], ixo[st],
-                                                st == 0 ? 1 : 0, r.st);
+          e = aero_host::launch_adj_batch_stage(grad, fact_mode, d_tab, d_ctl, d_work, nwork, st, 1, 0, ixs[st],
+                                                ixo[st], st == 0 ? 1 : 0, r.st);
       } else {
-        e = aero_host::launch_adj_batch_stage(grad, d_tab, d_ctl, d_work, nwork, 0, 0, u, u, 1 - u, 1, r.st);
+        e = aero_host::launch_adj_batch_stage(grad, fact_mode, d_tab, d_ctl, d_work, nwork, 0, 0, u, u, 1 - u, 1, r.st);
       }
       if (e == hipSuccess) e = aero_host::launch_adj_batch_control(d_tab, d_ctl, n, ctl->tol, ctl->max_iters, r.st);
     }

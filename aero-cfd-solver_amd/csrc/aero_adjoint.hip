@@ -917,7 +917,9 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_u
 // per cell serve both.  Each solve runs the same per-cell arithmetic as a single solve, so every solve
 // of a batch reproduces its single solve bit for bit; solves that have stopped are neither computed nor
 // written.
-template <bool WIDE>
+// FM: the solves' face-Jacobian form, 0 all unfactored, 1 all factored, 2 each solve's own (AdjBatchSolve
+// fact, taken per work item; both code paths then share the kernel's registers).
+template <bool WIDE, int FM>
 __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather_batch(
     const AdjBatchSolve* __restrict__ solves, const AdjBatchCtl* __restrict__ ctl, const int4* __restrict__ work) {
   __shared__ double acc[WIDE ? 24 * kEntryTile : 1];
@@ -932,22 +934,22 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_gather
     const double* p[2] = {sa.psi, sb.psi};
     double* o[2] = {sa.direct, sb.direct};
     double* g[2] = {sa.Gbar, sb.Gbar};
-    if (sa.fact)
-      adj_gather_group<WIDE, 2, true>(sa.am, sa.J, p, o, g, wk.z, acc);
+    if (FM == 1 || (FM == 2 && sa.fact))
+      adj_gather_group<WIDE, 2, FM != 0>(sa.am, sa.J, p, o, g, wk.z, acc);
     else
       adj_gather_group<WIDE, 2, false>(sa.am, sa.J, p, o, g, wk.z, acc);
   } else {
     const double* p[1] = {sa.psi};
     double* o[1] = {sa.direct};
     double* g[1] = {sa.Gbar};
-    if (sa.fact)
-      adj_gather_group<WIDE, 1, true>(sa.am, sa.J, p, o, g, wk.z, acc);
+    if (FM == 1 || (FM == 2 && sa.fact))
+      adj_gather_group<WIDE, 1, FM != 0>(sa.am, sa.J, p, o, g, wk.z, acc);
     else
       adj_gather_group<WIDE, 1, false>(sa.am, sa.J, p, o, g, wk.z, acc);
   }
 }
 
-template <bool WIDE, int GK>
+template <bool WIDE, int GK, int FM>
 __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_update_batch(
     const AdjBatchSolve* __restrict__ solves, const AdjBatchCtl* __restrict__ ctl, const int4* __restrict__ work,
     int stage, int rk3, int ix0, int ixs, int ixo, int want_partials) {
@@ -978,7 +980,8 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_u
     c2p_jacobian(u0, s0.o.g, jac);
     if (aa) {
       const AdjBatchSolve& s = solves[wk.x];
-      sqa = s.fact ? adj_update<true>(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+      sqa = (FM == 1 || (FM == 2 && s.fact))
+                ? adj_update<FM != 0>(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
                                       s.lam, s.cfl, s.x[ixo], s.psi, c)
                    : adj_update<false>(am.ld, u0, s.o, jac, wb[0], s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
                                        s.lam, s.cfl, s.x[ixo], s.psi, c);
@@ -986,7 +989,8 @@ __global__ __launch_bounds__(WIDE ? kWideThreads : kEntryTile) void k_adj_grad_u
     if (ab) {
       const AdjBatchSolve& s = solves[wk.y];
       const double* wq = wb[aa ? 1 : 0];
-      sqb = s.fact ? adj_update<true>(am.ld, u0, s.o, jac, wq, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
+      sqb = (FM == 1 || (FM == 2 && s.fact))
+                ? adj_update<FM != 0>(am.ld, u0, s.o, jac, wq, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
                                       s.lam, s.cfl, s.x[ixo], s.psi, c)
                    : adj_update<false>(am.ld, u0, s.o, jac, wq, s.direct, stage, rk3, s.x[ix0], s.x[ixs], s.rhs,
                                        s.lam, s.cfl, s.x[ixo], s.psi, c);
@@ -1356,28 +1360,41 @@ hipError_t launch_adj_stage(const AdjMesh& am, const double* J, bool fact, const
   return hipGetLastError();
 }
 
-template <bool WIDE>
+template <bool WIDE, int FM>
 static void adj_batch_stage_t(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
                               int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
                               hipStream_t s) {
   const dim3 grid(nwork), block(WIDE ? kWideThreads : kEntryTile);
-  hipLaunchKernelGGL(k_adj_gather_batch<WIDE>, grid, block, 0, s, solves, ctl, work);
+  hipLaunchKernelGGL((k_adj_gather_batch<WIDE, FM>), grid, block, 0, s, solves, ctl, work);
   if (grad == 1)
-    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 1>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0, ixs,
-                       ixo, want_partials);
+    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 1, FM>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0,
+                       ixs, ixo, want_partials);
   else
-    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 0>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0, ixs,
-                       ixo, want_partials);
+    hipLaunchKernelGGL((k_adj_grad_update_batch<WIDE, 0, FM>), grid, block, 0, s, solves, ctl, work, stage, rk3, ix0,
+                       ixs, ixo, want_partials);
+}
+
+template <int FM>
+static void adj_batch_stage_fm(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
+                               int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
+                               hipStream_t s) {
+  if (nwork < kWideBelow)
+    adj_batch_stage_t<true, FM>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
+  else
+    adj_batch_stage_t<false, FM>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
 }
 
 // Work items are (solve pair, entry group): the four-lanes-per-cell shape below kWideBelow of them.
-hipError_t launch_adj_batch_stage(int grad, const AdjBatchSolve* solves, const AdjBatchCtl* ctl, const int4* work,
-                                  int nwork, int stage, int rk3, int ix0, int ixs, int ixo, int want_partials,
-                                  hipStream_t s) {
-  if (nwork < kWideBelow)
-    adj_batch_stage_t<true>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
+// fact_mode: 0 every solve's Jacobians unfactored, 1 every solve's factored, 2 mixed.
+hipError_t launch_adj_batch_stage(int grad, int fact_mode, const AdjBatchSolve* solves, const AdjBatchCtl* ctl,
+                                  const int4* work, int nwork, int stage, int rk3, int ix0, int ixs, int ixo,
+                                  int want_partials, hipStream_t s) {
+  if (fact_mode == 1)
+    adj_batch_stage_fm<1>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
+  else if (fact_mode == 0)
+    adj_batch_stage_fm<0>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
   else
-    adj_batch_stage_t<false>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
+    adj_batch_stage_fm<2>(grad, solves, ctl, work, nwork, stage, rk3, ix0, ixs, ixo, want_partials, s);
   return hipGetLastError();
 }
 

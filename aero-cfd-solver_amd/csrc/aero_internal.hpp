@@ -300,9 +300,10 @@ hipError_t launch_objective_gradient(const aero_dev::AdjMesh& am, const double* 
                                      double* out, hipStream_t s);
 // Batched transposed stage over (solve a, solve b or -1, block) work items (two solves of one
 // linearisation share an item); the iterate ring indices select x[] of each solve.  Then the per-solve control (residual norm, iteration count, stopping tests).
-hipError_t launch_adj_batch_stage(int grad, const aero_dev::AdjBatchSolve* solves, const aero_dev::AdjBatchCtl* ctl,
-                                  const int4* work, int nwork, int stage, int rk3, int ix0, int ixs, int ixo,
-                                  int want_partials, hipStream_t s);
+// fact_mode: 0 every solve's face Jacobians unfactored, 1 every solve's factored, 2 mixed.
+hipError_t launch_adj_batch_stage(int grad, int fact_mode, const aero_dev::AdjBatchSolve* solves,
+                                  const aero_dev::AdjBatchCtl* ctl, const int4* work, int nwork, int stage, int rk3,
+                                  int ix0, int ixs, int ixo, int want_partials, hipStream_t s);
 hipError_t launch_adj_batch_control(const aero_dev::AdjBatchSolve* solves, aero_dev::AdjBatchCtl* ctl, int nsolves,
                                     double tol, int max_iters, hipStream_t s);
 hipError_t launch_adj_scale(const aero_dev::AdjMesh& am, const double* lam, double cfl, const double* U,
