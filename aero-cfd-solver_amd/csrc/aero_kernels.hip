@@ -235,6 +235,75 @@ __device__ __forceinline__ int wave_min_i(int v) {
   return v;
 }
 
+// One face slot of flux_body's phase 1: the face flux times the area staged in LDS slot i, the fallback
+// counters updated.  FAST (PLAIN): interior faces only, with the exact shortcuts of flux_of_states; for a
+// boundary slot or one outside the shortcuts' bounds (*ok false) nothing is staged or counted.
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, bool PLAIN, bool FAST>
+__device__ __forceinline__ void flux_slot(const DevMesh& m, int s, int i, const FaceOpts& o,
+                                          const double* __restrict__ U, const double* __restrict__ G, double* lds,
+                                          int* n_fo, int* n_diff, int* first_fail, bool* ok) {
+  const int smax = m.smax;
+  const int ld = m.ld;
+  const int cn = m.s_n[s];
+  if constexpr (FAST) {
+    if (cn < 0) {
+      *ok = false;
+      return;
+    }
+  }
+  const int co = m.s_o[s];
+  const int meta = m.s_meta[s];
+  const double nx = m.s_nx[s];
+  const double ny = m.s_ny[s];
+  const double A = m.s_A[s];
+  Flux F;
+  bool reverted = false;
+  if (cn >= 0) {
+    double go[8], gn[8];
+    double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
+    if (SECOND) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        go[j] = G[j * ld + co];
+        gn[j] = G[j * ld + cn];
+      }
+      const double fx = m.s_fx[s], fy = m.s_fy[s];
+      dox = fx - m.ccx[co];
+      doy = fy - m.ccy[co];
+      dnx = fx - m.ccx[cn];
+      dny = fy - m.ccy[cn];
+    }
+    double fr[8];
+    if (FROZEN && SECOND) {  // one 16-byte load per slot: eight 16-bit ratios in units of 2^-15
+      const uint4 q4 = reinterpret_cast<const uint4*>(o.frozen_slots)[s];
+      const unsigned int w[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        fr[2 * j] = static_cast<double>(w[j] & 0xffffu) * kRatioStep;
+        fr[2 * j + 1] = static_cast<double>(w[j] >> 16) * kRatioStep;
+      }
+    }
+    F = interior_face<SCHEME, LIMITER, SECOND, FROZEN, PLAIN, FAST>(load_u(U, ld, co), load_u(U, ld, cn), go, gn,
+                                                                    nx, ny, dox, doy, dnx, dny, o, &reverted, fr,
+                                                                    ok);
+  } else if constexpr (!FAST) {
+    F = boundary_face<SCHEME, PLAIN>(load_u(U, ld, co), (meta & kMetaBcMask) == 2, nx, ny, o);
+  }
+  if constexpr (FAST) {
+    if (!*ok) return;
+  }
+  if (reverted && (meta & kMetaPrimary)) {
+    ++*n_fo;
+    *first_fail = min(*first_fail, m.s_face[s]);
+  }
+  if (F.fallback && (meta & kMetaPrimary)) ++*n_diff;
+  lds[i] = F.f0 * A;
+  lds[smax + i] = F.f1 * A;
+  lds[2 * smax + i] = F.f2 * A;
+  lds[3 * smax + i] = F.f3 * A;
+  lds[4 * smax + i] = F.lam * A;
+}
+
 // Face flux + accumulation (residual_assembly.cpp:73-182), with the RK stage tail fused in MODE 1.
 // MODE 0: write R and lam.   MODE 1: LTS stage update into Unext (euler_solver.cpp:893-899,
 // preconditioning.cpp:51-76, time_integrator.cpp:31-104); optional R / lam write and norm partials.
@@ -255,56 +324,24 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
   const int ld = m.ld;
   const double g = o.g;
   int n_fo = 0, n_diff = 0, first_fail = 0x7fffffff;
-  for (int i = threadIdx.x; i < ns; i += BC) {
-    const int s = s0 + i;
-    const int co = m.s_o[s];
-    const int cn = m.s_n[s];
-    const int meta = m.s_meta[s];
-    const double nx = m.s_nx[s];
-    const double ny = m.s_ny[s];
-    const double A = m.s_A[s];
-    Flux F;
-    if (cn >= 0) {
-      double go[8], gn[8];
-      double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
-      if (SECOND) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          go[j] = G[j * ld + co];
-          gn[j] = G[j * ld + cn];
-        }
-        const double fx = m.s_fx[s], fy = m.s_fy[s];
-        dox = fx - m.ccx[co];
-        doy = fy - m.ccy[co];
-        dnx = fx - m.ccx[cn];
-        dny = fy - m.ccy[cn];
-      }
-      double fr[8];
-      if (FROZEN && SECOND) {  // one 16-byte load per slot: eight 16-bit ratios in units of 2^-15
-        const uint4 q4 = reinterpret_cast<const uint4*>(o.frozen_slots)[s];
-        const unsigned int w[4] = {q4.x, q4.y, q4.z, q4.w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          fr[2 * j] = static_cast<double>(w[j] & 0xffffu) * kRatioStep;
-          fr[2 * j + 1] = static_cast<double>(w[j] >> 16) * kRatioStep;
-        }
-      }
-      bool reverted;
-      F = interior_face<SCHEME, LIMITER, SECOND, FROZEN, PLAIN>(load_u(U, ld, co), load_u(U, ld, cn), go, gn, nx, ny,
-                                                                dox, doy, dnx, dny, o, &reverted, fr);
-      if (reverted && (meta & kMetaPrimary)) {
-        ++n_fo;
-        first_fail = min(first_fail, m.s_face[s]);
-      }
-    } else {
-      F = boundary_face<SCHEME, PLAIN>(load_u(U, ld, co), (meta & kMetaBcMask) == 2, nx, ny, o);
+  // PLAIN: the interior slots are evaluated with the exact shortcuts of the FAST face flux first; a
+  // boundary slot, or one outside the shortcuts' operand bounds, sets its bit in redo (bit k: this
+  // thread's k-th slot; a block has at most 64 KiB / 40 B = 1638 slots, checked at mesh creation, so at
+  // most 13 per thread) and is evaluated after the loop without them, which keeps the boundary and
+  // unbounded paths out of the loop's registers.
+  unsigned redo = 0;
+  for (int i = threadIdx.x, k = 0; i < ns; i += BC, ++k) {
+    bool ok = true;
+    flux_slot<SCHEME, LIMITER, SECOND, FROZEN, PLAIN, PLAIN>(m, s0 + i, i, o, U, G, lds, &n_fo, &n_diff, &first_fail,
+                                                             &ok);
+    if (!ok) redo |= 1u << k;
+  }
+  if (PLAIN && redo) {
+    for (int i = threadIdx.x, k = 0; i < ns; i += BC, ++k) {
+      if ((redo >> k) & 1u)
+        flux_slot<SCHEME, LIMITER, SECOND, FROZEN, PLAIN, false>(m, s0 + i, i, o, U, G, lds, &n_fo, &n_diff,
+                                                                 &first_fail, nullptr);
     }
-    if (F.fallback && (meta & kMetaPrimary)) ++n_diff;
-    f0[i] = F.f0 * A;
-    f1[i] = F.f1 * A;
-    f2[i] = F.f2 * A;
-    f3[i] = F.f3 * A;
-    fw[i] = F.lam * A;
   }
   // Fallback counters: one atomic per wavefront that saw any.
   {

@@ -57,6 +57,30 @@ __device__ __forceinline__ bool fin(const Dual& x) { return __builtin_isfinite(x
 __device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double vabs(double x) { return fabs(x); }
 
+// Quotients that share a denominator.  On gfx950, x / y in double is the compiler's IEEE expansion:
+// v_div_scale of both operands, a reciprocal of the scaled denominator refined by two Newton steps, the
+// scaled quotient, one residual correction (v_div_fmas) and v_div_fixup.  v_div_scale returns its
+// operand unchanged, and v_div_fmas is a plain fma, unless an operand is zero, denormal or tiny, 1 / y
+// or x / y leaves the normal range, or the exponents of x and y lie 768 or more apart.  For |x| and |y|
+// in [2^-300, 2^300] none of that happens, so the refined reciprocal depends on y alone: quot_den forms
+// it once, quot takes each numerator through the remaining steps, and the result is the correctly
+// rounded x / y, the same bits as the division.  Callers establish the operand range.
+constexpr double kQuotHi = 0x1p300;
+struct QuotDen { double y, r; };
+__device__ __forceinline__ QuotDen quot_den(double y) {
+  double r = __builtin_amdgcn_rcp(y);
+  double e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  return {y, r};
+}
+__device__ __forceinline__ double quot(double x, const QuotDen& d) {
+  const double q = x * d.r;
+  const double e = __builtin_fma(-d.y, q, x);
+  return __builtin_amdgcn_div_fixup(__builtin_fma(e, d.r, q), d.y, x);
+}
+
 // Residual-path options that vary per call (EulerResidualAssemblyConfig, residual_assembly.hpp:12-31).
 // The fields that depend on the freestream (state, stabilisation floor, preconditioner bounds) take the
 // scalar type, so a tangent can carry their derivatives with respect to Mach and angle of attack.
@@ -88,9 +112,10 @@ using FaceOpts = FaceOptsT<double>;
 
 // The face functions below take a compile-time PLAIN switch: PLAIN instantiations are the ones for
 // options with the low-Mach preconditioner, the low-Mach dissipation fix and the all-speed HLLC fix all
-// off (plain_options), where those branches are known not taken; the arithmetic is the same.
+// off (plain_options), where those branches are known not taken; the arithmetic is the same.  They also
+// require 1 < gamma <= 30, the range the exact shortcuts of the PLAIN face flux (flux_of_states) assume.
 template <class T> __host__ __device__ __forceinline__ bool plain_options(const FaceOptsT<T>& o) {
-  return !o.precond && !o.low_mach_fix && !o.as_fix;
+  return !o.precond && !o.low_mach_fix && !o.as_fix && o.g > 1.0 && o.g <= 30.0;
 }
 
 template <class T> __device__ __forceinline__ C4T<T> p2c(const P4T<T>& w, double g) {  // euler_flux.cpp:101-113
@@ -130,9 +155,49 @@ template <class T> __device__ __forceinline__ T sound(const P4T<T>& w, double g)
   return vsqrt(dmax(g * w.p / dmax(w.r, kFloor), T(0.0)));
 }
 
+// The same three maps with their quotients taken through quot (exact shortcuts of PLAIN face fluxes,
+// whose callers bound the operands): p2c_q divides by gamma - 1 (dg), c2p_q by the floored density and
+// returns that density's QuotDen, sound_q divides by it again (dmax(w.r, kFloor) is w.r there).
+__device__ __forceinline__ C4 p2c_q(const P4& w, double g, const QuotDen& dg) {
+  C4 c;
+  c.r = dmax(w.r, kFloor);
+  c.ru = c.r * w.u;
+  c.rv = c.r * w.v;
+  const double ke = 0.5 * c.r * (w.u * w.u + w.v * w.v);
+  const double p = dmax(w.p, kFloor);
+  c.E = quot(p, dg) + ke;
+  return c;
+}
+__device__ __forceinline__ P4 c2p_q(const C4& c, double g, QuotDen* d) {
+  P4 w;
+  w.r = dmax(c.r, kFloor);
+  *d = quot_den(w.r);
+  const double inv = quot(1.0, *d);
+  w.u = c.ru * inv;
+  w.v = c.rv * inv;
+  const double ke = 0.5 * w.r * (w.u * w.u + w.v * w.v);
+  w.p = dmax((g - 1.0) * (c.E - ke), kFloor);
+  return w;
+}
+__device__ __forceinline__ double sound_q(const P4& w, double g, const QuotDen& d) {
+  return vsqrt(dmax(quot(g * w.p, d), 0.0));
+}
+
 template <class T> struct SpeedsT { T lm, lp, sr; };
 
-template <class T> __device__ __forceinline__ SpeedsT<T> pspeeds(T un, T a, T s) {  // euler_flux.cpp:58-74
+// FAST (PLAIN face fluxes, where s is exactly 1): b = b2 = 1, so ut = (0.5 * 2) * un = un and disc =
+// (+-0) + 4 RN(a * a), whose square root halved is RN(sqrt(RN(a * a))) = a in binary floating point while
+// a * a stays normal and un is finite, which the callers' operand bounds guarantee: lm = un - a and
+// lp = un + a exactly.
+template <bool FAST = false, class T>
+__device__ __forceinline__ SpeedsT<T> pspeeds(T un, T a, T s) {  // euler_flux.cpp:58-74
+  if constexpr (FAST) {
+    SpeedsT<T> sp;
+    sp.lm = un - a;
+    sp.lp = un + a;
+    sp.sr = dmax4(vabs(un), vabs(sp.lm), vabs(sp.lp), T(kFloor));
+    return sp;
+  }
   const T b = dclamp(s, T(kMinScale), T(1.0));
   const T b2 = b * b;
   const T omb2 = 1.0 - b2;
@@ -243,17 +308,43 @@ __device__ __forceinline__ T all_speed_scale(const P4T<T>& pl, const P4T<T>& pr,
 template <class T> struct FluxT { T f0, f1, f2, f3, lam; int fallback; };
 using Flux = FluxT<double>;
 
-template <class T>
+// Primitive states and sound speeds of a face's two conservative states.  FAST (PLAIN face fluxes,
+// T = double, operands bounded by flux_of_states) takes the quotients by each side's density through
+// quot and keeps its QuotDen for the HLLC star states.
+template <class T> struct SidesT { P4T<T> pl, pr; T al, ar; QuotDen dl, dr; };
+template <bool FAST, class T>
+__device__ __forceinline__ SidesT<T> face_sides(const C4T<T>& l, const C4T<T>& r, double g) {
+  SidesT<T> s;
+  if constexpr (FAST) {
+    s.pl = c2p_q(l, g, &s.dl);
+    s.pr = c2p_q(r, g, &s.dr);
+    s.al = sound_q(s.pl, g, s.dl);
+    s.ar = sound_q(s.pr, g, s.dr);
+  } else {
+    s.pl = c2p(l, g);
+    s.pr = c2p(r, g);
+    s.al = sound(s.pl, g);
+    s.ar = sound(s.pr, g);
+  }
+  return s;
+}
+template <bool FAST, class T> __device__ __forceinline__ T div_side(T x, T rho, const QuotDen& d) {
+  if constexpr (FAST) return quot(x, d);
+  return x / rho;
+}
+
+template <bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> rusanov(const C4T<T>& l, const C4T<T>& r, T nx, T ny, T s, double g) {  // euler_flux.cpp:203-239
-  const P4T<T> pl = c2p(l, g);
-  const P4T<T> pr = c2p(r, g);
+  const SidesT<T> sides = face_sides<FAST>(l, r, g);
+  const P4T<T>& pl = sides.pl;
+  const P4T<T>& pr = sides.pr;
   const T unl = pl.u * nx + pl.v * ny;
   const T unr = pr.u * nx + pr.v * ny;
-  const T al = sound(pl, g);
-  const T ar = sound(pr, g);
+  const T al = sides.al;
+  const T ar = sides.ar;
   const T sphys = dmax(vabs(unl) + al, vabs(unr) + ar);
-  const SpeedsT<T> sl = pspeeds(unl, al, s);
-  const SpeedsT<T> sr = pspeeds(unr, ar, s);
+  const SpeedsT<T> sl = pspeeds<FAST>(unl, al, s);
+  const SpeedsT<T> sr = pspeeds<FAST>(unr, ar, s);
   const T sd = dmax(sl.sr, sr.sr);
   FluxT<T> F;
   F.lam = dmax(sphys, kFloor);
@@ -275,22 +366,23 @@ template <class T> __device__ __forceinline__ T sign_floor(T v) {  // euler_flux
   return (v >= 0.0) ? T(kFloor) : T(-kFloor);
 }
 
-template <bool PLAIN = false, class T>
+template <bool PLAIN = false, bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx, T ny, T s,
                                          const FaceOptsT<T>& o) {  // euler_flux.cpp:241-364
   const double g = o.g;
-  const P4T<T> pl = c2p(l, g);
-  const P4T<T> pr = c2p(r, g);
+  const SidesT<T> sides = face_sides<FAST>(l, r, g);
+  const P4T<T>& pl = sides.pl;
+  const P4T<T>& pr = sides.pr;
   const T unl = pl.u * nx + pl.v * ny;
   const T unr = pr.u * nx + pr.v * ny;
-  const T al = sound(pl, g);
-  const T ar = sound(pr, g);
+  const T al = sides.al;
+  const T ar = sides.ar;
   const T ps = all_speed_scale<PLAIN>(pl, pr, nx, ny, o);
   const T pavg = 0.5 * (pl.p + pr.p);
   const T plc = pavg + ps * (pl.p - pavg);
   const T prc = pavg + ps * (pr.p - pavg);
-  const SpeedsT<T> spl = pspeeds(unl, al, s);
-  const SpeedsT<T> spr = pspeeds(unr, ar, s);
+  const SpeedsT<T> spl = pspeeds<FAST>(unl, al, s);
+  const SpeedsT<T> spr = pspeeds<FAST>(unr, ar, s);
   const T sL = dmin(spl.lm, spr.lm);
   const T sR = dmax(spl.lp, spr.lp);
   const T sLp = dmin(unl - al, unr - ar);
@@ -299,7 +391,7 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
   F.lam = dmax(dmax(vabs(sLp), vabs(sRp)), kFloor);
   F.fallback = 0;
   if (vabs(sR - sL) < kFloor) {
-    F = rusanov(l, r, nx, ny, s, g);
+    F = rusanov<FAST>(l, r, nx, ny, s, g);
     F.fallback = 1;
     return F;
   }
@@ -319,7 +411,7 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
   const T rr = dmax(pr.r, kFloor);
   const T den = rl * (sL - unl) - rr * (sR - unr);
   if (vabs(den) < kFloor) {
-    F = rusanov(l, r, nx, ny, s, g);
+    F = rusanov<FAST>(l, r, nx, ny, s, g);
     F.fallback = 1;
     return F;
   }
@@ -334,7 +426,7 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
       ls0 = rs;
       ls1 = rs * (pl.u + sh * nx);
       ls2 = rs * (pl.v + sh * ny);
-      const T et = l.E / rl;
+      const T et = div_side<FAST>(l.E, rl, sides.dl);  // rl is pl.r
       const T pt = plc / sign_floor(rl * skmu);
       ls3 = rs * (et + sh * (sM + pt));
     }
@@ -348,14 +440,14 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
       rs0 = rs;
       rs1 = rs * (pr.u + sh * nx);
       rs2 = rs * (pr.v + sh * ny);
-      const T et = r.E / rr;
+      const T et = div_side<FAST>(r.E, rr, sides.dr);  // rr is pr.r
       const T pt = prc / sign_floor(rr * skmu);
       rs3 = rs * (et + sh * (sM + pt));
     }
   }
   if (!(fin(ls0) && fin(ls1) && fin(ls2) && fin(ls3)) || !(fin(rs0) && fin(rs1) && fin(rs2) && fin(rs3)) ||
       ls0 <= 0.0 || rs0 <= 0.0) {
-    F = rusanov(l, r, nx, ny, s, g);
+    F = rusanov<FAST>(l, r, nx, ny, s, g);
     F.fallback = 1;
     return F;
   }
@@ -371,17 +463,17 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
     F.f3 = fr3 + sR * (rs3 - r.E);
   }
   if (!(fin(F.f0) && fin(F.f1) && fin(F.f2) && fin(F.f3))) {
-    F = rusanov(l, r, nx, ny, s, g);
+    F = rusanov<FAST>(l, r, nx, ny, s, g);
     F.fallback = 1;
   }
   return F;
 }
 
-template <int SCHEME, bool PLAIN = false, class T>
+template <int SCHEME, bool PLAIN = false, bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> face_flux(const C4T<T>& l, const C4T<T>& r, T nx, T ny, T s,
                                               const FaceOptsT<T>& o) {  // euler_flux.cpp:366-390
-  if (SCHEME == 1) return hllc<PLAIN>(l, r, nx, ny, s, o);
-  return rusanov(l, r, nx, ny, s, o.g);
+  if (SCHEME == 1) return hllc<PLAIN, FAST>(l, r, nx, ny, s, o);
+  return rusanov<FAST>(l, r, nx, ny, s, o.g);
 }
 
 // Characteristic farfield state (boundary_conditions.cpp:27-78).

@@ -78,9 +78,28 @@ __device__ __forceinline__ void limiter_ratio(double qo, double qn, double gox, 
 // The face flux from the two face states' primitives (residual_assembly.cpp:122-137): conservative maps,
 // face beta, acoustic scale and the Riemann flux.  The stored face Jacobians of the frozen adjoint
 // differentiate this function of (L, R).
-template <int SCHEME, bool PLAIN, class T>
+//
+// FAST (PLAIN, T = double): the quotients by gamma - 1 and by each side's density go through quot, and the
+// wave speeds through pspeeds<FAST>, first.  With 1 < gamma <= 30 (plain_options), conservative
+// densities in [kFloor, 2^300] and energies at most 2^280 (checked below; p2c makes them positive and at
+// least kFloor / (gamma - 1)), every operand of those shortcuts lies where they are exact: the
+// pressures (gamma - 1) (E - ke) are at most 29 E, so gamma p <= 2^290; the kinetic energies are at
+// most E, so velocities and normal velocities stay finite; sound speeds lie within [2^-165, 2^165].  An
+// energy above the bound can only come from a pressure or velocity above it, whatever p2c_q returned.
+// *ok reports whether a face is within the bounds; the flux of one that is not (non-finite or extreme
+// states) is meaningless, and the caller evaluates that face again without FAST.
+template <int SCHEME, bool PLAIN, bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> flux_of_states(const P4T<T>& L, const P4T<T>& Rt, T nx, T ny,
-                                                   const FaceOptsT<T>& o) {
+                                                   const FaceOptsT<T>& o, bool* ok = nullptr) {
+  if constexpr (FAST) {
+    static_assert(PLAIN && std::is_same_v<T, double>, "FAST face fluxes are PLAIN and in double");
+    constexpr double kEnergyHi = 0x1p280;
+    const QuotDen dg = quot_den(o.g - 1.0);
+    const C4 ql = p2c_q(L, o.g, dg);
+    const C4 qr = p2c_q(Rt, o.g, dg);
+    *ok = (ql.r <= kQuotHi) & (qr.r <= kQuotHi) & (ql.E <= kEnergyHi) & (qr.E <= kEnergyHi);
+    return face_flux<SCHEME, PLAIN, true>(ql, qr, nx, ny, T(1.0), o);
+  }
   const C4T<T> ul = p2c(L, o.g);
   const C4T<T> ur = p2c(Rt, o.g);
   const T fb = face_beta<PLAIN>(ul, ur, o);
@@ -91,9 +110,10 @@ __device__ __forceinline__ FluxT<T> flux_of_states(const P4T<T>& L, const P4T<T>
 // Interior face with the first-order fallback (residual_assembly.cpp:107-137), the reconstruction
 // increments taken from src.  *reverted reports the fallback to cell values.
 // FROZEN: fr holds the face's frozen limiter ratios (owner side rho, u, v, p, then neighbour side).
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, bool PLAIN, class T, class Src>
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN, bool PLAIN, bool FAST = false, class T, class Src>
 __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4T<T>& un, const Src& src, T nx, T ny,
-                                                      const FaceOptsT<T>& o, bool* reverted, const double* fr) {
+                                                      const FaceOptsT<T>& o, bool* reverted, const double* fr,
+                                                      bool* ok = nullptr) {
   const double g = o.g;
   P4T<T> L = c2p(uo, g);
   P4T<T> Rt = c2p(un, g);
@@ -122,17 +142,18 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
       *reverted = true;
     }
   }
-  return flux_of_states<SCHEME, PLAIN>(L, Rt, nx, ny, o);
+  return flux_of_states<SCHEME, PLAIN, FAST>(L, Rt, nx, ny, o, ok);
 }
 
 // Interior face (residual_assembly.cpp:107-137).  go / gn are the cells' primitive gradients (rho_x,
 // rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the vectors from the cell centres to the face centre.
-template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, bool PLAIN = false, class T>
+// FAST: see flux_of_states (*ok).
+template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, bool PLAIN = false, bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
                                                   T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
-                                                  bool* reverted, const double* fr = nullptr) {
+                                                  bool* reverted, const double* fr = nullptr, bool* ok = nullptr) {
   const GradIncr<T> src{go, gn, dox, doy, dnx, dny};
-  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN, PLAIN>(uo, un, src, nx, ny, o, reverted, fr);
+  return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN, PLAIN, FAST>(uo, un, src, nx, ny, o, reverted, fr, ok);
 }
 
 // The same face from its reconstruction increments ao / an (owner / neighbour g . d of rho, u, v, p):

@@ -78,6 +78,24 @@ def test_residual_parity_256x64(hip, cells, kw):
             compare(hm, om, U, combo, ff, f"256x64 {cells}")
 
 
+@pytest.mark.parametrize("amp", [0.05, 0.6])
+def test_plain_residual_bitwise(hip, amp):
+    """With the preconditioner, the low-Mach fix and the all-speed fix off (the options the PLAIN flux
+    instantiations serve) the residual and spectral radius equal the FP64 oracle's bit for bit, for both
+    schemes, both limiters, both farfield conditions and first order.  The PLAIN face evaluation takes
+    shortcuts that are exact in IEEE arithmetic (shared correctly rounded quotients, the wave speeds of a
+    unit acoustic scale), so any rounding difference is a defect."""
+    m = helpers.mesh("256x64", cells="mixed", quad_layers=12, jitter=0.1, seed=777, rcm=True)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64(0.6, 3.0)
+    U = helpers.perturbed_state(m.cell_center, ff, amp, 99 + int(amp * 10))
+    combos = [(s, l, 1, 0, 0, 0, cbc) for s in (0, 1) for l in (0, 1) for cbc in (0, 1)]
+    combos += [(1, 0, 0, 0, 0, 0, 0), (0, 0, 0, 0, 0, 0, 1)]
+    with hip.HipMesh(m) as hm:
+        for combo in combos:
+            assert compare(hm, om, U, combo, ff, f"256x64 mixed amp={amp} bitwise"), combo
+
+
 def test_gradients_parity(hip):
     m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=5, rcm=True)
     om = ol.OracleMesh(m, np.float64)
