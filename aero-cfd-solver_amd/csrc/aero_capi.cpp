@@ -753,6 +753,13 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
     }
   }
   m->S = slot_off[m->nblk];
+  // The kernels gather at 32-bit byte offsets of cell, slot and gather-entry indices (at most 8 bytes per
+  // element): cells and slots stay below 2^28 (gather entries are at most twice the slots).
+  if (N >= (1 << 28) || m->S >= (1 << 28)) {
+    delete m;
+    set_err(err, errlen, "aero_hip_mesh_create: the mesh exceeds 2^28 cells or face slots");
+    return AERO_EINVAL;
+  }
   const int grad = opts ? opts->gradient : AERO_GRAD_GREEN_GAUSS;
   // doubles staged per slot: the larger of the gradient kernel's (LSQ 8, Green-Gauss 4) and the flux kernel's (5)
   const size_t lds_per_slot = grad == AERO_GRAD_LEAST_SQUARES ? 8 : 5;

@@ -23,12 +23,27 @@
 
 namespace aero_dev {
 
+// Gathers at a 32-bit byte offset from a wave-uniform base.  Indexing an array with a per-lane int costs
+// a sign extension and a 64-bit add per load; with the base in scalar registers and the offset formed
+// once per index, every load of that index takes the SGPR-base form of global_load and costs no vector
+// instruction.  Offsets stay below 2^32: meshes have fewer than 2^28 cells and face slots (checked at
+// mesh creation).
+template <class T> __device__ __forceinline__ T at_off(const T* base, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T> __device__ __forceinline__ void st_off(T* base, unsigned off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+__device__ __forceinline__ unsigned off8(int i) { return static_cast<unsigned>(i) << 3; }
+__device__ __forceinline__ unsigned off4(int i) { return static_cast<unsigned>(i) << 2; }
+
 __device__ __forceinline__ C4 load_u(const double* __restrict__ U, int ld, int c) {
+  const unsigned o = off8(c);
   C4 s;
-  s.r = U[c];
-  s.ru = U[ld + c];
-  s.rv = U[2 * ld + c];
-  s.E = U[3 * ld + c];
+  s.r = at_off(U, o);
+  s.ru = at_off(U + ld, o);
+  s.rv = at_off(U + 2 * ld, o);
+  s.E = at_off(U + 3 * ld, o);
   return s;
 }
 
@@ -142,9 +157,9 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   double* q3 = lds + 3 * smax;
   const int ld = m.ld;
   for (int i = threadIdx.x; i < ns; i += BC) {
-    const int s = s0 + i;
-    const int o = m.s_o[s];
-    const int n = m.s_n[s];
+    const unsigned s4 = off4(s0 + i);
+    const int o = at_off(m.s_o, s4);
+    const int n = at_off(m.s_n, s4);
     const P4 wo = c2p(load_u(U, ld, o), g);
     double a0 = wo.r, a1 = wo.u, a2 = wo.v, a3 = wo.p;
     if (n >= 0) {
@@ -163,12 +178,13 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   const int c = b * BC + threadIdx.x;
   if (c >= m.N) return;
   double gx0 = 0.0, gy0 = 0.0, gx1 = 0.0, gy1 = 0.0, gx2 = 0.0, gy2 = 0.0, gx3 = 0.0, gy3 = 0.0;
-  const int e1 = m.c_off[c + 1];
-  for (int e = m.c_off[c]; e < e1; ++e) {
-    const int ent = m.c_ent[e];
+  const int e1 = at_off(m.c_off, off4(c + 1));
+  for (int e = at_off(m.c_off, off4(c)); e < e1; ++e) {
+    const int ent = at_off(m.c_ent, off4(e));
     const int l = ent >> 1;
-    const double A = m.s_A[s0 + l];
-    const double x = m.s_nx[s0 + l] * A, y = m.s_ny[s0 + l] * A;
+    const unsigned sl = off8(s0 + l);
+    const double A = at_off(m.s_A, sl);
+    const double x = at_off(m.s_nx, sl) * A, y = at_off(m.s_ny, sl) * A;
     const double v0 = q0[l], v1 = q1[l], v2 = q2[l], v3 = q3[l];
     if (ent & 1) {
       gx0 -= v0 * x; gy0 -= v0 * y;
@@ -182,15 +198,16 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
       gx3 += v3 * x; gy3 += v3 * y;
     }
   }
-  const double iv = 1.0 / dmax(m.vol[c], 1.0e-12);
-  G[0 * ld + c] = gx0 * iv;
-  G[1 * ld + c] = gy0 * iv;
-  G[2 * ld + c] = gx1 * iv;
-  G[3 * ld + c] = gy1 * iv;
-  G[4 * ld + c] = gx2 * iv;
-  G[5 * ld + c] = gy2 * iv;
-  G[6 * ld + c] = gx3 * iv;
-  G[7 * ld + c] = gy3 * iv;
+  const unsigned c8 = off8(c);
+  const double iv = 1.0 / dmax(at_off(m.vol, c8), 1.0e-12);
+  st_off(G, c8, gx0 * iv);
+  st_off(G + ld, c8, gy0 * iv);
+  st_off(G + 2 * ld, c8, gx1 * iv);
+  st_off(G + 3 * ld, c8, gy1 * iv);
+  st_off(G + 4 * ld, c8, gx2 * iv);
+  st_off(G + 5 * ld, c8, gy2 * iv);
+  st_off(G + 6 * ld, c8, gx3 * iv);
+  st_off(G + 7 * ld, c8, gy3 * iv);
 }
 
 template <int BC, int GK>
@@ -244,34 +261,36 @@ __device__ __forceinline__ void flux_slot(const DevMesh& m, int s, int i, const 
                                           int* n_fo, int* n_diff, int* first_fail, bool* ok) {
   const int smax = m.smax;
   const int ld = m.ld;
-  const int cn = m.s_n[s];
+  const unsigned s4 = off4(s), s8 = off8(s);
+  const int cn = at_off(m.s_n, s4);
   if constexpr (FAST) {
     if (cn < 0) {
       *ok = false;
       return;
     }
   }
-  const int co = m.s_o[s];
-  const int meta = m.s_meta[s];
-  const double nx = m.s_nx[s];
-  const double ny = m.s_ny[s];
-  const double A = m.s_A[s];
+  const int co = at_off(m.s_o, s4);
+  const int meta = at_off(m.s_meta, s4);
+  const double nx = at_off(m.s_nx, s8);
+  const double ny = at_off(m.s_ny, s8);
+  const double A = at_off(m.s_A, s8);
   Flux F;
   bool reverted = false;
   if (cn >= 0) {
     double go[8], gn[8];
     double dox = 0.0, doy = 0.0, dnx = 0.0, dny = 0.0;
     if (SECOND) {
+      const unsigned oo = off8(co), on = off8(cn);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        go[j] = G[j * ld + co];
-        gn[j] = G[j * ld + cn];
+        go[j] = at_off(G + j * ld, oo);
+        gn[j] = at_off(G + j * ld, on);
       }
-      const double fx = m.s_fx[s], fy = m.s_fy[s];
-      dox = fx - m.ccx[co];
-      doy = fy - m.ccy[co];
-      dnx = fx - m.ccx[cn];
-      dny = fy - m.ccy[cn];
+      const double fx = at_off(m.s_fx, s8), fy = at_off(m.s_fy, s8);
+      dox = fx - at_off(m.ccx, oo);
+      doy = fy - at_off(m.ccy, oo);
+      dnx = fx - at_off(m.ccx, on);
+      dny = fy - at_off(m.ccy, on);
     }
     double fr[8];
     if (FROZEN && SECOND) {  // one 16-byte load per slot: eight 16-bit ratios in units of 2^-15
@@ -358,10 +377,11 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
   const int c = b * BC + threadIdx.x;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0, w = 0.0;
   const bool active = c < m.N;
+  const unsigned c8 = off8(c);
   if (active) {
-    const int e1 = m.c_off[c + 1];
-    for (int e = m.c_off[c]; e < e1; ++e) {
-      const int ent = m.c_ent[e];
+    const int e1 = at_off(m.c_off, off4(c + 1));
+    for (int e = at_off(m.c_off, off4(c)); e < e1; ++e) {
+      const int ent = at_off(m.c_ent, off4(e));
       const int l = ent >> 1;
       if (ent & 1) {
         r0 -= f0[l];
@@ -379,11 +399,11 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
   }
   if (MODE == 0) {
     if (active) {
-      R[c] = r0;
-      R[ld + c] = r1;
-      R[2 * ld + c] = r2;
-      R[3 * ld + c] = r3;
-      lam[c] = w;
+      st_off(R, c8, r0);
+      st_off(R + ld, c8, r1);
+      st_off(R + 2 * ld, c8, r2);
+      st_off(R + 3 * ld, c8, r3);
+      st_off(lam, c8, w);
     }
     return;
   } else {
@@ -391,11 +411,11 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
     int bad = 0;
     if (active) {
       if (st.write_res) {
-        R[c] = r0;
-        R[ld + c] = r1;
-        R[2 * ld + c] = r2;
-        R[3 * ld + c] = r3;
-        lam[c] = w;
+        st_off(R, c8, r0);
+        st_off(R + ld, c8, r1);
+        st_off(R + 2 * ld, c8, r2);
+        st_off(R + 3 * ld, c8, r3);
+        st_off(lam, c8, w);
       }
       const C4 us = load_u(U, ld, c);
       double p3 = r3;
@@ -430,10 +450,10 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
         }
       }
       enforce_physical(&s, g);
-      st.Unext[c] = s.r;
-      st.Unext[ld + c] = s.ru;
-      st.Unext[2 * ld + c] = s.rv;
-      st.Unext[3 * ld + c] = s.E;
+      st_off(st.Unext, c8, s.r);
+      st_off(st.Unext + ld, c8, s.ru);
+      st_off(st.Unext + 2 * ld, c8, s.rv);
+      st_off(st.Unext + 3 * ld, c8, s.E);
       if (st.last_stage) {
         if (!(fin(s.r) && fin(s.ru) && fin(s.rv) && fin(s.E))) {
           bad = 1;
@@ -442,7 +462,7 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
           if (!(fin(q.r) && fin(q.u) && fin(q.v) && fin(q.p))) bad = 1;
         }
         // residual norms of the un-preconditioned residual (euler_solver.cpp:315-333)
-        const double iv = 1.0 / dmax(m.vol[c], 1.0e-12);
+        const double iv = 1.0 / dmax(at_off(m.vol, c8), 1.0e-12);
         const double v0 = fabs(r0 * iv), v1 = fabs(r1 * iv), v2 = fabs(r2 * iv), v3 = fabs(r3 * iv);
         nl1 = v0 + v1 + v2 + v3;
         nl2 = v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3;
