@@ -191,7 +191,9 @@ typedef struct aero_hip_mesh aero_hip_mesh; /* opaque device mesh + state + stre
 int aero_hip_available(char* err, size_t errlen);
 
 /* Replaces cfd::cuda_backend::init_euler_device_buffers (cuda_backend.hpp:44-46): uploads the mesh
- * (AoS FP64 -> SoA FP64, optional renumbering, per-block face tables) and allocates state buffers. */
+ * (AoS FP64 -> SoA FP64, optional renumbering, per-block face tables) and allocates state buffers.
+ * AERO_EINVAL for meshes of 2^28 cells or face slots and more (the kernels gather at 32-bit byte
+ * offsets). */
 int aero_hip_mesh_create(const aero_mesh_desc* desc, const aero_hip_options* opts, aero_hip_mesh** out,
                          char* err, size_t errlen);
 
