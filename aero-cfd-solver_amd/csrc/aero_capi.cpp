@@ -708,7 +708,8 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
   // Per-block slot tables and per-cell gather entries.
   std::vector<int> slot_off(m->nblk + 1, 0), s_o, s_n, s_meta, s_face;
   std::vector<double> s_nx, s_ny, s_A, s_fx, s_fy;
-  std::vector<int> c_off(N + 1, 0), c_ent;
+  std::vector<int> c_off(N + 1, 0);
+  std::vector<unsigned short> c_ent;  // a block's slots fit 16 bits with the side bit (LDS bound below)
   c_ent.reserve(cf.size());
   std::vector<int> stamp(F, -1), local(F, -1), bf;
   int smax = 0;
@@ -747,7 +748,7 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
       const int oc = m->perm[c];
       for (int e = cf_off[oc]; e < cf_off[oc + 1]; ++e) {
         const int f = cf[e];
-        c_ent.push_back((local[f] << 1) | (d->face_neighbor[f] == oc ? 1 : 0));
+        c_ent.push_back(static_cast<unsigned short>((local[f] << 1) | (d->face_neighbor[f] == oc ? 1 : 0)));
       }
       c_off[c + 1] = static_cast<int>(c_ent.size());
     }
@@ -801,7 +802,8 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
   } while (0)
 
   AERO_SETUP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-  int *p_slot_off, *p_so, *p_sn, *p_meta, *p_face, *p_coff, *p_cent, *p_wo;
+  int *p_slot_off, *p_so, *p_sn, *p_meta, *p_face, *p_coff, *p_wo;
+  unsigned short* p_cent;
   double *p_nx, *p_ny, *p_A, *p_fx, *p_fy, *p_vol, *p_ccx, *p_ccy, *p_wnx, *p_wny, *p_wA, *p_wfx, *p_wfy;
   AERO_SETUP(upload(m, &p_slot_off, slot_off));
   AERO_SETUP(upload(m, &p_so, s_o));

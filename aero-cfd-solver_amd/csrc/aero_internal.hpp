@@ -42,7 +42,7 @@ struct DevMesh {
   const double* s_fx;   // face centre
   const double* s_fy;
   const int* c_off;     // [N + 1] per-cell entry offsets
-  const int* c_ent;     // (local slot << 1) | is_neighbour, increasing reference face index
+  const unsigned short* c_ent;  // (local slot << 1) | is_neighbour, increasing reference face index
   const double* vol;    // [N]
   const double* ccx;    // [N] cell centre
   const double* ccy;

@@ -36,6 +36,7 @@ template <class T> __device__ __forceinline__ void st_off(T* base, unsigned off,
 }
 __device__ __forceinline__ unsigned off8(int i) { return static_cast<unsigned>(i) << 3; }
 __device__ __forceinline__ unsigned off4(int i) { return static_cast<unsigned>(i) << 2; }
+__device__ __forceinline__ unsigned off2(int i) { return static_cast<unsigned>(i) << 1; }
 
 __device__ __forceinline__ C4 load_u(const double* __restrict__ U, int ld, int c) {
   const unsigned o = off8(c);
@@ -180,7 +181,7 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   double gx0 = 0.0, gy0 = 0.0, gx1 = 0.0, gy1 = 0.0, gx2 = 0.0, gy2 = 0.0, gx3 = 0.0, gy3 = 0.0;
   const int e1 = at_off(m.c_off, off4(c + 1));
   for (int e = at_off(m.c_off, off4(c)); e < e1; ++e) {
-    const int ent = at_off(m.c_ent, off4(e));
+    const int ent = at_off(m.c_ent, off2(e));
     const int l = ent >> 1;
     const unsigned sl = off8(s0 + l);
     const double A = at_off(m.s_A, sl);
@@ -381,7 +382,7 @@ __device__ __forceinline__ void flux_body(const DevMesh& m, int b, const FaceOpt
   if (active) {
     const int e1 = at_off(m.c_off, off4(c + 1));
     for (int e = at_off(m.c_off, off4(c)); e < e1; ++e) {
-      const int ent = at_off(m.c_ent, off4(e));
+      const int ent = at_off(m.c_ent, off2(e));
       const int l = ent >> 1;
       if (ent & 1) {
         r0 -= f0[l];
