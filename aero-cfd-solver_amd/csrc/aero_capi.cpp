@@ -706,7 +706,7 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
     }
   }
   // Per-block slot tables and per-cell gather entries.
-  std::vector<int> slot_off(m->nblk + 1, 0), s_o, s_n, s_meta, s_face;
+  std::vector<int> slot_off(m->nblk + 1, 0), s_o, s_nm, s_face;
   std::vector<double> s_nx, s_ny, s_A, s_fx, s_fy;
   std::vector<int> c_off(N + 1, 0);
   std::vector<unsigned short> c_ent;  // a block's slots fit 16 bits with the side bit (LDS bound below)
@@ -732,11 +732,10 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
       local[f] = i;
       const int o = d->face_owner[f], n = d->face_neighbor[f];
       const int io = m->iperm[o];
-      s_o.push_back(io);
-      s_n.push_back(n >= 0 ? m->iperm[n] : -1);
       int bc = 0;
       if (n < 0) bc = (d->face_bc && d->face_bc[f] == AERO_BC_SLIP_WALL) ? 2 : 1;
-      s_meta.push_back(bc | ((io / BC) == b ? aero_dev::kMetaPrimary : 0));
+      s_o.push_back(io);
+      s_nm.push_back((n >= 0 ? m->iperm[n] : -1) * 8 + (bc | ((io / BC) == b ? aero_dev::kMetaPrimary : 0)));
       s_face.push_back(f);
       s_nx.push_back(m->h_nx[f]);
       s_ny.push_back(m->h_ny[f]);
@@ -802,13 +801,12 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
   } while (0)
 
   AERO_SETUP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-  int *p_slot_off, *p_so, *p_sn, *p_meta, *p_face, *p_coff, *p_wo;
+  int *p_slot_off, *p_so, *p_snm, *p_face, *p_coff, *p_wo;
   unsigned short* p_cent;
   double *p_nx, *p_ny, *p_A, *p_fx, *p_fy, *p_vol, *p_ccx, *p_ccy, *p_wnx, *p_wny, *p_wA, *p_wfx, *p_wfy;
   AERO_SETUP(upload(m, &p_slot_off, slot_off));
   AERO_SETUP(upload(m, &p_so, s_o));
-  AERO_SETUP(upload(m, &p_sn, s_n));
-  AERO_SETUP(upload(m, &p_meta, s_meta));
+  AERO_SETUP(upload(m, &p_snm, s_nm));
   AERO_SETUP(upload(m, &p_face, s_face));
   AERO_SETUP(upload(m, &p_nx, s_nx));
   AERO_SETUP(upload(m, &p_ny, s_ny));
@@ -855,8 +853,7 @@ int mesh_create_layout(const aero_mesh_desc* d, const aero_hip_options* opts, in
   dm.grid = dm.xcd_remap ? ((m->nblk + 7) / 8) * 8 : m->nblk;
   dm.slot_off = p_slot_off;
   dm.s_o = p_so;
-  dm.s_n = p_sn;
-  dm.s_meta = p_meta;
+  dm.s_nm = p_snm;
   dm.s_face = p_face;
   dm.s_nx = p_nx;
   dm.s_ny = p_ny;

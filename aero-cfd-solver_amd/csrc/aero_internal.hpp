@@ -12,6 +12,9 @@ namespace aero_dev {
 // two blocks).
 constexpr int kMetaBcMask = 3;
 constexpr int kMetaPrimary = 4;
+// A face slot's neighbour and bc / primary bits packed in one int (cells < 2^28).
+__host__ __device__ __forceinline__ int slot_nb(int nm) { return nm >> 3; }
+__host__ __device__ __forceinline__ int slot_meta(int nm) { return nm & 7; }
 
 // Columns of the per-face scaling record (k_face_scales; AERO_FS_* in the public header).
 enum : int {
@@ -33,8 +36,7 @@ struct DevMesh {
   int nslots;    // face slots S over all blocks
   const int* slot_off;  // [nblk + 1]
   const int* s_o;       // [S] owner (internal id)
-  const int* s_n;       // [S] neighbour (internal id) or -1
-  const int* s_meta;    // [S] bc | primary
+  const int* s_nm;      // [S] neighbour (internal id, or -1) * 8 + (bc | primary): slot_nb, slot_meta
   const int* s_face;    // [S] reference face index
   const double* s_nx;   // [S]
   const double* s_ny;

@@ -81,7 +81,7 @@ __device__ __forceinline__ void lsq_gradient_body(const DevMesh& m, int b, const
   for (int i = threadIdx.x; i < ns; i += BC) {
     const int s = s0 + i;
     const int o = m.s_o[s];
-    const int n = m.s_n[s];
+    const int n = slot_nb(m.s_nm[s]);
     double wdx = 0.0, wdy = 0.0, dx = 0.0, dy = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
     if (n >= 0) {
       const P4 wo = c2p(load_u(U, ld, o), g);
@@ -160,7 +160,7 @@ __device__ __forceinline__ void gg_gradient_body(const DevMesh& m, int b, const 
   for (int i = threadIdx.x; i < ns; i += BC) {
     const unsigned s4 = off4(s0 + i);
     const int o = at_off(m.s_o, s4);
-    const int n = at_off(m.s_n, s4);
+    const int n = slot_nb(at_off(m.s_nm, s4));
     const P4 wo = c2p(load_u(U, ld, o), g);
     double a0 = wo.r, a1 = wo.u, a2 = wo.v, a3 = wo.p;
     if (n >= 0) {
@@ -263,7 +263,8 @@ __device__ __forceinline__ void flux_slot(const DevMesh& m, int s, int i, const 
   const int smax = m.smax;
   const int ld = m.ld;
   const unsigned s4 = off4(s), s8 = off8(s);
-  const int cn = at_off(m.s_n, s4);
+  const int nm = at_off(m.s_nm, s4);
+  const int cn = slot_nb(nm);
   if constexpr (FAST) {
     if (cn < 0) {
       *ok = false;
@@ -271,7 +272,7 @@ __device__ __forceinline__ void flux_slot(const DevMesh& m, int s, int i, const 
     }
   }
   const int co = at_off(m.s_o, s4);
-  const int meta = at_off(m.s_meta, s4);
+  const int meta = slot_meta(nm);
   const double nx = at_off(m.s_nx, s8);
   const double ny = at_off(m.s_ny, s8);
   const double A = at_off(m.s_A, s8);
@@ -615,8 +616,9 @@ __device__ __forceinline__ void limiter_capture_body(const DevMesh& m, int b, co
   const int ld = m.ld, F = m.nfaces;
   for (int i = threadIdx.x; i < ns; i += BC) {
     const int s = s0 + i;
-    const int co = m.s_o[s], cn = m.s_n[s];
-    if (cn < 0 || !(m.s_meta[s] & kMetaPrimary)) continue;
+    const int nm = m.s_nm[s];
+    const int co = m.s_o[s], cn = slot_nb(nm);
+    if (cn < 0 || !(slot_meta(nm) & kMetaPrimary)) continue;
     const P4 L = c2p(load_u(U, ld, co), g);
     const P4 Rt = c2p(load_u(U, ld, cn), g);
     const double fx = m.s_fx[s], fy = m.s_fy[s];
@@ -654,7 +656,7 @@ __global__ __launch_bounds__(256) void k_ratio_slots(DevMesh m, double* __restri
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= m.nslots) return;
   const int f = m.s_face[s];
-  const bool primary = (m.s_meta[s] & kMetaPrimary) != 0;
+  const bool primary = (slot_meta(m.s_nm[s]) & kMetaPrimary) != 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const size_t at = static_cast<size_t>(j) * m.nfaces + f;
@@ -673,10 +675,11 @@ __global__ __launch_bounds__(256) void k_face_scales(DevMesh m, FaceOpts o, cons
                                                      double* __restrict__ out) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= m.nslots) return;
-  const int meta = m.s_meta[s];
+  const int nm = m.s_nm[s];
+  const int meta = slot_meta(nm);
   if (!(meta & kMetaPrimary)) return;
   const int f = m.s_face[s];
-  const int cn = m.s_n[s];
+  const int cn = slot_nb(nm);
   const bool wall = cn < 0 && (meta & kMetaBcMask) == 2;
   const double g = o.g;
   const double nx = m.s_nx[s], ny = m.s_ny[s];
