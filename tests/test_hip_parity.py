@@ -96,6 +96,24 @@ def test_plain_residual_bitwise(hip, amp):
             assert compare(hm, om, U, combo, ff, f"256x64 mixed amp={amp} bitwise"), combo
 
 
+def test_plain_residual_bitwise_extreme_states(hip):
+    """Faces outside the operand bounds of the PLAIN shortcuts (conservative densities above 2^300 or
+    energies above 2^280, finite) are evaluated again with the plain operations after the slot loop: the
+    residual stays bit-equal to the FP64 oracle's."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=31, rcm=True)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64(0.5, 1.0)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.05, 77)
+    rng = np.random.default_rng(5)
+    hot = rng.choice(m.num_cells, size=12, replace=False)
+    U[hot[:6], 4] *= 1.0e88  # energies (pressures) above 2^280
+    U[hot[6:], :] *= 1.0e95  # densities above 2^300, same velocities
+    assert np.all(np.isfinite(U))
+    with hip.HipMesh(m) as hm:
+        for combo in [(1, 1, 1, 0, 0, 0, 0), (0, 0, 1, 0, 0, 0, 1), (1, 0, 0, 0, 0, 0, 0)]:
+            assert compare(hm, om, U, combo, ff, "64x14 extreme states bitwise"), combo
+
+
 def test_gradients_parity(hip):
     m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=5, rcm=True)
     om = ol.OracleMesh(m, np.float64)
