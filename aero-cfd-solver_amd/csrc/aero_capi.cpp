@@ -463,7 +463,8 @@ uint64_t mesh_fingerprint(const aero_mesh_desc* d, const aero_hip_options* opts,
 std::mutex g_tune_mu;
 std::map<uint64_t, Tuned> g_tuned;
 
-// Mean ms per residual evaluation of handle m (2 untimed, then 6 timed evaluations).
+// ms per residual evaluation of handle m: 10 untimed evaluations, then the fastest of 3 timed runs of 10
+// (a few milliseconds per layout; the first launches after the upload run cold).
 int probe_residual(aero_hip_mesh* m, double* ms, char* err, size_t errlen) {
   const int N = m->N;
   const double g = 1.4, p0 = 101325.0, r0 = p0 / (287.05 * 288.15), a0 = std::sqrt(g * p0 / r0), V = 0.5 * a0;
@@ -499,24 +500,28 @@ int probe_residual(aero_hip_mesh* m, double* ms, char* err, size_t errlen) {
   c.all_speed_cutoff = 0.25;
   c.all_speed_f_min = 0.05;
   c.all_speed_ramp = 1.0;
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 10; ++i)
     if (const int rc = aero_hip_residual_async(m, &c, err, errlen)) return rc;
   hipEvent_t e0, e1;
   AERO_HIP_TRY(hipEventCreate(&e0), "layout probe: hipEventCreate");
   AERO_HIP_TRY(hipEventCreate(&e1), "layout probe: hipEventCreate");
   int rc = AERO_OK;
-  const int reps = 6;
-  if (hipEventRecord(e0, m->stream) != hipSuccess) rc = AERO_EHIP;
-  for (int i = 0; i < reps && !rc; ++i) rc = aero_hip_residual_async(m, &c, err, errlen);
-  float t = 0.0f;
-  if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(&t, e0, e1) != hipSuccess)) {
-    set_err(err, errlen, "layout probe: event timing failed");
-    rc = AERO_EHIP;
+  const int reps = 10;
+  float best = -1.0f;
+  for (int run = 0; run < 3 && !rc; ++run) {
+    if (hipEventRecord(e0, m->stream) != hipSuccess) rc = AERO_EHIP;
+    for (int i = 0; i < reps && !rc; ++i) rc = aero_hip_residual_async(m, &c, err, errlen);
+    float t = 0.0f;
+    if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess)) {
+      set_err(err, errlen, "layout probe: event timing failed");
+      rc = AERO_EHIP;
+    }
+    if (!rc && (best < 0.0f || t < best)) best = t;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  *ms = static_cast<double>(t) / reps;
+  *ms = static_cast<double>(best) / reps;
   return rc;
 }
 
