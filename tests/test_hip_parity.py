@@ -62,7 +62,10 @@ def test_residual_parity_reference_meshes(hip, mname):
         for si, amp in enumerate((0.05, 0.9)):
             U = helpers.perturbed_state(m.cell_center, ff, amp, 7 + si)
             for combo in cases.RESIDUAL_COMBOS:
-                bitwise += compare(hm, om, U, combo, ff, f"{mname} U{si}")
+                same = compare(hm, om, U, combo, ff, f"{mname} U{si}")
+                bitwise += same
+                if combo[3:6] == (0, 0, 0):  # PLAIN options: the exact shortcuts must keep every bit
+                    assert same, (mname, si, combo)
     say(f"[parity] {mname}: {bitwise}/{2 * len(cases.RESIDUAL_COMBOS)} evaluations bitwise identical")
 
 
