@@ -186,8 +186,10 @@ def test_residual_parity_large(hip):
 
 @pytest.mark.parametrize("name", [n for n in cases.RUNS if n not in cases.TRAPPED])
 def test_run_case_parity(hip, name):
-    """Full pseudo-time runs: residual history, CL / CD / CM, final state and wall Cp match the FP64
-    oracle's driver to 1e-9 relative."""
+    """Full pseudo-time runs against the FP64 oracle's driver: the final state, the L-infinity residual
+    history and the fallback counters bit for bit (the stage arithmetic is the reference's, operation by
+    operation); the L1 / L2 norms and CL / CD / CM, sums taken in another order, and the wall Cp to 1e-9
+    relative."""
     run = cases.RUNS[name]
     m = helpers.mesh(run["mesh"])
     om = ol.OracleMesh(m, np.float64)
@@ -208,8 +210,8 @@ def test_run_case_parity(hip, name):
         atol = max(1e-9 * np.abs(hr[:, col]).max(), 1e-12 if col in (5, 6, 7) else 0.0)
         assert np.all((rel <= 1e-9) | (np.abs(a - b) <= atol)), (name, col, rel.max())
     assert np.array_equal(ho[:, 10:12], hr[:, 10:12]), "fallback counters differ"
-    sU = np.abs(ref["conserved"]).max(axis=0) + 1e-300
-    assert (np.abs(out["conserved"] - ref["conserved"]) / sU).max() <= 1e-9
+    assert np.array_equal(ho[:, 3], hr[:, 3]), "L-infinity residual history differs"
+    assert np.array_equal(out["conserved"], ref["conserved"]), "final state differs from the oracle driver's"
     for k in (5, 6, 7):
         assert abs(out["forces"][k] - ref["forces"][k]) <= 1e-9 * abs(ref["forces"][k]) + 1e-12
     assert np.allclose(out["wall_cp"], ref["cp"], rtol=1e-9, atol=1e-12)
