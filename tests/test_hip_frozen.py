@@ -50,6 +50,7 @@ def test_capture_and_frozen_residual_parity(hip):
             scale = helpers.flux_scale(om, cfg, U)
             assert (np.abs(Rh - Ro)[:, comp] / (scale[:, comp] + 1e-300)).max() <= 1e-12
             assert (np.abs(lh - lo) / lo).max() <= 1e-12
+            assert np.array_equal(Rh[:, comp], Ro[:, comp]) and np.array_equal(lh, lo), combo
             # frozen tangent against the oracle's
             rng = np.random.default_rng(2)
             dU = rng.standard_normal(U.shape) * np.abs(U).max(axis=0) * 1e-3
@@ -58,6 +59,7 @@ def test_capture_and_frozen_residual_parity(hip):
             _, dRo = ol.residual_jvp(om, cfg, U, dU, frozen=ro)
             sc = np.abs(dRo[:, comp]).max(axis=0, keepdims=True)
             assert (np.abs(dRh[:, comp] - dRo[:, comp]) / sc).max() <= 1e-11
+            assert np.array_equal(dRh[:, comp], dRo[:, comp]), combo
             # frozen adjoint: dot-product identity
             psi = rng.standard_normal(U.shape)
             psi[:, 3] = 0.0
@@ -83,6 +85,8 @@ def test_frozen_run_case_parity(hip):
     for col in (1, 2, 3, 5, 6, 7, 8):
         rel = np.abs(ho[:, col] - hr[:, col]) / np.maximum(np.abs(hr[:, col]), 1e-300)
         assert rel.max() <= 1e-9, (col, rel.max())
+    assert np.array_equal(ho[:, 3], hr[:, 3]), "L-infinity residual history differs"
+    assert np.array_equal(out["conserved"], ref["conserved"]), "final state differs from the oracle driver's"
 
 
 def _frozen_converge(hm, rc, U0, iters, cfl):
