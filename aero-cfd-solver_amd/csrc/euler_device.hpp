@@ -57,6 +57,18 @@ __device__ __forceinline__ bool fin(const Dual& x) { return __builtin_isfinite(x
 __device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double vabs(double x) { return fabs(x); }
 
+// max / min for the FAST face fluxes: there every operand is finite (the lanes whose operands could be
+// NaN fail the flux's bounds and are discarded) and no two operands are zeros of opposite sign, so the
+// single-instruction IEEE maxNum / minNum give the bits of std::max / std::min (dmax / dmin).
+template <bool FAST, class T> __device__ __forceinline__ T fmx(T a, T b) {
+  if constexpr (FAST) return __builtin_fmax(a, b);
+  else return dmax(a, b);
+}
+template <bool FAST, class T> __device__ __forceinline__ T fmn(T a, T b) {
+  if constexpr (FAST) return __builtin_fmin(a, b);
+  else return dmin(a, b);
+}
+
 // Quotients that share a denominator.  On gfx950, x / y in double is the compiler's IEEE expansion:
 // v_div_scale of both operands, a reciprocal of the scaled denominator refined by two Newton steps, the
 // scaled quotient, one residual correction (v_div_fmas) and v_div_fixup.  v_div_scale returns its
@@ -170,17 +182,17 @@ __device__ __forceinline__ C4 p2c_q(const P4& w, double g, const QuotDen& dg) {
 }
 __device__ __forceinline__ P4 c2p_q(const C4& c, double g, QuotDen* d) {
   P4 w;
-  w.r = dmax(c.r, kFloor);
+  w.r = fmx<true>(c.r, kFloor);
   *d = quot_den(w.r);
   const double inv = quot(1.0, *d);
   w.u = c.ru * inv;
   w.v = c.rv * inv;
   const double ke = 0.5 * w.r * (w.u * w.u + w.v * w.v);
-  w.p = dmax((g - 1.0) * (c.E - ke), kFloor);
+  w.p = fmx<true>((g - 1.0) * (c.E - ke), kFloor);
   return w;
 }
 __device__ __forceinline__ double sound_q(const P4& w, double g, const QuotDen& d) {
-  return vsqrt(dmax(quot(g * w.p, d), 0.0));
+  return vsqrt(fmx<true>(quot(g * w.p, d), 0.0));
 }
 
 template <class T> struct SpeedsT { T lm, lp, sr; };
@@ -195,7 +207,7 @@ __device__ __forceinline__ SpeedsT<T> pspeeds(T un, T a, T s) {  // euler_flux.c
     SpeedsT<T> sp;
     sp.lm = un - a;
     sp.lp = un + a;
-    sp.sr = dmax4(vabs(un), vabs(sp.lm), vabs(sp.lp), T(kFloor));
+    sp.sr = fmx<true>(fmx<true>(fmx<true>(vabs(un), vabs(sp.lm)), vabs(sp.lp)), T(kFloor));
     return sp;
   }
   const T b = dclamp(s, T(kMinScale), T(1.0));
@@ -342,12 +354,12 @@ __device__ __forceinline__ FluxT<T> rusanov(const C4T<T>& l, const C4T<T>& r, T 
   const T unr = pr.u * nx + pr.v * ny;
   const T al = sides.al;
   const T ar = sides.ar;
-  const T sphys = dmax(vabs(unl) + al, vabs(unr) + ar);
+  const T sphys = fmx<FAST>(vabs(unl) + al, vabs(unr) + ar);
   const SpeedsT<T> sl = pspeeds<FAST>(unl, al, s);
   const SpeedsT<T> sr = pspeeds<FAST>(unr, ar, s);
-  const T sd = dmax(sl.sr, sr.sr);
+  const T sd = fmx<FAST>(sl.sr, sr.sr);
   FluxT<T> F;
-  F.lam = dmax(sphys, kFloor);
+  F.lam = fmx<FAST>(sphys, T(kFloor));
   F.fallback = 0;
   // physical fluxes (euler_flux.cpp:28-39)
   const T fl0 = l.r * unl, fr0 = r.r * unr;
@@ -383,12 +395,12 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
   const T prc = pavg + ps * (pr.p - pavg);
   const SpeedsT<T> spl = pspeeds<FAST>(unl, al, s);
   const SpeedsT<T> spr = pspeeds<FAST>(unr, ar, s);
-  const T sL = dmin(spl.lm, spr.lm);
-  const T sR = dmax(spl.lp, spr.lp);
-  const T sLp = dmin(unl - al, unr - ar);
-  const T sRp = dmax(unl + al, unr + ar);
+  const T sL = fmn<FAST>(spl.lm, spr.lm);
+  const T sR = fmx<FAST>(spl.lp, spr.lp);
+  const T sLp = fmn<FAST>(unl - al, unr - ar);
+  const T sRp = fmx<FAST>(unl + al, unr + ar);
   FluxT<T> F;
-  F.lam = dmax(dmax(vabs(sLp), vabs(sRp)), kFloor);
+  F.lam = fmx<FAST>(fmx<FAST>(vabs(sLp), vabs(sRp)), T(kFloor));
   F.fallback = 0;
   if (vabs(sR - sL) < kFloor) {
     F = rusanov<FAST>(l, r, nx, ny, s, g);
@@ -407,8 +419,8 @@ __device__ __forceinline__ FluxT<T> hllc(const C4T<T>& l, const C4T<T>& r, T nx,
     F.f0 = fr0; F.f1 = fr1; F.f2 = fr2; F.f3 = fr3;
     return F;
   }
-  const T rl = dmax(pl.r, kFloor);
-  const T rr = dmax(pr.r, kFloor);
+  const T rl = fmx<FAST>(pl.r, T(kFloor));
+  const T rr = fmx<FAST>(pr.r, T(kFloor));
   const T den = rl * (sL - unl) - rr * (sR - unr);
   if (vabs(den) < kFloor) {
     F = rusanov<FAST>(l, r, nx, ny, s, g);
