@@ -430,7 +430,7 @@ int build_colour_layout(aero_hip_mesh* m, const aero_mesh_desc* d, char* err, si
 
 // The layout probe behind block_cells = 0: the residual (HLLC + venkat, second order, on a smooth
 // synthetic state) timed with HIP events in each candidate layout -- gather with 128, 256 and 512 cells per
-// block, and the face-coloured layout -- and the fastest kept, per mesh (keyed by a hash of its arrays and
+// block, and the face-coloured layout -- and the fastest kept (the largest-block gather within 2 % of it), per mesh (keyed by a hash of its arrays and
 // options) for the life of the process.
 struct Tuned {
   int BC = 256, accum = AERO_ACCUM_GATHER;
@@ -564,6 +564,16 @@ int tuned_layout(const aero_mesh_desc* d, const aero_hip_options* opts, int accu
   if (best >= 1e300) {
     set_err(err, errlen, "layout probe: no candidate layout could be built");
     return AERO_EINVAL;
+  }
+  // Within 2 % of the fastest (the run-to-run spread of these timings) the gather layout with the largest
+  // blocks is kept: it duplicates the fewest face slots, moves the fewest bytes and keeps the reference's
+  // summation order, and the choice stays the same from run to run.
+  for (int k = 2; k >= 0; --k) {
+    if (t.ms[k] > 0.0 && t.ms[k] <= 1.02 * best) {
+      t.BC = bcs[k];
+      t.accum = AERO_ACCUM_GATHER;
+      break;
+    }
   }
   g_tuned[key] = t;
   *out = t;

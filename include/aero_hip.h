@@ -63,7 +63,8 @@ typedef struct aero_hip_options {
    *  AERO_ACCUM_AUTO: gather, unless block_cells = 0.
    * With block_cells = 0 the layout probe times one residual evaluation (HLLC + venkat, second order, on a
    * smooth synthetic state) in every candidate -- gather at 128, 256 and 512 cells per block and, unless
-   * accumulation = GATHER, the coloured layout -- and keeps the fastest, cached per mesh for the process
+   * accumulation = GATHER, the coloured layout -- and keeps the fastest (the gather with the largest blocks when
+   * it is within 2 % of the fastest), cached per mesh for the process
    * (aero_hip_layout_probe reports the times).  Stage updates (aero_hip_stage, _iterate, _run_case and the
    * batches) always use the gather layout at the chosen block size. */
   int32_t accumulation;
