@@ -250,7 +250,7 @@ class HipMesh:
         ms = np.zeros(4)
         self.lib.aero_hip_layout_probe(self.h, _pd(ms))
         keys = ("gather128", "gather256", "gather512", "colour")
-        return {k: (round(float(v), 5) if v >= 0 else None) for k, v in zip(keys, ms)}
+        return {k: (float(v) if v >= 0 else None) for k, v in zip(keys, ms)}
 
     def stream(self) -> int:
         return self.lib.aero_hip_stream(self.h) or 0

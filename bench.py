@@ -854,7 +854,7 @@ def main() -> int:
     t_probe = time.time()
     hm = HipMesh(mesh, renumber=not args.no_renumber, block_cells=args.block_cells)
     layout = hm.layout()
-    layout["probe_ms"] = hm.layout_probe()
+    layout["probe_ms"] = {k: (round(v, 5) if v is not None else None) for k, v in hm.layout_probe().items()}
     layout["setup_seconds"] = round(time.time() - t_probe, 2)
     log(f"[bench] layout {layout['block_cells']} cells per block, {layout['accumulation']} (probe {layout['probe_ms']})")
     hm.set_state(U)

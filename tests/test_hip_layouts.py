@@ -88,8 +88,12 @@ def test_layout_probe(hip):
         lay, probe = hm.layout(), hm.layout_probe()
         say(f"[layout] probe {probe} -> block_cells {lay['block_cells']} {lay['accumulation']}")
         assert all(v is not None and v > 0 for v in probe.values()), probe
-        best = min(probe, key=probe.get)
-        assert best == (lay["accumulation"] if lay["accumulation"] == "colour" else f"gather{lay['block_cells']}")
+        # The choice: the gather layout with the largest blocks timed within 2 % of the fastest layout,
+        # else the fastest (colour).
+        fastest = min(probe.values())
+        near = [k for k in ("gather512", "gather256", "gather128") if probe[k] <= 1.02 * fastest]
+        expect = near[0] if near else min(probe, key=probe.get)
+        assert expect == (lay["accumulation"] if lay["accumulation"] == "colour" else f"gather{lay['block_cells']}")
         Rh, lh, dh = hm.residual(U, cfg)
         check(Rh, lh, dh, om, cfg, U, "probed layout")
     with hip.HipMesh(m, block_cells=0) as hm2:  # the same mesh: the cached choice, no second probe
