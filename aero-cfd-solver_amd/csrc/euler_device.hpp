@@ -529,12 +529,26 @@ __device__ __forceinline__ C4T<T> char_farfield_state(const C4T<T>& in, T nx, T 
 }
 
 // Limiter (reconstruction.cpp:121-146).
-template <int LIMITER, class T> __device__ __forceinline__ T limiter(T a, T b) {
+//
+// FAST (venkat, double): past a * b > 0 every term of den is positive, so den >= e2 >= 1e-12, and
+// |num| >= e2 |a|.  With den <= 2^300 and |num| >= 2^-700 (*ok reports both; the caller evaluates a face
+// that fails them again without FAST) the quotient's exponents lie less than 768 apart, 1 / den and
+// num / den (at least 2^-1000) are normal and |num| is far above 2^-969, so v_div_scale scales nothing
+// and quot gives the division's bits; x is then finite and non-zero, the den and finiteness tests cannot
+// fire, and with a non-zero the clamp to [min(a, 0), max(a, 0)] is one maxNum and one minNum.
+constexpr double kVenkatNumLo = 0x1p-700;
+template <int LIMITER, bool FAST = false, class T> __device__ __forceinline__ T limiter(T a, T b, bool* ok = nullptr) {
   if (LIMITER == 1) {
     if (a * b <= 0.0) return T(0.0);
     const T e2 = 1.0e-12 * (1.0 + a * a + b * b);
     const T num = (b * b + 2.0 * a * b + e2) * a;
     const T den = a * a + 2.0 * b * b + a * b + e2;
+    if constexpr (FAST) {
+      static_assert(std::is_same_v<T, double>, "FAST limiters are in double");
+      *ok = *ok & (den <= kQuotHi) & (vabs(num) >= kVenkatNumLo);
+      const double x = quot(num, quot_den(den));
+      return fmn<true>(fmx<true>(x, fmn<true>(a, 0.0)), fmx<true>(a, 0.0));
+    }
     if (vabs(den) <= 1.0e-20) return T(0.0);
     const T x = num / den;
     if (!fin(x)) return T(0.0);

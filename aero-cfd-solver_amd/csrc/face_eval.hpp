@@ -19,14 +19,22 @@ __device__ __forceinline__ T recon_incr(T gx, T gy, T dx, T dy) {
 
 // Sources of the increments of variable k (0..3 = rho, u, v, p) on the owner (da) and neighbour (db)
 // sides: from the cells' gradients and centre-to-face vectors, or given directly.
-template <class T>
+// NOZERO: gx dx + gy dy without the leading 0.0 +, for increments that only enter the limiters: the sum
+// differs from recon_incr's at most in the sign of a zero, and both limiters return +0 for a zero
+// increment (a * b <= 0).
+template <class T, bool NOZERO = false>
 struct GradIncr {
   const T* go;
   const T* gn;
   T dox, doy, dnx, dny;
   __device__ __forceinline__ void operator()(int k, T* da, T* db) const {
-    *da = recon_incr(go[2 * k], go[2 * k + 1], dox, doy);
-    *db = recon_incr(gn[2 * k], gn[2 * k + 1], dnx, dny);
+    if constexpr (NOZERO) {
+      *da = go[2 * k] * dox + go[2 * k + 1] * doy;
+      *db = gn[2 * k] * dnx + gn[2 * k + 1] * dny;
+    } else {
+      *da = recon_incr(go[2 * k], go[2 * k + 1], dox, doy);
+      *db = recon_incr(gn[2 * k], gn[2 * k + 1], dnx, dny);
+    }
   }
 };
 template <class T>
@@ -39,14 +47,15 @@ struct GivenIncr {
   }
 };
 
-// MUSCL reconstruction of primitive variable k (reconstruction.cpp:194-213).
-template <int LIMITER, class T, class Src>
-__device__ __forceinline__ void recon_var(T qo, T qn, int k, const Src& src, T* ql, T* qr) {
+// MUSCL reconstruction of primitive variable k (reconstruction.cpp:194-213).  FAST: the limiters' exact
+// shortcuts, *ok cleared where their bounds fail.
+template <int LIMITER, bool FAST = false, class T, class Src>
+__device__ __forceinline__ void recon_var(T qo, T qn, int k, const Src& src, T* ql, T* qr, bool* ok = nullptr) {
   const T dc = qn - qo;
   T da, db;
   src(k, &da, &db);
-  *ql = qo + limiter<LIMITER>(da, dc);
-  *qr = qn + limiter<LIMITER>(db, -dc);
+  *ql = qo + limiter<LIMITER, FAST>(da, dc, ok);
+  *qr = qn + limiter<LIMITER, FAST>(db, -dc, ok);
 }
 
 // Frozen limiter (beyond the reference): the limited increment is ratio * a with the ratio
@@ -118,6 +127,7 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
   P4T<T> L = c2p(uo, g);
   P4T<T> Rt = c2p(un, g);
   *reverted = false;
+  bool rok = true;  // FAST: the limiters' bounds
   if (SECOND) {
     P4T<T> l2, r2;
     if (FROZEN) {
@@ -126,10 +136,10 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
       recon_var_frozen(L.v, Rt.v, 2, src, fr[2], fr[6], &l2.v, &r2.v);
       recon_var_frozen(L.p, Rt.p, 3, src, fr[3], fr[7], &l2.p, &r2.p);
     } else {
-      recon_var<LIMITER>(L.r, Rt.r, 0, src, &l2.r, &r2.r);
-      recon_var<LIMITER>(L.u, Rt.u, 1, src, &l2.u, &r2.u);
-      recon_var<LIMITER>(L.v, Rt.v, 2, src, &l2.v, &r2.v);
-      recon_var<LIMITER>(L.p, Rt.p, 3, src, &l2.p, &r2.p);
+      recon_var<LIMITER, FAST>(L.r, Rt.r, 0, src, &l2.r, &r2.r, &rok);
+      recon_var<LIMITER, FAST>(L.u, Rt.u, 1, src, &l2.u, &r2.u, &rok);
+      recon_var<LIMITER, FAST>(L.v, Rt.v, 2, src, &l2.v, &r2.v, &rok);
+      recon_var<LIMITER, FAST>(L.p, Rt.p, 3, src, &l2.p, &r2.p, &rok);
     }
     l2.r = dmax(l2.r, kFloor);
     r2.r = dmax(r2.r, kFloor);
@@ -142,17 +152,19 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
       *reverted = true;
     }
   }
-  return flux_of_states<SCHEME, PLAIN, FAST>(L, Rt, nx, ny, o, ok);
+  const FluxT<T> F = flux_of_states<SCHEME, PLAIN, FAST>(L, Rt, nx, ny, o, ok);
+  if constexpr (FAST) *ok = *ok & rok;
+  return F;
 }
 
 // Interior face (residual_assembly.cpp:107-137).  go / gn are the cells' primitive gradients (rho_x,
 // rho_y, u_x, u_y, v_x, v_y, p_x, p_y); dO / dN the vectors from the cell centres to the face centre.
-// FAST: see flux_of_states (*ok).
+// FAST: see flux_of_states and limiter (*ok).
 template <int SCHEME, int LIMITER, bool SECOND, bool FROZEN = false, bool PLAIN = false, bool FAST = false, class T>
 __device__ __forceinline__ FluxT<T> interior_face(const C4T<T>& uo, const C4T<T>& un, const T* go, const T* gn,
                                                   T nx, T ny, T dox, T doy, T dnx, T dny, const FaceOptsT<T>& o,
                                                   bool* reverted, const double* fr = nullptr, bool* ok = nullptr) {
-  const GradIncr<T> src{go, gn, dox, doy, dnx, dny};
+  const GradIncr<T, FAST && !FROZEN> src{go, gn, dox, doy, dnx, dny};
   return interior_face_src<SCHEME, LIMITER, SECOND, FROZEN, PLAIN, FAST>(uo, un, src, nx, ny, o, reverted, fr, ok);
 }
 
