@@ -191,6 +191,19 @@ __device__ __forceinline__ P4 c2p_q(const C4& c, double g, QuotDen* d) {
   w.p = fmx<true>((g - 1.0) * (c.E - ke), kFloor);
   return w;
 }
+// c2p of a cell state with 1 / rho through quot: exact for densities up to 2^300 (at least kFloor
+// after the floor); *ok is cleared for one above the bound or NaN.
+__device__ __forceinline__ P4 c2p_cell_q(const C4& c, double g, bool* ok) {
+  P4 w;
+  w.r = dmax(c.r, kFloor);
+  *ok = *ok & (w.r <= kQuotHi);
+  const double inv = quot(1.0, quot_den(w.r));
+  w.u = c.ru * inv;
+  w.v = c.rv * inv;
+  const double ke = 0.5 * w.r * (w.u * w.u + w.v * w.v);
+  w.p = dmax((g - 1.0) * (c.E - ke), kFloor);
+  return w;
+}
 __device__ __forceinline__ double sound_q(const P4& w, double g, const QuotDen& d) {
   return vsqrt(fmx<true>(quot(g * w.p, d), 0.0));
 }

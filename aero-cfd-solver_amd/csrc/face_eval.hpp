@@ -124,10 +124,16 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
                                                       const FaceOptsT<T>& o, bool* reverted, const double* fr,
                                                       bool* ok = nullptr) {
   const double g = o.g;
-  P4T<T> L = c2p(uo, g);
-  P4T<T> Rt = c2p(un, g);
+  bool rok = true;  // FAST: the bounds of the cell densities and of the limiters
+  P4T<T> L, Rt;
+  if constexpr (FAST) {
+    L = c2p_cell_q(uo, g, &rok);
+    Rt = c2p_cell_q(un, g, &rok);
+  } else {
+    L = c2p(uo, g);
+    Rt = c2p(un, g);
+  }
   *reverted = false;
-  bool rok = true;  // FAST: the limiters' bounds
   if (SECOND) {
     P4T<T> l2, r2;
     if (FROZEN) {
@@ -141,10 +147,13 @@ __device__ __forceinline__ FluxT<T> interior_face_src(const C4T<T>& uo, const C4
       recon_var<LIMITER, FAST>(L.v, Rt.v, 2, src, &l2.v, &r2.v, &rok);
       recon_var<LIMITER, FAST>(L.p, Rt.p, 3, src, &l2.p, &r2.p, &rok);
     }
-    l2.r = dmax(l2.r, kFloor);
-    r2.r = dmax(r2.r, kFloor);
-    l2.p = dmax(l2.p, kFloor);
-    r2.p = dmax(r2.p, kFloor);
+    // FAST venkat: a NaN here needs a NaN cell value, which makes dc NaN and fails that limiter's bound
+    // (rok), so on the faces kept the floors are maxNum (std::max differs from it only on NaN).
+    constexpr bool kMaxNum = FAST && !FROZEN && LIMITER == 1;
+    l2.r = fmx<kMaxNum>(l2.r, T(kFloor));
+    r2.r = fmx<kMaxNum>(r2.r, T(kFloor));
+    l2.p = fmx<kMaxNum>(l2.p, T(kFloor));
+    r2.p = fmx<kMaxNum>(r2.p, T(kFloor));
     if (valid_prim(l2) && valid_prim(r2)) {
       L = l2;
       Rt = r2;
