@@ -118,6 +118,24 @@ def test_plain_residual_bitwise_extreme_states(hip):
             assert compare(hm, om, U, combo, ff, "64x14 extreme states bitwise"), combo
 
 
+def test_plain_residual_bitwise_limiter_bounds(hip):
+    """Faces whose states are within the face-flux bounds (energies up to 2^280) but whose venkat
+    denominators exceed 2^300 (pressure jumps above 2^150 between neighbours) are evaluated again with the
+    plain limiter: the residual stays bit-equal to the FP64 oracle's."""
+    m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=33, rcm=True)
+    om = ol.OracleMesh(m, np.float64)
+    ff = ff64(0.5, 1.0)
+    U = helpers.perturbed_state(m.cell_center, ff, 0.05, 79)
+    rng = np.random.default_rng(6)
+    hot = rng.choice(m.num_cells, size=10, replace=False)
+    U[hot, 4] *= 1.0e50  # energies near 2^166: pressures of the same size, jumps of 2^160 to the neighbours
+    p_hot = U[hot, 4].min() * (1.4 - 1.0) / 2.0
+    assert U[:, 4].max() < 2.0 ** 280 and p_hot > 2.0 ** 151  # inside the flux bounds, past the limiter's
+    with hip.HipMesh(m) as hm:
+        for combo in [(1, 1, 1, 0, 0, 0, 0), (0, 1, 1, 0, 0, 0, 1), (1, 0, 1, 0, 0, 0, 0)]:
+            assert compare(hm, om, U, combo, ff, "64x14 limiter bounds bitwise"), combo
+
+
 def test_gradients_parity(hip):
     m = helpers.mesh("64x14", cells="mixed", quad_layers=4, jitter=0.1, seed=5, rcm=True)
     om = ol.OracleMesh(m, np.float64)
